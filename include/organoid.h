@@ -1,0 +1,210 @@
+/*
+ * organoid.h -- C-ABI of the MI355X (gfx950) organoid timestep library.
+ *
+ * Plain C types only: pointers, sizes, int status codes.  Every entry point
+ * below is the device-resident counterpart of one function (or one fused
+ * group of functions) on the per-timestep path of the reference simulator;
+ * the comment on each names the reference function it replaces as
+ * file:line (files under functions/ and main_code/ of
+ * tristanguyomar/multicellular_phase_field_active_gel_organoids).
+ *
+ * The same header is included by the CPU oracle (oracle/organoid_oracle.c)
+ * for org_params, so both sides read identical constants.
+ *
+ * Conventions
+ *   - Planes are row-major: plane[n][i][j] = base[n*N*N + i*N + j], i is the
+ *     row ("x" in the reference, useful_functions.cpp:214), j the column.
+ *   - Per-cell fields (phi, rho, hfunc) hold ncell planes; shared fields
+ *     (lum, ECM, psi, ...) hold one plane.
+ *   - Host buffers passed to org_set_* / org_get_* are FP32 unless named _f64.
+ *   - Every entry point returns 0 on success or an ORG_E* code; the first
+ *     error is also kept in a sticky slot (org_last_error).  The reference has
+ *     no error channel (all its functions return void); see the per-code
+ *     comments for which reference behaviour each one turns into an error.
+ */
+#ifndef ORGANOID_H
+#define ORGANOID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    ORG_OK = 0,
+    ORG_EINVAL = 1,       /* bad argument / size                                */
+    ORG_EHIP = 2,         /* a HIP runtime call failed                          */
+    ORG_EEMPTY_CELL = 3,  /* no phi>0.5 pixel: the reference divides by zero
+                             (useful_functions.cpp:221)                         */
+    ORG_EWINDOW = 4,      /* a cell window (+stencil halo) leaves the grid
+                             interior; the reference reads out of range
+                             ("careful here", rho_functions.cpp:43)             */
+    ORG_ERELAX = 5,       /* a reshaping loop passed rimax ("Relaxation
+                             failure!", cell_phase_functions.cpp:565, 862)     */
+    ORG_ENOTBUILT = 6,    /* kernels for this build are missing                 */
+    ORG_ESTATE = 7        /* call order / state error                           */
+};
+
+/* ---- configuration ---------------------------------------------------------
+ * The reference keeps these as compile-time globals rendered from
+ * constants_template.cpp:1-130 (declared in constants.h:4-124).  Values are
+ * held in double; the FP32 GPU path rounds each to float once, as the
+ * reference's float globals do.                                              */
+typedef struct org_params {
+    int32_t N;                 /* N_s, grid side                 constants_template.cpp:145 */
+    int32_t ncell;             /* Ntot = N_ini = N_current       constants_template.cpp:158-160 */
+    int32_t zone_size;         /* half window side               constants_template.cpp:162 */
+    int32_t rimax;             /* reshaping iteration cap        constants_template.cpp:239 */
+    int32_t rdev_threshold;    /* lumen/ECM stop factor (int)    constants_template.cpp:130 */
+    int32_t rdev_ini;          /* lumen/ECM start factor (int)   constants_template.cpp:131 */
+    int32_t n_current;         /* N_current (divides cell rdev)  cell_phase_functions.cpp:560 */
+    int32_t use_s0_cell;       /* 1: per-cell target areas (extension), 0: global s0 */
+    double dt, dx;             /* constants_template.cpp:141-142 */
+    double R;                  /* R (reshaping tolerance radius) constants_template.cpp:150 */
+    double s0;                 /* target area                    constants_template.cpp:151 */
+    double sigma;              /* delta width                    constants_template.cpp:155 */
+    double coupling_PF_AG;     /*                                 constants_template.cpp:166 */
+    double tau, alpha, beta, eta;                       /* constants_template.cpp:208-211 */
+    double tau_lum, beta_lum_cells, eta_lum_cells, xi_lum;            /* :215-219 */
+    double alpha_ECM, tau_ECM, beta_ECM_cells, eta_ECM_cells, xi_ECM; /* :222-227 */
+    double beta_lum_ECM;                                              /* :230 */
+    double wi, rtol, rdt;                               /* :233, 241, 242 */
+    double friction;                                    /* :201 */
+    double visc_ap, visc_bas, visc_lat;                 /* :202-204 */
+    double a_ort_ap, a_ort_bas, a_ort_lat;              /* :181-183 */
+    double a_tan_ap, a_tan_bas, a_tan_lat;              /* :185-187 */
+    double b_ap, b_bas, b_lat;                          /* :189-191 */
+    double gen_ap, gen_bas, gen_lat;                    /* :193,196,198 */
+    double deg_ap, deg_bas, deg_lat;                    /* :194,197,199 */
+} org_params;
+
+/* ---- field identifiers (org_set_field / org_get_field) ------------------- */
+enum {
+    ORG_F_PHI = 0,     /* ncell planes  declare_tables.cpp:97  */
+    ORG_F_RHO = 1,     /* ncell planes  declare_tables.cpp:99  */
+    ORG_F_HFUNC = 2,   /* ncell planes  declare_tables.cpp:111 */
+    ORG_F_LUM = 3,     /* 1 plane       declare_tables.cpp:102 */
+    ORG_F_ECM = 4,     /* 1 plane       declare_tables.cpp:103 */
+    ORG_F_PSI = 5,     /* 1 plane       declare_tables.cpp:107 */
+    ORG_F_HLUM = 6,    /* 1 plane       declare_tables.cpp:112 */
+    ORG_F_HECM = 7,    /* 1 plane       declare_tables.cpp:113 */
+    ORG_F_LUMN = 8,    /* 1 plane       declare_tables.cpp:104 */
+    ORG_F_ECMN = 9,    /* 1 plane       declare_tables.cpp:105 */
+    ORG_F_COUNT = 10
+};
+
+/* window-local outputs (org_get_window): [ncell][2*zone_size][2*zone_size]
+ * anchored at (xc-zone_size, yc-zone_size) of each cell's current centre, or
+ * [ncell][M][M] anchored at (xc-zone_size+1, yc-zone_size+1) for the FFT box
+ * (M = 2*zone_size-2, useful_functions.cpp:618-619).                         */
+enum {
+    ORG_W_PHIN = 0,    /* phin after update_all_cells (window)            */
+    ORG_W_RHON = 1,    /* rhon after update_all_cortices (FFT box, M x M) */
+    ORG_W_VX = 2,      /* Re(F^-1 v_x)/M^2 (FFT box), rho_functions.cpp:449-450 */
+    ORG_W_VY = 3,      /* Re(F^-1 v_y)/M^2 (FFT box), rho_functions.cpp:455-456 */
+    ORG_W_COUNT = 4
+};
+
+typedef struct org_ctx org_ctx;
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* Allocate device state for one organoid.  s0_cell (ncell doubles) is used
+ * when p->use_s0_cell is 1, else ignored.  Replaces malloc_all
+ * (declare_tables.cpp:89-143) and ini_fft_all (declare_fft_tables.cpp:26-41);
+ * device memory is laid out once for the whole run.                          */
+int org_create(const org_params *p, const double *s0_cell, org_ctx **out);
+int org_destroy(org_ctx *ctx);
+
+/* Upload / download a whole field (all planes).  Upload plays the role of
+ * the driver's `#pragma acc data copyin` (run_with_active_gels.cpp:83).  After
+ * uploading phi and hfunc, org_prepare must run before the first step. */
+int org_set_field(org_ctx *ctx, int field, const float *host);
+int org_get_field(org_ctx *ctx, int field, float *host);
+/* Window-local debug outputs of the last step (see ORG_W_*). */
+int org_get_window(org_ctx *ctx, int which, float *host);
+/* Centres (ncell ints each), areas s and f0 (ncell doubles each). */
+int org_get_centres(org_ctx *ctx, int32_t *xc, int32_t *yc);
+int org_get_areas(org_ctx *ctx, double *s, double *f0);
+/* Reshaping iteration counts of the last step: cells[ncell], lum, ecm. */
+int org_get_iterations(org_ctx *ctx, int32_t *cells, int32_t *lum, int32_t *ecm);
+
+/* Scan the uploaded phi/hfunc once to seed each cell's support box (the
+ * rectangle outside which phi<=0.5 and hfunc==0; both fields only change
+ * inside the cell's windows afterwards).  Also zeroes psi, copies
+ * rho->rhon, lum->lumn, ECM->ECMn (run_with_active_gels.cpp:92-98). */
+int org_prepare(org_ctx *ctx);
+
+/* ---- per-function entry points (the plain drop-in form) -------------------
+ * Each runs over all cells in one launch where the reference launched one
+ * kernel per cell.  Fields the driver never reads back (norm_grad_phi,
+ * prod_norm_delta, the six turnover maps, pi_xx/xy/yy, J_x/J_y, temp_phase)
+ * are not materialised: they are recomputed inside the kernels that consume
+ * them.                                                                       */
+int org_find_centres_cells(org_ctx *ctx);          /* useful_functions.cpp:193-227 */
+int org_compute_hfunc_cells(org_ctx *ctx);         /* useful_functions.cpp:233-251 */
+int org_compute_hfunc_lum_ecm(org_ctx *ctx);       /* useful_functions.cpp:272-285 */
+int org_compute_psi(org_ctx *ctx);                 /* useful_functions.cpp:305-322 */
+int org_sum_s(org_ctx *ctx);                       /* useful_functions.cpp:168-190 */
+/* create_turnover_rates_tensions (rho_functions.cpp:286-405),
+ * compute_prod_norm_delta (rho_functions.cpp:13-30), the compute_pi_* family
+ * (rho_functions.cpp:32-115) and the whole per-cell spectral loop
+ * update_all_cortices (rho_functions.cpp:409-471), fused. */
+int org_update_all_cortices(org_ctx *ctx);
+int org_update_lumen_and_ECM(org_ctx *ctx);        /* cell_phase_functions.cpp:372-444
+                                                      + useful_functions.cpp:439-462 */
+int org_reshaping_lumen_or_ECM(org_ctx *ctx, int which_ecm);
+                                                   /* cell_phase_functions.cpp:709-871 */
+int org_update_all_cells(org_ctx *ctx);            /* cell_phase_functions.cpp:126-217
+                                                      (+ compute_normal_grad_phi
+                                                      useful_functions.cpp:326-354) */
+int org_reshaping_cells(org_ctx *ctx);             /* cell_phase_functions.cpp:447-580
+                                                      (+ compute_normal_to_phi
+                                                      useful_functions.cpp:356-388) */
+int org_commit_step(org_ctx *ctx);                 /* useful_functions.cpp:679-709 as
+                                                      called at run_with_active_gels.cpp:175-182 */
+
+/* ---- batched form -----------------------------------------------------------
+ * One full timestep = the loop body run_with_active_gels.cpp:118-184, with
+ * no host round trip inside a step.  use_graph=1 captures one step into a
+ * hipGraph on first use and replays it.  Reshaping of lumen/ECM loops on the
+ * device in chunks; a step that needs more than the captured chunk falls
+ * back to host-driven chunks (exact, slower).                                 */
+int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
+
+/* Device synchronisation and timing.  org_timer_* brackets work on the
+ * library's stream with HIP events; org_kernel_time returns the summed
+ * duration (ms) and launch count of one kernel family since the last reset,
+ * measured with events recorded around its launches on the same stream. */
+int org_sync(org_ctx *ctx);
+int org_profile_enable(org_ctx *ctx, int on);
+int org_kernel_time(org_ctx *ctx, int kernel_id, double *ms, int64_t *launches);
+enum {
+    ORG_K_CENTRES = 0, ORG_K_PREP = 1, ORG_K_PSI = 2, ORG_K_LUMECM = 3,
+    ORG_K_LUMRESHAPE = 4, ORG_K_SPECTRAL = 5, ORG_K_CELLS = 6, ORG_K_COMMIT = 7,
+    ORG_K_COUNT = 8
+};
+
+/* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of
+ * the last step for org_get_window.  ORG_OPT_GRAPH_LUM_ITERS (default 6): lumen
+ * and ECM reshaping sweeps captured per step in the hipGraph; a step that
+ * needs more makes org_step return ORG_ERELAX (use use_graph=0 for such runs). */
+enum { ORG_OPT_WRITE_VELOCITY = 0, ORG_OPT_GRAPH_LUM_ITERS = 1 };
+int org_set_option(org_ctx *ctx, int option, int value);
+
+/* Test hook (no counterpart in the reference): one unnormalised 128 x 128
+ * 2D DFT through the LDS transform used by org_update_all_cortices.  data:
+ * 128*128 interleaved (re, im) floats on the host, sign -1 or +1. */
+int org_debug_fft2d(float *data, int sign);
+
+/* Sticky first error and a readable message for any code. */
+int org_last_error(void);
+const char *org_error_string(int code);
+/* Library and kernel build identification (gfx target). */
+const char *org_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ORGANOID_H */

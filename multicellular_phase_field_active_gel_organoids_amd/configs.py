@@ -1,0 +1,170 @@
+"""Initial conditions and constants of the organoid workloads.
+
+The reference reads its initial state from ``ini_*.bin`` files written by its
+IC pipeline (functions/initialisation_functions.cpp:10-191,
+initialisation_code/prepare_initial_files.cpp:56-118).  The workloads here are
+synthetic and seeded; every array is built in float64 and handed to both the
+GPU library (rounded to float32) and the CPU oracle.
+
+Shapes:  phi, rho: (ncell, N, N);  lum, ECM: (N, N).
+
+* ``phi_n = 0.5 * (1 - tanh((|x - c_n| - R_n) / eps))`` with ``x = (i, j)``, i
+  the row index (the reference's "x", useful_functions.cpp:214).
+* ``rho_n = 1 + 0.01 * U(-1, 1)`` where ``0.1 < phi_n < 0.9``, else 0.
+* ``lum = 0`` and ``ECM = clip(1 - sum_n phi_n, 0, 1)`` (the outside-the-cyst
+  branch of ini_ECM_lumen, initialisation_functions.cpp:84-114, without a
+  lumen disc).
+
+Random numbers come from SplitMix64 (Vigna), ``U[0,1) = (x >> 11) * 2**-53``,
+seeded ``1000 + k`` for workload k, drawn in this order: centre jitter (2 per
+cell, row then column), radii (1 per cell), then rho noise in row-major order
+cell by cell over the band pixels only.
+
+Hex packings: lattice spacing ``2 * mean(R)`` centred on the grid centre,
+sites sorted by (distance, angle) and the first ``ncell`` kept; after jitter
+the cells are ordered by angle around the grid centre (from -pi), because
+the reference's "lateral" label uses index neighbours n-1, n+1 on a ring
+(rho_functions.cpp:298-312).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .params import OrgParams, grid_units
+
+__all__ = ["SplitMix64", "Workload", "make_workload", "WORKLOADS"]
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    """SplitMix64 generator (Steele, Lea, Flood 2014; constants as published
+    by Vigna).  Vectorised: the k-th output is mix(seed + k*gamma)."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def next_u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + k * _GAMMA
+            self.state = self.state + np.uint64(n) * _GAMMA
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+        return lo + (hi - lo) * u
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    params: OrgParams
+    phi: np.ndarray
+    rho: np.ndarray
+    lum: np.ndarray
+    ecm: np.ndarray
+    centres: np.ndarray       # (ncell, 2) float, (row, col)
+    radii: np.ndarray         # (ncell,)
+    s0_cell: np.ndarray | None
+    steps: int
+
+    @property
+    def hfunc(self) -> np.ndarray:
+        """The driver zeroes hfunc_cell before the loop (run_with_active_gels.cpp:74)."""
+        return np.zeros_like(self.phi)
+
+
+def _disc_fields(N, centres, radii, eps, rng):
+    nc = len(radii)
+    ii, jj = np.meshgrid(np.arange(N, dtype=np.float64), np.arange(N, dtype=np.float64), indexing="ij")
+    phi = np.empty((nc, N, N), np.float64)
+    for n in range(nc):
+        r = np.sqrt((ii - centres[n, 0]) ** 2 + (jj - centres[n, 1]) ** 2)
+        phi[n] = 0.5 * (1.0 - np.tanh((r - radii[n]) / eps))
+    rho = np.zeros_like(phi)
+    for n in range(nc):
+        band = (phi[n] > 0.1) & (phi[n] < 0.9)
+        cnt = int(band.sum())
+        rho[n][band] = 1.0 + 0.01 * rng.uniform(cnt, -1.0, 1.0)
+    lum = np.zeros((N, N), np.float64)
+    ecm = np.clip(1.0 - phi.sum(axis=0), 0.0, 1.0)
+    return phi, rho, lum, ecm
+
+
+def hex_sites(ncell: int, spacing: float, centre: tuple[float, float]) -> np.ndarray:
+    """The ``ncell`` hex-lattice sites nearest to ``centre`` (ties by angle)."""
+    kmax = int(math.ceil(math.sqrt(ncell))) + 3
+    pts = []
+    for a in range(-kmax, kmax + 1):
+        for b in range(-kmax, kmax + 1):
+            x = spacing * (a + 0.5 * b)
+            y = spacing * (b * math.sqrt(3.0) / 2.0)
+            d = math.hypot(x, y)
+            ang = math.atan2(y, x)
+            pts.append((round(d, 9), ang, x, y))
+    pts.sort()
+    sel = np.array([[centre[0] + p[2], centre[1] + p[3]] for p in pts[:ncell]], np.float64)
+    return sel
+
+
+def _hex_workload(name, N, ncell, r_lo, r_hi, seed, steps, eps=2.0, disc=None):
+    rng = SplitMix64(seed)
+    c = (N / 2.0, N / 2.0)
+    mean_r = 0.5 * (r_lo + r_hi)
+    sites = hex_sites(ncell, 2.0 * mean_r, c)
+    jit = rng.uniform(2 * ncell, -2.0, 2.0).reshape(ncell, 2)
+    centres = sites + jit
+    if r_lo == r_hi:
+        radii = np.full(ncell, float(r_lo))
+    else:
+        radii = rng.uniform(ncell, r_lo, r_hi)
+    ang = np.arctan2(centres[:, 1] - c[1], centres[:, 0] - c[0])
+    order = np.argsort(ang, kind="stable")
+    centres, radii = centres[order], radii[order]
+    phi, rho, lum, ecm = _disc_fields(N, centres, radii, eps, rng)
+    p = grid_units(N, ncell, float(radii.mean()), eps=eps)
+    s0_cell = None
+    if r_lo != r_hi:
+        p.use_s0_cell = 1
+        s0_cell = math.pi * radii ** 2
+    return Workload(name, p, phi, rho, lum, ecm, centres, radii, s0_cell, steps)
+
+
+def make_workload(name: str, N: int | None = None, ncell: int | None = None, steps: int | None = None) -> Workload:
+    """Build one of the named workloads.
+
+    ``two_cells_256``     2 cells, R=30, eps=2, 256^2, centres 55 apart on row 128
+    ``organoid16_512``    16 cells R=20 on a jittered hex packing, 512^2
+    ``organoid64_1024``   64 cells, radii U(16,24), jittered hex packing, 1024^2
+    ``organoid256_2048``  256 cells, radii U(14,26), jittered hex packing, 2048^2
+    ``hex``               custom N / ncell hex packing with R=20 (tests)
+    """
+    if name == "two_cells_256":
+        n_ = N or 256
+        rng = SplitMix64(1000)
+        centres = np.array([[n_ / 2.0, n_ / 2.0 - 27.5], [n_ / 2.0, n_ / 2.0 + 27.5]])
+        radii = np.array([30.0, 30.0])
+        phi, rho, lum, ecm = _disc_fields(n_, centres, radii, 2.0, rng)
+        p = grid_units(n_, 2, 30.0, eps=2.0)
+        return Workload(name, p, phi, rho, lum, ecm, centres, radii, None, steps or 1000)
+    if name == "organoid16_512":
+        return _hex_workload(name, N or 512, ncell or 16, 20.0, 20.0, 1001, steps or 5000)
+    if name == "organoid64_1024":
+        return _hex_workload(name, N or 1024, ncell or 64, 16.0, 24.0, 1003, steps or 10000)
+    if name == "organoid256_2048":
+        return _hex_workload(name, N or 2048, ncell or 256, 14.0, 26.0, 1004, steps or 20000)
+    if name == "hex":
+        return _hex_workload(name, N or 256, ncell or 4, 20.0, 20.0, 1099, steps or 10)
+    raise ValueError(f"unknown workload {name!r}")
+
+
+WORKLOADS = ("two_cells_256", "organoid16_512", "organoid64_1024", "organoid256_2048")

@@ -1,0 +1,572 @@
+/*
+ * organoid.hip -- host side of the C-ABI declared in include/organoid.h.
+ *
+ * One org_ctx owns the device state of one organoid and one HIP stream
+ * (default priority).  All kernels come from organoid_kernels.hip.
+ */
+#include "organoid_kernels.hip"
+
+namespace {
+
+std::atomic<int> g_last_error{0};
+
+int note_error(int code) {
+    int zero = 0;
+    if (code) g_last_error.compare_exchange_strong(zero, code);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            fprintf(stderr, "organoid: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                               \
+            return note_error(ORG_EHIP);                                               \
+        }                                                                              \
+    } while (0)
+
+}  // namespace
+
+struct org_ctx {
+    org_params p{};
+    DevP dp{};
+    hipStream_t stream = nullptr;
+    int N = 0, nc = 0, zs = 0, W = 0, M = 0;
+    size_t NN = 0, P = 0;
+    int grid_g = 0, ntiles = 0;
+    /* full planes */
+    float *phi = nullptr, *rho = nullptr, *hfunc = nullptr;
+    float *lum = nullptr, *ecm = nullptr, *psi = nullptr, *hlum = nullptr, *hecm = nullptr;
+    float *bufL0 = nullptr, *bufL1 = nullptr, *bufE0 = nullptr, *bufE1 = nullptr;
+    float *jlx = nullptr, *jly = nullptr, *jex = nullptr, *jey = nullptr;
+    /* window-local */
+    float *phin_win = nullptr, *rho_box = nullptr, *vxbox = nullptr, *vybox = nullptr, *jwin = nullptr;
+    float2 *spec = nullptr;
+    /* per cell / scalars */
+    int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
+    double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
+    float *f0 = nullptr;
+    unsigned *counters = nullptr;
+    LumState *lst = nullptr;
+    bool prepared = false;
+    int write_v = 1;
+    int graph_lum_iters = 6;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    /* profiling */
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
+    size_t ev_used[ORG_K_COUNT] = {};
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(org_ctx *c, T **ptr, size_t count) {
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, (count ? count : 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *ptr = (T *)q;
+    return ORG_OK;
+}
+
+/* Event bracket around one launch when profiling is on. */
+struct Prof {
+    org_ctx *c;
+    int kid;
+    hipEvent_t stop = nullptr;
+    Prof(org_ctx *c_, int kid_) : c(c_), kid(kid_) {
+        if (!c->prof) return;
+        auto &v = c->ev[kid];
+        if (c->ev_used[kid] == v.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { c->prof = false; return; }
+            v.push_back({a, b});
+        }
+        auto &pr = v[c->ev_used[kid]++];
+        (void)hipEventRecord(pr.first, c->stream);
+        stop = pr.second;
+    }
+    ~Prof() {
+        if (stop) (void)hipEventRecord(stop, c->stream);
+    }
+};
+
+int launch_check() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "organoid: kernel launch failed: %s\n", hipGetErrorString(e));
+        return note_error(ORG_EHIP);
+    }
+    return ORG_OK;
+}
+
+DevP make_devp(const org_params &p) {
+    DevP d{};
+    d.N = p.N; d.nc = p.ncell; d.zs = p.zone_size; d.W = 2 * p.zone_size; d.M = 2 * p.zone_size - 2;
+    d.rimax = p.rimax;
+    d.dt = (float)p.dt; d.dx = (float)p.dx; d.sigma = (float)p.sigma; d.cpl = (float)p.coupling_PF_AG;
+    d.tau = (float)p.tau; d.alpha = (float)p.alpha; d.beta = (float)p.beta; d.eta = (float)p.eta;
+    d.tau_lum = (float)p.tau_lum; d.beta_lc = (float)p.beta_lum_cells; d.eta_lc = (float)p.eta_lum_cells;
+    d.xi_lum = (float)p.xi_lum; d.alpha_E = (float)p.alpha_ECM; d.tau_E = (float)p.tau_ECM;
+    d.beta_ec = (float)p.beta_ECM_cells; d.eta_ec = (float)p.eta_ECM_cells; d.xi_E = (float)p.xi_ECM;
+    d.beta_le = (float)p.beta_lum_ECM;
+    d.wi = (float)p.wi; d.rdt = (float)p.rdt; d.friction = (float)p.friction; d.ncur = (float)p.n_current;
+    /* label order 0 apical, 1 basal, 2/3 lateral */
+    const double visc[4] = {p.visc_ap, p.visc_bas, p.visc_lat, p.visc_lat};
+    const double aort[4] = {p.a_ort_ap, p.a_ort_bas, p.a_ort_lat, p.a_ort_lat};
+    const double atn[4] = {p.a_tan_ap, p.a_tan_bas, p.a_tan_lat, p.a_tan_lat};
+    const double bb[4] = {p.b_ap, p.b_bas, p.b_lat, p.b_lat};
+    const double gen[4] = {p.gen_ap, p.gen_bas, p.gen_lat, p.gen_lat};
+    const double deg[4] = {p.deg_ap, p.deg_bas, p.deg_lat, p.deg_lat};
+    for (int k = 0; k < 4; k++) {
+        d.visc[k] = (float)visc[k]; d.aort[k] = (float)aort[k]; d.atan_[k] = (float)atn[k];
+        d.bb[k] = (float)bb[k]; d.gen[k] = (float)gen[k]; d.deg[k] = (float)deg[k];
+    }
+    /* stop tests as the reference evaluates them with float globals:
+     * rtol*2.0*M_PI*R*wi*rdt in double (cell_phase_functions.cpp:473-475),
+     * rdev_threshold*rtol*rdt in float (:720, :774) */
+    const float rtol = (float)p.rtol, R = (float)p.R, wi = (float)p.wi, rdt = (float)p.rdt;
+    d.thr_cell = (double)rtol * 2.0 * M_PI * (double)R * (double)wi * (double)rdt;
+    d.rdev0_cell = 2.0 * (double)rtol * 2.0 * M_PI * (double)R * (double)wi * (double)rdt;
+    d.thr_lum = (double)((float)p.rdev_threshold * rtol * rdt);
+    d.rdev0_lum = (double)((float)p.rdev_ini * rtol * rdt);
+    return d;
+}
+
+int check_ctx(org_ctx *c) {
+    if (!c) return note_error(ORG_EINVAL);
+    return ORG_OK;
+}
+
+int need_prepared(org_ctx *c) {
+    if (!c) return note_error(ORG_EINVAL);
+    if (!c->prepared) {
+        fprintf(stderr, "organoid: org_prepare must run after the fields are uploaded\n");
+        return note_error(ORG_ESTATE);
+    }
+    return ORG_OK;
+}
+
+/* Read the device error bitmask; map the lowest set code. */
+int harvest_device_error(org_ctx *c) {
+    int bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!bits) return ORG_OK;
+    for (int code = 1; code < 32; code++)
+        if (bits & (1 << code)) {
+            fprintf(stderr, "organoid: device reported: %s\n", org_error_string(code));
+            return note_error(code);
+        }
+    return ORG_OK;
+}
+
+/* ---- stage launchers ------------------------------------------------------ */
+int L_centroid(org_ctx *c) {
+    Prof pr(c, ORG_K_CENTRES);
+    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->err);
+    return launch_check();
+}
+int L_prep(org_ctx *c, int mode) {
+    Prof pr(c, ORG_K_PREP);
+    hipLaunchKernelGGL(k_prep, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc, c->box,
+                       c->s0, c->s, c->f0, mode);
+    return launch_check();
+}
+int L_hlum(org_ctx *c) {
+    Prof pr(c, ORG_K_LUMECM);
+    hipLaunchKernelGGL(k_hlum, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum, c->hecm,
+                       c->part_h, c->counters + 2, c->sum_ecm);
+    return launch_check();
+}
+int L_psi(org_ctx *c) {
+    Prof pr(c, ORG_K_PSI);
+    hipLaunchKernelGGL(k_psi, dim3(c->ntiles), dim3(256), 0, c->stream, c->dp, c->hfunc, c->xc, c->yc, c->box, c->psi);
+    return launch_check();
+}
+int L_lumecm_update(org_ctx *c) {
+    Prof pr(c, ORG_K_LUMECM);
+    hipLaunchKernelGGL(k_lumecm_update, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->psi,
+                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufE0, c->lst);
+    return launch_check();
+}
+int L_lum_normals(org_ctx *c, int fbase, int nf) {
+    Prof pr(c, ORG_K_LUMRESHAPE);
+    hipLaunchKernelGGL(k_lum_normals, dim3(c->grid_g, nf), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
+                       c->bufE0, c->jlx, c->jly, c->jex, c->jey, fbase);
+    return launch_check();
+}
+int L_lum_iter(org_ctx *c, int fbase, int nf) {
+    Prof pr(c, ORG_K_LUMRESHAPE);
+    hipLaunchKernelGGL(k_lum_iter, dim3(c->grid_g, nf), dim3(256), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
+                       c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->lst, c->part_l, c->counters, c->err, fbase);
+    return launch_check();
+}
+int L_commit_lumecm(org_ctx *c) {
+    Prof pr(c, ORG_K_LUMRESHAPE);
+    hipLaunchKernelGGL(k_commit_lumecm, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
+                       c->bufL1, c->bufE0, c->bufE1, c->lst, c->err);
+    return launch_check();
+}
+int L_spectral(org_ctx *c) {
+    Prof pr(c, ORG_K_SPECTRAL);
+    hipLaunchKernelGGL(k_spectral, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->lum, c->ecm,
+                       c->xc, c->yc, c->spec, c->rho_box, c->vxbox, c->vybox, c->write_v);
+    return launch_check();
+}
+int L_cells(org_ctx *c, int mode) {
+    Prof pr(c, ORG_K_CELLS);
+    hipLaunchKernelGGL(k_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->hfunc, c->psi,
+                       c->hlum, c->hecm, c->f0, c->xc, c->yc, c->phin_win, c->jwin, c->counts, c->err, mode);
+    return launch_check();
+}
+int L_commit_cells(org_ctx *c) {
+    Prof pr(c, ORG_K_COMMIT);
+    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->rho, c->rho_box, c->xc, c->yc);
+    return launch_check();
+}
+
+#define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
+
+/* Host-driven lumen/ECM loop: run sweeps until both stop tests passed,
+ * checking after every `chunk` sweeps (the reference reads rdev back every
+ * sweep, cell_phase_functions.cpp:848-869). */
+int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
+    LumState st[2];
+    for (long long it = 0;; it += chunk) {
+        for (int k = 0; k < chunk; k++) TRY(L_lum_iter(c, fbase, nf));
+        HIPCHK(hipMemcpyAsync(st, c->lst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        bool done = true;
+        for (int f = fbase; f < fbase + nf; f++) done = done && st[f].done;
+        if (done) return ORG_OK;
+        if (it > (long long)c->p.rimax + 4) return note_error(ORG_ERELAX);
+    }
+}
+
+/* One timestep, run_with_active_gels.cpp:118-184.  lum_iters < 0: host-driven
+ * lumen/ECM loop; otherwise exactly that many (stop-tested) sweeps. */
+int enqueue_step(org_ctx *c, int lum_iters) {
+    TRY(L_centroid(c));             /* :123 */
+    TRY(L_prep(c, 3));              /* :133, :142 */
+    TRY(L_hlum(c));                 /* :134 */
+    TRY(L_psi(c));                  /* :139 */
+    TRY(L_lumecm_update(c));        /* :157-158 */
+    TRY(L_lum_normals(c, 0, 2));    /* :160-161 */
+    if (lum_iters < 0) TRY(run_lum_loop(c, 0, 2, 2));
+    else
+        for (int k = 0; k < lum_iters; k++) TRY(L_lum_iter(c, 0, 2));
+    TRY(L_spectral(c));             /* :130, :137, :154 */
+    TRY(L_cells(c, 3));             /* :164-172 */
+    TRY(L_commit_cells(c));         /* :175-178 */
+    TRY(L_commit_lumecm(c));        /* :181-182 */
+    return ORG_OK;
+}
+
+}  // namespace
+
+/* ======================================================================== */
+extern "C" {
+
+int org_last_error(void) { return g_last_error.load(); }
+
+const char *org_error_string(int code) {
+    switch (code) {
+    case ORG_OK: return "ok";
+    case ORG_EINVAL: return "invalid argument";
+    case ORG_EHIP: return "HIP runtime error";
+    case ORG_EEMPTY_CELL: return "a cell has no phi>0.5 pixel (centroid undefined)";
+    case ORG_EWINDOW: return "a cell window (+1 halo) leaves the grid interior";
+    case ORG_ERELAX: return "reshaping did not converge (rimax, or graph sweep budget exceeded)";
+    case ORG_ENOTBUILT: return "kernels not built for this device";
+    case ORG_ESTATE: return "call order / state error";
+    default: return "unknown error";
+    }
+}
+
+const char *org_build_info(void) { return "organoid_hip gfx950"; }
+
+int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
+    if (!p || !out) return note_error(ORG_EINVAL);
+    *out = nullptr;
+    if (p->zone_size != ORG_ZS_MAX) {
+        fprintf(stderr, "organoid: zone_size %d unsupported (the LDS FFT box is %d, zone_size %d)\n",
+                p->zone_size, ORG_MB, ORG_ZS_MAX);
+        return note_error(ORG_EINVAL);
+    }
+    if (p->ncell < 2 || p->ncell > 1024 || p->N < 2 * p->zone_size + 6 || p->n_current <= 0) {
+        fprintf(stderr, "organoid: need 2 <= ncell <= 1024, N >= 2*zone_size+6, n_current > 0\n");
+        return note_error(ORG_EINVAL);
+    }
+    org_ctx *c = new org_ctx();
+    c->p = *p;
+    c->dp = make_devp(*p);
+    c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
+    c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
+    c->grid_g = (int)std::min<size_t>((c->NN + 1023) / 1024, 1024);
+    int nt = (c->N + 31) / 32;
+    c->ntiles = nt * nt;
+    int rc = ORG_OK;
+    auto A = [&](auto **ptr, size_t n) { if (!rc) rc = dalloc(c, ptr, n); };
+    A(&c->phi, c->P); A(&c->rho, c->P); A(&c->hfunc, c->P);
+    A(&c->lum, c->NN); A(&c->ecm, c->NN); A(&c->psi, c->NN); A(&c->hlum, c->NN); A(&c->hecm, c->NN);
+    A(&c->bufL0, c->NN); A(&c->bufL1, c->NN); A(&c->bufE0, c->NN); A(&c->bufE1, c->NN);
+    A(&c->jlx, c->NN); A(&c->jly, c->NN); A(&c->jex, c->NN); A(&c->jey, c->NN);
+    A(&c->phin_win, (size_t)c->nc * c->W * c->W);
+    A(&c->rho_box, (size_t)c->nc * c->M * c->M);
+    A(&c->vxbox, (size_t)c->nc * c->M * c->M);
+    A(&c->vybox, (size_t)c->nc * c->M * c->M);
+    A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
+    A(&c->spec, (size_t)c->nc * c->M * c->M);
+    A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
+    A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
+    A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
+    if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
+    if (rc) { org_destroy(c); return rc; }
+    std::vector<double> s0v(c->nc);
+    for (int n = 0; n < c->nc; n++) s0v[n] = (p->use_s0_cell && s0_cell) ? s0_cell[n] : p->s0;
+    std::vector<int> cen(c->nc, c->N / 2);
+    if (hipMemcpy(c->s0, s0v.data(), sizeof(double) * c->nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->xc, cen.data(), sizeof(int) * c->nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->yc, cen.data(), sizeof(int) * c->nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->err, 0, sizeof(int)) != hipSuccess || hipMemset(c->counters, 0, 4 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(c->counts, 0, sizeof(int) * c->nc) != hipSuccess || hipMemset(c->lst, 0, 2 * sizeof(LumState)) != hipSuccess ||
+        hipMemset(c->hfunc, 0, sizeof(float) * c->P) != hipSuccess) {
+        org_destroy(c);
+        return note_error(ORG_EHIP);
+    }
+    *out = c;
+    return ORG_OK;
+}
+
+int org_destroy(org_ctx *c) {
+    if (!c) return ORG_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (int k = 0; k < ORG_K_COUNT; k++)
+        for (auto &pr : c->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (void *q : c->allocs) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ORG_OK;
+}
+
+static float *field_ptr(org_ctx *c, int f, size_t *n) {
+    switch (f) {
+    case ORG_F_PHI: *n = c->P; return c->phi;
+    case ORG_F_RHO: *n = c->P; return c->rho;
+    case ORG_F_HFUNC: *n = c->P; return c->hfunc;
+    case ORG_F_LUM: *n = c->NN; return c->lum;
+    case ORG_F_ECM: *n = c->NN; return c->ecm;
+    case ORG_F_PSI: *n = c->NN; return c->psi;
+    case ORG_F_HLUM: *n = c->NN; return c->hlum;
+    case ORG_F_HECM: *n = c->NN; return c->hecm;
+    case ORG_F_LUMN: *n = c->NN; return c->bufL0;
+    case ORG_F_ECMN: *n = c->NN; return c->bufE0;
+    default: *n = 0; return nullptr;
+    }
+}
+
+int org_set_field(org_ctx *c, int field, const float *host) {
+    TRY(check_ctx(c));
+    size_t n;
+    float *d = field_ptr(c, field, &n);
+    if (!d || !host) return note_error(ORG_EINVAL);
+    HIPCHK(hipMemcpyAsync(d, host, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (field == ORG_F_PHI || field == ORG_F_HFUNC) c->prepared = false;
+    return ORG_OK;
+}
+
+int org_get_field(org_ctx *c, int field, float *host) {
+    TRY(check_ctx(c));
+    size_t n;
+    float *d = field_ptr(c, field, &n);
+    if (!d || !host) return note_error(ORG_EINVAL);
+    if (field == ORG_F_LUMN || field == ORG_F_ECMN) {
+        /* the last iterate of the reshaping loop */
+        LumState st[2];
+        HIPCHK(hipMemcpyAsync(st, c->lst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (field == ORG_F_LUMN && st[0].cur) d = c->bufL1;
+        if (field == ORG_F_ECMN && st[1].cur) d = c->bufE1;
+    }
+    HIPCHK(hipMemcpyAsync(host, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ORG_OK;
+}
+
+int org_get_window(org_ctx *c, int which, float *host) {
+    TRY(check_ctx(c));
+    if (!host) return note_error(ORG_EINVAL);
+    const float *d = nullptr;
+    size_t n = 0;
+    switch (which) {
+    case ORG_W_PHIN: d = c->phin_win; n = (size_t)c->nc * c->W * c->W; break;
+    case ORG_W_RHON: d = c->rho_box; n = (size_t)c->nc * c->M * c->M; break;
+    case ORG_W_VX: d = c->vxbox; n = (size_t)c->nc * c->M * c->M; break;
+    case ORG_W_VY: d = c->vybox; n = (size_t)c->nc * c->M * c->M; break;
+    default: return note_error(ORG_EINVAL);
+    }
+    HIPCHK(hipMemcpyAsync(host, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ORG_OK;
+}
+
+int org_get_centres(org_ctx *c, int32_t *xc, int32_t *yc) {
+    TRY(check_ctx(c));
+    if (!xc || !yc) return note_error(ORG_EINVAL);
+    HIPCHK(hipMemcpyAsync(xc, c->xc, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(yc, c->yc, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ORG_OK;
+}
+
+int org_get_areas(org_ctx *c, double *s, double *f0) {
+    TRY(check_ctx(c));
+    if (!s || !f0) return note_error(ORG_EINVAL);
+    std::vector<float> f(c->nc);
+    HIPCHK(hipMemcpyAsync(s, c->s, sizeof(double) * c->nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f.data(), c->f0, sizeof(float) * c->nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < c->nc; n++) f0[n] = f[n];
+    return ORG_OK;
+}
+
+int org_get_iterations(org_ctx *c, int32_t *cells, int32_t *lum, int32_t *ecm) {
+    TRY(check_ctx(c));
+    if (!cells || !lum || !ecm) return note_error(ORG_EINVAL);
+    LumState st[2];
+    HIPCHK(hipMemcpyAsync(cells, c->counts, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(st, c->lst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *lum = st[0].ri; *ecm = st[1].ri;
+    return ORG_OK;
+}
+
+int org_prepare(org_ctx *c) {
+    TRY(check_ctx(c));
+    hipLaunchKernelGGL(k_init_box, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->box);
+    TRY(launch_check());
+    HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->prepared = true;
+    return ORG_OK;
+}
+
+int org_set_option(org_ctx *c, int option, int value) {
+    TRY(check_ctx(c));
+    switch (option) {
+    case ORG_OPT_WRITE_VELOCITY: c->write_v = value ? 1 : 0; break;
+    case ORG_OPT_GRAPH_LUM_ITERS:
+        if (value < 1 || value > 1000) return note_error(ORG_EINVAL);
+        c->graph_lum_iters = value;
+        break;
+    default: return note_error(ORG_EINVAL);
+    }
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return ORG_OK;
+}
+
+/* ---- per-function entry points ----------------------------------------- */
+int org_find_centres_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_centroid(c)); return harvest_device_error(c); }
+int org_compute_hfunc_cells(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 1); }
+int org_sum_s(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 2); }
+int org_compute_hfunc_lum_ecm(org_ctx *c) { TRY(need_prepared(c)); return L_hlum(c); }
+int org_compute_psi(org_ctx *c) { TRY(need_prepared(c)); return L_psi(c); }
+int org_update_all_cortices(org_ctx *c) { TRY(need_prepared(c)); return L_spectral(c); }
+int org_update_lumen_and_ECM(org_ctx *c) { TRY(need_prepared(c)); return L_lumecm_update(c); }
+int org_reshaping_lumen_or_ECM(org_ctx *c, int which_ecm) {
+    TRY(need_prepared(c));
+    int f = which_ecm ? 1 : 0;
+    TRY(L_lum_normals(c, f, 1));
+    TRY(run_lum_loop(c, f, 1, 1));
+    return harvest_device_error(c);
+}
+int org_update_all_cells(org_ctx *c) { TRY(need_prepared(c)); return L_cells(c, 1); }
+int org_reshaping_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_cells(c, 2)); return harvest_device_error(c); }
+int org_commit_step(org_ctx *c) {
+    TRY(need_prepared(c));
+    TRY(L_commit_cells(c));
+    TRY(L_commit_lumecm(c));
+    return harvest_device_error(c);
+}
+
+/* ---- batched form ------------------------------------------------------- */
+int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
+    (void)it0;
+    TRY(need_prepared(c));
+    if (nsteps < 0) return note_error(ORG_EINVAL);
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c, -1));
+        return harvest_device_error(c);
+    }
+    if (!c->gexec) {
+        bool prof = c->prof;
+        c->prof = false;                 /* no event records inside the graph */
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(c, c->graph_lum_iters);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->prof = prof;
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return note_error(ORG_EHIP);
+        c->graph = g;
+        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    return harvest_device_error(c);
+}
+
+int org_sync(org_ctx *c) {
+    TRY(check_ctx(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ORG_OK;
+}
+
+int org_profile_enable(org_ctx *c, int on) {
+    TRY(check_ctx(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->prof = on != 0;
+    for (int k = 0; k < ORG_K_COUNT; k++) c->ev_used[k] = 0;
+    return ORG_OK;
+}
+
+int org_kernel_time(org_ctx *c, int kid, double *ms, int64_t *launches) {
+    TRY(check_ctx(c));
+    if (kid < 0 || kid >= ORG_K_COUNT || !ms || !launches) return note_error(ORG_EINVAL);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t k = 0; k < c->ev_used[kid]; k++) {
+        float v = 0.0f;
+        HIPCHK(hipEventElapsedTime(&v, c->ev[kid][k].first, c->ev[kid][k].second));
+        tot += v;
+    }
+    *ms = tot;
+    *launches = (int64_t)c->ev_used[kid];
+    c->ev_used[kid] = 0;
+    return ORG_OK;
+}
+
+int org_debug_fft2d(float *data, int sign) {
+    if (!data || (sign != 1 && sign != -1)) return note_error(ORG_EINVAL);
+    float2 *d = nullptr;
+    size_t bytes = sizeof(float2) * ORG_MB * ORG_MB;
+    HIPCHK(hipMalloc(&d, bytes));
+    int rc = ORG_OK;
+    if (hipMemcpy(d, data, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = note_error(ORG_EHIP);
+    if (!rc) {
+        hipLaunchKernelGGL(k_fft_test, dim3(1), dim3(1024), 0, 0, d, sign);
+        rc = launch_check();
+    }
+    if (!rc && hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = note_error(ORG_EHIP);
+    (void)hipFree(d);
+    return rc;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,1014 @@
+/*
+ * organoid.hip -- gfx950 kernels and C-ABI of the organoid timestep library.
+ *
+ * One library-owned device state per organoid (org_ctx).  Full N x N planes
+ * are kept for the fields that persist between steps or that the caller can
+ * read back (phi, rho, hfunc per cell; lum, ECM, psi, h(lum), h(ECM) shared);
+ * everything the reference materialises as a full plane only to consume it
+ * inside a cell's window (|grad phi|, prod_norm_delta, the six turnover maps,
+ * pi_xx/xy/yy, J_x/J_y, temp_phase, the eleven FFT buffers) is recomputed
+ * inside the kernel that consumes it, on the window only.
+ *
+ * Why the window restriction is exact: phi and hfunc of cell n change only
+ * inside cell n's window (cell_phase_functions.cpp:160-163, 484-487;
+ * useful_functions.cpp:242-245); rho changes only inside the FFT box
+ * (rho_functions.cpp:253-265) and on the ghost ring (boundary_conditions_cells,
+ * useful_functions.cpp:391-420).  Each cell keeps a "support box": the
+ * bounding rectangle of its initial {phi > 0.5} U {hfunc != 0} grown by every
+ * window it has used.  The centroid scan (useful_functions.cpp:193-227) and
+ * the area sum (useful_functions.cpp:168-190) over the support box equal the
+ * full-plane ones; psi = sum_n hfunc_n (useful_functions.cpp:305-322) changes
+ * only where some hfunc_n changed, i.e. inside this step's windows, and is
+ * recomputed there from every cell whose support box covers the pixel, in
+ * cell order, so the FP32 sum has the same operands in the same order.
+ *
+ * Kernels per step (all cells per launch):
+ *   k_centroid      integer centroid + window check + support box update
+ *   k_prep          hfunc on the window, FP64 area sum over the support box
+ *   k_hlum          h(lum), h(ECM) full plane + FP64 sum of h(ECM)
+ *   k_psi           psi on tiles covering the union of this step's windows
+ *   k_lumecm_update lumen / ECM explicit step + Dirichlet edges
+ *   k_lum_normals   |grad| of the old field, normals of the new one, periodic ring
+ *   k_lum_iter      one reshaping sweep of lumen and ECM, FP64 deviation,
+ *                   device-side stop test (last-block reduction)
+ *   k_spectral      per cell: active stress, 2 forward FFTs, the per-mode
+ *                   friction-viscosity solve, Hermitian split, inverse FFT of
+ *                   the velocity, rho*v, forward FFT, spectral divergence,
+ *                   inverse FFT -> new rho on the FFT box  (4 2D FFTs of
+ *                   128 x 128 in LDS instead of the reference's 10)
+ *   k_cells         per cell: phase-field update, normals, reshaping loop in
+ *                   LDS until the deviation test passes, phi <- second-to-last
+ *   k_commit_cells  rho box write-back + ghost ring;  k_commit_lumecm.
+ */
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "organoid.h"
+
+#define ORG_MB 128                 /* FFT box side supported by k_spectral */
+#define ORG_LS (ORG_MB + 1)        /* LDS row stride (float2) */
+#define ORG_ZS_MAX 65              /* zone_size supported by k_cells */
+#define ORG_WH_MAX (2 * ORG_ZS_MAX + 2)
+#define ORG_CELL_SLOTS 17          /* ceil((2*65)^2 / 1024) */
+
+__constant__ float kCos128[128] = {
+    1.000000000e+00f, 9.987954562e-01f, 9.951847267e-01f, 9.891765100e-01f,
+    9.807852804e-01f, 9.700312532e-01f, 9.569403357e-01f, 9.415440652e-01f,
+    9.238795325e-01f, 9.039892931e-01f, 8.819212643e-01f, 8.577286100e-01f,
+    8.314696123e-01f, 8.032075315e-01f, 7.730104534e-01f, 7.409511254e-01f,
+    7.071067812e-01f, 6.715589548e-01f, 6.343932842e-01f, 5.956993045e-01f,
+    5.555702330e-01f, 5.141027442e-01f, 4.713967368e-01f, 4.275550934e-01f,
+    3.826834324e-01f, 3.368898534e-01f, 2.902846773e-01f, 2.429801799e-01f,
+    1.950903220e-01f, 1.467304745e-01f, 9.801714033e-02f, 4.906767433e-02f,
+    6.123233996e-17f, -4.906767433e-02f, -9.801714033e-02f, -1.467304745e-01f,
+    -1.950903220e-01f, -2.429801799e-01f, -2.902846773e-01f, -3.368898534e-01f,
+    -3.826834324e-01f, -4.275550934e-01f, -4.713967368e-01f, -5.141027442e-01f,
+    -5.555702330e-01f, -5.956993045e-01f, -6.343932842e-01f, -6.715589548e-01f,
+    -7.071067812e-01f, -7.409511254e-01f, -7.730104534e-01f, -8.032075315e-01f,
+    -8.314696123e-01f, -8.577286100e-01f, -8.819212643e-01f, -9.039892931e-01f,
+    -9.238795325e-01f, -9.415440652e-01f, -9.569403357e-01f, -9.700312532e-01f,
+    -9.807852804e-01f, -9.891765100e-01f, -9.951847267e-01f, -9.987954562e-01f,
+    -1.000000000e+00f, -9.987954562e-01f, -9.951847267e-01f, -9.891765100e-01f,
+    -9.807852804e-01f, -9.700312532e-01f, -9.569403357e-01f, -9.415440652e-01f,
+    -9.238795325e-01f, -9.039892931e-01f, -8.819212643e-01f, -8.577286100e-01f,
+    -8.314696123e-01f, -8.032075315e-01f, -7.730104534e-01f, -7.409511254e-01f,
+    -7.071067812e-01f, -6.715589548e-01f, -6.343932842e-01f, -5.956993045e-01f,
+    -5.555702330e-01f, -5.141027442e-01f, -4.713967368e-01f, -4.275550934e-01f,
+    -3.826834324e-01f, -3.368898534e-01f, -2.902846773e-01f, -2.429801799e-01f,
+    -1.950903220e-01f, -1.467304745e-01f, -9.801714033e-02f, -4.906767433e-02f,
+    -1.836970199e-16f, 4.906767433e-02f, 9.801714033e-02f, 1.467304745e-01f,
+    1.950903220e-01f, 2.429801799e-01f, 2.902846773e-01f, 3.368898534e-01f,
+    3.826834324e-01f, 4.275550934e-01f, 4.713967368e-01f, 5.141027442e-01f,
+    5.555702330e-01f, 5.956993045e-01f, 6.343932842e-01f, 6.715589548e-01f,
+    7.071067812e-01f, 7.409511254e-01f, 7.730104534e-01f, 8.032075315e-01f,
+    8.314696123e-01f, 8.577286100e-01f, 8.819212643e-01f, 9.039892931e-01f,
+    9.238795325e-01f, 9.415440652e-01f, 9.569403357e-01f, 9.700312532e-01f,
+    9.807852804e-01f, 9.891765100e-01f, 9.951847267e-01f, 9.987954562e-01f,
+};
+__constant__ float kSin128[128] = {
+    0.000000000e+00f, 4.906767433e-02f, 9.801714033e-02f, 1.467304745e-01f,
+    1.950903220e-01f, 2.429801799e-01f, 2.902846773e-01f, 3.368898534e-01f,
+    3.826834324e-01f, 4.275550934e-01f, 4.713967368e-01f, 5.141027442e-01f,
+    5.555702330e-01f, 5.956993045e-01f, 6.343932842e-01f, 6.715589548e-01f,
+    7.071067812e-01f, 7.409511254e-01f, 7.730104534e-01f, 8.032075315e-01f,
+    8.314696123e-01f, 8.577286100e-01f, 8.819212643e-01f, 9.039892931e-01f,
+    9.238795325e-01f, 9.415440652e-01f, 9.569403357e-01f, 9.700312532e-01f,
+    9.807852804e-01f, 9.891765100e-01f, 9.951847267e-01f, 9.987954562e-01f,
+    1.000000000e+00f, 9.987954562e-01f, 9.951847267e-01f, 9.891765100e-01f,
+    9.807852804e-01f, 9.700312532e-01f, 9.569403357e-01f, 9.415440652e-01f,
+    9.238795325e-01f, 9.039892931e-01f, 8.819212643e-01f, 8.577286100e-01f,
+    8.314696123e-01f, 8.032075315e-01f, 7.730104534e-01f, 7.409511254e-01f,
+    7.071067812e-01f, 6.715589548e-01f, 6.343932842e-01f, 5.956993045e-01f,
+    5.555702330e-01f, 5.141027442e-01f, 4.713967368e-01f, 4.275550934e-01f,
+    3.826834324e-01f, 3.368898534e-01f, 2.902846773e-01f, 2.429801799e-01f,
+    1.950903220e-01f, 1.467304745e-01f, 9.801714033e-02f, 4.906767433e-02f,
+    1.224646799e-16f, -4.906767433e-02f, -9.801714033e-02f, -1.467304745e-01f,
+    -1.950903220e-01f, -2.429801799e-01f, -2.902846773e-01f, -3.368898534e-01f,
+    -3.826834324e-01f, -4.275550934e-01f, -4.713967368e-01f, -5.141027442e-01f,
+    -5.555702330e-01f, -5.956993045e-01f, -6.343932842e-01f, -6.715589548e-01f,
+    -7.071067812e-01f, -7.409511254e-01f, -7.730104534e-01f, -8.032075315e-01f,
+    -8.314696123e-01f, -8.577286100e-01f, -8.819212643e-01f, -9.039892931e-01f,
+    -9.238795325e-01f, -9.415440652e-01f, -9.569403357e-01f, -9.700312532e-01f,
+    -9.807852804e-01f, -9.891765100e-01f, -9.951847267e-01f, -9.987954562e-01f,
+    -1.000000000e+00f, -9.987954562e-01f, -9.951847267e-01f, -9.891765100e-01f,
+    -9.807852804e-01f, -9.700312532e-01f, -9.569403357e-01f, -9.415440652e-01f,
+    -9.238795325e-01f, -9.039892931e-01f, -8.819212643e-01f, -8.577286100e-01f,
+    -8.314696123e-01f, -8.032075315e-01f, -7.730104534e-01f, -7.409511254e-01f,
+    -7.071067812e-01f, -6.715589548e-01f, -6.343932842e-01f, -5.956993045e-01f,
+    -5.555702330e-01f, -5.141027442e-01f, -4.713967368e-01f, -4.275550934e-01f,
+    -3.826834324e-01f, -3.368898534e-01f, -2.902846773e-01f, -2.429801799e-01f,
+    -1.950903220e-01f, -1.467304745e-01f, -9.801714033e-02f, -4.906767433e-02f,
+};
+
+
+/* ------------------------------------------------------------------------ */
+/* device-side constants, rounded to float once as the reference's float
+ * globals are (constants.h:4-124).  Label order: 0 apical, 1 basal,
+ * 2 and 3 lateral (rho_functions.cpp:355-397). */
+struct DevP {
+    int N, nc, zs, W, M, rimax;
+    float dt, dx, sigma, cpl, tau, alpha, beta, eta;
+    float tau_lum, beta_lc, eta_lc, xi_lum, alpha_E, tau_E, beta_ec, eta_ec, xi_E, beta_le;
+    float wi, rdt, friction, ncur;
+    float visc[4], aort[4], atan_[4], bb[4], gen[4], deg[4];
+    double thr_cell, rdev0_cell, thr_lum, rdev0_lum;
+};
+
+static __device__ __forceinline__ void set_err(int *err, int code) { atomicOr(err, 1 << code); }
+
+static __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+static __device__ __forceinline__ float2 caddf(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+static __device__ __forceinline__ float2 csubf(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+/* h(u) = u^2 (3 - 2u), useful_functions.cpp:247 */
+static __device__ __forceinline__ float hfun(float u) { return u * u * (3.0f - 2.0f * u); }
+
+/* ---- block reductions (blockDim multiple of 64) --------------------------- */
+static __device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+static __device__ __forceinline__ long long wave_sum_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+/* Sum over the block, result valid in every thread.  red: >= 16 doubles. */
+static __device__ double block_sum_d(double v, double *red) {
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum_d(v);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < nw; k++) s += red[k];
+    return s;
+}
+
+/* ---- small in-register DFTs (radix-2, natural order in and out) ----------- */
+/* cos / sin of 2*pi*k/16 for k = 0..7 */
+#define C16_0 1.0f
+#define C16_1 0.923879533f
+#define C16_2 0.707106781f
+#define C16_3 0.382683432f
+template <int S>
+static __device__ __forceinline__ float2 w16(int k) {
+    /* exp(S * 2 pi i k / 16), k compile-time after unrolling */
+    const float c[8] = {C16_0, C16_1, C16_2, C16_3, 0.0f, -C16_3, -C16_2, -C16_1};
+    const float s[8] = {0.0f, C16_3, C16_2, C16_1, 1.0f, C16_1, C16_2, C16_3};
+    return make_float2(c[k], S * s[k]);
+}
+template <int N>
+static __device__ __forceinline__ int brev(int i) {
+    int r = 0;
+    for (int b = 1, m = N >> 1; b < N; b <<= 1, m >>= 1)
+        if (i & b) r |= m;
+    return r;
+}
+template <int N, int S>
+static __device__ __forceinline__ void dft_reg(float2 *v) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        int j = brev<N>(i);
+        if (i < j) { float2 t = v[i]; v[i] = v[j]; v[j] = t; }
+    }
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < N; i += len) {
+#pragma unroll
+            for (int k = 0; k < len / 2; k++) {
+                float2 w = w16<S>(k * (16 / len));
+                float2 b = cmulf(v[i + k + len / 2], w);
+                v[i + k + len / 2] = csubf(v[i + k], b);
+                v[i + k] = caddf(v[i + k], b);
+            }
+        }
+    }
+}
+
+/* One batch of 128-point DFTs over the lines of a 128 x 128 complex array in
+ * LDS, 8 threads per line, 1024 threads (128 = 16 x 8):
+ *   X[k2 + 16 k1] = sum_j1 W8^(j1 k1) W128^(j1 k2) sum_j2 x[j1 + 8 j2] W16^(j2 k2)
+ * es = element stride, ls = line stride (float2 units). */
+template <int S>
+static __device__ __forceinline__ void fft128_lines(float2 *sm, const float2 *tw, int es, int ls) {
+    const int t = threadIdx.x, line = t >> 3, sub = t & 7;
+    float2 *base = sm + line * ls;
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) v[m] = base[(sub + 8 * m) * es];
+    dft_reg<16, S>(v);
+#pragma unroll
+    for (int k2 = 1; k2 < 16; k2++) {
+        float2 w = tw[(sub * k2) & 127];
+        w.y *= (float)S;
+        v[k2] = cmulf(v[k2], w);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) base[(sub + 8 * k2) * es] = v[k2];
+    __syncthreads();
+    float2 u[16];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int j1 = 0; j1 < 8; j1++) u[8 * h + j1] = base[(j1 + 8 * (2 * sub + h)) * es];
+    __syncthreads();
+    dft_reg<8, S>(u);          /* W8 = W16^2: dft_reg<8> uses w16(k * 2) */
+    dft_reg<8, S>(u + 8);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k1 = 0; k1 < 8; k1++) base[((2 * sub + h) + 16 * k1) * es] = u[8 * h + k1];
+    __syncthreads();
+}
+
+/* Unnormalised 2D DFT of the 128 x 128 array in LDS (row stride ORG_LS),
+ * sign S as cufftExecC2C (useful_functions.cpp:650, 658). */
+template <int S>
+static __device__ __forceinline__ void fft2d_128(float2 *sm, const float2 *tw) {
+    fft128_lines<S>(sm, tw, 1, ORG_LS);    /* rows */
+    fft128_lines<S>(sm, tw, ORG_LS, 1);    /* columns */
+}
+
+/* ---- per-mode friction-viscosity solve, rho_functions.cpp:164-186 -------- */
+static __device__ __forceinline__ void velocity_mode(float eta, float xi, float kx, float ky,
+                                                     float2 pxx, float2 pxy, float2 pyy,
+                                                     float2 &vx, float2 &vy) {
+    float D = -(4.0f * eta * eta * kx * kx * ky * ky + xi * xi + 3.0f * eta * xi * (kx * kx + ky * ky)
+                + 2.0f * eta * eta * (kx * kx * kx * kx + ky * ky * ky * ky));
+    float A = xi + 2.0f * eta * (kx * kx + ky * ky / 2.0f);
+    float ex = eta * kx * ky;
+    float2 y, x;
+    y.x = (ex * (kx * pxx.y + ky * pxy.y) - A * (ky * pyy.y + kx * pxy.y)) / D;
+    y.y = (-ex * (kx * pxx.x + ky * pxy.x) + A * (ky * pyy.x + kx * pxy.x)) / D;
+    x.x = (kx * pxx.y + ky * pxy.y - ex * y.x) / A;
+    x.y = (-kx * pxx.x - ky * pxy.x - ex * y.y) / A;
+    vx = x; vy = y;
+}
+
+/* Turnover label of one pixel, rho_functions.cpp:324-353 (strict '>'). */
+static __device__ __forceinline__ int turnover_label(float ph, float l, float e, float p1, float p2) {
+    float c1l = ph * l, c1e = ph * e;
+    int index; float mx;
+    if (c1l > c1e) { index = 0; mx = c1l; } else { index = 1; mx = c1e; }
+    float c12 = ph * p1;
+    if (c12 > mx) { index = 2; mx = c12; }
+    float c13 = ph * p2;
+    if (c13 > mx) { index = 3; }
+    return index;
+}
+
+/* ======================================================================== */
+/* k_init_box: support box of each cell from the uploaded phi / hfunc.      */
+__global__ void k_init_box(DevP P, const float *__restrict__ phi, const float *__restrict__ hfunc,
+                           int *__restrict__ box) {
+    __shared__ int sred[4];
+    const int n = blockIdx.x, N = P.N;
+    const size_t NN = (size_t)N * N;
+    int i0 = N, i1 = -1, j0 = N, j1 = -1;
+    for (size_t e = threadIdx.x; e < NN; e += blockDim.x) {
+        int i = (int)(e / N), j = (int)(e % N);
+        if (phi[n * NN + e] > 0.5f || hfunc[n * NN + e] != 0.0f) {
+            i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
+        }
+    }
+    if (threadIdx.x == 0) { sred[0] = N; sred[1] = -1; sred[2] = N; sred[3] = -1; }
+    __syncthreads();
+    atomicMin(&sred[0], i0); atomicMax(&sred[1], i1); atomicMin(&sred[2], j0); atomicMax(&sred[3], j1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        box[4 * n + 0] = sred[0]; box[4 * n + 1] = sred[1] + 1;   /* half-open */
+        box[4 * n + 2] = sred[2]; box[4 * n + 3] = sred[3] + 1;
+    }
+}
+
+/* k_centroid: find_centres_cells (useful_functions.cpp:193-227) over the
+ * support box, window check, support box += window. */
+__global__ __launch_bounds__(256) void k_centroid(DevP P, const float *__restrict__ phi, int *__restrict__ xc,
+                                                  int *__restrict__ yc, int *__restrict__ box,
+                                                  int *__restrict__ err) {
+    __shared__ long long red[3][4];
+    const int n = blockIdx.x, N = P.N;
+    const size_t NN = (size_t)N * N;
+    const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
+    const int bw = max(j1 - j0, 0), bh = max(i1 - i0, 0);
+    long long sx = 0, sy = 0, c = 0;
+    const float *ph = phi + n * NN;
+    for (int e = threadIdx.x; e < bw * bh; e += blockDim.x) {
+        int i = i0 + e / bw, j = j0 + e % bw;
+        if (ph[(size_t)i * N + j] > 0.5f) { sx += i; sy += j; c += 1; }
+    }
+    sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = sx; red[1][w] = sy; red[2][w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long X = 0, Y = 0, C = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
+        int x = xc[n], y = yc[n];
+        if (C == 0) {
+            set_err(err, ORG_EEMPTY_CELL);
+        } else {
+            x = (int)(X / C);
+            y = (int)(Y / C);
+        }
+        /* window [x-zs, x+zs) plus one halo pixel inside rows/cols 1..N-2 */
+        int lo = P.zs + 2, hi = N - 2 - P.zs;
+        if (x < lo || x > hi || y < lo || y > hi) {
+            set_err(err, ORG_EWINDOW);
+            x = min(max(x, lo), hi);
+            y = min(max(y, lo), hi);
+        }
+        xc[n] = x; yc[n] = y;
+        box[4 * n + 0] = min(i0, x - P.zs); box[4 * n + 1] = max(i1, x + P.zs);
+        box[4 * n + 2] = min(j0, y - P.zs); box[4 * n + 3] = max(j1, y + P.zs);
+        if (bw == 0 || bh == 0) {
+            box[4 * n + 0] = x - P.zs; box[4 * n + 1] = x + P.zs;
+            box[4 * n + 2] = y - P.zs; box[4 * n + 3] = y + P.zs;
+        }
+    }
+}
+
+/* k_prep: mode&1 -> hfunc on the window (useful_functions.cpp:233-251);
+ * mode&2 -> s_n = sum over the support box of hfunc*dx*dx in FP64 and
+ * f0_n = alpha*(s0_n - s_n) (useful_functions.cpp:168-190). */
+__global__ __launch_bounds__(256) void k_prep(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
+                                              const int *__restrict__ xc, const int *__restrict__ yc,
+                                              const int *__restrict__ box, const double *__restrict__ s0,
+                                              double *__restrict__ s, float *__restrict__ f0, int mode) {
+    __shared__ double red[16];
+    const int n = blockIdx.x, N = P.N, W = P.W;
+    const size_t NN = (size_t)N * N;
+    const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
+    const float *ph = phi + n * NN;
+    float *hf = hfunc + n * NN;
+    if (mode & 1) {
+        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+            size_t g = (size_t)(x0 + e / W) * N + (y0 + e % W);
+            hf[g] = hfun(ph[g]);
+        }
+    }
+    if (mode & 2) {
+        const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
+        const int bw = j1 - j0, bh = i1 - i0;
+        double acc = 0.0;
+        for (int e = threadIdx.x; e < bw * bh; e += blockDim.x) {
+            int i = i0 + e / bw, j = j0 + e % bw;
+            size_t g = (size_t)i * N + j;
+            bool inw = (mode & 1) && i >= x0 && i < x0 + W && j >= y0 && j < y0 + W;
+            float h = inw ? hfun(ph[g]) : hf[g];   /* same value the window pass stored */
+            acc += (double)(h * P.dx * P.dx);
+        }
+        double tot = block_sum_d(acc, red);
+        if (threadIdx.x == 0) {
+            s[n] = tot;
+            f0[n] = P.alpha * ((float)s0[n] - (float)tot);
+        }
+    }
+}
+
+/* k_hlum: compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285)
+ * and the FP64 sum_ecm of update_lumen_and_ECM (cell_phase_functions.cpp:376-386),
+ * reduced by the last block to finish (fixed order: deterministic). */
+__global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
+                                              float *__restrict__ hlum, float *__restrict__ hecm,
+                                              double *__restrict__ part, unsigned *__restrict__ counter,
+                                              double *__restrict__ sum_out) {
+    __shared__ double red[16];
+    __shared__ bool last;
+    const size_t NN = (size_t)P.N * P.N;
+    double acc = 0.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
+        float a = lum[e], b = ecm[e];
+        float hb = hfun(b);
+        hlum[e] = hfun(a);
+        hecm[e] = hb;
+        acc += (double)(hb * P.dx * P.dx);
+    }
+    double tot = block_sum_d(acc, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tot;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double v = 0.0;
+        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        /* fixed-order combine: per-thread strided partials then block sum */
+        double t2 = block_sum_d(v, red);
+        if (threadIdx.x == 0) { *sum_out = t2; *counter = 0u; }
+    }
+}
+
+/* k_psi: compute_psi_kernels (useful_functions.cpp:305-322) on 32 x 32 tiles
+ * of the union of this step's windows. */
+__global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ xc,
+                                             const int *__restrict__ yc, const int *__restrict__ box,
+                                             float *__restrict__ psi) {
+    __shared__ int ub[4];
+    __shared__ int list[1024];
+    __shared__ int cnt;
+    const int N = P.N, ntx = (N + 31) / 32;
+    const int ti0 = (blockIdx.x / ntx) * 32, tj0 = (blockIdx.x % ntx) * 32;
+    if (threadIdx.x == 0) { ub[0] = N; ub[1] = 0; ub[2] = N; ub[3] = 0; }
+    __syncthreads();
+    int a0 = N, a1 = 0, b0 = N, b1 = 0;
+    for (int n = threadIdx.x; n < P.nc; n += blockDim.x) {
+        a0 = min(a0, xc[n] - P.zs); a1 = max(a1, xc[n] + P.zs);
+        b0 = min(b0, yc[n] - P.zs); b1 = max(b1, yc[n] + P.zs);
+    }
+    atomicMin(&ub[0], a0); atomicMax(&ub[1], a1); atomicMin(&ub[2], b0); atomicMax(&ub[3], b1);
+    __syncthreads();
+    const int u0 = ub[0], u1 = ub[1], v0 = ub[2], v1 = ub[3];
+    if (ti0 >= u1 || ti0 + 32 <= u0 || tj0 >= v1 || tj0 + 32 <= v0) return;
+    /* ordered list of cells whose support box meets the tile (wave 0) */
+    if (threadIdx.x < 64) {
+        int base = 0;
+        for (int c0 = 0; c0 < P.nc; c0 += 64) {
+            int n = c0 + threadIdx.x;
+            bool hit = false;
+            if (n < P.nc) {
+                const int *bx = box + 4 * n;
+                hit = !(ti0 >= bx[1] || ti0 + 32 <= bx[0] || tj0 >= bx[3] || tj0 + 32 <= bx[2]);
+            }
+            unsigned long long m = __ballot(hit);
+            if (hit) {
+                unsigned long long below = m & ((1ull << threadIdx.x) - 1ull);
+                int pos = base + __popcll(below);
+                if (pos < 1024) list[pos] = n;
+            }
+            base += __popcll(m);
+        }
+        if (threadIdx.x == 0) cnt = min(base, 1024);
+    }
+    __syncthreads();
+    const size_t NN = (size_t)N * N;
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        int i = ti0 + (e >> 5), j = tj0 + (e & 31);
+        if (i >= N || j >= N || i < u0 || i >= u1 || j < v0 || j >= v1) continue;
+        float acc = 0.0f;
+        for (int k = 0; k < cnt; k++) {
+            int n = list[k];
+            const int *bx = box + 4 * n;
+            if (i >= bx[0] && i < bx[1] && j >= bx[2] && j < bx[3]) acc += hfunc[n * NN + (size_t)i * N + j];
+        }
+        psi[(size_t)i * N + j] = acc;
+    }
+}
+
+/* 9-point Laplacian numerator, cell_phase_functions.cpp:198 and :416 */
+#define L9AT(t, g, N) (-12.0f * (t)[g] + 2.0f * ((t)[(g) + (N)] + (t)[(g) - (N)] + (t)[(g) + 1] + (t)[(g) - 1]) \
+                       + (t)[(g) + (N) + 1] + (t)[(g) + (N) - 1] + (t)[(g) - (N) - 1] + (t)[(g) - (N) + 1])
+
+/* lumen / ECM reshaping state: [0] done, [1] ri, [2] current buffer */
+struct LumState { int done, ri, cur, pad; };
+
+/* k_lumecm_update: update_lumen_and_ECM (cell_phase_functions.cpp:395-443),
+ * then boundary_conditions_lumen_and_ECM (useful_functions.cpp:439-462).
+ * Writes the first iterate of both reshaping loops and resets their state. */
+__global__ __launch_bounds__(256) void k_lumecm_update(DevP P, const float *__restrict__ lum,
+                                                       const float *__restrict__ ecm, const float *__restrict__ psi,
+                                                       const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                                       const double *__restrict__ sum_ecm, float *__restrict__ lumn,
+                                                       float *__restrict__ ecmn, LumState *__restrict__ lst) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    if (blockIdx.x == 0 && threadIdx.x < 2) {
+        bool run = P.rdev0_lum > P.thr_lum;
+        lst[threadIdx.x].done = run ? 0 : 1;
+        lst[threadIdx.x].ri = 0;
+        lst[threadIdx.x].cur = 0;
+    }
+    const float se = (float)(*sum_ecm);
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(e / N), j = (int)(e % N);
+        if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
+            lumn[e] = 0.0f;      /* lumen edges and corners 0 */
+            ecmn[e] = 1.0f;      /* ECM edges and corners 1 (:449-460) */
+            continue;
+        }
+        float l = lum[e], E = ecm[e], ps = psi[e];
+        float lap = L9AT(psi, e, (size_t)N) * inv4;
+        lumn[e] = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
+                          (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
+        ecmn[e] = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
+                          (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
+    }
+}
+
+/* periodic ghost-ring source index of boundary_conditions_periodic /
+ * boundary_conditions_cells (useful_functions.cpp:391-420, 505-526) */
+static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N - 2 : (i == N - 1 ? 1 : i); }
+
+/* k_lum_normals: |grad| of the old field (cell_phase_functions.cpp:722-741),
+ * normals from the new field's gradient (:744-771) with the periodic ring.
+ * blockIdx.y: 0 lumen, 1 ECM. */
+__global__ __launch_bounds__(256) void k_lum_normals(DevP P, const float *__restrict__ lum_old,
+                                                     const float *__restrict__ ecm_old, const float *__restrict__ lumn,
+                                                     const float *__restrict__ ecmn, float *__restrict__ jlx,
+                                                     float *__restrict__ jly, float *__restrict__ jex,
+                                                     float *__restrict__ jey, int fbase) {
+    const int N = P.N, f = fbase + blockIdx.y;
+    const size_t NN = (size_t)N * N;
+    const float *old = f ? ecm_old : lum_old;
+    const float *nw = f ? ecmn : lumn;
+    float *jx = f ? jex : jlx, *jy = f ? jey : jly;
+    const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
+        int i = ring_src((int)(e / N), N), j = ring_src((int)(e % N), N);
+        size_t g = (size_t)i * N + j;
+        float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+        float ng = sqrtf(c * (a * a + b * b));
+        float vx = 0.0f, vy = 0.0f;
+        if (ng != 0.0f) {
+            vx = h * (nw[g + N] - nw[g - N]) / ng;
+            vy = h * (nw[g + 1] - nw[g - 1]) / ng;
+        }
+        jx[e] = vx; jy[e] = vy;
+    }
+}
+
+/* Conservative interface flux (cell_phase_functions.cpp:511-547 and
+ * :802-838) applied to un at (i,j); u, jx, jy with row strides su, sj. */
+static __device__ __forceinline__ float reshape_flux(float un, const float *u, int su, const float *jx,
+                                                     const float *jy, int sj, float ka, float kc) {
+#define U_(a, b) u[(a) * su + (b)]
+#define JX_(a, b) jx[(a) * sj + (b)]
+#define JY_(a, b) jy[(a) * sj + (b)]
+    float u00 = U_(0, 0);
+    float up0 = U_(1, 0), um0 = U_(-1, 0), u0p = U_(0, 1), u0m = U_(0, -1);
+    float upp = U_(1, 1), upm = U_(1, -1), ump = U_(-1, 1), umm = U_(-1, -1);
+    float jx0 = JX_(0, 0), jy0 = JY_(0, 0);
+    float jxp = JX_(1, 0), jyp = JY_(1, 0), jxm = JX_(-1, 0), jym = JY_(-1, 0);
+    float jx0p = JX_(0, 1), jy0p = JY_(0, 1), jx0m = JX_(0, -1), jy0m = JY_(0, -1);
+    float g00 = u00 * (1.0f - u00);
+    float a = 0.5f * (jxp + jx0);
+    un += a * (ka * (up0 - u00) * a + ka * 0.25f * (upp + u0p - upm - u0m) * 0.5f * (jyp + jy0)
+               - kc * 0.5f * (up0 * (1.0f - up0) + g00));
+    a = 0.5f * (jy0p + jy0);
+    un += a * (ka * (u0p - u00) * a + ka * 0.25f * (upp + up0 - ump - um0) * 0.5f * (jx0p + jx0)
+               - kc * 0.5f * (u0p * (1.0f - u0p) + g00));
+    a = 0.5f * (jx0 + jxm);
+    un -= a * (ka * (u00 - um0) * a + ka * 0.25f * (u0p + ump - u0m - umm) * 0.5f * (jy0 + jym)
+               - kc * 0.5f * (g00 + um0 * (1.0f - um0)));
+    a = 0.5f * (jy0 + jy0m);
+    un -= a * (ka * (u00 - u0m) * a + ka * 0.25f * (up0 + upm - um0 - umm) * 0.5f * (jx0 + jx0m)
+               - kc * 0.5f * (g00 + u0m * (1.0f - u0m)));
+    return un;
+#undef U_
+#undef JX_
+#undef JY_
+}
+
+/* k_lum_iter: one sweep of reshaping_lumen_or_ECM's loop
+ * (cell_phase_functions.cpp:774-869) for the lumen (blockIdx.y = 0) and the
+ * ECM (1): X_{k+1} = X_k + flux(X_k) inside, Dirichlet edges (value side)
+ * and zero corners (useful_functions.cpp:422-437), deviation
+ * sum |X_{k+1}-X_k| dx^2 in FP64; the last block to finish applies the stop
+ * test.  A field whose loop has stopped skips the sweep. */
+__global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bufL0, float *__restrict__ bufL1,
+                                                  float *__restrict__ bufE0, float *__restrict__ bufE1,
+                                                  const float *__restrict__ jlx, const float *__restrict__ jly,
+                                                  const float *__restrict__ jex, const float *__restrict__ jey,
+                                                  LumState *__restrict__ lst, double *__restrict__ part,
+                                                  unsigned *__restrict__ counter, int *__restrict__ err, int fbase) {
+    __shared__ double red[16];
+    __shared__ bool last;
+    const int f = fbase + blockIdx.y, N = P.N;
+    const size_t NN = (size_t)N * N;
+    LumState s = lst[f];
+    if (s.done) return;
+    float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
+    const float *src = (s.ri & 1) ? b1 : b0;
+    float *dst = (s.ri & 1) ? b0 : b1;
+    const float *jx = f ? jex : jlx, *jy = f ? jey : jly;
+    const float side = f ? 1.0f : 0.0f;
+    const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
+    const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
+    double acc = 0.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(e / N), j = (int)(e % N);
+        float v;
+        bool ei = (i == 0 || i == N - 1), ej = (j == 0 || j == N - 1);
+        if (ei && ej) v = 0.0f;
+        else if (ei || ej) v = side;
+        else v = reshape_flux(src[e], src + e, N, jx + e, jy + e, N, ka, kc);
+        dst[e] = v;
+        acc += (double)(fabsf(v - src[e]) * P.dx * P.dx);
+    }
+    double tot = block_sum_d(acc, red);
+    if (threadIdx.x == 0) {
+        part[f * gridDim.x + blockIdx.x] = tot;
+        __threadfence();
+        unsigned prev = atomicAdd(&counter[f], 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double v = 0.0;
+        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+            v += __hip_atomic_load(&part[f * gridDim.x + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        double rdev = block_sum_d(v, red);
+        if (threadIdx.x == 0) {
+            LumState o = s;
+            o.cur = (s.ri + 1) & 1;
+            if (s.ri > P.rimax) { set_err(err, ORG_ERELAX); o.done = 1; }
+            o.ri = s.ri + 1;
+            if (!(rdev > P.thr_lum)) o.done = 1;
+            lst[f] = o;
+            counter[f] = 0u;
+        }
+    }
+}
+
+/* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
+ * run_with_active_gels.cpp:181-182) from the buffer holding the last iterate. */
+__global__ __launch_bounds__(256) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
+                                                       const float *__restrict__ bufL0, const float *__restrict__ bufL1,
+                                                       const float *__restrict__ bufE0, const float *__restrict__ bufE1,
+                                                       const LumState *__restrict__ lst, int *__restrict__ err) {
+    const size_t NN = (size_t)P.N * P.N;
+    LumState sl = lst[0], se = lst[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (!sl.done || !se.done)) set_err(err, ORG_ERELAX);
+    const float *L = sl.cur ? bufL1 : bufL0;
+    const float *E = se.cur ? bufE1 : bufE0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
+        lum[e] = L[e];
+        ecm[e] = E[e];
+    }
+}
+
+/* ======================================================================== */
+/* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
+ *
+ * One workgroup (1024 threads) per cell, one 128 x 128 complex field in LDS.
+ * By linearity of the reference's spectral update, Re(F^-1[F(rho) +
+ * dt F(react) + D]) / M^2 = rho + dt*react + Re(F^-1 D) / M^2, so the
+ * transforms of rho and of the reaction term (rho_functions.cpp:415-422,
+ * 462-464) cancel and are not taken.  Real pairs share one complex
+ * transform: F(pi_xx + i pi_xy), F(pi_yy), F^-1(H v_x + i H v_y) where
+ * H X(k) = (X(k) + conj X(-k)) / 2 is the part whose inverse is real (the
+ * reference keeps only Re, rho_functions.cpp:279), F(rho v_x + i rho v_y),
+ * and F^-1(H D).  The per-mode solve uses the viscosity of the real-space
+ * pixel with the mode's indices (rho_functions.cpp:149-150, 170). */
+__global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restrict__ phi,
+                                                   const float *__restrict__ rho, const float *__restrict__ lum,
+                                                   const float *__restrict__ ecm, const int *__restrict__ xc,
+                                                   const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                   float *__restrict__ rho_box, float *__restrict__ vxbox,
+                                                   float *__restrict__ vybox, int write_v) {
+    __shared__ float2 sm[ORG_MB * ORG_LS];
+    __shared__ float2 tw[128];
+    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
+    const int MM = ORG_MB * ORG_MB;
+    const size_t NN = (size_t)N * N;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;   /* FFT box origin, useful_functions.cpp:618-619 */
+    const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
+    if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
+    const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
+    float2 *SA = scratch + (size_t)n * MM;
+    float *rbox = rho_box + (size_t)n * MM;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
+    const float dnorm = 1.0f / (2.50662827463f * P.sigma);   /* 1/(sqrt(2 pi) sigma) */
+    const float kscale = 6.28318530718f / (ORG_MB * P.dx);
+    const float MMf = (float)MM;
+
+    float rr[16], pyy[16];
+    unsigned lab = 0u;
+    /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+        float v = ph[g];
+        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        int idx = turnover_label(v, lum[g], ecm[g], ph1[g], ph2[g]);
+        lab |= (unsigned)idx << (2 * q);
+        float r = rh[g];
+        float at = P.atan_[idx], ao = P.aort[idx], bb = P.bb[idx];
+        float r3 = r * r * r;
+        float base = -at * r3 + bb * r3 * r;
+        float pxx = base, pxy = 0.0f, pyv = base;
+        if (ng > 0.0f) {
+            float c = (at - ao) * r3 * inv4 / ng / ng;
+            pxx += c * dxp * dxp;
+            pxy += c * dxp * dyp;
+            pyv += c * dyp * dyp;
+        }
+        float d = v - 0.5f;
+        float pnd = ng * expf(-d * d * isig2) * dnorm;
+        float react = -P.deg[idx] * r + P.gen[idx] * pnd;
+        rbox[e] = r + P.dt * react;
+        rr[q] = r;
+        pyy[q] = pyv;
+        sm[i * ORG_LS + j] = make_float2(pxx, pxy);
+    }
+    __syncthreads();
+    fft2d_128<-1>(sm, tw);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        SA[e] = sm[i * ORG_LS + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        sm[i * ORG_LS + j] = make_float2(pyy[q], 0.0f);
+    }
+    __syncthreads();
+    fft2d_128<-1>(sm, tw);
+    /* P4: split, per-mode solve (rho_functions.cpp:133-190) */
+    float2 vx[16], vy[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+        float2 A = SA[e], Am = SA[im * ORG_MB + jm], B = sm[i * ORG_LS + j];
+        float2 pxx = make_float2(0.5f * (A.x + Am.x), 0.5f * (A.y - Am.y));
+        float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
+        int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
+        float eta = P.visc[(lab >> (2 * q)) & 3u];
+        velocity_mode(eta, P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vx[q], vy[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        sm[i * ORG_LS + j] = vx[q];
+        SA[e] = vy[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+        float2 xm = sm[im * ORG_LS + jm], ym = SA[im * ORG_MB + jm];
+        float2 hx = make_float2(0.5f * (vx[q].x + xm.x), 0.5f * (vx[q].y - xm.y));
+        float2 hy = make_float2(0.5f * (vy[q].x + ym.x), 0.5f * (vy[q].y - ym.y));
+        vx[q] = make_float2(hx.x - hy.y, hx.y + hy.x);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        sm[i * ORG_LS + j] = vx[q];
+    }
+    __syncthreads();
+    fft2d_128<1>(sm, tw);
+    /* P5: v = Re(.)/M^2 ; rho * v (rho_functions.cpp:267-284) */
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        float2 z = sm[i * ORG_LS + j];
+        if (write_v) {
+            vxbox[(size_t)n * MM + e] = z.x / MMf;
+            vybox[(size_t)n * MM + e] = z.y / MMf;
+        }
+        sm[i * ORG_LS + j] = make_float2(rr[q] * z.x / MMf, rr[q] * z.y / MMf);
+    }
+    __syncthreads();
+    fft2d_128<-1>(sm, tw);
+    /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249) */
+    const float cdt = P.dt * kscale;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+        float2 Y = sm[i * ORG_LS + j], Ym = sm[im * ORG_LS + jm];
+        float2 Pk = make_float2(0.5f * (Y.x + Ym.x), 0.5f * (Y.y - Ym.y));   /* F(rho vx)(k) */
+        float2 Qk = make_float2(0.5f * (Y.y + Ym.y), 0.5f * (Ym.x - Y.x));   /* F(rho vy)(k) */
+        float ir = (float)((i < ORG_MB / 2) ? i : i - ORG_MB), jr = (float)((j < ORG_MB / 2) ? j : j - ORG_MB);
+        float irm = (float)((im < ORG_MB / 2) ? im : im - ORG_MB), jrm = (float)((jm < ORG_MB / 2) ? jm : jm - ORG_MB);
+        float2 D = make_float2(cdt * (ir * Pk.y + jr * Qk.y), -cdt * (ir * Pk.x + jr * Qk.x));
+        /* at -k the transforms are conj(Pk), conj(Qk) */
+        float2 Dm = make_float2(cdt * (irm * -Pk.y + jrm * -Qk.y), -cdt * (irm * Pk.x + jrm * Qk.x));
+        vx[q] = make_float2(0.5f * (D.x + Dm.x), 0.5f * (D.y - Dm.y));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        sm[i * ORG_LS + j] = vx[q];
+    }
+    __syncthreads();
+    fft2d_128<1>(sm, tw);
+    /* P7: rho^{n+1} on the box (copy_to_rhon, rho_functions.cpp:253-265) */
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+        rbox[e] = rbox[e] + sm[i * ORG_LS + j].x / MMf;
+    }
+}
+
+/* ======================================================================== */
+/* k_cells: update_all_cells (cell_phase_functions.cpp:126-217, mode&1) and
+ * reshaping_cells (cell_phase_functions.cpp:447-580, mode&2) per cell, one
+ * 1024-thread workgroup per cell.  The reshaping loop keeps the relaxed
+ * field on the window plus a one-pixel ring in LDS, the normals (from the
+ * start-of-step phi, useful_functions.cpp:356-388) in a per-cell scratch,
+ * and the iterate in registers; the stop test runs in the workgroup. */
+__global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
+                                                const float *__restrict__ hfunc, const float *__restrict__ psi,
+                                                const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                                const float *__restrict__ f0, const int *__restrict__ xc,
+                                                const int *__restrict__ yc, float *__restrict__ phin_win,
+                                                float *__restrict__ jwin, int *__restrict__ counts,
+                                                int *__restrict__ err, int mode) {
+    __shared__ float phase[ORG_WH_MAX * ORG_WH_MAX];
+    __shared__ double red[16];
+    const int n = blockIdx.x, t = threadIdx.x, N = P.N, W = P.W, WH = W + 2;
+    const size_t NN = (size_t)N * N;
+    const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
+    float *ph = phi + n * NN;
+    const float *rh = rho + n * NN, *hf = hfunc + n * NN;
+    float *pw = phin_win + (size_t)n * W * W;
+    const int npx = W * W;
+    float pn[ORG_CELL_SLOTS];
+
+    if (mode & 1) {
+        const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+        const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
+        const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
+        const float fn = f0[n];
+#pragma unroll
+        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+            const int e = t + 1024 * q;
+            if (e >= npx) { pn[q] = 0.0f; continue; }
+            const int i = e / W, j = e % W;
+            const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+            float v = ph[g];
+            float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+            float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+            float out = v;
+            if (ng > 0.0f) {
+                float d = v - 0.5f;
+                out -= cplc * expf(-d * d * isig2) *
+                       ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) / ng;
+            }
+            /* temp_phase = psi - hfunc_n (cell_phase_functions.cpp:150) on the 3x3 stencil */
+            float tc = psi[g] - hf[g];
+            float tp0 = psi[g + N] - hf[g + N], tm0 = psi[g - N] - hf[g - N];
+            float t0p = psi[g + 1] - hf[g + 1], t0m = psi[g - 1] - hf[g - 1];
+            float tpp = psi[g + N + 1] - hf[g + N + 1], tpm = psi[g + N - 1] - hf[g + N - 1];
+            float tmm = psi[g - N - 1] - hf[g - N - 1], tmp = psi[g - N + 1] - hf[g - N + 1];
+            float lt = -12.0f * tc + 2.0f * (tp0 + tm0 + t0p + t0m) + tpp + tpm + tmm + tmp;
+            float ll = L9AT(hlum, g, (size_t)N), le = L9AT(hecm, g, (size_t)N);
+            out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
+                   (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * hlum[g] + P.eta_lc * ll * inv4
+                    - P.beta_ec * hecm[g] + P.eta_ec * le * inv4);
+            pn[q] = out;
+        }
+        if (!(mode & 2)) {
+#pragma unroll
+            for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+                const int e = t + 1024 * q;
+                if (e < npx) pw[e] = pn[q];
+            }
+            return;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+            const int e = t + 1024 * q;
+            pn[q] = (e < npx) ? pw[e] : 0.0f;
+        }
+    }
+
+    /* normals on the window plus ring, from the start-of-step phi */
+    float *jx = jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
+    {
+        const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
+        for (int e = t; e < WH * WH; e += blockDim.x) {
+            const int i = e / WH - 1, j = e % WH - 1;
+            const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+            float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+            float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+            float a = 0.0f, b = 0.0f;
+            if (ng > 0.0f) { a = h * dxp / ng; b = h * dyp / ng; }
+            jx[e] = a; jy[e] = b;
+            if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = ph[g];
+        }
+    }
+    __syncthreads();
+    const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
+    const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
+    const float dscale = P.dx * P.dx / P.ncur;
+    double rdev = P.rdev0_cell;
+    int ri = 0, cnt = 0;
+    float prev[ORG_CELL_SLOTS];
+    while (rdev > P.thr_cell) {
+#pragma unroll
+        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+            const int e = t + 1024 * q;
+            prev[q] = pn[q];
+            if (e < npx) phase[(e / W + 1) * WH + (e % W + 1)] = pn[q];
+        }
+        __syncthreads();
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+            const int e = t + 1024 * q;
+            if (e < npx) {
+                const int l = (e / W + 1) * WH + (e % W + 1);
+                float nv = reshape_flux(pn[q], phase + l, WH, jx + l, jy + l, WH, ka, kc);
+                acc += (double)(fabsf(nv - pn[q]) * dscale);
+                pn[q] = nv;
+            }
+        }
+        rdev = block_sum_d(acc, red);
+        cnt++;
+        if (ri > P.rimax) { if (t == 0) set_err(err, ORG_ERELAX); break; }
+        ri++;
+        __syncthreads();
+    }
+    /* phi <- second-to-last iterate; phin keeps the last */
+#pragma unroll
+    for (int q = 0; q < ORG_CELL_SLOTS; q++) {
+        const int e = t + 1024 * q;
+        if (e < npx) {
+            ph[(size_t)(x0 + e / W) * N + (y0 + e % W)] = prev[q];
+            pw[e] = pn[q];
+        }
+    }
+    if (t == 0) counts[n] = cnt;
+}
+
+/* k_commit_cells: rho <- rhon (copy_cell_tables_1_by_1,
+ * useful_functions.cpp:679-693, run_with_active_gels.cpp:175-178): the FFT box
+ * values plus the periodic ghost ring applied to rhon
+ * (rho_functions.cpp:469). */
+__global__ __launch_bounds__(256) void k_commit_cells(DevP P, float *__restrict__ rho,
+                                                      const float *__restrict__ rho_box, const int *__restrict__ xc,
+                                                      const int *__restrict__ yc) {
+    const int n = blockIdx.x, N = P.N, M = P.M;
+    const size_t NN = (size_t)N * N;
+    float *rh = rho + n * NN;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
+    const float *rb = rho_box + (size_t)n * M * M;
+    for (int e = threadIdx.x; e < M * M; e += blockDim.x)
+        rh[(size_t)(x0 + e / M) * N + (y0 + e % M)] = rb[e];
+    /* ghost ring: sources are rows/cols 1 and N-2, outside every box */
+    for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
+        rh[(size_t)i * N] = rh[(size_t)i * N + N - 2];
+        rh[(size_t)i * N + N - 1] = rh[(size_t)i * N + 1];
+        rh[i] = rh[(size_t)N * (N - 2) + i];
+        rh[(size_t)N * (N - 1) + i] = rh[(size_t)N + i];
+    }
+    if (threadIdx.x == 0) {
+        rh[0] = rh[(size_t)N * (N - 2) + N - 2];
+        rh[N - 1] = rh[(size_t)N * (N - 2) + 1];
+        rh[(size_t)N * (N - 1)] = rh[(size_t)N + N - 2];
+        rh[(size_t)N * (N - 1) + N - 1] = rh[(size_t)N + 1];
+    }
+}
+
+/* Test hook: one 128 x 128 2D DFT through the same LDS code path. */
+__global__ __launch_bounds__(1024) void k_fft_test(float2 *data, int sign) {
+    __shared__ float2 sm[ORG_MB * ORG_LS];
+    __shared__ float2 tw[128];
+    const int t = threadIdx.x;
+    if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
+    for (int e = t; e < ORG_MB * ORG_MB; e += 1024) sm[(e >> 7) * ORG_LS + (e & 127)] = data[e];
+    __syncthreads();
+    if (sign < 0) fft2d_128<-1>(sm, tw); else fft2d_128<1>(sm, tw);
+    for (int e = t; e < ORG_MB * ORG_MB; e += 1024) data[e] = sm[(e >> 7) * ORG_LS + (e & 127)];
+}

@@ -1,0 +1,276 @@
+"""Python host side of the organoid library: ctypes over the C-ABI
+(include/organoid.h, ``csrc/liborganoid_hip.so``).
+
+``Organoid`` mirrors the reference's per-timestep interface: one method per
+function the driver calls in its time loop (main_code/run_with_active_gels.cpp:
+110-184), same names, same order, same meaning.  Functions whose outputs the
+reference only consumes inside a cell's window (compute_normal_grad_phi,
+compute_prod_norm_delta, create_turnover_rates_tensions, compute_normal_to_phi,
+the boundary_conditions_* calls on those fields) are fused into the kernels
+that consume them; their methods exist and do nothing, so driver-order code
+reads like the reference.  ``step`` runs whole timesteps on the device
+(batched form, optionally replayed from a hipGraph).
+
+Errors: every C-ABI call returns a status; a non-zero status raises
+``OrganoidError`` (the reference has no error channel and would read out of
+range or divide by zero instead, see include/organoid.h).  There is no CPU
+fallback: without the built library, or without a GPU, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import OrgParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "csrc", "liborganoid_hip.so")
+
+F_PHI, F_RHO, F_HFUNC, F_LUM, F_ECM, F_PSI, F_HLUM, F_HECM, F_LUMN, F_ECMN = range(10)
+W_PHIN, W_RHON, W_VX, W_VY = range(4)
+K_NAMES = ("centres", "prep", "psi", "lumecm", "lumreshape", "spectral", "cells", "commit")
+OPT_WRITE_VELOCITY, OPT_GRAPH_LUM_ITERS = 0, 1
+
+_lib = None
+
+
+class OrganoidError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        msg = _lib.org_error_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} (code {code})")
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: run __graft_entry__.build() (or make -C csrc)")
+    lib = ctypes.CDLL(path)
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    lib.org_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    lib.org_destroy.argtypes = [vp]
+    for n in ("org_set_field", "org_get_field", "org_get_window"):
+        getattr(lib, n).argtypes = [vp, i32, vp]
+    lib.org_get_centres.argtypes = [vp, vp, vp]
+    lib.org_get_areas.argtypes = [vp, vp, vp]
+    lib.org_get_iterations.argtypes = [vp, vp, vp, vp]
+    for n in ("org_prepare", "org_find_centres_cells", "org_compute_hfunc_cells", "org_compute_hfunc_lum_ecm",
+              "org_compute_psi", "org_sum_s", "org_update_all_cortices", "org_update_lumen_and_ECM",
+              "org_update_all_cells", "org_reshaping_cells", "org_commit_step", "org_sync"):
+        getattr(lib, n).argtypes = [vp]
+    lib.org_reshaping_lumen_or_ECM.argtypes = [vp, i32]
+    lib.org_step.argtypes = [vp, i32, i32, i32]
+    lib.org_profile_enable.argtypes = [vp, i32]
+    lib.org_kernel_time.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    lib.org_set_option.argtypes = [vp, i32, i32]
+    lib.org_debug_fft2d.argtypes = [vp, i32]
+    lib.org_error_string.restype = ctypes.c_char_p
+    lib.org_error_string.argtypes = [i32]
+    lib.org_build_info.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _chk(rc: int, what: str) -> None:
+    if rc:
+        raise OrganoidError(rc, what)
+
+
+class Organoid:
+    """One organoid resident on the GPU (library-owned device memory)."""
+
+    def __init__(self, params: OrgParams, s0_cell=None):
+        self.lib = load_library()
+        self.params = params.copy()
+        self.nc, self.N = params.ncell, params.N
+        self.W, self.M = params.W, params.M
+        self._s0 = None if s0_cell is None else np.ascontiguousarray(s0_cell, np.float64)
+        h = ctypes.c_void_p()
+        _chk(self.lib.org_create(ctypes.byref(self.params), None if self._s0 is None else _p(self._s0),
+                                 ctypes.byref(h)), "org_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.org_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state in / out -------------------------------------------------
+    def _size(self, field: int) -> tuple:
+        per_cell = field in (F_PHI, F_RHO, F_HFUNC)
+        return (self.nc, self.N, self.N) if per_cell else (self.N, self.N)
+
+    def set_field(self, field: int, arr) -> None:
+        a = np.ascontiguousarray(arr, np.float32)
+        if a.shape != self._size(field):
+            raise ValueError(f"field {field}: shape {a.shape} != {self._size(field)}")
+        _chk(self.lib.org_set_field(self.h, field, _p(a)), "org_set_field")
+
+    def get_field(self, field: int) -> np.ndarray:
+        out = np.empty(self._size(field), np.float32)
+        _chk(self.lib.org_get_field(self.h, field, _p(out)), "org_get_field")
+        return out
+
+    def get_window(self, which: int) -> np.ndarray:
+        side = self.W if which == W_PHIN else self.M
+        out = np.empty((self.nc, side, side), np.float32)
+        _chk(self.lib.org_get_window(self.h, which, _p(out)), "org_get_window")
+        return out
+
+    def load_workload(self, wl) -> None:
+        """Upload phi, rho, lum, ECM and a zero hfunc_cell, then seed the
+        support boxes (the driver's copyin, run_with_active_gels.cpp:65-98)."""
+        self.set_field(F_PHI, wl.phi)
+        self.set_field(F_RHO, wl.rho)
+        self.set_field(F_LUM, wl.lum)
+        self.set_field(F_ECM, wl.ecm)
+        self.set_field(F_HFUNC, np.zeros_like(wl.phi))
+        self.prepare()
+
+    def prepare(self) -> None:
+        _chk(self.lib.org_prepare(self.h), "org_prepare")
+
+    def centres(self):
+        xc = np.zeros(self.nc, np.int32)
+        yc = np.zeros(self.nc, np.int32)
+        _chk(self.lib.org_get_centres(self.h, _p(xc), _p(yc)), "org_get_centres")
+        return xc, yc
+
+    def areas(self):
+        s = np.zeros(self.nc)
+        f0 = np.zeros(self.nc)
+        _chk(self.lib.org_get_areas(self.h, _p(s), _p(f0)), "org_get_areas")
+        return s, f0
+
+    def iterations(self):
+        cells = np.zeros(self.nc, np.int32)
+        lum, ecm = ctypes.c_int32(), ctypes.c_int32()
+        _chk(self.lib.org_get_iterations(self.h, _p(cells), ctypes.byref(lum), ctypes.byref(ecm)),
+             "org_get_iterations")
+        return cells, lum.value, ecm.value
+
+    def set_option(self, option: int, value: int) -> None:
+        _chk(self.lib.org_set_option(self.h, option, int(value)), "org_set_option")
+
+    # ---- the driver's time-loop calls (run_with_active_gels.cpp:123-182) --
+    def find_centres_cells(self):
+        _chk(self.lib.org_find_centres_cells(self.h), "find_centres_cells")
+
+    def create_turnover_rates_tensions(self):
+        """Fused into update_all_cortices (the maps are only read there)."""
+
+    def compute_hfunc_3d_tables_kernels_loop_n(self):
+        _chk(self.lib.org_compute_hfunc_cells(self.h), "compute_hfunc_3d_tables_kernels_loop_n")
+
+    def compute_all_hfunc_2d_tables_kernels(self):
+        _chk(self.lib.org_compute_hfunc_lum_ecm(self.h), "compute_all_hfunc_2d_tables_kernels")
+
+    def compute_normal_grad_phi(self):
+        """Recomputed on the windows by the kernels that read it."""
+
+    def boundary_conditions_cells(self):
+        """Ghost rings of fields that are only read on the windows: no-op."""
+
+    def compute_prod_norm_delta(self):
+        """Fused into update_all_cortices."""
+
+    def compute_psi_kernels(self):
+        _chk(self.lib.org_compute_psi(self.h), "compute_psi_kernels")
+
+    def sum_s_kernels(self):
+        _chk(self.lib.org_sum_s(self.h), "sum_s_kernels")
+        return self.areas()[0]
+
+    def update_all_cortices(self):
+        _chk(self.lib.org_update_all_cortices(self.h), "update_all_cortices")
+
+    def update_lumen_and_ECM(self):
+        _chk(self.lib.org_update_lumen_and_ECM(self.h), "update_lumen_and_ECM")
+
+    def boundary_conditions_lumen_and_ECM(self):
+        """Applied inside update_lumen_and_ECM."""
+
+    def reshaping_lumen_or_ECM(self, which_ecm: int):
+        _chk(self.lib.org_reshaping_lumen_or_ECM(self.h, int(which_ecm)), "reshaping_lumen_or_ECM")
+
+    def compute_normal_to_phi(self):
+        """Recomputed on the window by reshaping_cells."""
+
+    def update_all_cells(self):
+        _chk(self.lib.org_update_all_cells(self.h), "update_all_cells")
+
+    def reshaping_cells(self):
+        _chk(self.lib.org_reshaping_cells(self.h), "reshaping_cells")
+
+    def copy_tables(self):
+        """rho <- rhon, ECM <- ECMn, lum <- lumn (run_with_active_gels.cpp:175-182)."""
+        _chk(self.lib.org_commit_step(self.h), "copy tables")
+
+    def plain_step(self) -> None:
+        """One timestep through the per-function entry points, in the
+        driver's order (run_with_active_gels.cpp:123-182)."""
+        self.find_centres_cells()
+        self.create_turnover_rates_tensions()
+        self.compute_hfunc_3d_tables_kernels_loop_n()
+        self.compute_all_hfunc_2d_tables_kernels()
+        self.compute_normal_grad_phi()
+        self.boundary_conditions_cells()
+        self.compute_prod_norm_delta()
+        self.compute_psi_kernels()
+        self.sum_s_kernels()
+        self.update_all_cortices()
+        self.update_lumen_and_ECM()
+        self.boundary_conditions_lumen_and_ECM()
+        self.reshaping_lumen_or_ECM(0)
+        self.reshaping_lumen_or_ECM(1)
+        self.compute_normal_to_phi()
+        self.update_all_cells()
+        self.reshaping_cells()
+        self.copy_tables()
+
+    # ---- batched form ---------------------------------------------------
+    def step(self, nsteps: int = 1, graph: bool = False, it0: int = 0) -> None:
+        _chk(self.lib.org_step(self.h, it0, nsteps, 1 if graph else 0), "org_step")
+
+    def sync(self) -> None:
+        _chk(self.lib.org_sync(self.h), "org_sync")
+
+    def profile(self, on: bool) -> None:
+        _chk(self.lib.org_profile_enable(self.h, 1 if on else 0), "org_profile_enable")
+
+    def kernel_times(self) -> dict:
+        out = {}
+        for k, name in enumerate(K_NAMES):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            _chk(self.lib.org_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n)), "org_kernel_time")
+            out[name] = (ms.value, n.value)
+        return out
+
+
+def debug_fft2d(a: np.ndarray, sign: int) -> np.ndarray:
+    """128 x 128 unnormalised 2D DFT through the library's LDS transform."""
+    lib = load_library()
+    buf = np.ascontiguousarray(np.stack([a.real, a.imag], -1), np.float32)
+    _chk(lib.org_debug_fft2d(_p(buf), sign), "org_debug_fft2d")
+    return buf[..., 0].astype(np.float64) + 1j * buf[..., 1]

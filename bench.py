@@ -1,0 +1,154 @@
+"""Benchmark: cell-timesteps/s of the organoid timestep on one MI355X.
+
+Workload: ``organoid64_1024`` -- 64 cells (radii U(16,24)) on a jittered hex
+packing, 1024 x 1024 grid, zone_size 65 (FFT box 128), FP32 fields with FP64
+reductions, inputs resident in HBM before the timed region.
+
+A step is one full timestep (main_code/run_with_active_gels.cpp:118-184):
+centroids, turnover labels, h, psi, areas, the spectral cortex update of
+every cell, lumen/ECM update + reshaping, phase-field update + reshaping of
+every cell, copy-back.  The timed form is the batched one (one launch per
+stage for all cells, whole step replayed from a hipGraph).  The plain
+per-function drop-in form (the reference's call sequence with its host
+round trips) is timed beside it on a shorter run.
+
+Printed: one JSON line (see the repository's benchmark contract) with a
+``roofline`` object for the dominant kernel (algorithmic bytes per launch /
+HIP-event launch time on the library's stream) and a ``cpu_baseline``
+object (the FP64 CPU oracle on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-timesteps/s, 64 cells @1024^2: FP32+batch+FNO vs plain, phi L2err<=1e-3"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per cell per launch (FP32 = 4 B), M = FFT box side,
+# W = window side.  k_spectral: read phi_n, phi_n-1, phi_n+1, rho_n, lum, ECM
+# on the box and write rho_n (7 x 4 B per box pixel).  k_cells: read phi_n,
+# rho_n, hfunc_n, psi, h(lum), h(ECM) on the window, write phi_n and phin
+# (8 x 4 B per window pixel).
+def alg_bytes(kernel: str, M: int, W: int) -> int:
+    if kernel == "spectral":
+        return 28 * M * M
+    if kernel == "cells":
+        return 32 * W * W
+    raise KeyError(kernel)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="organoid64_1024")
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), --gpus must be 1"}))
+        return 2
+
+    import numpy as np
+
+    from multicellular_phase_field_active_gel_organoids_amd import host
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+
+    wl = make_workload(args.workload)
+    p = wl.params
+    nc = p.ncell
+    print(f"bench: workload {wl.name} ncell={nc} N={p.N}", file=sys.stderr, flush=True)
+
+    org = host.Organoid(p, wl.s0_cell)
+    org.load_workload(wl)
+    org.set_option(host.OPT_WRITE_VELOCITY, 0)
+
+    # warmup (also builds the graph), then the timed region
+    org.step(max(args.warmup, 1), graph=True)
+    org.sync()
+    t0 = time.perf_counter()
+    org.step(args.steps, graph=True)
+    org.sync()
+    dt = time.perf_counter() - t0
+    value = nc * args.steps / dt
+    ms_per_step = 1e3 * dt / args.steps
+    print(f"bench: batched {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel times with HIP events on the library stream (eager replay of
+    # the same step sequence)
+    nprof = min(200, args.steps)
+    org.profile(True)
+    org.step(nprof, graph=False)
+    org.sync()
+    kt = org.kernel_times()
+    org.profile(False)
+    per_kernel = {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}
+    dominant = max(("spectral", "cells"), key=lambda k: kt[k][0])
+    avg_ms = per_kernel[dominant][0]
+    bytes_launch = alg_bytes(dominant, p.M, p.W) * nc
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": f"k_{dominant}", "avg_launch_ms": round(avg_ms, 5),
+                "alg_bytes_per_launch": bytes_launch}
+    kernel_ms = {k: round(v[0], 5) for k, v in per_kernel.items()}
+    cells_it, lum_it, ecm_it = org.iterations()
+
+    # plain drop-in form: the reference's per-function call sequence
+    plain = host.Organoid(p, wl.s0_cell)
+    plain.load_workload(wl)
+    plain.plain_step()
+    plain.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        plain.plain_step()
+    plain.sync()
+    plain_rate = nc * args.plain_steps / (time.perf_counter() - t0)
+    plain.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(p, wl.s0_cell)
+        o.load_workload(wl)
+        t0 = time.perf_counter()
+        for it in range(args.cpu_steps):
+            o.step(it)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": round(nc * args.cpu_steps / cdt, 3), "unit": "cell-timesteps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} full timesteps of {wl.name} (64 cells, 1024^2) on the FP64 "
+                         f"CPU oracle (full-plane restatement of the reference), 1 thread, {cdt:.1f} s"}
+        o.close()
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "cell-timesteps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "cells": nc, "grid": p.N, "zone_size": p.zone_size,
+                   "form": "batched, hipGraph-replayed step (FNO surrogate not used)",
+                   "plain_dropin_cell_steps_per_s": round(plain_rate, 2),
+                   "speedup_vs_plain": round(value / plain_rate, 3),
+                   "kernel_ms": kernel_ms,
+                   "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    org.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

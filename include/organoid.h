@@ -164,6 +164,10 @@ int org_reshaping_cells(org_ctx *ctx);             /* cell_phase_functions.cpp:4
                                                       useful_functions.cpp:356-388) */
 int org_commit_step(org_ctx *ctx);                 /* useful_functions.cpp:679-709 as
                                                       called at run_with_active_gels.cpp:175-182 */
+int org_commit_rho(org_ctx *ctx);                  /* rho <- rhon for every cell,
+                                                      useful_functions.cpp:679-693 (:175-178) */
+int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
+                                                      useful_functions.cpp:695-709 (:181-182) */
 
 /* ---- batched form -----------------------------------------------------------
  * One full timestep = the loop body run_with_active_gels.cpp:118-184, with

@@ -496,6 +496,8 @@ int org_commit_step(org_ctx *c) {
     TRY(L_commit_lumecm(c));
     return harvest_device_error(c);
 }
+int org_commit_rho(org_ctx *c) { TRY(need_prepared(c)); return L_commit_cells(c); }
+int org_commit_lumecm(org_ctx *c) { TRY(need_prepared(c)); TRY(L_commit_lumecm(c)); return harvest_device_error(c); }
 
 /* ---- batched form ------------------------------------------------------- */
 int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {

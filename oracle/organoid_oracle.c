@@ -230,6 +230,7 @@ void orc_destroy(orc_state *st) {
 }
 
 int orc_real_bytes(void) { return (int)sizeof(real); }
+int orc_params_bytes(void) { return (int)sizeof(org_params); }
 
 static real *field_ptr(const orc_state *st, int f, size_t *n) {
     size_t P = st->P, NN = st->NN, MB = (size_t)st->M * st->M * st->nc;

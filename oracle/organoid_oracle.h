@@ -46,6 +46,7 @@ typedef struct orc_state orc_state;
 orc_state *orc_create(const org_params *p, const double *s0_cell);
 void orc_destroy(orc_state *st);
 int orc_real_bytes(void);                 /* 8 (FP64 build) or 4 (FP32 build) */
+int orc_params_bytes(void);               /* sizeof(org_params) as compiled */
 int64_t orc_field_size(const orc_state *st, int field);
 int orc_set_field(orc_state *st, int field, const double *src);
 int orc_get_field(const orc_state *st, int field, double *dst);

@@ -1,0 +1,93 @@
+"""The drop-in boundary: the C-ABI library exports every entry point its
+header declares; the C++ drop-in exports the reference's functions with
+their C++ signatures and links a program written in the reference driver's
+call order; on the GPU that program reproduces the batched form bit for bit.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from multicellular_phase_field_active_gel_organoids_amd.params import grid_units
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "multicellular_phase_field_active_gel_organoids_amd")
+LIB = os.path.join(PKG, "csrc", "liborganoid_hip.so")
+DROPIN = os.path.join(PKG, "csrc", "dropin", "liborganoid_dropin.so")
+DRIVER = os.path.join(ROOT, "tests", "dropin", "driver_two_cells")
+
+
+def _exports(path, demangle=False):
+    cmd = ["nm", "-D", "--defined-only"] + (["-C"] if demangle else []) + [path]
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True).stdout
+    return out
+
+
+def test_c_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "organoid.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*\*?\s*(org_\w+)\(", hdr, re.M))
+    assert len(declared) >= 25
+    exported = set(re.findall(r" T (org_\w+)$", _exports(LIB), re.M))
+    assert declared <= exported, declared - exported
+
+
+def test_dropin_exports_reference_signatures():
+    out = _exports(DROPIN, demangle=True)
+    for sig in [
+        "update_all_cortices(float*, float*, float*, int*, int*, float*, float*, float*, float*, float*, float*, "
+        "float*, float*, int, int, org_complex32*, org_complex32*, org_complex32*, org_complex32*, org_complex32*, "
+        "org_complex32*, org_complex32*, org_complex32*, org_complex32*, org_complex32*, org_complex32*)",
+        "update_all_cells(int, float*, float*, float*, float*, float*, float*, float*, int*, int*, float*, float*)",
+        "reshaping_cells(int, float*, float*, float*, float*, float*, int*, int*)",
+        "reshaping_lumen_or_ECM(int, float*, float*, float)",
+        "update_lumen_and_ECM(float*, float*, float*, float*, float*, float*, float*)",
+        "create_turnover_rates_tensions(float*, float*, float*, float*, float*, float*, float*, float*, float*, int*, int*)",
+        "compute_prod_norm_delta(float*, float*, float*, int*, int*)",
+        "find_centres_cells(float*, int*, int*)", "sum_s_kernels(float*, float*, float*)",
+        "saving_out_functions(int, float*, float*, float*, float*, float*, float*, float*, float*)",
+        "copy_cell_tables_1_by_1(int, float*, float*)", "copy_tables_ECM_or_lumen(float*, float*)",
+        "malloc_all()", "ini_fft_all()", "cufftPlan2d(int*, int, int, int)",
+    ]:
+        assert sig in out, sig
+    for glob in ("phi", "rhon", "hfunc_cell", "temp_fft", "fft_rho_vy", "planf", "xcenters"):
+        assert re.search(rf" [BD] {glob}$", out, re.M), glob
+
+
+def test_dropin_constants_match_workload_params():
+    p = grid_units(256, 2, 30.0)
+    src = open(os.path.join(ROOT, "tests", "dropin", "constants_two_cells.cpp")).read()
+    vals = dict(re.findall(r"^(?:const )?(?:int|float) (\w+) = ([^;]+);", src, re.M))
+    assert int(vals["N_s"]) == p.N and int(vals["Ntot"]) == p.ncell and int(vals["zone_size"]) == p.zone_size
+    for name, attr in (("dt", "dt"), ("dx", "dx"), ("wi", "wi"), ("rdt", "rdt"), ("s0", "s0"), ("R", "R"),
+                       ("eta", "eta"), ("alpha_ECM", "alpha_ECM"), ("friction", "friction")):
+        assert np.float32(float(vals[name])) == np.float32(getattr(p, attr)), name
+    assert int(vals["rdev_threshold"]) == p.rdev_threshold and int(vals["rdev_ini"]) == p.rdev_ini
+
+
+def test_driver_program_links():
+    assert os.path.exists(DRIVER), "build() builds tests/dropin/driver_two_cells"
+    out = subprocess.run(["ldd", DRIVER], capture_output=True, text=True).stdout
+    assert "liborganoid_dropin.so" in out and "not found" not in out
+
+
+@pytest.mark.gpu
+def test_driver_order_program_matches_batched_step(tmp_path):
+    from multicellular_phase_field_active_gel_organoids_amd import host
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+
+    wl = make_workload("two_cells_256")
+    for name, arr in (("phi", wl.phi), ("rho", wl.rho), ("lum", wl.lum), ("ecm", wl.ecm)):
+        arr.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    print("case drop-in driver two_cells_256, 3 steps", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    g = host.Organoid(wl.params)
+    g.load_workload(wl)
+    g.step(3)
+    nc, N = wl.params.ncell, wl.params.N
+    for name, f, shape in (("phi", host.F_PHI, (nc, N, N)), ("rho", host.F_RHO, (nc, N, N)),
+                           ("lum", host.F_LUM, (N, N)), ("ecm", host.F_ECM, (N, N))):
+        got = np.fromfile(tmp_path / f"out_{name}.f32", np.float32).reshape(shape)
+        assert np.array_equal(got, g.get_field(f)), name

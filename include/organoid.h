@@ -170,11 +170,14 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
                                                       useful_functions.cpp:695-709 (:181-182) */
 
 /* ---- batched form -----------------------------------------------------------
- * One full timestep = the loop body run_with_active_gels.cpp:118-184, with
- * no host round trip inside a step.  use_graph=1 captures one step into a
- * hipGraph on first use and replays it.  Reshaping of lumen/ECM loops on the
- * device in chunks; a step that needs more than the captured chunk falls
- * back to host-driven chunks (exact, slower).                                 */
+ * One full timestep = the loop body run_with_active_gels.cpp:118-184, one
+ * launch per stage for all cells.  use_graph=1 captures the step as two
+ * hipGraphs on first use and replays them: part 1 (centroids ... lumen/ECM
+ * update and the first reshaping sweeps) and part 2 (spectral cortex update,
+ * phase-field update and reshaping of every cell).  The lumen/ECM stop flags
+ * are copied to pinned memory between the two parts; the host reads them
+ * while part 2 runs and queues further sweeps only if a loop has not
+ * stopped.  Cell reshaping loops run entirely on the device.                  */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
 
 /* Device synchronisation and timing.  org_timer_* brackets work on the
@@ -191,9 +194,10 @@ enum {
 };
 
 /* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of
- * the last step for org_get_window.  ORG_OPT_GRAPH_LUM_ITERS (default 6): lumen
- * and ECM reshaping sweeps captured per step in the hipGraph; a step that
- * needs more makes org_step return ORG_ERELAX (use use_graph=0 for such runs). */
+ * the last step for org_get_window.  ORG_OPT_GRAPH_LUM_ITERS (default 4): lumen
+ * and ECM reshaping sweeps queued (and captured) with the first part of each
+ * step; a step that needs more gets further sweeps after the host has read
+ * the stop flags, so results never depend on this value. */
 enum { ORG_OPT_WRITE_VELOCITY = 0, ORG_OPT_GRAPH_LUM_ITERS = 1 };
 int org_set_option(org_ctx *ctx, int option, int value);
 

@@ -45,15 +45,21 @@ struct org_ctx {
     float2 *spec = nullptr;
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
+    int *uni = nullptr, *jbox = nullptr;   /* union of windows; boxes of nonzero lumen/ECM normals */
+    int lum_grid = 256;                    /* blocks per field of a reshaping sweep */
+    int last_sweeps = 0;                   /* sweeps the previous step needed */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
     float *f0 = nullptr;
     unsigned *counters = nullptr;
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
-    int graph_lum_iters = 6;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    int graph_lum_iters = 4;
+    hipGraphExec_t gexec = nullptr, gexec2 = nullptr;
+    hipGraph_t graph = nullptr, graph2 = nullptr;
+    LumState *lst_host = nullptr;     /* pinned: stop flags read behind lum_event */
+    hipEvent_t lum_event = nullptr;
+    long long extra_sweeps = 0;       /* sweeps launched after the captured ones */
     /* profiling */
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
@@ -167,13 +173,14 @@ int harvest_device_error(org_ctx *c) {
 /* ---- stage launchers ------------------------------------------------------ */
 int L_centroid(org_ctx *c) {
     Prof pr(c, ORG_K_CENTRES);
-    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->err);
+    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->uni,
+                       c->err);
     return launch_check();
 }
 int L_prep(org_ctx *c, int mode) {
     Prof pr(c, ORG_K_PREP);
-    hipLaunchKernelGGL(k_prep, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc, c->box,
-                       c->s0, c->s, c->f0, mode);
+    hipLaunchKernelGGL(k_prep, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc, c->box,
+                       c->s0, c->s, c->f0, c->uni, mode);
     return launch_check();
 }
 int L_hlum(org_ctx *c) {
@@ -184,25 +191,25 @@ int L_hlum(org_ctx *c) {
 }
 int L_psi(org_ctx *c) {
     Prof pr(c, ORG_K_PSI);
-    hipLaunchKernelGGL(k_psi, dim3(c->ntiles), dim3(256), 0, c->stream, c->dp, c->hfunc, c->xc, c->yc, c->box, c->psi);
+    hipLaunchKernelGGL(k_psi, dim3(c->ntiles), dim3(256), 0, c->stream, c->dp, c->hfunc, c->box, c->uni, c->psi);
     return launch_check();
 }
 int L_lumecm_update(org_ctx *c) {
     Prof pr(c, ORG_K_LUMECM);
     hipLaunchKernelGGL(k_lumecm_update, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->psi,
-                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufE0, c->lst);
+                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->lst, c->jbox);
     return launch_check();
 }
 int L_lum_normals(org_ctx *c, int fbase, int nf) {
     Prof pr(c, ORG_K_LUMRESHAPE);
     hipLaunchKernelGGL(k_lum_normals, dim3(c->grid_g, nf), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
-                       c->bufE0, c->jlx, c->jly, c->jex, c->jey, fbase);
+                       c->bufE0, c->jlx, c->jly, c->jex, c->jey, c->jbox, fbase);
     return launch_check();
 }
 int L_lum_iter(org_ctx *c, int fbase, int nf) {
     Prof pr(c, ORG_K_LUMRESHAPE);
-    hipLaunchKernelGGL(k_lum_iter, dim3(c->grid_g, nf), dim3(256), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
-                       c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->lst, c->part_l, c->counters, c->err, fbase);
+    hipLaunchKernelGGL(k_lum_iter, dim3(c->lum_grid, nf), dim3(256), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
+                       c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->lst, c->part_l, c->counters, c->err, fbase);
     return launch_check();
 }
 int L_commit_lumecm(org_ctx *c) {
@@ -225,7 +232,7 @@ int L_cells(org_ctx *c, int mode) {
 }
 int L_commit_cells(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
-    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->rho, c->rho_box, c->xc, c->yc);
+    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->xc, c->yc);
     return launch_check();
 }
 
@@ -247,23 +254,64 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
     }
 }
 
-/* One timestep, run_with_active_gels.cpp:118-184.  lum_iters < 0: host-driven
- * lumen/ECM loop; otherwise exactly that many (stop-tested) sweeps. */
-int enqueue_step(org_ctx *c, int lum_iters) {
+/* One timestep, run_with_active_gels.cpp:118-184, in two parts.
+ * Part 1: centroids, h, psi, areas, lumen/ECM update, normals and the first
+ * `lum_iters` stop-tested reshaping sweeps.  Part 2 (independent of the
+ * lumen/ECM loop): the spectral cortex update, the phase-field update and
+ * reshaping of every cell, rho copy-back.  The lumen/ECM loop is finished
+ * by finish_lum after part 2 is queued. */
+int enqueue_part1(org_ctx *c, int lum_iters) {
     TRY(L_centroid(c));             /* :123 */
     TRY(L_prep(c, 3));              /* :133, :142 */
     TRY(L_hlum(c));                 /* :134 */
     TRY(L_psi(c));                  /* :139 */
     TRY(L_lumecm_update(c));        /* :157-158 */
     TRY(L_lum_normals(c, 0, 2));    /* :160-161 */
-    if (lum_iters < 0) TRY(run_lum_loop(c, 0, 2, 2));
-    else
-        for (int k = 0; k < lum_iters; k++) TRY(L_lum_iter(c, 0, 2));
+    for (int k = 0; k < lum_iters; k++) TRY(L_lum_iter(c, 0, 2));
+    return ORG_OK;
+}
+int enqueue_part2(org_ctx *c) {
     TRY(L_spectral(c));             /* :130, :137, :154 */
     TRY(L_cells(c, 3));             /* :164-172 */
     TRY(L_commit_cells(c));         /* :175-178 */
+    return ORG_OK;
+}
+/* Snapshot the stop flags into pinned memory behind an event (queued after
+ * part 1, before part 2). */
+int mark_lum(org_ctx *c) {
+    HIPCHK(hipMemcpyAsync(c->lst_host, c->lst, 2 * sizeof(LumState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->lum_event, c->stream));
+    return ORG_OK;
+}
+/* Wait for the snapshot (part 2 keeps the device busy meanwhile), run more
+ * sweeps until both stop tests passed (reference: host while loop,
+ * cell_phase_functions.cpp:774-869), then lum <- lumn, ECM <- ECMn. */
+int finish_lum(org_ctx *c, int queued) {
+    HIPCHK(hipEventSynchronize(c->lum_event));
+    long long guard = 0;
+    int sweeps = queued;
+    /* first extra round: as many sweeps as the previous step needed beyond
+     * the queued ones (surplus sweeps stop at the device-side test) */
+    int round = std::max(2, c->last_sweeps - queued + 1);
+    while (!(c->lst_host[0].done && c->lst_host[1].done)) {
+        for (int k = 0; k < round; k++) TRY(L_lum_iter(c, 0, 2));
+        c->extra_sweeps += round;
+        sweeps += round;
+        TRY(mark_lum(c));
+        HIPCHK(hipEventSynchronize(c->lum_event));
+        if ((guard += round) > (long long)c->p.rimax + 4) return note_error(ORG_ERELAX);
+        round = 2;
+    }
+    c->last_sweeps = std::max(c->lst_host[0].ri, c->lst_host[1].ri);
+    (void)sweeps;
     TRY(L_commit_lumecm(c));        /* :181-182 */
     return ORG_OK;
+}
+int enqueue_step(org_ctx *c, int lum_iters) {
+    TRY(enqueue_part1(c, lum_iters));
+    TRY(mark_lum(c));
+    TRY(enqueue_part2(c));
+    return finish_lum(c, lum_iters);
 }
 
 }  // namespace
@@ -306,7 +354,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     c->dp = make_devp(*p);
     c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
     c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
-    c->grid_g = (int)std::min<size_t>((c->NN + 1023) / 1024, 1024);
+    c->grid_g = std::min(c->N, 1024);
     int nt = (c->N + 31) / 32;
     c->ntiles = nt * nt;
     int rc = ORG_OK;
@@ -320,11 +368,15 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vxbox, (size_t)c->nc * c->M * c->M);
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
-    A(&c->spec, (size_t)c->nc * c->M * c->M);
+    A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
+    A(&c->uni, 4); A(&c->jbox, 8);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
+    if (!rc && hipHostMalloc((void **)&c->lst_host, 2 * sizeof(LumState), hipHostMallocDefault) != hipSuccess)
+        rc = note_error(ORG_EHIP);
+    if (!rc && hipEventCreateWithFlags(&c->lum_event, hipEventDisableTiming) != hipSuccess) rc = note_error(ORG_EHIP);
     if (rc) { org_destroy(c); return rc; }
     std::vector<double> s0v(c->nc);
     for (int n = 0; n < c->nc; n++) s0v[n] = (p->use_s0_cell && s0_cell) ? s0_cell[n] : p->s0;
@@ -347,6 +399,10 @@ int org_destroy(org_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gexec2) (void)hipGraphExecDestroy(c->gexec2);
+    if (c->graph2) (void)hipGraphDestroy(c->graph2);
+    if (c->lum_event) (void)hipEventDestroy(c->lum_event);
+    if (c->lst_host) (void)hipHostFree(c->lst_host);
     for (int k = 0; k < ORG_K_COUNT; k++)
         for (auto &pr : c->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (void *q : c->allocs) (void)hipFree(q);
@@ -470,6 +526,8 @@ int org_set_option(org_ctx *c, int option, int value) {
     }
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->gexec2) { (void)hipGraphExecDestroy(c->gexec2); c->gexec2 = nullptr; }
+    if (c->graph2) { (void)hipGraphDestroy(c->graph2); c->graph2 = nullptr; }
     return ORG_OK;
 }
 
@@ -505,23 +563,33 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
     if (!use_graph) {
-        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c, -1));
+        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c, c->graph_lum_iters));
         return harvest_device_error(c);
     }
     if (!c->gexec) {
         bool prof = c->prof;
-        c->prof = false;                 /* no event records inside the graph */
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(c, c->graph_lum_iters);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->prof = false;                 /* no event records inside the graphs */
+        for (int part = 1; part <= 2; part++) {
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = part == 1 ? enqueue_part1(c, c->graph_lum_iters) : enqueue_part2(c);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc || e != hipSuccess) {
+                c->prof = prof;
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : note_error(ORG_EHIP);
+            }
+            (part == 1 ? c->graph : c->graph2) = g;
+            HIPCHK(hipGraphInstantiate(part == 1 ? &c->gexec : &c->gexec2, g, nullptr, nullptr, 0));
+        }
         c->prof = prof;
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        if (e != hipSuccess) return note_error(ORG_EHIP);
-        c->graph = g;
-        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     }
-    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    for (int s = 0; s < nsteps; s++) {
+        HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        TRY(mark_lum(c));
+        HIPCHK(hipGraphLaunch(c->gexec2, c->stream));
+        TRY(finish_lum(c, c->graph_lum_iters));
+    }
     return harvest_device_error(c);
 }
 

@@ -57,7 +57,8 @@
 #define ORG_LS (ORG_MB + 1)        /* LDS row stride (float2) */
 #define ORG_ZS_MAX 65              /* zone_size supported by k_cells */
 #define ORG_WH_MAX (2 * ORG_ZS_MAX + 2)
-#define ORG_CELL_SLOTS 17          /* ceil((2*65)^2 / 1024) */
+#define ORG_W (2 * ORG_ZS_MAX)     /* window side */
+#define ORG_WH (ORG_W + 2)         /* window plus one-pixel ring */
 
 __constant__ float kCos128[128] = {
     1.000000000e+00f, 9.987954562e-01f, 9.951847267e-01f, 9.891765100e-01f,
@@ -296,14 +297,15 @@ __global__ void k_init_box(DevP P, const float *__restrict__ phi, const float *_
     const int n = blockIdx.x, N = P.N;
     const size_t NN = (size_t)N * N;
     int i0 = N, i1 = -1, j0 = N, j1 = -1;
-    for (size_t e = threadIdx.x; e < NN; e += blockDim.x) {
-        int i = (int)(e / N), j = (int)(e % N);
-        if (phi[n * NN + e] > 0.5f || hfunc[n * NN + e] != 0.0f) {
-            i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
-        }
-    }
     if (threadIdx.x == 0) { sred[0] = N; sred[1] = -1; sred[2] = N; sred[3] = -1; }
     __syncthreads();
+    for (int i = threadIdx.x >> 6; i < N; i += blockDim.x >> 6) {
+        const float *pr = phi + n * NN + (size_t)i * N, *hr = hfunc + n * NN + (size_t)i * N;
+        for (int j = threadIdx.x & 63; j < N; j += 64)
+            if (pr[j] > 0.5f || hr[j] != 0.0f) {
+                i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
+            }
+    }
     atomicMin(&sred[0], i0); atomicMax(&sred[1], i1); atomicMin(&sred[2], j0); atomicMax(&sred[3], j1);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -313,28 +315,30 @@ __global__ void k_init_box(DevP P, const float *__restrict__ phi, const float *_
 }
 
 /* k_centroid: find_centres_cells (useful_functions.cpp:193-227) over the
- * support box, window check, support box += window. */
-__global__ __launch_bounds__(256) void k_centroid(DevP P, const float *__restrict__ phi, int *__restrict__ xc,
-                                                  int *__restrict__ yc, int *__restrict__ box,
-                                                  int *__restrict__ err) {
-    __shared__ long long red[3][4];
+ * support box, window check, support box += window.  Block 0 also resets the
+ * union-of-windows box that k_prep fills and k_psi reads. */
+__global__ __launch_bounds__(1024) void k_centroid(DevP P, const float *__restrict__ phi, int *__restrict__ xc,
+                                                   int *__restrict__ yc, int *__restrict__ box,
+                                                   int *__restrict__ uni, int *__restrict__ err) {
+    __shared__ long long red[3][16];
     const int n = blockIdx.x, N = P.N;
     const size_t NN = (size_t)N * N;
     const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
-    const int bw = max(j1 - j0, 0), bh = max(i1 - i0, 0);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (n == 0 && threadIdx.x == 0) { uni[0] = N; uni[1] = 0; uni[2] = N; uni[3] = 0; }
     long long sx = 0, sy = 0, c = 0;
     const float *ph = phi + n * NN;
-    for (int e = threadIdx.x; e < bw * bh; e += blockDim.x) {
-        int i = i0 + e / bw, j = j0 + e % bw;
-        if (ph[(size_t)i * N + j] > 0.5f) { sx += i; sy += j; c += 1; }
+    for (int i = i0 + w; i < i1; i += nw) {
+        const float *row = ph + (size_t)i * N;
+        for (int j = j0 + lane; j < j1; j += 64)
+            if (row[j] > 0.5f) { sx += i; sy += j; c += 1; }
     }
     sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) { red[0][w] = sx; red[1][w] = sy; red[2][w] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
         long long X = 0, Y = 0, C = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
+        for (int k = 0; k < nw; k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
         int x = xc[n], y = yc[n];
         if (C == 0) {
             set_err(err, ORG_EEMPTY_CELL);
@@ -350,44 +354,54 @@ __global__ __launch_bounds__(256) void k_centroid(DevP P, const float *__restric
             y = min(max(y, lo), hi);
         }
         xc[n] = x; yc[n] = y;
-        box[4 * n + 0] = min(i0, x - P.zs); box[4 * n + 1] = max(i1, x + P.zs);
-        box[4 * n + 2] = min(j0, y - P.zs); box[4 * n + 3] = max(j1, y + P.zs);
-        if (bw == 0 || bh == 0) {
+        if (i1 <= i0 || j1 <= j0) {
             box[4 * n + 0] = x - P.zs; box[4 * n + 1] = x + P.zs;
             box[4 * n + 2] = y - P.zs; box[4 * n + 3] = y + P.zs;
+        } else {
+            box[4 * n + 0] = min(i0, x - P.zs); box[4 * n + 1] = max(i1, x + P.zs);
+            box[4 * n + 2] = min(j0, y - P.zs); box[4 * n + 3] = max(j1, y + P.zs);
         }
     }
 }
 
 /* k_prep: mode&1 -> hfunc on the window (useful_functions.cpp:233-251);
  * mode&2 -> s_n = sum over the support box of hfunc*dx*dx in FP64 and
- * f0_n = alpha*(s0_n - s_n) (useful_functions.cpp:168-190). */
-__global__ __launch_bounds__(256) void k_prep(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
-                                              const int *__restrict__ xc, const int *__restrict__ yc,
-                                              const int *__restrict__ box, const double *__restrict__ s0,
-                                              double *__restrict__ s, float *__restrict__ f0, int mode) {
+ * f0_n = alpha*(s0_n - s_n) (useful_functions.cpp:168-190).  Both modes add
+ * the cell's window to the union box read by k_psi. */
+__global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
+                                               const int *__restrict__ xc, const int *__restrict__ yc,
+                                               const int *__restrict__ box, const double *__restrict__ s0,
+                                               double *__restrict__ s, float *__restrict__ f0,
+                                               int *__restrict__ uni, int mode) {
     __shared__ double red[16];
-    const int n = blockIdx.x, N = P.N, W = P.W;
+    const int n = blockIdx.x, N = P.N;
+    constexpr int W = ORG_W;
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     const float *ph = phi + n * NN;
     float *hf = hfunc + n * NN;
+    if (threadIdx.x == 0) {
+        atomicMin(&uni[0], x0); atomicMax(&uni[1], x0 + W);
+        atomicMin(&uni[2], y0); atomicMax(&uni[3], y0 + W);
+    }
     if (mode & 1) {
-        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
-            size_t g = (size_t)(x0 + e / W) * N + (y0 + e % W);
-            hf[g] = hfun(ph[g]);
+        for (int i = w; i < W; i += nw) {
+            const size_t g = (size_t)(x0 + i) * N + y0;
+            for (int j = lane; j < W; j += 64) hf[g + j] = hfun(ph[g + j]);
         }
     }
     if (mode & 2) {
         const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
-        const int bw = j1 - j0, bh = i1 - i0;
         double acc = 0.0;
-        for (int e = threadIdx.x; e < bw * bh; e += blockDim.x) {
-            int i = i0 + e / bw, j = j0 + e % bw;
-            size_t g = (size_t)i * N + j;
-            bool inw = (mode & 1) && i >= x0 && i < x0 + W && j >= y0 && j < y0 + W;
-            float h = inw ? hfun(ph[g]) : hf[g];   /* same value the window pass stored */
-            acc += (double)(h * P.dx * P.dx);
+        for (int i = i0 + w; i < i1; i += nw) {
+            const bool rowin = (mode & 1) && i >= x0 && i < x0 + W;
+            const float *pr = ph + (size_t)i * N, *hr = hf + (size_t)i * N;
+            for (int j = j0 + lane; j < j1; j += 64) {
+                /* in the window use the value this launch stored there */
+                float h = (rowin && j >= y0 && j < y0 + W) ? hfun(pr[j]) : hr[j];
+                acc += (double)(h * P.dx * P.dx);
+            }
         }
         double tot = block_sum_d(acc, red);
         if (threadIdx.x == 0) {
@@ -406,14 +420,17 @@ __global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ 
                                               double *__restrict__ sum_out) {
     __shared__ double red[16];
     __shared__ bool last;
-    const size_t NN = (size_t)P.N * P.N;
+    const int N = P.N;
     double acc = 0.0;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
-        float a = lum[e], b = ecm[e];
-        float hb = hfun(b);
-        hlum[e] = hfun(a);
-        hecm[e] = hb;
-        acc += (double)(hb * P.dx * P.dx);
+    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+        const size_t g = (size_t)i * N;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            float a = lum[g + j], b = ecm[g + j];
+            float hb = hfun(b);
+            hlum[g + j] = hfun(a);
+            hecm[g + j] = hb;
+            acc += (double)(hb * P.dx * P.dx);
+        }
     }
     double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
@@ -426,8 +443,8 @@ __global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ 
     if (last) {
         __threadfence();
         double v = 0.0;
-        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        /* fixed-order combine: per-thread strided partials then block sum */
+        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+            v += __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double t2 = block_sum_d(v, red);
         if (threadIdx.x == 0) { *sum_out = t2; *counter = 0u; }
     }
@@ -435,24 +452,13 @@ __global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ 
 
 /* k_psi: compute_psi_kernels (useful_functions.cpp:305-322) on 32 x 32 tiles
  * of the union of this step's windows. */
-__global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ xc,
-                                             const int *__restrict__ yc, const int *__restrict__ box,
-                                             float *__restrict__ psi) {
-    __shared__ int ub[4];
+__global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ box,
+                                             const int *__restrict__ uni, float *__restrict__ psi) {
     __shared__ int list[1024];
     __shared__ int cnt;
     const int N = P.N, ntx = (N + 31) / 32;
     const int ti0 = (blockIdx.x / ntx) * 32, tj0 = (blockIdx.x % ntx) * 32;
-    if (threadIdx.x == 0) { ub[0] = N; ub[1] = 0; ub[2] = N; ub[3] = 0; }
-    __syncthreads();
-    int a0 = N, a1 = 0, b0 = N, b1 = 0;
-    for (int n = threadIdx.x; n < P.nc; n += blockDim.x) {
-        a0 = min(a0, xc[n] - P.zs); a1 = max(a1, xc[n] + P.zs);
-        b0 = min(b0, yc[n] - P.zs); b1 = max(b1, yc[n] + P.zs);
-    }
-    atomicMin(&ub[0], a0); atomicMax(&ub[1], a1); atomicMin(&ub[2], b0); atomicMax(&ub[3], b1);
-    __syncthreads();
-    const int u0 = ub[0], u1 = ub[1], v0 = ub[2], v1 = ub[3];
+    const int u0 = uni[0], u1 = uni[1], v0 = uni[2], v1 = uni[3];
     if (ti0 >= u1 || ti0 + 32 <= u0 || tj0 >= v1 || tj0 + 32 <= v0) return;
     /* ordered list of cells whose support box meets the tile (wave 0) */
     if (threadIdx.x < 64) {
@@ -493,41 +499,53 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
 #define L9AT(t, g, N) (-12.0f * (t)[g] + 2.0f * ((t)[(g) + (N)] + (t)[(g) - (N)] + (t)[(g) + 1] + (t)[(g) - 1]) \
                        + (t)[(g) + (N) + 1] + (t)[(g) + (N) - 1] + (t)[(g) - (N) - 1] + (t)[(g) - (N) + 1])
 
-/* lumen / ECM reshaping state: [0] done, [1] ri, [2] current buffer */
+/* lumen / ECM reshaping loop state: stop flag, sweeps done, buffer holding
+ * the last iterate */
 struct LumState { int done, ri, cur, pad; };
 
 /* k_lumecm_update: update_lumen_and_ECM (cell_phase_functions.cpp:395-443),
  * then boundary_conditions_lumen_and_ECM (useful_functions.cpp:439-462).
- * Writes the first iterate of both reshaping loops and resets their state. */
+ * Writes the first iterate of both reshaping loops into both ping-pong
+ * buffers (sweeps rewrite only their active box), resets the loop states and
+ * the normals' boxes. */
 __global__ __launch_bounds__(256) void k_lumecm_update(DevP P, const float *__restrict__ lum,
                                                        const float *__restrict__ ecm, const float *__restrict__ psi,
                                                        const float *__restrict__ hlum, const float *__restrict__ hecm,
-                                                       const double *__restrict__ sum_ecm, float *__restrict__ lumn,
-                                                       float *__restrict__ ecmn, LumState *__restrict__ lst) {
+                                                       const double *__restrict__ sum_ecm, float *__restrict__ lumA,
+                                                       float *__restrict__ lumB, float *__restrict__ ecmA,
+                                                       float *__restrict__ ecmB, LumState *__restrict__ lst,
+                                                       int *__restrict__ jbox) {
     const int N = P.N;
-    const size_t NN = (size_t)N * N;
     if (blockIdx.x == 0 && threadIdx.x < 2) {
         bool run = P.rdev0_lum > P.thr_lum;
         lst[threadIdx.x].done = run ? 0 : 1;
         lst[threadIdx.x].ri = 0;
         lst[threadIdx.x].cur = 0;
+        int *b = jbox + 4 * threadIdx.x;
+        b[0] = N; b[1] = -1; b[2] = N; b[3] = -1;
     }
     const float se = (float)(*sum_ecm);
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(e / N), j = (int)(e % N);
-        if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
-            lumn[e] = 0.0f;      /* lumen edges and corners 0 */
-            ecmn[e] = 1.0f;      /* ECM edges and corners 1 (:449-460) */
-            continue;
+    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+        const size_t r = (size_t)i * N;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const size_t e = r + j;
+            float ln, en;
+            if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
+                ln = 0.0f;       /* lumen edges and corners 0 */
+                en = 1.0f;       /* ECM edges and corners 1 (:449-460) */
+            } else {
+                float l = lum[e], E = ecm[e], ps = psi[e];
+                float lap = L9AT(psi, e, (size_t)N) * inv4;
+                ln = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
+                             (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
+                en = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
+                             (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
+            }
+            lumA[e] = ln; lumB[e] = ln;
+            ecmA[e] = en; ecmB[e] = en;
         }
-        float l = lum[e], E = ecm[e], ps = psi[e];
-        float lap = L9AT(psi, e, (size_t)N) * inv4;
-        lumn[e] = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
-                          (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
-        ecmn[e] = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
-                          (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
     }
 }
 
@@ -536,30 +554,46 @@ __global__ __launch_bounds__(256) void k_lumecm_update(DevP P, const float *__re
 static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N - 2 : (i == N - 1 ? 1 : i); }
 
 /* k_lum_normals: |grad| of the old field (cell_phase_functions.cpp:722-741),
- * normals from the new field's gradient (:744-771) with the periodic ring.
- * blockIdx.y: 0 lumen, 1 ECM. */
+ * normals from the new field's gradient (:744-771) with the periodic ring,
+ * and the bounding box of the nonzero normals (jbox, inclusive).  A sweep
+ * changes a pixel only if the normal at it or at one of its four neighbours
+ * is nonzero (every flux term carries a face average of the normals).
+ * blockIdx.y + fbase: 0 lumen, 1 ECM. */
 __global__ __launch_bounds__(256) void k_lum_normals(DevP P, const float *__restrict__ lum_old,
                                                      const float *__restrict__ ecm_old, const float *__restrict__ lumn,
                                                      const float *__restrict__ ecmn, float *__restrict__ jlx,
                                                      float *__restrict__ jly, float *__restrict__ jex,
-                                                     float *__restrict__ jey, int fbase) {
+                                                     float *__restrict__ jey, int *__restrict__ jbox, int fbase) {
+    __shared__ int sb[4];
     const int N = P.N, f = fbase + blockIdx.y;
-    const size_t NN = (size_t)N * N;
     const float *old = f ? ecm_old : lum_old;
     const float *nw = f ? ecmn : lumn;
     float *jx = f ? jex : jlx, *jy = f ? jey : jly;
     const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
-        int i = ring_src((int)(e / N), N), j = ring_src((int)(e % N), N);
-        size_t g = (size_t)i * N + j;
-        float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
-        float ng = sqrtf(c * (a * a + b * b));
-        float vx = 0.0f, vy = 0.0f;
-        if (ng != 0.0f) {
-            vx = h * (nw[g + N] - nw[g - N]) / ng;
-            vy = h * (nw[g + 1] - nw[g - 1]) / ng;
+    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+    __syncthreads();
+    int b0 = N, b1 = -1, b2 = N, b3 = -1;
+    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+        const int si = ring_src(i, N);
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const size_t g = (size_t)si * N + ring_src(j, N);
+            float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+            float ng = sqrtf(c * (a * a + b * b));
+            float vx = 0.0f, vy = 0.0f;
+            if (ng != 0.0f) {
+                vx = h * (nw[g + N] - nw[g - N]) / ng;
+                vy = h * (nw[g + 1] - nw[g - 1]) / ng;
+            }
+            const size_t e = (size_t)i * N + j;
+            jx[e] = vx; jy[e] = vy;
+            if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
         }
-        jx[e] = vx; jy[e] = vy;
+    }
+    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+    __syncthreads();
+    if (threadIdx.x == 0 && sb[1] >= 0) {
+        int *gb = jbox + 4 * f;
+        atomicMin(&gb[0], sb[0]); atomicMax(&gb[1], sb[1]); atomicMin(&gb[2], sb[2]); atomicMax(&gb[3], sb[3]);
     }
 }
 
@@ -596,22 +630,24 @@ static __device__ __forceinline__ float reshape_flux(float un, const float *u, i
 }
 
 /* k_lum_iter: one sweep of reshaping_lumen_or_ECM's loop
- * (cell_phase_functions.cpp:774-869) for the lumen (blockIdx.y = 0) and the
- * ECM (1): X_{k+1} = X_k + flux(X_k) inside, Dirichlet edges (value side)
- * and zero corners (useful_functions.cpp:422-437), deviation
+ * (cell_phase_functions.cpp:774-869) for the lumen (field 0) and the ECM
+ * (1): X_{k+1} = X_k + flux(X_k) inside, Dirichlet edges (value side) and
+ * zero corners (useful_functions.cpp:422-437), deviation
  * sum |X_{k+1}-X_k| dx^2 in FP64; the last block to finish applies the stop
- * test.  A field whose loop has stopped skips the sweep. */
+ * test.  Interior pixels are swept only inside the normals' box grown by one
+ * pixel (outside it X_{k+1} = X_k exactly); the boundary ring every sweep.
+ * A field whose loop has stopped skips the sweep. */
 __global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bufL0, float *__restrict__ bufL1,
                                                   float *__restrict__ bufE0, float *__restrict__ bufE1,
                                                   const float *__restrict__ jlx, const float *__restrict__ jly,
                                                   const float *__restrict__ jex, const float *__restrict__ jey,
-                                                  LumState *__restrict__ lst, double *__restrict__ part,
-                                                  unsigned *__restrict__ counter, int *__restrict__ err, int fbase) {
+                                                  const int *__restrict__ jbox, LumState *__restrict__ lst,
+                                                  double *__restrict__ part, unsigned *__restrict__ counter,
+                                                  int *__restrict__ err, int fbase) {
     __shared__ double red[16];
     __shared__ bool last;
     const int f = fbase + blockIdx.y, N = P.N;
-    const size_t NN = (size_t)N * N;
-    LumState s = lst[f];
+    const LumState s = lst[f];
     if (s.done) return;
     float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
     const float *src = (s.ri & 1) ? b1 : b0;
@@ -620,16 +656,33 @@ __global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bu
     const float side = f ? 1.0f : 0.0f;
     const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
     const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
+    const float dx2 = P.dx * P.dx;
+    const int *jb = jbox + 4 * f;
+    const int r0 = max(jb[0] - 1, 1), r1 = min(jb[1] + 1, N - 2);
+    const int c0 = max(jb[2] - 1, 1), c1 = min(jb[3] + 1, N - 2);
     double acc = 0.0;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(e / N), j = (int)(e % N);
-        float v;
-        bool ei = (i == 0 || i == N - 1), ej = (j == 0 || j == N - 1);
-        if (ei && ej) v = 0.0f;
-        else if (ei || ej) v = side;
-        else v = reshape_flux(src[e], src + e, N, jx + e, jy + e, N, ka, kc);
+    for (int i = r0 + blockIdx.x; i <= r1; i += gridDim.x) {
+        const size_t r = (size_t)i * N;
+        for (int j = c0 + threadIdx.x; j <= c1; j += blockDim.x) {
+            const size_t e = r + j;
+            float u = src[e];
+            float v = reshape_flux(u, src + e, N, jx + e, jy + e, N, ka, kc);
+            dst[e] = v;
+            acc += (double)(fabsf(v - u) * dx2);
+        }
+    }
+    /* boundary ring: 4N-4 pixels, edges -> side, corners -> 0 */
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 4 * N - 4; k += gridDim.x * blockDim.x) {
+        int i, j;
+        if (k < N) { i = 0; j = k; }
+        else if (k < 2 * N) { i = N - 1; j = k - N; }
+        else if (k < 3 * N - 2) { i = k - 2 * N + 1; j = 0; }
+        else { i = k - (3 * N - 2) + 1; j = N - 1; }
+        const size_t e = (size_t)i * N + j;
+        const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
+        float v = corner ? 0.0f : side;
         dst[e] = v;
-        acc += (double)(fabsf(v - src[e]) * P.dx * P.dx);
+        acc += (double)(fabsf(v - src[e]) * dx2);
     }
     double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
@@ -663,21 +716,26 @@ __global__ __launch_bounds__(256) void k_commit_lumecm(DevP P, float *__restrict
                                                        const float *__restrict__ bufL0, const float *__restrict__ bufL1,
                                                        const float *__restrict__ bufE0, const float *__restrict__ bufE1,
                                                        const LumState *__restrict__ lst, int *__restrict__ err) {
-    const size_t NN = (size_t)P.N * P.N;
-    LumState sl = lst[0], se = lst[1];
+    const int N = P.N;
+    const LumState sl = lst[0], se = lst[1];
     if (blockIdx.x == 0 && threadIdx.x == 0 && (!sl.done || !se.done)) set_err(err, ORG_ERELAX);
     const float *L = sl.cur ? bufL1 : bufL0;
     const float *E = se.cur ? bufE1 : bufE0;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < NN; e += (size_t)gridDim.x * blockDim.x) {
-        lum[e] = L[e];
-        ecm[e] = E[e];
+    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+        const size_t r = (size_t)i * N;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            lum[r + j] = L[r + j];
+            ecm[r + j] = E[r + j];
+        }
     }
 }
 
 /* ======================================================================== */
 /* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
  *
- * One workgroup (1024 threads) per cell, one 128 x 128 complex field in LDS.
+ * One workgroup (1024 threads) per cell, one 128 x 128 complex field in LDS,
+ * two 128 x 128 complex scratch planes per cell in global memory (L2
+ * resident) for the spectra that are needed at k and -k.
  * By linearity of the reference's spectral update, Re(F^-1[F(rho) +
  * dt F(react) + D]) / M^2 = rho + dt*react + Re(F^-1 D) / M^2, so the
  * transforms of rho and of the reaction term (rho_functions.cpp:415-422,
@@ -696,13 +754,13 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
     __shared__ float2 sm[ORG_MB * ORG_LS];
     __shared__ float2 tw[128];
     const int n = blockIdx.x, t = threadIdx.x, N = P.N;
-    const int MM = ORG_MB * ORG_MB;
+    constexpr int MM = ORG_MB * ORG_MB;
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;   /* FFT box origin, useful_functions.cpp:618-619 */
     const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
     if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
     const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
-    float2 *SA = scratch + (size_t)n * MM;
+    float2 *SA = scratch + (size_t)n * 2 * MM, *SB = SA + MM;
     float *rbox = rho_box + (size_t)n * MM;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
@@ -710,10 +768,9 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
     const float kscale = 6.28318530718f / (ORG_MB * P.dx);
     const float MMf = (float)MM;
 
-    float rr[16], pyy[16];
     unsigned lab = 0u;
     /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const size_t g = (size_t)(x0 + i) * N + (y0 + j);
@@ -737,28 +794,21 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
         float pnd = ng * expf(-d * d * isig2) * dnorm;
         float react = -P.deg[idx] * r + P.gen[idx] * pnd;
         rbox[e] = r + P.dt * react;
-        rr[q] = r;
-        pyy[q] = pyv;
+        SB[e] = make_float2(pyv, 0.0f);
         sm[i * ORG_LS + j] = make_float2(pxx, pxy);
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         SA[e] = sm[i * ORG_LS + j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        sm[i * ORG_LS + j] = make_float2(pyy[q], 0.0f);
+        sm[i * ORG_LS + j] = SB[e];      /* same thread wrote SB[e] in P1 */
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-    /* P4: split, per-mode solve (rho_functions.cpp:133-190) */
-    float2 vx[16], vy[16];
-#pragma unroll
+    /* P4: split, per-mode solve (rho_functions.cpp:133-190); v_x -> LDS, v_y -> SB */
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
@@ -767,49 +817,48 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
         float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
         int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
         float eta = P.visc[(lab >> (2 * q)) & 3u];
-        velocity_mode(eta, P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vx[q], vy[q]);
+        float2 vx, vy;
+        velocity_mode(eta, P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vx, vy);
+        sm[i * ORG_LS + j] = vx;
+        SB[e] = vy;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        sm[i * ORG_LS + j] = vx[q];
-        SA[e] = vy[q];
-    }
-    __syncthreads();
-#pragma unroll
+    /* Z = H v_x + i H v_y -> SA */
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
-        float2 xm = sm[im * ORG_LS + jm], ym = SA[im * ORG_MB + jm];
-        float2 hx = make_float2(0.5f * (vx[q].x + xm.x), 0.5f * (vx[q].y - xm.y));
-        float2 hy = make_float2(0.5f * (vy[q].x + ym.x), 0.5f * (vy[q].y - ym.y));
-        vx[q] = make_float2(hx.x - hy.y, hx.y + hy.x);
+        float2 x = sm[i * ORG_LS + j], xm = sm[im * ORG_LS + jm];
+        float2 y = SB[e], ym = SB[im * ORG_MB + jm];
+        float2 hx = make_float2(0.5f * (x.x + xm.x), 0.5f * (x.y - xm.y));
+        float2 hy = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y - ym.y));
+        SA[e] = make_float2(hx.x - hy.y, hx.y + hy.x);
     }
     __syncthreads();
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        sm[i * ORG_LS + j] = vx[q];
+        const int e = t + 1024 * q;
+        sm[(e >> 7) * ORG_LS + (e & 127)] = SA[e];
     }
     __syncthreads();
     fft2d_128<1>(sm, tw);
     /* P5: v = Re(.)/M^2 ; rho * v (rho_functions.cpp:267-284) */
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         float2 z = sm[i * ORG_LS + j];
+        float r = rh[(size_t)(x0 + i) * N + (y0 + j)];
         if (write_v) {
             vxbox[(size_t)n * MM + e] = z.x / MMf;
             vybox[(size_t)n * MM + e] = z.y / MMf;
         }
-        sm[i * ORG_LS + j] = make_float2(rr[q] * z.x / MMf, rr[q] * z.y / MMf);
+        sm[i * ORG_LS + j] = make_float2(r * z.x / MMf, r * z.y / MMf);
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-    /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249) */
+    /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249) -> SA */
     const float cdt = P.dt * kscale;
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
@@ -821,31 +870,31 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
         float2 D = make_float2(cdt * (ir * Pk.y + jr * Qk.y), -cdt * (ir * Pk.x + jr * Qk.x));
         /* at -k the transforms are conj(Pk), conj(Qk) */
         float2 Dm = make_float2(cdt * (irm * -Pk.y + jrm * -Qk.y), -cdt * (irm * Pk.x + jrm * Qk.x));
-        vx[q] = make_float2(0.5f * (D.x + Dm.x), 0.5f * (D.y - Dm.y));
+        SA[e] = make_float2(0.5f * (D.x + Dm.x), 0.5f * (D.y - Dm.y));
     }
     __syncthreads();
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        sm[i * ORG_LS + j] = vx[q];
+        const int e = t + 1024 * q;
+        sm[(e >> 7) * ORG_LS + (e & 127)] = SA[e];
     }
     __syncthreads();
     fft2d_128<1>(sm, tw);
     /* P7: rho^{n+1} on the box (copy_to_rhon, rho_functions.cpp:253-265) */
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        rbox[e] = rbox[e] + sm[i * ORG_LS + j].x / MMf;
+        const int e = t + 1024 * q;
+        rbox[e] = rbox[e] + sm[(e >> 7) * ORG_LS + (e & 127)].x / MMf;
     }
 }
 
 /* ======================================================================== */
 /* k_cells: update_all_cells (cell_phase_functions.cpp:126-217, mode&1) and
  * reshaping_cells (cell_phase_functions.cpp:447-580, mode&2) per cell, one
- * 1024-thread workgroup per cell.  The reshaping loop keeps the relaxed
- * field on the window plus a one-pixel ring in LDS, the normals (from the
- * start-of-step phi, useful_functions.cpp:356-388) in a per-cell scratch,
- * and the iterate in registers; the stop test runs in the workgroup. */
+ * 1024-thread workgroup per cell.  The relaxed field (window plus a
+ * one-pixel ring) and the iterate live in LDS, the normals (from the
+ * start-of-step phi, useful_functions.cpp:356-388) in a per-cell scratch;
+ * the stop test runs in the workgroup. */
 __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
                                                 const float *__restrict__ hfunc, const float *__restrict__ psi,
                                                 const float *__restrict__ hlum, const float *__restrict__ hecm,
@@ -853,27 +902,24 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
                                                 const int *__restrict__ yc, float *__restrict__ phin_win,
                                                 float *__restrict__ jwin, int *__restrict__ counts,
                                                 int *__restrict__ err, int mode) {
-    __shared__ float phase[ORG_WH_MAX * ORG_WH_MAX];
+    __shared__ float phase[ORG_WH * ORG_WH];
+    __shared__ float pnl[ORG_WH * ORG_WH];
     __shared__ double red[16];
-    const int n = blockIdx.x, t = threadIdx.x, N = P.N, W = P.W, WH = W + 2;
+    constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
+    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
     float *ph = phi + n * NN;
     const float *rh = rho + n * NN, *hf = hfunc + n * NN;
-    float *pw = phin_win + (size_t)n * W * W;
-    const int npx = W * W;
-    float pn[ORG_CELL_SLOTS];
+    float *pw = phin_win + (size_t)n * NPX;
 
     if (mode & 1) {
         const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
         const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
         const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
         const float fn = f0[n];
-#pragma unroll
-        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-            const int e = t + 1024 * q;
-            if (e >= npx) { pn[q] = 0.0f; continue; }
-            const int i = e / W, j = e % W;
+        for (int e = t; e < NPX; e += 1024) {
+            const int i = e / W, j = e - i * W;
             const size_t g = (size_t)(x0 + i) * N + (y0 + j);
             float v = ph[g];
             float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
@@ -895,21 +941,14 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
             out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
                    (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * hlum[g] + P.eta_lc * ll * inv4
                     - P.beta_ec * hecm[g] + P.eta_ec * le * inv4);
-            pn[q] = out;
+            if (mode & 2) pnl[(i + 1) * WH + (j + 1)] = out;
+            else pw[e] = out;
         }
-        if (!(mode & 2)) {
-#pragma unroll
-            for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-                const int e = t + 1024 * q;
-                if (e < npx) pw[e] = pn[q];
-            }
-            return;
-        }
+        if (!(mode & 2)) return;
     } else {
-#pragma unroll
-        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-            const int e = t + 1024 * q;
-            pn[q] = (e < npx) ? pw[e] : 0.0f;
+        for (int e = t; e < NPX; e += 1024) {
+            const int i = e / W, j = e - i * W;
+            pnl[(i + 1) * WH + (j + 1)] = pw[e];
         }
     }
 
@@ -917,8 +956,8 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
     float *jx = jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
     {
         const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
-        for (int e = t; e < WH * WH; e += blockDim.x) {
-            const int i = e / WH - 1, j = e % WH - 1;
+        for (int e = t; e < WH * WH; e += 1024) {
+            const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
             const size_t g = (size_t)(x0 + i) * N + (y0 + j);
             float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
             float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
@@ -934,40 +973,30 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
     const float dscale = P.dx * P.dx / P.ncur;
     double rdev = P.rdev0_cell;
     int ri = 0, cnt = 0;
-    float prev[ORG_CELL_SLOTS];
     while (rdev > P.thr_cell) {
-#pragma unroll
-        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-            const int e = t + 1024 * q;
-            prev[q] = pn[q];
-            if (e < npx) phase[(e / W + 1) * WH + (e % W + 1)] = pn[q];
+        for (int e = t; e < NPX; e += 1024) {          /* phase <- phasen (:480-492) */
+            const int i = e / W, l = (i + 1) * WH + (e - i * W + 1);
+            phase[l] = pnl[l];
         }
         __syncthreads();
         double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-            const int e = t + 1024 * q;
-            if (e < npx) {
-                const int l = (e / W + 1) * WH + (e % W + 1);
-                float nv = reshape_flux(pn[q], phase + l, WH, jx + l, jy + l, WH, ka, kc);
-                acc += (double)(fabsf(nv - pn[q]) * dscale);
-                pn[q] = nv;
-            }
+        for (int e = t; e < NPX; e += 1024) {          /* flux update (:498-549) */
+            const int i = e / W, l = (i + 1) * WH + (e - i * W + 1);
+            float u = pnl[l];
+            float nv = reshape_flux(u, phase + l, WH, jx + l, jy + l, WH, ka, kc);
+            acc += (double)(fabsf(nv - u) * dscale);
+            pnl[l] = nv;
         }
-        rdev = block_sum_d(acc, red);
+        rdev = block_sum_d(acc, red);                  /* (:554-562) */
         cnt++;
         if (ri > P.rimax) { if (t == 0) set_err(err, ORG_ERELAX); break; }
         ri++;
-        __syncthreads();
     }
-    /* phi <- second-to-last iterate; phin keeps the last */
-#pragma unroll
-    for (int q = 0; q < ORG_CELL_SLOTS; q++) {
-        const int e = t + 1024 * q;
-        if (e < npx) {
-            ph[(size_t)(x0 + e / W) * N + (y0 + e % W)] = prev[q];
-            pw[e] = pn[q];
-        }
+    /* phi <- second-to-last iterate (still in phase); phin keeps the last */
+    for (int e = t; e < NPX; e += 1024) {
+        const int i = e / W, j = e - i * W, l = (i + 1) * WH + (j + 1);
+        ph[(size_t)(x0 + i) * N + (y0 + j)] = phase[l];
+        pw[e] = pnl[l];
     }
     if (t == 0) counts[n] = cnt;
 }
@@ -976,16 +1005,17 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
  * useful_functions.cpp:679-693, run_with_active_gels.cpp:175-178): the FFT box
  * values plus the periodic ghost ring applied to rhon
  * (rho_functions.cpp:469). */
-__global__ __launch_bounds__(256) void k_commit_cells(DevP P, float *__restrict__ rho,
-                                                      const float *__restrict__ rho_box, const int *__restrict__ xc,
-                                                      const int *__restrict__ yc) {
-    const int n = blockIdx.x, N = P.N, M = P.M;
+__global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict__ rho,
+                                                       const float *__restrict__ rho_box, const int *__restrict__ xc,
+                                                       const int *__restrict__ yc) {
+    const int n = blockIdx.x, N = P.N;
+    constexpr int M = ORG_MB;
     const size_t NN = (size_t)N * N;
     float *rh = rho + n * NN;
     const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
     const float *rb = rho_box + (size_t)n * M * M;
     for (int e = threadIdx.x; e < M * M; e += blockDim.x)
-        rh[(size_t)(x0 + e / M) * N + (y0 + e % M)] = rb[e];
+        rh[(size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127))] = rb[e];
     /* ghost ring: sources are rows/cols 1 and N-2, outside every box */
     for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
         rh[(size_t)i * N] = rh[(size_t)i * N + N - 2];

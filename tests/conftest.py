@@ -14,8 +14,8 @@ def pytest_configure(config):
 
 def rel_l2(a, b):
     import numpy as np
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 

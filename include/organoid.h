@@ -170,14 +170,12 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
                                                       useful_functions.cpp:695-709 (:181-182) */
 
 /* ---- batched form -----------------------------------------------------------
- * One full timestep = the loop body run_with_active_gels.cpp:118-184, one
- * launch per stage for all cells.  use_graph=1 captures the step as two
- * hipGraphs on first use and replays them: part 1 (centroids ... lumen/ECM
- * update and the first reshaping sweeps) and part 2 (spectral cortex update,
- * phase-field update and reshaping of every cell).  The lumen/ECM stop flags
- * are copied to pinned memory between the two parts; the host reads them
- * while part 2 runs and queues further sweeps only if a loop has not
- * stopped.  Cell reshaping loops run entirely on the device.                  */
+ * One full timestep = the loop body run_with_active_gels.cpp:118-184 with no
+ * host round trip: one launch per stage for all cells, and one launch for
+ * the spectral cortex update, the phase-field update and reshaping of every
+ * cell and the lumen/ECM reshaping loops together (their stop tests run on
+ * the device).  use_graph=1 captures the step into a hipGraph on first use
+ * and replays it.                                                            */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
 
 /* Device synchronisation and timing.  org_timer_* brackets work on the
@@ -194,11 +192,8 @@ enum {
 };
 
 /* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of
- * the last step for org_get_window.  ORG_OPT_GRAPH_LUM_ITERS (default 4): lumen
- * and ECM reshaping sweeps queued (and captured) with the first part of each
- * step; a step that needs more gets further sweeps after the host has read
- * the stop flags, so results never depend on this value. */
-enum { ORG_OPT_WRITE_VELOCITY = 0, ORG_OPT_GRAPH_LUM_ITERS = 1 };
+ * the last step for org_get_window. */
+enum { ORG_OPT_WRITE_VELOCITY = 0 };
 int org_set_option(org_ctx *ctx, int option, int value);
 
 /* Test hook (no counterpart in the reference): one unnormalised 128 x 128

@@ -32,6 +32,11 @@ struct org_ctx {
     org_params p{};
     DevP dp{};
     hipStream_t stream = nullptr;
+    double *part_lc = nullptr;           /* loop role partial deviations [2][2][blocks] */
+    unsigned *lum_bar = nullptr;         /* loop role barrier: count, generation */
+    int *lum_abort = nullptr;
+    long long wait_limit = 0;            /* loop role barrier timeout, wall-clock ticks */
+    float *phi_win = nullptr;
     int N = 0, nc = 0, zs = 0, W = 0, M = 0;
     size_t NN = 0, P = 0;
     int grid_g = 0, ntiles = 0;
@@ -46,20 +51,15 @@ struct org_ctx {
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
     int *uni = nullptr, *jbox = nullptr;   /* union of windows; boxes of nonzero lumen/ECM normals */
-    int lum_grid = 256;                    /* blocks per field of a reshaping sweep */
-    int last_sweeps = 0;                   /* sweeps the previous step needed */
+    int lum_grid = ORG_LUM_BLOCKS;         /* workgroups per field of a reshaping sweep */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
     float *f0 = nullptr;
     unsigned *counters = nullptr;
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
-    int graph_lum_iters = 4;
-    hipGraphExec_t gexec = nullptr, gexec2 = nullptr;
-    hipGraph_t graph = nullptr, graph2 = nullptr;
-    LumState *lst_host = nullptr;     /* pinned: stop flags read behind lum_event */
-    hipEvent_t lum_event = nullptr;
-    long long extra_sweeps = 0;       /* sweeps launched after the captured ones */
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
     /* profiling */
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
@@ -82,8 +82,9 @@ int dalloc(org_ctx *c, T **ptr, size_t count) {
 struct Prof {
     org_ctx *c;
     int kid;
+    hipStream_t st;
     hipEvent_t stop = nullptr;
-    Prof(org_ctx *c_, int kid_) : c(c_), kid(kid_) {
+    Prof(org_ctx *c_, int kid_, hipStream_t st_ = nullptr) : c(c_), kid(kid_), st(st_ ? st_ : c_->stream) {
         if (!c->prof) return;
         auto &v = c->ev[kid];
         if (c->ev_used[kid] == v.size()) {
@@ -92,11 +93,11 @@ struct Prof {
             v.push_back({a, b});
         }
         auto &pr = v[c->ev_used[kid]++];
-        (void)hipEventRecord(pr.first, c->stream);
+        (void)hipEventRecord(pr.first, st);
         stop = pr.second;
     }
     ~Prof() {
-        if (stop) (void)hipEventRecord(stop, c->stream);
+        if (stop) (void)hipEventRecord(stop, st);
     }
 };
 
@@ -185,7 +186,7 @@ int L_prep(org_ctx *c, int mode) {
 }
 int L_hlum(org_ctx *c) {
     Prof pr(c, ORG_K_LUMECM);
-    hipLaunchKernelGGL(k_hlum, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum, c->hecm,
+    hipLaunchKernelGGL(k_hlum, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum, c->hecm,
                        c->part_h, c->counters + 2, c->sum_ecm);
     return launch_check();
 }
@@ -196,25 +197,26 @@ int L_psi(org_ctx *c) {
 }
 int L_lumecm_update(org_ctx *c) {
     Prof pr(c, ORG_K_LUMECM);
-    hipLaunchKernelGGL(k_lumecm_update, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->psi,
-                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->lst, c->jbox);
+    hipLaunchKernelGGL(k_lumecm_update, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->psi,
+                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->lst, c->jbox,
+                       c->lum_abort, c->lum_bar);
     return launch_check();
 }
 int L_lum_normals(org_ctx *c, int fbase, int nf) {
     Prof pr(c, ORG_K_LUMRESHAPE);
-    hipLaunchKernelGGL(k_lum_normals, dim3(c->grid_g, nf), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
+    hipLaunchKernelGGL(k_lum_normals, dim3(c->grid_g, nf), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
                        c->bufE0, c->jlx, c->jly, c->jex, c->jey, c->jbox, fbase);
     return launch_check();
 }
 int L_lum_iter(org_ctx *c, int fbase, int nf) {
     Prof pr(c, ORG_K_LUMRESHAPE);
-    hipLaunchKernelGGL(k_lum_iter, dim3(c->lum_grid, nf), dim3(256), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
+    hipLaunchKernelGGL(k_lum_iter, dim3(c->lum_grid, nf), dim3(1024), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
                        c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->lst, c->part_l, c->counters, c->err, fbase);
     return launch_check();
 }
 int L_commit_lumecm(org_ctx *c) {
     Prof pr(c, ORG_K_LUMRESHAPE);
-    hipLaunchKernelGGL(k_commit_lumecm, dim3(c->grid_g), dim3(256), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
+    hipLaunchKernelGGL(k_commit_lumecm, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
                        c->bufL1, c->bufE0, c->bufE1, c->lst, c->err);
     return launch_check();
 }
@@ -232,10 +234,28 @@ int L_cells(org_ctx *c, int mode) {
 }
 int L_commit_cells(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
-    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->xc, c->yc);
+    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
+                       (const float *)nullptr, c->xc, c->yc);
     return launch_check();
 }
-
+/* spectral + cells + lumen/ECM loops in one launch; phi's new window goes
+ * to phi_win */
+int L_cortex_cells(org_ctx *c) {
+    Prof pr(c, ORG_K_SPECTRAL);
+    LumBufs lb{c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox,
+               c->lst, c->part_lc, c->lum_bar, c->lum_abort, c->wait_limit};
+    hipLaunchKernelGGL(k_cortex_cells, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, c->phi,
+                       c->rho, c->lum, c->ecm, c->hfunc, c->psi, c->hlum, c->hecm, c->f0, c->xc, c->yc, c->spec,
+                       c->rho_box, c->vxbox, c->vybox, c->write_v, c->phin_win, c->phi_win, c->jwin, c->counts,
+                       c->err, lb);
+    return launch_check();
+}
+int L_commit_cells_win(org_ctx *c) {
+    Prof pr(c, ORG_K_COMMIT);
+    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
+                       (const float *)c->phi_win, c->xc, c->yc);
+    return launch_check();
+}
 #define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
 
 /* Host-driven lumen/ECM loop: run sweeps until both stop tests passed,
@@ -254,64 +274,22 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
     }
 }
 
-/* One timestep, run_with_active_gels.cpp:118-184, in two parts.
- * Part 1: centroids, h, psi, areas, lumen/ECM update, normals and the first
- * `lum_iters` stop-tested reshaping sweeps.  Part 2 (independent of the
- * lumen/ECM loop): the spectral cortex update, the phase-field update and
- * reshaping of every cell, rho copy-back.  The lumen/ECM loop is finished
- * by finish_lum after part 2 is queued. */
-int enqueue_part1(org_ctx *c, int lum_iters) {
+/* One timestep, run_with_active_gels.cpp:118-184.  Part 1: centroids, h,
+ * psi, areas, lumen/ECM update and normals.  Part 2: one launch for the
+ * spectral cortex update and the phase-field update/reshaping of every cell
+ * and the lumen/ECM reshaping loops, then the copy-back.  No host round
+ * trip: the whole step can be captured in one hipGraph. */
+int enqueue_step(org_ctx *c) {
     TRY(L_centroid(c));             /* :123 */
     TRY(L_prep(c, 3));              /* :133, :142 */
     TRY(L_hlum(c));                 /* :134 */
     TRY(L_psi(c));                  /* :139 */
     TRY(L_lumecm_update(c));        /* :157-158 */
     TRY(L_lum_normals(c, 0, 2));    /* :160-161 */
-    for (int k = 0; k < lum_iters; k++) TRY(L_lum_iter(c, 0, 2));
-    return ORG_OK;
-}
-int enqueue_part2(org_ctx *c) {
-    TRY(L_spectral(c));             /* :130, :137, :154 */
-    TRY(L_cells(c, 3));             /* :164-172 */
-    TRY(L_commit_cells(c));         /* :175-178 */
-    return ORG_OK;
-}
-/* Snapshot the stop flags into pinned memory behind an event (queued after
- * part 1, before part 2). */
-int mark_lum(org_ctx *c) {
-    HIPCHK(hipMemcpyAsync(c->lst_host, c->lst, 2 * sizeof(LumState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipEventRecord(c->lum_event, c->stream));
-    return ORG_OK;
-}
-/* Wait for the snapshot (part 2 keeps the device busy meanwhile), run more
- * sweeps until both stop tests passed (reference: host while loop,
- * cell_phase_functions.cpp:774-869), then lum <- lumn, ECM <- ECMn. */
-int finish_lum(org_ctx *c, int queued) {
-    HIPCHK(hipEventSynchronize(c->lum_event));
-    long long guard = 0;
-    int sweeps = queued;
-    /* first extra round: as many sweeps as the previous step needed beyond
-     * the queued ones (surplus sweeps stop at the device-side test) */
-    int round = std::max(2, c->last_sweeps - queued + 1);
-    while (!(c->lst_host[0].done && c->lst_host[1].done)) {
-        for (int k = 0; k < round; k++) TRY(L_lum_iter(c, 0, 2));
-        c->extra_sweeps += round;
-        sweeps += round;
-        TRY(mark_lum(c));
-        HIPCHK(hipEventSynchronize(c->lum_event));
-        if ((guard += round) > (long long)c->p.rimax + 4) return note_error(ORG_ERELAX);
-        round = 2;
-    }
-    c->last_sweeps = std::max(c->lst_host[0].ri, c->lst_host[1].ri);
-    (void)sweeps;
+    TRY(L_cortex_cells(c));         /* :130, :137, :154, :160-172 */
+    TRY(L_commit_cells_win(c));     /* :172 phi window, :175-178 */
     TRY(L_commit_lumecm(c));        /* :181-182 */
     return ORG_OK;
-}
-int enqueue_step(org_ctx *c, int lum_iters) {
-    TRY(enqueue_part1(c, lum_iters));
-    TRY(mark_lum(c));
-    TRY(enqueue_part2(c));
-    return finish_lum(c, lum_iters);
 }
 
 }  // namespace
@@ -354,7 +332,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     c->dp = make_devp(*p);
     c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
     c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
-    c->grid_g = std::min(c->N, 1024);
+    c->grid_g = std::min(c->N, 128);
     int nt = (c->N + 31) / 32;
     c->ntiles = nt * nt;
     int rc = ORG_OK;
@@ -369,14 +347,21 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 8);
+    A(&c->uni, 4); A(&c->jbox, 8); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
-    if (!rc && hipHostMalloc((void **)&c->lst_host, 2 * sizeof(LumState), hipHostMallocDefault) != hipSuccess)
-        rc = note_error(ORG_EHIP);
-    if (!rc && hipEventCreateWithFlags(&c->lum_event, hipEventDisableTiming) != hipSuccess) rc = note_error(ORG_EHIP);
+    if (!rc) {
+        /* loop-role barrier timeout: 2 s of the device's wall clock */
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            rc = note_error(ORG_EHIP);
+        else
+            c->wait_limit = 2000LL * khz;
+    }
     if (rc) { org_destroy(c); return rc; }
     std::vector<double> s0v(c->nc);
     for (int n = 0; n < c->nc; n++) s0v[n] = (p->use_s0_cell && s0_cell) ? s0_cell[n] : p->s0;
@@ -386,7 +371,8 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         hipMemcpy(c->yc, cen.data(), sizeof(int) * c->nc, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->err, 0, sizeof(int)) != hipSuccess || hipMemset(c->counters, 0, 4 * sizeof(unsigned)) != hipSuccess ||
         hipMemset(c->counts, 0, sizeof(int) * c->nc) != hipSuccess || hipMemset(c->lst, 0, 2 * sizeof(LumState)) != hipSuccess ||
-        hipMemset(c->hfunc, 0, sizeof(float) * c->P) != hipSuccess) {
+        hipMemset(c->hfunc, 0, sizeof(float) * c->P) != hipSuccess ||
+        hipMemset(c->lum_bar, 0, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(c->lum_abort, 0, sizeof(int)) != hipSuccess) {
         org_destroy(c);
         return note_error(ORG_EHIP);
     }
@@ -399,10 +385,6 @@ int org_destroy(org_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    if (c->gexec2) (void)hipGraphExecDestroy(c->gexec2);
-    if (c->graph2) (void)hipGraphDestroy(c->graph2);
-    if (c->lum_event) (void)hipEventDestroy(c->lum_event);
-    if (c->lst_host) (void)hipHostFree(c->lst_host);
     for (int k = 0; k < ORG_K_COUNT; k++)
         for (auto &pr : c->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (void *q : c->allocs) (void)hipFree(q);
@@ -518,16 +500,11 @@ int org_set_option(org_ctx *c, int option, int value) {
     TRY(check_ctx(c));
     switch (option) {
     case ORG_OPT_WRITE_VELOCITY: c->write_v = value ? 1 : 0; break;
-    case ORG_OPT_GRAPH_LUM_ITERS:
-        if (value < 1 || value > 1000) return note_error(ORG_EINVAL);
-        c->graph_lum_iters = value;
-        break;
+
     default: return note_error(ORG_EINVAL);
     }
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (c->gexec2) { (void)hipGraphExecDestroy(c->gexec2); c->gexec2 = nullptr; }
-    if (c->graph2) { (void)hipGraphDestroy(c->graph2); c->graph2 = nullptr; }
     return ORG_OK;
 }
 
@@ -563,33 +540,25 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
     if (!use_graph) {
-        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c, c->graph_lum_iters));
+        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c));
         return harvest_device_error(c);
     }
     if (!c->gexec) {
         bool prof = c->prof;
-        c->prof = false;                 /* no event records inside the graphs */
-        for (int part = 1; part <= 2; part++) {
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = part == 1 ? enqueue_part1(c, c->graph_lum_iters) : enqueue_part2(c);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (rc || e != hipSuccess) {
-                c->prof = prof;
-                if (g) (void)hipGraphDestroy(g);
-                return rc ? rc : note_error(ORG_EHIP);
-            }
-            (part == 1 ? c->graph : c->graph2) = g;
-            HIPCHK(hipGraphInstantiate(part == 1 ? &c->gexec : &c->gexec2, g, nullptr, nullptr, 0));
-        }
+        c->prof = false;                 /* no event records inside the graph */
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->prof = prof;
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : note_error(ORG_EHIP);
+        }
+        c->graph = g;
+        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     }
-    for (int s = 0; s < nsteps; s++) {
-        HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-        TRY(mark_lum(c));
-        HIPCHK(hipGraphLaunch(c->gexec2, c->stream));
-        TRY(finish_lum(c, c->graph_lum_iters));
-    }
+    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     return harvest_device_error(c);
 }
 

@@ -414,7 +414,7 @@ __global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__
 /* k_hlum: compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285)
  * and the FP64 sum_ecm of update_lumen_and_ECM (cell_phase_functions.cpp:376-386),
  * reduced by the last block to finish (fixed order: deterministic). */
-__global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
+__global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
                                               float *__restrict__ hlum, float *__restrict__ hecm,
                                               double *__restrict__ part, unsigned *__restrict__ counter,
                                               double *__restrict__ sum_out) {
@@ -455,6 +455,7 @@ __global__ __launch_bounds__(256) void k_hlum(DevP P, const float *__restrict__ 
 __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ box,
                                              const int *__restrict__ uni, float *__restrict__ psi) {
     __shared__ int list[1024];
+    __shared__ int4 lbox[1024];
     __shared__ int cnt;
     const int N = P.N, ntx = (N + 31) / 32;
     const int ti0 = (blockIdx.x / ntx) * 32, tj0 = (blockIdx.x % ntx) * 32;
@@ -474,7 +475,10 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
             if (hit) {
                 unsigned long long below = m & ((1ull << threadIdx.x) - 1ull);
                 int pos = base + __popcll(below);
-                if (pos < 1024) list[pos] = n;
+                if (pos < 1024) {
+                    list[pos] = n;
+                    lbox[pos] = make_int4(box[4 * n], box[4 * n + 1], box[4 * n + 2], box[4 * n + 3]);
+                }
             }
             base += __popcll(m);
         }
@@ -487,9 +491,8 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
         if (i >= N || j >= N || i < u0 || i >= u1 || j < v0 || j >= v1) continue;
         float acc = 0.0f;
         for (int k = 0; k < cnt; k++) {
-            int n = list[k];
-            const int *bx = box + 4 * n;
-            if (i >= bx[0] && i < bx[1] && j >= bx[2] && j < bx[3]) acc += hfunc[n * NN + (size_t)i * N + j];
+            const int4 bx = lbox[k];
+            if (i >= bx.x && i < bx.y && j >= bx.z && j < bx.w) acc += hfunc[list[k] * NN + (size_t)i * N + j];
         }
         psi[(size_t)i * N + j] = acc;
     }
@@ -508,14 +511,16 @@ struct LumState { int done, ri, cur, pad; };
  * Writes the first iterate of both reshaping loops into both ping-pong
  * buffers (sweeps rewrite only their active box), resets the loop states and
  * the normals' boxes. */
-__global__ __launch_bounds__(256) void k_lumecm_update(DevP P, const float *__restrict__ lum,
+__global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__restrict__ lum,
                                                        const float *__restrict__ ecm, const float *__restrict__ psi,
                                                        const float *__restrict__ hlum, const float *__restrict__ hecm,
                                                        const double *__restrict__ sum_ecm, float *__restrict__ lumA,
                                                        float *__restrict__ lumB, float *__restrict__ ecmA,
                                                        float *__restrict__ ecmB, LumState *__restrict__ lst,
-                                                       int *__restrict__ jbox) {
+                                                       int *__restrict__ jbox, int *__restrict__ abort_flag,
+                                                       unsigned *__restrict__ bar) {
     const int N = P.N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *abort_flag = 0; bar[0] = 0u; }
     if (blockIdx.x == 0 && threadIdx.x < 2) {
         bool run = P.rdev0_lum > P.thr_lum;
         lst[threadIdx.x].done = run ? 0 : 1;
@@ -559,7 +564,7 @@ static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N
  * changes a pixel only if the normal at it or at one of its four neighbours
  * is nonzero (every flux term carries a face average of the normals).
  * blockIdx.y + fbase: 0 lumen, 1 ECM. */
-__global__ __launch_bounds__(256) void k_lum_normals(DevP P, const float *__restrict__ lum_old,
+__global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__restrict__ lum_old,
                                                      const float *__restrict__ ecm_old, const float *__restrict__ lumn,
                                                      const float *__restrict__ ecmn, float *__restrict__ jlx,
                                                      float *__restrict__ jly, float *__restrict__ jex,
@@ -629,50 +634,39 @@ static __device__ __forceinline__ float reshape_flux(float un, const float *u, i
 #undef JY_
 }
 
-/* k_lum_iter: one sweep of reshaping_lumen_or_ECM's loop
- * (cell_phase_functions.cpp:774-869) for the lumen (field 0) and the ECM
- * (1): X_{k+1} = X_k + flux(X_k) inside, Dirichlet edges (value side) and
- * zero corners (useful_functions.cpp:422-437), deviation
- * sum |X_{k+1}-X_k| dx^2 in FP64; the last block to finish applies the stop
- * test.  Interior pixels are swept only inside the normals' box grown by one
- * pixel (outside it X_{k+1} = X_k exactly); the boundary ring every sweep.
- * A field whose loop has stopped skips the sweep. */
-__global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bufL0, float *__restrict__ bufL1,
-                                                  float *__restrict__ bufE0, float *__restrict__ bufE1,
-                                                  const float *__restrict__ jlx, const float *__restrict__ jly,
-                                                  const float *__restrict__ jex, const float *__restrict__ jey,
-                                                  const int *__restrict__ jbox, LumState *__restrict__ lst,
-                                                  double *__restrict__ part, unsigned *__restrict__ counter,
-                                                  int *__restrict__ err, int fbase) {
-    __shared__ double red[16];
-    __shared__ bool last;
-    const int f = fbase + blockIdx.y, N = P.N;
-    const LumState s = lst[f];
-    if (s.done) return;
-    float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
-    const float *src = (s.ri & 1) ? b1 : b0;
-    float *dst = (s.ri & 1) ? b0 : b1;
-    const float *jx = f ? jex : jlx, *jy = f ? jey : jly;
-    const float side = f ? 1.0f : 0.0f;
+/* One reshaping sweep of reshaping_lumen_or_ECM's loop
+ * (cell_phase_functions.cpp:774-869) for one field: X_{k+1} = X_k + flux(X_k)
+ * inside, Dirichlet edges (value side) and zero corners
+ * (useful_functions.cpp:422-437).  Interior pixels are swept only inside
+ * the normals' box grown by one pixel (outside it X_{k+1} = X_k exactly);
+ * the boundary ring every sweep.  The work is split over `nblk` workgroups
+ * of 1024 threads as one flat pixel range; returns this thread's share of
+ * the deviation sum |X_{k+1}-X_k| dx^2.  The same decomposition is used by
+ * the stand-alone and the fused loop so their sums agree bit for bit. */
+#define ORG_LUM_BLOCKS 64
+static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *__restrict__ src,
+                                                   float *__restrict__ dst, const float *__restrict__ jx,
+                                                   const float *__restrict__ jy, const int *__restrict__ jb,
+                                                   float side, int blk, int nblk) {
+    const int N = P.N;
     const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
     const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
     const float dx2 = P.dx * P.dx;
-    const int *jb = jbox + 4 * f;
     const int r0 = max(jb[0] - 1, 1), r1 = min(jb[1] + 1, N - 2);
     const int c0 = max(jb[2] - 1, 1), c1 = min(jb[3] + 1, N - 2);
+    const int bw = c1 - c0 + 1, bh = r1 - r0 + 1;
+    const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
+    const int stride = nblk * 1024;
     double acc = 0.0;
-    for (int i = r0 + blockIdx.x; i <= r1; i += gridDim.x) {
-        const size_t r = (size_t)i * N;
-        for (int j = c0 + threadIdx.x; j <= c1; j += blockDim.x) {
-            const size_t e = r + j;
-            float u = src[e];
-            float v = reshape_flux(u, src + e, N, jx + e, jy + e, N, ka, kc);
-            dst[e] = v;
-            acc += (double)(fabsf(v - u) * dx2);
-        }
+    for (int k = blk * 1024 + (int)threadIdx.x; k < npx; k += stride) {
+        const int i = r0 + k / bw, j = c0 + k % bw;
+        const size_t e = (size_t)i * N + j;
+        float u = src[e];
+        float v = reshape_flux(u, src + e, N, jx + e, jy + e, N, ka, kc);
+        dst[e] = v;
+        acc += (double)(fabsf(v - u) * dx2);
     }
-    /* boundary ring: 4N-4 pixels, edges -> side, corners -> 0 */
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 4 * N - 4; k += gridDim.x * blockDim.x) {
+    for (int k = blk * 1024 + (int)threadIdx.x; k < 4 * N - 4; k += stride) {
         int i, j;
         if (k < N) { i = 0; j = k; }
         else if (k < 2 * N) { i = N - 1; j = k - N; }
@@ -684,6 +678,48 @@ __global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bu
         dst[e] = v;
         acc += (double)(fabsf(v - src[e]) * dx2);
     }
+    return acc;
+}
+
+/* Sum of one field's per-workgroup deviations in a fixed order, valid in
+ * every thread (1024 threads, nblk <= 1024). */
+static __device__ __forceinline__ double sum_parts(const double *part, int nblk, double *red) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+        v += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return block_sum_d(v, red);
+}
+
+/* Stop test after a sweep (cell_phase_functions.cpp:774, 862-868). */
+static __device__ __forceinline__ void lum_stop(const DevP &P, LumState &s, double rdev, int *err) {
+    s.cur = (s.ri + 1) & 1;
+    if (s.ri > P.rimax) { set_err(err, ORG_ERELAX); s.done = 1; }
+    s.ri += 1;
+    if (!(rdev > P.thr_lum)) s.done = 1;
+}
+
+/* k_lum_iter: one sweep of the lumen (field 0) and / or ECM (1) loop;
+ * ORG_LUM_BLOCKS workgroups per field; the last workgroup to finish applies
+ * the stop test.  A field whose loop has stopped skips the sweep.  Used by
+ * the per-function entry point, whose host loop reads the stop flag after
+ * every sweep as the reference does. */
+__global__ __launch_bounds__(1024) void k_lum_iter(DevP P, float *__restrict__ bufL0, float *__restrict__ bufL1,
+                                                   float *__restrict__ bufE0, float *__restrict__ bufE1,
+                                                   const float *__restrict__ jlx, const float *__restrict__ jly,
+                                                   const float *__restrict__ jex, const float *__restrict__ jey,
+                                                   const int *__restrict__ jbox, LumState *__restrict__ lst,
+                                                   double *__restrict__ part, unsigned *__restrict__ counter,
+                                                   int *__restrict__ err, int fbase) {
+    __shared__ double red[16];
+    __shared__ bool last;
+    const int f = fbase + blockIdx.y;
+    const LumState s = lst[f];
+    if (s.done) return;
+    float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
+    const float *src = (s.ri & 1) ? b1 : b0;
+    float *dst = (s.ri & 1) ? b0 : b1;
+    double acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jbox + 4 * f, f ? 1.0f : 0.0f, blockIdx.x,
+                           gridDim.x);
     double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
         part[f * gridDim.x + blockIdx.x] = tot;
@@ -694,25 +730,88 @@ __global__ __launch_bounds__(256) void k_lum_iter(DevP P, float *__restrict__ bu
     __syncthreads();
     if (last) {
         __threadfence();
-        double v = 0.0;
-        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-            v += __hip_atomic_load(&part[f * gridDim.x + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double rdev = block_sum_d(v, red);
+        double rdev = sum_parts(part + f * gridDim.x, gridDim.x, red);
         if (threadIdx.x == 0) {
             LumState o = s;
-            o.cur = (s.ri + 1) & 1;
-            if (s.ri > P.rimax) { set_err(err, ORG_ERELAX); o.done = 1; }
-            o.ri = s.ri + 1;
-            if (!(rdev > P.thr_lum)) o.done = 1;
+            lum_stop(P, o, rdev, err);
             lst[f] = o;
             counter[f] = 0u;
         }
     }
 }
 
+/* Barrier among the `nblk` loop workgroups of one launch (generation
+ * counter, sleeping poll).  Only loop workgroups ever wait, and only on each
+ * other, and every other workgroup of the launch finishes on its own, so all
+ * loop workgroups become resident eventually.  A wait longer than
+ * `limit` wall-clock ticks sets ORG_ERELAX and the abort flag instead of
+ * waiting on. */
+static __device__ bool lum_barrier(unsigned *bar, int nblk, int *abort_flag, int *err, long long limit) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        unsigned g = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned arrived = atomicAdd(&bar[0], 1u);
+        if (arrived == (unsigned)nblk - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            atomicAdd(&bar[1], 1u);
+        } else {
+            long long t0 = wall_clock64();
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    wall_clock64() - t0 > limit) {
+                    atomicExch(abort_flag, 1);
+                    set_err(err, ORG_ERELAX);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+/* Both reshaping loops (lumen and ECM) run to their stop tests by the loop
+ * workgroups of k_cortex_cells, one barrier per sweep.  Every loop workgroup
+ * sums the per-workgroup deviations in the same order, so all take the same
+ * stop decision after the same sweep.  part: [2 parities][2 fields][nblk]. */
+static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *red, float *bufL0, float *bufL1,
+                                     float *bufE0, float *bufE1, const float *jlx, const float *jly,
+                                     const float *jex, const float *jey, const int *jbox, LumState *lst,
+                                     double *part, unsigned *bar, int *abort_flag, int *err, long long limit) {
+    LumState s[2] = {lst[0], lst[1]};
+    for (int k = 0; !(s[0].done && s[1].done); k++) {
+        double *pk = part + (size_t)(k & 1) * 2 * nblk;
+        for (int f = 0; f < 2; f++) {
+            double acc = 0.0;
+            if (!s[f].done) {
+                float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
+                const float *src = (s[f].ri & 1) ? b1 : b0;
+                float *dst = (s[f].ri & 1) ? b0 : b1;
+                acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jbox + 4 * f, f ? 1.0f : 0.0f, blk, nblk);
+            }
+            double tot = block_sum_d(acc, red);
+            if (threadIdx.x == 0) pk[f * nblk + blk] = tot;
+        }
+        if (!lum_barrier(bar, nblk, abort_flag, err, limit)) return;
+        for (int f = 0; f < 2; f++) {
+            if (s[f].done) continue;
+            double rdev = sum_parts(pk + f * nblk, nblk, red);
+            lum_stop(P, s[f], rdev, err);
+        }
+    }
+    if (blk == 0 && threadIdx.x < 2) lst[threadIdx.x] = s[threadIdx.x];
+}
+
 /* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
  * run_with_active_gels.cpp:181-182) from the buffer holding the last iterate. */
-__global__ __launch_bounds__(256) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
+__global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
                                                        const float *__restrict__ bufL0, const float *__restrict__ bufL1,
                                                        const float *__restrict__ bufE0, const float *__restrict__ bufE1,
                                                        const LumState *__restrict__ lst, int *__restrict__ err) {
@@ -745,15 +844,14 @@ __global__ __launch_bounds__(256) void k_commit_lumecm(DevP P, float *__restrict
  * reference keeps only Re, rho_functions.cpp:279), F(rho v_x + i rho v_y),
  * and F^-1(H D).  The per-mode solve uses the viscosity of the real-space
  * pixel with the mode's indices (rho_functions.cpp:149-150, 170). */
-__global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restrict__ phi,
-                                                   const float *__restrict__ rho, const float *__restrict__ lum,
-                                                   const float *__restrict__ ecm, const int *__restrict__ xc,
-                                                   const int *__restrict__ yc, float2 *__restrict__ scratch,
-                                                   float *__restrict__ rho_box, float *__restrict__ vxbox,
-                                                   float *__restrict__ vybox, int write_v) {
-    __shared__ float2 sm[ORG_MB * ORG_LS];
-    __shared__ float2 tw[128];
-    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
+static __device__ __forceinline__ void spectral_body(const DevP &P, const int n, float2 *__restrict__ sm,
+                                                     float2 *__restrict__ tw, const float *__restrict__ phi,
+                                                     const float *__restrict__ rho, const float *__restrict__ lum,
+                                                     const float *__restrict__ ecm, const int *__restrict__ xc,
+                                                     const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                     float *__restrict__ rho_box, float *__restrict__ vxbox,
+                                                     float *__restrict__ vybox, int write_v) {
+    const int t = threadIdx.x, N = P.N;
     constexpr int MM = ORG_MB * ORG_MB;
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;   /* FFT box origin, useful_functions.cpp:618-619 */
@@ -888,6 +986,17 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
     }
 }
 
+__global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restrict__ phi,
+                                                   const float *__restrict__ rho, const float *__restrict__ lum,
+                                                   const float *__restrict__ ecm, const int *__restrict__ xc,
+                                                   const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                   float *__restrict__ rho_box, float *__restrict__ vxbox,
+                                                   float *__restrict__ vybox, int write_v) {
+    __shared__ float2 sm[ORG_MB * ORG_LS];
+    __shared__ float2 tw[128];
+    spectral_body(P, blockIdx.x, sm, tw, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
+}
+
 /* ======================================================================== */
 /* k_cells: update_all_cells (cell_phase_functions.cpp:126-217, mode&1) and
  * reshaping_cells (cell_phase_functions.cpp:447-580, mode&2) per cell, one
@@ -895,18 +1004,17 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
  * one-pixel ring) and the iterate live in LDS, the normals (from the
  * start-of-step phi, useful_functions.cpp:356-388) in a per-cell scratch;
  * the stop test runs in the workgroup. */
-__global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
-                                                const float *__restrict__ hfunc, const float *__restrict__ psi,
-                                                const float *__restrict__ hlum, const float *__restrict__ hecm,
-                                                const float *__restrict__ f0, const int *__restrict__ xc,
-                                                const int *__restrict__ yc, float *__restrict__ phin_win,
-                                                float *__restrict__ jwin, int *__restrict__ counts,
-                                                int *__restrict__ err, int mode) {
-    __shared__ float phase[ORG_WH * ORG_WH];
-    __shared__ float pnl[ORG_WH * ORG_WH];
-    __shared__ double red[16];
+static __device__ __forceinline__ void cells_body(const DevP &P, const int n, float *__restrict__ phase,
+                                                  float *__restrict__ pnl, double *__restrict__ red,
+                                                  float *__restrict__ phi, const float *__restrict__ rho,
+                                                  const float *__restrict__ hfunc, const float *__restrict__ psi,
+                                                  const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                                  const float *__restrict__ f0, const int *__restrict__ xc,
+                                                  const int *__restrict__ yc, float *__restrict__ phin_win,
+                                                  float *__restrict__ phi_win, float *__restrict__ jwin,
+                                                  int *__restrict__ counts, int *__restrict__ err, int mode) {
     constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
-    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
+    const int t = threadIdx.x, N = P.N;
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
     float *ph = phi + n * NN;
@@ -992,13 +1100,82 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
         if (ri > P.rimax) { if (t == 0) set_err(err, ORG_ERELAX); break; }
         ri++;
     }
-    /* phi <- second-to-last iterate (still in phase); phin keeps the last */
+    /* phi <- second-to-last iterate (still in phase); phin keeps the last.
+     * With phi_win the window goes to a per-cell buffer that k_commit_cells
+     * writes back, so other cells' workgroups of the same launch still read
+     * the start-of-step phi. */
+    float *pwin = phi_win ? phi_win + (size_t)n * NPX : nullptr;
     for (int e = t; e < NPX; e += 1024) {
         const int i = e / W, j = e - i * W, l = (i + 1) * WH + (j + 1);
-        ph[(size_t)(x0 + i) * N + (y0 + j)] = phase[l];
+        if (pwin) pwin[e] = phase[l];
+        else ph[(size_t)(x0 + i) * N + (y0 + j)] = phase[l];
         pw[e] = pnl[l];
     }
     if (t == 0) counts[n] = cnt;
+}
+
+__global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
+                                                const float *__restrict__ hfunc, const float *__restrict__ psi,
+                                                const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                                const float *__restrict__ f0, const int *__restrict__ xc,
+                                                const int *__restrict__ yc, float *__restrict__ phin_win,
+                                                float *__restrict__ jwin, int *__restrict__ counts,
+                                                int *__restrict__ err, int mode) {
+    __shared__ float phase[ORG_WH * ORG_WH];
+    __shared__ float pnl[ORG_WH * ORG_WH];
+    __shared__ double red[16];
+    cells_body(P, blockIdx.x, phase, pnl, red, phi, rho, hfunc, psi, hlum, hecm, f0, xc, yc, phin_win, nullptr, jwin,
+               counts, err, mode);
+}
+
+/* Buffers of the lumen/ECM loop role of k_cortex_cells. */
+struct LumBufs {
+    float *L0, *L1, *E0, *E1;
+    const float *jlx, *jly, *jex, *jey;
+    const int *jbox;
+    LumState *lst;
+    double *part;
+    unsigned *bar;
+    int *abort_flag;
+    long long limit;
+};
+
+/* k_cortex_cells: one launch for the rest of the step.
+ *   workgroups 0..ncell-1         spectral cortex update of cell n
+ *   workgroups ncell..2*ncell-1   phase-field update + reshaping of cell n
+ *   workgroups 2*ncell..          the lumen and ECM reshaping loops
+ * The three roles read only the start-of-step state (the cell roles read
+ * lum/ECM; the loop role writes only its ping-pong buffers), so they run
+ * concurrently; phi's new window goes to phi_win.  LDS is shared between
+ * the roles. */
+#define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8)
+#define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8)
+#define ORG_LDS_FUSED (ORG_LDS_SPECTRAL > ORG_LDS_CELLS ? ORG_LDS_SPECTRAL : ORG_LDS_CELLS)
+__global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
+                                                       const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                       const float *__restrict__ hfunc, const float *__restrict__ psi,
+                                                       const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                                       const float *__restrict__ f0, const int *__restrict__ xc,
+                                                       const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                       float *__restrict__ rho_box, float *__restrict__ vxbox,
+                                                       float *__restrict__ vybox, int write_v,
+                                                       float *__restrict__ phin_win, float *__restrict__ phi_win,
+                                                       float *__restrict__ jwin, int *__restrict__ counts,
+                                                       int *__restrict__ err, LumBufs lb) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
+    if ((int)blockIdx.x >= 2 * P.nc) {
+        lum_loop_body(P, blockIdx.x - 2 * P.nc, gridDim.x - 2 * P.nc, (double *)lds, lb.L0, lb.L1, lb.E0, lb.E1,
+                      lb.jlx, lb.jly, lb.jex, lb.jey, lb.jbox, lb.lst, lb.part, lb.bar, lb.abort_flag, err, lb.limit);
+    } else if ((int)blockIdx.x < P.nc) {
+        float2 *sm = (float2 *)lds;
+        spectral_body(P, blockIdx.x, sm, sm + ORG_MB * ORG_LS, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox,
+                      vybox, write_v);
+    } else {
+        float *phase = (float *)lds;
+        double *red = (double *)(lds + 2 * ORG_WH * ORG_WH * 4);
+        cells_body(P, blockIdx.x - P.nc, phase, phase + ORG_WH * ORG_WH, red, phi, rho, hfunc, psi, hlum, hecm, f0,
+                   xc, yc, phin_win, phi_win, jwin, counts, err, 3);
+    }
 }
 
 /* k_commit_cells: rho <- rhon (copy_cell_tables_1_by_1,
@@ -1006,12 +1183,23 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
  * values plus the periodic ghost ring applied to rhon
  * (rho_functions.cpp:469). */
 __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict__ rho,
-                                                       const float *__restrict__ rho_box, const int *__restrict__ xc,
+                                                       const float *__restrict__ rho_box, float *__restrict__ phi,
+                                                       const float *__restrict__ phi_win, const int *__restrict__ xc,
                                                        const int *__restrict__ yc) {
     const int n = blockIdx.x, N = P.N;
     constexpr int M = ORG_MB;
     const size_t NN = (size_t)N * N;
     float *rh = rho + n * NN;
+    if (phi_win) {        /* phi window from k_cortex_cells (reshaping_cells' phase) */
+        constexpr int W = ORG_W;
+        const int wx = xc[n] - P.zs, wy = yc[n] - P.zs;
+        const float *pw = phi_win + (size_t)n * W * W;
+        float *ph = phi + n * NN;
+        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+            const int i = e / W;
+            ph[(size_t)(wx + i) * N + (wy + e - i * W)] = pw[e];
+        }
+    }
     const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
     const float *rb = rho_box + (size_t)n * M * M;
     for (int e = threadIdx.x; e < M * M; e += blockDim.x)

@@ -346,6 +346,7 @@ int orc_find_centres_cells(orc_state *st) {
 
 /* compute_hfunc_3d_tables_kernels_loop_n, useful_functions.cpp:233-251 */
 int orc_compute_hfunc_cells(orc_state *st) {
+    if (check_windows(st)) return st->err;
     int zs = st->zs;
     for (int n = 0; n < st->nc; n++)
         for (int i = st->xc[n] - zs; i < st->xc[n] + zs; i++)
@@ -463,6 +464,7 @@ int orc_compute_normal_to_phi(orc_state *st) {
  * n1, n2 are index neighbours on a ring (:298-312); label by strict '>'
  * (:328-353). */
 int orc_create_turnover_rates_tensions(orc_state *st) {
+    if (check_windows(st)) return st->err;
     const org_params *p = &st->p;
     int zs = st->zs, nc = st->nc;
     for (int n = 0; n < nc; n++) {
@@ -502,6 +504,7 @@ int orc_create_turnover_rates_tensions(orc_state *st) {
 
 /* compute_prod_norm_delta, rho_functions.cpp:13-30 */
 int orc_compute_prod_norm_delta(orc_state *st) {
+    if (check_windows(st)) return st->err;
     int zs = st->zs;
     real sigma = (real)st->p.sigma;
     for (int n = 0; n < st->nc; n++)
@@ -659,6 +662,7 @@ static void copy_to_rhon(orc_state *st, int n, const cplx *t) {
 
 /* update_all_cortices, rho_functions.cpp:409-471 */
 int orc_update_all_cortices(orc_state *st) {
+    if (check_windows(st)) return st->err;
     for (int nc = 0; nc < st->nc; nc++) {
         load_temp_fft(st, nc, st->rho, st->temp_fft);                 /* :415 */
         exec_c2c(st, st->temp_fft, st->fft_rho, -1);                   /* :417 */
@@ -827,6 +831,7 @@ int orc_reshaping_lumen_or_ECM(orc_state *st, int which_ecm) {
 /* update_all_cells, cell_phase_functions.cpp:126-217, then
  * boundary_conditions_cells(phin) (run_with_active_gels.cpp:170). */
 int orc_update_all_cells(orc_state *st) {
+    if (check_windows(st)) return st->err;
     const org_params *p = &st->p;
     int N = st->N, zs = st->zs;
     real dx = (real)p->dx, dt = (real)p->dt, tau = (real)p->tau, sigma = (real)p->sigma;
@@ -873,6 +878,7 @@ int orc_update_all_cells(orc_state *st) {
  * when sum |phasen-phase| dx^2 / N_current <= rtol*2*pi*R*wi*rdt.  phase
  * ends at the second-to-last iterate. */
 int orc_reshaping_cells(orc_state *st) {
+    if (check_windows(st)) return st->err;
     const org_params *p = &st->p;
     int N = st->N, zs = st->zs;
     real dx = (real)p->dx, wi = (real)p->wi, rdt = (real)p->rdt, rtol = (real)p->rtol, R = (real)p->R;

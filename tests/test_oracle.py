@@ -200,6 +200,16 @@ def test_centroid_integer_truncation():
     assert (xc[1], yc[1]) == (80, 110)    # (80+81)/2 = 80, (110+111)/2 = 110
 
 
+def test_window_functions_reject_out_of_range_windows():
+    p = grid_units(100, 2, 5.0)
+    o = orc.Oracle(p)
+    o.set_centres([50, 50], [50, 50])          # 130-wide windows do not fit a 100^2 grid
+    for f in ("compute_hfunc_cells", "create_turnover_rates_tensions", "compute_prod_norm_delta",
+              "update_all_cortices", "update_all_cells", "reshaping_cells"):
+        with pytest.raises(RuntimeError):
+            o.call(f)
+
+
 def test_window_touching_edge_is_an_error():
     p = grid_units(200, 2, 5.0)
     o = orc.Oracle(p)
@@ -289,3 +299,145 @@ def test_workload_shapes_and_windows():
     s = hex_sites(7, 40.0, (0.0, 0.0))
     d = np.sort(np.hypot(s[:, 0], s[:, 1]))
     np.testing.assert_allclose(d, [0] + [40.0] * 6, atol=1e-9)
+
+
+def _oracle_blank(N=64, nc=2, R=5.0):
+    p = grid_units(N, nc, R)
+    return p, orc.Oracle(p)
+
+
+def test_h_is_the_cubic_smoothstep():
+    # the cubic with h(0)=0, h(1)=1, h'(0)=h'(1)=0 is unique; check those and
+    # the symmetry h(1-u) = 1-h(u) through compute_all_hfunc_2d_tables_kernels
+    p, o = _oracle_blank(N=16)
+    u = np.linspace(0.0, 1.0, 256).reshape(16, 16)
+    o.set("lum", u); o.set("ecm", 1.0 - u)
+    o.call("compute_hfunc_lum_ecm")
+    h, hm = o.get("hlum"), o.get("hecm")
+    np.testing.assert_allclose(h + hm, 1.0, atol=1e-14)
+    flat = h.ravel()
+    assert flat[0] == 0.0 and flat[-1] == 1.0
+    d = np.diff(flat) / np.diff(u.ravel())
+    assert d[0] < 0.02 and d[-1] < 0.02 and d.max() < 1.51 and (d >= 0).all()
+
+
+def test_grad_norm_and_normals_on_linear_fields():
+    p, o = _oracle_blank(N=40)
+    N = p.N
+    i, j = np.meshgrid(np.arange(N, dtype=float), np.arange(N, dtype=float), indexing="ij")
+    a, b = 0.003, -0.004
+    phi = np.stack([0.5 + a * i + b * j, 0.2 + 0 * i])
+    o.set("phi", phi)
+    o.call("compute_normal_grad_phi")
+    o.call("compute_normal_to_phi")
+    g, jx, jy = o.get("ngp"), o.get("jx"), o.get("jy")
+    inner = (0, slice(1, N - 1), slice(1, N - 1))
+    np.testing.assert_allclose(g[inner], np.hypot(a, b) / p.dx, rtol=1e-10)
+    # unit normal along grad phi where grad != 0, zero where it vanishes
+    np.testing.assert_allclose(jx[inner], a / np.hypot(a, b), rtol=1e-10)
+    np.testing.assert_allclose(jy[inner], b / np.hypot(a, b), rtol=1e-10)
+    assert (jx[1] == 0).all() and (jy[1] == 0).all()
+
+
+def test_prod_norm_delta_is_gaussian_in_phi():
+    p, o = _oracle_blank(N=200)
+    N, s = p.N, p.sigma
+    i = np.arange(N, dtype=float)[:, None] * np.ones((1, N))
+    phi = np.stack([0.5 + 0.01 * (i - 100.0), np.zeros((N, N))])
+    o.set("phi", phi)
+    o.set_centres([100, 100], [100, 100])
+    o.call("compute_normal_grad_phi")
+    o.call("compute_prod_norm_delta")
+    pnd = o.get("pnd")[0]
+    g = 0.01 / p.dx
+    # delta_sigma peaks at 1/(sqrt(2 pi) sigma) and falls by e^-1/2 at phi = 0.5 + sigma
+    assert abs(pnd[100, 100] - g / (np.sqrt(2 * np.pi) * s)) < 1e-12
+    k = 20     # phi = 0.5 + 0.2 = 0.5 + sigma
+    np.testing.assert_allclose(pnd[100 + k, 100] / pnd[100, 100], np.exp(-0.5), rtol=1e-10)
+
+
+def test_update_all_cells_against_continuous_terms():
+    """phin - phi = dt/tau phi(1-phi) [f0 - beta T + eta lap T] - (c/tau) dt delta(phi) grad rho.grad phi/|grad phi|,
+    T = psi - h_n; with linear phi, rho and quadratic T the central differences are exact."""
+    p, o = _oracle_blank(N=200, nc=2, R=5.0)
+    p.eta_lum_cells = p.eta_ECM_cells = 0.0
+    p.beta_lum_cells = p.beta_ECM_cells = 0.0
+    o = orc.Oracle(p)
+    N = p.N
+    i, j = np.meshgrid(np.arange(N, dtype=float), np.arange(N, dtype=float), indexing="ij")
+    a, b, cr = 0.002, 0.001, 0.01
+    phi0 = 0.5 + a * (i - 100) + b * (j - 100)
+    rho0 = cr * (i - 100) - 2 * cr * (j - 100)
+    T = 1e-4 * ((i - 100) ** 2 + 2 * (j - 100) ** 2)           # lap T = 1e-4 * (2 + 4)
+    o.set("phi", np.stack([phi0, np.zeros((N, N))]))
+    o.set("rho", np.stack([rho0, np.zeros((N, N))]))
+    o.set("hfunc", np.zeros((2, N, N)))
+    o.set("psi", T)
+    o.set_centres([100, 100], [100, 100])
+    o.call("compute_normal_grad_phi")
+    # f0 from sum_s with hfunc = 0: f0 = alpha * s0
+    o.call("sum_s")
+    o.call("update_all_cells")
+    phin = o.get("phin")[0]
+    w = (slice(40, 160), slice(40, 160))
+    ph = phi0[w]
+    gp = np.array([a, b]) / p.dx
+    gr = np.array([cr, -2 * cr]) / p.dx
+    delta = np.exp(-(ph - 0.5) ** 2 / (2 * p.sigma ** 2)) / (np.sqrt(2 * np.pi) * p.sigma)
+    coupling = p.coupling_PF_AG / p.tau * p.dt * delta * (gr @ gp) / np.linalg.norm(gp)
+    f0 = p.alpha * p.s0
+    growth = p.dt / p.tau * ph * (1 - ph) * (f0 - p.beta * T[w] + p.eta * 6e-4)
+    np.testing.assert_allclose(phin[w] - ph, growth - coupling, rtol=1e-9, atol=1e-13)
+
+
+def test_cell_reshaping_keeps_second_to_last_iterate():
+    wl = make_workload("two_cells_256")
+    o = orc.Oracle(wl.params); o.load_workload(wl)
+    for f in ("find_centres_cells", "compute_normal_grad_phi", "compute_normal_to_phi"):
+        o.call(f)
+    o.call("reshaping_cells")
+    phi, phin = o.get("phi"), o.get("phin")
+    xc, yc = o.centres()
+    zs, p = wl.params.zone_size, wl.params
+    thr = p.rtol * 2 * np.pi * p.R * p.wi * p.rdt
+    for n in range(2):
+        w = (n, slice(xc[n] - zs, xc[n] + zs), slice(yc[n] - zs, yc[n] + zs))
+        dev = np.abs(phin[w] - phi[w]).sum() * p.dx ** 2 / p.n_current
+        assert dev <= thr            # the sweep that stopped the loop ...
+        assert dev > 0               # ... is not copied back into phi
+
+
+def test_lumen_reshaping_loop_rules():
+    wl = make_workload("two_cells_256")
+    p = wl.params
+    o = orc.Oracle(p); o.load_workload(wl)
+    for f in ("find_centres_cells", "compute_hfunc_cells", "compute_hfunc_lum_ecm", "compute_psi"):
+        o.call(f)
+    o.call("update_lumen_and_ECM")
+    e0 = o.get("ecmn")
+    o.call("reshaping_lumen_or_ECM", 1)
+    e1 = o.get("ecmn")
+    _, _, sweeps = o.iterations()
+    assert sweeps >= 1
+    # divergence form: the interior sum changes only through the boundary ring,
+    # where the normals vanish (ECM = 1 there), so it is conserved
+    inner = (slice(2, p.N - 2), slice(2, p.N - 2))
+    assert abs(e1[inner].sum() - e0[inner].sum()) < 1e-9 * e0[inner].sum()
+    # boundary_conditions_dirichlet: edges get the side value, corners 0
+    assert e1[0, 0] == 0 and e1[0, p.N - 1] == 0 and (e1[0, 1:-1] == 1).all()
+    # a loop whose start deviation is not above its threshold does not run
+    p2 = p.copy(); p2.rdev_ini = p2.rdev_threshold
+    o2 = orc.Oracle(p2); o2.load_workload(wl)
+    for f in ("find_centres_cells", "compute_hfunc_cells", "compute_hfunc_lum_ecm", "compute_psi"):
+        o2.call(f)
+    o2.call("update_lumen_and_ECM")
+    before = o2.get("ecmn")
+    o2.call("reshaping_lumen_or_ECM", 1)
+    assert o2.iterations()[2] == 0 and np.array_equal(o2.get("ecmn"), before)
+
+
+def test_psi_is_sum_of_cell_h():
+    wl = make_workload("hex", N=256, ncell=4)
+    o = orc.Oracle(wl.params); o.load_workload(wl)
+    o.call("find_centres_cells"); o.call("compute_hfunc_cells"); o.call("compute_psi")
+    np.testing.assert_allclose(o.get("psi"), o.get("hfunc").sum(axis=0), rtol=1e-14, atol=1e-15)

@@ -26,6 +26,8 @@ int note_error(int code) {
         }                                                                              \
     } while (0)
 
+#define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
+
 }  // namespace
 
 struct org_ctx {
@@ -50,7 +52,8 @@ struct org_ctx {
     float2 *spec = nullptr;
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
-    int *uni = nullptr, *jbox = nullptr;   /* union of windows; boxes of nonzero lumen/ECM normals */
+    int *uni = nullptr, *jbox = nullptr;   /* union of windows; per-workgroup boxes of nonzero normals */
+    int *ring_done = nullptr;              /* rho ghost ring already written since org_prepare */
     int lum_grid = ORG_LUM_BLOCKS;         /* workgroups per field of a reshaping sweep */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
     float *f0 = nullptr;
@@ -178,6 +181,12 @@ int L_centroid(org_ctx *c) {
                        c->err);
     return launch_check();
 }
+int L_cell_prep(org_ctx *c) {
+    Prof pr(c, ORG_K_CENTRES);
+    hipLaunchKernelGGL(k_cell_prep, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc,
+                       c->box, c->s0, c->s, c->f0, c->uni, c->err);
+    return launch_check();
+}
 int L_prep(org_ctx *c, int mode) {
     Prof pr(c, ORG_K_PREP);
     hipLaunchKernelGGL(k_prep, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc, c->box,
@@ -198,8 +207,8 @@ int L_psi(org_ctx *c) {
 int L_lumecm_update(org_ctx *c) {
     Prof pr(c, ORG_K_LUMECM);
     hipLaunchKernelGGL(k_lumecm_update, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->psi,
-                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->lst, c->jbox,
-                       c->lum_abort, c->lum_bar);
+                       c->hlum, c->hecm, c->sum_ecm, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->lst, c->lum_abort,
+                       c->lum_bar);
     return launch_check();
 }
 int L_lum_normals(org_ctx *c, int fbase, int nf) {
@@ -211,13 +220,14 @@ int L_lum_normals(org_ctx *c, int fbase, int nf) {
 int L_lum_iter(org_ctx *c, int fbase, int nf) {
     Prof pr(c, ORG_K_LUMRESHAPE);
     hipLaunchKernelGGL(k_lum_iter, dim3(c->lum_grid, nf), dim3(1024), 0, c->stream, c->dp, c->bufL0, c->bufL1, c->bufE0,
-                       c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->lst, c->part_l, c->counters, c->err, fbase);
+                       c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->grid_g, c->lst, c->part_l, c->counters,
+                       c->err, fbase);
     return launch_check();
 }
 int L_commit_lumecm(org_ctx *c) {
     Prof pr(c, ORG_K_LUMRESHAPE);
     hipLaunchKernelGGL(k_commit_lumecm, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->bufL0,
-                       c->bufL1, c->bufE0, c->bufE1, c->lst, c->err);
+                       c->bufL1, c->bufE0, c->bufE1, c->lst, c->err, c->uni);
     return launch_check();
 }
 int L_spectral(org_ctx *c) {
@@ -235,14 +245,16 @@ int L_cells(org_ctx *c, int mode) {
 int L_commit_cells(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
     hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
-                       (const float *)nullptr, c->xc, c->yc);
+                       (const float *)nullptr, c->xc, c->yc, (const int *)c->ring_done);
+    TRY(launch_check());
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->ring_done, 1);
     return launch_check();
 }
 /* spectral + cells + lumen/ECM loops in one launch; phi's new window goes
  * to phi_win */
 int L_cortex_cells(org_ctx *c) {
     Prof pr(c, ORG_K_SPECTRAL);
-    LumBufs lb{c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox,
+    LumBufs lb{c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->grid_g,
                c->lst, c->part_lc, c->lum_bar, c->lum_abort, c->wait_limit};
     hipLaunchKernelGGL(k_cortex_cells, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, c->phi,
                        c->rho, c->lum, c->ecm, c->hfunc, c->psi, c->hlum, c->hecm, c->f0, c->xc, c->yc, c->spec,
@@ -253,11 +265,11 @@ int L_cortex_cells(org_ctx *c) {
 int L_commit_cells_win(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
     hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
-                       (const float *)c->phi_win, c->xc, c->yc);
+                       (const float *)c->phi_win, c->xc, c->yc, (const int *)c->ring_done);
+    TRY(launch_check());
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->ring_done, 1);
     return launch_check();
 }
-#define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
-
 /* Host-driven lumen/ECM loop: run sweeps until both stop tests passed,
  * checking after every `chunk` sweeps (the reference reads rdev back every
  * sweep, cell_phase_functions.cpp:848-869). */
@@ -280,8 +292,7 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
  * and the lumen/ECM reshaping loops, then the copy-back.  No host round
  * trip: the whole step can be captured in one hipGraph. */
 int enqueue_step(org_ctx *c) {
-    TRY(L_centroid(c));             /* :123 */
-    TRY(L_prep(c, 3));              /* :133, :142 */
+    TRY(L_cell_prep(c));            /* :123, :133, :142 */
     TRY(L_hlum(c));                 /* :134 */
     TRY(L_psi(c));                  /* :139 */
     TRY(L_lumecm_update(c));        /* :157-158 */
@@ -333,7 +344,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
     c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
     c->grid_g = std::min(c->N, 128);
-    int nt = (c->N + 31) / 32;
+    int nt = (c->N + 15) / 16;
     c->ntiles = nt * nt;
     int rc = ORG_OK;
     auto A = [&](auto **ptr, size_t n) { if (!rc) rc = dalloc(c, ptr, n); };
@@ -347,7 +358,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 8); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->ring_done, 1); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
@@ -416,7 +427,7 @@ int org_set_field(org_ctx *c, int field, const float *host) {
     if (!d || !host) return note_error(ORG_EINVAL);
     HIPCHK(hipMemcpyAsync(d, host, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (field == ORG_F_PHI || field == ORG_F_HFUNC) c->prepared = false;
+    if (field == ORG_F_PHI || field == ORG_F_HFUNC || field == ORG_F_RHO) c->prepared = false;
     return ORG_OK;
 }
 
@@ -491,6 +502,12 @@ int org_prepare(org_ctx *c) {
     hipLaunchKernelGGL(k_init_box, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->box);
     TRY(launch_check());
     HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipMemsetAsync(c->ring_done, 0, sizeof(int), c->stream));
+    {
+        const int u[4] = {c->N, 0, c->N, 0};
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->uni, u, sizeof(u), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->prepared = true;
     return ORG_OK;

@@ -220,10 +220,13 @@ static __device__ __forceinline__ void dft_reg(float2 *v) {
 /* One batch of 128-point DFTs over the lines of a 128 x 128 complex array in
  * LDS, 8 threads per line, 1024 threads (128 = 16 x 8):
  *   X[k2 + 16 k1] = sum_j1 W8^(j1 k1) W128^(j1 k2) sum_j2 x[j1 + 8 j2] W16^(j2 k2)
- * es = element stride, ls = line stride (float2 units). */
+ * es = element stride, ls = line stride (float2 units).  Consecutive lanes
+ * take consecutive lines with the same sub-index: along columns that is
+ * contiguous, along rows it strides by the odd row pitch, so neither pass
+ * has LDS bank conflicts, and the twiddle index is uniform in a wave. */
 template <int S>
 static __device__ __forceinline__ void fft128_lines(float2 *sm, const float2 *tw, int es, int ls) {
-    const int t = threadIdx.x, line = t >> 3, sub = t & 7;
+    const int t = threadIdx.x, line = t & 127, sub = t >> 7;
     float2 *base = sm + line * ls;
     float2 v[16];
 #pragma unroll
@@ -411,6 +414,81 @@ __global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__
     }
 }
 
+/* k_cell_prep: k_centroid and k_prep (mode 3) of one cell in one workgroup
+ * (batched form).  The centre and support box pass through LDS; the union
+ * box is reset by k_commit_lumecm at the end of the previous step. */
+__global__ __launch_bounds__(1024) void k_cell_prep(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
+                                                    int *__restrict__ xc, int *__restrict__ yc, int *__restrict__ box,
+                                                    const double *__restrict__ s0, double *__restrict__ s,
+                                                    float *__restrict__ f0, int *__restrict__ uni,
+                                                    int *__restrict__ err) {
+    __shared__ long long red[3][16];
+    __shared__ double dred[16];
+    __shared__ int sh[6];
+    const int n = blockIdx.x, N = P.N;
+    constexpr int W = ORG_W;
+    const size_t NN = (size_t)N * N;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const float *ph = phi + n * NN;
+    float *hf = hfunc + n * NN;
+    /* centroid over the support box (useful_functions.cpp:193-227) */
+    {
+        const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
+        long long sx = 0, sy = 0, c = 0;
+        for (int i = i0 + w; i < i1; i += nw) {
+            const float *row = ph + (size_t)i * N;
+            for (int j = j0 + lane; j < j1; j += 64)
+                if (row[j] > 0.5f) { sx += i; sy += j; c += 1; }
+        }
+        sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
+        if (lane == 0) { red[0][w] = sx; red[1][w] = sy; red[2][w] = c; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long X = 0, Y = 0, C = 0;
+            for (int k = 0; k < nw; k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
+            int x = xc[n], y = yc[n];
+            if (C == 0) set_err(err, ORG_EEMPTY_CELL);
+            else { x = (int)(X / C); y = (int)(Y / C); }
+            int lo = P.zs + 2, hi = N - 2 - P.zs;
+            if (x < lo || x > hi || y < lo || y > hi) {
+                set_err(err, ORG_EWINDOW);
+                x = min(max(x, lo), hi);
+                y = min(max(y, lo), hi);
+            }
+            xc[n] = x; yc[n] = y;
+            int b0, b1, b2, b3;
+            if (i1 <= i0 || j1 <= j0) { b0 = x - P.zs; b1 = x + P.zs; b2 = y - P.zs; b3 = y + P.zs; }
+            else { b0 = min(i0, x - P.zs); b1 = max(i1, x + P.zs); b2 = min(j0, y - P.zs); b3 = max(j1, y + P.zs); }
+            box[4 * n + 0] = b0; box[4 * n + 1] = b1; box[4 * n + 2] = b2; box[4 * n + 3] = b3;
+            sh[0] = x - P.zs; sh[1] = y - P.zs; sh[2] = b0; sh[3] = b1; sh[4] = b2; sh[5] = b3;
+            atomicMin(&uni[0], x - P.zs); atomicMax(&uni[1], x - P.zs + W);
+            atomicMin(&uni[2], y - P.zs); atomicMax(&uni[3], y - P.zs + W);
+        }
+        __syncthreads();
+    }
+    const int x0 = sh[0], y0 = sh[1], i0 = sh[2], i1 = sh[3], j0 = sh[4], j1 = sh[5];
+    /* hfunc on the window (useful_functions.cpp:233-251) */
+    for (int i = w; i < W; i += nw) {
+        const size_t g = (size_t)(x0 + i) * N + y0;
+        for (int j = lane; j < W; j += 64) hf[g + j] = hfun(ph[g + j]);
+    }
+    /* area over the support box, FP64 (useful_functions.cpp:168-190) */
+    double acc = 0.0;
+    for (int i = i0 + w; i < i1; i += nw) {
+        const bool rowin = i >= x0 && i < x0 + W;
+        const float *pr = ph + (size_t)i * N, *hr = hf + (size_t)i * N;
+        for (int j = j0 + lane; j < j1; j += 64) {
+            float h = (rowin && j >= y0 && j < y0 + W) ? hfun(pr[j]) : hr[j];
+            acc += (double)(h * P.dx * P.dx);
+        }
+    }
+    double tot = block_sum_d(acc, dred);
+    if (threadIdx.x == 0) {
+        s[n] = tot;
+        f0[n] = P.alpha * ((float)s0[n] - (float)tot);
+    }
+}
+
 /* k_hlum: compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285)
  * and the FP64 sum_ecm of update_lumen_and_ECM (cell_phase_functions.cpp:376-386),
  * reduced by the last block to finish (fixed order: deterministic). */
@@ -457,10 +535,10 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
     __shared__ int list[1024];
     __shared__ int4 lbox[1024];
     __shared__ int cnt;
-    const int N = P.N, ntx = (N + 31) / 32;
-    const int ti0 = (blockIdx.x / ntx) * 32, tj0 = (blockIdx.x % ntx) * 32;
+    const int N = P.N, ntx = (N + 15) / 16;
+    const int ti0 = (blockIdx.x / ntx) * 16, tj0 = (blockIdx.x % ntx) * 16;
     const int u0 = uni[0], u1 = uni[1], v0 = uni[2], v1 = uni[3];
-    if (ti0 >= u1 || ti0 + 32 <= u0 || tj0 >= v1 || tj0 + 32 <= v0) return;
+    if (ti0 >= u1 || ti0 + 16 <= u0 || tj0 >= v1 || tj0 + 16 <= v0) return;
     /* ordered list of cells whose support box meets the tile (wave 0) */
     if (threadIdx.x < 64) {
         int base = 0;
@@ -469,7 +547,7 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
             bool hit = false;
             if (n < P.nc) {
                 const int *bx = box + 4 * n;
-                hit = !(ti0 >= bx[1] || ti0 + 32 <= bx[0] || tj0 >= bx[3] || tj0 + 32 <= bx[2]);
+                hit = !(ti0 >= bx[1] || ti0 + 16 <= bx[0] || tj0 >= bx[3] || tj0 + 16 <= bx[2]);
             }
             unsigned long long m = __ballot(hit);
             if (hit) {
@@ -486,9 +564,10 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
     }
     __syncthreads();
     const size_t NN = (size_t)N * N;
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        int i = ti0 + (e >> 5), j = tj0 + (e & 31);
-        if (i >= N || j >= N || i < u0 || i >= u1 || j < v0 || j >= v1) continue;
+    {
+        const int e = threadIdx.x;
+        const int i = ti0 + (e >> 4), j = tj0 + (e & 15);
+        if (i >= N || j >= N || i < u0 || i >= u1 || j < v0 || j >= v1) return;
         float acc = 0.0f;
         for (int k = 0; k < cnt; k++) {
             const int4 bx = lbox[k];
@@ -517,8 +596,7 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
                                                        const double *__restrict__ sum_ecm, float *__restrict__ lumA,
                                                        float *__restrict__ lumB, float *__restrict__ ecmA,
                                                        float *__restrict__ ecmB, LumState *__restrict__ lst,
-                                                       int *__restrict__ jbox, int *__restrict__ abort_flag,
-                                                       unsigned *__restrict__ bar) {
+                                                       int *__restrict__ abort_flag, unsigned *__restrict__ bar) {
     const int N = P.N;
     if (blockIdx.x == 0 && threadIdx.x == 0) { *abort_flag = 0; bar[0] = 0u; }
     if (blockIdx.x == 0 && threadIdx.x < 2) {
@@ -526,8 +604,6 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
         lst[threadIdx.x].done = run ? 0 : 1;
         lst[threadIdx.x].ri = 0;
         lst[threadIdx.x].cur = 0;
-        int *b = jbox + 4 * threadIdx.x;
-        b[0] = N; b[1] = -1; b[2] = N; b[3] = -1;
     }
     const float se = (float)(*sum_ecm);
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
@@ -568,7 +644,8 @@ __global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__res
                                                      const float *__restrict__ ecm_old, const float *__restrict__ lumn,
                                                      const float *__restrict__ ecmn, float *__restrict__ jlx,
                                                      float *__restrict__ jly, float *__restrict__ jex,
-                                                     float *__restrict__ jey, int *__restrict__ jbox, int fbase) {
+                                                     float *__restrict__ jey, int *__restrict__ jbox_part,
+                                                     int fbase) {
     __shared__ int sb[4];
     const int N = P.N, f = fbase + blockIdx.y;
     const float *old = f ? ecm_old : lum_old;
@@ -596,10 +673,25 @@ __global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__res
     }
     if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
     __syncthreads();
-    if (threadIdx.x == 0 && sb[1] >= 0) {
-        int *gb = jbox + 4 * f;
-        atomicMin(&gb[0], sb[0]); atomicMax(&gb[1], sb[1]); atomicMin(&gb[2], sb[2]); atomicMax(&gb[3], sb[3]);
+    if (threadIdx.x < 4) jbox_part[((size_t)f * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = sb[threadIdx.x];
+}
+
+/* Bounding box (inclusive) of field f's nonzero normals from the
+ * per-workgroup boxes written by k_lum_normals; valid in every thread. */
+static __device__ __forceinline__ int4 reduce_jbox(const int *__restrict__ part, int nparts, int f, int N) {
+    __shared__ int sb[4];
+    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+    __syncthreads();
+    int b0 = N, b1 = -1, b2 = N, b3 = -1;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        const int *q = part + ((size_t)f * nparts + k) * 4;
+        if (q[1] >= 0) { b0 = min(b0, q[0]); b1 = max(b1, q[1]); b2 = min(b2, q[2]); b3 = max(b3, q[3]); }
     }
+    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+    __syncthreads();
+    int4 r = make_int4(sb[0], sb[1], sb[2], sb[3]);
+    __syncthreads();
+    return r;
 }
 
 /* Conservative interface flux (cell_phase_functions.cpp:511-547 and
@@ -646,14 +738,14 @@ static __device__ __forceinline__ float reshape_flux(float un, const float *u, i
 #define ORG_LUM_BLOCKS 64
 static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *__restrict__ src,
                                                    float *__restrict__ dst, const float *__restrict__ jx,
-                                                   const float *__restrict__ jy, const int *__restrict__ jb,
+                                                   const float *__restrict__ jy, const int4 jb,
                                                    float side, int blk, int nblk) {
     const int N = P.N;
     const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
     const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
     const float dx2 = P.dx * P.dx;
-    const int r0 = max(jb[0] - 1, 1), r1 = min(jb[1] + 1, N - 2);
-    const int c0 = max(jb[2] - 1, 1), c1 = min(jb[3] + 1, N - 2);
+    const int r0 = max(jb.x - 1, 1), r1 = min(jb.y + 1, N - 2);
+    const int c0 = max(jb.z - 1, 1), c1 = min(jb.w + 1, N - 2);
     const int bw = c1 - c0 + 1, bh = r1 - r0 + 1;
     const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
     const int stride = nblk * 1024;
@@ -707,19 +799,19 @@ __global__ __launch_bounds__(1024) void k_lum_iter(DevP P, float *__restrict__ b
                                                    float *__restrict__ bufE0, float *__restrict__ bufE1,
                                                    const float *__restrict__ jlx, const float *__restrict__ jly,
                                                    const float *__restrict__ jex, const float *__restrict__ jey,
-                                                   const int *__restrict__ jbox, LumState *__restrict__ lst,
-                                                   double *__restrict__ part, unsigned *__restrict__ counter,
-                                                   int *__restrict__ err, int fbase) {
+                                                   const int *__restrict__ jbox_part, int nparts,
+                                                   LumState *__restrict__ lst, double *__restrict__ part,
+                                                   unsigned *__restrict__ counter, int *__restrict__ err, int fbase) {
     __shared__ double red[16];
     __shared__ bool last;
     const int f = fbase + blockIdx.y;
     const LumState s = lst[f];
     if (s.done) return;
+    const int4 jb = reduce_jbox(jbox_part, nparts, f, P.N);
     float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
     const float *src = (s.ri & 1) ? b1 : b0;
     float *dst = (s.ri & 1) ? b0 : b1;
-    double acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jbox + 4 * f, f ? 1.0f : 0.0f, blockIdx.x,
-                           gridDim.x);
+    double acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jb, f ? 1.0f : 0.0f, blockIdx.x, gridDim.x);
     double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
         part[f * gridDim.x + blockIdx.x] = tot;
@@ -783,9 +875,11 @@ static __device__ bool lum_barrier(unsigned *bar, int nblk, int *abort_flag, int
  * stop decision after the same sweep.  part: [2 parities][2 fields][nblk]. */
 static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *red, float *bufL0, float *bufL1,
                                      float *bufE0, float *bufE1, const float *jlx, const float *jly,
-                                     const float *jex, const float *jey, const int *jbox, LumState *lst,
-                                     double *part, unsigned *bar, int *abort_flag, int *err, long long limit) {
+                                     const float *jex, const float *jey, const int *jbox_part, int nparts,
+                                     LumState *lst, double *part, unsigned *bar, int *abort_flag, int *err,
+                                     long long limit) {
     LumState s[2] = {lst[0], lst[1]};
+    const int4 jb[2] = {reduce_jbox(jbox_part, nparts, 0, P.N), reduce_jbox(jbox_part, nparts, 1, P.N)};
     for (int k = 0; !(s[0].done && s[1].done); k++) {
         double *pk = part + (size_t)(k & 1) * 2 * nblk;
         for (int f = 0; f < 2; f++) {
@@ -794,7 +888,7 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
                 float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
                 const float *src = (s[f].ri & 1) ? b1 : b0;
                 float *dst = (s[f].ri & 1) ? b0 : b1;
-                acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jbox + 4 * f, f ? 1.0f : 0.0f, blk, nblk);
+                acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jb[f], f ? 1.0f : 0.0f, blk, nblk);
             }
             double tot = block_sum_d(acc, red);
             if (threadIdx.x == 0) pk[f * nblk + blk] = tot;
@@ -814,9 +908,11 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
 __global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
                                                        const float *__restrict__ bufL0, const float *__restrict__ bufL1,
                                                        const float *__restrict__ bufE0, const float *__restrict__ bufE1,
-                                                       const LumState *__restrict__ lst, int *__restrict__ err) {
+                                                       const LumState *__restrict__ lst, int *__restrict__ err,
+                                                       int *__restrict__ uni) {
     const int N = P.N;
     const LumState sl = lst[0], se = lst[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { uni[0] = N; uni[1] = 0; uni[2] = N; uni[3] = 0; }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (!sl.done || !se.done)) set_err(err, ORG_ERELAX);
     const float *L = sl.cur ? bufL1 : bufL0;
     const float *E = se.cur ? bufE1 : bufE0;
@@ -1132,7 +1228,8 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
 struct LumBufs {
     float *L0, *L1, *E0, *E1;
     const float *jlx, *jly, *jex, *jey;
-    const int *jbox;
+    const int *jbox_part;
+    int nparts;
     LumState *lst;
     double *part;
     unsigned *bar;
@@ -1165,7 +1262,8 @@ __global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
     if ((int)blockIdx.x >= 2 * P.nc) {
         lum_loop_body(P, blockIdx.x - 2 * P.nc, gridDim.x - 2 * P.nc, (double *)lds, lb.L0, lb.L1, lb.E0, lb.E1,
-                      lb.jlx, lb.jly, lb.jex, lb.jey, lb.jbox, lb.lst, lb.part, lb.bar, lb.abort_flag, err, lb.limit);
+                      lb.jlx, lb.jly, lb.jex, lb.jey, lb.jbox_part, lb.nparts, lb.lst, lb.part, lb.bar, lb.abort_flag,
+                      err, lb.limit);
     } else if ((int)blockIdx.x < P.nc) {
         float2 *sm = (float2 *)lds;
         spectral_body(P, blockIdx.x, sm, sm + ORG_MB * ORG_LS, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox,
@@ -1185,7 +1283,7 @@ __global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict
 __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict__ rho,
                                                        const float *__restrict__ rho_box, float *__restrict__ phi,
                                                        const float *__restrict__ phi_win, const int *__restrict__ xc,
-                                                       const int *__restrict__ yc) {
+                                                       const int *__restrict__ yc, const int *__restrict__ ring_done) {
     const int n = blockIdx.x, N = P.N;
     constexpr int M = ORG_MB;
     const size_t NN = (size_t)N * N;
@@ -1204,7 +1302,12 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
     const float *rb = rho_box + (size_t)n * M * M;
     for (int e = threadIdx.x; e < M * M; e += blockDim.x)
         rh[(size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127))] = rb[e];
-    /* ghost ring: sources are rows/cols 1 and N-2, outside every box */
+    /* ghost ring: its sources, rows/cols 1 and N-2, lie outside every FFT box
+     * (windows keep one pixel inside rows/cols 1..N-2) and are never written,
+     * so the ring is the same after every step: it is written on the first
+     * commit after org_prepare only (ring_done, set by the host-side flag
+     * kernel after this launch). */
+    if (*ring_done) return;
     for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
         rh[(size_t)i * N] = rh[(size_t)i * N + N - 2];
         rh[(size_t)i * N + N - 1] = rh[(size_t)i * N + 1];
@@ -1217,6 +1320,10 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
         rh[(size_t)N * (N - 1)] = rh[(size_t)N + N - 2];
         rh[(size_t)N * (N - 1) + N - 1] = rh[(size_t)N + 1];
     }
+}
+
+__global__ void k_set_flag(int *flag, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
 /* Test hook: one 128 x 128 2D DFT through the same LDS code path. */

@@ -53,7 +53,7 @@ struct org_ctx {
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
     int *uni = nullptr, *jbox = nullptr;   /* union of windows; per-workgroup boxes of nonzero normals */
-    int *ring_done = nullptr;              /* rho ghost ring already written since org_prepare */
+    bool ring_pending = true;              /* rho ghost ring not yet written since org_prepare */
     int lum_grid = ORG_LUM_BLOCKS;         /* workgroups per field of a reshaping sweep */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
     float *f0 = nullptr;
@@ -181,12 +181,6 @@ int L_centroid(org_ctx *c) {
                        c->err);
     return launch_check();
 }
-int L_cell_prep(org_ctx *c) {
-    Prof pr(c, ORG_K_CENTRES);
-    hipLaunchKernelGGL(k_cell_prep, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc,
-                       c->box, c->s0, c->s, c->f0, c->uni, c->err);
-    return launch_check();
-}
 int L_prep(org_ctx *c, int mode) {
     Prof pr(c, ORG_K_PREP);
     hipLaunchKernelGGL(k_prep, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc, c->yc, c->box,
@@ -245,9 +239,7 @@ int L_cells(org_ctx *c, int mode) {
 int L_commit_cells(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
     hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
-                       (const float *)nullptr, c->xc, c->yc, (const int *)c->ring_done);
-    TRY(launch_check());
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->ring_done, 1);
+                       (const float *)nullptr, c->xc, c->yc);
     return launch_check();
 }
 /* spectral + cells + lumen/ECM loops in one launch; phi's new window goes
@@ -262,13 +254,27 @@ int L_cortex_cells(org_ctx *c) {
                        c->err, lb);
     return launch_check();
 }
-int L_commit_cells_win(org_ctx *c) {
-    Prof pr(c, ORG_K_COMMIT);
-    hipLaunchKernelGGL(k_commit_cells, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
-                       (const float *)c->phi_win, c->xc, c->yc, (const int *)c->ring_done);
-    TRY(launch_check());
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->ring_done, 1);
+int L_pre(org_ctx *c) {
+    Prof pr(c, ORG_K_CENTRES);
+    hipLaunchKernelGGL(k_pre, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc,
+                       c->yc, c->box, c->s0, c->s, c->f0, c->uni, c->err, c->lum, c->ecm, c->hlum, c->hecm, c->part_h,
+                       c->counters + 2, c->sum_ecm);
     return launch_check();
+}
+int L_post(org_ctx *c) {
+    Prof pr(c, ORG_K_COMMIT);
+    hipLaunchKernelGGL(k_post, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
+                       (const float *)c->phi_win, c->xc, c->yc, c->lum, c->ecm, c->bufL0, c->bufL1, c->bufE0,
+                       c->bufE1, c->lst, c->err, c->uni);
+    return launch_check();
+}
+/* the one-time rho ghost ring (see k_ring_rho) */
+int L_ring_if_pending(org_ctx *c) {
+    if (!c->ring_pending) return ORG_OK;
+    hipLaunchKernelGGL(k_ring_rho, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->rho);
+    TRY(launch_check());
+    c->ring_pending = false;
+    return ORG_OK;
 }
 /* Host-driven lumen/ECM loop: run sweeps until both stop tests passed,
  * checking after every `chunk` sweeps (the reference reads rdev back every
@@ -292,14 +298,12 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
  * and the lumen/ECM reshaping loops, then the copy-back.  No host round
  * trip: the whole step can be captured in one hipGraph. */
 int enqueue_step(org_ctx *c) {
-    TRY(L_cell_prep(c));            /* :123, :133, :142 */
-    TRY(L_hlum(c));                 /* :134 */
+    TRY(L_pre(c));                  /* :123, :133-134, :142 */
     TRY(L_psi(c));                  /* :139 */
     TRY(L_lumecm_update(c));        /* :157-158 */
     TRY(L_lum_normals(c, 0, 2));    /* :160-161 */
     TRY(L_cortex_cells(c));         /* :130, :137, :154, :160-172 */
-    TRY(L_commit_cells_win(c));     /* :172 phi window, :175-178 */
-    TRY(L_commit_lumecm(c));        /* :181-182 */
+    TRY(L_post(c));                 /* :172 phi window, :175-182 */
     return ORG_OK;
 }
 
@@ -358,7 +362,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->ring_done, 1); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
@@ -502,7 +506,6 @@ int org_prepare(org_ctx *c) {
     hipLaunchKernelGGL(k_init_box, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->box);
     TRY(launch_check());
     HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
-    HIPCHK(hipMemsetAsync(c->ring_done, 0, sizeof(int), c->stream));
     {
         const int u[4] = {c->N, 0, c->N, 0};
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -510,6 +513,7 @@ int org_prepare(org_ctx *c) {
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->prepared = true;
+    c->ring_pending = true;
     return ORG_OK;
 }
 
@@ -545,10 +549,15 @@ int org_reshaping_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_cells(c, 2));
 int org_commit_step(org_ctx *c) {
     TRY(need_prepared(c));
     TRY(L_commit_cells(c));
+    TRY(L_ring_if_pending(c));
     TRY(L_commit_lumecm(c));
     return harvest_device_error(c);
 }
-int org_commit_rho(org_ctx *c) { TRY(need_prepared(c)); return L_commit_cells(c); }
+int org_commit_rho(org_ctx *c) {
+    TRY(need_prepared(c));
+    TRY(L_commit_cells(c));
+    return L_ring_if_pending(c);
+}
 int org_commit_lumecm(org_ctx *c) { TRY(need_prepared(c)); TRY(L_commit_lumecm(c)); return harvest_device_error(c); }
 
 /* ---- batched form ------------------------------------------------------- */
@@ -556,6 +565,11 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     (void)it0;
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
+    if (nsteps > 0 && c->ring_pending) {   /* first step after org_prepare: eager, plus the ghost ring */
+        TRY(enqueue_step(c));
+        TRY(L_ring_if_pending(c));
+        nsteps--;
+    }
     if (!use_graph) {
         for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c));
         return harvest_device_error(c);

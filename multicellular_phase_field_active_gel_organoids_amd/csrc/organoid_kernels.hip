@@ -417,15 +417,16 @@ __global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__
 /* k_cell_prep: k_centroid and k_prep (mode 3) of one cell in one workgroup
  * (batched form).  The centre and support box pass through LDS; the union
  * box is reset by k_commit_lumecm at the end of the previous step. */
-__global__ __launch_bounds__(1024) void k_cell_prep(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
-                                                    int *__restrict__ xc, int *__restrict__ yc, int *__restrict__ box,
-                                                    const double *__restrict__ s0, double *__restrict__ s,
-                                                    float *__restrict__ f0, int *__restrict__ uni,
-                                                    int *__restrict__ err) {
+static __device__ __forceinline__ void cell_prep_body(const DevP &P, const int n, const float *__restrict__ phi,
+                                                      float *__restrict__ hfunc, int *__restrict__ xc,
+                                                      int *__restrict__ yc, int *__restrict__ box,
+                                                      const double *__restrict__ s0, double *__restrict__ s,
+                                                      float *__restrict__ f0, int *__restrict__ uni,
+                                                      int *__restrict__ err) {
     __shared__ long long red[3][16];
     __shared__ double dred[16];
     __shared__ int sh[6];
-    const int n = blockIdx.x, N = P.N;
+    const int N = P.N;
     constexpr int W = ORG_W;
     const size_t NN = (size_t)N * N;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -492,15 +493,17 @@ __global__ __launch_bounds__(1024) void k_cell_prep(DevP P, const float *__restr
 /* k_hlum: compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285)
  * and the FP64 sum_ecm of update_lumen_and_ECM (cell_phase_functions.cpp:376-386),
  * reduced by the last block to finish (fixed order: deterministic). */
-__global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
-                                              float *__restrict__ hlum, float *__restrict__ hecm,
-                                              double *__restrict__ part, unsigned *__restrict__ counter,
-                                              double *__restrict__ sum_out) {
+static __device__ __forceinline__ void hlum_body(const DevP &P, const int blk, const int nblk,
+                                                 const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                 float *__restrict__ hlum, float *__restrict__ hecm,
+                                                 double *__restrict__ part, unsigned *__restrict__ counter,
+                                                 double *__restrict__ sum_out) {
     __shared__ double red[16];
     __shared__ bool last;
     const int N = P.N;
     double acc = 0.0;
-    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+#pragma unroll 4
+    for (int i = blk; i < N; i += nblk) {
         const size_t g = (size_t)i * N;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
             float a = lum[g + j], b = ecm[g + j];
@@ -512,20 +515,44 @@ __global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__
     }
     double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = tot;
+        part[blk] = tot;
         __threadfence();
         unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
+        last = (prev == (unsigned)nblk - 1);
     }
     __syncthreads();
     if (last) {
         __threadfence();
         double v = 0.0;
-        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+        for (int k = threadIdx.x; k < nblk; k += blockDim.x)
             v += __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double t2 = block_sum_d(v, red);
         if (threadIdx.x == 0) { *sum_out = t2; *counter = 0u; }
     }
+}
+
+__global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
+                                               float *__restrict__ hlum, float *__restrict__ hecm,
+                                               double *__restrict__ part, unsigned *__restrict__ counter,
+                                               double *__restrict__ sum_out) {
+    hlum_body(P, blockIdx.x, gridDim.x, lum, ecm, hlum, hecm, part, counter, sum_out);
+}
+
+/* k_pre: centroid + prep of every cell (workgroups 0..ncell-1) and
+ * h(lum), h(ECM), sum h(ECM) (the other nblk workgroups) in one launch; the
+ * two roles are independent. */
+__global__ __launch_bounds__(1024) void k_pre(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
+                                              int *__restrict__ xc, int *__restrict__ yc, int *__restrict__ box,
+                                              const double *__restrict__ s0, double *__restrict__ s,
+                                              float *__restrict__ f0, int *__restrict__ uni, int *__restrict__ err,
+                                              const float *__restrict__ lum, const float *__restrict__ ecm,
+                                              float *__restrict__ hlum, float *__restrict__ hecm,
+                                              double *__restrict__ part, unsigned *__restrict__ counter,
+                                              double *__restrict__ sum_out) {
+    if ((int)blockIdx.x < P.nc)
+        cell_prep_body(P, blockIdx.x, phi, hfunc, xc, yc, box, s0, s, f0, uni, err);
+    else
+        hlum_body(P, blockIdx.x - P.nc, gridDim.x - P.nc, lum, ecm, hlum, hecm, part, counter, sum_out);
 }
 
 /* k_psi: compute_psi_kernels (useful_functions.cpp:305-322) on 32 x 32 tiles
@@ -608,6 +635,7 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
     const float se = (float)(*sum_ecm);
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
+#pragma unroll 2
     for (int i = blockIdx.x; i < N; i += gridDim.x) {
         const size_t r = (size_t)i * N;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
@@ -655,6 +683,7 @@ __global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__res
     if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
     __syncthreads();
     int b0 = N, b1 = -1, b2 = N, b3 = -1;
+#pragma unroll 2
     for (int i = blockIdx.x; i < N; i += gridDim.x) {
         const int si = ring_src(i, N);
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
@@ -905,24 +934,36 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
 
 /* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
  * run_with_active_gels.cpp:181-182) from the buffer holding the last iterate. */
-__global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
-                                                       const float *__restrict__ bufL0, const float *__restrict__ bufL1,
-                                                       const float *__restrict__ bufE0, const float *__restrict__ bufE1,
-                                                       const LumState *__restrict__ lst, int *__restrict__ err,
-                                                       int *__restrict__ uni) {
+static __device__ __forceinline__ void commit_lumecm_body(const DevP &P, const int blk, const int nblk,
+                                                          float *__restrict__ lum, float *__restrict__ ecm,
+                                                          const float *__restrict__ bufL0,
+                                                          const float *__restrict__ bufL1,
+                                                          const float *__restrict__ bufE0,
+                                                          const float *__restrict__ bufE1,
+                                                          const LumState *__restrict__ lst, int *__restrict__ err,
+                                                          int *__restrict__ uni) {
     const int N = P.N;
     const LumState sl = lst[0], se = lst[1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) { uni[0] = N; uni[1] = 0; uni[2] = N; uni[3] = 0; }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (!sl.done || !se.done)) set_err(err, ORG_ERELAX);
+    if (blk == 0 && threadIdx.x == 0) { uni[0] = N; uni[1] = 0; uni[2] = N; uni[3] = 0; }
+    if (blk == 0 && threadIdx.x == 0 && (!sl.done || !se.done)) set_err(err, ORG_ERELAX);
     const float *L = sl.cur ? bufL1 : bufL0;
     const float *E = se.cur ? bufE1 : bufE0;
-    for (int i = blockIdx.x; i < N; i += gridDim.x) {
+#pragma unroll 4
+    for (int i = blk; i < N; i += nblk) {
         const size_t r = (size_t)i * N;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
             lum[r + j] = L[r + j];
             ecm[r + j] = E[r + j];
         }
     }
+}
+
+__global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restrict__ lum, float *__restrict__ ecm,
+                                                        const float *__restrict__ bufL0, const float *__restrict__ bufL1,
+                                                        const float *__restrict__ bufE0, const float *__restrict__ bufE1,
+                                                        const LumState *__restrict__ lst, int *__restrict__ err,
+                                                        int *__restrict__ uni) {
+    commit_lumecm_body(P, blockIdx.x, gridDim.x, lum, ecm, bufL0, bufL1, bufE0, bufE1, lst, err, uni);
 }
 
 /* ======================================================================== */
@@ -941,7 +982,8 @@ __global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restric
  * and F^-1(H D).  The per-mode solve uses the viscosity of the real-space
  * pixel with the mode's indices (rho_functions.cpp:149-150, 170). */
 static __device__ __forceinline__ void spectral_body(const DevP &P, const int n, float2 *__restrict__ sm,
-                                                     float2 *__restrict__ tw, const float *__restrict__ phi,
+                                                     float2 *__restrict__ tw, unsigned char *__restrict__ lab,
+                                                     const float *__restrict__ phi,
                                                      const float *__restrict__ rho, const float *__restrict__ lum,
                                                      const float *__restrict__ ecm, const int *__restrict__ xc,
                                                      const int *__restrict__ yc, float2 *__restrict__ scratch,
@@ -949,12 +991,14 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
                                                      float *__restrict__ vybox, int write_v) {
     const int t = threadIdx.x, N = P.N;
     constexpr int MM = ORG_MB * ORG_MB;
+    constexpr int NPAIR = 63 * ORG_MB + 2 * 65;      /* mode pairs {k, -k} incl. 4 self-paired modes */
     const size_t NN = (size_t)N * N;
     const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;   /* FFT box origin, useful_functions.cpp:618-619 */
     const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
     if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
     const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
-    float2 *SA = scratch + (size_t)n * 2 * MM, *SB = SA + MM;
+    float2 *SA = scratch + (size_t)n * 2 * MM;
+    float *SB = (float *)(SA + MM);
     float *rbox = rho_box + (size_t)n * MM;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
@@ -962,7 +1006,6 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float kscale = 6.28318530718f / (ORG_MB * P.dx);
     const float MMf = (float)MM;
 
-    unsigned lab = 0u;
     /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
 #pragma unroll 2
     for (int q = 0; q < 16; q++) {
@@ -972,7 +1015,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
         float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
         int idx = turnover_label(v, lum[g], ecm[g], ph1[g], ph2[g]);
-        lab |= (unsigned)idx << (2 * q);
+        lab[e] = (unsigned char)idx;
         float r = rh[g];
         float at = P.atan_[idx], ao = P.aort[idx], bb = P.bb[idx];
         float r3 = r * r * r;
@@ -988,51 +1031,42 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         float pnd = ng * expf(-d * d * isig2) * dnorm;
         float react = -P.deg[idx] * r + P.gen[idx] * pnd;
         rbox[e] = r + P.dt * react;
-        SB[e] = make_float2(pyv, 0.0f);
+        SB[e] = pyv;
         sm[i * ORG_LS + j] = make_float2(pxx, pxy);
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
 #pragma unroll 2
     for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        SA[e] = sm[i * ORG_LS + j];
-        sm[i * ORG_LS + j] = SB[e];      /* same thread wrote SB[e] in P1 */
+        const int e = t + 1024 * q, l = (e >> 7) * ORG_LS + (e & 127);
+        SA[e] = sm[l];
+        sm[l] = make_float2(SB[e], 0.0f);      /* this thread wrote SB[e] in P1 */
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-    /* P4: split, per-mode solve (rho_functions.cpp:133-190); v_x -> LDS, v_y -> SB */
-#pragma unroll 2
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+    /* P4: per pair {k, -k}: split F(pi_xx + i pi_xy), solve both modes
+     * (rho_functions.cpp:133-190), keep the part of v whose inverse is real,
+     * Z = H v_x + i H v_y written in place at k and -k. */
+    for (int u = t; u < NPAIR; u += 1024) {
+        int i, j;
+        if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
+        else { const int w = u - 63 * ORG_MB; i = (w < 65) ? 0 : 64; j = (w < 65) ? w : w - 65; }
         const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
-        float2 A = SA[e], Am = SA[im * ORG_MB + jm], B = sm[i * ORG_LS + j];
-        float2 pxx = make_float2(0.5f * (A.x + Am.x), 0.5f * (A.y - Am.y));
-        float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
-        int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
-        float eta = P.visc[(lab >> (2 * q)) & 3u];
-        float2 vx, vy;
-        velocity_mode(eta, P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vx, vy);
-        sm[i * ORG_LS + j] = vx;
-        SB[e] = vy;
-    }
-    __syncthreads();
-    /* Z = H v_x + i H v_y -> SA */
-#pragma unroll 2
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
-        float2 x = sm[i * ORG_LS + j], xm = sm[im * ORG_LS + jm];
-        float2 y = SB[e], ym = SB[im * ORG_MB + jm];
-        float2 hx = make_float2(0.5f * (x.x + xm.x), 0.5f * (x.y - xm.y));
-        float2 hy = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y - ym.y));
-        SA[e] = make_float2(hx.x - hy.y, hx.y + hy.x);
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q;
-        sm[(e >> 7) * ORG_LS + (e & 127)] = SA[e];
+        const int ek = i * ORG_MB + j, em = im * ORG_MB + jm;
+        const int lk = i * ORG_LS + j, lm = im * ORG_LS + jm;
+        const float2 A = SA[ek], Am = SA[em], B = sm[lk], Bm = sm[lm];
+        const float2 pxx = make_float2(0.5f * (A.x + Am.x), 0.5f * (A.y - Am.y));
+        const float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
+        const float2 pxxm = make_float2(pxx.x, -pxx.y), pxym = make_float2(pxy.x, -pxy.y);
+        const int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
+        const int irm = (im < ORG_MB / 2) ? im : im - ORG_MB, jrm = (jm < ORG_MB / 2) ? jm : jm - ORG_MB;
+        float2 vxk, vyk, vxm, vym;
+        velocity_mode(P.visc[lab[ek]], P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vxk, vyk);
+        velocity_mode(P.visc[lab[em]], P.friction, kscale * irm, kscale * jrm, pxxm, pxym, Bm, vxm, vym);
+        const float2 hx = make_float2(0.5f * (vxk.x + vxm.x), 0.5f * (vxk.y - vxm.y));
+        const float2 hy = make_float2(0.5f * (vyk.x + vym.x), 0.5f * (vyk.y - vym.y));
+        sm[lk] = make_float2(hx.x - hy.y, hx.y + hy.x);
+        if (em != ek) sm[lm] = make_float2(hx.x + hy.y, hy.x - hx.y);
     }
     __syncthreads();
     fft2d_128<1>(sm, tw);
@@ -1050,27 +1084,27 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-    /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249) -> SA */
+    /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249)
+     * per pair, the part whose inverse is real, in place */
     const float cdt = P.dt * kscale;
-#pragma unroll 2
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+    for (int u = t; u < NPAIR; u += 1024) {
+        int i, j;
+        if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
+        else { const int w = u - 63 * ORG_MB; i = (w < 65) ? 0 : 64; j = (w < 65) ? w : w - 65; }
         const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
-        float2 Y = sm[i * ORG_LS + j], Ym = sm[im * ORG_LS + jm];
-        float2 Pk = make_float2(0.5f * (Y.x + Ym.x), 0.5f * (Y.y - Ym.y));   /* F(rho vx)(k) */
-        float2 Qk = make_float2(0.5f * (Y.y + Ym.y), 0.5f * (Ym.x - Y.x));   /* F(rho vy)(k) */
-        float ir = (float)((i < ORG_MB / 2) ? i : i - ORG_MB), jr = (float)((j < ORG_MB / 2) ? j : j - ORG_MB);
-        float irm = (float)((im < ORG_MB / 2) ? im : im - ORG_MB), jrm = (float)((jm < ORG_MB / 2) ? jm : jm - ORG_MB);
-        float2 D = make_float2(cdt * (ir * Pk.y + jr * Qk.y), -cdt * (ir * Pk.x + jr * Qk.x));
+        const int lk = i * ORG_LS + j, lm = im * ORG_LS + jm;
+        const float2 Y = sm[lk], Ym = sm[lm];
+        const float2 Pk = make_float2(0.5f * (Y.x + Ym.x), 0.5f * (Y.y - Ym.y));   /* F(rho vx)(k) */
+        const float2 Qk = make_float2(0.5f * (Y.y + Ym.y), 0.5f * (Ym.x - Y.x));   /* F(rho vy)(k) */
+        const float ir = (float)((i < ORG_MB / 2) ? i : i - ORG_MB), jr = (float)((j < ORG_MB / 2) ? j : j - ORG_MB);
+        const float irm = (float)((im < ORG_MB / 2) ? im : im - ORG_MB);
+        const float jrm = (float)((jm < ORG_MB / 2) ? jm : jm - ORG_MB);
+        const float2 D = make_float2(cdt * (ir * Pk.y + jr * Qk.y), -cdt * (ir * Pk.x + jr * Qk.x));
         /* at -k the transforms are conj(Pk), conj(Qk) */
-        float2 Dm = make_float2(cdt * (irm * -Pk.y + jrm * -Qk.y), -cdt * (irm * Pk.x + jrm * Qk.x));
-        SA[e] = make_float2(0.5f * (D.x + Dm.x), 0.5f * (D.y - Dm.y));
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q;
-        sm[(e >> 7) * ORG_LS + (e & 127)] = SA[e];
+        const float2 Dm = make_float2(cdt * (irm * -Pk.y + jrm * -Qk.y), -cdt * (irm * Pk.x + jrm * Qk.x));
+        const float2 H = make_float2(0.5f * (D.x + Dm.x), 0.5f * (D.y - Dm.y));
+        sm[lk] = H;
+        if (lm != lk) sm[lm] = make_float2(H.x, -H.y);
     }
     __syncthreads();
     fft2d_128<1>(sm, tw);
@@ -1090,7 +1124,8 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
                                                    float *__restrict__ vybox, int write_v) {
     __shared__ float2 sm[ORG_MB * ORG_LS];
     __shared__ float2 tw[128];
-    spectral_body(P, blockIdx.x, sm, tw, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
+    __shared__ unsigned char lab[ORG_MB * ORG_MB];
+    spectral_body(P, blockIdx.x, sm, tw, lab, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
 }
 
 /* ======================================================================== */
@@ -1245,7 +1280,7 @@ struct LumBufs {
  * lum/ECM; the loop role writes only its ping-pong buffers), so they run
  * concurrently; phi's new window goes to phi_win.  LDS is shared between
  * the roles. */
-#define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8)
+#define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8 + ORG_MB * ORG_MB)
 #define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8)
 #define ORG_LDS_FUSED (ORG_LDS_SPECTRAL > ORG_LDS_CELLS ? ORG_LDS_SPECTRAL : ORG_LDS_CELLS)
 __global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
@@ -1266,8 +1301,8 @@ __global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict
                       err, lb.limit);
     } else if ((int)blockIdx.x < P.nc) {
         float2 *sm = (float2 *)lds;
-        spectral_body(P, blockIdx.x, sm, sm + ORG_MB * ORG_LS, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox,
-                      vybox, write_v);
+        spectral_body(P, blockIdx.x, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), phi,
+                      rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
     } else {
         float *phase = (float *)lds;
         double *red = (double *)(lds + 2 * ORG_WH * ORG_WH * 4);
@@ -1276,38 +1311,67 @@ __global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict
     }
 }
 
-/* k_commit_cells: rho <- rhon (copy_cell_tables_1_by_1,
- * useful_functions.cpp:679-693, run_with_active_gels.cpp:175-178): the FFT box
- * values plus the periodic ghost ring applied to rhon
- * (rho_functions.cpp:469). */
-__global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict__ rho,
-                                                       const float *__restrict__ rho_box, float *__restrict__ phi,
-                                                       const float *__restrict__ phi_win, const int *__restrict__ xc,
-                                                       const int *__restrict__ yc, const int *__restrict__ ring_done) {
-    const int n = blockIdx.x, N = P.N;
-    constexpr int M = ORG_MB;
+/* rho <- rhon on the FFT box and phi's new window (copy_cell_tables_1_by_1,
+ * useful_functions.cpp:679-693, run_with_active_gels.cpp:172-178). */
+static __device__ __forceinline__ void commit_cell_body(const DevP &P, const int n, float *__restrict__ rho,
+                                                        const float *__restrict__ rho_box, float *__restrict__ phi,
+                                                        const float *__restrict__ phi_win, const int *__restrict__ xc,
+                                                        const int *__restrict__ yc) {
+    const int N = P.N;
+    constexpr int M = ORG_MB, W = ORG_W;
     const size_t NN = (size_t)N * N;
     float *rh = rho + n * NN;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
+    const float *rb = rho_box + (size_t)n * M * M;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < M * M; e += blockDim.x)
+        rh[(size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127))] = rb[e];
     if (phi_win) {        /* phi window from k_cortex_cells (reshaping_cells' phase) */
-        constexpr int W = ORG_W;
-        const int wx = xc[n] - P.zs, wy = yc[n] - P.zs;
+        const int wx = x0 - 1, wy = y0 - 1;
         const float *pw = phi_win + (size_t)n * W * W;
         float *ph = phi + n * NN;
+#pragma unroll 4
         for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
             const int i = e / W;
             ph[(size_t)(wx + i) * N + (wy + e - i * W)] = pw[e];
         }
     }
-    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
-    const float *rb = rho_box + (size_t)n * M * M;
-    for (int e = threadIdx.x; e < M * M; e += blockDim.x)
-        rh[(size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127))] = rb[e];
-    /* ghost ring: its sources, rows/cols 1 and N-2, lie outside every FFT box
-     * (windows keep one pixel inside rows/cols 1..N-2) and are never written,
-     * so the ring is the same after every step: it is written on the first
-     * commit after org_prepare only (ring_done, set by the host-side flag
-     * kernel after this launch). */
-    if (*ring_done) return;
+}
+
+__global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict__ rho,
+                                                       const float *__restrict__ rho_box, float *__restrict__ phi,
+                                                       const float *__restrict__ phi_win, const int *__restrict__ xc,
+                                                       const int *__restrict__ yc) {
+    commit_cell_body(P, blockIdx.x, rho, rho_box, phi, phi_win, xc, yc);
+}
+
+/* k_post: rho/phi copy-back of every cell (workgroups 0..ncell-1) and
+ * lum <- lumn, ECM <- ECMn (the other workgroups) in one launch. */
+__global__ __launch_bounds__(1024) void k_post(DevP P, float *__restrict__ rho, const float *__restrict__ rho_box,
+                                               float *__restrict__ phi, const float *__restrict__ phi_win,
+                                               const int *__restrict__ xc, const int *__restrict__ yc,
+                                               float *__restrict__ lum, float *__restrict__ ecm,
+                                               const float *__restrict__ bufL0, const float *__restrict__ bufL1,
+                                               const float *__restrict__ bufE0, const float *__restrict__ bufE1,
+                                               const LumState *__restrict__ lst, int *__restrict__ err,
+                                               int *__restrict__ uni) {
+    if ((int)blockIdx.x < P.nc)
+        commit_cell_body(P, blockIdx.x, rho, rho_box, phi, phi_win, xc, yc);
+    else
+        commit_lumecm_body(P, blockIdx.x - P.nc, gridDim.x - P.nc, lum, ecm, bufL0, bufL1, bufE0, bufE1, lst, err,
+                           uni);
+}
+
+/* k_ring_rho: periodic ghost ring of every rho plane (the
+ * boundary_conditions_cells(rhon) of rho_functions.cpp:469, carried into rho
+ * by the copy at run_with_active_gels.cpp:175-178).  Its sources, rows/cols
+ * 1 and N-2, lie outside every FFT box (windows keep one pixel inside
+ * rows/cols 1..N-2) and are never written, so the ring is the same after
+ * every step: the library writes it once, after the first step following
+ * org_prepare. */
+__global__ __launch_bounds__(1024) void k_ring_rho(DevP P, float *__restrict__ rho) {
+    const int n = blockIdx.x, N = P.N;
+    float *rh = rho + (size_t)n * N * N;
     for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
         rh[(size_t)i * N] = rh[(size_t)i * N + N - 2];
         rh[(size_t)i * N + N - 1] = rh[(size_t)i * N + 1];
@@ -1320,10 +1384,6 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
         rh[(size_t)N * (N - 1)] = rh[(size_t)N + N - 2];
         rh[(size_t)N * (N - 1) + N - 1] = rh[(size_t)N + 1];
     }
-}
-
-__global__ void k_set_flag(int *flag, int v) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
 /* Test hook: one 128 x 128 2D DFT through the same LDS code path. */

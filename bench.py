@@ -100,9 +100,11 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": "k_cortex_cells", "avg_launch_ms": round(avg_ms, 5),
                 "alg_bytes_per_launch": bytes_launch}
-    names = {"centres": "k_centroid", "prep": "k_prep", "psi": "k_psi", "lumecm": "k_hlum+k_lumecm_update",
+    # profiling slots of the batched step's launches (the prep and cells
+    # slots are used by the per-function entry points only)
+    names = {"centres": "k_pre", "prep": "k_prep", "psi": "k_psi", "lumecm": "k_lumecm_update",
              "lumreshape": "k_lum_normals", "spectral": "k_cortex_cells", "cells": "k_cells",
-             "commit": "k_commit_cells+k_commit_lumecm"}
+             "commit": "k_post"}
     kernel_ms = {names[k]: round(v[0], 5) for k, v in per_kernel.items() if v[1]}
     cells_it, lum_it, ecm_it = org.iterations()
 

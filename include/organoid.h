@@ -44,7 +44,9 @@ enum {
     ORG_ERELAX = 5,       /* a reshaping loop passed rimax ("Relaxation
                              failure!", cell_phase_functions.cpp:565, 862)     */
     ORG_ENOTBUILT = 6,    /* kernels for this build are missing                 */
-    ORG_ESTATE = 7        /* call order / state error                           */
+    ORG_ESTATE = 7,       /* call order / state error                           */
+    ORG_ETIMEOUT = 8      /* a wait between workgroups of one launch passed its
+                             time limit (2 s); the step's results are invalid  */
 };
 
 /* ---- configuration ---------------------------------------------------------
@@ -188,7 +190,8 @@ int org_kernel_time(org_ctx *ctx, int kernel_id, double *ms, int64_t *launches);
 enum {
     ORG_K_CENTRES = 0, ORG_K_PREP = 1, ORG_K_PSI = 2, ORG_K_LUMECM = 3,
     ORG_K_LUMRESHAPE = 4, ORG_K_SPECTRAL = 5, ORG_K_CELLS = 6, ORG_K_COMMIT = 7,
-    ORG_K_COUNT = 8
+    ORG_K_STEP = 8,       /* the batched step's main launch (org_step) */
+    ORG_K_COUNT = 9
 };
 
 /* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of

@@ -34,10 +34,10 @@ struct org_ctx {
     org_params p{};
     DevP dp{};
     hipStream_t stream = nullptr;
-    double *part_lc = nullptr;           /* loop role partial deviations [2][2][blocks] */
-    unsigned *lum_bar = nullptr;         /* loop role barrier: count, generation */
+    double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][blocks] */
+    unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier: count, generation */
     int *lum_abort = nullptr;
-    long long wait_limit = 0;            /* loop role barrier timeout, wall-clock ticks */
+    long long wait_limit = 0;            /* k_step's wait timeout, wall-clock ticks */
     float *phi_win = nullptr;
     int N = 0, nc = 0, zs = 0, W = 0, M = 0;
     size_t NN = 0, P = 0;
@@ -53,6 +53,9 @@ struct org_ctx {
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
     int *uni = nullptr, *jbox = nullptr;   /* union of windows; per-workgroup boxes of nonzero normals */
+    int *xn = nullptr, *yn = nullptr, *bn = nullptr;  /* next batched step's centres and support boxes */
+    bool next_ready = false;               /* xn, yn, bn hold the centres of the current phi */
+    int npsi = 128;                        /* psi workgroups of k_pre */
     bool ring_pending = true;              /* rho ghost ring not yet written since org_prepare */
     int lum_grid = ORG_LUM_BLOCKS;         /* workgroups per field of a reshaping sweep */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
@@ -153,6 +156,7 @@ int check_ctx(org_ctx *c) {
 
 int need_prepared(org_ctx *c) {
     if (!c) return note_error(ORG_EINVAL);
+    c->next_ready = false;
     if (!c->prepared) {
         fprintf(stderr, "organoid: org_prepare must run after the fields are uploaded\n");
         return note_error(ORG_ESTATE);
@@ -177,8 +181,16 @@ int harvest_device_error(org_ctx *c) {
 /* ---- stage launchers ------------------------------------------------------ */
 int L_centroid(org_ctx *c) {
     Prof pr(c, ORG_K_CENTRES);
-    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->uni,
-                       c->err);
+    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->xc,
+                       c->yc, c->box, c->uni, c->err);
+    return launch_check();
+}
+/* centres and support boxes for the next batched step, from the current
+ * phi (what k_post computes at the end of every batched step) */
+int L_centroid_next(org_ctx *c) {
+    Prof pr(c, ORG_K_CENTRES);
+    hipLaunchKernelGGL(k_centroid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->xc, c->yc, c->box, c->xn,
+                       c->yn, c->bn, c->uni, c->err);
     return launch_check();
 }
 int L_prep(org_ctx *c, int mode) {
@@ -242,30 +254,37 @@ int L_commit_cells(org_ctx *c) {
                        (const float *)nullptr, c->xc, c->yc);
     return launch_check();
 }
-/* spectral + cells + lumen/ECM loops in one launch; phi's new window goes
- * to phi_win */
-int L_cortex_cells(org_ctx *c) {
-    Prof pr(c, ORG_K_SPECTRAL);
-    LumBufs lb{c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey, c->jbox, c->grid_g,
-               c->lst, c->part_lc, c->lum_bar, c->lum_abort, c->wait_limit};
-    hipLaunchKernelGGL(k_cortex_cells, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, c->phi,
-                       c->rho, c->lum, c->ecm, c->hfunc, c->psi, c->hlum, c->hecm, c->f0, c->xc, c->yc, c->spec,
-                       c->rho_box, c->vxbox, c->vybox, c->write_v, c->phin_win, c->phi_win, c->jwin, c->counts,
-                       c->err, lb);
-    return launch_check();
+StepArgs step_args(org_ctx *c) {
+    StepArgs a{};
+    a.phi = c->phi; a.rho = c->rho; a.hfunc = c->hfunc;
+    a.xc = c->xc; a.yc = c->yc; a.box = c->box; a.uni = c->uni; a.counts = c->counts;
+    a.xn = c->xn; a.yn = c->yn; a.bn = c->bn; a.npsi = c->npsi;
+    a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
+    a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
+    a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nvb = c->grid_g;
+    a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
+    a.jlx = c->jlx; a.jly = c->jly; a.jex = c->jex; a.jey = c->jey;
+    a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc;
+    a.spec = c->spec; a.rho_box = c->rho_box; a.vxbox = c->vxbox; a.vybox = c->vybox; a.write_v = c->write_v;
+    a.phin_win = c->phin_win; a.phi_win = c->phi_win; a.jwin = c->jwin;
+    a.bar = c->lum_bar; a.abort_flag = c->lum_abort; a.err = c->err; a.limit = c->wait_limit;
+    return a;
 }
+/* centroids, hfunc, areas and h(lum), h(ECM) of the step (see k_pre) */
 int L_pre(org_ctx *c) {
     Prof pr(c, ORG_K_CENTRES);
-    hipLaunchKernelGGL(k_pre, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, c->phi, c->hfunc, c->xc,
-                       c->yc, c->box, c->s0, c->s, c->f0, c->uni, c->err, c->lum, c->ecm, c->hlum, c->hecm, c->part_h,
-                       c->counters + 2, c->sum_ecm);
+    hipLaunchKernelGGL(k_pre, dim3(c->nc + c->npsi + c->grid_g), dim3(1024), 0, c->stream, c->dp, step_args(c));
+    return launch_check();
+}
+/* the rest of the step up to the copy-back, one launch (see k_step) */
+int L_step(org_ctx *c) {
+    Prof pr(c, ORG_K_STEP);
+    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
 int L_post(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
-    hipLaunchKernelGGL(k_post, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, c->rho, c->rho_box, c->phi,
-                       (const float *)c->phi_win, c->xc, c->yc, c->lum, c->ecm, c->bufL0, c->bufL1, c->bufE0,
-                       c->bufE1, c->lst, c->err, c->uni);
+    hipLaunchKernelGGL(k_post, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
 /* the one-time rho ghost ring (see k_ring_rho) */
@@ -292,18 +311,14 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
     }
 }
 
-/* One timestep, run_with_active_gels.cpp:118-184.  Part 1: centroids, h,
- * psi, areas, lumen/ECM update and normals.  Part 2: one launch for the
- * spectral cortex update and the phase-field update/reshaping of every cell
- * and the lumen/ECM reshaping loops, then the copy-back.  No host round
- * trip: the whole step can be captured in one hipGraph. */
+/* One timestep, run_with_active_gels.cpp:118-184: centroids, hfunc, areas
+ * and h(lum), h(ECM) (:123-142), the rest up to the copy-back (:130-172),
+ * the copy-back (:172-182).  No host round trip: the whole step can be
+ * captured in one hipGraph. */
 int enqueue_step(org_ctx *c) {
-    TRY(L_pre(c));                  /* :123, :133-134, :142 */
-    TRY(L_psi(c));                  /* :139 */
-    TRY(L_lumecm_update(c));        /* :157-158 */
-    TRY(L_lum_normals(c, 0, 2));    /* :160-161 */
-    TRY(L_cortex_cells(c));         /* :130, :137, :154, :160-172 */
-    TRY(L_post(c));                 /* :172 phi window, :175-182 */
+    TRY(L_pre(c));
+    TRY(L_step(c));
+    TRY(L_post(c));
     return ORG_OK;
 }
 
@@ -321,9 +336,10 @@ const char *org_error_string(int code) {
     case ORG_EHIP: return "HIP runtime error";
     case ORG_EEMPTY_CELL: return "a cell has no phi>0.5 pixel (centroid undefined)";
     case ORG_EWINDOW: return "a cell window (+1 halo) leaves the grid interior";
-    case ORG_ERELAX: return "reshaping did not converge (rimax, or graph sweep budget exceeded)";
+    case ORG_ERELAX: return "reshaping did not converge (rimax)";
     case ORG_ENOTBUILT: return "kernels not built for this device";
     case ORG_ESTATE: return "call order / state error";
+    case ORG_ETIMEOUT: return "a wait between workgroups of one launch timed out";
     default: return "unknown error";
     }
 }
@@ -364,12 +380,13 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
     A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
-    A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
+    A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
+    A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
     if (!rc) {
-        /* loop-role barrier timeout: 2 s of the device's wall clock */
+        /* timeout of the waits inside k_step: 2 s of the device's wall clock */
         int dev = 0, khz = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
@@ -506,6 +523,8 @@ int org_prepare(org_ctx *c) {
     hipLaunchKernelGGL(k_init_box, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->box);
     TRY(launch_check());
     HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipMemsetAsync(c->lum_bar, 0, sizeof(unsigned), c->stream));
+    HIPCHK(hipMemsetAsync(c->lum_abort, 0, sizeof(int), c->stream));
     {
         const int u[4] = {c->N, 0, c->N, 0};
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -514,6 +533,7 @@ int org_prepare(org_ctx *c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->prepared = true;
     c->ring_pending = true;
+    c->next_ready = false;
     return ORG_OK;
 }
 
@@ -529,7 +549,9 @@ int org_set_option(org_ctx *c, int option, int value) {
     return ORG_OK;
 }
 
-/* ---- per-function entry points ----------------------------------------- */
+/* ---- per-function entry points -----------------------------------------
+ * Each may change phi or the centres, so the next batched step recomputes
+ * its centres first. */
 int org_find_centres_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_centroid(c)); return harvest_device_error(c); }
 int org_compute_hfunc_cells(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 1); }
 int org_sum_s(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 2); }
@@ -563,8 +585,13 @@ int org_commit_lumecm(org_ctx *c) { TRY(need_prepared(c)); TRY(L_commit_lumecm(c
 /* ---- batched form ------------------------------------------------------- */
 int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     (void)it0;
+    if (!c) return note_error(ORG_EINVAL);
+    const bool ready = c->next_ready;
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
+    if (nsteps == 0) { c->next_ready = ready; return ORG_OK; }
+    if (!ready) TRY(L_centroid_next(c));   /* k_post provides them after every batched step */
+    c->next_ready = true;
     if (nsteps > 0 && c->ring_pending) {   /* first step after org_prepare: eager, plus the ghost ring */
         TRY(enqueue_step(c));
         TRY(L_ring_if_pending(c));

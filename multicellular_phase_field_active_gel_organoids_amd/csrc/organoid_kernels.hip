@@ -22,23 +22,28 @@
  * recomputed there from every cell whose support box covers the pixel, in
  * cell order, so the FP32 sum has the same operands in the same order.
  *
- * Kernels per step (all cells per launch):
+ * Per-function kernels (the reference's call sequence, all cells per launch):
  *   k_centroid      integer centroid + window check + support box update
  *   k_prep          hfunc on the window, FP64 area sum over the support box
  *   k_hlum          h(lum), h(ECM) full plane + FP64 sum of h(ECM)
  *   k_psi           psi on tiles covering the union of this step's windows
- *   k_lumecm_update lumen / ECM explicit step + Dirichlet edges
- *   k_lum_normals   |grad| of the old field, normals of the new one, periodic ring
- *   k_lum_iter      one reshaping sweep of lumen and ECM, FP64 deviation,
- *                   device-side stop test (last-block reduction)
  *   k_spectral      per cell: active stress, 2 forward FFTs, the per-mode
  *                   friction-viscosity solve, Hermitian split, inverse FFT of
  *                   the velocity, rho*v, forward FFT, spectral divergence,
  *                   inverse FFT -> new rho on the FFT box  (4 2D FFTs of
  *                   128 x 128 in LDS instead of the reference's 10)
+ *   k_lumecm_update lumen / ECM explicit step + Dirichlet edges
+ *   k_lum_normals   |grad| of the old field, normals of the new one, periodic ring
+ *   k_lum_iter      one reshaping sweep of lumen and ECM, FP64 deviation,
+ *                   device-side stop test (last-block reduction)
  *   k_cells         per cell: phase-field update, normals, reshaping loop in
  *                   LDS until the deviation test passes, phi <- second-to-last
- *   k_commit_cells  rho box write-back + ghost ring;  k_commit_lumecm.
+ *   k_commit_cells  rho box write-back + phi window;  k_commit_lumecm.
+ * Batched step (two launches, replayable from one hipGraph):
+ *   k_step          everything above up to the copy-back, workgroups in three
+ *                   roles that signal each other through device counters
+ *   k_post          the copy-back (k_commit_cells + k_commit_lumecm)
+ *   k_ring_rho      rho's ghost ring, once after the first step
  */
 #include <hip/hip_runtime.h>
 
@@ -173,6 +178,17 @@ static __device__ double block_sum_d(double v, double *red) {
     double s = 0.0;
     for (int k = 0; k < nw; k++) s += red[k];
     return s;
+}
+
+/* ---- synchronisation between workgroups of one launch ----------------------
+ * Each XCD has its own L2, so making stores visible to a workgroup on another
+ * XCD takes an agent-scope release (the writer's L2 is written back) and an
+ * agent-scope acquire on the reader's side (its L2 is invalidated).  Both are
+ * costly and disturb every workgroup resident on the XCD, so only the
+ * lumen/ECM workgroups of k_step synchronise, through lum_barrier. */
+static __device__ __forceinline__ void drain_stores() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
 }
 
 /* ---- small in-register DFTs (radix-2, natural order in and out) ----------- */
@@ -317,54 +333,128 @@ __global__ void k_init_box(DevP P, const float *__restrict__ phi, const float *_
     }
 }
 
-/* k_centroid: find_centres_cells (useful_functions.cpp:193-227) over the
- * support box, window check, support box += window.  Block 0 also resets the
- * union-of-windows box that k_prep fills and k_psi reads. */
-__global__ __launch_bounds__(1024) void k_centroid(DevP P, const float *__restrict__ phi, int *__restrict__ xc,
-                                                   int *__restrict__ yc, int *__restrict__ box,
-                                                   int *__restrict__ uni, int *__restrict__ err) {
-    __shared__ long long red[3][16];
-    const int n = blockIdx.x, N = P.N;
-    const size_t NN = (size_t)N * N;
-    const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    if (n == 0 && threadIdx.x == 0) { uni[0] = N; uni[1] = 0; uni[2] = N; uni[3] = 0; }
+/* ---- centroid and area over a support box ---------------------------------
+ * A box [i0,i1) x [j0,j1) is walked as one flat pixel range: pixel e sits at
+ * (i0 + e / bw, j0 + e % bw) and thread t takes e = t, t + blockDim, ...;
+ * the (row, column) pair advances incrementally, so only one division per
+ * thread is needed and consecutive lanes read consecutive columns. */
+
+/* Integer centroid sums of {phi > 0.5} over the box (useful_functions.cpp:
+ * 193-227); totals valid in thread 0.  red: >= 48 long longs of LDS. */
+static __device__ void centroid_sums(const float *__restrict__ ph, int N, int i0, int i1, int j0, int j1,
+                                     long long *red, long long &X, long long &Y, long long &C) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6, nt = blockDim.x;
+    const int bw = j1 - j0, bh = i1 - i0;
+    const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
     long long sx = 0, sy = 0, c = 0;
-    const float *ph = phi + n * NN;
-    for (int i = i0 + w; i < i1; i += nw) {
-        const float *row = ph + (size_t)i * N;
-        for (int j = j0 + lane; j < j1; j += 64)
-            if (row[j] > 0.5f) { sx += i; sy += j; c += 1; }
+    if (npx > 0) {
+        const int di = nt / bw, dj = nt - di * bw;
+        int i = t / bw, j = t - i * bw;
+#pragma unroll 8
+        for (int e = t; e < npx; e += nt) {
+            if (ph[(size_t)(i0 + i) * N + (j0 + j)] > 0.5f) { sx += i0 + i; sy += j0 + j; c += 1; }
+            i += di; j += dj;
+            if (j >= bw) { j -= bw; i++; }
+        }
     }
     sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
-    if (lane == 0) { red[0][w] = sx; red[1][w] = sy; red[2][w] = c; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        long long X = 0, Y = 0, C = 0;
-        for (int k = 0; k < nw; k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
-        int x = xc[n], y = yc[n];
-        if (C == 0) {
-            set_err(err, ORG_EEMPTY_CELL);
-        } else {
-            x = (int)(X / C);
-            y = (int)(Y / C);
+    if (lane == 0) { red[w] = sx; red[16 + w] = sy; red[32 + w] = c; }
+    __syncthreads();
+    X = 0; Y = 0; C = 0;
+    if (t == 0)
+        for (int k = 0; k < nw; k++) { X += red[k]; Y += red[16 + k]; C += red[32 + k]; }
+}
+
+/* Centre of cell n from the sums (integer division as the reference's
+ * int(sum/count), useful_functions.cpp:221-222; the previous centre when the
+ * cell is empty), the window check, and the support box (i0,i1,j0,j1) grown
+ * by the window, stored to xo, yo, bo.  Thread 0 only. */
+static __device__ void place_window(const DevP &P, int n, long long X, long long Y, long long C, int xprev,
+                                    int yprev, int i0, int i1, int j0, int j1, int *__restrict__ xo,
+                                    int *__restrict__ yo, int *__restrict__ bo, int *__restrict__ err) {
+    const int N = P.N;
+    int x = xprev, y = yprev;
+    if (C == 0) set_err(err, ORG_EEMPTY_CELL);
+    else { x = (int)(X / C); y = (int)(Y / C); }
+    /* window [x-zs, x+zs) plus one halo pixel inside rows/cols 1..N-2 */
+    const int lo = P.zs + 2, hi = N - 2 - P.zs;
+    if (x < lo || x > hi || y < lo || y > hi) {
+        set_err(err, ORG_EWINDOW);
+        x = min(max(x, lo), hi);
+        y = min(max(y, lo), hi);
+    }
+    xo[n] = x; yo[n] = y;
+    int b0, b1, b2, b3;
+    if (i1 <= i0 || j1 <= j0) { b0 = x - P.zs; b1 = x + P.zs; b2 = y - P.zs; b3 = y + P.zs; }
+    else { b0 = min(i0, x - P.zs); b1 = max(i1, x + P.zs); b2 = min(j0, y - P.zs); b3 = max(j1, y + P.zs); }
+    bo[4 * n + 0] = b0; bo[4 * n + 1] = b1; bo[4 * n + 2] = b2; bo[4 * n + 3] = b3;
+}
+
+/* Centroid of cell n over its support box box_in (find_centres_cells,
+ * useful_functions.cpp:193-227), placed by place_window into xo, yo, bo.
+ * red: >= 48 long longs of LDS. */
+static __device__ void cell_centroid(const DevP &P, int n, const float *__restrict__ phi,
+                                     const int *__restrict__ xin, const int *__restrict__ yin,
+                                     const int *__restrict__ box_in, int *__restrict__ xo, int *__restrict__ yo,
+                                     int *__restrict__ bo, int *__restrict__ err, long long *red) {
+    const int i0 = box_in[4 * n], i1 = box_in[4 * n + 1], j0 = box_in[4 * n + 2], j1 = box_in[4 * n + 3];
+    const int xp = xin[n], yp = yin[n];
+    long long X, Y, C;
+    centroid_sums(phi + n * (size_t)P.N * P.N, P.N, i0, i1, j0, j1, red, X, Y, C);
+    if (threadIdx.x == 0) place_window(P, n, X, Y, C, xp, yp, i0, i1, j0, j1, xo, yo, bo, err);
+}
+
+/* This thread's share of s_n = sum over the box of h*dx*dx in FP64
+ * (sum_s_kernels, useful_functions.cpp:168-190).  With win, h = hfun(phi)
+ * inside the window [x0, x0+W) x [y0, y0+W) and is stored to hfunc there
+ * (compute_hfunc_3d_tables_kernels_loop_n, useful_functions.cpp:233-251);
+ * elsewhere hfunc is read.  Loads are issued 8 pixels at a time. */
+static __device__ double area_sum(const DevP &P, const float *__restrict__ ph, float *__restrict__ hf, int i0,
+                                  int i1, int j0, int j1, bool win, int x0, int y0) {
+    constexpr int B = 8, W = ORG_W;
+    const int t = threadIdx.x, N = P.N, nt = blockDim.x;
+    const int bw = j1 - j0, bh = i1 - i0;
+    const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
+    double acc = 0.0;
+    if (npx == 0) return acc;
+    const int di = nt / bw, dj = nt - di * bw;
+    int i = t / bw, j = t - i * bw;
+    for (int e0 = t; e0 < npx; e0 += B * nt) {
+        int off[B];
+        bool in[B];
+        float v[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int gi = i0 + i, gj = j0 + j;
+            off[k] = gi * N + gj;
+            in[k] = win && gi >= x0 && gi < x0 + W && gj >= y0 && gj < y0 + W;
+            v[k] = (e0 + k * nt < npx) ? (in[k] ? ph : hf)[off[k]] : 0.0f;
+            i += di; j += dj;
+            if (j >= bw) { j -= bw; i++; }
         }
-        /* window [x-zs, x+zs) plus one halo pixel inside rows/cols 1..N-2 */
-        int lo = P.zs + 2, hi = N - 2 - P.zs;
-        if (x < lo || x > hi || y < lo || y > hi) {
-            set_err(err, ORG_EWINDOW);
-            x = min(max(x, lo), hi);
-            y = min(max(y, lo), hi);
-        }
-        xc[n] = x; yc[n] = y;
-        if (i1 <= i0 || j1 <= j0) {
-            box[4 * n + 0] = x - P.zs; box[4 * n + 1] = x + P.zs;
-            box[4 * n + 2] = y - P.zs; box[4 * n + 3] = y + P.zs;
-        } else {
-            box[4 * n + 0] = min(i0, x - P.zs); box[4 * n + 1] = max(i1, x + P.zs);
-            box[4 * n + 2] = min(j0, y - P.zs); box[4 * n + 3] = max(j1, y + P.zs);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            if (e0 + k * nt < npx) {
+                const float h = in[k] ? hfun(v[k]) : v[k];
+                if (in[k]) hf[off[k]] = h;
+                acc += (double)(h * P.dx * P.dx);
+            }
         }
     }
+    return acc;
+}
+
+/* k_centroid: find_centres_cells (useful_functions.cpp:193-227) over the
+ * support box, window check, support box += window; the result goes to
+ * (xo, yo, bo), which may be the inputs.  Block 0 also resets the
+ * union-of-windows box that k_prep fills and k_psi reads. */
+__global__ __launch_bounds__(1024) void k_centroid(DevP P, const float *__restrict__ phi, const int *xin,
+                                                   const int *yin, const int *box_in, int *xo, int *yo, int *bo,
+                                                   int *__restrict__ uni, int *__restrict__ err) {
+    __shared__ long long red[48];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { uni[0] = P.N; uni[1] = 0; uni[2] = P.N; uni[3] = 0; }
+    cell_centroid(P, blockIdx.x, phi, xin, yin, box_in, xo, yo, bo, err, red);
 }
 
 /* k_prep: mode&1 -> hfunc on the window (useful_functions.cpp:233-251);
@@ -395,18 +485,12 @@ __global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__
         }
     }
     if (mode & 2) {
-        const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
-        double acc = 0.0;
-        for (int i = i0 + w; i < i1; i += nw) {
-            const bool rowin = (mode & 1) && i >= x0 && i < x0 + W;
-            const float *pr = ph + (size_t)i * N, *hr = hf + (size_t)i * N;
-            for (int j = j0 + lane; j < j1; j += 64) {
-                /* in the window use the value this launch stored there */
-                float h = (rowin && j >= y0 && j < y0 + W) ? hfun(pr[j]) : hr[j];
-                acc += (double)(h * P.dx * P.dx);
-            }
-        }
-        double tot = block_sum_d(acc, red);
+        /* the window's hfunc is read back: same values as area_sum's
+         * in-register hfun in the batched form */
+        __syncthreads();
+        const double acc = area_sum(P, ph, hf, box[4 * n], box[4 * n + 1], box[4 * n + 2], box[4 * n + 3], false,
+                                    x0, y0);
+        const double tot = block_sum_d(acc, red);
         if (threadIdx.x == 0) {
             s[n] = tot;
             f0[n] = P.alpha * ((float)s0[n] - (float)tot);
@@ -414,92 +498,44 @@ __global__ __launch_bounds__(1024) void k_prep(DevP P, const float *__restrict__
     }
 }
 
-/* k_cell_prep: k_centroid and k_prep (mode 3) of one cell in one workgroup
- * (batched form).  The centre and support box pass through LDS; the union
- * box is reset by k_commit_lumecm at the end of the previous step. */
-static __device__ __forceinline__ void cell_prep_body(const DevP &P, const int n, const float *__restrict__ phi,
-                                                      float *__restrict__ hfunc, int *__restrict__ xc,
-                                                      int *__restrict__ yc, int *__restrict__ box,
-                                                      const double *__restrict__ s0, double *__restrict__ s,
-                                                      float *__restrict__ f0, int *__restrict__ uni,
-                                                      int *__restrict__ err) {
-    __shared__ long long red[3][16];
-    __shared__ double dred[16];
-    __shared__ int sh[6];
-    const int N = P.N;
-    constexpr int W = ORG_W;
-    const size_t NN = (size_t)N * N;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    const float *ph = phi + n * NN;
-    float *hf = hfunc + n * NN;
-    /* centroid over the support box (useful_functions.cpp:193-227) */
-    {
-        const int i0 = box[4 * n], i1 = box[4 * n + 1], j0 = box[4 * n + 2], j1 = box[4 * n + 3];
-        long long sx = 0, sy = 0, c = 0;
-        for (int i = i0 + w; i < i1; i += nw) {
-            const float *row = ph + (size_t)i * N;
-            for (int j = j0 + lane; j < j1; j += 64)
-                if (row[j] > 0.5f) { sx += i; sy += j; c += 1; }
-        }
-        sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
-        if (lane == 0) { red[0][w] = sx; red[1][w] = sy; red[2][w] = c; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long X = 0, Y = 0, C = 0;
-            for (int k = 0; k < nw; k++) { X += red[0][k]; Y += red[1][k]; C += red[2][k]; }
-            int x = xc[n], y = yc[n];
-            if (C == 0) set_err(err, ORG_EEMPTY_CELL);
-            else { x = (int)(X / C); y = (int)(Y / C); }
-            int lo = P.zs + 2, hi = N - 2 - P.zs;
-            if (x < lo || x > hi || y < lo || y > hi) {
-                set_err(err, ORG_EWINDOW);
-                x = min(max(x, lo), hi);
-                y = min(max(y, lo), hi);
-            }
-            xc[n] = x; yc[n] = y;
-            int b0, b1, b2, b3;
-            if (i1 <= i0 || j1 <= j0) { b0 = x - P.zs; b1 = x + P.zs; b2 = y - P.zs; b3 = y + P.zs; }
-            else { b0 = min(i0, x - P.zs); b1 = max(i1, x + P.zs); b2 = min(j0, y - P.zs); b3 = max(j1, y + P.zs); }
-            box[4 * n + 0] = b0; box[4 * n + 1] = b1; box[4 * n + 2] = b2; box[4 * n + 3] = b3;
-            sh[0] = x - P.zs; sh[1] = y - P.zs; sh[2] = b0; sh[3] = b1; sh[4] = b2; sh[5] = b3;
-            atomicMin(&uni[0], x - P.zs); atomicMax(&uni[1], x - P.zs + W);
-            atomicMin(&uni[2], y - P.zs); atomicMax(&uni[3], y - P.zs + W);
-        }
-        __syncthreads();
+/* hfunc on the window and area of cell n (the batched form of k_prep); the
+ * centre and support box come from the previous step's copy-back (xn, yn,
+ * bn) and are published to xc, yc, box.  red: >= 16 doubles of LDS. */
+static __device__ void cell_prep(const DevP &P, const int n, const float *__restrict__ phi,
+                                 float *__restrict__ hfunc, const int *__restrict__ xn, const int *__restrict__ yn,
+                                 const int *__restrict__ bn, int *__restrict__ xc, int *__restrict__ yc,
+                                 int *__restrict__ box, const double *__restrict__ s0, double *__restrict__ s,
+                                 float *__restrict__ f0, double *red) {
+    const size_t NN = (size_t)P.N * P.N;
+    const int x = xn[n], y = yn[n];
+    const int b0 = bn[4 * n], b1 = bn[4 * n + 1], b2 = bn[4 * n + 2], b3 = bn[4 * n + 3];
+    if (threadIdx.x == 0) {
+        xc[n] = x; yc[n] = y;
+        box[4 * n] = b0; box[4 * n + 1] = b1; box[4 * n + 2] = b2; box[4 * n + 3] = b3;
     }
-    const int x0 = sh[0], y0 = sh[1], i0 = sh[2], i1 = sh[3], j0 = sh[4], j1 = sh[5];
-    /* hfunc on the window (useful_functions.cpp:233-251) */
-    for (int i = w; i < W; i += nw) {
-        const size_t g = (size_t)(x0 + i) * N + y0;
-        for (int j = lane; j < W; j += 64) hf[g + j] = hfun(ph[g + j]);
-    }
-    /* area over the support box, FP64 (useful_functions.cpp:168-190) */
-    double acc = 0.0;
-    for (int i = i0 + w; i < i1; i += nw) {
-        const bool rowin = i >= x0 && i < x0 + W;
-        const float *pr = ph + (size_t)i * N, *hr = hf + (size_t)i * N;
-        for (int j = j0 + lane; j < j1; j += 64) {
-            float h = (rowin && j >= y0 && j < y0 + W) ? hfun(pr[j]) : hr[j];
-            acc += (double)(h * P.dx * P.dx);
-        }
-    }
-    double tot = block_sum_d(acc, dred);
+    const double acc = area_sum(P, phi + n * NN, hfunc + n * NN, b0, b1, b2, b3, true, x - P.zs, y - P.zs);
+    const double tot = block_sum_d(acc, red);
     if (threadIdx.x == 0) {
         s[n] = tot;
         f0[n] = P.alpha * ((float)s0[n] - (float)tot);
     }
 }
 
-/* k_hlum: compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285)
- * and the FP64 sum_ecm of update_lumen_and_ECM (cell_phase_functions.cpp:376-386),
- * reduced by the last block to finish (fixed order: deterministic). */
-static __device__ __forceinline__ void hlum_body(const DevP &P, const int blk, const int nblk,
-                                                 const float *__restrict__ lum, const float *__restrict__ ecm,
-                                                 float *__restrict__ hlum, float *__restrict__ hecm,
-                                                 double *__restrict__ part, unsigned *__restrict__ counter,
-                                                 double *__restrict__ sum_out) {
-    __shared__ double red[16];
-    __shared__ bool last;
+/* Sum of per-workgroup partials in a fixed order, valid in every thread
+ * (nblk <= blockDim). */
+static __device__ __forceinline__ double sum_parts(const double *part, int nblk, double *red) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+        v += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return block_sum_d(v, red);
+}
+
+/* compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285) on rows
+ * blk, blk + nblk, ...; returns this thread's share of the FP64 sum of
+ * h(ECM) dx^2 of update_lumen_and_ECM (cell_phase_functions.cpp:376-386). */
+static __device__ __forceinline__ double hlum_rows(const DevP &P, const int blk, const int nblk,
+                                                   const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                   float *__restrict__ hlum, float *__restrict__ hecm) {
     const int N = P.N;
     double acc = 0.0;
 #pragma unroll 4
@@ -513,7 +549,19 @@ static __device__ __forceinline__ void hlum_body(const DevP &P, const int blk, c
             acc += (double)(hb * P.dx * P.dx);
         }
     }
-    double tot = block_sum_d(acc, red);
+    return acc;
+}
+
+/* k_hlum: h(lum), h(ECM) and sum h(ECM), the sum reduced by the last block
+ * to finish in a fixed order (deterministic). */
+__global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
+                                               float *__restrict__ hlum, float *__restrict__ hecm,
+                                               double *__restrict__ part, unsigned *__restrict__ counter,
+                                               double *__restrict__ sum_out) {
+    __shared__ double red[16];
+    __shared__ bool last;
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const double tot = block_sum_d(hlum_rows(P, blk, nblk, lum, ecm, hlum, hecm), red);
     if (threadIdx.x == 0) {
         part[blk] = tot;
         __threadfence();
@@ -523,84 +571,118 @@ static __device__ __forceinline__ void hlum_body(const DevP &P, const int blk, c
     __syncthreads();
     if (last) {
         __threadfence();
-        double v = 0.0;
-        for (int k = threadIdx.x; k < nblk; k += blockDim.x)
-            v += __hip_atomic_load(&part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double t2 = block_sum_d(v, red);
+        const double t2 = sum_parts(part, nblk, red);
         if (threadIdx.x == 0) { *sum_out = t2; *counter = 0u; }
     }
 }
 
-__global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
-                                               float *__restrict__ hlum, float *__restrict__ hecm,
-                                               double *__restrict__ part, unsigned *__restrict__ counter,
-                                               double *__restrict__ sum_out) {
-    hlum_body(P, blockIdx.x, gridDim.x, lum, ecm, hlum, hecm, part, counter, sum_out);
-}
-
-/* k_pre: centroid + prep of every cell (workgroups 0..ncell-1) and
- * h(lum), h(ECM), sum h(ECM) (the other nblk workgroups) in one launch; the
- * two roles are independent. */
-__global__ __launch_bounds__(1024) void k_pre(DevP P, const float *__restrict__ phi, float *__restrict__ hfunc,
-                                              int *__restrict__ xc, int *__restrict__ yc, int *__restrict__ box,
-                                              const double *__restrict__ s0, double *__restrict__ s,
-                                              float *__restrict__ f0, int *__restrict__ uni, int *__restrict__ err,
-                                              const float *__restrict__ lum, const float *__restrict__ ecm,
-                                              float *__restrict__ hlum, float *__restrict__ hecm,
-                                              double *__restrict__ part, unsigned *__restrict__ counter,
-                                              double *__restrict__ sum_out) {
-    if ((int)blockIdx.x < P.nc)
-        cell_prep_body(P, blockIdx.x, phi, hfunc, xc, yc, box, s0, s, f0, uni, err);
-    else
-        hlum_body(P, blockIdx.x - P.nc, gridDim.x - P.nc, lum, ecm, hlum, hecm, part, counter, sum_out);
-}
-
-/* k_psi: compute_psi_kernels (useful_functions.cpp:305-322) on 32 x 32 tiles
- * of the union of this step's windows. */
-__global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ box,
-                                             const int *__restrict__ uni, float *__restrict__ psi) {
-    __shared__ int list[1024];
-    __shared__ int4 lbox[1024];
-    __shared__ int cnt;
-    const int N = P.N, ntx = (N + 15) / 16;
-    const int ti0 = (blockIdx.x / ntx) * 16, tj0 = (blockIdx.x % ntx) * 16;
-    const int u0 = uni[0], u1 = uni[1], v0 = uni[2], v1 = uni[3];
-    if (ti0 >= u1 || ti0 + 16 <= u0 || tj0 >= v1 || tj0 + 16 <= v0) return;
-    /* ordered list of cells whose support box meets the tile (wave 0) */
-    if (threadIdx.x < 64) {
-        int base = 0;
-        for (int c0 = 0; c0 < P.nc; c0 += 64) {
-            int n = c0 + threadIdx.x;
+/* psi = sum_n hfunc_n (compute_psi_kernels, useful_functions.cpp:305-322) on
+ * one 16 x 16 tile per group of 256 threads (ti0 < 0: the group has no tile
+ * and only joins the barriers).  The cells whose support box meets the tile
+ * are listed 64 at a time in cell order by the group's first wave, so the
+ * FP32 sum takes its operands in cell order, as the reference's loop over n
+ * does.  With phi != nullptr, hfunc_n inside cell n's window (origin from
+ * xc, yc) is taken as hfun(phi_n) (the value the same step's prep stores
+ * there), so psi needs no wait for the prep.  Per group: a 64-entry list in
+ * LDS (ORG_PSI_LDS_GROUP bytes).  Every thread of the workgroup calls this
+ * together (it synchronises the workgroup). */
+#define ORG_PSI_LDS_GROUP (64 * 4 + 64 * 16 + 64 * 8 + 16)
+static __device__ void psi_tile(const DevP &P, const float *__restrict__ hfunc, const float *__restrict__ phi,
+                                const int *__restrict__ box, const int *__restrict__ xc, const int *__restrict__ yc,
+                                const int4 u, const int ti0, const int tj0, unsigned char *lq,
+                                float *__restrict__ psi) {
+    constexpr int W = ORG_W;
+    int *lcell = (int *)lq;
+    int4 *lbox = (int4 *)(lq + 64 * 4);
+    int2 *lwin = (int2 *)(lq + 64 * 4 + 64 * 16);
+    int *lcnt = (int *)(lq + 64 * 4 + 64 * 16 + 64 * 8);
+    const int gt = threadIdx.x & 255, N = P.N;
+    const size_t NN = (size_t)N * N;
+    const int i = ti0 + (gt >> 4), j = tj0 + (gt & 15);
+    const bool act = ti0 >= 0 && i < N && j < N && i >= u.x && i < u.y && j >= u.z && j < u.w;
+    float acc = 0.0f;
+    for (int c0 = 0; c0 < P.nc; c0 += 64) {
+        if (gt < 64) {
+            const int n = c0 + gt;
             bool hit = false;
-            if (n < P.nc) {
-                const int *bx = box + 4 * n;
-                hit = !(ti0 >= bx[1] || ti0 + 16 <= bx[0] || tj0 >= bx[3] || tj0 + 16 <= bx[2]);
+            int4 bx = make_int4(0, 0, 0, 0);
+            if (ti0 >= 0 && n < P.nc) {
+                bx = make_int4(box[4 * n], box[4 * n + 1], box[4 * n + 2], box[4 * n + 3]);
+                hit = !(ti0 >= bx.y || ti0 + 16 <= bx.x || tj0 >= bx.w || tj0 + 16 <= bx.z);
             }
-            unsigned long long m = __ballot(hit);
+            const unsigned long long m = __ballot(hit);
             if (hit) {
-                unsigned long long below = m & ((1ull << threadIdx.x) - 1ull);
-                int pos = base + __popcll(below);
-                if (pos < 1024) {
-                    list[pos] = n;
-                    lbox[pos] = make_int4(box[4 * n], box[4 * n + 1], box[4 * n + 2], box[4 * n + 3]);
+                const int pos = __popcll(m & ((1ull << gt) - 1ull));
+                lcell[pos] = n;
+                lbox[pos] = bx;
+                lwin[pos] = phi ? make_int2(xc[n] - P.zs, yc[n] - P.zs) : make_int2(-1, -1);
+            }
+            if (gt == 0) *lcnt = __popcll(m);
+        }
+        __syncthreads();
+        if (act) {
+            const int cnt = *lcnt;
+            const size_t e = (size_t)i * N + j;
+            for (int k = 0; k < cnt; k++) {
+                const int4 bx = lbox[k];
+                if (i >= bx.x && i < bx.y && j >= bx.z && j < bx.w) {
+                    const int2 w = lwin[k];
+                    const size_t o = lcell[k] * NN + e;
+                    if (w.x >= 0 && i >= w.x && i < w.x + W && j >= w.y && j < w.y + W) acc += hfun(phi[o]);
+                    else acc += hfunc[o];
                 }
             }
-            base += __popcll(m);
         }
-        if (threadIdx.x == 0) cnt = min(base, 1024);
+        __syncthreads();
+    }
+    if (act) psi[(size_t)i * N + j] = acc;
+}
+
+/* k_psi: psi on the 16 x 16 tiles of the grid that meet the union of this
+ * step's windows (psi changes nowhere else). */
+__global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ hfunc, const int *__restrict__ box,
+                                             const int *__restrict__ uni, float *__restrict__ psi) {
+    __shared__ __attribute__((aligned(16))) unsigned char lq[ORG_PSI_LDS_GROUP];
+    const int ntx = (P.N + 15) / 16;
+    const int ti0 = (blockIdx.x / ntx) * 16, tj0 = (blockIdx.x % ntx) * 16;
+    const int4 u = make_int4(uni[0], uni[1], uni[2], uni[3]);
+    if (ti0 >= u.y || ti0 + 16 <= u.x || tj0 >= u.w || tj0 + 16 <= u.z) return;
+    psi_tile(P, hfunc, nullptr, box, nullptr, nullptr, u, ti0, tj0, lq, psi);
+}
+
+/* psi of the batched step (k_pre): the union of the windows from the
+ * centres xc, yc, then every tile meeting it, dealt round-robin to the
+ * 256-thread groups of workgroups blk = 0..nblk-1 (1024 threads each), with
+ * hfun(phi) inside each cell's window.  scr: 4 * ORG_PSI_LDS_GROUP + 16
+ * bytes of LDS. */
+static __device__ void psi_tiles_pre(const DevP &P, const int blk, const int nblk, const float *__restrict__ hfunc,
+                                     const float *__restrict__ phi, const int *__restrict__ box,
+                                     const int *__restrict__ xc, const int *__restrict__ yc,
+                                     float *__restrict__ psi, unsigned char *scr) {
+    int *su = (int *)(scr + 4 * ORG_PSI_LDS_GROUP);
+    const int g = threadIdx.x >> 8, ng = blockDim.x >> 8;
+    if (threadIdx.x < 64) {           /* union of the windows, wave 0 */
+        int a0 = P.N, a1 = 0, a2 = P.N, a3 = 0;
+        for (int n = threadIdx.x; n < P.nc; n += 64) {
+            const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
+            a0 = min(a0, x0); a1 = max(a1, x0 + ORG_W); a2 = min(a2, y0); a3 = max(a3, y0 + ORG_W);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            a0 = min(a0, __shfl_xor(a0, o, 64)); a1 = max(a1, __shfl_xor(a1, o, 64));
+            a2 = min(a2, __shfl_xor(a2, o, 64)); a3 = max(a3, __shfl_xor(a3, o, 64));
+        }
+        if (threadIdx.x == 0) { su[0] = a0; su[1] = a1; su[2] = a2; su[3] = a3; }
     }
     __syncthreads();
-    const size_t NN = (size_t)N * N;
-    {
-        const int e = threadIdx.x;
-        const int i = ti0 + (e >> 4), j = tj0 + (e & 15);
-        if (i >= N || j >= N || i < u0 || i >= u1 || j < v0 || j >= v1) return;
-        float acc = 0.0f;
-        for (int k = 0; k < cnt; k++) {
-            const int4 bx = lbox[k];
-            if (i >= bx.x && i < bx.y && j >= bx.z && j < bx.w) acc += hfunc[list[k] * NN + (size_t)i * N + j];
-        }
-        psi[(size_t)i * N + j] = acc;
+    const int4 u = make_int4(su[0], su[1], su[2], su[3]);
+    if (u.y <= u.x || u.w <= u.z) return;
+    const int tr0 = u.x >> 4, tc0 = u.z >> 4;
+    const int ntc = ((u.w - 1) >> 4) - tc0 + 1, ntile = (((u.y - 1) >> 4) - tr0 + 1) * ntc;
+    for (int base = blk * ng; base < ntile; base += nblk * ng) {
+        const int k = base + g;
+        int ti0 = -1, tj0 = -1;
+        if (k < ntile) { ti0 = (tr0 + k / ntc) * 16; tj0 = (tc0 + k % ntc) * 16; }
+        psi_tile(P, hfunc, phi, box, xc, yc, u, ti0, tj0, scr + g * ORG_PSI_LDS_GROUP, psi);
     }
 }
 
@@ -612,11 +694,51 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
  * the last iterate */
 struct LumState { int done, ri, cur, pad; };
 
-/* k_lumecm_update: update_lumen_and_ECM (cell_phase_functions.cpp:395-443),
- * then boundary_conditions_lumen_and_ECM (useful_functions.cpp:439-462).
- * Writes the first iterate of both reshaping loops into both ping-pong
- * buffers (sweeps rewrite only their active box), resets the loop states and
- * the normals' boxes. */
+/* update_lumen_and_ECM (cell_phase_functions.cpp:395-443) and
+ * boundary_conditions_lumen_and_ECM (useful_functions.cpp:439-462) on row i;
+ * the first iterate of both reshaping loops goes to both ping-pong buffers
+ * (sweeps rewrite only their active box). */
+static __device__ __forceinline__ void lumecm_row(const DevP &P, const int i, const float se,
+                                                  const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                  const float *__restrict__ psi, const float *__restrict__ hlum,
+                                                  const float *__restrict__ hecm, float *__restrict__ lumA,
+                                                  float *__restrict__ lumB, float *__restrict__ ecmA,
+                                                  float *__restrict__ ecmB) {
+    const int N = P.N;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
+    const size_t r = (size_t)i * N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const size_t e = r + j;
+        float ln, en;
+        if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
+            ln = 0.0f;       /* lumen edges and corners 0 */
+            en = 1.0f;       /* ECM edges and corners 1 (:449-460) */
+        } else {
+            float l = lum[e], E = ecm[e], ps = psi[e];
+            float lap = L9AT(psi, e, (size_t)N) * inv4;
+            ln = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
+                         (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
+            en = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
+                         (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
+        }
+        lumA[e] = ln; lumB[e] = ln;
+        ecmA[e] = en; ecmB[e] = en;
+    }
+}
+
+/* Loop states of both fields at the start of the reshaping loops. */
+static __device__ __forceinline__ void lum_reset_states(const DevP &P, LumState *__restrict__ lst) {
+    if (threadIdx.x < 2) {
+        const bool run = P.rdev0_lum > P.thr_lum;
+        lst[threadIdx.x].done = run ? 0 : 1;
+        lst[threadIdx.x].ri = 0;
+        lst[threadIdx.x].cur = 0;
+    }
+}
+
+/* k_lumecm_update: the lumen / ECM update of every row; resets the loop
+ * states, the loop barrier and the abort flag. */
 __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__restrict__ lum,
                                                        const float *__restrict__ ecm, const float *__restrict__ psi,
                                                        const float *__restrict__ hlum, const float *__restrict__ hecm,
@@ -624,50 +746,104 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
                                                        float *__restrict__ lumB, float *__restrict__ ecmA,
                                                        float *__restrict__ ecmB, LumState *__restrict__ lst,
                                                        int *__restrict__ abort_flag, unsigned *__restrict__ bar) {
-    const int N = P.N;
     if (blockIdx.x == 0 && threadIdx.x == 0) { *abort_flag = 0; bar[0] = 0u; }
-    if (blockIdx.x == 0 && threadIdx.x < 2) {
-        bool run = P.rdev0_lum > P.thr_lum;
-        lst[threadIdx.x].done = run ? 0 : 1;
-        lst[threadIdx.x].ri = 0;
-        lst[threadIdx.x].cur = 0;
-    }
+    if (blockIdx.x == 0) lum_reset_states(P, lst);
     const float se = (float)(*sum_ecm);
-    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
-    const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
 #pragma unroll 2
-    for (int i = blockIdx.x; i < N; i += gridDim.x) {
-        const size_t r = (size_t)i * N;
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
-            const size_t e = r + j;
-            float ln, en;
-            if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
-                ln = 0.0f;       /* lumen edges and corners 0 */
-                en = 1.0f;       /* ECM edges and corners 1 (:449-460) */
-            } else {
-                float l = lum[e], E = ecm[e], ps = psi[e];
-                float lap = L9AT(psi, e, (size_t)N) * inv4;
-                ln = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
-                             (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
-                en = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
-                             (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
-            }
-            lumA[e] = ln; lumB[e] = ln;
-            ecmA[e] = en; ecmB[e] = en;
-        }
-    }
+    for (int i = blockIdx.x; i < P.N; i += gridDim.x)
+        lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, lumA, lumB, ecmA, ecmB);
 }
 
 /* periodic ghost-ring source index of boundary_conditions_periodic /
  * boundary_conditions_cells (useful_functions.cpp:391-420, 505-526) */
 static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N - 2 : (i == N - 1 ? 1 : i); }
 
-/* k_lum_normals: |grad| of the old field (cell_phase_functions.cpp:722-741),
- * normals from the new field's gradient (:744-771) with the periodic ring,
- * and the bounding box of the nonzero normals (jbox, inclusive).  A sweep
- * changes a pixel only if the normal at it or at one of its four neighbours
- * is nonzero (every flux term carries a face average of the normals).
- * blockIdx.y + fbase: 0 lumen, 1 ECM. */
+/* Normals of one field on row i: |grad| of the old field
+ * (cell_phase_functions.cpp:722-741), normals from the new field's gradient
+ * (:744-771) with the periodic ring; grows (b0..b3) to the bounding box of
+ * the nonzero normals (inclusive).  A sweep changes a pixel only if the
+ * normal at it or at one of its four neighbours is nonzero (every flux term
+ * carries a face average of the normals). */
+static __device__ __forceinline__ void normals_row(const DevP &P, const int i, const float *__restrict__ old,
+                                                   const float *__restrict__ nw, float *__restrict__ jx,
+                                                   float *__restrict__ jy, int &b0, int &b1, int &b2, int &b3) {
+    const int N = P.N;
+    const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
+    const int si = ring_src(i, N);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const size_t g = (size_t)si * N + ring_src(j, N);
+        float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+        float ng = sqrtf(c * (a * a + b * b));
+        float vx = 0.0f, vy = 0.0f;
+        if (ng != 0.0f) {
+            vx = h * (nw[g + N] - nw[g - N]) / ng;
+            vy = h * (nw[g + 1] - nw[g - 1]) / ng;
+        }
+        const size_t e = (size_t)i * N + j;
+        jx[e] = vx; jy[e] = vy;
+        if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
+    }
+}
+
+/* Normals of field f on rows blk, blk + nblk, ...; the workgroup's box of
+ * nonzero normals goes to jbox_part[(f * nblk + blk) * 4 ..]. sb: 4 ints
+ * of LDS. */
+static __device__ void normals_rows(const DevP &P, const int f, const int blk, const int nblk,
+                                    const float *__restrict__ old, const float *__restrict__ nw,
+                                    float *__restrict__ jx, float *__restrict__ jy, int *__restrict__ jbox_part,
+                                    int *sb) {
+    const int N = P.N;
+    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+    __syncthreads();
+    int b0 = N, b1 = -1, b2 = N, b3 = -1;
+#pragma unroll 2
+    for (int i = blk; i < N; i += nblk) normals_row(P, i, old, nw, jx, jy, b0, b1, b2, b3);
+    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+    __syncthreads();
+    if (threadIdx.x < 4) jbox_part[((size_t)f * nblk + blk) * 4 + threadIdx.x] = sb[threadIdx.x];
+}
+
+/* Lumen/ECM update and normals of both fields on the row band [r0, r1) of
+ * workgroup blk.  The normals of a row read the updated fields on the rows
+ * around its ring source (ring_src(i) +- 1), so the workgroup updates those
+ * rows itself: the band, one halo row on each side and, for the first and
+ * last bands, the rows the periodic ring maps to.  Rows updated by two
+ * workgroups get the same values from both.  No synchronisation between
+ * workgroups is needed; the boxes of nonzero normals go to jbox_part. */
+static __device__ void lumecm_normals_band(const DevP &P, const int blk, const int nblk, const float se,
+                                           const float *__restrict__ lum, const float *__restrict__ ecm,
+                                           const float *__restrict__ psi, const float *__restrict__ hlum,
+                                           const float *__restrict__ hecm, float *__restrict__ L0,
+                                           float *__restrict__ L1, float *__restrict__ E0, float *__restrict__ E1,
+                                           float *__restrict__ jlx, float *__restrict__ jly, float *__restrict__ jex,
+                                           float *__restrict__ jey, int *__restrict__ jbox_part, int *sb) {
+    const int N = P.N;
+    const int r0 = (int)(((long long)blk * N) / nblk), r1 = (int)(((long long)(blk + 1) * N) / nblk);
+#pragma unroll 2
+    for (int i = max(r0 - 1, 0); i < min(r1 + 1, N); i++)
+        lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, L0, L1, E0, E1);
+    if (r0 == 0)
+        for (int i = max(N - 3, 0); i < N; i++) lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, L0, L1, E0, E1);
+    if (r1 == N)
+        for (int i = 0; i < min(3, N); i++) lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, L0, L1, E0, E1);
+    drain_stores();
+    __syncthreads();
+    for (int f = 0; f < 2; f++) {
+        const float *old = f ? ecm : lum, *nw = f ? E0 : L0;
+        float *jx = f ? jex : jlx, *jy = f ? jey : jly;
+        if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+        __syncthreads();
+        int b0 = N, b1 = -1, b2 = N, b3 = -1;
+#pragma unroll 2
+        for (int i = r0; i < r1; i++) normals_row(P, i, old, nw, jx, jy, b0, b1, b2, b3);
+        if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+        __syncthreads();
+        if (threadIdx.x < 4) jbox_part[((size_t)f * nblk + blk) * 4 + threadIdx.x] = sb[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+/* k_lum_normals: normals of field fbase + blockIdx.y (0 lumen, 1 ECM). */
 __global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__restrict__ lum_old,
                                                      const float *__restrict__ ecm_old, const float *__restrict__ lumn,
                                                      const float *__restrict__ ecmn, float *__restrict__ jlx,
@@ -675,34 +851,9 @@ __global__ __launch_bounds__(1024) void k_lum_normals(DevP P, const float *__res
                                                      float *__restrict__ jey, int *__restrict__ jbox_part,
                                                      int fbase) {
     __shared__ int sb[4];
-    const int N = P.N, f = fbase + blockIdx.y;
-    const float *old = f ? ecm_old : lum_old;
-    const float *nw = f ? ecmn : lumn;
-    float *jx = f ? jex : jlx, *jy = f ? jey : jly;
-    const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
-    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
-    __syncthreads();
-    int b0 = N, b1 = -1, b2 = N, b3 = -1;
-#pragma unroll 2
-    for (int i = blockIdx.x; i < N; i += gridDim.x) {
-        const int si = ring_src(i, N);
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
-            const size_t g = (size_t)si * N + ring_src(j, N);
-            float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
-            float ng = sqrtf(c * (a * a + b * b));
-            float vx = 0.0f, vy = 0.0f;
-            if (ng != 0.0f) {
-                vx = h * (nw[g + N] - nw[g - N]) / ng;
-                vy = h * (nw[g + 1] - nw[g - 1]) / ng;
-            }
-            const size_t e = (size_t)i * N + j;
-            jx[e] = vx; jy[e] = vy;
-            if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
-        }
-    }
-    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
-    __syncthreads();
-    if (threadIdx.x < 4) jbox_part[((size_t)f * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = sb[threadIdx.x];
+    const int f = fbase + blockIdx.y;
+    normals_rows(P, f, blockIdx.x, gridDim.x, f ? ecm_old : lum_old, f ? ecmn : lumn, f ? jex : jlx, f ? jey : jly,
+                 jbox_part, sb);
 }
 
 /* Bounding box (inclusive) of field f's nonzero normals from the
@@ -802,15 +953,6 @@ static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *_
     return acc;
 }
 
-/* Sum of one field's per-workgroup deviations in a fixed order, valid in
- * every thread (1024 threads, nblk <= 1024). */
-static __device__ __forceinline__ double sum_parts(const double *part, int nblk, double *red) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-        v += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return block_sum_d(v, red);
-}
-
 /* Stop test after a sweep (cell_phase_functions.cpp:774, 862-868). */
 static __device__ __forceinline__ void lum_stop(const DevP &P, LumState &s, double rdev, int *err) {
     s.cur = (s.ri + 1) & 1;
@@ -861,52 +1003,54 @@ __global__ __launch_bounds__(1024) void k_lum_iter(DevP P, float *__restrict__ b
     }
 }
 
-/* Barrier among the `nblk` loop workgroups of one launch (generation
- * counter, sleeping poll).  Only loop workgroups ever wait, and only on each
- * other, and every other workgroup of the launch finishes on its own, so all
- * loop workgroups become resident eventually.  A wait longer than
- * `limit` wall-clock ticks sets ORG_ERELAX and the abort flag instead of
- * waiting on. */
-static __device__ bool lum_barrier(unsigned *bar, int nblk, int *abort_flag, int *err, long long limit) {
+/* Barrier among the `nblk` lumen/ECM workgroups of k_step.  bar[0] counts
+ * arrivals over the whole launch (k_post clears it); the e-th barrier of a
+ * workgroup releases once it reaches e * nblk, so no reset is needed between
+ * barriers.  Every wave drains its stores, thread 0 releases (agent scope),
+ * arrives, polls with a sleep and acquires.  The workgroups that wait here
+ * wait only on each other and every other workgroup of the launch finishes
+ * on its own, so all of them become resident.  A wait longer than `limit`
+ * wall-clock ticks, or one that finds the abort flag raised, raises the abort
+ * flag and ORG_ETIMEOUT and returns false instead of waiting on. */
+static __device__ bool lum_barrier(unsigned *bar, int nblk, int &epoch, int *abort_flag, int *err, long long limit) {
     __shared__ int ok;
+    epoch++;
+    drain_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
-        ok = 1;
-        unsigned g = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned arrived = atomicAdd(&bar[0], 1u);
-        if (arrived == (unsigned)nblk - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            atomicAdd(&bar[1], 1u);
-        } else {
-            long long t0 = wall_clock64();
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(2);
-                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                    wall_clock64() - t0 > limit) {
-                    atomicExch(abort_flag, 1);
-                    set_err(err, ORG_ERELAX);
-                    ok = 0;
-                    break;
-                }
+        int good = 1;
+        const unsigned target = (unsigned)epoch * (unsigned)nblk;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned seen = atomicAdd(&bar[0], 1u) + 1u;
+        const long long t0 = wall_clock64();
+        while (seen < target) {
+            __builtin_amdgcn_s_sleep(2);
+            seen = __hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen >= target) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > limit) {
+                atomicExch(abort_flag, 1);
+                set_err(err, ORG_ETIMEOUT);
+                good = 0;
+                break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = good;
     }
     __syncthreads();
     return ok != 0;
 }
 
 /* Both reshaping loops (lumen and ECM) run to their stop tests by the loop
- * workgroups of k_cortex_cells, one barrier per sweep.  Every loop workgroup
+ * workgroups of k_step, one barrier per sweep.  Every loop workgroup
  * sums the per-workgroup deviations in the same order, so all take the same
  * stop decision after the same sweep.  part: [2 parities][2 fields][nblk]. */
 static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *red, float *bufL0, float *bufL1,
                                      float *bufE0, float *bufE1, const float *jlx, const float *jly,
                                      const float *jex, const float *jey, const int *jbox_part, int nparts,
-                                     LumState *lst, double *part, unsigned *bar, int *abort_flag, int *err,
-                                     long long limit) {
+                                     LumState *lst, double *part, unsigned *bar, int &epoch, int *abort_flag,
+                                     int *err, long long limit) {
     LumState s[2] = {lst[0], lst[1]};
     const int4 jb[2] = {reduce_jbox(jbox_part, nparts, 0, P.N), reduce_jbox(jbox_part, nparts, 1, P.N)};
     for (int k = 0; !(s[0].done && s[1].done); k++) {
@@ -922,7 +1066,7 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
             double tot = block_sum_d(acc, red);
             if (threadIdx.x == 0) pk[f * nblk + blk] = tot;
         }
-        if (!lum_barrier(bar, nblk, abort_flag, err, limit)) return;
+        if (!lum_barrier(bar, nblk, epoch, abort_flag, err, limit)) return;
         for (int f = 0; f < 2; f++) {
             if (s[f].done) continue;
             double rdev = sum_parts(pk + f * nblk, nblk, red);
@@ -985,15 +1129,15 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
                                                      float2 *__restrict__ tw, unsigned char *__restrict__ lab,
                                                      const float *__restrict__ phi,
                                                      const float *__restrict__ rho, const float *__restrict__ lum,
-                                                     const float *__restrict__ ecm, const int *__restrict__ xc,
-                                                     const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                     const float *__restrict__ ecm, const int x0, const int y0,
+                                                     float2 *__restrict__ scratch,
                                                      float *__restrict__ rho_box, float *__restrict__ vxbox,
                                                      float *__restrict__ vybox, int write_v) {
+    /* (x0, y0): FFT box origin = centre - zone_size + 1, useful_functions.cpp:618-619 */
     const int t = threadIdx.x, N = P.N;
     constexpr int MM = ORG_MB * ORG_MB;
     constexpr int NPAIR = 63 * ORG_MB + 2 * 65;      /* mode pairs {k, -k} incl. 4 self-paired modes */
     const size_t NN = (size_t)N * N;
-    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;   /* FFT box origin, useful_functions.cpp:618-619 */
     const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
     if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
     const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
@@ -1007,7 +1151,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float MMf = (float)MM;
 
     /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
-#pragma unroll 2
+#pragma unroll 4
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const size_t g = (size_t)(x0 + i) * N + (y0 + j);
@@ -1036,7 +1180,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-#pragma unroll 2
+#pragma unroll 4
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, l = (e >> 7) * ORG_LS + (e & 127);
         SA[e] = sm[l];
@@ -1071,7 +1215,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     __syncthreads();
     fft2d_128<1>(sm, tw);
     /* P5: v = Re(.)/M^2 ; rho * v (rho_functions.cpp:267-284) */
-#pragma unroll 2
+#pragma unroll 4
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         float2 z = sm[i * ORG_LS + j];
@@ -1109,7 +1253,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     __syncthreads();
     fft2d_128<1>(sm, tw);
     /* P7: rho^{n+1} on the box (copy_to_rhon, rho_functions.cpp:253-265) */
-#pragma unroll 2
+#pragma unroll 4
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q;
         rbox[e] = rbox[e] + sm[(e >> 7) * ORG_LS + (e & 127)].x / MMf;
@@ -1125,88 +1269,93 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
     __shared__ float2 sm[ORG_MB * ORG_LS];
     __shared__ float2 tw[128];
     __shared__ unsigned char lab[ORG_MB * ORG_MB];
-    spectral_body(P, blockIdx.x, sm, tw, lab, phi, rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
+    const int n = blockIdx.x;
+    spectral_body(P, n, sm, tw, lab, phi, rho, lum, ecm, xc[n] - P.zs + 1, yc[n] - P.zs + 1, scratch, rho_box,
+                  vxbox, vybox, write_v);
 }
 
 /* ======================================================================== */
-/* k_cells: update_all_cells (cell_phase_functions.cpp:126-217, mode&1) and
- * reshaping_cells (cell_phase_functions.cpp:447-580, mode&2) per cell, one
- * 1024-thread workgroup per cell.  The relaxed field (window plus a
+/* Phase field of one cell, one 1024-thread workgroup per cell:
+ * update_all_cells (cell_phase_functions.cpp:126-217) and reshaping_cells
+ * (cell_phase_functions.cpp:447-580).  The relaxed field (window plus a
  * one-pixel ring) and the iterate live in LDS, the normals (from the
  * start-of-step phi, useful_functions.cpp:356-388) in a per-cell scratch;
- * the stop test runs in the workgroup. */
-static __device__ __forceinline__ void cells_body(const DevP &P, const int n, float *__restrict__ phase,
-                                                  float *__restrict__ pnl, double *__restrict__ red,
-                                                  float *__restrict__ phi, const float *__restrict__ rho,
-                                                  const float *__restrict__ hfunc, const float *__restrict__ psi,
-                                                  const float *__restrict__ hlum, const float *__restrict__ hecm,
-                                                  const float *__restrict__ f0, const int *__restrict__ xc,
-                                                  const int *__restrict__ yc, float *__restrict__ phin_win,
-                                                  float *__restrict__ phi_win, float *__restrict__ jwin,
-                                                  int *__restrict__ counts, int *__restrict__ err, int mode) {
+ * the stop test runs in the workgroup.  (x0, y0): window origin. */
+
+/* Phase-field update on the window; result to the LDS iterate pnl (row
+ * stride W + 2, one-pixel ring) or, with pnl == nullptr, to pw.  psi at
+ * window pixel (i, j) is ps0[i * pst + j] (the psi plane or a copy of the
+ * window plus ring in LDS). */
+static __device__ void cells_update(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
+                                    const float *__restrict__ rh, const float *__restrict__ hf,
+                                    const float *__restrict__ ps0, const int pst, const float *__restrict__ hlum,
+                                    const float *__restrict__ hecm, const float fn, float *__restrict__ pnl,
+                                    float *__restrict__ pw) {
+    constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
+    const int N = P.N;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
+    const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
+    for (int e = threadIdx.x; e < NPX; e += 1024) {
+        const int i = e / W, j = e - i * W;
+        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+        float v = ph[g];
+        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        float out = v;
+        if (ng > 0.0f) {
+            float d = v - 0.5f;
+            out -= cplc * expf(-d * d * isig2) *
+                   ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) / ng;
+        }
+        /* temp_phase = psi - hfunc_n (cell_phase_functions.cpp:150) on the 3x3 stencil */
+        const float *q = ps0 + i * pst + j;
+        float tc = q[0] - hf[g];
+        float tp0 = q[pst] - hf[g + N], tm0 = q[-pst] - hf[g - N];
+        float t0p = q[1] - hf[g + 1], t0m = q[-1] - hf[g - 1];
+        float tpp = q[pst + 1] - hf[g + N + 1], tpm = q[pst - 1] - hf[g + N - 1];
+        float tmm = q[-pst - 1] - hf[g - N - 1], tmp = q[-pst + 1] - hf[g - N + 1];
+        float lt = -12.0f * tc + 2.0f * (tp0 + tm0 + t0p + t0m) + tpp + tpm + tmm + tmp;
+        float ll = L9AT(hlum, g, (size_t)N), le = L9AT(hecm, g, (size_t)N);
+        out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
+               (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * hlum[g] + P.eta_lc * ll * inv4
+                - P.beta_ec * hecm[g] + P.eta_ec * le * inv4);
+        if (pnl) pnl[(i + 1) * WH + (j + 1)] = out;
+        else pw[e] = out;
+    }
+}
+
+/* Normals on the window plus ring from the start-of-step phi (to jx, jy,
+ * row stride W + 2) and the ring of the relaxed field (to phase). */
+static __device__ void cells_normals(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
+                                     float *__restrict__ jx, float *__restrict__ jy, float *__restrict__ phase) {
+    constexpr int W = ORG_W, WH = ORG_WH;
+    const int N = P.N;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
+    for (int e = threadIdx.x; e < WH * WH; e += 1024) {
+        const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
+        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        float a = 0.0f, b = 0.0f;
+        if (ng > 0.0f) { a = h * dxp / ng; b = h * dyp / ng; }
+        jx[e] = a; jy[e] = b;
+        if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = ph[g];
+    }
+}
+
+/* Reshaping loop until the deviation test passes (cell_phase_functions.cpp:
+ * 470-575); phi <- second-to-last iterate (still in phase), phin <- last.
+ * With pwin the window goes to a per-cell buffer that the copy-back writes
+ * to phi, so other workgroups of the same launch still read the
+ * start-of-step phi.  Expects pnl and the phase ring written and a barrier
+ * after them. */
+static __device__ void cells_relax(const DevP &P, const int n, const int x0, const int y0, float *__restrict__ ph,
+                                   float *__restrict__ phase, float *__restrict__ pnl, const float *__restrict__ jx,
+                                   const float *__restrict__ jy, double *__restrict__ red, float *__restrict__ pw,
+                                   float *__restrict__ pwin, int *__restrict__ counts, int *__restrict__ err) {
     constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
     const int t = threadIdx.x, N = P.N;
-    const size_t NN = (size_t)N * N;
-    const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
-    float *ph = phi + n * NN;
-    const float *rh = rho + n * NN, *hf = hfunc + n * NN;
-    float *pw = phin_win + (size_t)n * NPX;
-
-    if (mode & 1) {
-        const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
-        const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
-        const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
-        const float fn = f0[n];
-        for (int e = t; e < NPX; e += 1024) {
-            const int i = e / W, j = e - i * W;
-            const size_t g = (size_t)(x0 + i) * N + (y0 + j);
-            float v = ph[g];
-            float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-            float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
-            float out = v;
-            if (ng > 0.0f) {
-                float d = v - 0.5f;
-                out -= cplc * expf(-d * d * isig2) *
-                       ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) / ng;
-            }
-            /* temp_phase = psi - hfunc_n (cell_phase_functions.cpp:150) on the 3x3 stencil */
-            float tc = psi[g] - hf[g];
-            float tp0 = psi[g + N] - hf[g + N], tm0 = psi[g - N] - hf[g - N];
-            float t0p = psi[g + 1] - hf[g + 1], t0m = psi[g - 1] - hf[g - 1];
-            float tpp = psi[g + N + 1] - hf[g + N + 1], tpm = psi[g + N - 1] - hf[g + N - 1];
-            float tmm = psi[g - N - 1] - hf[g - N - 1], tmp = psi[g - N + 1] - hf[g - N + 1];
-            float lt = -12.0f * tc + 2.0f * (tp0 + tm0 + t0p + t0m) + tpp + tpm + tmm + tmp;
-            float ll = L9AT(hlum, g, (size_t)N), le = L9AT(hecm, g, (size_t)N);
-            out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
-                   (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * hlum[g] + P.eta_lc * ll * inv4
-                    - P.beta_ec * hecm[g] + P.eta_ec * le * inv4);
-            if (mode & 2) pnl[(i + 1) * WH + (j + 1)] = out;
-            else pw[e] = out;
-        }
-        if (!(mode & 2)) return;
-    } else {
-        for (int e = t; e < NPX; e += 1024) {
-            const int i = e / W, j = e - i * W;
-            pnl[(i + 1) * WH + (j + 1)] = pw[e];
-        }
-    }
-
-    /* normals on the window plus ring, from the start-of-step phi */
-    float *jx = jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
-    {
-        const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
-        for (int e = t; e < WH * WH; e += 1024) {
-            const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
-            const size_t g = (size_t)(x0 + i) * N + (y0 + j);
-            float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-            float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
-            float a = 0.0f, b = 0.0f;
-            if (ng > 0.0f) { a = h * dxp / ng; b = h * dyp / ng; }
-            jx[e] = a; jy[e] = b;
-            if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = ph[g];
-        }
-    }
-    __syncthreads();
     const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
     const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
     const float dscale = P.dx * P.dx / P.ncur;
@@ -1231,11 +1380,6 @@ static __device__ __forceinline__ void cells_body(const DevP &P, const int n, fl
         if (ri > P.rimax) { if (t == 0) set_err(err, ORG_ERELAX); break; }
         ri++;
     }
-    /* phi <- second-to-last iterate (still in phase); phin keeps the last.
-     * With phi_win the window goes to a per-cell buffer that k_commit_cells
-     * writes back, so other cells' workgroups of the same launch still read
-     * the start-of-step phi. */
-    float *pwin = phi_win ? phi_win + (size_t)n * NPX : nullptr;
     for (int e = t; e < NPX; e += 1024) {
         const int i = e / W, j = e - i * W, l = (i + 1) * WH + (j + 1);
         if (pwin) pwin[e] = phase[l];
@@ -1245,6 +1389,8 @@ static __device__ __forceinline__ void cells_body(const DevP &P, const int n, fl
     if (t == 0) counts[n] = cnt;
 }
 
+/* k_cells: update_all_cells (mode&1) and / or reshaping_cells (mode&2) of
+ * every cell; mode 1 leaves phin in phin_win, mode 2 starts from it. */
 __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
                                                 const float *__restrict__ hfunc, const float *__restrict__ psi,
                                                 const float *__restrict__ hlum, const float *__restrict__ hecm,
@@ -1255,59 +1401,142 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
     __shared__ float phase[ORG_WH * ORG_WH];
     __shared__ float pnl[ORG_WH * ORG_WH];
     __shared__ double red[16];
-    cells_body(P, blockIdx.x, phase, pnl, red, phi, rho, hfunc, psi, hlum, hecm, f0, xc, yc, phin_win, nullptr, jwin,
-               counts, err, mode);
+    constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
+    const int n = blockIdx.x;
+    const size_t NN = (size_t)P.N * P.N;
+    const int x0 = xc[n] - P.zs, y0 = yc[n] - P.zs;
+    float *ph = phi + n * NN;
+    float *pw = phin_win + (size_t)n * NPX;
+    if (mode & 1) {
+        cells_update(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi + (size_t)x0 * P.N + y0, P.N, hlum, hecm, f0[n],
+                     (mode & 2) ? pnl : nullptr, pw);
+        if (!(mode & 2)) return;
+    } else {
+        for (int e = threadIdx.x; e < NPX; e += 1024) {
+            const int i = e / W, j = e - i * W;
+            pnl[(i + 1) * WH + (j + 1)] = pw[e];
+        }
+    }
+    float *jx = jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
+    cells_normals(P, x0, y0, ph, jx, jy, phase);
+    __syncthreads();
+    cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, pw, nullptr, counts, err);
 }
 
-/* Buffers of the lumen/ECM loop role of k_cortex_cells. */
-struct LumBufs {
-    float *L0, *L1, *E0, *E1;
-    const float *jlx, *jly, *jex, *jey;
-    const int *jbox_part;
-    int nparts;
+/* Arguments of k_pre, k_step and k_post. */
+struct StepArgs {
+    /* per-cell planes and scalars; xn, yn, bn: centres and support boxes of
+     * the next step, from k_post (or k_centroid before the first step) */
+    float *phi, *rho, *hfunc;
+    int *xc, *yc, *box, *xn, *yn, *bn, *uni, *counts;
+    const double *s0;
+    double *s;
+    float *f0;
+    /* shared planes */
+    float *lum, *ecm;
+    float *psi, *hlum, *hecm;
+    double *part_h, *sum_ecm;
+    int nvb;                 /* row blocks of the h(lum), h(ECM) pass (as k_hlum's grid) */
+    int npsi;                /* psi workgroups of k_pre */
+    /* lumen / ECM loops */
+    float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
+    int *jbox_part;
     LumState *lst;
-    double *part;
+    double *part_lc;
+    /* cortex */
+    float2 *spec;
+    float *rho_box, *vxbox, *vybox;
+    int write_v;
+    /* phase field */
+    float *phin_win, *phi_win, *jwin;
+    /* barrier of the lumen/ECM workgroups, abort flag */
     unsigned *bar;
-    int *abort_flag;
+    int *abort_flag, *err;
     long long limit;
 };
 
-/* k_cortex_cells: one launch for the rest of the step.
- *   workgroups 0..ncell-1         spectral cortex update of cell n
- *   workgroups ncell..2*ncell-1   phase-field update + reshaping of cell n
- *   workgroups 2*ncell..          the lumen and ECM reshaping loops
- * The three roles read only the start-of-step state (the cell roles read
- * lum/ECM; the loop role writes only its ping-pong buffers), so they run
- * concurrently; phi's new window goes to phi_win.  LDS is shared between
- * the roles. */
+/* k_pre: the first part of a timestep (run_with_active_gels.cpp:123-142),
+ * workgroups in three independent roles:
+ *   0 .. ncell-1       cell n: centre and support box from (xn, yn, bn),
+ *                      hfunc on the window and the FP64 area
+ *                      (compute_hfunc_3d_tables_kernels_loop_n, sum_s_kernels)
+ *   next npsi          psi on the union of the windows (compute_psi_kernels),
+ *                      with hfun(phi) inside each window
+ *   last nvb           h(lum), h(ECM) and the per-block sums of h(ECM)
+ *                      (compute_all_hfunc_2d_tables_kernels) with k_hlum's
+ *                      row blocks */
+#define ORG_LDS_PRE (4 * ORG_PSI_LDS_GROUP + 16)
+__global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
+    const int b = blockIdx.x;
+    if (b < P.nc) {
+        cell_prep(P, b, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
+    } else if (b < P.nc + a.npsi) {
+        psi_tiles_pre(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.psi, scr);
+    } else {
+        const int vb = b - P.nc - a.npsi;
+        const double tot = block_sum_d(hlum_rows(P, vb, a.nvb, a.lum, a.ecm, a.hlum, a.hecm), (double *)scr);
+        if (threadIdx.x == 0) a.part_h[vb] = tot;
+    }
+}
+
+/* k_step: the rest of the timestep up to the copy-back
+ * (run_with_active_gels.cpp:130-172) in one launch, workgroups in three
+ * roles:
+ *   0 .. ncell-1       spectral cortex update of cell n (update_all_cortices
+ *                      with its turnover labels, normals and prod_norm_delta)
+ *   next ORG_LUM_BLOCKS  lumen/ECM: update_lumen_and_ECM and the normals of
+ *                      reshaping_lumen_or_ECM per row band, barrier, both
+ *                      reshaping loops to their stop tests (one barrier per
+ *                      sweep)
+ *   last ncell         phase field of cell n: update_all_cells and
+ *                      reshaping_cells; the new window goes to phi_win
+ * Every role reads only the state at the start of the launch and its own
+ * results; only the lumen/ECM workgroups synchronise, among themselves.  The
+ * copy-back (k_post) runs after the launch.  The FP sums take the same
+ * operands in the same order as the per-function kernels, so both forms
+ * agree bit for bit. */
 #define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8 + ORG_MB * ORG_MB)
 #define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8)
 #define ORG_LDS_FUSED (ORG_LDS_SPECTRAL > ORG_LDS_CELLS ? ORG_LDS_SPECTRAL : ORG_LDS_CELLS)
-__global__ __launch_bounds__(1024) void k_cortex_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
-                                                       const float *__restrict__ lum, const float *__restrict__ ecm,
-                                                       const float *__restrict__ hfunc, const float *__restrict__ psi,
-                                                       const float *__restrict__ hlum, const float *__restrict__ hecm,
-                                                       const float *__restrict__ f0, const int *__restrict__ xc,
-                                                       const int *__restrict__ yc, float2 *__restrict__ scratch,
-                                                       float *__restrict__ rho_box, float *__restrict__ vxbox,
-                                                       float *__restrict__ vybox, int write_v,
-                                                       float *__restrict__ phin_win, float *__restrict__ phi_win,
-                                                       float *__restrict__ jwin, int *__restrict__ counts,
-                                                       int *__restrict__ err, LumBufs lb) {
+__global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
-    if ((int)blockIdx.x >= 2 * P.nc) {
-        lum_loop_body(P, blockIdx.x - 2 * P.nc, gridDim.x - 2 * P.nc, (double *)lds, lb.L0, lb.L1, lb.E0, lb.E1,
-                      lb.jlx, lb.jly, lb.jex, lb.jey, lb.jbox_part, lb.nparts, lb.lst, lb.part, lb.bar, lb.abort_flag,
-                      err, lb.limit);
-    } else if ((int)blockIdx.x < P.nc) {
+    const int nc = P.nc, b = blockIdx.x;
+    const size_t NN = (size_t)P.N * P.N;
+    if (b < nc) {
         float2 *sm = (float2 *)lds;
-        spectral_body(P, blockIdx.x, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), phi,
-                      rho, lum, ecm, xc, yc, scratch, rho_box, vxbox, vybox, write_v);
+        spectral_body(P, b, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), a.phi, a.rho,
+                      a.lum, a.ecm, a.xc[b] - P.zs + 1, a.yc[b] - P.zs + 1, a.spec, a.rho_box, a.vxbox, a.vybox,
+                      a.write_v);
+    } else if (b < nc + ORG_LUM_BLOCKS) {
+        const int blk = b - nc, nblk = ORG_LUM_BLOCKS;
+        double *red = (double *)lds;
+        int *sb = (int *)(lds + 128);
+        int epoch = 0;
+        const double se = sum_parts(a.part_h, a.nvb, red);
+        if (blk == 0) {
+            if (threadIdx.x == 0) *a.sum_ecm = se;
+            lum_reset_states(P, a.lst);
+        }
+        lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0, a.E1,
+                            a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
+        if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
+        lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
+                      a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit);
     } else {
-        float *phase = (float *)lds;
-        double *red = (double *)(lds + 2 * ORG_WH * ORG_WH * 4);
-        cells_body(P, blockIdx.x - P.nc, phase, phase + ORG_WH * ORG_WH, red, phi, rho, hfunc, psi, hlum, hecm, f0,
-                   xc, yc, phin_win, phi_win, jwin, counts, err, 3);
+        constexpr int WH = ORG_WH, NPX = ORG_W * ORG_W;
+        const int n = b - nc - ORG_LUM_BLOCKS;
+        float *phase = (float *)lds, *pnl = phase + WH * WH;
+        double *red = (double *)(lds + 2 * WH * WH * 4);
+        const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
+        float *ph = a.phi + n * NN;
+        float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
+        cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi + (size_t)x0 * P.N + y0, P.N, a.hlum,
+                     a.hecm, a.f0[n], pnl, nullptr);
+        cells_normals(P, x0, y0, ph, jx, jy, phase);
+        __syncthreads();
+        cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
+                    a.phi_win + (size_t)n * NPX, a.counts, a.err);
     }
 }
 
@@ -1326,7 +1555,7 @@ static __device__ __forceinline__ void commit_cell_body(const DevP &P, const int
 #pragma unroll 4
     for (int e = threadIdx.x; e < M * M; e += blockDim.x)
         rh[(size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127))] = rb[e];
-    if (phi_win) {        /* phi window from k_cortex_cells (reshaping_cells' phase) */
+    if (phi_win) {        /* phi window from k_step (reshaping_cells' phase) */
         const int wx = x0 - 1, wy = y0 - 1;
         const float *pw = phi_win + (size_t)n * W * W;
         float *ph = phi + n * NN;
@@ -1345,21 +1574,24 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
     commit_cell_body(P, blockIdx.x, rho, rho_box, phi, phi_win, xc, yc);
 }
 
-/* k_post: rho/phi copy-back of every cell (workgroups 0..ncell-1) and
- * lum <- lumn, ECM <- ECMn (the other workgroups) in one launch. */
-__global__ __launch_bounds__(1024) void k_post(DevP P, float *__restrict__ rho, const float *__restrict__ rho_box,
-                                               float *__restrict__ phi, const float *__restrict__ phi_win,
-                                               const int *__restrict__ xc, const int *__restrict__ yc,
-                                               float *__restrict__ lum, float *__restrict__ ecm,
-                                               const float *__restrict__ bufL0, const float *__restrict__ bufL1,
-                                               const float *__restrict__ bufE0, const float *__restrict__ bufE1,
-                                               const LumState *__restrict__ lst, int *__restrict__ err,
-                                               int *__restrict__ uni) {
-    if ((int)blockIdx.x < P.nc)
-        commit_cell_body(P, blockIdx.x, rho, rho_box, phi, phi_win, xc, yc);
-    else
-        commit_lumecm_body(P, blockIdx.x - P.nc, gridDim.x - P.nc, lum, ecm, bufL0, bufL1, bufE0, bufE1, lst, err,
-                           uni);
+/* k_post: the copy-back of a timestep (run_with_active_gels.cpp:172-182):
+ * rho box and phi window of every cell (workgroups 0..ncell-1), then that
+ * cell's centre and support box for the next step (find_centres_cells of
+ * the next iteration, useful_functions.cpp:193-227) into (xn, yn, bn);
+ * lum <- lumn, ECM <- ECMn (the other workgroups).  Clears k_step's barrier
+ * counter and abort flag. */
+__global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
+    __shared__ long long red[48];
+    const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    if (b < P.nc) {
+        commit_cell_body(P, b, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
+        drain_stores();
+        __syncthreads();
+        cell_centroid(P, b, a.phi, a.xc, a.yc, a.box, a.xn, a.yn, a.bn, a.err, red);
+    } else {
+        commit_lumecm_body(P, b - P.nc, gridDim.x - P.nc, a.lum, a.ecm, a.L0, a.L1, a.E0, a.E1, a.lst, a.err, a.uni);
+    }
 }
 
 /* k_ring_rho: periodic ghost ring of every rho plane (the

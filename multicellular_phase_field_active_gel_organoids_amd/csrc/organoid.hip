@@ -28,6 +28,10 @@ int note_error(int code) {
 
 #define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
 
+/* steps per replay of the batched step's graph (the launch of a graph costs
+ * more than the gaps between its kernels) */
+#define ORG_GRAPH_STEPS 8
+
 }  // namespace
 
 struct org_ctx {
@@ -56,6 +60,9 @@ struct org_ctx {
     int *xn = nullptr, *yn = nullptr, *bn = nullptr;  /* next batched step's centres and support boxes */
     bool next_ready = false;               /* xn, yn, bn hold the centres of the current phi */
     int npsi = 128;                        /* psi workgroups of k_pre */
+    int nh = 0;                            /* h(lum), h(ECM) workgroups of k_pre */
+    int nparts_h = 0;                      /* parts of the sum of h(ECM): 64 x 64 tiles + the ring */
+    int *live = nullptr, *live_part = nullptr;  /* live boxes of lum and ECM, k_post's partial boxes */
     bool ring_pending = true;              /* rho ghost ring not yet written since org_prepare */
     int lum_grid = ORG_LUM_BLOCKS;         /* workgroups per field of a reshaping sweep */
     double *s = nullptr, *s0 = nullptr, *sum_ecm = nullptr, *part_h = nullptr, *part_l = nullptr;
@@ -64,8 +71,8 @@ struct org_ctx {
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; one step */
+    hipGraph_t graph[2] = {nullptr, nullptr};
     /* profiling */
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
@@ -185,6 +192,16 @@ int L_centroid(org_ctx *c) {
                        c->yc, c->box, c->uni, c->err);
     return launch_check();
 }
+/* the lumen/ECM state of the batched step from the current fields (see
+ * k_live_init) */
+int L_live_init(org_ctx *c) {
+    hipLaunchKernelGGL(k_live_reset, dim3(1), dim3(256), 0, c->stream, c->live, c->live_part, c->grid_g, c->N);
+    TRY(launch_check());
+    hipLaunchKernelGGL(k_live_init, dim3(c->nparts_h), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum,
+                       c->hecm, c->part_h, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey,
+                       c->live);
+    return launch_check();
+}
 /* centres and support boxes for the next batched step, from the current
  * phi (what k_post computes at the end of every batched step) */
 int L_centroid_next(org_ctx *c) {
@@ -201,7 +218,7 @@ int L_prep(org_ctx *c, int mode) {
 }
 int L_hlum(org_ctx *c) {
     Prof pr(c, ORG_K_LUMECM);
-    hipLaunchKernelGGL(k_hlum, dim3(c->grid_g), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum, c->hecm,
+    hipLaunchKernelGGL(k_hlum, dim3(c->nparts_h), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum, c->hecm,
                        c->part_h, c->counters + 2, c->sum_ecm);
     return launch_check();
 }
@@ -261,7 +278,8 @@ StepArgs step_args(org_ctx *c) {
     a.xn = c->xn; a.yn = c->yn; a.bn = c->bn; a.npsi = c->npsi;
     a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
     a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
-    a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nvb = c->grid_g;
+    a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nh = c->nh;
+    a.live = c->live; a.live_part = c->live_part; a.nlp = c->grid_g;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
     a.jlx = c->jlx; a.jly = c->jly; a.jex = c->jex; a.jey = c->jey;
     a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc;
@@ -273,7 +291,7 @@ StepArgs step_args(org_ctx *c) {
 /* centroids, hfunc, areas and h(lum), h(ECM) of the step (see k_pre) */
 int L_pre(org_ctx *c) {
     Prof pr(c, ORG_K_CENTRES);
-    hipLaunchKernelGGL(k_pre, dim3(c->nc + c->npsi + c->grid_g), dim3(1024), 0, c->stream, c->dp, step_args(c));
+    hipLaunchKernelGGL(k_pre, dim3(c->nc + c->npsi + c->nh), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
 /* the rest of the step up to the copy-back, one launch (see k_step) */
@@ -364,6 +382,11 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
     c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
     c->grid_g = std::min(c->N, 128);
+    {
+        const int t = (c->N + 63) / 64;
+        c->nparts_h = t * t + 1;
+        c->nh = std::min(c->nparts_h, 128);
+    }
     int nt = (c->N + 15) / 16;
     c->ntiles = nt * nt;
     int rc = ORG_OK;
@@ -381,8 +404,9 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
-    A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc); A(&c->counts, c->nc); A(&c->err, 1);
-    A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, 2048); A(&c->part_l, 2 * 2048);
+    A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);
+    A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->grid_g); A(&c->counts, c->nc); A(&c->err, 1);
+    A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, (size_t)c->nparts_h); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
     if (!rc) {
@@ -415,8 +439,10 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
 int org_destroy(org_ctx *c) {
     if (!c) return ORG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    }
     for (int k = 0; k < ORG_K_COUNT; k++)
         for (auto &pr : c->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (void *q : c->allocs) (void)hipFree(q);
@@ -449,6 +475,7 @@ int org_set_field(org_ctx *c, int field, const float *host) {
     HIPCHK(hipMemcpyAsync(d, host, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (field == ORG_F_PHI || field == ORG_F_HFUNC || field == ORG_F_RHO) c->prepared = false;
+    c->next_ready = false;
     return ORG_OK;
 }
 
@@ -544,8 +571,10 @@ int org_set_option(org_ctx *c, int option, int value) {
 
     default: return note_error(ORG_EINVAL);
     }
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+        if (c->graph[k]) { (void)hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }
+    }
     return ORG_OK;
 }
 
@@ -590,7 +619,10 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
     if (nsteps == 0) { c->next_ready = ready; return ORG_OK; }
-    if (!ready) TRY(L_centroid_next(c));   /* k_post provides them after every batched step */
+    if (!ready) {                          /* k_post keeps both up to date after every batched step */
+        TRY(L_centroid_next(c));
+        TRY(L_live_init(c));
+    }
     c->next_ready = true;
     if (nsteps > 0 && c->ring_pending) {   /* first step after org_prepare: eager, plus the ghost ring */
         TRY(enqueue_step(c));
@@ -601,11 +633,15 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
         for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c));
         return harvest_device_error(c);
     }
-    if (!c->gexec) {
+    /* graphs of ORG_GRAPH_STEPS steps and of one step (the remainder) */
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) continue;
+        const int nst = k ? 1 : ORG_GRAPH_STEPS;
         bool prof = c->prof;
         c->prof = false;                 /* no event records inside the graph */
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(c);
+        int rc = ORG_OK;
+        for (int q = 0; q < nst && !rc; q++) rc = enqueue_step(c);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->prof = prof;
@@ -613,10 +649,12 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
             if (g) (void)hipGraphDestroy(g);
             return rc ? rc : note_error(ORG_EHIP);
         }
-        c->graph = g;
-        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->graph[k] = g;
+        HIPCHK(hipGraphInstantiate(&c->gexec[k], c->graph[k], nullptr, nullptr, 0));
     }
-    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    int s = 0;
+    for (; s + ORG_GRAPH_STEPS <= nsteps; s += ORG_GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
+    for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec[1], c->stream));
     return harvest_device_error(c);
 }
 

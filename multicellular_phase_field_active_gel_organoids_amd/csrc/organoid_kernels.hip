@@ -48,6 +48,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -159,6 +160,15 @@ static __device__ __forceinline__ float2 csubf(float2 a, float2 b) { return make
 /* h(u) = u^2 (3 - 2u), useful_functions.cpp:247 */
 static __device__ __forceinline__ float hfun(float u) { return u * u * (3.0f - 2.0f * u); }
 
+/* Hardware reciprocal, square root and 2^x (v_rcp_f32, v_sqrt_f32,
+ * v_exp_f32, about 1 ulp each) for the per-pixel and per-mode arithmetic,
+ * in place of the multi-instruction IEEE sequences of '/', sqrtf and expf:
+ * those loops are bound by vector ALU issue, and the results stay within
+ * FP32 rounding noise of the reference's float arithmetic. */
+static __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+static __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+static __device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
+
 /* ---- block reductions (blockDim multiple of 64) --------------------------- */
 static __device__ __forceinline__ double wave_sum_d(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -189,6 +199,20 @@ static __device__ double block_sum_d(double v, double *red) {
 static __device__ __forceinline__ void drain_stores() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
+}
+
+/* Merge this thread's inclusive box (b1 < 0: empty) into the LDS box sb[0..3]:
+ * min/max across the wave first, then one LDS atomic per wave (same-address
+ * LDS atomics from every thread serialise). */
+static __device__ __forceinline__ void box_to_lds(int b0, int b1, int b2, int b3, int *sb) {
+    if (b1 < 0) { b0 = INT_MAX; b2 = INT_MAX; }
+    for (int o = 32; o > 0; o >>= 1) {
+        b0 = min(b0, __shfl_xor(b0, o, 64)); b1 = max(b1, __shfl_xor(b1, o, 64));
+        b2 = min(b2, __shfl_xor(b2, o, 64)); b3 = max(b3, __shfl_xor(b3, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && b1 >= 0) {
+        atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3);
+    }
 }
 
 /* ---- small in-register DFTs (radix-2, natural order in and out) ----------- */
@@ -289,10 +313,11 @@ static __device__ __forceinline__ void velocity_mode(float eta, float xi, float 
     float A = xi + 2.0f * eta * (kx * kx + ky * ky / 2.0f);
     float ex = eta * kx * ky;
     float2 y, x;
-    y.x = (ex * (kx * pxx.y + ky * pxy.y) - A * (ky * pyy.y + kx * pxy.y)) / D;
-    y.y = (-ex * (kx * pxx.x + ky * pxy.x) + A * (ky * pyy.x + kx * pxy.x)) / D;
-    x.x = (kx * pxx.y + ky * pxy.y - ex * y.x) / A;
-    x.y = (-kx * pxx.x - ky * pxy.x - ex * y.y) / A;
+    const float rD = hw_rcp(D), rA = hw_rcp(A);
+    y.x = (ex * (kx * pxx.y + ky * pxy.y) - A * (ky * pyy.y + kx * pxy.y)) * rD;
+    y.y = (-ex * (kx * pxx.x + ky * pxy.x) + A * (ky * pyy.x + kx * pxy.x)) * rD;
+    x.x = (kx * pxx.y + ky * pxy.y - ex * y.x) * rA;
+    x.y = (-kx * pxx.x - ky * pxy.x - ex * y.y) * rA;
     vx = x; vy = y;
 }
 
@@ -530,30 +555,63 @@ static __device__ __forceinline__ double sum_parts(const double *part, int nblk,
     return block_sum_d(v, red);
 }
 
-/* compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285) on rows
- * blk, blk + nblk, ...; returns this thread's share of the FP64 sum of
- * h(ECM) dx^2 of update_lumen_and_ECM (cell_phase_functions.cpp:376-386). */
-static __device__ __forceinline__ double hlum_rows(const DevP &P, const int blk, const int nblk,
-                                                   const float *__restrict__ lum, const float *__restrict__ ecm,
-                                                   float *__restrict__ hlum, float *__restrict__ hecm) {
+/* ---- h(lum), h(ECM) and the FP64 sum of h(ECM) ---------------------------
+ * compute_all_hfunc_2d_tables_kernels (useful_functions.cpp:272-285) and the
+ * sum of h(ECM) dx^2 of update_lumen_and_ECM (cell_phase_functions.cpp:
+ * 376-386).  The sum is split into fixed parts: one per 64 x 64 tile of the
+ * grid (its interior pixels, tile k row-major over ceil(N/64)^2 tiles) and
+ * one for the boundary ring (part ntiles); each part is a fixed-order block
+ * reduction and the total adds the parts in index order, so a part can be
+ * recomputed alone (when its pixels may have changed) and the total stays
+ * the same bit for bit. */
+#define ORG_HT 64
+static __device__ __forceinline__ int hl_ntiles(int N) { const int t = (N + ORG_HT - 1) / ORG_HT; return t * t; }
+
+/* ring pixel k = 0 .. 4N-5: rows 0 and N-1, then columns 0 and N-1 */
+static __device__ __forceinline__ void ring_px(int k, int N, int &i, int &j) {
+    if (k < N) { i = 0; j = k; }
+    else if (k < 2 * N) { i = N - 1; j = k - N; }
+    else if (k < 3 * N - 2) { i = k - 2 * N + 1; j = 0; }
+    else { i = k - (3 * N - 2) + 1; j = N - 1; }
+}
+
+/* part k (tile, or the ring for k == ntiles): stores h(lum), h(ECM) on its
+ * pixels, returns this thread's share of its sum. */
+static __device__ double hlum_part(const DevP &P, const int k, const float *__restrict__ lum,
+                                   const float *__restrict__ ecm, float *__restrict__ hlum,
+                                   float *__restrict__ hecm) {
     const int N = P.N;
     double acc = 0.0;
+    if (k == hl_ntiles(N)) {
+        for (int q = threadIdx.x; q < 4 * N - 4; q += blockDim.x) {
+            int i, j;
+            ring_px(q, N, i, j);
+            const size_t g = (size_t)i * N + j;
+            const float a = lum[g], b = ecm[g], hb = hfun(b);
+            hlum[g] = hfun(a);
+            hecm[g] = hb;
+            acc += (double)(hb * P.dx * P.dx);
+        }
+        return acc;
+    }
+    const int nt = (N + ORG_HT - 1) / ORG_HT;
+    const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
 #pragma unroll 4
-    for (int i = blk; i < N; i += nblk) {
-        const size_t g = (size_t)i * N;
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
-            float a = lum[g + j], b = ecm[g + j];
-            float hb = hfun(b);
-            hlum[g + j] = hfun(a);
-            hecm[g + j] = hb;
+    for (int e = threadIdx.x; e < ORG_HT * ORG_HT; e += blockDim.x) {
+        const int i = ti + e / ORG_HT, j = tj + e % ORG_HT;
+        if (i >= 1 && i < N - 1 && j >= 1 && j < N - 1) {
+            const size_t g = (size_t)i * N + j;
+            const float a = lum[g], b = ecm[g], hb = hfun(b);
+            hlum[g] = hfun(a);
+            hecm[g] = hb;
             acc += (double)(hb * P.dx * P.dx);
         }
     }
     return acc;
 }
 
-/* k_hlum: h(lum), h(ECM) and sum h(ECM), the sum reduced by the last block
- * to finish in a fixed order (deterministic). */
+/* k_hlum: every part (grid = ntiles + 1), the total reduced by the last
+ * block to finish (fixed order: deterministic). */
 __global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
                                                float *__restrict__ hlum, float *__restrict__ hecm,
                                                double *__restrict__ part, unsigned *__restrict__ counter,
@@ -561,7 +619,7 @@ __global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__
     __shared__ double red[16];
     __shared__ bool last;
     const int blk = blockIdx.x, nblk = gridDim.x;
-    const double tot = block_sum_d(hlum_rows(P, blk, nblk, lum, ecm, hlum, hecm), red);
+    const double tot = block_sum_d(hlum_part(P, blk, lum, ecm, hlum, hecm), red);
     if (threadIdx.x == 0) {
         part[blk] = tot;
         __threadfence();
@@ -574,6 +632,32 @@ __global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__
         const double t2 = sum_parts(part, nblk, red);
         if (threadIdx.x == 0) { *sum_out = t2; *counter = 0u; }
     }
+}
+
+/* ---- live boxes of the lumen and ECM fields --------------------------------
+ * A pixel of a committed field X is live if X is not exactly 0 or 1 there, or
+ * differs from one of its four neighbours.  Elsewhere the update leaves X
+ * unchanged (the update is X + c X (1 - X) (...)) and both central
+ * differences of X vanish, so the normals are zero.  live[4f..4f+3] is a
+ * half-open box (rows i0..i1, columns j0..j1) holding every pixel that has
+ * been live since the lumen/ECM state was last initialised (it only grows).
+ * With the box inside [3, N-3)^2 (box mode) the update, the normals and the
+ * copy-back of field f touch only the box grown by one pixel plus the
+ * boundary ring, and the ring's normals are zero (their periodic sources,
+ * rows/columns 1 and N-2, are not live); otherwise every pixel is processed
+ * (full mode).  Invariant kept between steps: outside the live box grown by
+ * one and the ring, both ping-pong buffers hold the committed field. */
+static __device__ __forceinline__ int4 ld_box(const int *live, int f) {
+    return make_int4(live[4 * f], live[4 * f + 1], live[4 * f + 2], live[4 * f + 3]);
+}
+static __device__ __forceinline__ bool box_empty(const int4 b) { return b.y <= b.x || b.w <= b.z; }
+static __device__ __forceinline__ bool box_mode(const int4 b, const int N) {
+    return box_empty(b) || (b.x >= 3 && b.y <= N - 3 && b.z >= 3 && b.w <= N - 3);
+}
+/* is the pixel e (interior) of the committed field X live */
+static __device__ __forceinline__ bool is_live(const float *__restrict__ X, const size_t e, const int N) {
+    const float v = X[e];
+    return (v != 0.0f && v != 1.0f) || X[e + N] != v || X[e - N] != v || X[e + 1] != v || X[e - 1] != v;
 }
 
 /* psi = sum_n hfunc_n (compute_psi_kernels, useful_functions.cpp:305-322) on
@@ -694,10 +778,30 @@ static __device__ void psi_tiles_pre(const DevP &P, const int blk, const int nbl
  * the last iterate */
 struct LumState { int done, ri, cur, pad; };
 
-/* update_lumen_and_ECM (cell_phase_functions.cpp:395-443) and
- * boundary_conditions_lumen_and_ECM (useful_functions.cpp:439-462) on row i;
- * the first iterate of both reshaping loops goes to both ping-pong buffers
- * (sweeps rewrite only their active box). */
+/* update_lumen_and_ECM (cell_phase_functions.cpp:395-443) at an interior
+ * pixel e: the new lumen (f = 0) or ECM (f = 1) value. */
+static __device__ __forceinline__ float lumecm_px(const DevP &P, const int f, const size_t e, const float se,
+                                                  const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                  const float *__restrict__ psi, const float *__restrict__ hlum,
+                                                  const float *__restrict__ hecm) {
+    const int N = P.N;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float ps = psi[e];
+    const float lap = L9AT(psi, e, (size_t)N) * inv4;
+    if (f == 0) {
+        const float l = lum[e];
+        return l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
+                       (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
+    }
+    const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
+    const float E = ecm[e];
+    return E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
+                   (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
+}
+
+/* update_lumen_and_ECM on row i, then boundary_conditions_lumen_and_ECM
+ * (useful_functions.cpp:439-462: lumen edges and corners 0, ECM 1); the
+ * first iterate of both reshaping loops goes to both ping-pong buffers. */
 static __device__ __forceinline__ void lumecm_row(const DevP &P, const int i, const float se,
                                                   const float *__restrict__ lum, const float *__restrict__ ecm,
                                                   const float *__restrict__ psi, const float *__restrict__ hlum,
@@ -705,22 +809,16 @@ static __device__ __forceinline__ void lumecm_row(const DevP &P, const int i, co
                                                   float *__restrict__ lumB, float *__restrict__ ecmA,
                                                   float *__restrict__ ecmB) {
     const int N = P.N;
-    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
-    const float ecm_area = (float)(N * N) * P.dx * P.dx - se;
     const size_t r = (size_t)i * N;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const size_t e = r + j;
         float ln, en;
         if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
-            ln = 0.0f;       /* lumen edges and corners 0 */
-            en = 1.0f;       /* ECM edges and corners 1 (:449-460) */
+            ln = 0.0f;
+            en = 1.0f;
         } else {
-            float l = lum[e], E = ecm[e], ps = psi[e];
-            float lap = L9AT(psi, e, (size_t)N) * inv4;
-            ln = l + (1.0f / P.tau_lum) * l * (1.0f - l) * P.dt *
-                         (P.xi_lum - P.beta_lc * ps + P.eta_lc * lap - P.beta_le * hecm[e]);
-            en = E + (1.0f / P.tau_E) * E * (1.0f - E) * P.dt *
-                         (P.alpha_E * ecm_area + P.xi_E - P.beta_ec * ps + P.eta_ec * lap - P.beta_le * hlum[e]);
+            ln = lumecm_px(P, 0, e, se, lum, ecm, psi, hlum, hecm);
+            en = lumecm_px(P, 1, e, se, lum, ecm, psi, hlum, hecm);
         }
         lumA[e] = ln; lumB[e] = ln;
         ecmA[e] = en; ecmB[e] = en;
@@ -773,11 +871,12 @@ static __device__ __forceinline__ void normals_row(const DevP &P, const int i, c
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const size_t g = (size_t)si * N + ring_src(j, N);
         float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
-        float ng = sqrtf(c * (a * a + b * b));
+        float ng = hw_sqrt(c * (a * a + b * b));
         float vx = 0.0f, vy = 0.0f;
         if (ng != 0.0f) {
-            vx = h * (nw[g + N] - nw[g - N]) / ng;
-            vy = h * (nw[g + 1] - nw[g - 1]) / ng;
+            const float rng = hw_rcp(ng);
+            vx = h * (nw[g + N] - nw[g - N]) * rng;
+            vy = h * (nw[g + 1] - nw[g - 1]) * rng;
         }
         const size_t e = (size_t)i * N + j;
         jx[e] = vx; jy[e] = vy;
@@ -798,7 +897,7 @@ static __device__ void normals_rows(const DevP &P, const int f, const int blk, c
     int b0 = N, b1 = -1, b2 = N, b3 = -1;
 #pragma unroll 2
     for (int i = blk; i < N; i += nblk) normals_row(P, i, old, nw, jx, jy, b0, b1, b2, b3);
-    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+    box_to_lds(b0, b1, b2, b3, sb);
     __syncthreads();
     if (threadIdx.x < 4) jbox_part[((size_t)f * nblk + blk) * 4 + threadIdx.x] = sb[threadIdx.x];
 }
@@ -836,10 +935,85 @@ static __device__ void lumecm_normals_band(const DevP &P, const int blk, const i
         int b0 = N, b1 = -1, b2 = N, b3 = -1;
 #pragma unroll 2
         for (int i = r0; i < r1; i++) normals_row(P, i, old, nw, jx, jy, b0, b1, b2, b3);
-        if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+        box_to_lds(b0, b1, b2, b3, sb);
         __syncthreads();
         if (threadIdx.x < 4) jbox_part[((size_t)f * nblk + blk) * 4 + threadIdx.x] = sb[threadIdx.x];
         __syncthreads();
+    }
+}
+
+/* Box mode: the update (both buffers) and the normals of field f on its
+ * live box B, the box's rows split into bands over the workgroups.  The band
+ * updates its rows plus one halo row each side and the box's columns plus
+ * one each side, which is every updated value its normals read; rows
+ * updated by two workgroups get the same values from both.  Outside B grown
+ * by one the buffers already hold the (unchanged) field, and the normals
+ * outside B are zero and already stored as such (B holds every pixel that
+ * was ever live).  The workgroup's box of nonzero normals goes to
+ * jbox_part. */
+static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4 B, const int blk, const int nblk,
+                                          const float se, const float *__restrict__ lum,
+                                          const float *__restrict__ ecm, const float *__restrict__ psi,
+                                          const float *__restrict__ hlum, const float *__restrict__ hecm,
+                                          float *__restrict__ XA, float *__restrict__ XB, float *__restrict__ jx,
+                                          float *__restrict__ jy, int *__restrict__ jbox_part, int *sb) {
+    const int N = P.N, nt = blockDim.x;
+    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+    if (!box_empty(B)) {
+        const int nr = B.y - B.x;
+        const int r0 = B.x + (int)(((long long)blk * nr) / nblk), r1 = B.x + (int)(((long long)(blk + 1) * nr) / nblk);
+        if (r1 > r0) {
+            /* update: rows [r0-1, r1+1) within [B.x-1, B.y+1), columns [B.z-1, B.w+1) */
+            const int ua = r0 - 1, ub = r1 + 1, ca = B.z - 1, cw = B.w - B.z + 2;
+            const int npx = (ub - ua) * cw;
+#pragma unroll 4
+            for (int q = threadIdx.x; q < npx; q += nt) {
+                const int i = ua + q / cw, j = ca + q % cw;
+                const size_t e = (size_t)i * N + j;
+                const float v = lumecm_px(P, f, e, se, lum, ecm, psi, hlum, hecm);
+                XA[e] = v; XB[e] = v;
+            }
+        }
+        drain_stores();
+        __syncthreads();
+        if (r1 > r0) {
+            const float *old = f ? ecm : lum;
+            const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
+            const int bw = B.w - B.z, npx = (r1 - r0) * bw;
+            int b0 = N, b1 = -1, b2 = N, b3 = -1;
+#pragma unroll 4
+            for (int q = threadIdx.x; q < npx; q += nt) {
+                const int i = r0 + q / bw, j = B.z + q % bw;
+                const size_t g = (size_t)i * N + j;
+                const float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+                const float ng = hw_sqrt(c * (a * a + b * b));
+                float vx = 0.0f, vy = 0.0f;
+                if (ng != 0.0f) {
+                    const float rng = hw_rcp(ng);
+                    vx = h * (XA[g + N] - XA[g - N]) * rng;
+                    vy = h * (XA[g + 1] - XA[g - 1]) * rng;
+                }
+                jx[g] = vx; jy[g] = vy;
+                if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
+            }
+            box_to_lds(b0, b1, b2, b3, sb);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) jbox_part[((size_t)f * nblk + blk) * 4 + threadIdx.x] = sb[threadIdx.x];
+    __syncthreads();
+}
+
+/* Box mode: the boundary ring of both fields in both buffers (lumen 0,
+ * ECM 1, useful_functions.cpp:439-462); its normals are zero. */
+static __device__ void lumecm_ring(const DevP &P, float *__restrict__ L0, float *__restrict__ L1,
+                                   float *__restrict__ E0, float *__restrict__ E1) {
+    for (int q = threadIdx.x; q < 4 * P.N - 4; q += blockDim.x) {
+        int i, j;
+        ring_px(q, P.N, i, j);
+        const size_t e = (size_t)i * P.N + j;
+        L0[e] = 0.0f; L1[e] = 0.0f;
+        E0[e] = 1.0f; E1[e] = 1.0f;
     }
 }
 
@@ -867,7 +1041,7 @@ static __device__ __forceinline__ int4 reduce_jbox(const int *__restrict__ part,
         const int *q = part + ((size_t)f * nparts + k) * 4;
         if (q[1] >= 0) { b0 = min(b0, q[0]); b1 = max(b1, q[1]); b2 = min(b2, q[2]); b3 = max(b3, q[3]); }
     }
-    if (b1 >= 0) { atomicMin(&sb[0], b0); atomicMax(&sb[1], b1); atomicMin(&sb[2], b2); atomicMax(&sb[3], b3); }
+    box_to_lds(b0, b1, b2, b3, sb);
     __syncthreads();
     int4 r = make_int4(sb[0], sb[1], sb[2], sb[3]);
     __syncthreads();
@@ -1110,6 +1284,115 @@ __global__ __launch_bounds__(1024) void k_commit_lumecm(DevP P, float *__restric
     commit_lumecm_body(P, blockIdx.x, gridDim.x, lum, ecm, bufL0, bufL1, bufE0, bufE1, lst, err, uni);
 }
 
+/* Copy-back of field f from its final buffer F (lum <- lumn or ECM <- ECMn)
+ * by workgroup blk of nblk.  Full mode: every pixel.  Box mode: the live box
+ * B grown by one pixel and the ring (the only pixels that can differ), then
+ * the bounding box (inclusive) of the live pixels of the new field within B
+ * grown by two (a neighbour of a changed pixel may have become live) goes to
+ * live_part[(f * nblk + blk) * 4 ..] (empty: i1 < 0). */
+static __device__ void commit_field(const DevP &P, const int f, const int4 B, const int blk, const int nblk,
+                                    float *__restrict__ X, const float *__restrict__ F, int *__restrict__ live_part,
+                                    int *sb) {
+    const int N = P.N, nt = blockDim.x;
+    int *lp = live_part + ((size_t)f * nblk + blk) * 4;
+    if (threadIdx.x < 4) lp[threadIdx.x] = (threadIdx.x & 1) ? -1 : N;
+    if (!box_mode(B, N)) {
+#pragma unroll 4
+        for (int i = blk; i < N; i += nblk) {
+            const size_t r = (size_t)i * N;
+            for (int j = threadIdx.x; j < N; j += nt) X[r + j] = F[r + j];
+        }
+        return;
+    }
+    for (int q = blk * nt + threadIdx.x; q < 4 * N - 4; q += nblk * nt) {
+        int i, j;
+        ring_px(q, N, i, j);
+        X[(size_t)i * N + j] = F[(size_t)i * N + j];
+    }
+    if (box_empty(B)) return;
+    {
+        const int ca = B.z - 1, cw = B.w - B.z + 2, npx = (B.y - B.x + 2) * cw;
+#pragma unroll 4
+        for (int q = blk * nt + threadIdx.x; q < npx; q += nblk * nt) {
+            const size_t e = (size_t)(B.x - 1 + q / cw) * N + (ca + q % cw);
+            X[e] = F[e];
+        }
+    }
+    if (threadIdx.x == 0) { sb[0] = N; sb[1] = -1; sb[2] = N; sb[3] = -1; }
+    __syncthreads();
+    int b0 = N, b1 = -1, b2 = N, b3 = -1;
+    {
+        const int ra = max(B.x - 2, 1), rb = min(B.y + 2, N - 1), ca = max(B.z - 2, 1), cb = min(B.w + 2, N - 1);
+        const int cw = cb - ca, npx = (rb - ra) * cw;
+#pragma unroll 4
+        for (int q = blk * nt + threadIdx.x; q < npx; q += nblk * nt) {
+            const int i = ra + q / cw, j = ca + q % cw;
+            if (is_live(F, (size_t)i * N + j, N)) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
+        }
+    }
+    box_to_lds(b0, b1, b2, b3, sb);
+    __syncthreads();
+    if (threadIdx.x < 4) lp[threadIdx.x] = sb[threadIdx.x];
+    __syncthreads();
+}
+
+/* k_live_init: the lumen/ECM state of the batched step from scratch (before
+ * the first batched step, or after the fields were set or stepped by the
+ * per-function entry points): live boxes of lum and ECM (reset, with the
+ * copy-back's partial boxes, by k_live_reset), both ping-pong buffers <- the
+ * fields, normals <- 0, every part of the h(lum)/h(ECM) pass.  Workgroup
+ * k < ntiles: tile k; the last: the ring. */
+__global__ void k_live_reset(int *live, int *live_part, int nparts, int N) {
+    if (threadIdx.x < 8) live[threadIdx.x] = (threadIdx.x & 1) ? 0 : N;
+    for (int q = threadIdx.x; q < 8 * nparts; q += blockDim.x) live_part[q] = (q & 1) ? -1 : N;
+}
+__global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restrict__ lum,
+                                                    const float *__restrict__ ecm, float *__restrict__ hlum,
+                                                    float *__restrict__ hecm, double *__restrict__ part,
+                                                    float *__restrict__ L0, float *__restrict__ L1,
+                                                    float *__restrict__ E0, float *__restrict__ E1,
+                                                    float *__restrict__ jlx, float *__restrict__ jly,
+                                                    float *__restrict__ jex, float *__restrict__ jey,
+                                                    int *__restrict__ live) {
+    __shared__ double red[16];
+    __shared__ int sb[8];
+    const int k = blockIdx.x, N = P.N;
+    const double tot = block_sum_d(hlum_part(P, k, lum, ecm, hlum, hecm), red);
+    if (threadIdx.x == 0) part[k] = tot;
+    if (threadIdx.x < 8) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : N;
+    __syncthreads();
+    int bx[2][4] = {{N, -1, N, -1}, {N, -1, N, -1}};
+    auto px = [&](int i, int j) {
+        const size_t e = (size_t)i * N + j;
+        const float a = lum[e], b = ecm[e];
+        L0[e] = a; L1[e] = a; E0[e] = b; E1[e] = b;
+        jlx[e] = 0.0f; jly[e] = 0.0f; jex[e] = 0.0f; jey[e] = 0.0f;
+        if (i >= 1 && i < N - 1 && j >= 1 && j < N - 1) {
+            for (int f = 0; f < 2; f++)
+                if (is_live(f ? ecm : lum, e, N)) {
+                    bx[f][0] = min(bx[f][0], i); bx[f][1] = max(bx[f][1], i);
+                    bx[f][2] = min(bx[f][2], j); bx[f][3] = max(bx[f][3], j);
+                }
+        }
+    };
+    if (k == hl_ntiles(N)) {
+        for (int q = threadIdx.x; q < 4 * N - 4; q += blockDim.x) { int i, j; ring_px(q, N, i, j); px(i, j); }
+    } else {
+        const int nt = (N + ORG_HT - 1) / ORG_HT, ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+        for (int e = threadIdx.x; e < ORG_HT * ORG_HT; e += blockDim.x) {
+            const int i = ti + e / ORG_HT, j = tj + e % ORG_HT;
+            if (i >= 1 && i < N - 1 && j >= 1 && j < N - 1) px(i, j);
+        }
+    }
+    for (int f = 0; f < 2; f++) box_to_lds(bx[f][0], bx[f][1], bx[f][2], bx[f][3], sb + 4 * f);
+    __syncthreads();
+    if (threadIdx.x < 2 && sb[4 * threadIdx.x + 1] >= 0) {
+        const int f = threadIdx.x;
+        atomicMin(&live[4 * f], sb[4 * f]); atomicMax(&live[4 * f + 1], sb[4 * f + 1] + 1);
+        atomicMin(&live[4 * f + 2], sb[4 * f + 2]); atomicMax(&live[4 * f + 3], sb[4 * f + 3] + 1);
+    }
+}
+
 /* ======================================================================== */
 /* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
  *
@@ -1151,13 +1434,13 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float MMf = (float)MM;
 
     /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
-#pragma unroll 4
+#pragma unroll 8
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         const size_t g = (size_t)(x0 + i) * N + (y0 + j);
         float v = ph[g];
         float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
         int idx = turnover_label(v, lum[g], ecm[g], ph1[g], ph2[g]);
         lab[e] = (unsigned char)idx;
         float r = rh[g];
@@ -1166,13 +1449,14 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         float base = -at * r3 + bb * r3 * r;
         float pxx = base, pxy = 0.0f, pyv = base;
         if (ng > 0.0f) {
-            float c = (at - ao) * r3 * inv4 / ng / ng;
+            const float rng = hw_rcp(ng);
+            float c = (at - ao) * r3 * inv4 * rng * rng;
             pxx += c * dxp * dxp;
             pxy += c * dxp * dyp;
             pyv += c * dyp * dyp;
         }
         float d = v - 0.5f;
-        float pnd = ng * expf(-d * d * isig2) * dnorm;
+        float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
         float react = -P.deg[idx] * r + P.gen[idx] * pnd;
         rbox[e] = r + P.dt * react;
         SB[e] = pyv;
@@ -1191,6 +1475,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     /* P4: per pair {k, -k}: split F(pi_xx + i pi_xy), solve both modes
      * (rho_functions.cpp:133-190), keep the part of v whose inverse is real,
      * Z = H v_x + i H v_y written in place at k and -k. */
+#pragma unroll 4
     for (int u = t; u < NPAIR; u += 1024) {
         int i, j;
         if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
@@ -1231,6 +1516,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249)
      * per pair, the part whose inverse is real, in place */
     const float cdt = P.dt * kscale;
+#pragma unroll 4
     for (int u = t; u < NPAIR; u += 1024) {
         int i, j;
         if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
@@ -1301,12 +1587,12 @@ static __device__ void cells_update(const DevP &P, const int x0, const int y0, c
         const size_t g = (size_t)(x0 + i) * N + (y0 + j);
         float v = ph[g];
         float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
         float out = v;
         if (ng > 0.0f) {
             float d = v - 0.5f;
-            out -= cplc * expf(-d * d * isig2) *
-                   ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) / ng;
+            out -= cplc * hw_exp(-d * d * isig2) *
+                   ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) * hw_rcp(ng);
         }
         /* temp_phase = psi - hfunc_n (cell_phase_functions.cpp:150) on the 3x3 stencil */
         const float *q = ps0 + i * pst + j;
@@ -1336,9 +1622,9 @@ static __device__ void cells_normals(const DevP &P, const int x0, const int y0, 
         const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
         const size_t g = (size_t)(x0 + i) * N + (y0 + j);
         float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = sqrtf(inv4 * (dxp * dxp + dyp * dyp));
+        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
         float a = 0.0f, b = 0.0f;
-        if (ng > 0.0f) { a = h * dxp / ng; b = h * dyp / ng; }
+        if (ng > 0.0f) { const float rng = hw_rcp(ng); a = h * dxp * rng; b = h * dyp * rng; }
         jx[e] = a; jy[e] = b;
         if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = ph[g];
     }
@@ -1436,8 +1722,10 @@ struct StepArgs {
     float *lum, *ecm;
     float *psi, *hlum, *hecm;
     double *part_h, *sum_ecm;
-    int nvb;                 /* row blocks of the h(lum), h(ECM) pass (as k_hlum's grid) */
+    int nh;                  /* h(lum), h(ECM) workgroups of k_pre */
     int npsi;                /* psi workgroups of k_pre */
+    int *live, *live_part;   /* live boxes of lum and ECM; per-workgroup boxes of live pixels from k_post */
+    int nlp;                 /* k_post's lumen/ECM workgroups (partial boxes per field) */
     /* lumen / ECM loops */
     float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
     int *jbox_part;
@@ -1462,9 +1750,12 @@ struct StepArgs {
  *                      (compute_hfunc_3d_tables_kernels_loop_n, sum_s_kernels)
  *   next npsi          psi on the union of the windows (compute_psi_kernels),
  *                      with hfun(phi) inside each window
- *   last nvb           h(lum), h(ECM) and the per-block sums of h(ECM)
- *                      (compute_all_hfunc_2d_tables_kernels) with k_hlum's
- *                      row blocks */
+ *   last nh            h(lum), h(ECM) and the parts of the sum of h(ECM)
+ *                      (compute_all_hfunc_2d_tables_kernels) that the last
+ *                      copy-back may have changed: the ring and, in box mode,
+ *                      the tiles meeting a live box grown by one (all tiles in
+ *                      full mode).  The first of them also adds the live
+ *                      pixels found by the last copy-back to the live boxes. */
 #define ORG_LDS_PRE (4 * ORG_PSI_LDS_GROUP + 16)
 __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
@@ -1474,9 +1765,44 @@ __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     } else if (b < P.nc + a.npsi) {
         psi_tiles_pre(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.psi, scr);
     } else {
-        const int vb = b - P.nc - a.npsi;
-        const double tot = block_sum_d(hlum_rows(P, vb, a.nvb, a.lum, a.ecm, a.hlum, a.hecm), (double *)scr);
-        if (threadIdx.x == 0) a.part_h[vb] = tot;
+        const int hb = b - P.nc - a.npsi, N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+        const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+        if (hb == 0 && threadIdx.x < 128) {
+            /* live <- live U the copy-back's partial boxes (wave f for field
+             * f).  Other workgroups of this role may read the box before or
+             * after: both hold every pixel the last copy-back changed. */
+            const int f = threadIdx.x >> 6, l = threadIdx.x & 63;
+            int c0 = N, c1 = -1, c2 = N, c3 = -1;
+            for (int k = l; k < a.nlp; k += 64) {
+                const int *q = a.live_part + ((size_t)f * a.nlp + k) * 4;
+                if (q[1] >= 0) { c0 = min(c0, q[0]); c1 = max(c1, q[1]); c2 = min(c2, q[2]); c3 = max(c3, q[3]); }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                c0 = min(c0, __shfl_xor(c0, o, 64)); c1 = max(c1, __shfl_xor(c1, o, 64));
+                c2 = min(c2, __shfl_xor(c2, o, 64)); c3 = max(c3, __shfl_xor(c3, o, 64));
+            }
+            if (l == 0 && c1 >= 0) {
+                const int4 B = ld_box(a.live, f);
+                int4 r = make_int4(c0, c1 + 1, c2, c3 + 1);
+                if (!box_empty(B)) r = make_int4(min(B.x, r.x), max(B.y, r.y), min(B.z, r.z), max(B.w, r.w));
+                a.live[4 * f] = r.x; a.live[4 * f + 1] = r.y; a.live[4 * f + 2] = r.z; a.live[4 * f + 3] = r.w;
+            }
+        }
+        for (int k = hb; k <= ntl; k += a.nh) {
+            bool need = full || k == ntl;
+            if (!need) {
+                const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+                for (int f = 0; f < 2 && !need; f++) {
+                    const int4 B = f ? BE : BL;
+                    need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
+                           tj + ORG_HT > B.z - 1;
+                }
+            }
+            if (!need) continue;
+            const double tot = block_sum_d(hlum_part(P, k, a.lum, a.ecm, a.hlum, a.hecm), (double *)scr);
+            if (threadIdx.x == 0) a.part_h[k] = tot;
+        }
     }
 }
 
@@ -1513,13 +1839,22 @@ __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
         double *red = (double *)lds;
         int *sb = (int *)(lds + 128);
         int epoch = 0;
-        const double se = sum_parts(a.part_h, a.nvb, red);
+        const double se = sum_parts(a.part_h, hl_ntiles(P.N) + 1, red);
         if (blk == 0) {
             if (threadIdx.x == 0) *a.sum_ecm = se;
             lum_reset_states(P, a.lst);
         }
-        lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0, a.E1,
-                            a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
+        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+        if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
+            if (blk == 0) lumecm_ring(P, a.L0, a.L1, a.E0, a.E1);
+            lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
+                               a.jlx, a.jly, a.jbox_part, sb);
+            lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
+                               a.jex, a.jey, a.jbox_part, sb);
+        } else {
+            lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0,
+                                a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
+        }
         if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
         lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
                       a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit);
@@ -1578,8 +1913,8 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
  * rho box and phi window of every cell (workgroups 0..ncell-1), then that
  * cell's centre and support box for the next step (find_centres_cells of
  * the next iteration, useful_functions.cpp:193-227) into (xn, yn, bn);
- * lum <- lumn, ECM <- ECMn (the other workgroups).  Clears k_step's barrier
- * counter and abort flag. */
+ * lum <- lumn, ECM <- ECMn (the other workgroups, see commit_field).  Clears
+ * k_step's barrier counter and abort flag. */
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
     __shared__ long long red[48];
     const int b = blockIdx.x;
@@ -1590,7 +1925,15 @@ __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
         __syncthreads();
         cell_centroid(P, b, a.phi, a.xc, a.yc, a.box, a.xn, a.yn, a.bn, a.err, red);
     } else {
-        commit_lumecm_body(P, b - P.nc, gridDim.x - P.nc, a.lum, a.ecm, a.L0, a.L1, a.E0, a.E1, a.lst, a.err, a.uni);
+        const int blk = b - P.nc, nblk = gridDim.x - P.nc;
+        const LumState sl = a.lst[0], se = a.lst[1];
+        if (blk == 0 && threadIdx.x == 0) {
+            a.uni[0] = P.N; a.uni[1] = 0; a.uni[2] = P.N; a.uni[3] = 0;
+            if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
+        }
+        int *sb = (int *)red;
+        commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
+        commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
     }
 }
 

@@ -138,3 +138,60 @@ def test_hex_organoid_steps(rel):
         o.step(it)
     for name, f in (("phi", host.F_PHI), ("rho", host.F_RHO), ("ecm", host.F_ECM)):
         assert rel(g.get_field(f), o.get(name)) < 2e-5, name
+
+
+def _plain_vs_batched(wl, steps, fields=(host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM, host.F_PSI,
+                                           host.F_HFUNC, host.F_HLUM, host.F_HECM)):
+    a = host.Organoid(wl.params); a.load_workload(wl)
+    b = host.Organoid(wl.params); b.load_workload(wl)
+    for _ in range(steps):
+        a.plain_step()
+    b.step(steps, graph=True)
+    for f in fields:
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    assert a.iterations()[1:] == b.iterations()[1:]
+
+
+def test_batched_equals_plain_full_mode():
+    """ECM live next to the boundary: every pixel is processed"""
+    wl = make_workload("two_cells_256")
+    N = wl.params.N
+    wl.ecm = wl.ecm.astype(np.float32)
+    wl.ecm[1, 1:N - 1] = 0.999         # a live strip next to the boundary ring
+    print("case two_cells_256 full-mode ECM, 4 steps", flush=True)
+    _plain_vs_batched(wl, 4)
+
+
+def test_batched_equals_plain_lumen_disc():
+    """a lumen disc at the organoid centre: both fields in box mode, lumen box non-empty"""
+    wl = make_workload("hex", N=320, ncell=6)
+    N = wl.params.N
+    i, j = np.mgrid[0:N, 0:N]
+    r = np.hypot(i - N / 2, j - N / 2)
+    wl.lum = np.clip(0.5 * (1.0 - np.tanh((r - 12.0) / 2.0)), 0.0, 1.0).astype(np.float32)
+    wl.lum[r > 40] = 0.0
+    print("case hex N=320 ncell=6 lumen disc, 6 steps", flush=True)
+    _plain_vs_batched(wl, 6)
+
+
+def test_batched_equals_plain_long():
+    """live boxes grow over many steps"""
+    wl = make_workload("hex", N=320, ncell=6)
+    print("case hex N=320 ncell=6, 40 steps", flush=True)
+    _plain_vs_batched(wl, 40)
+
+
+def test_plain_and_batched_interleaved():
+    """per-function steps between batched steps re-initialise the batched state"""
+    wl = make_workload("two_cells_256")
+    a = host.Organoid(wl.params); a.load_workload(wl)
+    b = host.Organoid(wl.params); b.load_workload(wl)
+    for _ in range(6):
+        a.plain_step()
+    print("case two_cells_256 batched 2 + plain 1 + batched 3", flush=True)
+    b.step(2, graph=True)
+    b.plain_step()
+    b.step(3, graph=True)
+    for f in (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM, host.F_PSI, host.F_HFUNC):
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    assert all((p == q).all() for p, q in zip(a.centres(), b.centres()))

@@ -160,6 +160,18 @@ static __device__ __forceinline__ float2 csubf(float2 a, float2 b) { return make
 /* h(u) = u^2 (3 - 2u), useful_functions.cpp:247 */
 static __device__ __forceinline__ float hfun(float u) { return u * u * (3.0f - 2.0f * u); }
 
+/* Load from a uniform base at a 32-bit byte offset: the global_load form
+ * with the base in scalar registers and one vector register per offset
+ * (plane[index] with a 64-bit index needs two per load, which limits how many
+ * loads a loop can keep in flight). */
+static __device__ __forceinline__ float ldb(const float *__restrict__ base, unsigned byte_off) {
+    return *(const float *)((const char *)base + byte_off);
+}
+
+static __device__ __forceinline__ float2 ldb2(const float2 *__restrict__ base, unsigned byte_off) {
+    return *(const float2 *)((const char *)base + byte_off);
+}
+
 /* Hardware reciprocal, square root and 2^x (v_rcp_f32, v_sqrt_f32,
  * v_exp_f32, about 1 ulp each) for the per-pixel and per-mode arithmetic,
  * in place of the multi-instruction IEEE sequences of '/', sqrtf and expf:
@@ -380,6 +392,55 @@ static __device__ void centroid_sums(const float *__restrict__ ph, int N, int i0
             if (ph[(size_t)(i0 + i) * N + (j0 + j)] > 0.5f) { sx += i0 + i; sy += j0 + j; c += 1; }
             i += di; j += dj;
             if (j >= bw) { j -= bw; i++; }
+        }
+    }
+    sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
+    __syncthreads();
+    if (lane == 0) { red[w] = sx; red[16 + w] = sy; red[32 + w] = c; }
+    __syncthreads();
+    X = 0; Y = 0; C = 0;
+    if (t == 0)
+        for (int k = 0; k < nw; k++) { X += red[k]; Y += red[16 + k]; C += red[32 + k]; }
+}
+
+/* centroid_sums with the pixels of the window [x0, x0+W) x [y0, y0+W) taken
+ * from win (LDS, row stride W + 2, one-pixel ring) instead of ph. */
+static __device__ void centroid_sums_win(const float *__restrict__ ph, int N, int i0, int i1, int j0, int j1,
+                                         int x0, int y0, const float *win, long long *red, long long &X,
+                                         long long &Y, long long &C) {
+    constexpr int W = ORG_W, WH = ORG_WH;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6, nt = blockDim.x;
+    const int bw = j1 - j0, bh = i1 - i0;
+    const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
+    long long sx = 0, sy = 0, c = 0;
+    if (npx > 0) {
+        /* 8 pixels per thread at a time: the plane loads unconditional
+         * (inside the box) and issued before any use; window pixels then
+         * take the LDS value instead */
+        const int di = nt / bw, dj = nt - di * bw;
+        int i = t / bw, j = t - i * bw;
+        const unsigned b0 = 4u * ((unsigned)i0 * (unsigned)N + (unsigned)j0);
+#pragma unroll 1
+        for (int e0 = t; e0 < npx; e0 += 8 * nt) {
+            int pi[8], pj[8];
+            float vg[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                pi[k] = i0 + i; pj[k] = j0 + j;
+                const unsigned g = 4u * ((unsigned)pi[k] * (unsigned)N + (unsigned)pj[k]);
+                vg[k] = ldb(ph, e0 + k * nt < npx ? g : b0);
+                i += di; j += dj;
+                if (j >= bw) { j -= bw; i++; }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (e0 + k * nt >= npx) break;
+                const int wi = pi[k] - x0, wj = pj[k] - y0;
+                const bool in = wi >= 0 && wi < W && wj >= 0 && wj < W;
+                const float v = in ? win[(wi + 1) * WH + (wj + 1)] : vg[k];
+                if (v > 0.5f) { sx += pi[k]; sy += pj[k]; c += 1; }
+            }
         }
     }
     sx = wave_sum_ll(sx); sy = wave_sum_ll(sy); c = wave_sum_ll(c);
@@ -705,16 +766,27 @@ static __device__ void psi_tile(const DevP &P, const float *__restrict__ hfunc, 
         }
         __syncthreads();
         if (act) {
+            /* 4 listed cells at a time: loads unconditional (every listed
+             * plane holds the pixel), a cell whose box misses the pixel
+             * adds +0, which leaves the sum unchanged bit for bit */
             const int cnt = *lcnt;
             const size_t e = (size_t)i * N + j;
-            for (int k = 0; k < cnt; k++) {
-                const int4 bx = lbox[k];
-                if (i >= bx.x && i < bx.y && j >= bx.z && j < bx.w) {
+            for (int k0 = 0; k0 < cnt; k0 += 4) {
+                float x[4];
+                bool cov[4], inw[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int k = min(k0 + kk, cnt - 1);
+                    const int4 bx = lbox[k];
                     const int2 w = lwin[k];
-                    const size_t o = lcell[k] * NN + e;
-                    if (w.x >= 0 && i >= w.x && i < w.x + W && j >= w.y && j < w.y + W) acc += hfun(phi[o]);
-                    else acc += hfunc[o];
+                    cov[kk] = k0 + kk < cnt && i >= bx.x && i < bx.y && j >= bx.z && j < bx.w;
+                    inw[kk] = w.x >= 0 && i >= w.x && i < w.x + W && j >= w.y && j < w.y + W;
+                    const float *pl = inw[kk] ? phi : hfunc;
+                    x[kk] = pl[lcell[k] * NN + e];
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) acc += cov[kk] ? (inw[kk] ? hfun(x[kk]) : x[kk]) : 0.0f;
             }
         }
         __syncthreads();
@@ -925,8 +997,7 @@ static __device__ void lumecm_normals_band(const DevP &P, const int blk, const i
         for (int i = max(N - 3, 0); i < N; i++) lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, L0, L1, E0, E1);
     if (r1 == N)
         for (int i = 0; i < min(3, N); i++) lumecm_row(P, i, se, lum, ecm, psi, hlum, hecm, L0, L1, E0, E1);
-    drain_stores();
-    __syncthreads();
+    __syncthreads();          /* the workgroup reads back its own stores */
     for (int f = 0; f < 2; f++) {
         const float *old = f ? ecm : lum, *nw = f ? E0 : L0;
         float *jx = f ? jex : jlx, *jy = f ? jey : jly;
@@ -974,8 +1045,7 @@ static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4
                 XA[e] = v; XB[e] = v;
             }
         }
-        drain_stores();
-        __syncthreads();
+        __syncthreads();      /* the workgroup reads back its own stores */
         if (r1 > r0) {
             const float *old = f ? ecm : lum;
             const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
@@ -1078,6 +1148,15 @@ static __device__ __forceinline__ float reshape_flux(float un, const float *u, i
 #undef U_
 #undef JX_
 #undef JY_
+}
+
+/* reshape_flux with the normals already loaded: jx[0..4], jy[0..4] at the
+ * pixel and its +row, -row, +column, -column neighbours. */
+static __device__ __forceinline__ float reshape_flux_j(float un, const float *u, int su, const float *jx,
+                                                       const float *jy, float ka, float kc) {
+    const float ja[3 * 3] = {0.0f, jx[2], 0.0f, jx[4], jx[0], jx[3], 0.0f, jx[1], 0.0f};
+    const float jb[3 * 3] = {0.0f, jy[2], 0.0f, jy[4], jy[0], jy[3], 0.0f, jy[1], 0.0f};
+    return reshape_flux(un, u, su, ja + 4, jb + 4, 3, ka, kc);
 }
 
 /* One reshaping sweep of reshaping_lumen_or_ECM's loop
@@ -1433,69 +1512,120 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float kscale = 6.28318530718f / (ORG_MB * P.dx);
     const float MMf = (float)MM;
 
-    /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209) */
-#pragma unroll 8
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
-        float v = ph[g];
-        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
-        int idx = turnover_label(v, lum[g], ecm[g], ph1[g], ph2[g]);
-        lab[e] = (unsigned char)idx;
-        float r = rh[g];
-        float at = P.atan_[idx], ao = P.aort[idx], bb = P.bb[idx];
-        float r3 = r * r * r;
-        float base = -at * r3 + bb * r3 * r;
-        float pxx = base, pxy = 0.0f, pyv = base;
-        if (ng > 0.0f) {
-            const float rng = hw_rcp(ng);
-            float c = (at - ao) * r3 * inv4 * rng * rng;
-            pxx += c * dxp * dxp;
-            pxy += c * dxp * dyp;
-            pyv += c * dyp * dyp;
+    /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209), in
+     * batches of 4 pixels per thread: the 40 loads of a batch issue before
+     * any use (byte offsets from uniform bases, scheduling barrier) */
+    const unsigned rN = 4u * (unsigned)N;
+#pragma unroll 1
+    for (int qb = 0; qb < 16; qb += 4) {
+        float fv[4], fxp[4], fxm[4], fyp[4], fym[4], fl[4], fe[4], f1[4], f2[4], fr[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int e = t + 1024 * (qb + k), i = e >> 7, j = e & 127;
+            const unsigned g = 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + j));
+            fv[k] = ldb(ph, g); fxp[k] = ldb(ph, g + rN); fxm[k] = ldb(ph, g - rN);
+            fyp[k] = ldb(ph, g + 4); fym[k] = ldb(ph, g - 4);
+            fl[k] = ldb(lum, g); fe[k] = ldb(ecm, g); f1[k] = ldb(ph1, g); f2[k] = ldb(ph2, g);
+            fr[k] = ldb(rh, g);
         }
-        float d = v - 0.5f;
-        float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
-        float react = -P.deg[idx] * r + P.gen[idx] * pnd;
-        rbox[e] = r + P.dt * react;
-        SB[e] = pyv;
-        sm[i * ORG_LS + j] = make_float2(pxx, pxy);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int e = t + 1024 * (qb + k), i = e >> 7, j = e & 127;
+            const float v = fv[k];
+            const float dxp = fxp[k] - fxm[k], dyp = fyp[k] - fym[k];
+            const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+            const int idx = turnover_label(v, fl[k], fe[k], f1[k], f2[k]);
+            lab[e] = (unsigned char)idx;
+            const float r = fr[k];
+            const float at = P.atan_[idx], ao = P.aort[idx], bb = P.bb[idx];
+            const float r3 = r * r * r;
+            const float base = -at * r3 + bb * r3 * r;
+            float pxx = base, pxy = 0.0f, pyv = base;
+            if (ng > 0.0f) {
+                const float rng = hw_rcp(ng);
+                const float c = (at - ao) * r3 * inv4 * rng * rng;
+                pxx += c * dxp * dxp;
+                pxy += c * dxp * dyp;
+                pyv += c * dyp * dyp;
+            }
+            const float d = v - 0.5f;
+            const float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
+            const float react = -P.deg[idx] * r + P.gen[idx] * pnd;
+            rbox[e] = r + P.dt * react;
+            SB[e] = pyv;
+            sm[i * ORG_LS + j] = make_float2(pxx, pxy);
+        }
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
-#pragma unroll 4
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, l = (e >> 7) * ORG_LS + (e & 127);
-        SA[e] = sm[l];
-        sm[l] = make_float2(SB[e], 0.0f);      /* this thread wrote SB[e] in P1 */
+    {
+        float yy[16];                      /* this thread wrote SB[e] in P1 */
+#pragma unroll
+        for (int q = 0; q < 16; q++) yy[q] = ldb(SB, 4u * (unsigned)(t + 1024 * q));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int e = t + 1024 * q, l = (e >> 7) * ORG_LS + (e & 127);
+            SA[e] = sm[l];
+            sm[l] = make_float2(yy[q], 0.0f);
+        }
     }
     __syncthreads();
     fft2d_128<-1>(sm, tw);
     /* P4: per pair {k, -k}: split F(pi_xx + i pi_xy), solve both modes
      * (rho_functions.cpp:133-190), keep the part of v whose inverse is real,
      * Z = H v_x + i H v_y written in place at k and -k. */
-#pragma unroll 4
-    for (int u = t; u < NPAIR; u += 1024) {
-        int i, j;
-        if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
-        else { const int w = u - 63 * ORG_MB; i = (w < 65) ? 0 : 64; j = (w < 65) ? w : w - 65; }
-        const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
-        const int ek = i * ORG_MB + j, em = im * ORG_MB + jm;
-        const int lk = i * ORG_LS + j, lm = im * ORG_LS + jm;
-        const float2 A = SA[ek], Am = SA[em], B = sm[lk], Bm = sm[lm];
-        const float2 pxx = make_float2(0.5f * (A.x + Am.x), 0.5f * (A.y - Am.y));
-        const float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
-        const float2 pxxm = make_float2(pxx.x, -pxx.y), pxym = make_float2(pxy.x, -pxy.y);
-        const int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
-        const int irm = (im < ORG_MB / 2) ? im : im - ORG_MB, jrm = (jm < ORG_MB / 2) ? jm : jm - ORG_MB;
-        float2 vxk, vyk, vxm, vym;
-        velocity_mode(P.visc[lab[ek]], P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vxk, vyk);
-        velocity_mode(P.visc[lab[em]], P.friction, kscale * irm, kscale * jrm, pxxm, pxym, Bm, vxm, vym);
-        const float2 hx = make_float2(0.5f * (vxk.x + vxm.x), 0.5f * (vxk.y - vxm.y));
-        const float2 hy = make_float2(0.5f * (vyk.x + vym.x), 0.5f * (vyk.y - vym.y));
-        sm[lk] = make_float2(hx.x - hy.y, hx.y + hy.x);
-        if (em != ek) sm[lm] = make_float2(hx.x + hy.y, hy.x - hx.y);
+    {
+        /* pairs u = t + 1024 m, m = 0..8 (NPAIR = 8 * 1024 + 2), in groups of
+         * 4 (the last, m = 8, exists for t < 2 only); a group's spectra are
+         * loaded before any use */
+        auto pair_ij = [](int u, int &i, int &j) {
+            if (u < 63 * ORG_MB) { i = 1 + (u >> 7); j = u & 127; }
+            else { const int w = u - 63 * ORG_MB; i = (w < 65) ? 0 : 64; j = (w < 65) ? w : w - 65; }
+        };
+        auto solve_pair = [&](int u, float2 A, float2 Am) {
+            int i, j;
+            pair_ij(u, i, j);
+            const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+            const int ek = i * ORG_MB + j, em = im * ORG_MB + jm;
+            const int lk = i * ORG_LS + j, lm = im * ORG_LS + jm;
+            const float2 B = sm[lk], Bm = sm[lm];
+            const float2 pxx = make_float2(0.5f * (A.x + Am.x), 0.5f * (A.y - Am.y));
+            const float2 pxy = make_float2(0.5f * (A.y + Am.y), 0.5f * (Am.x - A.x));
+            const float2 pxxm = make_float2(pxx.x, -pxx.y), pxym = make_float2(pxy.x, -pxy.y);
+            const int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
+            const int irm = (im < ORG_MB / 2) ? im : im - ORG_MB, jrm = (jm < ORG_MB / 2) ? jm : jm - ORG_MB;
+            float2 vxk, vyk, vxm, vym;
+            velocity_mode(P.visc[lab[ek]], P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vxk, vyk);
+            velocity_mode(P.visc[lab[em]], P.friction, kscale * irm, kscale * jrm, pxxm, pxym, Bm, vxm, vym);
+            const float2 hx = make_float2(0.5f * (vxk.x + vxm.x), 0.5f * (vxk.y - vxm.y));
+            const float2 hy = make_float2(0.5f * (vyk.x + vym.x), 0.5f * (vyk.y - vym.y));
+            sm[lk] = make_float2(hx.x - hy.y, hx.y + hy.x);
+            if (em != ek) sm[lm] = make_float2(hx.x + hy.y, hy.x - hx.y);
+        };
+#pragma unroll 1
+        for (int m0 = 0; m0 < 8; m0 += 4) {
+            float2 A[4], Am[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int i, j;
+                pair_ij(t + 1024 * (m0 + k), i, j);
+                const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+                A[k] = ldb2(SA, 8u * (unsigned)(i * ORG_MB + j));
+                Am[k] = ldb2(SA, 8u * (unsigned)(im * ORG_MB + jm));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) solve_pair(t + 1024 * (m0 + k), A[k], Am[k]);
+        }
+        const int u8 = t + 1024 * 8;
+        if (u8 < NPAIR) {
+            int i, j;
+            pair_ij(u8, i, j);
+            const int im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+            solve_pair(u8, SA[i * ORG_MB + j], SA[im * ORG_MB + jm]);
+        }
     }
     __syncthreads();
     fft2d_128<1>(sm, tw);
@@ -1504,7 +1634,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     for (int q = 0; q < 16; q++) {
         const int e = t + 1024 * q, i = e >> 7, j = e & 127;
         float2 z = sm[i * ORG_LS + j];
-        float r = rh[(size_t)(x0 + i) * N + (y0 + j)];
+        const float r = ldb(rh, 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + j)));
         if (write_v) {
             vxbox[(size_t)n * MM + e] = z.x / MMf;
             vybox[(size_t)n * MM + e] = z.y / MMf;
@@ -1569,64 +1699,124 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
  * the stop test runs in the workgroup.  (x0, y0): window origin. */
 
 /* Phase-field update on the window; result to the LDS iterate pnl (row
- * stride W + 2, one-pixel ring) or, with pnl == nullptr, to pw.  psi at
- * window pixel (i, j) is ps0[i * pst + j] (the psi plane or a copy of the
- * window plus ring in LDS). */
+ * stride W + 2, one-pixel ring) or, with pnl == nullptr, to pw.  Planes are
+ * addressed with 32-bit unsigned offsets from their (uniform) base, so the
+ * 36 loads of a pixel take one register each and issue together. */
 static __device__ void cells_update(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
                                     const float *__restrict__ rh, const float *__restrict__ hf,
-                                    const float *__restrict__ ps0, const int pst, const float *__restrict__ hlum,
+                                    const float *__restrict__ psi, const float *__restrict__ hlum,
                                     const float *__restrict__ hecm, const float fn, float *__restrict__ pnl,
-                                    float *__restrict__ pw) {
+                                    float *__restrict__ pw, float *__restrict__ jx = nullptr,
+                                    float *__restrict__ jy = nullptr) {
     constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
-    const int N = P.N;
+    const unsigned N = (unsigned)P.N;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
     const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
+#pragma unroll 1
     for (int e = threadIdx.x; e < NPX; e += 1024) {
         const int i = e / W, j = e - i * W;
-        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
-        float v = ph[g];
-        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
-        float out = v;
-        if (ng > 0.0f) {
-            float d = v - 0.5f;
-            out -= cplc * hw_exp(-d * d * isig2) *
-                   ((rh[g + N] - rh[g - N]) * dxp + (rh[g + 1] - rh[g - 1]) * dyp) * hw_rcp(ng);
+        const unsigned g = 4u * ((unsigned)(x0 + i) * N + (unsigned)(y0 + j)), r = 4u * N;
+        /* all 45 loads first, in one basic block, so they issue together */
+        const unsigned o[9] = {g, g + r, g - r, g + 4, g - 4, g + r + 4, g + r - 4, g - r - 4, g - r + 4};
+        float fp[5], fr[4], fs[9], fh[9], fl[9], fe[9];
+#pragma unroll
+        for (int k = 0; k < 5; k++) fp[k] = ldb(ph, o[k]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) fr[k] = ldb(rh, o[k + 1]);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            fs[k] = ldb(psi, o[k]); fh[k] = ldb(hf, o[k]); fl[k] = ldb(hlum, o[k]); fe[k] = ldb(hecm, o[k]);
         }
+        __builtin_amdgcn_sched_barrier(0);   /* keep the loads ahead of their uses */
+        const float v = fp[0];
+        const float dxp = fp[1] - fp[2], dyp = fp[3] - fp[4];
+        const float drx = fr[0] - fr[1], dry = fr[2] - fr[3];
+        const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+        const float d = v - 0.5f;
+        const float cterm = cplc * hw_exp(-d * d * isig2) * (drx * dxp + dry * dyp) * hw_rcp(ng);
+        float out = (ng > 0.0f) ? v - cterm : v;
         /* temp_phase = psi - hfunc_n (cell_phase_functions.cpp:150) on the 3x3 stencil */
-        const float *q = ps0 + i * pst + j;
-        float tc = q[0] - hf[g];
-        float tp0 = q[pst] - hf[g + N], tm0 = q[-pst] - hf[g - N];
-        float t0p = q[1] - hf[g + 1], t0m = q[-1] - hf[g - 1];
-        float tpp = q[pst + 1] - hf[g + N + 1], tpm = q[pst - 1] - hf[g + N - 1];
-        float tmm = q[-pst - 1] - hf[g - N - 1], tmp = q[-pst + 1] - hf[g - N + 1];
-        float lt = -12.0f * tc + 2.0f * (tp0 + tm0 + t0p + t0m) + tpp + tpm + tmm + tmp;
-        float ll = L9AT(hlum, g, (size_t)N), le = L9AT(hecm, g, (size_t)N);
+        float tt[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) tt[k] = fs[k] - fh[k];
+        const float tc = tt[0];
+        const float lt = -12.0f * tc + 2.0f * (tt[1] + tt[2] + tt[3] + tt[4]) + tt[5] + tt[6] + tt[7] + tt[8];
+        const float ll = -12.0f * fl[0] + 2.0f * (fl[1] + fl[2] + fl[3] + fl[4]) + fl[5] + fl[6] + fl[7] + fl[8];
+        const float le = -12.0f * fe[0] + 2.0f * (fe[1] + fe[2] + fe[3] + fe[4]) + fe[5] + fe[6] + fe[7] + fe[8];
         out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
-               (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * hlum[g] + P.eta_lc * ll * inv4
-                - P.beta_ec * hecm[g] + P.eta_ec * le * inv4);
+               (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * fl[0] + P.eta_lc * ll * inv4
+                - P.beta_ec * fe[0] + P.eta_ec * le * inv4);
         if (pnl) pnl[(i + 1) * WH + (j + 1)] = out;
         else pw[e] = out;
+        if (jx) {     /* the normals of reshaping_cells at this pixel (see cells_normals) */
+            const float h = 1.0f / (2.0f * P.dx), rng = hw_rcp(ng);
+            const bool nz = ng > 0.0f;
+            jx[(i + 1) * WH + (j + 1)] = nz ? h * dxp * rng : 0.0f;
+            jy[(i + 1) * WH + (j + 1)] = nz ? h * dyp * rng : 0.0f;
+        }
+    }
+}
+
+/* cells_normals on the one-pixel ring around the window only (the window's
+ * normals come from cells_update). */
+static __device__ void cells_normals_ring(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
+                                          float *__restrict__ jx, float *__restrict__ jy,
+                                          float *__restrict__ phase) {
+    constexpr int WH = ORG_WH, NR = 4 * WH - 4;
+    const unsigned N = (unsigned)P.N, r = 4u * N;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
+    for (int q = threadIdx.x; q < NR; q += blockDim.x) {
+        int a, b;
+        if (q < WH) { a = 0; b = q; }
+        else if (q < 2 * WH) { a = WH - 1; b = q - WH; }
+        else if (q < 3 * WH - 2) { a = q - 2 * WH + 1; b = 0; }
+        else { a = q - (3 * WH - 2) + 1; b = WH - 1; }
+        const unsigned g = 4u * ((unsigned)(x0 + a - 1) * N + (unsigned)(y0 + b - 1));
+        const float c0 = ldb(ph, g), dxp = ldb(ph, g + r) - ldb(ph, g - r), dyp = ldb(ph, g + 4) - ldb(ph, g - 4);
+        const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+        const float rng = hw_rcp(ng);
+        const bool nz = ng > 0.0f;
+        const int e = a * WH + b;
+        jx[e] = nz ? h * dxp * rng : 0.0f;
+        jy[e] = nz ? h * dyp * rng : 0.0f;
+        phase[e] = c0;
     }
 }
 
 /* Normals on the window plus ring from the start-of-step phi (to jx, jy,
- * row stride W + 2) and the ring of the relaxed field (to phase). */
+ * row stride W + 2) and the ring of the relaxed field (to phase).  Pixels in
+ * batches of 4 per thread, their 20 loads issued before any use. */
 static __device__ void cells_normals(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
                                      float *__restrict__ jx, float *__restrict__ jy, float *__restrict__ phase) {
-    constexpr int W = ORG_W, WH = ORG_WH;
-    const int N = P.N;
+    constexpr int W = ORG_W, WH = ORG_WH, NP = WH * WH, B = 4;
+    const unsigned N = (unsigned)P.N, r = 4u * N;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx), h = 1.0f / (2.0f * P.dx);
-    for (int e = threadIdx.x; e < WH * WH; e += 1024) {
-        const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
-        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
-        float dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
-        float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
-        float a = 0.0f, b = 0.0f;
-        if (ng > 0.0f) { const float rng = hw_rcp(ng); a = h * dxp * rng; b = h * dyp * rng; }
-        jx[e] = a; jy[e] = b;
-        if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = ph[g];
+#pragma unroll 1
+    for (int e0 = threadIdx.x; e0 < NP; e0 += B * 1024) {
+        float c0[B], xp[B], xm[B], yp[B], ym[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int e = min(e0 + 1024 * k, NP - 1);
+            const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
+            const unsigned g = 4u * ((unsigned)(x0 + i) * N + (unsigned)(y0 + j));
+            c0[k] = ldb(ph, g); xp[k] = ldb(ph, g + r); xm[k] = ldb(ph, g - r);
+            yp[k] = ldb(ph, g + 4); ym[k] = ldb(ph, g - 4);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int e = e0 + 1024 * k;
+            if (e >= NP) break;
+            const int i = e / WH - 1, j = e - (i + 1) * WH - 1;
+            const float dxp = xp[k] - xm[k], dyp = yp[k] - ym[k];
+            const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+            const float rng = hw_rcp(ng);
+            const bool nz = ng > 0.0f;
+            jx[e] = nz ? h * dxp * rng : 0.0f;
+            jy[e] = nz ? h * dyp * rng : 0.0f;
+            if (i < 0 || j < 0 || i >= W || j >= W) phase[e] = c0[k];
+        }
     }
 }
 
@@ -1654,12 +1844,33 @@ static __device__ void cells_relax(const DevP &P, const int n, const int x0, con
         }
         __syncthreads();
         double acc = 0.0;
-        for (int e = t; e < NPX; e += 1024) {          /* flux update (:498-549) */
-            const int i = e / W, l = (i + 1) * WH + (e - i * W + 1);
-            float u = pnl[l];
-            float nv = reshape_flux(u, phase + l, WH, jx + l, jy + l, WH, ka, kc);
-            acc += (double)(fabsf(nv - u) * dscale);
-            pnl[l] = nv;
+        /* flux update (:498-549); 4 pixels per thread at a time, their 40
+         * normal loads issued before any use */
+#pragma unroll 1
+        for (int e0 = t; e0 < NPX; e0 += 4 * 1024) {
+            float nx[4][5], ny[4][5];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int e = min(e0 + 1024 * k, NPX - 1);
+                const int i = e / W, l = (i + 1) * WH + (e - i * W + 1);
+                const unsigned b = 4u * (unsigned)l, r = 4u * WH;
+                nx[k][0] = ldb(jx, b); nx[k][1] = ldb(jx, b + r); nx[k][2] = ldb(jx, b - r);
+                nx[k][3] = ldb(jx, b + 4); nx[k][4] = ldb(jx, b - 4);
+                ny[k][0] = ldb(jy, b); ny[k][1] = ldb(jy, b + r); ny[k][2] = ldb(jy, b - r);
+                ny[k][3] = ldb(jy, b + 4); ny[k][4] = ldb(jy, b - 4);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int e = e0 + 1024 * k;
+                if (e < NPX) {
+                    const int i = e / W, l = (i + 1) * WH + (e - i * W + 1);
+                    const float u = pnl[l];
+                    const float nv = reshape_flux_j(u, phase + l, WH, nx[k], ny[k], ka, kc);
+                    acc += (double)(fabsf(nv - u) * dscale);
+                    pnl[l] = nv;
+                }
+            }
         }
         rdev = block_sum_d(acc, red);                  /* (:554-562) */
         cnt++;
@@ -1694,8 +1905,8 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
     float *ph = phi + n * NN;
     float *pw = phin_win + (size_t)n * NPX;
     if (mode & 1) {
-        cells_update(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi + (size_t)x0 * P.N + y0, P.N, hlum, hecm, f0[n],
-                     (mode & 2) ? pnl : nullptr, pw);
+        cells_update(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi, hlum, hecm, f0[n], (mode & 2) ? pnl : nullptr,
+                     pw);
         if (!(mode & 2)) return;
     } else {
         for (int e = threadIdx.x; e < NPX; e += 1024) {
@@ -1816,14 +2027,15 @@ __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
  *                      reshaping loops to their stop tests (one barrier per
  *                      sweep)
  *   last ncell         phase field of cell n: update_all_cells and
- *                      reshaping_cells; the new window goes to phi_win
+ *                      reshaping_cells; the new window goes to phi_win; then
+ *                      the next step's centre and support box to (xn, yn, bn)
  * Every role reads only the state at the start of the launch and its own
  * results; only the lumen/ECM workgroups synchronise, among themselves.  The
  * copy-back (k_post) runs after the launch.  The FP sums take the same
  * operands in the same order as the per-function kernels, so both forms
  * agree bit for bit. */
 #define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8 + ORG_MB * ORG_MB)
-#define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8)
+#define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8 + 48 * 8)
 #define ORG_LDS_FUSED (ORG_LDS_SPECTRAL > ORG_LDS_CELLS ? ORG_LDS_SPECTRAL : ORG_LDS_CELLS)
 __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
@@ -1866,12 +2078,21 @@ __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
         const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
         float *ph = a.phi + n * NN;
         float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
-        cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi + (size_t)x0 * P.N + y0, P.N, a.hlum,
-                     a.hecm, a.f0[n], pnl, nullptr);
-        cells_normals(P, x0, y0, ph, jx, jy, phase);
-        __syncthreads();
+        cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, a.f0[n], pnl, nullptr,
+                     jx, jy);
+        cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
+        __syncthreads();      /* the workgroup reads back its own stores */
         cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
                     a.phi_win + (size_t)n * NPX, a.counts, a.err);
+        /* the next step's centre and support box (find_centres_cells of the
+         * next iteration, useful_functions.cpp:193-227) from the new phi:
+         * the window from LDS, the rest of the support box from the plane */
+        long long *lred = (long long *)(lds + 2 * WH * WH * 4 + 128);
+        const int i0 = a.box[4 * n], i1 = a.box[4 * n + 1], j0 = a.box[4 * n + 2], j1 = a.box[4 * n + 3];
+        long long X, Y, C;
+        centroid_sums_win(ph, P.N, i0, i1, j0, j1, x0, y0, phase, lred, X, Y, C);
+        if (threadIdx.x == 0)
+            place_window(P, n, X, Y, C, a.xc[n], a.yc[n], i0, i1, j0, j1, a.xn, a.yn, a.bn, a.err);
     }
 }
 
@@ -1910,20 +2131,15 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
 }
 
 /* k_post: the copy-back of a timestep (run_with_active_gels.cpp:172-182):
- * rho box and phi window of every cell (workgroups 0..ncell-1), then that
- * cell's centre and support box for the next step (find_centres_cells of
- * the next iteration, useful_functions.cpp:193-227) into (xn, yn, bn);
- * lum <- lumn, ECM <- ECMn (the other workgroups, see commit_field).  Clears
- * k_step's barrier counter and abort flag. */
+ * rho box and phi window of every cell (workgroups 0..ncell-1); lum <- lumn,
+ * ECM <- ECMn (the other workgroups, see commit_field).  Clears k_step's
+ * barrier counter and abort flag. */
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
     __shared__ long long red[48];
     const int b = blockIdx.x;
     if (b == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
     if (b < P.nc) {
         commit_cell_body(P, b, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
-        drain_stores();
-        __syncthreads();
-        cell_centroid(P, b, a.phi, a.xc, a.yc, a.box, a.xn, a.yn, a.bn, a.err, red);
     } else {
         const int blk = b - P.nc, nblk = gridDim.x - P.nc;
         const LumState sl = a.lst[0], se = a.lst[1];

@@ -7,10 +7,10 @@ reductions, inputs resident in HBM before the timed region.
 A step is one full timestep (main_code/run_with_active_gels.cpp:118-184):
 centroids, turnover labels, h, psi, areas, the spectral cortex update of
 every cell, lumen/ECM update + reshaping, phase-field update + reshaping of
-every cell, copy-back.  The timed form is the batched one (one launch per
-stage for all cells, whole step replayed from a hipGraph).  The plain
-per-function drop-in form (the reference's call sequence with its host
-round trips) is timed beside it on a shorter run.
+every cell, copy-back.  The timed form is the batched one (three launches per
+step for all cells -- k_pre, k_step, k_post -- replayed from hipGraphs of 8
+steps).  The plain per-function drop-in form (the reference's call sequence
+with its host round trips) is timed beside it on a shorter run.
 
 Printed: one JSON line (see the repository's benchmark contract) with a
 ``roofline`` object for the dominant kernel (algorithmic bytes per launch /
@@ -31,17 +31,47 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-timesteps/s, 64 cells @1024^2: FP32+batch+FNO vs plain, phi L2err<=1e-3"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per cell of the dominant launch, k_cortex_cells
+# Algorithmic HBM bytes per cell of the dominant launch, k_step
 # (FP32 = 4 B; M = FFT box side, W = window side):
-#   spectral role: read phi_n, phi_n-1, phi_n+1, rho_n, lum, ECM on the box,
-#                  write rho_n                      -> 7 x 4 B x M^2
-#   cell role:     read phi_n, rho_n, hfunc_n, psi, h(lum), h(ECM) on the
-#                  window, write phi_n and phin     -> 8 x 4 B x W^2
-# The lumen/ECM loop role of the same launch is not counted (its traffic
-# depends on the active box and the sweep count), so `achieved` is a lower
-# bound.
+#   spectral role:    read phi_n, phi_n-1, phi_n+1, rho_n, lum, ECM on the box,
+#                     write rho_n on the box                   -> 7 x 4 B x M^2
+#   phase-field role: read phi_n, rho_n, hfunc_n, psi, h(lum), h(ECM) on the
+#                     window, write phi_n and phin, read phi_n for the next
+#                     centroid (support box >= window)           -> 9 x 4 B x W^2
+# The lumen/ECM role of the same launch is not counted (its traffic depends
+# on the live box and the sweep count), so `achieved` is a lower bound.
 def alg_bytes_per_cell(M: int, W: int) -> int:
-    return 28 * M * M + 32 * W * W
+    return 28 * M * M + 36 * W * W
+
+
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_organoid64_1024_pmc.md")
+
+
+def measured_traffic(path: str = PMC_SUMMARY):
+    """HBM bytes per k_step launch from the committed rocprofv3 --pmc summary
+    (line 'traffic_bytes_per_launch: <int>'), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        return None
+    return None
+
+
+def calib_copy(mib: int) -> int:
+    """A plain copy kernel of known size (reads and writes `mib` MiB) for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -53,7 +83,11 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        return calib_copy(args.calib_copy)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), --gpus must be 1"}))
@@ -93,18 +127,19 @@ def main() -> int:
     kt = org.kernel_times()
     org.profile(False)
     per_kernel = {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}
-    avg_ms = per_kernel["spectral"][0]          # profiling slot of k_cortex_cells
+    avg_ms = per_kernel["step"][0]              # profiling slot of k_step
     bytes_launch = alg_bytes_per_cell(p.M, p.W) * nc
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if wl.name == "organoid64_1024" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_cortex_cells", "avg_launch_ms": round(avg_ms, 5),
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5),
                 "alg_bytes_per_launch": bytes_launch}
-    # profiling slots of the batched step's launches (the prep and cells
-    # slots are used by the per-function entry points only)
+    # profiling slots of the batched step's launches (the other slots are
+    # used by the per-function entry points)
     names = {"centres": "k_pre", "prep": "k_prep", "psi": "k_psi", "lumecm": "k_lumecm_update",
-             "lumreshape": "k_lum_normals", "spectral": "k_cortex_cells", "cells": "k_cells",
-             "commit": "k_post"}
+             "lumreshape": "k_lum_normals", "spectral": "k_spectral", "cells": "k_cells",
+             "commit": "k_post", "step": "k_step"}
     kernel_ms = {names[k]: round(v[0], 5) for k, v in per_kernel.items() if v[1]}
     cells_it, lum_it, ecm_it = org.iterations()
 
@@ -117,7 +152,7 @@ def main() -> int:
     for _ in range(args.plain_steps):
         plain.plain_step()
     plain.sync()
-    plain_rate = nc * args.plain_steps / (time.perf_counter() - t0)
+    plain_rate = nc * args.plain_steps / (time.perf_counter() - t0) if args.plain_steps > 0 else None
     plain.close()
 
     cpu = None
@@ -141,10 +176,10 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "cells": nc, "grid": p.N, "zone_size": p.zone_size,
-                   "form": "batched: one launch per stage for all cells, whole step replayed from one hipGraph "
-                           "(FNO surrogate not used)",
-                   "plain_dropin_cell_steps_per_s": round(plain_rate, 2),
-                   "speedup_vs_plain": round(value / plain_rate, 3),
+                   "form": "batched: k_pre + k_step + k_post per step for all cells, replayed from hipGraphs "
+                           "of 8 steps (FNO surrogate not used)",
+                   "plain_dropin_cell_steps_per_s": round(plain_rate, 2) if plain_rate else None,
+                   "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
                    "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it}},
         "roofline": roofline,

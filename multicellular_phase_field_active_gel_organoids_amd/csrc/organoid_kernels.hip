@@ -30,7 +30,7 @@
  *   k_spectral      per cell: active stress, 2 forward FFTs, the per-mode
  *                   friction-viscosity solve, Hermitian split, inverse FFT of
  *                   the velocity, rho*v, forward FFT, spectral divergence,
- *                   inverse FFT -> new rho on the FFT box  (4 2D FFTs of
+ *                   inverse FFT -> new rho on the FFT box  (5 2D FFTs of
  *                   128 x 128 in LDS instead of the reference's 10)
  *   k_lumecm_update lumen / ECM explicit step + Dirichlet edges
  *   k_lum_normals   |grad| of the old field, normals of the new one, periodic ring
@@ -1476,8 +1476,8 @@ __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restr
 /* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
  *
  * One workgroup (1024 threads) per cell, one 128 x 128 complex field in LDS,
- * two 128 x 128 complex scratch planes per cell in global memory (L2
- * resident) for the spectra that are needed at k and -k.
+ * a 128 x 128 complex and a 128 x 128 real scratch plane per cell in global
+ * memory (L2 resident): F(pi_xx + i pi_xy), needed at k and -k, and pi_yy.
  * By linearity of the reference's spectral update, Re(F^-1[F(rho) +
  * dt F(react) + D]) / M^2 = rho + dt*react + Re(F^-1 D) / M^2, so the
  * transforms of rho and of the reaction term (rho_functions.cpp:415-422,

@@ -191,7 +191,8 @@ enum {
     ORG_K_CENTRES = 0, ORG_K_PREP = 1, ORG_K_PSI = 2, ORG_K_LUMECM = 3,
     ORG_K_LUMRESHAPE = 4, ORG_K_SPECTRAL = 5, ORG_K_CELLS = 6, ORG_K_COMMIT = 7,
     ORG_K_STEP = 8,       /* the batched step's main launch (org_step) */
-    ORG_K_COUNT = 9
+    ORG_K_MID = 9,        /* batched: copy-back merged with the next step's first part */
+    ORG_K_COUNT = 10
 };
 
 /* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of

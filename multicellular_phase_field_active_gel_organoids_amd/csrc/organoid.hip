@@ -58,9 +58,11 @@ struct org_ctx {
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
     int *uni = nullptr, *jbox = nullptr;   /* union of windows; per-workgroup boxes of nonzero normals */
     int *xn = nullptr, *yn = nullptr, *bn = nullptr;  /* next batched step's centres and support boxes */
+    int *xp = nullptr, *yp = nullptr;      /* copy of a step's centres for k_mid */
     bool next_ready = false;               /* xn, yn, bn hold the centres of the current phi */
-    int npsi = 128;                        /* psi workgroups of k_pre */
+    int npsi = 0;                          /* psi workgroups of k_pre and k_mid */
     int nh = 0;                            /* h(lum), h(ECM) workgroups of k_pre */
+    int nlp = 0;                           /* lumen/ECM copy-back workgroups of k_mid and k_post */
     int nparts_h = 0;                      /* parts of the sum of h(ECM): 64 x 64 tiles + the ring */
     int *live = nullptr, *live_part = nullptr;  /* live boxes of lum and ECM, k_post's partial boxes */
     bool ring_pending = true;              /* rho ghost ring not yet written since org_prepare */
@@ -195,7 +197,7 @@ int L_centroid(org_ctx *c) {
 /* the lumen/ECM state of the batched step from the current fields (see
  * k_live_init) */
 int L_live_init(org_ctx *c) {
-    hipLaunchKernelGGL(k_live_reset, dim3(1), dim3(256), 0, c->stream, c->live, c->live_part, c->grid_g, c->N);
+    hipLaunchKernelGGL(k_live_reset, dim3(1), dim3(256), 0, c->stream, c->live, c->live_part, c->nlp, c->N);
     TRY(launch_check());
     hipLaunchKernelGGL(k_live_init, dim3(c->nparts_h), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum,
                        c->hecm, c->part_h, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey,
@@ -276,10 +278,11 @@ StepArgs step_args(org_ctx *c) {
     a.phi = c->phi; a.rho = c->rho; a.hfunc = c->hfunc;
     a.xc = c->xc; a.yc = c->yc; a.box = c->box; a.uni = c->uni; a.counts = c->counts;
     a.xn = c->xn; a.yn = c->yn; a.bn = c->bn; a.npsi = c->npsi;
+    a.xp = c->xp; a.yp = c->yp;
     a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
     a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
     a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nh = c->nh;
-    a.live = c->live; a.live_part = c->live_part; a.nlp = c->grid_g;
+    a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
     a.jlx = c->jlx; a.jly = c->jly; a.jex = c->jex; a.jey = c->jey;
     a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc;
@@ -300,9 +303,15 @@ int L_step(org_ctx *c) {
     hipLaunchKernelGGL(k_step, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
+/* copy-back of one step and the first part of the next (see k_mid) */
+int L_mid(org_ctx *c) {
+    Prof pr(c, ORG_K_MID);
+    hipLaunchKernelGGL(k_mid, dim3(c->nc + c->npsi + c->nlp), dim3(1024), 0, c->stream, c->dp, step_args(c));
+    return launch_check();
+}
 int L_post(org_ctx *c) {
     Prof pr(c, ORG_K_COMMIT);
-    hipLaunchKernelGGL(k_post, dim3(c->nc + c->grid_g), dim3(1024), 0, c->stream, c->dp, step_args(c));
+    hipLaunchKernelGGL(k_post, dim3(c->nc + c->nlp), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
 /* the one-time rho ghost ring (see k_ring_rho) */
@@ -329,14 +338,18 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
     }
 }
 
-/* One timestep, run_with_active_gels.cpp:118-184: centroids, hfunc, areas
- * and h(lum), h(ECM) (:123-142), the rest up to the copy-back (:130-172),
- * the copy-back (:172-182).  No host round trip: the whole step can be
- * captured in one hipGraph. */
-int enqueue_step(org_ctx *c) {
+/* nsteps timesteps, run_with_active_gels.cpp:118-184: hfunc, areas, psi
+ * and h(lum), h(ECM) of the first step (:123-142), then per step the rest up
+ * to the copy-back (:130-172) followed by the copy-back merged with the next
+ * step's first part, or by the plain copy-back (:172-182) after the last.
+ * No host round trip: the sequence can be captured in one hipGraph. */
+int enqueue_steps(org_ctx *c, int nsteps) {
+    if (nsteps <= 0) return ORG_OK;
     TRY(L_pre(c));
-    TRY(L_step(c));
-    TRY(L_post(c));
+    for (int s = 0; s < nsteps; s++) {
+        TRY(L_step(c));
+        TRY(s + 1 < nsteps ? L_mid(c) : L_post(c));
+    }
     return ORG_OK;
 }
 
@@ -385,7 +398,17 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     {
         const int t = (c->N + 63) / 64;
         c->nparts_h = t * t + 1;
-        c->nh = std::min(c->nparts_h, 128);
+        /* k_pre and k_mid: the cells' workgroups plus two equal roles that
+         * together fill the remaining CUs (one 1024-thread workgroup per CU
+         * starts at once; more wait for a free CU) */
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+        const int role = std::max(32, std::max(64, ncu - c->nc) / 2);
+        c->npsi = role;
+        c->nh = std::min(c->nparts_h, role);
+        c->nlp = role;
     }
     int nt = (c->N + 15) / 16;
     c->ntiles = nt * nt;
@@ -405,7 +428,8 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
     A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);
-    A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->grid_g); A(&c->counts, c->nc); A(&c->err, 1);
+    A(&c->xp, c->nc); A(&c->yp, c->nc);
+    A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->nlp); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, (size_t)c->nparts_h); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
@@ -625,12 +649,12 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     }
     c->next_ready = true;
     if (nsteps > 0 && c->ring_pending) {   /* first step after org_prepare: eager, plus the ghost ring */
-        TRY(enqueue_step(c));
+        TRY(enqueue_steps(c, 1));
         TRY(L_ring_if_pending(c));
         nsteps--;
     }
     if (!use_graph) {
-        for (int s = 0; s < nsteps; s++) TRY(enqueue_step(c));
+        TRY(enqueue_steps(c, nsteps));
         return harvest_device_error(c);
     }
     /* graphs of ORG_GRAPH_STEPS steps and of one step (the remainder) */
@@ -640,8 +664,7 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
         bool prof = c->prof;
         c->prof = false;                 /* no event records inside the graph */
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = ORG_OK;
-        for (int q = 0; q < nst && !rc; q++) rc = enqueue_step(c);
+        int rc = enqueue_steps(c, nst);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->prof = prof;

@@ -728,19 +728,22 @@ static __device__ __forceinline__ bool is_live(const float *__restrict__ X, cons
  * FP32 sum takes its operands in cell order, as the reference's loop over n
  * does.  With phi != nullptr, hfunc_n inside cell n's window (origin from
  * xc, yc) is taken as hfun(phi_n) (the value the same step's prep stores
- * there), so psi needs no wait for the prep.  Per group: a 64-entry list in
- * LDS (ORG_PSI_LDS_GROUP bytes).  Every thread of the workgroup calls this
+ * there), so psi needs no wait for the prep; with phi_win != nullptr as well,
+ * phi_n inside its previous window (origin xo, yo) is read from phi_win (the
+ * new window not yet copied back).  Per group: a 64-entry list in LDS
+ * (ORG_PSI_LDS_GROUP bytes).  Every thread of the workgroup calls this
  * together (it synchronises the workgroup). */
-#define ORG_PSI_LDS_GROUP (64 * 4 + 64 * 16 + 64 * 8 + 16)
+#define ORG_PSI_LDS_GROUP (64 * 4 + 64 * 16 + 64 * 16 + 16)
 static __device__ void psi_tile(const DevP &P, const float *__restrict__ hfunc, const float *__restrict__ phi,
                                 const int *__restrict__ box, const int *__restrict__ xc, const int *__restrict__ yc,
-                                const int4 u, const int ti0, const int tj0, unsigned char *lq,
-                                float *__restrict__ psi) {
+                                const float *__restrict__ phi_win, const int *__restrict__ xo,
+                                const int *__restrict__ yo, const int4 u, const int ti0, const int tj0,
+                                unsigned char *lq, float *__restrict__ psi) {
     constexpr int W = ORG_W;
     int *lcell = (int *)lq;
     int4 *lbox = (int4 *)(lq + 64 * 4);
-    int2 *lwin = (int2 *)(lq + 64 * 4 + 64 * 16);
-    int *lcnt = (int *)(lq + 64 * 4 + 64 * 16 + 64 * 8);
+    int4 *lwin = (int4 *)(lq + 64 * 4 + 64 * 16);
+    int *lcnt = (int *)(lq + 64 * 4 + 64 * 16 + 64 * 16);
     const int gt = threadIdx.x & 255, N = P.N;
     const size_t NN = (size_t)N * N;
     const int i = ti0 + (gt >> 4), j = tj0 + (gt & 15);
@@ -760,7 +763,8 @@ static __device__ void psi_tile(const DevP &P, const float *__restrict__ hfunc, 
                 const int pos = __popcll(m & ((1ull << gt) - 1ull));
                 lcell[pos] = n;
                 lbox[pos] = bx;
-                lwin[pos] = phi ? make_int2(xc[n] - P.zs, yc[n] - P.zs) : make_int2(-1, -1);
+                lwin[pos] = make_int4(phi ? xc[n] - P.zs : -1, phi ? yc[n] - P.zs : -1,
+                                      phi_win ? xo[n] - P.zs : -1, phi_win ? yo[n] - P.zs : -1);
             }
             if (gt == 0) *lcnt = __popcll(m);
         }
@@ -778,11 +782,15 @@ static __device__ void psi_tile(const DevP &P, const float *__restrict__ hfunc, 
                 for (int kk = 0; kk < 4; kk++) {
                     const int k = min(k0 + kk, cnt - 1);
                     const int4 bx = lbox[k];
-                    const int2 w = lwin[k];
+                    const int4 w = lwin[k];
                     cov[kk] = k0 + kk < cnt && i >= bx.x && i < bx.y && j >= bx.z && j < bx.w;
                     inw[kk] = w.x >= 0 && i >= w.x && i < w.x + W && j >= w.y && j < w.y + W;
-                    const float *pl = inw[kk] ? phi : hfunc;
-                    x[kk] = pl[lcell[k] * NN + e];
+                    const bool ino = w.z >= 0 && i >= w.z && i < w.z + W && j >= w.w && j < w.w + W;
+                    const bool fw = inw[kk] && ino;
+                    const float *pb = fw ? phi_win : (inw[kk] ? phi : hfunc);
+                    const size_t po = fw ? (size_t)lcell[k] * (W * W) + (size_t)((i - w.z) * W + (j - w.w))
+                                         : (size_t)lcell[k] * NN + e;
+                    x[kk] = pb[po];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -803,18 +811,19 @@ __global__ __launch_bounds__(256) void k_psi(DevP P, const float *__restrict__ h
     const int ti0 = (blockIdx.x / ntx) * 16, tj0 = (blockIdx.x % ntx) * 16;
     const int4 u = make_int4(uni[0], uni[1], uni[2], uni[3]);
     if (ti0 >= u.y || ti0 + 16 <= u.x || tj0 >= u.w || tj0 + 16 <= u.z) return;
-    psi_tile(P, hfunc, nullptr, box, nullptr, nullptr, u, ti0, tj0, lq, psi);
+    psi_tile(P, hfunc, nullptr, box, nullptr, nullptr, nullptr, nullptr, nullptr, u, ti0, tj0, lq, psi);
 }
 
-/* psi of the batched step (k_pre): the union of the windows from the
- * centres xc, yc, then every tile meeting it, dealt round-robin to the
- * 256-thread groups of workgroups blk = 0..nblk-1 (1024 threads each), with
- * hfun(phi) inside each cell's window.  scr: 4 * ORG_PSI_LDS_GROUP + 16
- * bytes of LDS. */
-static __device__ void psi_tiles_pre(const DevP &P, const int blk, const int nblk, const float *__restrict__ hfunc,
-                                     const float *__restrict__ phi, const int *__restrict__ box,
-                                     const int *__restrict__ xc, const int *__restrict__ yc,
-                                     float *__restrict__ psi, unsigned char *scr) {
+/* psi of the batched step: the union of the windows from the centres xc, yc,
+ * then every tile meeting it, dealt round-robin to the 256-thread groups of
+ * workgroups blk = 0..nblk-1 (1024 threads each), with hfun(phi) inside each
+ * cell's window (phi_win, xo, yo: see psi_tile).  scr: 4 * ORG_PSI_LDS_GROUP
+ * + 16 bytes of LDS. */
+static __device__ void psi_tiles_step(const DevP &P, const int blk, const int nblk, const float *__restrict__ hfunc,
+                                      const float *__restrict__ phi, const int *__restrict__ box,
+                                      const int *__restrict__ xc, const int *__restrict__ yc,
+                                      const float *__restrict__ phi_win, const int *__restrict__ xo,
+                                      const int *__restrict__ yo, float *__restrict__ psi, unsigned char *scr) {
     int *su = (int *)(scr + 4 * ORG_PSI_LDS_GROUP);
     const int g = threadIdx.x >> 8, ng = blockDim.x >> 8;
     if (threadIdx.x < 64) {           /* union of the windows, wave 0 */
@@ -838,7 +847,7 @@ static __device__ void psi_tiles_pre(const DevP &P, const int blk, const int nbl
         const int k = base + g;
         int ti0 = -1, tj0 = -1;
         if (k < ntile) { ti0 = (tr0 + k / ntc) * 16; tj0 = (tc0 + k % ntc) * 16; }
-        psi_tile(P, hfunc, phi, box, xc, yc, u, ti0, tj0, scr + g * ORG_PSI_LDS_GROUP, psi);
+        psi_tile(P, hfunc, phi, box, xc, yc, phi_win, xo, yo, u, ti0, tj0, scr + g * ORG_PSI_LDS_GROUP, psi);
     }
 }
 
@@ -1926,6 +1935,7 @@ struct StepArgs {
      * the next step, from k_post (or k_centroid before the first step) */
     float *phi, *rho, *hfunc;
     int *xc, *yc, *box, *xn, *yn, *bn, *uni, *counts;
+    int *xp, *yp;            /* copy of this step's centres for k_mid (whose preps overwrite xc, yc) */
     const double *s0;
     double *s;
     float *f0;
@@ -1954,6 +1964,36 @@ struct StepArgs {
     long long limit;
 };
 
+/* Live box of field f merged with the partial boxes of live pixels that the
+ * last copy-back found (wave 0 reduces them); valid in every thread.  sb: 4
+ * ints of LDS. */
+static __device__ int4 merged_live(const DevP &P, const StepArgs &a, const int f, int *sb) {
+    const int N = P.N;
+    if (threadIdx.x < 64) {
+        int c0 = N, c1 = -1, c2 = N, c3 = -1;
+        for (int k = threadIdx.x; k < a.nlp; k += 64) {
+            const int *q = a.live_part + ((size_t)f * a.nlp + k) * 4;
+            if (q[1] >= 0) { c0 = min(c0, q[0]); c1 = max(c1, q[1]); c2 = min(c2, q[2]); c3 = max(c3, q[3]); }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            c0 = min(c0, __shfl_xor(c0, o, 64)); c1 = max(c1, __shfl_xor(c1, o, 64));
+            c2 = min(c2, __shfl_xor(c2, o, 64)); c3 = max(c3, __shfl_xor(c3, o, 64));
+        }
+        if (threadIdx.x == 0) {
+            int4 r = ld_box(a.live, f);
+            if (c1 >= 0) {
+                const int4 q = make_int4(c0, c1 + 1, c2, c3 + 1);
+                r = box_empty(r) ? q : make_int4(min(r.x, q.x), max(r.y, q.y), min(r.z, q.z), max(r.w, q.w));
+            }
+            sb[0] = r.x; sb[1] = r.y; sb[2] = r.z; sb[3] = r.w;
+        }
+    }
+    __syncthreads();
+    const int4 r = make_int4(sb[0], sb[1], sb[2], sb[3]);
+    __syncthreads();
+    return r;
+}
+
 /* k_pre: the first part of a timestep (run_with_active_gels.cpp:123-142),
  * workgroups in three independent roles:
  *   0 .. ncell-1       cell n: centre and support box from (xn, yn, bn),
@@ -1965,8 +2005,7 @@ struct StepArgs {
  *                      (compute_all_hfunc_2d_tables_kernels) that the last
  *                      copy-back may have changed: the ring and, in box mode,
  *                      the tiles meeting a live box grown by one (all tiles in
- *                      full mode).  The first of them also adds the live
- *                      pixels found by the last copy-back to the live boxes. */
+ *                      full mode). */
 #define ORG_LDS_PRE (4 * ORG_PSI_LDS_GROUP + 16)
 __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
@@ -1974,32 +2013,11 @@ __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     if (b < P.nc) {
         cell_prep(P, b, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
     } else if (b < P.nc + a.npsi) {
-        psi_tiles_pre(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.psi, scr);
+        psi_tiles_step(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, nullptr, nullptr, nullptr, a.psi, scr);
     } else {
         const int hb = b - P.nc - a.npsi, N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
         const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
         const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-        if (hb == 0 && threadIdx.x < 128) {
-            /* live <- live U the copy-back's partial boxes (wave f for field
-             * f).  Other workgroups of this role may read the box before or
-             * after: both hold every pixel the last copy-back changed. */
-            const int f = threadIdx.x >> 6, l = threadIdx.x & 63;
-            int c0 = N, c1 = -1, c2 = N, c3 = -1;
-            for (int k = l; k < a.nlp; k += 64) {
-                const int *q = a.live_part + ((size_t)f * a.nlp + k) * 4;
-                if (q[1] >= 0) { c0 = min(c0, q[0]); c1 = max(c1, q[1]); c2 = min(c2, q[2]); c3 = max(c3, q[3]); }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                c0 = min(c0, __shfl_xor(c0, o, 64)); c1 = max(c1, __shfl_xor(c1, o, 64));
-                c2 = min(c2, __shfl_xor(c2, o, 64)); c3 = max(c3, __shfl_xor(c3, o, 64));
-            }
-            if (l == 0 && c1 >= 0) {
-                const int4 B = ld_box(a.live, f);
-                int4 r = make_int4(c0, c1 + 1, c2, c3 + 1);
-                if (!box_empty(B)) r = make_int4(min(B.x, r.x), max(B.y, r.y), min(B.z, r.z), max(B.w, r.w));
-                a.live[4 * f] = r.x; a.live[4 * f + 1] = r.y; a.live[4 * f + 2] = r.z; a.live[4 * f + 3] = r.w;
-            }
-        }
         for (int k = hb; k <= ntl; k += a.nh) {
             bool need = full || k == ntl;
             if (!need) {
@@ -2056,7 +2074,14 @@ __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
             if (threadIdx.x == 0) *a.sum_ecm = se;
             lum_reset_states(P, a.lst);
         }
-        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+        /* live boxes of this step: the stored ones plus what the last
+         * copy-back found (every workgroup merges the same parts; the first
+         * stores the result, and merging again gives the same boxes) */
+        const int4 BL = merged_live(P, a, 0, sb), BE = merged_live(P, a, 1, sb);
+        if (blk == 0 && threadIdx.x == 0) {
+            a.live[0] = BL.x; a.live[1] = BL.y; a.live[2] = BL.z; a.live[3] = BL.w;
+            a.live[4] = BE.x; a.live[5] = BE.y; a.live[6] = BE.z; a.live[7] = BE.w;
+        }
         if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
             if (blk == 0) lumecm_ring(P, a.L0, a.L1, a.E0, a.E1);
             lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
@@ -2091,8 +2116,10 @@ __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
         const int i0 = a.box[4 * n], i1 = a.box[4 * n + 1], j0 = a.box[4 * n + 2], j1 = a.box[4 * n + 3];
         long long X, Y, C;
         centroid_sums_win(ph, P.N, i0, i1, j0, j1, x0, y0, phase, lred, X, Y, C);
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
             place_window(P, n, X, Y, C, a.xc[n], a.yc[n], i0, i1, j0, j1, a.xn, a.yn, a.bn, a.err);
+            a.xp[n] = a.xc[n]; a.yp[n] = a.yc[n];
+        }
     }
 }
 
@@ -2150,6 +2177,63 @@ __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
         int *sb = (int *)red;
         commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
         commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
+    }
+}
+
+/* k_mid: the copy-back of one timestep and the first part of the next
+ * (k_post then k_pre) in one launch, workgroups in three independent roles:
+ *   0 .. ncell-1   cell n: rho box and phi window copy-back, then (same
+ *                  workgroup, after its own stores) hfunc on the next window
+ *                  and the area, with the centre and support box from
+ *                  (xn, yn, bn)
+ *   next npsi      psi of the next step: hfun(phi) inside each next window,
+ *                  phi inside the previous window read from phi_win (the
+ *                  copy-back of the first role may be under way), hfunc
+ *                  elsewhere (the preps write hfunc only inside the next
+ *                  windows)
+ *   last nlp       lum <- lumn, ECM <- ECMn with the live pixels (commit_field),
+ *                  and h(lum), h(ECM) of the changed tiles from lumn, ECMn
+ * The previous centres come from (xp, yp), which the preps do not touch. */
+#define ORG_LDS_MID (4 * ORG_PSI_LDS_GROUP + 16)
+__global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];
+    const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    if (b < P.nc) {
+        commit_cell_body(P, b, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
+        __syncthreads();      /* the prep reads back this workgroup's phi window */
+        cell_prep(P, b, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
+    } else if (b < P.nc + a.npsi) {
+        psi_tiles_step(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.phi_win, a.xp, a.yp, a.psi, scr);
+    } else {
+        const int blk = b - P.nc - a.npsi, nblk = a.nlp, N = P.N;
+        const LumState sl = a.lst[0], se = a.lst[1];
+        if (blk == 0 && threadIdx.x == 0) {
+            a.uni[0] = N; a.uni[1] = 0; a.uni[2] = N; a.uni[3] = 0;
+            if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
+        }
+        const float *FL = sl.cur ? a.L1 : a.L0, *FE = se.cur ? a.E1 : a.E0;
+        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+        int *sb = (int *)scr;
+        commit_field(P, 0, BL, blk, nblk, a.lum, FL, a.live_part, sb);
+        commit_field(P, 1, BE, blk, nblk, a.ecm, FE, a.live_part, sb);
+        /* h of the new fields on every part the step may have changed */
+        const int nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+        const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+        for (int k = blk; k <= ntl; k += nblk) {
+            bool need = full || k == ntl;
+            if (!need) {
+                const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+                for (int f = 0; f < 2 && !need; f++) {
+                    const int4 B = f ? BE : BL;
+                    need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
+                           tj + ORG_HT > B.z - 1;
+                }
+            }
+            if (!need) continue;
+            const double tot = block_sum_d(hlum_part(P, k, FL, FE, a.hlum, a.hecm), (double *)(scr + 64));
+            if (threadIdx.x == 0) a.part_h[k] = tot;
+        }
     }
 }
 

@@ -247,6 +247,18 @@ static __device__ __forceinline__ int brev(int i) {
         if (i & b) r |= m;
     return r;
 }
+/* a * exp(S * 2 pi i m / 16) for m = 0..7 known at compile time after
+ * unrolling: the multiplications by 1, S i and (+-1 + S i)/sqrt(2) are
+ * written out (a general complex product would keep the x*0 terms) */
+template <int S>
+static __device__ __forceinline__ float2 twmul16(float2 a, int m) {
+    if (m == 0) return a;
+    if (m == 4) return make_float2(-S * a.y, S * a.x);
+    if (m == 2) return make_float2(C16_2 * (a.x - S * a.y), C16_2 * (S * a.x + a.y));
+    if (m == 6) return make_float2(-C16_2 * (a.x + S * a.y), C16_2 * (S * a.x - a.y));
+    return cmulf(a, w16<S>(m));
+}
+
 template <int N, int S>
 static __device__ __forceinline__ void dft_reg(float2 *v) {
 #pragma unroll
@@ -260,8 +272,7 @@ static __device__ __forceinline__ void dft_reg(float2 *v) {
         for (int i = 0; i < N; i += len) {
 #pragma unroll
             for (int k = 0; k < len / 2; k++) {
-                float2 w = w16<S>(k * (16 / len));
-                float2 b = cmulf(v[i + k + len / 2], w);
+                float2 b = twmul16<S>(v[i + k + len / 2], k * (16 / len));
                 v[i + k + len / 2] = csubf(v[i + k], b);
                 v[i + k] = caddf(v[i + k], b);
             }
@@ -331,6 +342,13 @@ static __device__ __forceinline__ void velocity_mode(float eta, float xi, float 
     x.x = (kx * pxx.y + ky * pxy.y - ex * y.x) * rA;
     x.y = (-kx * pxx.x - ky * pxy.x - ex * y.y) * rA;
     vx = x; vy = y;
+}
+
+/* Parameter of a turnover label (0 apical, 1 basal, 2 and 3 lateral, which
+ * share their values) from the uniform table a[]: a select chain instead of
+ * a per-lane indexed load. */
+static __device__ __forceinline__ float by_label(const float *a, int idx) {
+    return idx == 0 ? a[0] : (idx == 1 ? a[1] : a[2]);
 }
 
 /* Turnover label of one pixel, rho_functions.cpp:324-353 (strict '>'). */
@@ -1547,7 +1565,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
             const int idx = turnover_label(v, fl[k], fe[k], f1[k], f2[k]);
             lab[e] = (unsigned char)idx;
             const float r = fr[k];
-            const float at = P.atan_[idx], ao = P.aort[idx], bb = P.bb[idx];
+            const float at = by_label(P.atan_, idx), ao = by_label(P.aort, idx), bb = by_label(P.bb, idx);
             const float r3 = r * r * r;
             const float base = -at * r3 + bb * r3 * r;
             float pxx = base, pxy = 0.0f, pyv = base;
@@ -1560,7 +1578,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
             }
             const float d = v - 0.5f;
             const float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
-            const float react = -P.deg[idx] * r + P.gen[idx] * pnd;
+            const float react = -by_label(P.deg, idx) * r + by_label(P.gen, idx) * pnd;
             rbox[e] = r + P.dt * react;
             SB[e] = pyv;
             sm[i * ORG_LS + j] = make_float2(pxx, pxy);
@@ -1606,8 +1624,8 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
             const int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
             const int irm = (im < ORG_MB / 2) ? im : im - ORG_MB, jrm = (jm < ORG_MB / 2) ? jm : jm - ORG_MB;
             float2 vxk, vyk, vxm, vym;
-            velocity_mode(P.visc[lab[ek]], P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vxk, vyk);
-            velocity_mode(P.visc[lab[em]], P.friction, kscale * irm, kscale * jrm, pxxm, pxym, Bm, vxm, vym);
+            velocity_mode(by_label(P.visc, lab[ek]), P.friction, kscale * ir, kscale * jr, pxx, pxy, B, vxk, vyk);
+            velocity_mode(by_label(P.visc, lab[em]), P.friction, kscale * irm, kscale * jrm, pxxm, pxym, Bm, vxm, vym);
             const float2 hx = make_float2(0.5f * (vxk.x + vxm.x), 0.5f * (vxk.y - vxm.y));
             const float2 hy = make_float2(0.5f * (vyk.x + vym.x), 0.5f * (vyk.y - vym.y));
             sm[lk] = make_float2(hx.x - hy.y, hx.y + hy.x);

@@ -126,6 +126,21 @@ extern void update_lumen_and_ECM(float *lum, float *lumn, float *ECM, float *ECM
 extern void reshaping_cells(int, float *, float *, float *, float *, float *, int *, int *);
 extern void reshaping_lumen_or_ECM(int, float *, float *, float);
 
+/* ---- initialisation_functions.h:1-12 (host code, organoid_initial.cpp) ----
+ * Analytic initial state of the IC program (prepare_initial_files.cpp:56-66)
+ * and the readers of the .bin state files (run_with_active_gels.cpp:63-66).
+ * The .bin layout keeps the reference's 8-byte records: element k at byte 8k.
+ * A missing or short file aborts with a message. */
+extern void ini_phin(float *, int n);                        /* initialisation_functions.cpp:36 */
+extern void ini_phi(float *);                                 /* :117 */
+extern void ini_rho(float *, float *, float rho0, float tol, int n);  /* :64 */
+extern void ini_rhon(float *, int n);                         /* :75 */
+extern void ini_ECM_lumen(float *, float *, float *);         /* :82 */
+extern void ini_phi_files(float *);                           /* :10 */
+extern void ini_lumen_files(float *);                         /* :163 */
+extern void ini_ECM_files(float *);                           /* :174 */
+extern void ini_rho_files(float *);                           /* :43 */
+
 /* Library-side handle of the organoid behind the globals (no counterpart in
  * the reference): created on the first timestep call, destroyed at exit. */
 org_ctx *organoid_dropin_context(void);

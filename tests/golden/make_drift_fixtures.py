@@ -1,0 +1,106 @@
+"""Write the long-trajectory fixtures of the FP64 oracle (test data generator).
+
+For each case below the FP64 CPU oracle (oracle/organoid_oracle.c, a
+restatement of the reference's time loop, main_code/run_with_active_gels.cpp:
+102-185) runs the workload from its seeded initial state and records
+
+* per step: the integer centres of every cell (useful_functions.cpp:193-227)
+  and the reshaping iteration counts (cells, lumen, ECM);
+* at each checkpoint: phi, rho, lum and ECM.  Only the part that differs from
+  the initial state is stored, as the bounding box of the changed pixels per
+  cell (phi, rho) or per plane (lum, ECM), in float32; the rest of the field
+  is the initial state, which ``make_workload`` rebuilds bit for bit;
+* for cases with ``velocity``: the last step's cortex velocity (vx, vy) on
+  every cell's FFT box, float32.
+
+Usage:  python tests/golden/make_drift_fixtures.py [case ...]
+(cases: two_cells_256, organoid16_512; all by default, run in parallel).
+Runtime on one core: about 5 min (two_cells_256, 10^4 steps) and 18 min
+(organoid16_512, 5000 steps).
+"""
+from __future__ import annotations
+
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+# case -> (checkpoint steps, store velocity)
+CASES = {
+    "two_cells_256": ((1000, 10000), True),
+    "organoid16_512": ((1000, 5000), False),
+}
+
+
+def fixture_path(case: str) -> str:
+    return os.path.join(HERE, f"drift_{case}.npz")
+
+
+def changed_box(a: np.ndarray, a0: np.ndarray):
+    """Bounding box [i0, i1) x [j0, j1) of the pixels where a != a0 (empty:
+    a zero-size box at the origin)."""
+    d = a != a0
+    if not d.any():
+        return 0, 0, 0, 0
+    rows = np.flatnonzero(d.any(axis=1))
+    cols = np.flatnonzero(d.any(axis=0))
+    return int(rows[0]), int(rows[-1]) + 1, int(cols[0]), int(cols[-1]) + 1
+
+
+def _store(out: dict, key: str, a: np.ndarray, a0: np.ndarray) -> None:
+    if a.ndim == 2:
+        a, a0 = a[None], a0[None]
+    boxes = np.array([changed_box(a[n], a0[n]) for n in range(a.shape[0])], np.int32)
+    out[key + "_box"] = boxes
+    for n, (i0, i1, j0, j1) in enumerate(boxes):
+        out[f"{key}_{n}"] = a[n, i0:i1, j0:j1].astype(np.float32)
+
+
+def run_case(case: str) -> str:
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+    from oracle.oracle import Oracle
+
+    checkpoints, velocity = CASES[case]
+    wl = make_workload(case)
+    o = Oracle(wl.params, wl.s0_cell)
+    o.load_workload(wl)
+    nc = wl.params.ncell
+    nsteps = max(checkpoints)
+    centres = np.zeros((nsteps, nc, 2), np.int16)
+    it_cells = np.zeros((nsteps, nc), np.int16)
+    it_lum = np.zeros(nsteps, np.int16)
+    it_ecm = np.zeros(nsteps, np.int16)
+    out = {"checkpoints": np.array(checkpoints, np.int32)}
+    t0 = time.time()
+    for it in range(nsteps):
+        o.step(it)
+        xc, yc = o.centres()
+        centres[it, :, 0], centres[it, :, 1] = xc, yc
+        c, lum, ecm = o.iterations()
+        it_cells[it], it_lum[it], it_ecm[it] = c, lum, ecm
+        if it + 1 in checkpoints:
+            k = f"s{it + 1}"
+            _store(out, k + "_phi", o.get("phi"), wl.phi)
+            _store(out, k + "_rho", o.get("rho"), wl.rho)
+            _store(out, k + "_lum", o.get("lum"), wl.lum)
+            _store(out, k + "_ecm", o.get("ecm"), wl.ecm)
+            if velocity and it + 1 == nsteps:
+                out["vx"] = o.get("vx").astype(np.float32)
+                out["vy"] = o.get("vy").astype(np.float32)
+            print(f"{case}: step {it + 1} stored ({time.time() - t0:.0f} s)", flush=True)
+    out.update(centres=centres, it_cells=it_cells, it_lum=it_lum, it_ecm=it_ecm)
+    np.savez_compressed(fixture_path(case), **out)
+    return fixture_path(case)
+
+
+if __name__ == "__main__":
+    cases = sys.argv[1:] or list(CASES)
+    with mp.Pool(len(cases)) as pool:
+        for p in pool.imap_unordered(run_case, cases):
+            print("wrote", p, os.path.getsize(p), "bytes", flush=True)

@@ -74,6 +74,53 @@ def calib_copy(mib: int) -> int:
     return 0
 
 
+def sweep_rates(host, steps: int) -> dict:
+    """The uneven 8-organoid sweep (configs.make_sweep): cell-timesteps/s of
+    one batch (one launch per stage for all organoids) and of the same
+    organoids stepped one after another, each with its own graphs."""
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_sweep
+    wls = make_sweep()
+
+    def members():
+        out = []
+        for w in wls:
+            m = host.Organoid(w.params, w.s0_cell)
+            m.load_workload(w)
+            m.set_option(host.OPT_WRITE_VELOCITY, 0)
+            out.append(m)
+        return out
+
+    cells = sum(w.params.ncell for w in wls)
+    ms = members()
+    with host.Batch(ms) as b:
+        b.step(16)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        tb = time.perf_counter() - t0
+    for m in ms:
+        m.close()
+    ms = members()
+    for m in ms:
+        m.step(16)
+        m.sync()
+    t0 = time.perf_counter()
+    for m in ms:
+        m.step(steps)
+    for m in ms:
+        m.sync()
+    ts = time.perf_counter() - t0
+    for m in ms:
+        m.close()
+    print(f"bench: sweep8 batch {1e3 * tb / steps:.4f} ms/step, one by one {1e3 * ts / steps:.4f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"organoids": len(wls), "cells": cells, "grids": [w.params.N for w in wls],
+            "steps": steps, "batch_cell_steps_per_s": round(cells * steps / tb, 2),
+            "batch_ms_per_step": round(1e3 * tb / steps, 5),
+            "one_by_one_cell_steps_per_s": round(cells * steps / ts, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +130,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=160,
+                    help="steps of the uneven 8-organoid sweep batch (0: skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -155,6 +204,10 @@ def main() -> int:
     plain_rate = nc * args.plain_steps / (time.perf_counter() - t0) if args.plain_steps > 0 else None
     plain.close()
 
+    sweep = None
+    if args.sweep_steps > 0:
+        sweep = sweep_rates(host, args.sweep_steps)
+
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import Oracle
@@ -181,7 +234,8 @@ def main() -> int:
                    "plain_dropin_cell_steps_per_s": round(plain_rate, 2) if plain_rate else None,
                    "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
-                   "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it}},
+                   "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it},
+                   "sweep8": sweep},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

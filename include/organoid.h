@@ -157,13 +157,25 @@ int org_update_all_cortices(org_ctx *ctx);
 int org_update_lumen_and_ECM(org_ctx *ctx);        /* cell_phase_functions.cpp:372-444
                                                       + useful_functions.cpp:439-462 */
 int org_reshaping_lumen_or_ECM(org_ctx *ctx, int which_ecm);
-                                                   /* cell_phase_functions.cpp:709-871 */
+                                                   /* cell_phase_functions.cpp:709-871; the
+                                                      field ends at the second-to-last iterate,
+                                                      lumn / ECMn at the last (:777-785) */
 int org_update_all_cells(org_ctx *ctx);            /* cell_phase_functions.cpp:126-217
                                                       (+ compute_normal_grad_phi
                                                       useful_functions.cpp:326-354) */
 int org_reshaping_cells(org_ctx *ctx);             /* cell_phase_functions.cpp:447-580
                                                       (+ compute_normal_to_phi
                                                       useful_functions.cpp:356-388) */
+/* Initial-condition pipeline (initialisation_code/prepare_initial_files.cpp).
+ * reshaping_cells with phin = phi (the program's copy, :87-90);
+ * update_rho_to_shape (cell_phase_functions.cpp:42-62) on every plane;
+ * org_relax_initial_state: the program's whole device part (:82-110), with
+ * psi, h(lum), h(ECM) zero (the program never computes them), ending with
+ * phi, lum and ECM at their second-to-last reshaping iterates as the
+ * program saves them. */
+int org_reshaping_cells_from_phi(org_ctx *ctx);
+int org_update_rho_to_shape(org_ctx *ctx, float rho0, float tol);
+int org_relax_initial_state(org_ctx *ctx, float rho0, float tol);
 int org_commit_step(org_ctx *ctx);                 /* useful_functions.cpp:679-709 as
                                                       called at run_with_active_gels.cpp:175-182 */
 int org_commit_rho(org_ctx *ctx);                  /* rho <- rhon for every cell,
@@ -179,6 +191,28 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
  * the device).  use_graph=1 captures the step into a hipGraph on first use
  * and replays it.                                                            */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
+
+/* ---- batches of organoids ----------------------------------------------------
+ * The counterpart of a setup_simulation.py parameter sweep
+ * (main_code/setup_simulation.py:29-78 renders one constants file and one
+ * run directory per combination, each run stepping alone): the organoids of
+ * a batch, each an org_ctx with its own grid size, cell count and constants,
+ * advance together with one launch per stage for all of them
+ * (org_batch_step = org_step over every member, one replayable hipGraph).
+ * Every member evolves bit for bit as it would under org_step.
+ *
+ * org_batch_create takes 1..255 distinct, uploaded contexts that belong to no
+ * other batch; it sets their workgroup counts for the shared launches
+ * (org_step keeps working on a member).  Destroy the batch before any of its
+ * contexts.  Errors: ORG_EINVAL (bad arguments, or more lumen/ECM workgroups
+ * than the device can hold at once), ORG_ESTATE (a member not prepared), and
+ * the members' device errors after org_batch_step (the first one is
+ * returned). */
+typedef struct org_batch org_batch;
+int org_batch_create(org_ctx *const *ctxs, int n, org_batch **out);
+int org_batch_step(org_batch *batch, int nsteps, int use_graph);
+int org_batch_sync(org_batch *batch);
+int org_batch_destroy(org_batch *batch);
 
 /* Device synchronisation and timing.  org_timer_* brackets work on the
  * library's stream with HIP events; org_kernel_time returns the summed

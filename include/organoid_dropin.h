@@ -126,6 +126,15 @@ extern void update_lumen_and_ECM(float *lum, float *lumn, float *ECM, float *ECM
 extern void reshaping_cells(int, float *, float *, float *, float *, float *, int *, int *);
 extern void reshaping_lumen_or_ECM(int, float *, float *, float);
 
+/* ---- the IC program's other calls (prepare_initial_files.cpp:67, 100) ----
+ * update_rho_to_shape runs on the device (cell_phase_functions.cpp:42-62);
+ * check_nan counts NaNs in the first two cell planes on the host
+ * (useful_functions.cpp:11-25).  reshaping_cells after
+ * copy_cell_tables_1_by_1(n, phin, phi) for the last cell (and before any
+ * update_all_cells) starts from phi, as the IC program's does. */
+extern void update_rho_to_shape(float *rho, float *phi, float rho0, float tol);
+extern int check_nan(float *table);
+
 /* ---- initialisation_functions.h:1-12 (host code, organoid_initial.cpp) ----
  * Analytic initial state of the IC program (prepare_initial_files.cpp:56-66)
  * and the readers of the .bin state files (run_with_active_gels.cpp:63-66).

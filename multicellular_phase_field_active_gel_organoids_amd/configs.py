@@ -35,7 +35,7 @@ import numpy as np
 
 from .params import OrgParams, grid_units
 
-__all__ = ["SplitMix64", "Workload", "make_workload", "WORKLOADS"]
+__all__ = ["SplitMix64", "Workload", "make_workload", "make_sweep", "WORKLOADS"]
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -165,6 +165,49 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
     if name == "hex":
         return _hex_workload(name, N or 256, ncell or 4, 20.0, 20.0, 1099, steps or 10)
     raise ValueError(f"unknown workload {name!r}")
+
+
+def _fits(wl: Workload, margin: int = 6) -> bool:
+    """Every cell's window plus a margin of 6 pixels (room to move) lies inside
+    rows/cols 1..N-2 at the start."""
+    zs, N = wl.params.zone_size, wl.params.N
+    lo = np.floor(wl.centres).min() - zs - margin
+    hi = np.ceil(wl.centres).max() + zs + margin
+    return lo >= 1 and hi <= N - 2
+
+
+SWEEP_SIZES = (256, 384, 512, 768, 1024)
+
+
+def make_sweep(name: str = "sweep8", n_org: int = 8, steps: int = 2000) -> list:
+    """An uneven batch of organoids (seed 1002): per organoid, in draw order,
+    the grid side from {256, 384, 512, 768, 1024}, the cell count from
+    4..64, the factors of a_ort, a_tan, b and friction, each log-uniform over
+    +/- 0.5 decade around the template value, and a seed for its hex workload
+    (radii U(12, 28), jitter, rho noise; see ``_hex_workload``).  A draw whose
+    hex disc plus windows does not fit its grid is redrawn (cell count and
+    seed) until it fits."""
+    from .sweep import log_uniform
+    if name != "sweep8":
+        raise ValueError(f"unknown sweep {name!r}")
+    rng = SplitMix64(1002)
+    out = []
+    for o in range(n_org):
+        N = SWEEP_SIZES[int(rng.uniform(1)[0] * len(SWEEP_SIZES))]
+        fa, ft, fb, ff = (log_uniform(float(u), 1.0) for u in rng.uniform(4))
+        while True:
+            ncell = 4 + int(rng.uniform(1)[0] * 61)
+            seed = int(rng.next_u64(1)[0] & np.uint64(0x7FFFFFFF))
+            wl = _hex_workload(f"{name}[{o}]", N, ncell, 12.0, 28.0, seed, steps)
+            if _fits(wl):
+                break
+        p = wl.params
+        p.a_ort_ap, p.a_ort_bas, p.a_ort_lat = p.a_ort_ap * fa, p.a_ort_bas * fa, p.a_ort_lat * fa
+        p.a_tan_ap, p.a_tan_bas, p.a_tan_lat = p.a_tan_ap * ft, p.a_tan_bas * ft, p.a_tan_lat * ft
+        p.b_ap, p.b_bas, p.b_lat = p.b_ap * fb, p.b_bas * fb, p.b_lat * fb
+        p.friction = p.friction * ff
+        out.append(wl)
+    return out
 
 
 WORKLOADS = ("two_cells_256", "organoid16_512", "organoid64_1024", "organoid256_2048")

@@ -15,6 +15,7 @@
  */
 #include "organoid_dropin.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -42,6 +43,7 @@ cufftHandle planf, planr;
 namespace {
 
 org_ctx *g_ctx = nullptr;
+bool g_phin_is_phi = false;    /* phin <- phi copied since the last update_all_cells */
 
 [[noreturn]] void die(const char *what, int rc) {
     fprintf(stderr, "organoid drop-in: %s failed: %s (code %d)\n", what, org_error_string(rc), rc);
@@ -194,18 +196,37 @@ void reshaping_lumen_or_ECM(int, float *field, float *, float) {
 
 void update_all_cells(int, float *, float *, float *, float *, float *, float *, float *, int *, int *, float *,
                       float *) {
+    g_phin_is_phi = false;
     chk(org_update_all_cells(ctx()), "update_all_cells");
 }
 
 void reshaping_cells(int, float *, float *, float *, float *, float *, int *, int *) {
-    chk(org_reshaping_cells(ctx()), "reshaping_cells");
+    if (g_phin_is_phi) chk(org_reshaping_cells_from_phi(ctx()), "reshaping_cells");
+    else chk(org_reshaping_cells(ctx()), "reshaping_cells");
+}
+
+/* cell_phase_functions.cpp:42-62 */
+void update_rho_to_shape(float *, float *, float rho0_, float tol_) {
+    chk(org_update_rho_to_shape(ctx(), rho0_, tol_), "update_rho_to_shape");
+}
+
+/* useful_functions.cpp:11-25: NaNs in the first two cell planes (host) */
+int check_nan(float *table) {
+    int c = 0;
+    const size_t pl = plane();
+    for (size_t k = 0; k < pl; k++) c += std::isnan(table[k]) + std::isnan(table[pl + k]);
+    return c;
 }
 
 /* useful_functions.cpp:679-693.  Before the loop the driver copies
  * phin<-phi, rhon<-rho (run_with_active_gels.cpp:94-98): the device state
- * needs no copy.  In the loop it copies rho<-rhon cell by cell (:175-178);
- * the device commits all cells with the last one. */
-void copy_cell_tables_1_by_1(int n_, float *phase, float *) {
+ * needs no copy; phin<-phi only marks that a reshaping_cells before the next
+ * update_all_cells starts from phi (the IC program's order,
+ * prepare_initial_files.cpp:87-100).  In the loop the driver copies
+ * rho<-rhon cell by cell (:175-178); the device commits all cells with the
+ * last one. */
+void copy_cell_tables_1_by_1(int n_, float *phase, float *phasen) {
+    if (phase == phin && phasen == phi && n_ == Ntot - 1) g_phin_is_phi = true;
     if (!g_ctx) return;
     if (phase == rho && n_ == Ntot - 1) chk(org_commit_rho(g_ctx), "copy rho<-rhon");
 }

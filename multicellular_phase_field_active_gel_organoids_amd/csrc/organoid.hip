@@ -63,6 +63,9 @@ struct org_ctx {
     int npsi = 0;                          /* psi workgroups of k_pre and k_mid */
     int nh = 0;                            /* h(lum), h(ECM) workgroups of k_pre */
     int nlp = 0;                           /* lumen/ECM copy-back workgroups of k_mid and k_post */
+    int nlp_cap = 0;                       /* partial boxes allocated per field (nlp never exceeds it) */
+    int nlum = ORG_LUM_BLOCKS;             /* lumen/ECM workgroups of k_step in a batch */
+    bool in_batch = false;                 /* an org_batch holds this context */
     int nparts_h = 0;                      /* parts of the sum of h(ECM): 64 x 64 tiles + the ring */
     int *live = nullptr, *live_part = nullptr;  /* live boxes of lum and ECM, k_post's partial boxes */
     bool ring_pending = true;              /* rho ghost ring not yet written since org_prepare */
@@ -80,6 +83,16 @@ struct org_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
     size_t ev_used[ORG_K_COUNT] = {};
     std::vector<void *> allocs;
+};
+
+struct org_batch {
+    std::vector<org_ctx *> ctx;
+    hipStream_t stream = nullptr;
+    BatchEnt *ent = nullptr;               /* one entry per organoid (device) */
+    int *bmap[4] = {nullptr, nullptr, nullptr, nullptr};   /* block maps of k_pre, k_step, k_mid, k_post */
+    int grid[4] = {0, 0, 0, 0};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; one step */
+    hipGraph_t graph[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -282,7 +295,7 @@ StepArgs step_args(org_ctx *c) {
     a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
     a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
     a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nh = c->nh;
-    a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp;
+    a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp; a.nlum = c->nlum;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
     a.jlx = c->jlx; a.jly = c->jly; a.jex = c->jex; a.jey = c->jey;
     a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc;
@@ -353,6 +366,58 @@ int enqueue_steps(org_ctx *c, int nsteps) {
     return ORG_OK;
 }
 
+/* ---- batches ------------------------------------------------------------ */
+#define ORG_BATCH_MAX 255
+enum { BK_PRE = 0, BK_STEP = 1, BK_MID = 2, BK_POST = 3 };
+
+int LB(org_batch *b, int k) {
+    const ORG_CONST BatchEnt *ent = (const ORG_CONST BatchEnt *)b->ent;
+    const ORG_CONST int *map = (const ORG_CONST int *)b->bmap[k];
+    switch (k) {
+    case BK_PRE: hipLaunchKernelGGL(k_pre_batch, dim3(b->grid[k]), dim3(1024), 0, b->stream, ent, map); break;
+    case BK_STEP: hipLaunchKernelGGL(k_step_batch, dim3(b->grid[k]), dim3(1024), 0, b->stream, ent, map); break;
+    case BK_MID: hipLaunchKernelGGL(k_mid_batch, dim3(b->grid[k]), dim3(1024), 0, b->stream, ent, map); break;
+    default: hipLaunchKernelGGL(k_post_batch, dim3(b->grid[k]), dim3(1024), 0, b->stream, ent, map); break;
+    }
+    return launch_check();
+}
+
+/* the stage sequence of enqueue_steps over every organoid of the batch */
+int enqueue_batch_steps(org_batch *b, int nsteps) {
+    if (nsteps <= 0) return ORG_OK;
+    TRY(LB(b, BK_PRE));
+    for (int s = 0; s < nsteps; s++) {
+        TRY(LB(b, BK_STEP));
+        TRY(LB(b, s + 1 < nsteps ? BK_MID : BK_POST));
+    }
+    return ORG_OK;
+}
+
+int batch_upload_entries(org_batch *b) {
+    std::vector<BatchEnt> h(b->ctx.size());
+    for (size_t o = 0; o < b->ctx.size(); o++) { h[o].P = b->ctx[o]->dp; h[o].a = step_args(b->ctx[o]); }
+    HIPCHK(hipMemcpyAsync(b->ent, h.data(), sizeof(BatchEnt) * h.size(), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return ORG_OK;
+}
+
+void batch_drop_graphs(org_batch *b) {
+    for (int k = 0; k < 2; k++) {
+        if (b->gexec[k]) { (void)hipGraphExecDestroy(b->gexec[k]); b->gexec[k] = nullptr; }
+        if (b->graph[k]) { (void)hipGraphDestroy(b->graph[k]); b->graph[k] = nullptr; }
+    }
+}
+
+int batch_harvest(org_batch *b) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int rc = ORG_OK;
+    for (org_ctx *c : b->ctx) {
+        const int r = harvest_device_error(c);
+        if (r && !rc) rc = r;
+    }
+    return rc;
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -409,6 +474,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         c->npsi = role;
         c->nh = std::min(c->nparts_h, role);
         c->nlp = role;
+        c->nlp_cap = role;
     }
     int nt = (c->N + 15) / 16;
     c->ntiles = nt * nt;
@@ -452,6 +518,9 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         hipMemset(c->err, 0, sizeof(int)) != hipSuccess || hipMemset(c->counters, 0, 4 * sizeof(unsigned)) != hipSuccess ||
         hipMemset(c->counts, 0, sizeof(int) * c->nc) != hipSuccess || hipMemset(c->lst, 0, 2 * sizeof(LumState)) != hipSuccess ||
         hipMemset(c->hfunc, 0, sizeof(float) * c->P) != hipSuccess ||
+        hipMemset(c->hlum, 0, sizeof(float) * c->NN) != hipSuccess ||
+        hipMemset(c->hecm, 0, sizeof(float) * c->NN) != hipSuccess ||
+        hipMemset(c->sum_ecm, 0, sizeof(double)) != hipSuccess ||
         hipMemset(c->lum_bar, 0, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(c->lum_abort, 0, sizeof(int)) != hipSuccess) {
         org_destroy(c);
         return note_error(ORG_EHIP);
@@ -617,10 +686,57 @@ int org_reshaping_lumen_or_ECM(org_ctx *c, int which_ecm) {
     int f = which_ecm ? 1 : 0;
     TRY(L_lum_normals(c, f, 1));
     TRY(run_lum_loop(c, f, 1, 1));
-    return harvest_device_error(c);
+    TRY(harvest_device_error(c));
+    /* the reference's loop copies lum <- lumn before every sweep
+     * (cell_phase_functions.cpp:777-785): its first argument ends at the
+     * second-to-last iterate, the other buffer of the pair */
+    LumState st[2];
+    HIPCHK(hipMemcpyAsync(st, c->lst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st[f].ri >= 1) {
+        const float *prev = f ? (st[f].cur ? c->bufE0 : c->bufE1) : (st[f].cur ? c->bufL0 : c->bufL1);
+        HIPCHK(hipMemcpyAsync(f ? c->ecm : c->lum, prev, sizeof(float) * c->NN, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return ORG_OK;
 }
 int org_update_all_cells(org_ctx *c) { TRY(need_prepared(c)); return L_cells(c, 1); }
 int org_reshaping_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_cells(c, 2)); return harvest_device_error(c); }
+int org_reshaping_cells_from_phi(org_ctx *c) {
+    TRY(need_prepared(c));
+    TRY(L_cells(c, 2 | 4));
+    return harvest_device_error(c);
+}
+int org_update_rho_to_shape(org_ctx *c, float rho0, float tol) {
+    TRY(need_prepared(c));
+    hipLaunchKernelGGL(k_rho_to_shape, dim3(1024), dim3(256), 0, c->stream, c->phi, c->rho, c->P, rho0, tol);
+    TRY(launch_check());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ORG_OK;
+}
+/* The IC program's device part (prepare_initial_files.cpp:82-110): centres,
+ * reshaping_cells from phin = phi, update_rho_to_shape, update_lumen_and_ECM
+ * and both reshaping loops.  The program never computes psi, h(lum), h(ECM)
+ * (created, not copied in, :80): they and the sum of h(ECM) are taken as
+ * zero.  No copy-back follows, so lum and ECM end at their second-to-last
+ * iterates and phi at its second-to-last reshaping iterate, as in the
+ * program's saved frame (:108). */
+int org_relax_initial_state(org_ctx *c, float rho0, float tol) {
+    TRY(need_prepared(c));
+    TRY(L_centroid(c));
+    TRY(harvest_device_error(c));
+    TRY(org_reshaping_cells_from_phi(c));
+    TRY(org_update_rho_to_shape(c, rho0, tol));
+    HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipMemsetAsync(c->hlum, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipMemsetAsync(c->hecm, 0, sizeof(float) * c->NN, c->stream));
+    HIPCHK(hipMemsetAsync(c->sum_ecm, 0, sizeof(double), c->stream));
+    TRY(org_update_lumen_and_ECM(c));
+    TRY(org_reshaping_lumen_or_ECM(c, 0));
+    TRY(org_reshaping_lumen_or_ECM(c, 1));
+    c->next_ready = false;
+    return harvest_device_error(c);
+}
 int org_commit_step(org_ctx *c) {
     TRY(need_prepared(c));
     TRY(L_commit_cells(c));
@@ -708,6 +824,153 @@ int org_kernel_time(org_ctx *c, int kid, double *ms, int64_t *launches) {
     *ms = tot;
     *launches = (int64_t)c->ev_used[kid];
     c->ev_used[kid] = 0;
+    return ORG_OK;
+}
+
+/* ---- batches of organoids ------------------------------------------------ */
+int org_batch_create(org_ctx *const *ctxs, int n, org_batch **out) {
+    if (!ctxs || !out || n < 1 || n > ORG_BATCH_MAX) return note_error(ORG_EINVAL);
+    *out = nullptr;
+    for (int o = 0; o < n; o++) {
+        if (!ctxs[o] || ctxs[o]->in_batch) return note_error(ORG_EINVAL);
+        for (int q = 0; q < o; q++)
+            if (ctxs[q] == ctxs[o]) return note_error(ORG_EINVAL);
+    }
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    /* role sizes: the per-cell roles have one workgroup per cell; the
+     * others share the CUs in proportion to the grid areas.  The lumen/ECM
+     * workgroups of k_step wait on each other, so all of them must be
+     * resident at once: their total stays within the CU count. */
+    double area = 0.0;
+    long long cells = 0;
+    for (int o = 0; o < n; o++) { area += (double)ctxs[o]->N * ctxs[o]->N; cells += ctxs[o]->nc; }
+    const int lum_budget = std::max(n, std::min(ncu / 2, ORG_LUM_BLOCKS * n));
+    const int spare = std::max(64, ncu - (int)std::min<long long>(cells, ncu));
+    std::vector<int> nlum(n), role(n);
+    int lum_tot = 0;
+    for (int o = 0; o < n; o++) {
+        const double w = (double)ctxs[o]->N * ctxs[o]->N / area;
+        nlum[o] = std::max(1, std::min(ORG_LUM_BLOCKS, (int)(lum_budget * w)));
+        role[o] = std::max(2, (int)std::lround(0.5 * spare * w));
+        lum_tot += nlum[o];
+    }
+    if (lum_tot > ncu - 1) {
+        fprintf(stderr, "organoid: a batch of %d organoids needs more lumen/ECM workgroups than CUs\n", n);
+        return note_error(ORG_EINVAL);
+    }
+    org_batch *b = new org_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    int rc = ORG_OK;
+    for (int o = 0; o < n && !rc; o++) {
+        org_ctx *c = ctxs[o];
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = note_error(ORG_EHIP);
+        c->nlum = nlum[o];
+        c->npsi = role[o];
+        c->nh = std::min(c->nparts_h, role[o]);
+        c->nlp = std::min(c->nlp_cap, role[o]);
+        c->next_ready = false;          /* partial live boxes are re-initialised for the new nlp */
+        for (int k = 0; k < 2; k++) {    /* the context's own graphs hold the old role sizes */
+            if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+            if (c->graph[k]) { (void)hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }
+        }
+        c->in_batch = true;
+    }
+    /* block maps: (role << 28) | (organoid << 20) | index */
+    std::vector<int> maps[4];
+    auto put = [](std::vector<int> &m, int role_, int o, int cnt) {
+        for (int i = 0; i < cnt; i++) m.push_back((role_ << 28) | (o << 20) | i);
+    };
+    for (int o = 0; o < n; o++) {
+        const org_ctx *c = ctxs[o];
+        put(maps[BK_PRE], 0, o, c->nc); put(maps[BK_PRE], 1, o, c->npsi); put(maps[BK_PRE], 2, o, c->nh);
+        put(maps[BK_STEP], 1, o, c->nlum);
+        put(maps[BK_MID], 0, o, c->nc); put(maps[BK_MID], 1, o, c->npsi); put(maps[BK_MID], 2, o, c->nlp);
+        put(maps[BK_POST], 0, o, c->nc); put(maps[BK_POST], 2, o, c->nlp);
+    }
+    for (int o = 0; o < n; o++) { put(maps[BK_STEP], 0, o, ctxs[o]->nc); put(maps[BK_STEP], 2, o, ctxs[o]->nc); }
+    if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
+    if (!rc && hipMalloc(&b->ent, sizeof(BatchEnt) * n) != hipSuccess) rc = note_error(ORG_EHIP);
+    for (int k = 0; k < 4 && !rc; k++) {
+        b->grid[k] = (int)maps[k].size();
+        if (hipMalloc(&b->bmap[k], sizeof(int) * maps[k].size()) != hipSuccess ||
+            hipMemcpy(b->bmap[k], maps[k].data(), sizeof(int) * maps[k].size(), hipMemcpyHostToDevice) != hipSuccess)
+            rc = note_error(ORG_EHIP);
+    }
+    if (!rc) rc = batch_upload_entries(b);
+    if (rc) { org_batch_destroy(b); return rc; }
+    *out = b;
+    return ORG_OK;
+}
+
+int org_batch_destroy(org_batch *b) {
+    if (!b) return ORG_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    for (org_ctx *c : b->ctx) c->in_batch = false;
+    (void)hipFree(b->ent);
+    for (int k = 0; k < 4; k++) (void)hipFree(b->bmap[k]);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return ORG_OK;
+}
+
+int org_batch_step(org_batch *b, int nsteps, int use_graph) {
+    if (!b || nsteps < 0) return note_error(ORG_EINVAL);
+    if (nsteps == 0) return ORG_OK;
+    /* every organoid ready for the batched step (what org_step does first) */
+    for (org_ctx *c : b->ctx) {
+        if (!c->prepared) {
+            fprintf(stderr, "organoid: org_prepare must run after the fields are uploaded\n");
+            return note_error(ORG_ESTATE);
+        }
+        if (!c->next_ready) {
+            TRY(L_centroid_next(c));
+            TRY(L_live_init(c));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->next_ready = true;
+    }
+    TRY(batch_upload_entries(b));       /* options such as the velocity output may have changed */
+    bool ring = false;
+    for (org_ctx *c : b->ctx) ring = ring || c->ring_pending;
+    if (ring) {                          /* first step after org_prepare: eager, then the ghost rings */
+        TRY(enqueue_batch_steps(b, 1));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (org_ctx *c : b->ctx) {
+            TRY(L_ring_if_pending(c));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        nsteps--;
+    }
+    if (!use_graph) {
+        TRY(enqueue_batch_steps(b, nsteps));
+        return batch_harvest(b);
+    }
+    for (int k = 0; k < 2; k++) {
+        if (b->gexec[k]) continue;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_batch_steps(b, k ? 1 : ORG_GRAPH_STEPS);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : note_error(ORG_EHIP);
+        }
+        b->graph[k] = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec[k], b->graph[k], nullptr, nullptr, 0));
+    }
+    int s = 0;
+    for (; s + ORG_GRAPH_STEPS <= nsteps; s += ORG_GRAPH_STEPS) HIPCHK(hipGraphLaunch(b->gexec[0], b->stream));
+    for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(b->gexec[1], b->stream));
+    return batch_harvest(b);
+}
+
+int org_batch_sync(org_batch *b) {
+    if (!b) return note_error(ORG_EINVAL);
+    HIPCHK(hipStreamSynchronize(b->stream));
     return ORG_OK;
 }
 

@@ -955,6 +955,21 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
  * boundary_conditions_cells (useful_functions.cpp:391-420, 505-526) */
 static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N - 2 : (i == N - 1 ? 1 : i); }
 
+/* Central difference of the old field for |grad| in the lumen/ECM normals.
+ * The reference divides the new field's gradient by the old field's
+ * (cell_phase_functions.cpp:734-763), which is ill-conditioned where the old
+ * gradient vanishes: at an extremum whose neighbours are equal in exact
+ * arithmetic, FP32 rounding leaves a difference of one or two ulps, |grad|
+ * becomes ~1e-12 and the normal ~1e3, and the reshaping sweeps then overflow
+ * to inf/NaN within a few steps.  A difference of at most 2 FP32 epsilons of
+ * its larger operand (2 to 4 ulps, rounding noise rather than a resolved
+ * gradient) is taken as zero, so such a pixel gets the zero normal the
+ * reference gives an exactly flat one. */
+static __device__ __forceinline__ float old_diff(const float p, const float m) {
+    const float d = p - m;
+    return fabsf(d) <= 2.0f * 1.1920929e-7f * fmaxf(fabsf(p), fabsf(m)) ? 0.0f : d;
+}
+
 /* Normals of one field on row i: |grad| of the old field
  * (cell_phase_functions.cpp:722-741), normals from the new field's gradient
  * (:744-771) with the periodic ring; grows (b0..b3) to the bounding box of
@@ -969,7 +984,7 @@ static __device__ __forceinline__ void normals_row(const DevP &P, const int i, c
     const int si = ring_src(i, N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const size_t g = (size_t)si * N + ring_src(j, N);
-        float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+        float a = old_diff(old[g + N], old[g - N]), b = old_diff(old[g + 1], old[g - 1]);
         float ng = hw_sqrt(c * (a * a + b * b));
         float vx = 0.0f, vy = 0.0f;
         if (ng != 0.0f) {
@@ -1082,7 +1097,7 @@ static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4
             for (int q = threadIdx.x; q < npx; q += nt) {
                 const int i = r0 + q / bw, j = B.z + q % bw;
                 const size_t g = (size_t)i * N + j;
-                const float a = old[g + N] - old[g - N], b = old[g + 1] - old[g - 1];
+                const float a = old_diff(old[g + N], old[g - N]), b = old_diff(old[g + 1], old[g - 1]);
                 const float ng = hw_sqrt(c * (a * a + b * b));
                 float vx = 0.0f, vy = 0.0f;
                 if (ng != 0.0f) {
@@ -1914,7 +1929,8 @@ static __device__ void cells_relax(const DevP &P, const int n, const int x0, con
 }
 
 /* k_cells: update_all_cells (mode&1) and / or reshaping_cells (mode&2) of
- * every cell; mode 1 leaves phin in phin_win, mode 2 starts from it. */
+ * every cell; mode 1 leaves phin in phin_win, mode 2 starts from it, mode
+ * 2|4 starts from phi itself. */
 __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi, const float *__restrict__ rho,
                                                 const float *__restrict__ hfunc, const float *__restrict__ psi,
                                                 const float *__restrict__ hlum, const float *__restrict__ hecm,
@@ -1935,6 +1951,11 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
         cells_update(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi, hlum, hecm, f0[n], (mode & 2) ? pnl : nullptr,
                      pw);
         if (!(mode & 2)) return;
+    } else if (mode & 4) {  /* phin = phi (the IC program's copy, prepare_initial_files.cpp:87-90) */
+        for (int e = threadIdx.x; e < NPX; e += 1024) {
+            const int i = e / W, j = e - i * W;
+            pnl[(i + 1) * WH + (j + 1)] = ph[(size_t)(x0 + i) * P.N + (y0 + j)];
+        }
     } else {
         for (int e = threadIdx.x; e < NPX; e += 1024) {
             const int i = e / W, j = e - i * W;
@@ -1965,6 +1986,7 @@ struct StepArgs {
     int npsi;                /* psi workgroups of k_pre */
     int *live, *live_part;   /* live boxes of lum and ECM; per-workgroup boxes of live pixels from k_post */
     int nlp;                 /* k_post's lumen/ECM workgroups (partial boxes per field) */
+    int nlum;                /* lumen/ECM workgroups of k_step in a batch (ORG_LUM_BLOCKS alone) */
     /* lumen / ECM loops */
     float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
     int *jbox_part;
@@ -2025,32 +2047,37 @@ static __device__ int4 merged_live(const DevP &P, const StepArgs &a, const int f
  *                      the tiles meeting a live box grown by one (all tiles in
  *                      full mode). */
 #define ORG_LDS_PRE (4 * ORG_PSI_LDS_GROUP + 16)
+static __device__ __forceinline__ void pre_cell(DevP P, StepArgs a, const int n, unsigned char *scr) {
+    cell_prep(P, n, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
+}
+static __device__ __forceinline__ void pre_psi(DevP P, StepArgs a, const int blk, unsigned char *scr) {
+    psi_tiles_step(P, blk, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, nullptr, nullptr, nullptr, a.psi, scr);
+}
+static __device__ __forceinline__ void pre_hlum(DevP P, StepArgs a, const int hb, unsigned char *scr) {
+    const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+    const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+    const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+    for (int k = hb; k <= ntl; k += a.nh) {
+        bool need = full || k == ntl;
+        if (!need) {
+            const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+            for (int f = 0; f < 2 && !need; f++) {
+                const int4 B = f ? BE : BL;
+                need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
+                       tj + ORG_HT > B.z - 1;
+            }
+        }
+        if (!need) continue;
+        const double tot = block_sum_d(hlum_part(P, k, a.lum, a.ecm, a.hlum, a.hecm), (double *)scr);
+        if (threadIdx.x == 0) a.part_h[k] = tot;
+    }
+}
 __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
     const int b = blockIdx.x;
-    if (b < P.nc) {
-        cell_prep(P, b, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
-    } else if (b < P.nc + a.npsi) {
-        psi_tiles_step(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, nullptr, nullptr, nullptr, a.psi, scr);
-    } else {
-        const int hb = b - P.nc - a.npsi, N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
-        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
-        const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-        for (int k = hb; k <= ntl; k += a.nh) {
-            bool need = full || k == ntl;
-            if (!need) {
-                const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
-                for (int f = 0; f < 2 && !need; f++) {
-                    const int4 B = f ? BE : BL;
-                    need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
-                           tj + ORG_HT > B.z - 1;
-                }
-            }
-            if (!need) continue;
-            const double tot = block_sum_d(hlum_part(P, k, a.lum, a.ecm, a.hlum, a.hecm), (double *)scr);
-            if (threadIdx.x == 0) a.part_h[k] = tot;
-        }
-    }
+    if (b < P.nc) pre_cell(P, a, b, scr);
+    else if (b < P.nc + a.npsi) pre_psi(P, a, b - P.nc, scr);
+    else pre_hlum(P, a, b - P.nc - a.npsi, scr);
 }
 
 /* k_step: the rest of the timestep up to the copy-back
@@ -2073,72 +2100,77 @@ __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
 #define ORG_LDS_SPECTRAL (ORG_MB * ORG_LS * 8 + 128 * 8 + ORG_MB * ORG_MB)
 #define ORG_LDS_CELLS (2 * ORG_WH * ORG_WH * 4 + 16 * 8 + 48 * 8)
 #define ORG_LDS_FUSED (ORG_LDS_SPECTRAL > ORG_LDS_CELLS ? ORG_LDS_SPECTRAL : ORG_LDS_CELLS)
+static __device__ __forceinline__ void step_spectral(DevP P, StepArgs a, const int b,
+                                                     unsigned char *lds) {
+    float2 *sm = (float2 *)lds;
+    spectral_body(P, b, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), a.phi, a.rho,
+                  a.lum, a.ecm, a.xc[b] - P.zs + 1, a.yc[b] - P.zs + 1, a.spec, a.rho_box, a.vxbox, a.vybox,
+                  a.write_v);
+}
+static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
+                                                   unsigned char *lds) {
+    double *red = (double *)lds;
+    int *sb = (int *)(lds + 128);
+    int epoch = 0;
+    const double se = sum_parts(a.part_h, hl_ntiles(P.N) + 1, red);
+    if (blk == 0) {
+        if (threadIdx.x == 0) *a.sum_ecm = se;
+        lum_reset_states(P, a.lst);
+    }
+    /* live boxes of this step: the stored ones plus what the last
+     * copy-back found (every workgroup merges the same parts; the first
+     * stores the result, and merging again gives the same boxes) */
+    const int4 BL = merged_live(P, a, 0, sb), BE = merged_live(P, a, 1, sb);
+    if (blk == 0 && threadIdx.x == 0) {
+        a.live[0] = BL.x; a.live[1] = BL.y; a.live[2] = BL.z; a.live[3] = BL.w;
+        a.live[4] = BE.x; a.live[5] = BE.y; a.live[6] = BE.z; a.live[7] = BE.w;
+    }
+    if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
+        if (blk == 0) lumecm_ring(P, a.L0, a.L1, a.E0, a.E1);
+        lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
+                           a.jlx, a.jly, a.jbox_part, sb);
+        lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
+                           a.jex, a.jey, a.jbox_part, sb);
+    } else {
+        lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0,
+                            a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
+    }
+    if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
+    lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
+                  a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit);
+}
+static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int n, unsigned char *lds) {
+    constexpr int WH = ORG_WH, NPX = ORG_W * ORG_W;
+    const size_t NN = (size_t)P.N * P.N;
+    float *phase = (float *)lds, *pnl = phase + WH * WH;
+    double *red = (double *)(lds + 2 * WH * WH * 4);
+    const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
+    float *ph = a.phi + n * NN;
+    float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
+    cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, a.f0[n], pnl, nullptr,
+                 jx, jy);
+    cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
+    __syncthreads();      /* the workgroup reads back its own stores */
+    cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
+                a.phi_win + (size_t)n * NPX, a.counts, a.err);
+    /* the next step's centre and support box (find_centres_cells of the
+     * next iteration, useful_functions.cpp:193-227) from the new phi:
+     * the window from LDS, the rest of the support box from the plane */
+    long long *lred = (long long *)(lds + 2 * WH * WH * 4 + 128);
+    const int i0 = a.box[4 * n], i1 = a.box[4 * n + 1], j0 = a.box[4 * n + 2], j1 = a.box[4 * n + 3];
+    long long X, Y, C;
+    centroid_sums_win(ph, P.N, i0, i1, j0, j1, x0, y0, phase, lred, X, Y, C);
+    if (threadIdx.x == 0) {
+        place_window(P, n, X, Y, C, a.xc[n], a.yc[n], i0, i1, j0, j1, a.xn, a.yn, a.bn, a.err);
+        a.xp[n] = a.xc[n]; a.yp[n] = a.yc[n];
+    }
+}
 __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
     const int nc = P.nc, b = blockIdx.x;
-    const size_t NN = (size_t)P.N * P.N;
-    if (b < nc) {
-        float2 *sm = (float2 *)lds;
-        spectral_body(P, b, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), a.phi, a.rho,
-                      a.lum, a.ecm, a.xc[b] - P.zs + 1, a.yc[b] - P.zs + 1, a.spec, a.rho_box, a.vxbox, a.vybox,
-                      a.write_v);
-    } else if (b < nc + ORG_LUM_BLOCKS) {
-        const int blk = b - nc, nblk = ORG_LUM_BLOCKS;
-        double *red = (double *)lds;
-        int *sb = (int *)(lds + 128);
-        int epoch = 0;
-        const double se = sum_parts(a.part_h, hl_ntiles(P.N) + 1, red);
-        if (blk == 0) {
-            if (threadIdx.x == 0) *a.sum_ecm = se;
-            lum_reset_states(P, a.lst);
-        }
-        /* live boxes of this step: the stored ones plus what the last
-         * copy-back found (every workgroup merges the same parts; the first
-         * stores the result, and merging again gives the same boxes) */
-        const int4 BL = merged_live(P, a, 0, sb), BE = merged_live(P, a, 1, sb);
-        if (blk == 0 && threadIdx.x == 0) {
-            a.live[0] = BL.x; a.live[1] = BL.y; a.live[2] = BL.z; a.live[3] = BL.w;
-            a.live[4] = BE.x; a.live[5] = BE.y; a.live[6] = BE.z; a.live[7] = BE.w;
-        }
-        if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
-            if (blk == 0) lumecm_ring(P, a.L0, a.L1, a.E0, a.E1);
-            lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
-                               a.jlx, a.jly, a.jbox_part, sb);
-            lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
-                               a.jex, a.jey, a.jbox_part, sb);
-        } else {
-            lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0,
-                                a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
-        }
-        if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
-        lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
-                      a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit);
-    } else {
-        constexpr int WH = ORG_WH, NPX = ORG_W * ORG_W;
-        const int n = b - nc - ORG_LUM_BLOCKS;
-        float *phase = (float *)lds, *pnl = phase + WH * WH;
-        double *red = (double *)(lds + 2 * WH * WH * 4);
-        const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
-        float *ph = a.phi + n * NN;
-        float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
-        cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, a.f0[n], pnl, nullptr,
-                     jx, jy);
-        cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
-        __syncthreads();      /* the workgroup reads back its own stores */
-        cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
-                    a.phi_win + (size_t)n * NPX, a.counts, a.err);
-        /* the next step's centre and support box (find_centres_cells of the
-         * next iteration, useful_functions.cpp:193-227) from the new phi:
-         * the window from LDS, the rest of the support box from the plane */
-        long long *lred = (long long *)(lds + 2 * WH * WH * 4 + 128);
-        const int i0 = a.box[4 * n], i1 = a.box[4 * n + 1], j0 = a.box[4 * n + 2], j1 = a.box[4 * n + 3];
-        long long X, Y, C;
-        centroid_sums_win(ph, P.N, i0, i1, j0, j1, x0, y0, phase, lred, X, Y, C);
-        if (threadIdx.x == 0) {
-            place_window(P, n, X, Y, C, a.xc[n], a.yc[n], i0, i1, j0, j1, a.xn, a.yn, a.bn, a.err);
-            a.xp[n] = a.xc[n]; a.yp[n] = a.yc[n];
-        }
-    }
+    if (b < nc) step_spectral(P, a, b, lds);
+    else if (b < nc + ORG_LUM_BLOCKS) step_lumecm(P, a, b - nc, ORG_LUM_BLOCKS, lds);
+    else step_cells(P, a, b - nc - ORG_LUM_BLOCKS, lds);
 }
 
 /* rho <- rhon on the FFT box and phi's new window (copy_cell_tables_1_by_1,
@@ -2179,23 +2211,25 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
  * rho box and phi window of every cell (workgroups 0..ncell-1); lum <- lumn,
  * ECM <- ECMn (the other workgroups, see commit_field).  Clears k_step's
  * barrier counter and abort flag. */
+static __device__ __forceinline__ void post_cell(DevP P, StepArgs a, const int n) {
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
+}
+static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
+                                                   int *sb) {
+    const LumState sl = a.lst[0], se = a.lst[1];
+    if (blk == 0 && threadIdx.x == 0) {
+        a.uni[0] = P.N; a.uni[1] = 0; a.uni[2] = P.N; a.uni[3] = 0;
+        if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
+    }
+    commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
+    commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
+}
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
     __shared__ long long red[48];
     const int b = blockIdx.x;
-    if (b == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
-    if (b < P.nc) {
-        commit_cell_body(P, b, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
-    } else {
-        const int blk = b - P.nc, nblk = gridDim.x - P.nc;
-        const LumState sl = a.lst[0], se = a.lst[1];
-        if (blk == 0 && threadIdx.x == 0) {
-            a.uni[0] = P.N; a.uni[1] = 0; a.uni[2] = P.N; a.uni[3] = 0;
-            if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
-        }
-        int *sb = (int *)red;
-        commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
-        commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
-    }
+    if (b < P.nc) post_cell(P, a, b);
+    else post_lumecm(P, a, b - P.nc, a.nlp, (int *)red);
 }
 
 /* k_mid: the copy-back of one timestep and the first part of the next
@@ -2213,45 +2247,129 @@ __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
  *                  and h(lum), h(ECM) of the changed tiles from lumn, ECMn
  * The previous centres come from (xp, yp), which the preps do not touch. */
 #define ORG_LDS_MID (4 * ORG_PSI_LDS_GROUP + 16)
+static __device__ __forceinline__ void mid_cell(DevP P, StepArgs a, const int n, unsigned char *scr) {
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
+    __syncthreads();      /* the prep reads back this workgroup's phi window */
+    cell_prep(P, n, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
+}
+static __device__ __forceinline__ void mid_psi(DevP P, StepArgs a, const int blk, unsigned char *scr) {
+    psi_tiles_step(P, blk, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.phi_win, a.xp, a.yp, a.psi, scr);
+}
+static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int blk,
+                                                  unsigned char *scr) {
+    const int nblk = a.nlp, N = P.N;
+    const LumState sl = a.lst[0], se = a.lst[1];
+    if (blk == 0 && threadIdx.x == 0) {
+        a.uni[0] = N; a.uni[1] = 0; a.uni[2] = N; a.uni[3] = 0;
+        if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
+    }
+    const float *FL = sl.cur ? a.L1 : a.L0, *FE = se.cur ? a.E1 : a.E0;
+    const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+    int *sb = (int *)scr;
+    commit_field(P, 0, BL, blk, nblk, a.lum, FL, a.live_part, sb);
+    commit_field(P, 1, BE, blk, nblk, a.ecm, FE, a.live_part, sb);
+    /* h of the new fields on every part the step may have changed */
+    const int nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+    const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+    for (int k = blk; k <= ntl; k += nblk) {
+        bool need = full || k == ntl;
+        if (!need) {
+            const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+            for (int f = 0; f < 2 && !need; f++) {
+                const int4 B = f ? BE : BL;
+                need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
+                       tj + ORG_HT > B.z - 1;
+            }
+        }
+        if (!need) continue;
+        const double tot = block_sum_d(hlum_part(P, k, FL, FE, a.hlum, a.hecm), (double *)(scr + 64));
+        if (threadIdx.x == 0) a.part_h[k] = tot;
+    }
+}
 __global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];
     const int b = blockIdx.x;
-    if (b == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
-    if (b < P.nc) {
-        commit_cell_body(P, b, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
-        __syncthreads();      /* the prep reads back this workgroup's phi window */
-        cell_prep(P, b, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
-    } else if (b < P.nc + a.npsi) {
-        psi_tiles_step(P, b - P.nc, a.npsi, a.hfunc, a.phi, a.bn, a.xn, a.yn, a.phi_win, a.xp, a.yp, a.psi, scr);
-    } else {
-        const int blk = b - P.nc - a.npsi, nblk = a.nlp, N = P.N;
-        const LumState sl = a.lst[0], se = a.lst[1];
-        if (blk == 0 && threadIdx.x == 0) {
-            a.uni[0] = N; a.uni[1] = 0; a.uni[2] = N; a.uni[3] = 0;
-            if (!sl.done || !se.done) set_err(a.err, ORG_ERELAX);
+    if (b < P.nc) mid_cell(P, a, b, scr);
+    else if (b < P.nc + a.npsi) mid_psi(P, a, b - P.nc, scr);
+    else mid_lumecm(P, a, b - P.nc - a.npsi, scr);
+}
+
+/* ---- batches: every organoid of a set in one launch per stage -----------
+ * A batch runs k_pre, (k_step, k_mid) x (n - 1), k_step, k_post over several
+ * organoids at once (grids, cell counts and constants may all differ).  Each
+ * workgroup reads one word of a block map: the organoid (bits 20..27), the
+ * role (bits 28..29, in the order of the single-organoid kernel) and the
+ * index within the role (bits 0..19), then runs that role with the
+ * organoid's constants and buffers.  The roles are the same functions as in
+ * k_pre, k_step, k_mid and k_post, so every organoid of a batch evolves bit
+ * for bit as it would alone.  In k_step the block map lists the lumen/ECM
+ * workgroups of every organoid first: they are the only ones that wait (on
+ * workgroups of the same organoid and role), and the host keeps their total
+ * within the CU count, so all of them are resident together.  The tables
+ * live in constant memory (address space 4): the role arguments load through
+ * scalar loads as kernel arguments do. */
+struct BatchEnt {
+    DevP P;
+    StepArgs a;
+};
+#define ORG_CONST __attribute__((address_space(4)))
+#define ORG_BENT(v) (((v) >> 20) & 0xff)
+#define ORG_BROLE(v) (((v) >> 28) & 3)
+#define ORG_BIDX(v) ((v) & 0xfffff)
+
+__global__ __launch_bounds__(1024) void k_pre_batch(const ORG_CONST BatchEnt *ent, const ORG_CONST int *bmap) {
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
+    const int v = bmap[blockIdx.x], r = ORG_BROLE(v), i = ORG_BIDX(v);
+    const BatchEnt *E = (const BatchEnt *)(ent + ORG_BENT(v));
+    if (r == 0) pre_cell(E->P, E->a, i, scr);
+    else if (r == 1) pre_psi(E->P, E->a, i, scr);
+    else pre_hlum(E->P, E->a, i, scr);
+}
+
+__global__ __launch_bounds__(1024) void k_step_batch(const ORG_CONST BatchEnt *ent, const ORG_CONST int *bmap) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
+    const int v = bmap[blockIdx.x], r = ORG_BROLE(v), i = ORG_BIDX(v);
+    const BatchEnt *E = (const BatchEnt *)(ent + ORG_BENT(v));
+    if (r == 0) step_spectral(E->P, E->a, i, lds);
+    else if (r == 1) step_lumecm(E->P, E->a, i, E->a.nlum, lds);
+    else step_cells(E->P, E->a, i, lds);
+}
+
+__global__ __launch_bounds__(1024) void k_mid_batch(const ORG_CONST BatchEnt *ent, const ORG_CONST int *bmap) {
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];
+    const int v = bmap[blockIdx.x], r = ORG_BROLE(v), i = ORG_BIDX(v);
+    const BatchEnt *E = (const BatchEnt *)(ent + ORG_BENT(v));
+    if (r == 0) mid_cell(E->P, E->a, i, scr);
+    else if (r == 1) mid_psi(E->P, E->a, i, scr);
+    else mid_lumecm(E->P, E->a, i, scr);
+}
+
+__global__ __launch_bounds__(1024) void k_post_batch(const ORG_CONST BatchEnt *ent, const ORG_CONST int *bmap) {
+    __shared__ long long red[48];
+    const int v = bmap[blockIdx.x], r = ORG_BROLE(v), i = ORG_BIDX(v);
+    const BatchEnt *E = (const BatchEnt *)(ent + ORG_BENT(v));
+    if (r == 0) post_cell(E->P, E->a, i);
+    else post_lumecm(E->P, E->a, i, E->a.nlp, (int *)red);
+}
+
+/* update_rho_to_shape (cell_phase_functions.cpp:42-62) on every plane:
+ * where phi != 0, rho = rho0 exp(-1/(4 h (1 - h) + tol) + 1/(1 + tol)) with
+ * h = phi^2 (3 - 2 phi); then rho = 0 wherever it is below
+ * rho0 exp(-1/(2 tol) + 1/(1 + tol)).  Double arithmetic, as the
+ * reference's double literals make it. */
+__global__ __launch_bounds__(256) void k_rho_to_shape(const float *__restrict__ phi, float *__restrict__ rho,
+                                                      size_t count, float rho0, float tol) {
+    const double floor_v = (double)rho0 * exp(-1.0 / (2.0 * (double)tol) + 1.0 / (1.0 + (double)tol));
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        const float p = phi[e];
+        float r = rho[e];
+        if (p != 0.0f) {
+            const double pd = p, h = pd * pd * (3.0 - 2.0 * pd);
+            r = (float)((double)rho0 * exp(-1.0 / (4.0 * h * (1.0 - h) + (double)tol) + 1.0 / (1.0 + (double)tol)));
         }
-        const float *FL = sl.cur ? a.L1 : a.L0, *FE = se.cur ? a.E1 : a.E0;
-        const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
-        int *sb = (int *)scr;
-        commit_field(P, 0, BL, blk, nblk, a.lum, FL, a.live_part, sb);
-        commit_field(P, 1, BE, blk, nblk, a.ecm, FE, a.live_part, sb);
-        /* h of the new fields on every part the step may have changed */
-        const int nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
-        const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-        for (int k = blk; k <= ntl; k += nblk) {
-            bool need = full || k == ntl;
-            if (!need) {
-                const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
-                for (int f = 0; f < 2 && !need; f++) {
-                    const int4 B = f ? BE : BL;
-                    need = !box_empty(B) && ti < B.y + 1 && ti + ORG_HT > B.x - 1 && tj < B.w + 1 &&
-                           tj + ORG_HT > B.z - 1;
-                }
-            }
-            if (!need) continue;
-            const double tot = block_sum_d(hlum_part(P, k, FL, FE, a.hlum, a.hecm), (double *)(scr + 64));
-            if (threadIdx.x == 0) a.part_h[k] = tot;
-        }
+        if ((double)r < floor_v) r = 0.0f;
+        rho[e] = r;
     }
 }
 

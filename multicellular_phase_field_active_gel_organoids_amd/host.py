@@ -64,11 +64,18 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
               "org_update_all_cells", "org_reshaping_cells", "org_commit_step", "org_sync"):
         getattr(lib, n).argtypes = [vp]
     lib.org_reshaping_lumen_or_ECM.argtypes = [vp, i32]
+    lib.org_reshaping_cells_from_phi.argtypes = [vp]
+    lib.org_update_rho_to_shape.argtypes = [vp, ctypes.c_float, ctypes.c_float]
+    lib.org_relax_initial_state.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     lib.org_step.argtypes = [vp, i32, i32, i32]
     lib.org_profile_enable.argtypes = [vp, i32]
     lib.org_kernel_time.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.org_set_option.argtypes = [vp, i32, i32]
     lib.org_debug_fft2d.argtypes = [vp, i32]
+    lib.org_batch_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
+    lib.org_batch_step.argtypes = [vp, i32, i32]
+    lib.org_batch_sync.argtypes = [vp]
+    lib.org_batch_destroy.argtypes = [vp]
     lib.org_error_string.restype = ctypes.c_char_p
     lib.org_error_string.argtypes = [i32]
     lib.org_build_info.restype = ctypes.c_char_p
@@ -227,6 +234,22 @@ class Organoid:
         """rho <- rhon, ECM <- ECMn, lum <- lumn (run_with_active_gels.cpp:175-182)."""
         _chk(self.lib.org_commit_step(self.h), "copy tables")
 
+    # ---- the IC program's device part (prepare_initial_files.cpp:82-110) --
+    def reshaping_cells_from_phi(self):
+        """reshaping_cells with phin = phi (the IC program's copy, :87-90)."""
+        _chk(self.lib.org_reshaping_cells_from_phi(self.h), "reshaping_cells (phin = phi)")
+
+    def update_rho_to_shape(self, rho0: float = 0.001, tol: float = 0.01):
+        """cell_phase_functions.cpp:42-62 on every plane."""
+        _chk(self.lib.org_update_rho_to_shape(self.h, float(rho0), float(tol)), "update_rho_to_shape")
+
+    def relax_initial_state(self, rho0: float = 0.001, tol: float = 0.01):
+        """The whole IC relaxation: centres, cell reshaping from phi, rho to
+        shape, lumen/ECM update (psi, h(lum), h(ECM) zero, as the program
+        never computes them) and both reshaping loops; phi, lum and ECM end at
+        their second-to-last iterates, as the program saves them."""
+        _chk(self.lib.org_relax_initial_state(self.h, float(rho0), float(tol)), "relax_initial_state")
+
     def plain_step(self) -> None:
         """One timestep through the per-function entry points, in the
         driver's order (run_with_active_gels.cpp:123-182)."""
@@ -266,6 +289,49 @@ class Organoid:
             _chk(self.lib.org_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n)), "org_kernel_time")
             out[name] = (ms.value, n.value)
         return out
+
+
+class Batch:
+    """Organoids that advance together, one launch per stage for all of them
+    (the counterpart of a setup_simulation.py sweep, whose runs the reference
+    starts one by one, main_code/setup_simulation.py:66-98).  Each member
+    evolves exactly as under ``Organoid.step``; the members stay usable
+    (fields, centres, iteration counts) and must outlive the batch."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = ctypes.c_void_p()
+        _chk(self.lib.org_batch_create(arr, len(self.members), ctypes.byref(h)), "org_batch_create")
+        self.h = h
+
+    @property
+    def cells(self) -> int:
+        return sum(m.nc for m in self.members)
+
+    def step(self, nsteps: int = 1, graph: bool = True) -> None:
+        _chk(self.lib.org_batch_step(self.h, int(nsteps), 1 if graph else 0), "org_batch_step")
+
+    def sync(self) -> None:
+        _chk(self.lib.org_batch_sync(self.h), "org_batch_sync")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.org_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def debug_fft2d(a: np.ndarray, sign: int) -> np.ndarray:

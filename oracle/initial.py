@@ -126,3 +126,37 @@ def dat_text(plane: np.ndarray, fmt: str = "%.15f") -> str:
     for row in np.asarray(plane, np.float32):
         rows.append("".join((fmt % float(v)) + "\n" for v in row) + "\n")
     return "".join(rows)
+
+
+def update_rho_to_shape(phi: np.ndarray, rho: np.ndarray, rho0: float, tol: float) -> np.ndarray:
+    """cell_phase_functions.cpp:42-62: where phi != 0, rho takes the ini_rho
+    value of phi; then every value below rho0*exp(-1/(2 tol) + 1/(1 + tol))
+    is set to 0 (also where phi == 0).  Double arithmetic."""
+    out = np.array(rho, np.float32, copy=True)
+    nz = phi != 0.0
+    out[nz] = ini_rho(np.asarray(phi, np.float32)[nz], rho0, tol) if nz.any() else out[nz]
+    # ini_rho already zeroes its values below the floor; the others:
+    thr = float(f32(rho0)) * math.exp(-1.0 / (2.0 * float(f32(tol))) + 1.0 / (1.0 + float(f32(tol))))
+    out[out.astype(np.float64) < thr] = 0.0
+    return out
+
+
+def relax_initial_state(o, rho0: float, tol: float) -> None:
+    """The IC program's device part (prepare_initial_files.cpp:82-110) on an
+    oracle.Oracle: centres, phin <- phi, |grad phi| and normals,
+    reshaping_cells, update_rho_to_shape, then update_lumen_and_ECM with psi,
+    h(lum), h(ECM) zero (the program never computes them) and both reshaping
+    loops; no copy-back."""
+    o.call("find_centres_cells")
+    o.set("phin", o.get("phi"))
+    o.call("compute_normal_grad_phi")
+    o.call("compute_normal_to_phi")
+    o.call("reshaping_cells")
+    o.set("rho", update_rho_to_shape(o.get("phi"), o.get("rho"), rho0, tol))
+    z = np.zeros((o.N, o.N))
+    o.set("psi", z)
+    o.set("hlum", z)
+    o.set("hecm", z)
+    o.call("update_lumen_and_ECM")
+    o.call("reshaping_lumen_or_ECM", 0)
+    o.call("reshaping_lumen_or_ECM", 1)

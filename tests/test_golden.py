@@ -1,0 +1,35 @@
+"""The committed long-trajectory fixtures (tests/golden/drift_*.npz, written
+by tests/golden/make_drift_fixtures.py) still describe the current FP64
+oracle: its first steps reproduce the recorded centres and reshaping
+iteration counts, and the stored boxes are consistent."""
+import os
+
+import numpy as np
+import pytest
+
+from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+from oracle.oracle import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("case,steps", [("two_cells_256", 12), ("organoid16_512", 3)])
+def test_fixture_matches_the_oracle(case, steps):
+    fx = np.load(os.path.join(GOLDEN, f"drift_{case}.npz"))
+    wl = make_workload(case)
+    nc, N = wl.params.ncell, wl.params.N
+    assert fx["centres"].shape == (max(fx["checkpoints"]), nc, 2)
+    o = Oracle(wl.params, wl.s0_cell)
+    o.load_workload(wl)
+    for it in range(steps):
+        o.step(it)
+        xc, yc = o.centres()
+        assert (xc == fx["centres"][it, :, 0]).all() and (yc == fx["centres"][it, :, 1]).all(), it
+        cells, lum, ecm = o.iterations()
+        assert (cells == fx["it_cells"][it]).all() and lum == fx["it_lum"][it] and ecm == fx["it_ecm"][it], it
+    for s in fx["checkpoints"]:
+        for name in ("phi", "rho", "lum", "ecm"):
+            boxes = fx[f"s{s}_{name}_box"]
+            for n, (i0, i1, j0, j1) in enumerate(boxes):
+                assert 0 <= i0 <= i1 <= N and 0 <= j0 <= j1 <= N
+                assert fx[f"s{s}_{name}_{n}"].shape == (i1 - i0, j1 - j0)

@@ -2,19 +2,22 @@
 
 The fixtures (tests/golden/drift_<case>.npz, written by
 tests/golden/make_drift_fixtures.py) hold the FP64 oracle's fields at
-checkpoints and its centres and reshaping iteration counts at every step.
-The GPU runs the same seeded workload through the batched step (hipGraphs of
-one step for two_cells_256, of eight steps, i.e. the k_mid path, for
-organoid16_512) and the test reports, per field and checkpoint, the relative
-L2 error over all cells x N^2 and the largest per-cell relative L2 error,
-plus the number of steps whose integer centres or iteration counts differ.
+checkpoints and its centres and reshaping iteration counts at every step,
+plus the drift of the oracle built in FP32 (the reference's own float
+arithmetic) over the same steps.  The GPU runs the same seeded workload
+through the batched step (hipGraphs of one step for two_cells_256, of eight
+steps, i.e. the k_mid path, for organoid16_512) and the test reports, per
+field and checkpoint, the relative L2 error over all cells x N^2 and the
+largest per-cell relative L2 error, plus the number of compared steps whose
+integer centres or iteration counts differ.
 
 Tolerances: phi relative L2 <= 1e-3 at every checkpoint (up to 10^4 steps),
-the bound the port is specified to; rho, lum, ECM and the velocity <= 1e-3
-as well.  Centres and iteration counts are compared on every recorded step;
-a mismatch is reported and allowed on at most 1 % of the steps (a centroid is
-an integer truncation of a sum over phi > 0.5, so an FP32 field that differs
-from the FP64 one in the last bits can flip a pixel at the threshold).
+the bound the port is specified to.  Every field, the velocity and the
+mismatch counts: at most twice what the FP32 build of the oracle shows
+against the FP64 one (plus 1e-6, and 1 % of the compared steps for the
+counts).  FP32 trajectories separate from the FP64 one on their own (the
+reshaping stop tests and the integer centroids are threshold decisions), so
+this is the bar an FP32 implementation can be held to.
 """
 import os
 
@@ -62,7 +65,7 @@ def test_long_trajectory_drift(case, chunk):
     g = host.Organoid(wl.params, wl.s0_cell)
     g.load_workload(wl)
     print(f"case {case}: {nsteps} steps in graphs of {chunk}", flush=True)
-    bad_centres = bad_cells = bad_lum = bad_ecm = 0
+    bad = dict(centre=0, cells=0, lum=0, ecm=0)
     compared = 0
     for it in range(0, nsteps, chunk):
         g.step(chunk, graph=True, it0=it)
@@ -70,24 +73,32 @@ def test_long_trajectory_drift(case, chunk):
         xc, yc = g.centres()
         cells, lum, ecm = g.iterations()
         compared += 1
-        bad_centres += int((xc != fx["centres"][last, :, 0]).any() or (yc != fx["centres"][last, :, 1]).any())
-        bad_cells += int((cells != fx["it_cells"][last]).any())
-        bad_lum += int(lum != fx["it_lum"][last])
-        bad_ecm += int(ecm != fx["it_ecm"][last])
+        bad["centre"] += int((xc != fx["centres"][last, :, 0]).any() or (yc != fx["centres"][last, :, 1]).any())
+        bad["cells"] += int((cells != fx["it_cells"][last]).any())
+        bad["lum"] += int(lum != fx["it_lum"][last])
+        bad["ecm"] += int(ecm != fx["it_ecm"][last])
         if it + chunk in checkpoints:
             s = it + chunk
             for name, f in FIELDS:
                 want = reconstruct(fx, f"s{s}_{name}", getattr(wl, name))
                 got = g.get_field(f)
                 r, rc = rel_l2(got, want), per_cell_max(got, want)
-                print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e}  max per cell {rc:.3e}", flush=True)
-                assert r <= 1e-3, (s, name, r)
+                ref = float(fx[f"f32_{name}_s{s}"])
+                print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e}  max per cell {rc:.3e}  (FP32 oracle {ref:.3e})",
+                      flush=True)
+                assert r <= 2.0 * ref + 1e-6, (s, name, r, ref)
+                if name == "phi":
+                    assert r <= 1e-3, (s, name, r)
             if s == nsteps and "vx" in fx:
                 for name, w in (("vx", host.W_VX), ("vy", host.W_VY)):
                     r = rel_l2(g.get_window(w), fx[name].astype(np.float64))
-                    print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e} (last step, FFT boxes)", flush=True)
-                    assert r <= 1e-3, (s, name, r)
-    print(f"  steps compared {compared}: centre mismatches {bad_centres}, cell iteration mismatches "
-          f"{bad_cells}, lumen {bad_lum}, ECM {bad_ecm}", flush=True)
-    limit = max(1, compared // 100)
-    assert bad_centres <= limit and bad_cells <= limit and bad_lum <= limit and bad_ecm <= limit
+                    ref = float(fx[f"f32_{name}"])
+                    print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e} (last step, FFT boxes; FP32 oracle {ref:.3e})",
+                          flush=True)
+                    assert r <= 2.0 * ref + 1e-6, (s, name, r, ref)
+    print(f"  steps compared {compared}: centre mismatches {bad['centre']}, cell iteration mismatches "
+          f"{bad['cells']}, lumen {bad['lum']}, ECM {bad['ecm']}", flush=True)
+    for k, v in bad.items():
+        ref = int(fx[f"f32_{k}_mismatch"])
+        print(f"    {k}: FP32 oracle {ref} of {nsteps} steps", flush=True)
+        assert v <= 2.0 * ref * compared / nsteps + 0.01 * compared, (k, v, ref)

@@ -95,6 +95,18 @@ def test_ini_ECM_lumen_partition():
     assert np.all(e2[keep] == 7.0)
 
 
+def test_update_rho_to_shape_closed_forms():
+    rho0, tol = 0.001, 0.01
+    floor_v = rho0 * math.exp(-1.0 / (2 * tol) + 1.0 / (1 + tol))
+    phi = np.array([0.0, 0.0, 0.0, 0.5, 1.0, 0.3], np.float32)
+    rho = np.array([0.5, np.float32(floor_v) * 0.5, 7.0, 3.0, 3.0, 3.0], np.float32)
+    out = oi.update_rho_to_shape(phi, rho, rho0, tol)
+    assert out[0] == np.float32(0.5) and out[2] == np.float32(7.0)   # phi == 0: kept above the floor
+    assert out[1] == 0.0                                             # ... and zeroed below it
+    assert out[3] == np.float32(rho0) and out[4] == 0.0              # interface peak, bulk
+    assert out[5] == oi.ini_rho(phi[5:6], rho0, tol)[0]
+
+
 def test_bin_layout_and_dat_text():
     slab = np.arange(10, dtype=np.float32) + 0.25
     raw = oi.bin_bytes(slab, 6, 4)

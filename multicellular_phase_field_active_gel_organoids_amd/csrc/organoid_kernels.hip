@@ -689,6 +689,84 @@ static __device__ double hlum_part(const DevP &P, const int k, const float *__re
     return acc;
 }
 
+/* The 9-point Laplacian numerator of update_all_cells (cell_phase_functions.
+ * cpp:198) as the reference's FP32 build evaluates it: the double literals
+ * -12.0 and 2.0 promote everything but the sum of the four edge neighbours to
+ * double.  t: the 3 x 3 stencil (index 0 the centre, 1-4 the edges i+1, i-1,
+ * j+1, j-1, 5-8 the corners i+1 j+1, i+1 j-1, i-1 j-1, i-1 j+1). */
+static __device__ __forceinline__ double l9d(const float *t) {
+    return -12.0 * (double)t[0] + 2.0 * (double)(t[1] + t[2] + t[3] + t[4]) + (double)t[5] + (double)t[6] +
+           (double)t[7] + (double)t[8];
+}
+
+/* The lumen/ECM part of update_all_cells' bracket (cell_phase_functions.cpp:
+ * 202-210) at one pixel, the same for every cell, from the 3 x 3 stencils of
+ * h(lum) and h(ECM), in double as the reference's FP32 build forms it
+ * (inv4 = 1 / (4 dx^2)):
+ *   -beta_L h(L) + eta_L L9(h(L)) - beta_E h(E) + eta_E L9(h(E)).
+ * The bracket's terms cancel to a small fraction of their size; formed in
+ * float, their rounding doubles how fast the phase fields separate from the
+ * FP64 solution over 10^4 steps. */
+static __device__ __forceinline__ double lumecm_bracket(const DevP &P, const float *l, const float *e,
+                                                        const double inv4) {
+    return -(double)(P.beta_lc * l[0]) + (double)P.eta_lc * l9d(l) * inv4 - (double)(P.beta_ec * e[0]) +
+           (double)P.eta_ec * l9d(e) * inv4;
+}
+
+/* The lumen/ECM bracket of part k of the h pass into lbrk, from the fields
+ * lsrc, esrc (h = hfun of them, the values the h planes hold): tile k's
+ * pixels in rows/cols 1..N-2, with h of the tile grown by one staged in LDS
+ * (lds: ORG_LDS_LBRK bytes); the ring part (k = number of tiles): the first
+ * interior rows and columns, whose stencils reach the ring.  The bracket
+ * changes only where lum or ECM changed or next to such a pixel. */
+#define ORG_LDS_LBRK (2 * (ORG_HT + 2) * (ORG_HT + 2) * 4)
+static __device__ void lbrk_part(const DevP &P, const int k, const float *__restrict__ lsrc,
+                                 const float *__restrict__ esrc, double *__restrict__ lbrk, float *lds) {
+    const int N = P.N;
+    const double inv4 = 1.0 / (4.0 * (double)P.dx * (double)P.dx);
+    if (k == hl_ntiles(N)) {
+        const int nr = 2 * (N - 2), nq = nr + 2 * (N - 4);
+        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            int i, j;
+            if (q < nr) { i = (q < N - 2) ? 1 : N - 2; j = 1 + (q < N - 2 ? q : q - (N - 2)); }
+            else { const int w = q - nr; j = (w < N - 4) ? 1 : N - 2; i = 2 + (w < N - 4 ? w : w - (N - 4)); }
+            const size_t g = (size_t)i * N + j;
+            const size_t o[9] = {g, g + N, g - N, g + 1, g - 1, g + N + 1, g + N - 1, g - N - 1, g - N + 1};
+            float l[9], e[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { l[m] = hfun(lsrc[o[m]]); e[m] = hfun(esrc[o[m]]); }
+            lbrk[g] = lumecm_bracket(P, l, e, inv4);
+        }
+        return;
+    }
+    constexpr int TW = ORG_HT + 2;
+    float *hl = lds, *he = lds + TW * TW;
+    const int nt = (N + ORG_HT - 1) / ORG_HT;
+    const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+    for (int q = threadIdx.x; q < TW * TW; q += blockDim.x) {
+        const int i = ti - 1 + q / TW, j = tj - 1 + q % TW;
+        if (i >= 0 && i < N && j >= 0 && j < N) {
+            const size_t g = (size_t)i * N + j;
+            hl[q] = hfun(lsrc[g]);
+            he[q] = hfun(esrc[g]);
+        }
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int e0 = threadIdx.x; e0 < ORG_HT * ORG_HT; e0 += blockDim.x) {
+        const int r = e0 / ORG_HT, c = e0 % ORG_HT, i = ti + r, j = tj + c;
+        if (i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2) {
+            const int q = (r + 1) * TW + (c + 1);
+            const int o[9] = {q, q + TW, q - TW, q + 1, q - 1, q + TW + 1, q + TW - 1, q - TW - 1, q - TW + 1};
+            float l[9], e[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { l[m] = hl[o[m]]; e[m] = he[o[m]]; }
+            lbrk[(size_t)i * N + j] = lumecm_bracket(P, l, e, inv4);
+        }
+    }
+    __syncthreads();      /* the next part reuses the LDS */
+}
+
 /* k_hlum: every part (grid = ntiles + 1), the total reduced by the last
  * block to finish (fixed order: deterministic). */
 __global__ __launch_bounds__(1024) void k_hlum(DevP P, const float *__restrict__ lum, const float *__restrict__ ecm,
@@ -956,15 +1034,19 @@ __global__ __launch_bounds__(1024) void k_lumecm_update(DevP P, const float *__r
 static __device__ __forceinline__ int ring_src(int i, int N) { return i == 0 ? N - 2 : (i == N - 1 ? 1 : i); }
 
 /* Central difference of the old field for |grad| in the lumen/ECM normals.
- * The reference divides the new field's gradient by the old field's
+ * The reference divides the new field's gradient by the old field's |grad|
  * (cell_phase_functions.cpp:734-763), which is ill-conditioned where the old
  * gradient vanishes: at an extremum whose neighbours are equal in exact
- * arithmetic, FP32 rounding leaves a difference of one or two ulps, |grad|
- * becomes ~1e-12 and the normal ~1e3, and the reshaping sweeps then overflow
- * to inf/NaN within a few steps.  A difference of at most 2 FP32 epsilons of
- * its larger operand (2 to 4 ulps, rounding noise rather than a resolved
- * gradient) is taken as zero, so such a pixel gets the zero normal the
- * reference gives an exactly flat one. */
+ * arithmetic (an ECM pocket between cells), FP32 rounding leaves a difference
+ * of a few ulps, |grad| becomes ~1e-12 and the normal ~1e3, and the reshaping
+ * sweeps then overflow to inf/NaN within a few steps.  Two safeguards: a
+ * difference of at most 2 FP32 epsilons of its larger operand (2 to 4 ulps,
+ * rounding noise rather than a resolved gradient) is taken as zero, and a
+ * normal longer than ORG_JMAX (rounding noise that accumulated over steps in
+ * such a pocket) is set to zero, the normal the reference gives an exactly
+ * flat pixel.  Where the gradient is resolved, the normal's length stays
+ * near 1 (the FP64 oracle's largest is 2, at the first step). */
+#define ORG_JMAX 4.0f
 static __device__ __forceinline__ float old_diff(const float p, const float m) {
     const float d = p - m;
     return fabsf(d) <= 2.0f * 1.1920929e-7f * fmaxf(fabsf(p), fabsf(m)) ? 0.0f : d;
@@ -993,6 +1075,7 @@ static __device__ __forceinline__ void normals_row(const DevP &P, const int i, c
             vy = h * (nw[g + 1] - nw[g - 1]) * rng;
         }
         const size_t e = (size_t)i * N + j;
+        if (vx * vx + vy * vy > ORG_JMAX * ORG_JMAX) { vx = 0.0f; vy = 0.0f; }
         jx[e] = vx; jy[e] = vy;
         if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
     }
@@ -1105,6 +1188,7 @@ static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4
                     vx = h * (XA[g + N] - XA[g - N]) * rng;
                     vy = h * (XA[g + 1] - XA[g - 1]) * rng;
                 }
+                if (vx * vx + vy * vy > ORG_JMAX * ORG_JMAX) { vx = 0.0f; vy = 0.0f; }
                 jx[g] = vx; jy[g] = vy;
                 if (vx != 0.0f || vy != 0.0f) { b0 = min(b0, i); b1 = max(b1, i); b2 = min(b2, j); b3 = max(b3, j); }
             }
@@ -1340,12 +1424,14 @@ static __device__ bool lum_barrier(unsigned *bar, int nblk, int &epoch, int *abo
 /* Both reshaping loops (lumen and ECM) run to their stop tests by the loop
  * workgroups of k_step, one barrier per sweep.  Every loop workgroup
  * sums the per-workgroup deviations in the same order, so all take the same
- * stop decision after the same sweep.  part: [2 parities][2 fields][nblk]. */
-static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *red, float *bufL0, float *bufL1,
+ * stop decision after the same sweep.  part: [2 parities][2 fields][nblk].
+ * Returns false if a barrier timed out; cur_out: the buffers holding the
+ * last iterates (every workgroup knows them). */
+static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *red, float *bufL0, float *bufL1,
                                      float *bufE0, float *bufE1, const float *jlx, const float *jly,
                                      const float *jex, const float *jey, const int *jbox_part, int nparts,
                                      LumState *lst, double *part, unsigned *bar, int &epoch, int *abort_flag,
-                                     int *err, long long limit) {
+                                     int *err, long long limit, int *cur_out) {
     LumState s[2] = {lst[0], lst[1]};
     const int4 jb[2] = {reduce_jbox(jbox_part, nparts, 0, P.N), reduce_jbox(jbox_part, nparts, 1, P.N)};
     for (int k = 0; !(s[0].done && s[1].done); k++) {
@@ -1361,7 +1447,7 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
             double tot = block_sum_d(acc, red);
             if (threadIdx.x == 0) pk[f * nblk + blk] = tot;
         }
-        if (!lum_barrier(bar, nblk, epoch, abort_flag, err, limit)) return;
+        if (!lum_barrier(bar, nblk, epoch, abort_flag, err, limit)) return false;
         for (int f = 0; f < 2; f++) {
             if (s[f].done) continue;
             double rdev = sum_parts(pk + f * nblk, nblk, red);
@@ -1369,6 +1455,9 @@ static __device__ void lum_loop_body(const DevP &P, int blk, int nblk, double *r
         }
     }
     if (blk == 0 && threadIdx.x < 2) lst[threadIdx.x] = s[threadIdx.x];
+    cur_out[0] = s[0].cur;
+    cur_out[1] = s[1].cur;
+    return true;
 }
 
 /* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
@@ -1461,7 +1550,8 @@ static __device__ void commit_field(const DevP &P, const int f, const int4 B, co
  * the first batched step, or after the fields were set or stepped by the
  * per-function entry points): live boxes of lum and ECM (reset, with the
  * copy-back's partial boxes, by k_live_reset), both ping-pong buffers <- the
- * fields, normals <- 0, every part of the h(lum)/h(ECM) pass.  Workgroup
+ * fields, normals <- 0, every part of the h(lum)/h(ECM) pass and both
+ * buffers of the lumen/ECM bracket.  Workgroup
  * k < ntiles: tile k; the last: the ring. */
 __global__ void k_live_reset(int *live, int *live_part, int nparts, int N) {
     if (threadIdx.x < 8) live[threadIdx.x] = (threadIdx.x & 1) ? 0 : N;
@@ -1474,12 +1564,16 @@ __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restr
                                                     float *__restrict__ E0, float *__restrict__ E1,
                                                     float *__restrict__ jlx, float *__restrict__ jly,
                                                     float *__restrict__ jex, float *__restrict__ jey,
-                                                    int *__restrict__ live) {
+                                                    int *__restrict__ live, double *__restrict__ lbrk0,
+                                                    double *__restrict__ lbrk1) {
     __shared__ double red[16];
     __shared__ int sb[8];
+    __shared__ __attribute__((aligned(16))) float lds[ORG_LDS_LBRK / 4];
     const int k = blockIdx.x, N = P.N;
     const double tot = block_sum_d(hlum_part(P, k, lum, ecm, hlum, hecm), red);
     if (threadIdx.x == 0) part[k] = tot;
+    lbrk_part(P, k, lum, ecm, lbrk0, lds);
+    lbrk_part(P, k, lum, ecm, lbrk1, lds);
     if (threadIdx.x < 8) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : N;
     __syncthreads();
     int bx[2][4] = {{N, -1, N, -1}, {N, -1, N, -1}};
@@ -1744,33 +1838,43 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
  * stride W + 2, one-pixel ring) or, with pnl == nullptr, to pw.  Planes are
  * addressed with 32-bit unsigned offsets from their (uniform) base, so the
  * 36 loads of a pixel take one register each and issue together. */
+template <bool SHARED>
 static __device__ void cells_update(const DevP &P, const int x0, const int y0, const float *__restrict__ ph,
                                     const float *__restrict__ rh, const float *__restrict__ hf,
                                     const float *__restrict__ psi, const float *__restrict__ hlum,
-                                    const float *__restrict__ hecm, const float fn, float *__restrict__ pnl,
-                                    float *__restrict__ pw, float *__restrict__ jx = nullptr,
-                                    float *__restrict__ jy = nullptr) {
+                                    const float *__restrict__ hecm, const double *__restrict__ lbrk,
+                                    const float fn, float *__restrict__ pnl, float *__restrict__ pw,
+                                    float *__restrict__ jx = nullptr, float *__restrict__ jy = nullptr) {
     constexpr int W = ORG_W, WH = ORG_WH, NPX = ORG_W * ORG_W;
     const unsigned N = (unsigned)P.N;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const double inv4d = 1.0 / (4.0 * (double)P.dx * (double)P.dx), itau = 1.0 / (double)P.tau;
     const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
     const float cplc = P.cpl / P.tau * (P.dt / P.dx / P.dx / 4.0f) / (2.50662827463f * P.sigma);
 #pragma unroll 1
     for (int e = threadIdx.x; e < NPX; e += 1024) {
         const int i = e / W, j = e - i * W;
         const unsigned g = 4u * ((unsigned)(x0 + i) * N + (unsigned)(y0 + j)), r = 4u * N;
-        /* all 45 loads first, in one basic block, so they issue together */
+        /* all loads first, in one basic block, so they issue together: with
+         * SHARED the lumen/ECM part of the bracket comes from lbrk (28
+         * loads), otherwise it is formed here from h(lum), h(ECM) (45) */
         const unsigned o[9] = {g, g + r, g - r, g + 4, g - 4, g + r + 4, g + r - 4, g - r - 4, g - r + 4};
         float fp[5], fr[4], fs[9], fh[9], fl[9], fe[9];
+        double lb = 0.0;
 #pragma unroll
         for (int k = 0; k < 5; k++) fp[k] = ldb(ph, o[k]);
 #pragma unroll
         for (int k = 0; k < 4; k++) fr[k] = ldb(rh, o[k + 1]);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            fs[k] = ldb(psi, o[k]); fh[k] = ldb(hf, o[k]); fl[k] = ldb(hlum, o[k]); fe[k] = ldb(hecm, o[k]);
+        for (int k = 0; k < 9; k++) { fs[k] = ldb(psi, o[k]); fh[k] = ldb(hf, o[k]); }
+        if (SHARED) {
+            lb = *(const double *)((const char *)lbrk + 2u * g);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) { fl[k] = ldb(hlum, o[k]); fe[k] = ldb(hecm, o[k]); }
         }
         __builtin_amdgcn_sched_barrier(0);   /* keep the loads ahead of their uses */
+        if (!SHARED) lb = lumecm_bracket(P, fl, fe, inv4d);
         const float v = fp[0];
         const float dxp = fp[1] - fp[2], dyp = fp[3] - fp[4];
         const float drx = fr[0] - fr[1], dry = fr[2] - fr[3];
@@ -1782,13 +1886,10 @@ static __device__ void cells_update(const DevP &P, const int x0, const int y0, c
         float tt[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) tt[k] = fs[k] - fh[k];
-        const float tc = tt[0];
-        const float lt = -12.0f * tc + 2.0f * (tt[1] + tt[2] + tt[3] + tt[4]) + tt[5] + tt[6] + tt[7] + tt[8];
-        const float ll = -12.0f * fl[0] + 2.0f * (fl[1] + fl[2] + fl[3] + fl[4]) + fl[5] + fl[6] + fl[7] + fl[8];
-        const float le = -12.0f * fe[0] + 2.0f * (fe[1] + fe[2] + fe[3] + fe[4]) + fe[5] + fe[6] + fe[7] + fe[8];
-        out += (1.0f / P.tau) * v * (1.0f - v) * P.dt *
-               (fn - P.beta * tc + P.eta * lt * inv4 - P.beta_lc * fl[0] + P.eta_lc * ll * inv4
-                - P.beta_ec * fe[0] + P.eta_ec * le * inv4);
+        /* the bracket and the increment in double, as in the reference's
+         * FP32 build (its double literals promote them); phin is float */
+        const double br = (double)(fn - P.beta * tt[0]) + (double)P.eta * l9d(tt) * inv4d + lb;
+        out = (float)((double)out + itau * (double)v * (1.0 - (double)v) * (double)P.dt * br);
         if (pnl) pnl[(i + 1) * WH + (j + 1)] = out;
         else pw[e] = out;
         if (jx) {     /* the normals of reshaping_cells at this pixel (see cells_normals) */
@@ -1948,8 +2049,8 @@ __global__ __launch_bounds__(1024) void k_cells(DevP P, float *__restrict__ phi,
     float *ph = phi + n * NN;
     float *pw = phin_win + (size_t)n * NPX;
     if (mode & 1) {
-        cells_update(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi, hlum, hecm, f0[n], (mode & 2) ? pnl : nullptr,
-                     pw);
+        cells_update<false>(P, x0, y0, ph, rho + n * NN, hfunc + n * NN, psi, hlum, hecm, nullptr, f0[n],
+                            (mode & 2) ? pnl : nullptr, pw);
         if (!(mode & 2)) return;
     } else if (mode & 4) {  /* phin = phi (the IC program's copy, prepare_initial_files.cpp:87-90) */
         for (int e = threadIdx.x; e < NPX; e += 1024) {
@@ -1981,6 +2082,11 @@ struct StepArgs {
     /* shared planes */
     float *lum, *ecm;
     float *psi, *hlum, *hecm;
+    /* lumen/ECM part of the cell update's bracket, two buffers: k_step's
+     * phase-field role reads lbrk[par], its lumen/ECM role writes the next
+     * step's into lbrk[1 - par]; the copy-back flips par */
+    double *lbrk0, *lbrk1;
+    int *lbrk_par;
     double *part_h, *sum_ecm;
     int nh;                  /* h(lum), h(ECM) workgroups of k_pre */
     int npsi;                /* psi workgroups of k_pre */
@@ -2136,8 +2242,31 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
                             a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
     }
     if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
-    lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
-                  a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit);
+    int cur[2];
+    if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
+                       a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur))
+        return;
+    /* the lumen/ECM part of the next step's cell-update bracket from the
+     * final iterates (the fields the copy-back commits), into the buffer this
+     * step's phase-field role does not read; on the parts that can have
+     * changed: the live boxes grown by two (the bracket's stencil reaches one
+     * pixel past the copied region), all of them in full mode, and the ring */
+    const float *FL = cur[0] ? a.L1 : a.L0, *FE = cur[1] ? a.E1 : a.E0;
+    double *nxt = *a.lbrk_par ? a.lbrk0 : a.lbrk1;
+    const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+    const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+    for (int k = blk; k <= ntl; k += nblk) {
+        bool need = full || k == ntl;
+        if (!need) {
+            const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+            for (int f = 0; f < 2 && !need; f++) {
+                const int4 B = f ? BE : BL;
+                need = !box_empty(B) && ti < B.y + 2 && ti + ORG_HT > B.x - 2 && tj < B.w + 2 &&
+                       tj + ORG_HT > B.z - 2;
+            }
+        }
+        if (need) lbrk_part(P, k, FL, FE, nxt, (float *)(lds + 256));
+    }
 }
 static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int n, unsigned char *lds) {
     constexpr int WH = ORG_WH, NPX = ORG_W * ORG_W;
@@ -2147,8 +2276,8 @@ static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int 
     const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
     float *ph = a.phi + n * NN;
     float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
-    cells_update(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, a.f0[n], pnl, nullptr,
-                 jx, jy);
+    cells_update<true>(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm,
+                       *a.lbrk_par ? a.lbrk1 : a.lbrk0, a.f0[n], pnl, nullptr, jx, jy);
     cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
     __syncthreads();      /* the workgroup reads back its own stores */
     cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
@@ -2212,7 +2341,7 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
  * ECM <- ECMn (the other workgroups, see commit_field).  Clears k_step's
  * barrier counter and abort flag. */
 static __device__ __forceinline__ void post_cell(DevP P, StepArgs a, const int n) {
-    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; *a.lbrk_par ^= 1; }
     commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
 }
 static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
@@ -2248,7 +2377,7 @@ __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
  * The previous centres come from (xp, yp), which the preps do not touch. */
 #define ORG_LDS_MID (4 * ORG_PSI_LDS_GROUP + 16)
 static __device__ __forceinline__ void mid_cell(DevP P, StepArgs a, const int n, unsigned char *scr) {
-    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; *a.lbrk_par ^= 1; }
     commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
     __syncthreads();      /* the prep reads back this workgroup's phi window */
     cell_prep(P, n, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);

@@ -1,0 +1,807 @@
+/*
+ * fno.hip -- the Fourier-neural-operator velocity surrogate on gfx950
+ * (include/organoid_fno.h): one batched BF16 matrix-core GEMM (MFMA
+ * 16x16x32, FP32 accumulation) that every layer, transform and gradient is
+ * expressed in, a few element-wise kernels, and the forward / backward /
+ * Adam orchestration.
+ *
+ * The truncated spectral convolution runs as four GEMMs per direction:
+ * along y a real GEMM with the 2*modes (cos, -sin) rows of the real DFT,
+ * along x a real GEMM over (x, re/im) with the complex twiddles of the 2*modes
+ * retained kx, the per-mode complex channel mix as a real 2w x 2w GEMM per
+ * mode, and the transposed pair for the inverse.  Layouts (B batch, w width,
+ * P = M*M pixels, m = modes):
+ *   v, u, dv      [B][w][M][M]          (channel planes)
+ *   Z, dZ         [B][w][M][2][m]        y-transform: row (b,c,x), (re|im, ky)
+ *   Zh, dZh       [B][w][2m][2][m]       x-transform: (kx_idx, re|im, ky)
+ *   Xh, Yh        [nmodes][B][w][2]      per-mode channel vectors
+ *   Zp, dZp       [B][w][M][2][m]        inverse x-transform
+ * Every GEMM call below states its operands in those layouts.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "organoid.h"
+#include "organoid_fno.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define FG_BM 64
+#define FG_BN 64
+#define FG_BK 32
+#define FG_LD (FG_BK + 8)     /* LDS row stride (bf16): 80 B, keeps 16-B fragment reads aligned */
+
+static __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+static __device__ __forceinline__ float gelu_d(float x) {
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+template <bool AF>
+static __device__ __forceinline__ float ld_op(const void *p, int64_t o) {
+    return AF ? ((const float *)p)[o] : (float)((const bf16 *)p)[o];
+}
+
+/* C = epilogue(alpha A B), one 64 x 64 tile of C per 256-thread workgroup,
+ * 4 waves of 32 x 32 (2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
+ * LDS.  Tiles are staged with the contiguous dimension across threads
+ * (k if the operand's k-stride is 1, else m or n), converted to BF16, and
+ * stored k-contiguous so each lane reads its 8-element fragment as one
+ * 16-byte LDS load: A fragment rows = lane % 16, k = 8 (lane / 16) .. +7;
+ * B fragment columns = lane % 16, same k; result rows 4 (lane / 16) .. +3,
+ * column lane % 16. */
+template <bool AF, bool BF>
+__global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p) {
+    __shared__ __attribute__((aligned(16))) bf16 As[FG_BM * FG_LD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[FG_BN * FG_LD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int z = blockIdx.z, z1 = z / p.nz2, z2 = z - z1 * p.nz2;
+    const int m0 = blockIdx.x * FG_BM, n0 = blockIdx.y * FG_BN;
+    const int64_t oA = (int64_t)z1 * p.sA1 + (int64_t)z2 * p.sA2;
+    const int64_t oB = (int64_t)z1 * p.sB1 + (int64_t)z2 * p.sB2;
+    const int64_t oC = (int64_t)z1 * p.sC1 + (int64_t)z2 * p.sC2;
+    const bool akc = p.sAk == 1, bkc = p.sBk == 1;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < p.K; k0 += FG_BK) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int e = t * 8 + j;
+            const int m = akc ? (e >> 5) : (e & 63), k = akc ? (e & 31) : (e >> 6);
+            const int gm = m0 + m, gk = k0 + k;
+            float v = 0.0f;
+            if (gm < p.M && gk < p.K) v = ld_op<AF>(p.A, oA + (int64_t)gm * p.sAm + (int64_t)gk * p.sAk);
+            As[m * FG_LD + k] = (bf16)v;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int e = t * 8 + j;
+            const int n = bkc ? (e >> 5) : (e & 63), k = bkc ? (e & 31) : (e >> 6);
+            const int gn = n0 + n, gk = k0 + k;
+            float v = 0.0f;
+            if (gn < p.N && gk < p.K) v = ld_op<BF>(p.B, oB + (int64_t)gk * p.sBk + (int64_t)gn * p.sBn);
+            Bs[n * FG_LD + k] = (bf16)v;
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            a[i] = *(const bf16x8 *)&As[(wm + i * 16 + (lane & 15)) * FG_LD + (lane >> 4) * 8];
+            b[i] = *(const bf16x8 *)&Bs[(wn + i * 16 + (lane & 15)) * FG_LD + (lane >> 4) * 8];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r, n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= p.M || n >= p.N) continue;
+                const int64_t o = oC + (int64_t)m * p.sCm + (int64_t)n * p.sCn;
+                float val = p.alpha * acc[i][j][r];
+                if (p.c_mode == 1) val += ((const float *)p.C)[o];
+                if (p.bias) val += p.bias[n];
+                const float av = p.act ? gelu(val) : val;
+                if (p.c_mode == 2) ((bf16 *)p.C)[o] = (bf16)av;
+                else ((float *)p.C)[o] = val;
+                if (p.C2) ((bf16 *)p.C2)[o] = (bf16)av;
+            }
+}
+
+/* Strided 5-D copy with conversion: out[i0..i4] (+)= in[i0..i4] (i4 fastest). */
+struct Copy5 {
+    const void *in;
+    void *out;
+    int d[5];
+    int64_t si[5], so[5];
+    int in_f32, out_bf16, accumulate;
+};
+__global__ __launch_bounds__(256) void k_copy5(Copy5 c, int64_t total) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = q, oi = 0, oo = 0;
+#pragma unroll
+        for (int a = 4; a >= 0; a--) {
+            const int64_t x = r % c.d[a];
+            r /= c.d[a];
+            oi += x * c.si[a];
+            oo += x * c.so[a];
+        }
+        const float v = c.in_f32 ? ((const float *)c.in)[oi] : (float)((const bf16 *)c.in)[oi];
+        if (c.out_bf16) ((bf16 *)c.out)[oo] = (bf16)v;
+        else if (c.accumulate) ((float *)c.out)[oo] += v;
+        else ((float *)c.out)[oo] = v;
+    }
+}
+
+__global__ void k_cvt_bf16(const float *__restrict__ x, bf16 *__restrict__ y, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        y[q] = (bf16)x[q];
+}
+
+/* Rr, Ri [nm][w][w] -> the real 2w x 2w matrix of each mode's complex mix,
+ * rows (i, re|im), columns (o, re|im):  [Rr Ri; -Ri Rr] per (i, o). */
+__global__ void k_expand_R(const float *__restrict__ Rr, const float *__restrict__ Ri, bf16 *__restrict__ Wx,
+                           int nm, int w) {
+    const int64_t n = (int64_t)nm * w * w;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int mode = (int)(q / (w * w)), i = (int)(q / w % w), o = (int)(q % w);
+        const float rr = Rr[q], ri = Ri[q];
+        bf16 *W = Wx + (int64_t)mode * 4 * w * w;
+        const int W2 = 2 * w;
+        W[(2 * i) * W2 + 2 * o] = (bf16)rr;
+        W[(2 * i + 1) * W2 + 2 * o] = (bf16)(-ri);
+        W[(2 * i) * W2 + 2 * o + 1] = (bf16)ri;
+        W[(2 * i + 1) * W2 + 2 * o + 1] = (bf16)rr;
+    }
+}
+
+/* Gradient of the expanded matrices back onto (Rr, Ri), accumulated. */
+__global__ void k_fold_dR(const float *__restrict__ G, float *__restrict__ dRr, float *__restrict__ dRi, int nm,
+                          int w) {
+    const int64_t n = (int64_t)nm * w * w;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int mode = (int)(q / (w * w)), i = (int)(q / w % w), o = (int)(q % w);
+        const float *g = G + (int64_t)mode * 4 * w * w;
+        const int W2 = 2 * w;
+        dRr[q] += g[(2 * i) * W2 + 2 * o] + g[(2 * i + 1) * W2 + 2 * o + 1];
+        dRi[q] += g[(2 * i) * W2 + 2 * o + 1] - g[(2 * i + 1) * W2 + 2 * o];
+    }
+}
+
+/* du = dv * GELU'(u) (in place allowed). */
+__global__ void k_gelu_bwd(const float *dv, const float *__restrict__ u, float *du, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        du[q] = dv[q] * gelu_d(u[q]);
+}
+
+/* g[o] += sum over b, p of X[b][o][p]  (one workgroup per o, FP64 sum in a
+ * fixed order). */
+__global__ __launch_bounds__(256) void k_rowsum(const float *__restrict__ X, float *__restrict__ g, int B, int O,
+                                                int64_t P) {
+    __shared__ double red[256];
+    const int o = blockIdx.x;
+    double acc = 0.0;
+    for (int b = 0; b < B; b++) {
+        const float *x = X + ((int64_t)b * O + o) * P;
+        for (int64_t p = threadIdx.x; p < P; p += 256) acc += (double)x[p];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g[o] += (float)red[0];
+}
+
+/* g[i] += sum over z of part[z][i]. */
+__global__ void k_zsum(const float *__restrict__ part, float *__restrict__ g, int nz, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int z = 0; z < nz; z++) s += part[(int64_t)z * n + q];
+        g[q] += s;
+    }
+}
+
+/* The network input [B][4][P] (BF16) from (phi, rho) crops [B][2][P]. */
+__global__ void k_lift_in(const float *__restrict__ in, bf16 *__restrict__ x, int B, int M, float s0, float s1) {
+    const int64_t P = (int64_t)M * M, n = (int64_t)B * P;
+    const float inv = 1.0f / (float)M;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = q / P, p = q % P;
+        const float *ib = in + b * 2 * P;
+        bf16 *xb = x + b * 4 * P;
+        xb[p] = (bf16)(ib[p] * s0);
+        xb[P + p] = (bf16)(ib[P + p] * s1);
+        xb[2 * P + p] = (bf16)((float)(p / M) * inv);
+        xb[3 * P + p] = (bf16)((float)(p % M) * inv);
+    }
+}
+
+/* Loss terms: d = out - target / out_scale; dout = 2 d / count; per-block
+ * FP64 partial sums of d^2. */
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out, const float *__restrict__ tgt,
+                                              float *__restrict__ dout, double *__restrict__ part, int B,
+                                              int64_t P, float is0, float is1) {
+    __shared__ double red[256];
+    const int64_t n = (int64_t)B * 2 * P;
+    const float sc = 2.0f / (float)n;
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(q / P % 2);
+        const float d = out[q] - tgt[q] * (c ? is1 : is0);
+        dout[q] = sc * d;
+        acc += (double)d * (double)d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* out[b][c][p] *= scale[c] */
+__global__ void k_scale_out(float *__restrict__ out, int B, int64_t P, float s0, float s1) {
+    const int64_t n = (int64_t)B * 2 * P;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        out[q] *= (q / P % 2) ? s1 : s0;
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const float gq = g[q];
+        const float mq = b1 * m[q] + (1.0f - b1) * gq, vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+        m[q] = mq;
+        v[q] = vq;
+        p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+    }
+}
+
+/* ======================================================================== */
+struct fno_net {
+    fno_config cfg{};
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int64_t P = 0;
+    int nm = 0, L = 0, w = 0, m = 0, M = 0, proj = 0, Bmax = 0;
+    size_t npar = 0;
+    float in_scale[2] = {1.0f, 1.0f}, out_scale[2] = {1.0f, 1.0f};
+    /* parameters: offsets into the flat array */
+    size_t oP = 0, obP = 0, oQ1 = 0, oc1 = 0, oQ2 = 0, oc2 = 0;
+    std::vector<size_t> oRr, oRi, oW, ob;
+    float *par = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
+    /* BF16 copies of the weights */
+    bf16 *wP = nullptr, *wQ1 = nullptr, *wQ2 = nullptr;
+    std::vector<bf16 *> wW, wX;
+    /* transform tables (BF16) */
+    bf16 *Fy = nullptr, *T = nullptr, *Ti = nullptr, *Fi = nullptr;
+    /* saved activations */
+    bf16 *xin = nullptr, *q = nullptr;
+    std::vector<bf16 *> v, xh;
+    std::vector<float *> u;
+    float *qpre = nullptr, *out = nullptr;
+    /* workspace */
+    float *Z = nullptr, *Zh = nullptr, *Yh = nullptr, *Zp = nullptr;
+    bf16 *Yh2 = nullptr;
+    float *dout = nullptr, *dq = nullptr, *dv0 = nullptr, *dv1 = nullptr, *dXh = nullptr, *gexp = nullptr,
+          *part = nullptr;
+    bf16 *dYh = nullptr, *dZh = nullptr;
+    double *lpart = nullptr, *lpart_h = nullptr;
+    std::vector<void *> allocs;
+    bool weights_fresh = false;
+};
+
+namespace {
+
+constexpr int KC = 2048;     /* K chunk of the weight-gradient GEMMs */
+constexpr int LOSS_BLOCKS = 1024;
+
+int halloc(fno_net *n, void **p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return ORG_EHIP;
+    n->allocs.push_back(*p);
+    return ORG_OK;
+}
+template <typename T>
+int dal(fno_net *n, T **p, size_t count) {
+    return halloc(n, (void **)p, count * sizeof(T));
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
+
+int launch_ok() { return hipGetLastError() == hipSuccess ? ORG_OK : ORG_EHIP; }
+
+int gemm(fno_net *n, const fno_gemm_args &a) {
+    if (a.M <= 0 || a.N <= 0 || a.K <= 0 || a.nz1 <= 0 || a.nz2 <= 0 || !a.A || !a.B || !a.C) return ORG_EINVAL;
+    if (a.c_mode < 0 || a.c_mode > 2) return ORG_EINVAL;
+    const int64_t nz = (int64_t)a.nz1 * a.nz2;
+    if (nz > 65535) return ORG_EINVAL;
+    dim3 g((a.M + FG_BM - 1) / FG_BM, (a.N + FG_BN - 1) / FG_BN, (unsigned)nz);
+    if (a.a_f32 && a.b_f32) k_fno_gemm<true, true><<<g, dim3(256), 0, n->st>>>(a);
+    else if (a.a_f32) k_fno_gemm<true, false><<<g, dim3(256), 0, n->st>>>(a);
+    else if (a.b_f32) k_fno_gemm<false, true><<<g, dim3(256), 0, n->st>>>(a);
+    else k_fno_gemm<false, false><<<g, dim3(256), 0, n->st>>>(a);
+    return launch_ok();
+}
+
+fno_gemm_args G0() {
+    fno_gemm_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.nz1 = a.nz2 = 1;
+    a.alpha = 1.0f;
+    return a;
+}
+
+int copy5(fno_net *n, const void *in, int in_f32, void *out, int out_bf16, int accumulate, const int d[5],
+          const int64_t si[5], const int64_t so[5]) {
+    Copy5 c;
+    c.in = in; c.out = out; c.in_f32 = in_f32; c.out_bf16 = out_bf16; c.accumulate = accumulate;
+    int64_t tot = 1;
+    for (int a = 0; a < 5; a++) { c.d[a] = d[a]; c.si[a] = si[a]; c.so[a] = so[a]; tot *= d[a]; }
+    hipLaunchKernelGGL(k_copy5, dim3(grid_for(tot)), dim3(256), 0, n->st, c, tot);
+    return launch_ok();
+}
+
+/* [B][w][2m][2][m] (kx_idx, re|im, ky) <-> [nm][B][w][2] (mode = kx_idx*m + ky) */
+int to_modes(fno_net *n, int B, const void *in, int in_f32, void *out, int out_bf16) {
+    const int w = n->w, m = n->m;
+    const int d[5] = {B, w, 2 * m, 2, m};
+    const int64_t si[5] = {(int64_t)w * 4 * m * m, 4 * m * m, 2 * m, m, 1};
+    const int64_t so[5] = {2 * w, 2, (int64_t)m * B * 2 * w, 1, (int64_t)B * 2 * w};
+    return copy5(n, in, in_f32, out, out_bf16, 0, d, si, so);
+}
+int from_modes(fno_net *n, int B, const void *in, int in_f32, void *out, int out_bf16) {
+    const int w = n->w, m = n->m;
+    const int d[5] = {B, w, 2 * m, 2, m};
+    const int64_t si[5] = {2 * w, 2, (int64_t)m * B * 2 * w, 1, (int64_t)B * 2 * w};
+    const int64_t so[5] = {(int64_t)w * 4 * m * m, 4 * m * m, 2 * m, m, 1};
+    return copy5(n, in, in_f32, out, out_bf16, 0, d, si, so);
+}
+
+#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+/* BF16 copies of the weights and the expanded mode matrices. */
+int prep_weights(fno_net *n) {
+    auto cvt = [&](size_t off, size_t cnt, bf16 *dst) {
+        hipLaunchKernelGGL(k_cvt_bf16, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->par + off, dst, (int64_t)cnt);
+        return launch_ok();
+    };
+    TRY(cvt(n->oP, (size_t)n->w * 4, n->wP));
+    TRY(cvt(n->oQ1, (size_t)n->proj * n->w, n->wQ1));
+    TRY(cvt(n->oQ2, (size_t)2 * n->proj, n->wQ2));
+    for (int l = 0; l < n->L; l++) {
+        TRY(cvt(n->oW[l], (size_t)n->w * n->w, n->wW[l]));
+        const int64_t cnt = (int64_t)n->nm * n->w * n->w;
+        hipLaunchKernelGGL(k_expand_R, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->par + n->oRr[l],
+                           n->par + n->oRi[l], n->wX[l], n->nm, n->w);
+        TRY(launch_ok());
+    }
+    return ORG_OK;
+}
+
+/* Pointwise GEMM over the pixels of each batch member:
+ * C[b][n][p] (=, +=) sum_k A[b][k][p] Wt[k][n]   (A: [B][K][P]; Wt with
+ * strides (sk, sn)). */
+fno_gemm_args pointwise(fno_net *n, int B, const void *A, int a_f32, int K, const void *Wt, int64_t sk,
+                        int64_t sn, void *C, int N) {
+    fno_gemm_args a = G0();
+    a.A = A; a.sAm = 1; a.sAk = n->P; a.sA1 = (int64_t)K * n->P; a.a_f32 = a_f32;
+    a.B = Wt; a.sBk = sk; a.sBn = sn;
+    a.C = C; a.sCm = 1; a.sCn = n->P; a.sC1 = (int64_t)N * n->P;
+    a.M = (int)n->P; a.N = N; a.K = K; a.nz1 = B;
+    return a;
+}
+
+/* Weight gradient g[mo][no] += sum_{b,p} X[b][mo][p] Y[b][no][p]
+ * (X: [B][Mo][P], Y: [B][No][P]) through K-chunked partials. */
+int wgrad(fno_net *n, int B, const void *X, int x_f32, int Mo, const void *Y, int y_f32, int No, float *g) {
+    const int nch = (int)((n->P + KC - 1) / KC);
+    fno_gemm_args a = G0();
+    a.A = X; a.sAm = n->P; a.sAk = 1; a.sA1 = (int64_t)Mo * n->P; a.sA2 = KC; a.a_f32 = x_f32;
+    a.B = Y; a.sBk = 1; a.sBn = n->P; a.sB1 = (int64_t)No * n->P; a.sB2 = KC; a.b_f32 = y_f32;
+    a.C = n->part; a.sCm = No; a.sCn = 1; a.sC1 = (int64_t)nch * Mo * No; a.sC2 = (int64_t)Mo * No;
+    a.M = Mo; a.N = No; a.K = (int)std::min<int64_t>(KC, n->P); a.nz1 = B; a.nz2 = nch;
+    if (n->P % KC != 0 && n->P > KC) return ORG_EINVAL;
+    TRY(gemm(n, a));
+    const int64_t cnt = (int64_t)Mo * No;
+    hipLaunchKernelGGL(k_zsum, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->part, g, B * nch, cnt);
+    return launch_ok();
+}
+
+int rowsum(fno_net *n, int B, const float *X, int O, float *g) {
+    hipLaunchKernelGGL(k_rowsum, dim3(O), dim3(256), 0, n->st, X, g, B, O, n->P);
+    return launch_ok();
+}
+
+/* Spectral part of layer l: u[l] = K_l(v[l]) (stored, overwriting). */
+int spectral_fwd(fno_net *n, int B, int l) {
+    const int w = n->w, m = n->m, M = n->M;
+    const int64_t rows = (int64_t)B * w * M;
+    if (rows > INT32_MAX) return ORG_EINVAL;
+    fno_gemm_args a = G0();
+    /* y: Z[row][(ri,ky)] = sum_y v[row][y] Fy[(ri,ky)][y] */
+    a.A = n->v[l]; a.sAm = M; a.sAk = 1;
+    a.B = n->Fy; a.sBk = 1; a.sBn = M;
+    a.C = n->Z; a.sCm = 2 * m; a.sCn = 1;
+    a.M = (int)rows; a.N = 2 * m; a.K = M;
+    TRY(gemm(n, a));
+    /* x: Zh[b,c][(kx,ri')][ky] = sum_(x,ri) T[(kx,ri')][(x,ri)] Z[b,c][(x,ri)][ky] */
+    a = G0();
+    a.A = n->T; a.sAm = 2 * M; a.sAk = 1;
+    a.B = n->Z; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)M * 2 * m; a.b_f32 = 1;
+    a.C = n->Zh; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)4 * m * m;
+    a.M = 4 * m; a.N = m; a.K = 2 * M; a.nz1 = B * w;
+    TRY(gemm(n, a));
+    TRY(to_modes(n, B, n->Zh, 1, n->xh[l], 1));
+    /* per mode: Yh[mode][b][(o,ri)] = sum_(c,ri) Xh[mode][b][(c,ri)] Wx[mode][(c,ri)][(o,ri)] */
+    a = G0();
+    a.A = n->xh[l]; a.sAm = 2 * w; a.sAk = 1; a.sA1 = (int64_t)B * 2 * w;
+    a.B = n->wX[l]; a.sBk = 2 * w; a.sBn = 1; a.sB1 = (int64_t)4 * w * w;
+    a.C = n->Yh; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)B * 2 * w;
+    a.M = B; a.N = 2 * w; a.K = 2 * w; a.nz1 = n->nm;
+    TRY(gemm(n, a));
+    TRY(from_modes(n, B, n->Yh, 1, n->Yh2, 1));
+    /* inverse x: Zp[b,o][(x,ri)][ky] = sum_(kx,ri') Ti[(x,ri)][(kx,ri')] Yh2[b,o][(kx,ri')][ky] */
+    a = G0();
+    a.A = n->Ti; a.sAm = 4 * m; a.sAk = 1;
+    a.B = n->Yh2; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)4 * m * m;
+    a.C = n->Zp; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)M * 2 * m;
+    a.M = 2 * M; a.N = m; a.K = 4 * m; a.nz1 = B * w;
+    TRY(gemm(n, a));
+    /* inverse y: u[row][y] = sum_(ri,ky) Zp[row][(ri,ky)] Fi[(ri,ky)][y] */
+    a = G0();
+    a.A = n->Zp; a.sAm = 2 * m; a.sAk = 1; a.a_f32 = 1;
+    a.B = n->Fi; a.sBk = M; a.sBn = 1;
+    a.C = n->u[l]; a.sCm = M; a.sCn = 1;
+    a.M = (int)rows; a.N = M; a.K = 2 * m;
+    return gemm(n, a);
+}
+
+int forward(fno_net *n, const float *in, int B) {
+    TRY(prep_weights(n));
+    const int w = n->w;
+    hipLaunchKernelGGL(k_lift_in, dim3(grid_for((int64_t)B * n->P)), dim3(256), 0, n->st, in, n->xin, B, n->M,
+                       n->in_scale[0], n->in_scale[1]);
+    TRY(launch_ok());
+    /* lift: v0[b][c][p] = sum_i P[c][i] x[b][i][p] + bP[c] */
+    fno_gemm_args a = pointwise(n, B, n->xin, 0, 4, n->wP, 1, 4, n->v[0], w);
+    a.c_mode = 2; a.bias = n->par + n->obP;
+    TRY(gemm(n, a));
+    for (int l = 0; l < n->L; l++) {
+        TRY(spectral_fwd(n, B, l));
+        /* u[l] += W_l v[l] + b_l;  v[l+1] = GELU(u[l]) (identity after the last layer) */
+        a = pointwise(n, B, n->v[l], 0, w, n->wW[l], 1, w, n->u[l], w);
+        a.c_mode = 1; a.bias = n->par + n->ob[l]; a.C2 = n->v[l + 1]; a.act = (l < n->L - 1);
+        TRY(gemm(n, a));
+    }
+    /* projection */
+    a = pointwise(n, B, n->v[n->L], 0, w, n->wQ1, 1, w, n->qpre, n->proj);
+    a.bias = n->par + n->oc1; a.C2 = n->q; a.act = 1;
+    TRY(gemm(n, a));
+    a = pointwise(n, B, n->q, 0, n->proj, n->wQ2, 1, n->proj, n->out, 2);
+    a.bias = n->par + n->oc2;
+    return gemm(n, a);
+}
+
+/* Backward of the spectral part of layer l: du (f32 [B][w][P]) -> gradient
+ * of (Rr, Ri) accumulated, dv += K_l^T du. */
+int spectral_bwd(fno_net *n, int B, int l, const float *du, float *dv) {
+    const int w = n->w, m = n->m, M = n->M;
+    const int64_t rows = (int64_t)B * w * M;
+    fno_gemm_args a = G0();
+    /* dZp[row][(ri,ky)] = sum_y du[row][y] Fi[(ri,ky)][y] */
+    a.A = du; a.sAm = M; a.sAk = 1; a.a_f32 = 1;
+    a.B = n->Fi; a.sBk = 1; a.sBn = M;
+    a.C = n->Zp; a.sCm = 2 * m; a.sCn = 1;
+    a.M = (int)rows; a.N = 2 * m; a.K = M;
+    TRY(gemm(n, a));
+    /* dYh2[b,o][(kx,ri')][ky] = sum_(x,ri) Ti[(x,ri)][(kx,ri')] dZp[b,o][(x,ri)][ky] */
+    a = G0();
+    a.A = n->Ti; a.sAm = 1; a.sAk = 4 * m;
+    a.B = n->Zp; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)M * 2 * m; a.b_f32 = 1;
+    a.C = n->Zh; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)4 * m * m;
+    a.M = 4 * m; a.N = m; a.K = 2 * M; a.nz1 = B * w;
+    TRY(gemm(n, a));
+    TRY(to_modes(n, B, n->Zh, 1, n->dYh, 1));
+    /* dWx[mode] = Xh[mode]^T dYh[mode] */
+    a = G0();
+    a.A = n->xh[l]; a.sAm = 1; a.sAk = 2 * w; a.sA1 = (int64_t)B * 2 * w;
+    a.B = n->dYh; a.sBk = 2 * w; a.sBn = 1; a.sB1 = (int64_t)B * 2 * w;
+    a.C = n->gexp; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)4 * w * w;
+    a.M = 2 * w; a.N = 2 * w; a.K = B; a.nz1 = n->nm;
+    TRY(gemm(n, a));
+    const int64_t cnt = (int64_t)n->nm * w * w;
+    hipLaunchKernelGGL(k_fold_dR, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->gexp, n->grad + n->oRr[l],
+                       n->grad + n->oRi[l], n->nm, w);
+    TRY(launch_ok());
+    /* dXh[mode] = dYh[mode] Wx[mode]^T */
+    a = G0();
+    a.A = n->dYh; a.sAm = 2 * w; a.sAk = 1; a.sA1 = (int64_t)B * 2 * w;
+    a.B = n->wX[l]; a.sBk = 1; a.sBn = 2 * w; a.sB1 = (int64_t)4 * w * w;
+    a.C = n->dXh; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)B * 2 * w;
+    a.M = B; a.N = 2 * w; a.K = 2 * w; a.nz1 = n->nm;
+    TRY(gemm(n, a));
+    TRY(from_modes(n, B, n->dXh, 1, n->dZh, 1));
+    /* dZ[b,c][(x,ri)][ky] = sum_(kx,ri') T[(kx,ri')][(x,ri)] dZh[b,c][(kx,ri')][ky] */
+    a = G0();
+    a.A = n->T; a.sAm = 1; a.sAk = 2 * M;
+    a.B = n->dZh; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)4 * m * m;
+    a.C = n->Z; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)M * 2 * m;
+    a.M = 2 * M; a.N = m; a.K = 4 * m; a.nz1 = B * w;
+    TRY(gemm(n, a));
+    /* dv[row][y] += sum_(ri,ky) dZ[row][(ri,ky)] Fy[(ri,ky)][y] */
+    a = G0();
+    a.A = n->Z; a.sAm = 2 * m; a.sAk = 1; a.a_f32 = 1;
+    a.B = n->Fy; a.sBk = M; a.sBn = 1;
+    a.C = dv; a.sCm = M; a.sCn = 1; a.c_mode = 1;
+    a.M = (int)rows; a.N = M; a.K = 2 * m;
+    return gemm(n, a);
+}
+
+int backward(fno_net *n, int B) {
+    const int w = n->w, proj = n->proj;
+    const int64_t P = n->P;
+    if (hipMemsetAsync(n->grad, 0, n->npar * sizeof(float), n->st) != hipSuccess) return ORG_EHIP;
+    /* projection Q2 */
+    TRY(wgrad(n, B, n->dout, 1, 2, n->q, 0, proj, n->grad + n->oQ2));
+    TRY(rowsum(n, B, n->dout, 2, n->grad + n->oc2));
+    fno_gemm_args a = pointwise(n, B, n->dout, 1, 2, n->wQ2, proj, 1, n->dq, proj);
+    TRY(gemm(n, a));
+    const int64_t nq = (int64_t)B * proj * P;
+    hipLaunchKernelGGL(k_gelu_bwd, dim3(grid_for(nq)), dim3(256), 0, n->st, n->dq, n->qpre, n->dq, nq);
+    TRY(launch_ok());
+    /* projection Q1 */
+    TRY(wgrad(n, B, n->dq, 1, proj, n->v[n->L], 0, w, n->grad + n->oQ1));
+    TRY(rowsum(n, B, n->dq, proj, n->grad + n->oc1));
+    float *dv = n->dv0, *dvn = n->dv1;
+    a = pointwise(n, B, n->dq, 1, proj, n->wQ1, w, 1, dv, w);
+    TRY(gemm(n, a));
+    const int64_t nv = (int64_t)B * w * P;
+    for (int l = n->L - 1; l >= 0; l--) {
+        if (l < n->L - 1) {
+            hipLaunchKernelGGL(k_gelu_bwd, dim3(grid_for(nv)), dim3(256), 0, n->st, dv, n->u[l], dv, nv);
+            TRY(launch_ok());
+        }
+        const float *du = dv;
+        TRY(wgrad(n, B, du, 1, w, n->v[l], 0, w, n->grad + n->oW[l]));
+        TRY(rowsum(n, B, du, w, n->grad + n->ob[l]));
+        /* dv_l = W_l^T du (pointwise), then the spectral part's transpose */
+        a = pointwise(n, B, du, 1, w, n->wW[l], w, 1, dvn, w);
+        TRY(gemm(n, a));
+        TRY(spectral_bwd(n, B, l, du, dvn));
+        std::swap(dv, dvn);
+    }
+    /* lift */
+    TRY(wgrad(n, B, dv, 1, w, n->xin, 0, 4, n->grad + n->oP));
+    return rowsum(n, B, dv, w, n->grad + n->obP);
+}
+
+/* Transform tables in double, rounded to BF16 once. */
+int make_tables(fno_net *n) {
+    const int M = n->M, m = n->m;
+    const double tw = 2.0 * M_PI / M;
+    auto kx_of = [&](int idx) { return idx < m ? idx : M - 2 * m + idx; };
+    std::vector<float> fy((size_t)2 * m * M), fi((size_t)2 * m * M), T((size_t)4 * m * 2 * M),
+        Ti((size_t)2 * M * 4 * m);
+    for (int ri = 0; ri < 2; ri++)
+        for (int ky = 0; ky < m; ky++)
+            for (int y = 0; y < M; y++) {
+                const double th = tw * ((ky * y) % M), c = (ky == 0) ? 1.0 : 2.0;
+                fy[(size_t)(ri * m + ky) * M + y] = (float)(ri ? -std::sin(th) : std::cos(th));
+                fi[(size_t)(ri * m + ky) * M + y] = (float)((ri ? -std::sin(th) : std::cos(th)) * c / ((double)M * M));
+            }
+    for (int k = 0; k < 2 * m; k++)
+        for (int x = 0; x < M; x++) {
+            const double th = tw * (((int64_t)kx_of(k) * x) % M), c = std::cos(th), s = std::sin(th);
+            /* forward (e^{-i th}): re' = c re + s im, im' = -s re + c im */
+            T[(size_t)(2 * k) * 2 * M + 2 * x] = (float)c;
+            T[(size_t)(2 * k) * 2 * M + 2 * x + 1] = (float)s;
+            T[(size_t)(2 * k + 1) * 2 * M + 2 * x] = (float)(-s);
+            T[(size_t)(2 * k + 1) * 2 * M + 2 * x + 1] = (float)c;
+            /* inverse (e^{+i th}): re = c re' - s im', im = s re' + c im' */
+            Ti[(size_t)(2 * x) * 4 * m + 2 * k] = (float)c;
+            Ti[(size_t)(2 * x) * 4 * m + 2 * k + 1] = (float)(-s);
+            Ti[(size_t)(2 * x + 1) * 4 * m + 2 * k] = (float)s;
+            Ti[(size_t)(2 * x + 1) * 4 * m + 2 * k + 1] = (float)c;
+        }
+    auto up = [&](const std::vector<float> &h, bf16 **d) -> int {
+        std::vector<bf16> b(h.size());
+        for (size_t i = 0; i < h.size(); i++) b[i] = (bf16)h[i];
+        TRY(dal(n, d, h.size()));
+        return hipMemcpy(*d, b.data(), b.size() * sizeof(bf16), hipMemcpyHostToDevice) == hipSuccess ? ORG_OK
+                                                                                                  : ORG_EHIP;
+    };
+    TRY(up(fy, &n->Fy));
+    TRY(up(fi, &n->Fi));
+    TRY(up(T, &n->T));
+    TRY(up(Ti, &n->Ti));
+    return ORG_OK;
+}
+
+int setdev(fno_net *n) { return hipSetDevice(n->dev) == hipSuccess ? ORG_OK : ORG_EHIP; }
+
+}  // namespace
+
+extern "C" {
+
+int fno_create(const fno_config *cfg, int device, fno_net **out) {
+    if (!cfg || !out) return ORG_EINVAL;
+    const fno_config c = *cfg;
+    if (c.M < 8 || c.M > 1024 || (c.M & 1) || c.width < 1 || c.width > 256 || c.modes < 1 || 2 * c.modes > c.M ||
+        c.layers < 1 || c.layers > 16 || c.proj < 1 || c.proj > 1024 || c.batch_max < 1)
+        return ORG_EINVAL;
+    if ((int64_t)c.M * c.M > KC && ((int64_t)c.M * c.M) % KC != 0) return ORG_EINVAL;
+    if ((int64_t)c.batch_max * c.width * c.M > INT32_MAX / 2) return ORG_EINVAL;
+    if ((int64_t)c.batch_max * c.width > 65535) return ORG_EINVAL;
+    fno_net *n = new fno_net();
+    n->cfg = c; n->dev = device;
+    n->M = c.M; n->w = c.width; n->m = c.modes; n->L = c.layers; n->proj = c.proj; n->Bmax = c.batch_max;
+    n->P = (int64_t)c.M * c.M;
+    n->nm = 2 * c.modes * c.modes;
+    if (setdev(n)) { delete n; return ORG_EHIP; }
+    const int w = n->w, L = n->L, B = n->Bmax;
+    size_t off = 0;
+    n->oP = off; off += (size_t)w * 4;
+    n->obP = off; off += w;
+    for (int l = 0; l < L; l++) {
+        n->oRr.push_back(off); off += (size_t)n->nm * w * w;
+        n->oRi.push_back(off); off += (size_t)n->nm * w * w;
+        n->oW.push_back(off); off += (size_t)w * w;
+        n->ob.push_back(off); off += w;
+    }
+    n->oQ1 = off; off += (size_t)n->proj * w;
+    n->oc1 = off; off += n->proj;
+    n->oQ2 = off; off += (size_t)2 * n->proj;
+    n->oc2 = off; off += 2;
+    n->npar = off;
+    int rc = ORG_OK;
+    auto A = [&](auto **p, size_t cnt) { if (!rc) rc = dal(n, p, cnt); };
+    A(&n->par, n->npar); A(&n->grad, n->npar); A(&n->am, n->npar); A(&n->av, n->npar);
+    A(&n->wP, (size_t)w * 4); A(&n->wQ1, (size_t)n->proj * w); A(&n->wQ2, (size_t)2 * n->proj);
+    n->wW.assign(L, nullptr); n->wX.assign(L, nullptr); n->u.assign(L, nullptr); n->xh.assign(L, nullptr);
+    n->v.assign(L + 1, nullptr);
+    for (int l = 0; l < L; l++) {
+        A(&n->wW[l], (size_t)w * w);
+        A(&n->wX[l], (size_t)n->nm * 4 * w * w);
+        A(&n->u[l], (size_t)B * w * n->P);
+        A(&n->xh[l], (size_t)n->nm * B * 2 * w);
+    }
+    for (int l = 0; l <= L; l++) A(&n->v[l], (size_t)B * w * n->P);
+    A(&n->xin, (size_t)B * 4 * n->P);
+    A(&n->qpre, (size_t)B * n->proj * n->P); A(&n->q, (size_t)B * n->proj * n->P);
+    A(&n->out, (size_t)B * 2 * n->P);
+    const size_t zsz = (size_t)B * w * n->M * 2 * n->m;
+    A(&n->Z, zsz); A(&n->Zp, zsz);
+    A(&n->Zh, (size_t)B * w * 4 * n->m * n->m); A(&n->Yh2, (size_t)B * w * 4 * n->m * n->m);
+    A(&n->Yh, (size_t)n->nm * B * 2 * w);
+    A(&n->dout, (size_t)B * 2 * n->P); A(&n->dq, (size_t)B * n->proj * n->P);
+    A(&n->dv0, (size_t)B * w * n->P); A(&n->dv1, (size_t)B * w * n->P);
+    A(&n->dXh, (size_t)n->nm * B * 2 * w); A(&n->dYh, (size_t)n->nm * B * 2 * w);
+    A(&n->dZh, (size_t)B * w * 4 * n->m * n->m);
+    A(&n->gexp, (size_t)n->nm * 4 * w * w);
+    const int nch = (int)((n->P + KC - 1) / KC);
+    A(&n->part, (size_t)B * nch * (size_t)std::max(n->proj, 4) * std::max(w, 4) + (size_t)B * nch * 2 * n->proj);
+    A(&n->lpart, LOSS_BLOCKS);
+    if (!rc && hipHostMalloc((void **)&n->lpart_h, LOSS_BLOCKS * sizeof(double)) != hipSuccess) rc = ORG_EHIP;
+    if (!rc) rc = make_tables(n);
+    if (!rc) {
+        if (hipMemset(n->par, 0, n->npar * sizeof(float)) != hipSuccess ||
+            hipMemset(n->am, 0, n->npar * sizeof(float)) != hipSuccess ||
+            hipMemset(n->av, 0, n->npar * sizeof(float)) != hipSuccess)
+            rc = ORG_EHIP;
+    }
+    if (rc) { fno_destroy(n); return rc; }
+    *out = n;
+    return ORG_OK;
+}
+
+int fno_destroy(fno_net *n) {
+    if (!n) return ORG_EINVAL;
+    (void)hipSetDevice(n->dev);
+    for (void *p : n->allocs) (void)hipFree(p);
+    if (n->lpart_h) (void)hipHostFree(n->lpart_h);
+    delete n;
+    return ORG_OK;
+}
+
+size_t fno_param_count(const fno_net *n) { return n ? n->npar : 0; }
+
+int fno_set_params(fno_net *n, const float *host, size_t cnt) {
+    if (!n || !host || cnt != n->npar) return ORG_EINVAL;
+    TRY(setdev(n));
+    if (hipStreamSynchronize(n->st) != hipSuccess) return ORG_EHIP;
+    return hipMemcpy(n->par, host, cnt * sizeof(float), hipMemcpyHostToDevice) == hipSuccess ? ORG_OK : ORG_EHIP;
+}
+
+int fno_get_params(fno_net *n, float *host, size_t cnt) {
+    if (!n || !host || cnt != n->npar) return ORG_EINVAL;
+    TRY(setdev(n));
+    if (hipStreamSynchronize(n->st) != hipSuccess) return ORG_EHIP;
+    return hipMemcpy(host, n->par, cnt * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? ORG_OK : ORG_EHIP;
+}
+
+int fno_set_scales(fno_net *n, const float in_scale[2], const float out_scale[2]) {
+    if (!n || !in_scale || !out_scale) return ORG_EINVAL;
+    for (int c = 0; c < 2; c++) {
+        if (!(std::isfinite(in_scale[c]) && std::isfinite(out_scale[c]) && out_scale[c] != 0.0f)) return ORG_EINVAL;
+        n->in_scale[c] = in_scale[c];
+        n->out_scale[c] = out_scale[c];
+    }
+    return ORG_OK;
+}
+
+int fno_set_stream(fno_net *n, void *stream) {
+    if (!n) return ORG_EINVAL;
+    n->st = (hipStream_t)stream;
+    return ORG_OK;
+}
+
+int fno_forward(fno_net *n, const float *in, int B, float *out) {
+    if (!n || !in || !out || B < 1 || B > n->Bmax) return ORG_EINVAL;
+    TRY(setdev(n));
+    TRY(forward(n, in, B));
+    if (hipMemcpyAsync(out, n->out, (size_t)B * 2 * n->P * sizeof(float), hipMemcpyDeviceToDevice, n->st) !=
+        hipSuccess)
+        return ORG_EHIP;
+    hipLaunchKernelGGL(k_scale_out, dim3(grid_for((int64_t)B * 2 * n->P)), dim3(256), 0, n->st, out, B, n->P,
+                       n->out_scale[0], n->out_scale[1]);
+    return launch_ok();
+}
+
+int fno_loss_grad(fno_net *n, const float *in, const float *target, int B, double *loss, float *grad_host) {
+    if (!n || !in || !target || B < 1 || B > n->Bmax) return ORG_EINVAL;
+    TRY(setdev(n));
+    TRY(forward(n, in, B));
+    hipLaunchKernelGGL(k_loss, dim3(LOSS_BLOCKS), dim3(256), 0, n->st, n->out, target, n->dout, n->lpart, B, n->P,
+                       1.0f / n->out_scale[0], 1.0f / n->out_scale[1]);
+    TRY(launch_ok());
+    TRY(backward(n, B));
+    if (hipMemcpyAsync(n->lpart_h, n->lpart, LOSS_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, n->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(n->st) != hipSuccess)
+        return ORG_EHIP;
+    if (loss) {
+        double s = 0.0;
+        for (int i = 0; i < LOSS_BLOCKS; i++) s += n->lpart_h[i];
+        *loss = s / ((double)B * 2 * n->P);
+    }
+    if (grad_host &&
+        hipMemcpy(grad_host, n->grad, n->npar * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return ORG_EHIP;
+    return ORG_OK;
+}
+
+int fno_adam_step(fno_net *n, float lr, float b1, float b2, float eps, int t) {
+    if (!n || t < 1 || !(lr >= 0.0f) || !(b1 >= 0.0f && b1 < 1.0f) || !(b2 >= 0.0f && b2 < 1.0f)) return ORG_EINVAL;
+    TRY(setdev(n));
+    const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)n->npar)), dim3(256), 0, n->st, n->par, n->grad, n->am, n->av,
+                       (int64_t)n->npar, lr, b1, b2, eps, c1, c2);
+    return launch_ok();
+}
+
+int fno_gemm(fno_net *n, const fno_gemm_args *a) {
+    if (!n || !a) return ORG_EINVAL;
+    TRY(setdev(n));
+    return gemm(n, *a);
+}
+
+}  // extern "C"

@@ -47,6 +47,40 @@ def alg_bytes_per_cell(M: int, W: int) -> int:
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_organoid64_1024_pmc.md")
 
 
+def surrogate_rates(host, org, nc: int, steps: int) -> dict:
+    """The same organoid stepped with the FNO velocity surrogate (exact solve
+    every 10th step, judge threshold off so every other step uses the
+    network), and the network's forward pass alone.  Untrained weights: the
+    time does not depend on them."""
+    import torch
+
+    from multicellular_phase_field_active_gel_organoids_amd import fno
+    cfg = fno.FnoConfig(batch_max=nc)
+    net = fno.FnoNet(cfg)
+    net.set_params(fno.init_params(cfg, 0))
+    org.step_surrogate(10, net=net, k_exact=10, threshold=1e30, it0=1)
+    org.sync()
+    t0 = time.perf_counter()
+    exact, _ = org.step_surrogate(steps, net=net, k_exact=10, threshold=1e30, it0=1)
+    org.sync()
+    dt = time.perf_counter() - t0
+    x = torch.rand((nc, 2, cfg.M, cfg.M), device="cuda")
+    out = torch.empty_like(x)
+    net.forward(x, out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        net.forward(x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd = e0.elapsed_time(e1) / 10
+    net.close()
+    return {"cell_steps_per_s": round(nc * steps / dt, 2), "ms_per_step": round(1e3 * dt / steps, 4),
+            "fno_forward_ms": round(fwd, 4), "exact_cell_solves": int(exact.sum()), "cell_steps": nc * steps,
+            "form": "eager steps; FNO (4 layers, width 32, 12 modes, BF16 MFMA) for the velocity, exact solve "
+                    "every 10th step; untrained weights (timing only)"}
+
+
 def measured_traffic(path: str = PMC_SUMMARY):
     """HBM bytes per k_step launch from the committed rocprofv3 --pmc summary
     (line 'traffic_bytes_per_launch: <int>'), or None."""
@@ -132,6 +166,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=160,
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")
+    ap.add_argument("--surrogate-steps", type=int, default=40,
+                    help="steps timed with the FNO velocity surrogate (0: skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -208,6 +244,10 @@ def main() -> int:
     if args.sweep_steps > 0:
         sweep = sweep_rates(host, args.sweep_steps)
 
+    surrogate = None
+    if args.surrogate_steps > 0:
+        surrogate = surrogate_rates(host, org, nc, args.surrogate_steps)
+
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import Oracle
@@ -235,7 +275,7 @@ def main() -> int:
                    "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
                    "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it},
-                   "sweep8": sweep},
+                   "sweep8": sweep, "surrogate": surrogate},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

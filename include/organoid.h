@@ -192,6 +192,36 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
  * and replays it.                                                            */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
 
+/* ---- the velocity surrogate in the batched step ------------------------------
+ * No counterpart in the reference (its velocity always comes from the
+ * spectral solve inside update_all_cortices, rho_functions.cpp:134-190);
+ * the network is the FNO of organoid_fno.h.  Both run the batched step
+ * eagerly (no graph), one step at a time.
+ *
+ * org_surrogate_samples: nsteps exact steps from it0, recording for step s
+ * and cell n, at index s*ncell + n, the network input (phi_n, rho_n) on the
+ * cell's FFT box at the start of the step (x_dev, [.][2][M][M] FP32), the
+ * solve's velocity (v_dev, [.][2][M][M], vx then vy) and, if res_dev is not
+ * NULL, the judge's residual of that exact velocity (res_dev, [.] FP32: the
+ * floor the finite-difference residual has on an exact solution).
+ *
+ * org_step_surrogate: nsteps steps in which the cortex velocity of every
+ * cell comes from the network (net), or, with net NULL, from v_given
+ * ([ncell][2][M][M] on the device, the same for every step: a caller's own
+ * surrogate, and the test of this path), except that (a) every step with
+ * (it0 + s) % k_exact == 0 solves every cell exactly, and (b) a cell whose
+ * relative residual of the force balance (xi v - eta (lap v + grad div v) +
+ * div pi, finite differences on the periodic box, over |div pi|) exceeds
+ * `threshold` is solved exactly in that step.  exact_cells (nsteps int32,
+ * may be NULL): cells solved exactly per step; residuals (nsteps*ncell
+ * FP32, may be NULL): the judge's residual per step and cell (-1 where the
+ * step was exact by (a)).  The network must have M = 128 and batch_max >=
+ * ncell; it runs on the organoid's stream for the call. */
+typedef struct fno_net fno_net;
+int org_surrogate_samples(org_ctx *ctx, int it0, int nsteps, float *x_dev, float *v_dev, float *res_dev);
+int org_step_surrogate(org_ctx *ctx, fno_net *net, const float *v_given, int it0, int nsteps, int k_exact,
+                       float threshold, int32_t *exact_cells, float *residuals);
+
 /* ---- batches of organoids ----------------------------------------------------
  * The counterpart of a setup_simulation.py parameter sweep
  * (main_code/setup_simulation.py:29-78 renders one constants file and one

@@ -72,6 +72,9 @@ int fno_get_params(fno_net *net, float *host, size_t n);
 int fno_set_scales(fno_net *net, const float in_scale[2], const float out_scale[2]);
 /* The stream the library launches on (0: the null stream). */
 int fno_set_stream(fno_net *net, void *hip_stream);
+int fno_get_stream(const fno_net *net, void **hip_stream);
+/* The configuration the network was created with. */
+int fno_get_config(const fno_net *net, fno_config *out);
 
 /* Inference.  in: [batch][2][M][M] FP32 (phi, rho) on the device,
  * out: [batch][2][M][M] FP32 (vx, vy). */

@@ -754,6 +754,18 @@ int fno_set_stream(fno_net *n, void *stream) {
     return ORG_OK;
 }
 
+int fno_get_stream(const fno_net *n, void **stream) {
+    if (!n || !stream) return ORG_EINVAL;
+    *stream = (void *)n->st;
+    return ORG_OK;
+}
+
+int fno_get_config(const fno_net *n, fno_config *out) {
+    if (!n || !out) return ORG_EINVAL;
+    *out = n->cfg;
+    return ORG_OK;
+}
+
 int fno_forward(fno_net *n, const float *in, int B, float *out) {
     if (!n || !in || !out || B < 1 || B > n->Bmax) return ORG_EINVAL;
     TRY(setdev(n));

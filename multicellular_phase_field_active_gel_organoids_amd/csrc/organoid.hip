@@ -5,6 +5,7 @@
  * (default priority).  All kernels come from organoid_kernels.hip.
  */
 #include "organoid_kernels.hip"
+#include "organoid_fno.h"
 
 namespace {
 
@@ -78,6 +79,11 @@ struct org_ctx {
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
+    /* velocity surrogate: network input and output per cell, per-cell switch,
+     * per-step judge residuals [cap][ncell] and exact-cell counts [cap] */
+    float *sur_x = nullptr, *sur_v = nullptr, *sur_res = nullptr;
+    int *sur_use = nullptr, *sur_nexact = nullptr;
+    int sur_cap = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; one step */
     hipGraph_t graph[2] = {nullptr, nullptr};
     /* profiling */
@@ -320,6 +326,15 @@ int L_step(org_ctx *c) {
     return launch_check();
 }
 /* copy-back of one step and the first part of the next (see k_mid) */
+/* k_step with the surrogate velocities of the cells whose switch is set */
+int L_step_v(org_ctx *c, const float *vg, const int *use) {
+    Prof pr(c, ORG_K_STEP);
+    StepArgs a = step_args(c);
+    a.vgiven = vg;
+    a.vuse = use;
+    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, a);
+    return launch_check();
+}
 int L_mid(org_ctx *c) {
     Prof pr(c, ORG_K_MID);
     hipLaunchKernelGGL(k_mid, dim3(c->nc + c->npsi + c->nlp), dim3(1024), 0, c->stream, c->dp, step_args(c));
@@ -799,6 +814,133 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     int s = 0;
     for (; s + ORG_GRAPH_STEPS <= nsteps; s += ORG_GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
     for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec[1], c->stream));
+    return harvest_device_error(c);
+}
+
+/* ---- velocity surrogate ----------------------------------------------------- */
+namespace {
+/* buffers of the surrogate for nsteps steps (per-step arrays regrown) */
+int sur_buffers(org_ctx *c, int nsteps) {
+    const size_t MM = (size_t)ORG_MB * ORG_MB;
+    if (!c->sur_x) {
+        TRY(dalloc(c, &c->sur_x, (size_t)c->nc * 2 * MM));
+        TRY(dalloc(c, &c->sur_v, (size_t)c->nc * 2 * MM));
+        TRY(dalloc(c, &c->sur_use, (size_t)c->nc));
+    }
+    if (nsteps > c->sur_cap) {
+        for (void *q : {(void *)c->sur_res, (void *)c->sur_nexact}) {
+            if (!q) continue;
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(q));
+            c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), q));
+        }
+        c->sur_res = nullptr; c->sur_nexact = nullptr;
+        TRY(dalloc(c, &c->sur_res, (size_t)nsteps * c->nc));
+        TRY(dalloc(c, &c->sur_nexact, (size_t)nsteps));
+        c->sur_cap = nsteps;
+    }
+    return ORG_OK;
+}
+/* the batched step's set-up before its first launch (see org_step) */
+int step_prologue(org_ctx *c) {
+    if (!c->next_ready) {
+        TRY(L_centroid_next(c));
+        TRY(L_live_init(c));
+    }
+    c->next_ready = true;
+    return ORG_OK;
+}
+/* after step s of a per-step loop: copy-back, the first step's rho ring,
+ * and whether the next step needs k_pre (k_mid prepared it otherwise) */
+int step_epilogue(org_ctx *c, bool last, bool &need_pre) {
+    if (last || c->ring_pending) {
+        TRY(L_post(c));
+        TRY(L_ring_if_pending(c));
+        need_pre = true;
+    } else {
+        TRY(L_mid(c));
+        need_pre = false;
+    }
+    return ORG_OK;
+}
+}  // namespace
+
+int org_surrogate_samples(org_ctx *c, int it0, int nsteps, float *x, float *v, float *res) {
+    (void)it0;
+    if (!c || !x || !v || nsteps < 1) return note_error(ORG_EINVAL);
+    TRY(need_prepared(c));
+    TRY(sur_buffers(c, 1));
+    TRY(step_prologue(c));
+    const size_t MM = (size_t)ORG_MB * ORG_MB, per = (size_t)c->nc * 2 * MM;
+    const int wv = c->write_v;
+    c->write_v = 1;
+    int rc = ORG_OK;
+    bool need_pre = true;
+    for (int s = 0; s < nsteps && !rc; s++) {
+        if (need_pre && (rc = L_pre(c))) break;
+        hipLaunchKernelGGL(k_crops, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->xc, c->yc,
+                           x + s * per);
+        if ((rc = launch_check()) || (rc = L_step(c))) break;
+        hipLaunchKernelGGL(k_pack_v, dim3(c->nc), dim3(1024), 0, c->stream, c->vxbox, c->vybox, v + s * per);
+        if ((rc = launch_check())) break;
+        if (res) {        /* phi, rho, lum, ECM and the centres are still the step's own until the copy-back */
+            hipLaunchKernelGGL(k_vresid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->lum,
+                               c->ecm, c->xc, c->yc, c->spec, v + s * per, INFINITY, res + (size_t)s * c->nc,
+                               c->sur_use, c->sur_nexact);
+            if ((rc = launch_check())) break;
+        }
+        rc = step_epilogue(c, s + 1 == nsteps, need_pre);
+    }
+    c->write_v = wv;
+    if (rc) return note_error(rc);
+    return harvest_device_error(c);
+}
+
+int org_step_surrogate(org_ctx *c, fno_net *net, const float *v_given, int it0, int nsteps, int k_exact, float thr,
+                       int32_t *exact_cells, float *residuals) {
+    if (!c || (!net && !v_given) || nsteps < 1 || k_exact < 1 || !(thr >= 0.0f)) return note_error(ORG_EINVAL);
+    if (net) {
+        fno_config fc;
+        if (fno_get_config(net, &fc) || fc.M != ORG_MB || fc.batch_max < c->nc) return note_error(ORG_EINVAL);
+    }
+    TRY(need_prepared(c));
+    TRY(sur_buffers(c, nsteps));
+    TRY(step_prologue(c));
+    void *old_stream = nullptr;
+    if (net && (fno_get_stream(net, &old_stream) || fno_set_stream(net, (void *)c->stream)))
+        return note_error(ORG_EINVAL);
+    const float *vsrc = net ? c->sur_v : v_given;
+    HIPCHK(hipMemsetAsync(c->sur_nexact, 0, sizeof(int) * nsteps, c->stream));
+    int rc = ORG_OK;
+    bool need_pre = true;
+    for (int s = 0; s < nsteps && !rc; s++) {
+        if (need_pre && (rc = L_pre(c))) break;
+        float *res = c->sur_res + (size_t)s * c->nc;
+        if ((it0 + s) % k_exact == 0) {
+            hipLaunchKernelGGL(k_fill_use, dim3(1), dim3(256), 0, c->stream, c->sur_use, c->nc, 0,
+                               c->sur_nexact + s, res);
+            if ((rc = launch_check())) break;
+        } else {
+            if (net) {
+                hipLaunchKernelGGL(k_crops, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->xc,
+                                   c->yc, c->sur_x);
+                if ((rc = launch_check()) || (rc = fno_forward(net, c->sur_x, c->nc, c->sur_v))) break;
+            }
+            hipLaunchKernelGGL(k_vresid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->lum,
+                               c->ecm, c->xc, c->yc, c->spec, vsrc, thr, res, c->sur_use, c->sur_nexact + s);
+            if ((rc = launch_check())) break;
+        }
+        if ((rc = L_step_v(c, vsrc, c->sur_use))) break;
+        rc = step_epilogue(c, s + 1 == nsteps, need_pre);
+    }
+    if (net) (void)fno_set_stream(net, old_stream);
+    if (rc) return note_error(rc);
+    if (exact_cells)
+        HIPCHK(hipMemcpyAsync(exact_cells, c->sur_nexact, sizeof(int) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    if (residuals)
+        HIPCHK(hipMemcpyAsync(residuals, c->sur_res, sizeof(float) * nsteps * c->nc, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return harvest_device_error(c);
 }
 

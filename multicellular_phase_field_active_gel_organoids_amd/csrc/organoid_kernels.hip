@@ -47,6 +47,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -1608,6 +1609,24 @@ __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restr
     }
 }
 
+/* Active stress pi of one box pixel (rho_functions.cpp:32-115): label idx,
+ * rho r, |grad phi| ng from the central differences (dxp, dyp). */
+static __device__ __forceinline__ void active_stress(const DevP &P, const int idx, const float r, const float ng,
+                                                     const float dxp, const float dyp, const float inv4,
+                                                     float &pxx, float &pxy, float &pyv) {
+    const float at = by_label(P.atan_, idx), ao = by_label(P.aort, idx), bb = by_label(P.bb, idx);
+    const float r3 = r * r * r;
+    const float base = -at * r3 + bb * r3 * r;
+    pxx = base; pxy = 0.0f; pyv = base;
+    if (ng > 0.0f) {
+        const float rng = hw_rcp(ng);
+        const float c = (at - ao) * r3 * inv4 * rng * rng;
+        pxx += c * dxp * dxp;
+        pxy += c * dxp * dyp;
+        pyv += c * dyp * dyp;
+    }
+}
+
 /* ======================================================================== */
 /* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
  *
@@ -1623,6 +1642,7 @@ __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restr
  * reference keeps only Re, rho_functions.cpp:279), F(rho v_x + i rho v_y),
  * and F^-1(H D).  The per-mode solve uses the viscosity of the real-space
  * pixel with the mode's indices (rho_functions.cpp:149-150, 170). */
+template <bool GIVEN>
 static __device__ __forceinline__ void spectral_body(const DevP &P, const int n, float2 *__restrict__ sm,
                                                      float2 *__restrict__ tw, unsigned char *__restrict__ lab,
                                                      const float *__restrict__ phi,
@@ -1630,8 +1650,12 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
                                                      const float *__restrict__ ecm, const int x0, const int y0,
                                                      float2 *__restrict__ scratch,
                                                      float *__restrict__ rho_box, float *__restrict__ vxbox,
-                                                     float *__restrict__ vybox, int write_v) {
-    /* (x0, y0): FFT box origin = centre - zone_size + 1, useful_functions.cpp:618-619 */
+                                                     float *__restrict__ vybox, int write_v,
+                                                     const float *__restrict__ vg) {
+    /* (x0, y0): FFT box origin = centre - zone_size + 1, useful_functions.cpp:618-619.
+     * vg: if not null, this cell's velocity on the box ([2][M*M], vx then vy,
+     * from the surrogate; GIVEN): the active stress, its transforms and the
+     * per-mode solve are skipped and vg stands for v in the advection. */
     const int t = threadIdx.x, N = P.N;
     constexpr int MM = ORG_MB * ORG_MB;
     constexpr int NPAIR = 63 * ORG_MB + 2 * 65;      /* mode pairs {k, -k} incl. 4 self-paired modes */
@@ -1674,25 +1698,31 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
             const int idx = turnover_label(v, fl[k], fe[k], f1[k], f2[k]);
             lab[e] = (unsigned char)idx;
             const float r = fr[k];
-            const float at = by_label(P.atan_, idx), ao = by_label(P.aort, idx), bb = by_label(P.bb, idx);
-            const float r3 = r * r * r;
-            const float base = -at * r3 + bb * r3 * r;
-            float pxx = base, pxy = 0.0f, pyv = base;
-            if (ng > 0.0f) {
-                const float rng = hw_rcp(ng);
-                const float c = (at - ao) * r3 * inv4 * rng * rng;
-                pxx += c * dxp * dxp;
-                pxy += c * dxp * dyp;
-                pyv += c * dyp * dyp;
+            if (!GIVEN) {
+                float pxx, pxy, pyv;
+                active_stress(P, idx, r, ng, dxp, dyp, inv4, pxx, pxy, pyv);
+                SB[e] = pyv;
+                sm[i * ORG_LS + j] = make_float2(pxx, pxy);
             }
             const float d = v - 0.5f;
             const float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
             const float react = -by_label(P.deg, idx) * r + by_label(P.gen, idx) * pnd;
             rbox[e] = r + P.dt * react;
-            SB[e] = pyv;
-            sm[i * ORG_LS + j] = make_float2(pxx, pxy);
         }
     }
+    if (GIVEN) {          /* given velocity: rho v straight into the transform buffer */
+#pragma unroll 4
+        for (int q = 0; q < 16; q++) {
+            const int e = t + 1024 * q, i = e >> 7, j = e & 127;
+            const float r = ldb(rh, 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + j)));
+            const float vx = vg[e], vy = vg[MM + e];
+            if (write_v) {
+                vxbox[(size_t)n * MM + e] = vx;
+                vybox[(size_t)n * MM + e] = vy;
+            }
+            sm[i * ORG_LS + j] = make_float2(r * vx, r * vy);
+        }
+    } else {
     __syncthreads();
     fft2d_128<-1>(sm, tw);
     {
@@ -1777,6 +1807,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         }
         sm[i * ORG_LS + j] = make_float2(r * z.x / MMf, r * z.y / MMf);
     }
+    }                     /* the solve (exact velocity) */
     __syncthreads();
     fft2d_128<-1>(sm, tw);
     /* P6: spectral divergence of update_fft_rho (rho_functions.cpp:230-249)
@@ -1822,8 +1853,117 @@ __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restri
     __shared__ float2 tw[128];
     __shared__ unsigned char lab[ORG_MB * ORG_MB];
     const int n = blockIdx.x;
-    spectral_body(P, n, sm, tw, lab, phi, rho, lum, ecm, xc[n] - P.zs + 1, yc[n] - P.zs + 1, scratch, rho_box,
-                  vxbox, vybox, write_v);
+    spectral_body<false>(P, n, sm, tw, lab, phi, rho, lum, ecm, xc[n] - P.zs + 1, yc[n] - P.zs + 1, scratch,
+                         rho_box, vxbox, vybox, write_v, nullptr);
+}
+
+/* ---- the velocity surrogate's side of the step --------------------------- */
+/* (phi_n, rho_n) on each cell's FFT box (origin centre - zone_size + 1):
+ * out [ncell][2][M*M]. */
+__global__ __launch_bounds__(1024) void k_crops(DevP P, const float *__restrict__ phi, const float *__restrict__ rho,
+                                                const int *__restrict__ xc, const int *__restrict__ yc,
+                                                float *__restrict__ out) {
+    constexpr int MM = ORG_MB * ORG_MB;
+    const int n = blockIdx.x, N = P.N;
+    const size_t NN = (size_t)N * N;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
+    float *o = out + (size_t)n * 2 * MM;
+    for (int e = threadIdx.x; e < MM; e += 1024) {
+        const size_t g = n * NN + (size_t)(x0 + (e >> 7)) * N + (y0 + (e & 127));
+        o[e] = phi[g];
+        o[MM + e] = rho[g];
+    }
+}
+
+/* The box velocities of the last solve, [ncell][2][M*M] (vx then vy). */
+__global__ __launch_bounds__(1024) void k_pack_v(const float *__restrict__ vxbox, const float *__restrict__ vybox,
+                                                 float *__restrict__ out) {
+    constexpr int MM = ORG_MB * ORG_MB;
+    const int n = blockIdx.x;
+    for (int e = threadIdx.x; e < MM; e += 1024) {
+        out[(size_t)n * 2 * MM + e] = vxbox[(size_t)n * MM + e];
+        out[(size_t)n * 2 * MM + MM + e] = vybox[(size_t)n * MM + e];
+    }
+}
+
+/* use[n] = val for every cell, *count = ncell * val (a fill without
+ * hipMemsetAsync). */
+__global__ void k_fill_use(int *__restrict__ use, int nc, int val, int *__restrict__ count, float *__restrict__ res) {
+    for (int n = threadIdx.x; n < nc; n += blockDim.x) {
+        use[n] = val;
+        if (res) res[n] = -1.0f;
+    }
+    if (threadIdx.x == 0 && count) *count = val ? 0 : nc;
+}
+
+/* Judge of a given velocity v ([ncell][2][M*M]) on each cell's box: the
+ * finite-difference residual of the force balance the per-mode solve
+ * inverts (velocity_mode: xi v - eta (lap v + grad div v) = -div pi, i.e.
+ * in Fourier space (xi + eta (2 kx^2 + ky^2)) vx + eta kx ky vy =
+ * -i (kx pi_xx + ky pi_xy) and its y counterpart), with central differences
+ * on the periodic box and the viscosity of each pixel's label,
+ *   res = |xi v - eta (lap v + grad div v) + div pi| / |div pi|   (L2 over
+ * the box).  use[n] = res <= thr; cells over the threshold add one to
+ * *nfall.  pi goes through the cell's spectral scratch. */
+__global__ __launch_bounds__(1024) void k_vresid(DevP P, const float *__restrict__ phi, const float *__restrict__ rho,
+                                                 const float *__restrict__ lum, const float *__restrict__ ecm,
+                                                 const int *__restrict__ xc, const int *__restrict__ yc,
+                                                 float2 *__restrict__ scratch, const float *__restrict__ v,
+                                                 float thr, float *__restrict__ res, int *__restrict__ use,
+                                                 int *__restrict__ nfall) {
+    constexpr int MM = ORG_MB * ORG_MB;
+    __shared__ unsigned char lab[MM];
+    __shared__ double red[16];
+    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
+    const size_t NN = (size_t)N * N;
+    const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
+    const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
+    float2 *SA = scratch + (size_t)n * 2 * MM;
+    float *SB = (float *)(SA + MM);
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    for (int e = t; e < MM; e += 1024) {
+        const int i = e >> 7, j = e & 127;
+        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+        const float vv = ph[g], dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+        const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+        const int idx = turnover_label(vv, lum[g], ecm[g], ph1[g], ph2[g]);
+        lab[e] = (unsigned char)idx;
+        float pxx, pxy, pyv;
+        active_stress(P, idx, rh[g], ng, dxp, dyp, inv4, pxx, pxy, pyv);
+        SA[e] = make_float2(pxx, pxy);
+        SB[e] = pyv;
+    }
+    __syncthreads();      /* the workgroup reads back its own stores */
+    const float *vx = v + (size_t)n * 2 * MM, *vy = vx + MM;
+    const float h1 = 1.0f / (2.0f * P.dx), h2 = 1.0f / (P.dx * P.dx), h11 = 1.0f / (4.0f * P.dx * P.dx);
+    double rr = 0.0, ff = 0.0;
+    for (int e = t; e < MM; e += 1024) {
+        const int i = e >> 7, j = e & 127;
+        const int ip = ((i + 1) & 127) << 7, im = ((i - 1) & 127) << 7, ic = i << 7;
+        const int jp = (j + 1) & 127, jm = (j - 1) & 127;
+        const float eta = by_label(P.visc, lab[e]);
+        const float vxc = vx[e], vyc = vy[e];
+        const float dxx_x = (vx[ip + j] - 2.0f * vxc + vx[im + j]) * h2, dyy_x = (vx[ic + jp] - 2.0f * vxc + vx[ic + jm]) * h2;
+        const float dxx_y = (vy[ip + j] - 2.0f * vyc + vy[im + j]) * h2, dyy_y = (vy[ic + jp] - 2.0f * vyc + vy[ic + jm]) * h2;
+        const float dxy_x = (vx[ip + jp] - vx[ip + jm] - vx[im + jp] + vx[im + jm]) * h11;
+        const float dxy_y = (vy[ip + jp] - vy[ip + jm] - vy[im + jp] + vy[im + jm]) * h11;
+        const float fx = (SA[ip + j].x - SA[im + j].x) * h1 + (SA[ic + jp].y - SA[ic + jm].y) * h1;
+        const float fy = (SA[ip + j].y - SA[im + j].y) * h1 + (SB[ic + jp] - SB[ic + jm]) * h1;
+        const float rx = P.friction * vxc - eta * (2.0f * dxx_x + dyy_x + dxy_y) + fx;
+        const float ry = P.friction * vyc - eta * (dxx_y + 2.0f * dyy_y + dxy_x) + fy;
+        rr += (double)rx * rx + (double)ry * ry;
+        ff += (double)fx * fx + (double)fy * fy;
+    }
+    rr = block_sum_d(rr, red);
+    ff = block_sum_d(ff, red);
+    if (t == 0) {
+        const float q = ff > 0.0 ? (float)sqrt(rr / ff) : (rr > 0.0 ? INFINITY : 0.0f);
+        res[n] = q;
+        const int ok = q <= thr;
+        use[n] = ok;
+        if (!ok) atomicAdd(nfall, 1);
+    }
 }
 
 /* ======================================================================== */
@@ -2102,6 +2242,10 @@ struct StepArgs {
     float2 *spec;
     float *rho_box, *vxbox, *vybox;
     int write_v;
+    /* surrogate velocities [ncell][2][M*M] and per-cell switch (1: use
+     * vgiven instead of the solve); null: every cell solves */
+    const float *vgiven;
+    const int *vuse;
     /* phase field */
     float *phin_win, *phi_win, *jwin;
     /* barrier of the lumen/ECM workgroups, abort flag */
@@ -2209,9 +2353,14 @@ __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
 static __device__ __forceinline__ void step_spectral(DevP P, StepArgs a, const int b,
                                                      unsigned char *lds) {
     float2 *sm = (float2 *)lds;
-    spectral_body(P, b, sm, sm + ORG_MB * ORG_LS, (unsigned char *)(sm + ORG_MB * ORG_LS + 128), a.phi, a.rho,
-                  a.lum, a.ecm, a.xc[b] - P.zs + 1, a.yc[b] - P.zs + 1, a.spec, a.rho_box, a.vxbox, a.vybox,
-                  a.write_v);
+    unsigned char *lab = (unsigned char *)(sm + ORG_MB * ORG_LS + 128);
+    const int x0 = a.xc[b] - P.zs + 1, y0 = a.yc[b] - P.zs + 1;
+    if (a.vuse && a.vuse[b])
+        spectral_body<true>(P, b, sm, sm + ORG_MB * ORG_LS, lab, a.phi, a.rho, a.lum, a.ecm, x0, y0, a.spec,
+                            a.rho_box, a.vxbox, a.vybox, a.write_v, a.vgiven + (size_t)b * 2 * ORG_MB * ORG_MB);
+    else
+        spectral_body<false>(P, b, sm, sm + ORG_MB * ORG_LS, lab, a.phi, a.rho, a.lum, a.ecm, x0, y0, a.spec,
+                             a.rho_box, a.vxbox, a.vybox, a.write_v, nullptr);
 }
 static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
                                                    unsigned char *lds) {

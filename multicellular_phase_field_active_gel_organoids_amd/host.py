@@ -50,6 +50,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is missing: run __graft_entry__.build() (or make -C csrc)")
+    try:
+        # torch's wheel carries its own HIP runtime under the same soname;
+        # loaded first, it is the one runtime the process uses (loaded
+        # after this library, torch finds no GPU)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     vp, i32 = ctypes.c_void_p, ctypes.c_int
     lib.org_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
@@ -76,6 +83,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_batch_step.argtypes = [vp, i32, i32]
     lib.org_batch_sync.argtypes = [vp]
     lib.org_batch_destroy.argtypes = [vp]
+    lib.org_surrogate_samples.argtypes = [vp, i32, i32, vp, vp, vp]
+    lib.org_step_surrogate.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, vp, vp]
     lib.org_error_string.restype = ctypes.c_char_p
     lib.org_error_string.argtypes = [i32]
     lib.org_build_info.restype = ctypes.c_char_p
@@ -176,6 +185,41 @@ class Organoid:
         _chk(self.lib.org_get_iterations(self.h, _p(cells), ctypes.byref(lum), ctypes.byref(ecm)),
              "org_get_iterations")
         return cells, lum.value, ecm.value
+
+    def surrogate_samples(self, nsteps: int, it0: int = 0, residuals: bool = False):
+        """Run nsteps exact batched steps and return, per step and cell, the
+        (phi_n, rho_n) box crop at the step's start, the solve's (vx, vy) on
+        the box ([nsteps*ncell, 2, M, M] CUDA tensors) and, if asked, the
+        judge's residual of that exact velocity (org_surrogate_samples)."""
+        import torch
+        M = self.M
+        x = torch.empty((nsteps * self.nc, 2, M, M), dtype=torch.float32, device="cuda")
+        v = torch.empty_like(x)
+        r = torch.empty(nsteps * self.nc, dtype=torch.float32, device="cuda") if residuals else None
+        torch.cuda.synchronize()
+        _chk(self.lib.org_surrogate_samples(self.h, it0, nsteps, ctypes.c_void_p(x.data_ptr()),
+                                            ctypes.c_void_p(v.data_ptr()),
+                                            ctypes.c_void_p(r.data_ptr()) if residuals else None),
+             "org_surrogate_samples")
+        self.sync()
+        return (x, v, r) if residuals else (x, v)
+
+    def step_surrogate(self, nsteps: int, net=None, v_given=None, k_exact: int = 10, threshold: float = 0.1,
+                       it0: int = 0):
+        """nsteps batched steps with the velocity from the FNO `net` (or the
+        fixed CUDA tensor v_given [ncell, 2, M, M]), the exact solve every
+        k_exact steps and wherever the judge's residual exceeds threshold
+        (org_step_surrogate).  Returns (exact cells per step, residuals
+        [nsteps, ncell])."""
+        import torch
+        exact = np.zeros(nsteps, np.int32)
+        res = np.zeros((nsteps, self.nc), np.float32)
+        torch.cuda.synchronize()
+        _chk(self.lib.org_step_surrogate(self.h, net.h if net is not None else None,
+                                         ctypes.c_void_p(v_given.data_ptr()) if v_given is not None else None,
+                                         it0, nsteps, k_exact, float(threshold), _p(exact), _p(res)),
+             "org_step_surrogate")
+        return exact, res
 
     def set_option(self, option: int, value: int) -> None:
         _chk(self.lib.org_set_option(self.h, option, int(value)), "org_set_option")

@@ -1,0 +1,157 @@
+"""The velocity surrogate's side of the batched step (org_surrogate_samples,
+org_step_surrogate): the recorded training pairs are the exact step's own
+inputs and velocities, the judge's fallback and the every-k exact steps give
+the exact trajectory bit for bit, and a given velocity enters the cortex
+update in place of the solve's."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from multicellular_phase_field_active_gel_organoids_amd import host  # noqa: E402
+from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload  # noqa: E402
+
+FIELDS = (("phi", host.F_PHI), ("rho", host.F_RHO), ("lum", host.F_LUM), ("ecm", host.F_ECM))
+
+
+def _org(wl):
+    g = host.Organoid(wl.params, wl.s0_cell)
+    g.load_workload(wl)
+    return g
+
+
+def _crop(plane, xc, yc, zs, M):
+    return plane[xc - zs + 1:xc - zs + 1 + M, yc - zs + 1:yc - zs + 1 + M]
+
+
+def test_samples_are_the_exact_steps_inputs_and_velocities():
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: 4 recorded steps vs org_step", flush=True)
+    g1, g2 = _org(wl), _org(wl)
+    x, v, r = g1.surrogate_samples(4, residuals=True)
+    g2.step(1, graph=False)
+    xc, yc = g2.centres()
+    nc, zs, M = wl.params.ncell, wl.params.zone_size, wl.params.M
+    x0, v0 = x[:nc].cpu().numpy(), v[:nc].cpu().numpy()
+    for n in range(nc):
+        assert np.array_equal(x0[n, 0], _crop(wl.phi[n].astype(np.float32), xc[n], yc[n], zs, M))
+        assert np.array_equal(x0[n, 1], _crop(wl.rho[n].astype(np.float32), xc[n], yc[n], zs, M))
+    assert np.array_equal(v0[:, 0], g2.get_window(host.W_VX))
+    assert np.array_equal(v0[:, 1], g2.get_window(host.W_VY))
+    g2.step(3, graph=False)
+    for name, f in FIELDS:
+        assert np.array_equal(g1.get_field(f), g2.get_field(f)), name
+    rr = r.cpu().numpy()
+    print(f"  judge residual of the exact velocity: {np.array2string(rr, precision=4)}", flush=True)
+    assert np.isfinite(rr).all() and (rr > 0).all() and (rr < 1).all()
+
+
+def test_fallback_and_exact_steps_give_the_exact_trajectory():
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: 5 surrogate steps with every cell falling back", flush=True)
+    g1, g2 = _org(wl), _org(wl)
+    nc = wl.params.ncell
+    zero = torch.zeros((nc, 2, wl.params.M, wl.params.M), dtype=torch.float32, device="cuda")
+    exact, res = g1.step_surrogate(5, v_given=zero, k_exact=3, threshold=0.5, it0=0)
+    g2.step(5, graph=False)
+    for name, f in FIELDS:
+        assert np.array_equal(g1.get_field(f), g2.get_field(f)), name
+    assert (exact == nc).all()
+    for s in range(5):
+        if s % 3 == 0:
+            assert (res[s] == -1.0).all()
+        else:             # v = 0 leaves exactly the active-stress divergence: residual 1
+            assert (res[s] == 1.0).all(), res[s]
+
+
+def test_given_exact_velocity_replaces_the_solve():
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: one step with the solve's own velocity given", flush=True)
+    gs, g1, g2 = _org(wl), _org(wl), _org(wl)
+    _, v, r = gs.surrogate_samples(1, residuals=True)
+    exact, res = g1.step_surrogate(1, v_given=v, k_exact=2, threshold=1e30, it0=1)
+    g2.step(1, graph=False)
+    assert exact[0] == 0
+    assert np.array_equal(res[0], r.cpu().numpy())
+    assert np.array_equal(g1.get_field(host.F_PHI), g2.get_field(host.F_PHI))
+    a, b = g1.get_field(host.F_RHO).astype(np.float64), g2.get_field(host.F_RHO).astype(np.float64)
+    d = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"  rho rel L2 given vs solved velocity: {d:.3e}", flush=True)
+    assert d < 1e-6
+    for name, f in (("lum", host.F_LUM), ("ecm", host.F_ECM)):
+        assert np.array_equal(g1.get_field(f), g2.get_field(f)), name
+
+
+def test_trained_surrogate_ablation():
+    """What the trained network gains over the method without it, measured
+    on organoid16_512: train on pairs from the exact solve, then run the
+    same steps from one state exact, with the network (exact every 10th
+    step) and with a zero velocity, and compare time and trajectories."""
+    import time
+    from multicellular_phase_field_active_gel_organoids_amd import fno
+    wl = make_workload("organoid16_512")
+    nc = wl.params.ncell
+    print("case organoid16_512: surrogate ablation", flush=True)
+    g = _org(wl)
+    g.step(100)
+    x, v, floor = g.surrogate_samples(48, residuals=True)
+    g.step(40)
+    xv, vv = g.surrogate_samples(6)
+    cfg = fno.FnoConfig(batch_max=nc)
+    net = fno.FnoNet(cfg)
+    try:
+        net.set_params(fno.init_params(cfg, 2))
+        sx, sy = fno.fit_scales(x[::4].cpu().numpy(), v[::4].cpu().numpy())
+        net.set_scales(sx, sy)
+        losses = fno.train(net, x, v, steps=300, batch=nc, lr=2e-3, lr_final=2e-4)
+        pred = torch.cat([net.forward(xv[b:b + nc].contiguous()) for b in range(0, xv.shape[0], nc)])
+        err = ((pred - vv).double().flatten(1).norm(dim=1) / vv.double().flatten(1).norm(dim=1)).cpu().numpy()
+        state = {f: g.get_field(f) for _, f in FIELDS}
+
+        def fresh():
+            h = host.Organoid(wl.params, wl.s0_cell)
+            for f, a in state.items():
+                h.set_field(f, a)
+            h.set_field(host.F_HFUNC, np.zeros_like(state[host.F_PHI]))
+            h.prepare()
+            return h
+        NS = 30
+        runs = {}
+        for name in ("exact", "fno", "zero"):
+            h = fresh()
+            h.step(1, graph=False)
+            h.sync()
+            t0 = time.perf_counter()
+            if name == "exact":
+                h.step(NS, graph=False)
+                ex = np.full(NS, nc)
+            elif name == "fno":
+                ex, res = h.step_surrogate(NS, net=net, k_exact=10, threshold=1e30, it0=1)
+            else:
+                zero = torch.zeros((nc, 2, wl.params.M, wl.params.M), dtype=torch.float32, device="cuda")
+                ex, _ = h.step_surrogate(NS, v_given=zero, k_exact=10, threshold=1e30, it0=1)
+            h.sync()
+            runs[name] = (h, (time.perf_counter() - t0) / NS, ex)
+        ref = {f: runs["exact"][0].get_field(f).astype(np.float64) for f in (host.F_PHI, host.F_RHO)}
+
+        def drift(h, f):
+            return np.linalg.norm(h.get_field(f) - ref[f]) / np.linalg.norm(ref[f])
+        print(f"  training loss {losses[0]:.3e} -> {np.mean(losses[-20:]):.3e}; validation rel L2 per cell "
+              f"median {np.median(err):.3f}, 90% {np.percentile(err, 90):.3f}", flush=True)
+        print(f"  judge residual of exact velocities {np.median(floor.cpu().numpy()):.3f} (median), of the "
+              f"network's {np.median(res[res >= 0]):.3f}", flush=True)
+        for name, (h, t, ex) in runs.items():
+            print(f"  {name:5s}: {1e3 * t:.3f} ms/step, exact cell solves {int(ex.sum())} of {NS * nc}, drift vs "
+                  f"exact phi {drift(h, host.F_PHI):.3e} rho {drift(h, host.F_RHO):.3e}", flush=True)
+        assert np.mean(losses[-20:]) < 0.2 * losses[0]
+        assert np.median(err) < 0.5           # a zero prediction scores 1.0
+        for name in ("fno", "zero"):          # exact every 10th step (it0 = 1: steps 9, 19, 29)
+            ex = runs[name][2]
+            assert [s for s in range(NS) if ex[s] == nc] == [s for s in range(NS) if (1 + s) % 10 == 0]
+            assert ex.sum() == 3 * nc
+        d_fno, d_zero = drift(runs["fno"][0], host.F_RHO), drift(runs["zero"][0], host.F_RHO)
+        assert np.isfinite(d_fno) and d_fno < d_zero
+    finally:
+        net.close()

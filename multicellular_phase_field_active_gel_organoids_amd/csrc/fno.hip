@@ -55,42 +55,77 @@ static __device__ __forceinline__ float ld_op(const void *p, int64_t o) {
  * 16-byte LDS load: A fragment rows = lane % 16, k = 8 (lane / 16) .. +7;
  * B fragment columns = lane % 16, same k; result rows 4 (lane / 16) .. +3,
  * column lane % 16. */
+/* One K segment of a GEMM: C += A B over k < K (strides as fno_gemm_args). */
+struct GemmSeg {
+    const void *A;
+    int64_t sAm, sAk, sA1, sA2;
+    const void *B;
+    int64_t sBk, sBn, sB1, sB2;
+    int K;
+};
+
+/* Eight consecutive operand elements from a 16-byte aligned offset. */
+template <bool F32>
+static __device__ __forceinline__ void ld8(const void *p, int64_t o, float (&v)[8]) {
+    if (F32) {
+        const float4 a = *(const float4 *)((const float *)p + o), b = *(const float4 *)((const float *)p + o + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        const bf16x8 a = *(const bf16x8 *)((const bf16 *)p + o);
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = (float)a[i];
+    }
+}
+
+/* Stage one 64 x 32 operand tile (rows r0.., k0..) into LDS as [row][k]:
+ * thread t takes 8 elements consecutive along the operand's unit-stride
+ * dimension (k if kc, else the row), as one vector load when they are in
+ * range and 16-byte aligned (vec: the offsets are multiples of 8 BF16 / 4
+ * FP32 elements), and stores them k-contiguous. */
+template <bool F32>
+static __device__ __forceinline__ void stage_tile(const void *p, const int64_t o0, const int64_t s_row,
+                                                  const int64_t s_k, const bool kc, const bool vec, const int r0,
+                                                  const int nrow, const int k0, const int K, bf16 *S) {
+    const int e0 = threadIdx.x * 8;
+    const int r = kc ? (e0 >> 5) : (e0 & 63), k = kc ? (e0 & 31) : (e0 >> 6);
+    const int gr = r0 + r, gk = k0 + k;
+    float v[8];
+    const bool inb = kc ? (gr < nrow && gk + 8 <= K) : (gk < K && gr + 8 <= nrow);
+    if (vec && (kc || s_row == 1) && inb) {
+        ld8<F32>(p, o0 + (int64_t)gr * s_row + (int64_t)gk * s_k, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int rr = kc ? gr : gr + j, kk = kc ? gk + j : gk;
+            v[j] = (rr < nrow && kk < K) ? ld_op<F32>(p, o0 + (int64_t)rr * s_row + (int64_t)kk * s_k) : 0.0f;
+        }
+    }
+    if (kc) {
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; j++) b[j] = (bf16)v[j];
+        *(bf16x8 *)&S[r * FG_LD + k] = b;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) S[(r + j) * FG_LD + k] = (bf16)v[j];
+    }
+}
+
 template <bool AF, bool BF>
-__global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p) {
-    __shared__ __attribute__((aligned(16))) bf16 As[FG_BM * FG_LD];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[FG_BN * FG_LD];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int z = blockIdx.z, z1 = z / p.nz2, z2 = z - z1 * p.nz2;
-    const int m0 = blockIdx.x * FG_BM, n0 = blockIdx.y * FG_BN;
-    const int64_t oA = (int64_t)z1 * p.sA1 + (int64_t)z2 * p.sA2;
-    const int64_t oB = (int64_t)z1 * p.sB1 + (int64_t)z2 * p.sB2;
-    const int64_t oC = (int64_t)z1 * p.sC1 + (int64_t)z2 * p.sC2;
-    const bool akc = p.sAk == 1, bkc = p.sBk == 1;
+static __device__ __forceinline__ void gemm_seg(const GemmSeg &g, const int M, const int N, const int z1,
+                                                const int z2, const int m0, const int n0, bf16 *As, bf16 *Bs,
+                                                f32x4 (&acc)[2][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t oA = (int64_t)z1 * g.sA1 + (int64_t)z2 * g.sA2;
+    const int64_t oB = (int64_t)z1 * g.sB1 + (int64_t)z2 * g.sB2;
+    const bool akc = g.sAk == 1, bkc = g.sBk == 1;
+    const int va = AF ? 4 : 8, vb = BF ? 4 : 8;
+    const bool veca = oA % va == 0 && (akc ? g.sAm : g.sAk) % va == 0;
+    const bool vecb = oB % vb == 0 && (bkc ? g.sBn : g.sBk) % vb == 0;
     const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-    f32x4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < p.K; k0 += FG_BK) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int e = t * 8 + j;
-            const int m = akc ? (e >> 5) : (e & 63), k = akc ? (e & 31) : (e >> 6);
-            const int gm = m0 + m, gk = k0 + k;
-            float v = 0.0f;
-            if (gm < p.M && gk < p.K) v = ld_op<AF>(p.A, oA + (int64_t)gm * p.sAm + (int64_t)gk * p.sAk);
-            As[m * FG_LD + k] = (bf16)v;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int e = t * 8 + j;
-            const int n = bkc ? (e >> 5) : (e & 63), k = bkc ? (e & 31) : (e >> 6);
-            const int gn = n0 + n, gk = k0 + k;
-            float v = 0.0f;
-            if (gn < p.N && gk < p.K) v = ld_op<BF>(p.B, oB + (int64_t)gk * p.sBk + (int64_t)gn * p.sBn);
-            Bs[n * FG_LD + k] = (bf16)v;
-        }
+    for (int k0 = 0; k0 < g.K; k0 += FG_BK) {
+        stage_tile<AF>(g.A, oA, g.sAm, g.sAk, akc, veca, m0, M, k0, g.K, As);
+        stage_tile<BF>(g.B, oB, g.sBn, g.sBk, bkc, vecb, n0, N, k0, g.K, Bs);
         __syncthreads();
         bf16x8 a[2], b[2];
 #pragma unroll
@@ -104,6 +139,35 @@ __global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p) {
             for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+}
+
+/* C = epilogue(alpha A B), one 64 x 64 tile of C per 256-thread workgroup,
+ * 4 waves of 32 x 32 (2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
+ * LDS.  Tiles are staged with the contiguous dimension across threads
+ * (k if the operand's k-stride is 1, else m or n), converted to BF16, and
+ * stored k-contiguous so each lane reads its 8-element fragment as one
+ * 16-byte LDS load: A fragment rows = lane % 16, k = 8 (lane / 16) .. +7;
+ * B fragment columns = lane % 16, same k; result rows 4 (lane / 16) .. +3,
+ * column lane % 16.  SEG2: a second K segment (BF16 operands, s2) adds into
+ * the same accumulators before the epilogue; bias_m: the bias is indexed by
+ * the row instead of the column. */
+template <bool AF, bool BF, bool SEG2>
+__global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p, GemmSeg s2, int bias_m) {
+    __shared__ __attribute__((aligned(16))) bf16 As[FG_BM * FG_LD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[FG_BN * FG_LD];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = blockIdx.z, z1 = z / p.nz2, z2 = z - z1 * p.nz2;
+    const int m0 = blockIdx.x * FG_BM, n0 = blockIdx.y * FG_BN;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const GemmSeg s1 = {p.A, p.sAm, p.sAk, p.sA1, p.sA2, p.B, p.sBk, p.sBn, p.sB1, p.sB2, p.K};
+    gemm_seg<AF, BF>(s1, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
+    if (SEG2) gemm_seg<false, false>(s2, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
+    const int64_t oC = (int64_t)z1 * p.sC1 + (int64_t)z2 * p.sC2;
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -115,7 +179,7 @@ __global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p) {
                 const int64_t o = oC + (int64_t)m * p.sCm + (int64_t)n * p.sCn;
                 float val = p.alpha * acc[i][j][r];
                 if (p.c_mode == 1) val += ((const float *)p.C)[o];
-                if (p.bias) val += p.bias[n];
+                if (p.bias) val += p.bias[bias_m ? m : n];
                 const float av = p.act ? gelu(val) : val;
                 if (p.c_mode == 2) ((bf16 *)p.C)[o] = (bf16)av;
                 else ((float *)p.C)[o] = val;
@@ -218,18 +282,83 @@ __global__ void k_zsum(const float *__restrict__ part, float *__restrict__ g, in
     }
 }
 
-/* The network input [B][4][P] (BF16) from (phi, rho) crops [B][2][P]. */
-__global__ void k_lift_in(const float *__restrict__ in, bf16 *__restrict__ x, int B, int M, float s0, float s1) {
+/* The lift v0[b][c][p] = sum_i P[c][i] in_i + bP[c] with the input channels
+ * in = (phi s0, rho s1, i / M, j / M) from the crops [B][2][P]; FP32
+ * arithmetic, v0 stored BF16.  xin (training: the BF16 input the weight
+ * gradient of P needs) may be null. */
+__global__ __launch_bounds__(256) void k_lift(const float *__restrict__ in, const float *__restrict__ Pw,
+                                              const float *__restrict__ bP, bf16 *__restrict__ v0,
+                                              bf16 *__restrict__ xin, int B, int M, int w, float s0, float s1) {
     const int64_t P = (int64_t)M * M, n = (int64_t)B * P;
     const float inv = 1.0f / (float)M;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = q / P, p = q % P;
         const float *ib = in + b * 2 * P;
-        bf16 *xb = x + b * 4 * P;
-        xb[p] = (bf16)(ib[p] * s0);
-        xb[P + p] = (bf16)(ib[P + p] * s1);
-        xb[2 * P + p] = (bf16)((float)(p / M) * inv);
-        xb[3 * P + p] = (bf16)((float)(p % M) * inv);
+        const float x0 = ib[p] * s0, x1 = ib[P + p] * s1, x2 = (float)(p / M) * inv, x3 = (float)(p % M) * inv;
+        if (xin) {
+            bf16 *xb = xin + b * 4 * P;
+            xb[p] = (bf16)x0; xb[P + p] = (bf16)x1; xb[2 * P + p] = (bf16)x2; xb[3 * P + p] = (bf16)x3;
+        }
+        bf16 *vb = v0 + b * w * P + p;
+        for (int c = 0; c < w; c++)
+            vb[(int64_t)c * P] = (bf16)(Pw[4 * c] * x0 + Pw[4 * c + 1] * x1 + Pw[4 * c + 2] * x2 + Pw[4 * c + 3] * x3 + bP[c]);
+    }
+}
+
+/* Inference projection for width 32 and 128 hidden channels:
+ * out[b][o][p] = scale_o (sum_j Q2[o][j] GELU(sum_c Q1[j][c] v[b][c][p] + c1[j]) + c2[o]).
+ * One 64-pixel tile per 256-thread workgroup: the 32-channel tile staged in
+ * LDS as [pixel][channel], Q1 v on the matrix cores (one MFMA 16x16x32 per
+ * 16 hidden channels, wave w on pixels 16w..16w+15), GELU and the 128 -> 2
+ * contraction in registers, the sum over the 16 lanes holding one pixel's
+ * hidden channels by lane shuffles.  Nothing of the hidden layer reaches
+ * memory. */
+__global__ __launch_bounds__(256) void k_proj_fused(const bf16 *__restrict__ v, const bf16 *__restrict__ Q1,
+                                                    const float *__restrict__ c1, const float *__restrict__ Q2,
+                                                    const float *__restrict__ c2, float *__restrict__ out,
+                                                    int64_t P, float s0, float s1) {
+    constexpr int W = 32, LD = W + 8;
+    __shared__ __attribute__((aligned(16))) bf16 As[64 * LD];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t b = blockIdx.y, p0 = (int64_t)blockIdx.x * 64;
+    const bf16 *vb = v + b * W * P;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int e = t * 8 + j, c = e >> 6, px = e & 63;
+        As[px * LD + c] = (p0 + px < P) ? vb[(int64_t)c * P + p0 + px] : (bf16)0.0f;
+    }
+    __syncthreads();
+    const bf16x8 a = *(const bf16x8 *)&As[(wv * 16 + (lane & 15)) * LD + (lane >> 4) * 8];
+    float o0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, o1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int jb = 0; jb < 8; jb++) {
+        const int j = jb * 16 + (lane & 15);
+        const bf16x8 bq = *(const bf16x8 *)&Q1[j * W + (lane >> 4) * 8];
+        const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        const float cj = c1[j], q0 = Q2[j], q1 = Q2[128 + j];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float h = gelu(acc[r] + cj);
+            o0[r] += q0 * h;
+            o1[r] += q1 * h;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            o0[r] += __shfl_xor(o0[r], m, 64);
+            o1[r] += __shfl_xor(o1[r], m, 64);
+        }
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t p = p0 + wv * 16 + (lane >> 4) * 4 + r;
+            if (p < P) {
+                out[(b * 2) * P + p] = (o0[r] + c2[0]) * s0;
+                out[(b * 2 + 1) * P + p] = (o1[r] + c2[1]) * s1;
+            }
+        }
     }
 }
 
@@ -290,7 +419,7 @@ struct fno_net {
     std::vector<size_t> oRr, oRi, oW, ob;
     float *par = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
     /* BF16 copies of the weights */
-    bf16 *wP = nullptr, *wQ1 = nullptr, *wQ2 = nullptr;
+    bf16 *wQ1 = nullptr, *wQ2 = nullptr;
     std::vector<bf16 *> wW, wX;
     /* transform tables (BF16) */
     bf16 *Fy = nullptr, *T = nullptr, *Ti = nullptr, *Fi = nullptr;
@@ -335,10 +464,24 @@ int gemm(fno_net *n, const fno_gemm_args &a) {
     const int64_t nz = (int64_t)a.nz1 * a.nz2;
     if (nz > 65535) return ORG_EINVAL;
     dim3 g((a.M + FG_BM - 1) / FG_BM, (a.N + FG_BN - 1) / FG_BN, (unsigned)nz);
-    if (a.a_f32 && a.b_f32) k_fno_gemm<true, true><<<g, dim3(256), 0, n->st>>>(a);
-    else if (a.a_f32) k_fno_gemm<true, false><<<g, dim3(256), 0, n->st>>>(a);
-    else if (a.b_f32) k_fno_gemm<false, true><<<g, dim3(256), 0, n->st>>>(a);
-    else k_fno_gemm<false, false><<<g, dim3(256), 0, n->st>>>(a);
+    const GemmSeg none{};
+    if (a.a_f32 && a.b_f32) k_fno_gemm<true, true, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
+    else if (a.a_f32) k_fno_gemm<true, false, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
+    else if (a.b_f32) k_fno_gemm<false, true, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
+    else k_fno_gemm<false, false, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
+    return launch_ok();
+}
+
+/* A GEMM with a second K segment (FP32 A, BF16 B in the first; BF16 both in
+ * the second) and the bias per row. */
+int gemm2(fno_net *n, const fno_gemm_args &a, const GemmSeg &s2) {
+    if (a.M <= 0 || a.N <= 0 || a.K <= 0 || s2.K <= 0 || a.nz1 <= 0 || a.nz2 <= 0 || !a.A || !a.B || !a.C ||
+        !s2.A || !s2.B || !a.a_f32 || a.b_f32 || a.c_mode == 1)
+        return ORG_EINVAL;
+    const int64_t nz = (int64_t)a.nz1 * a.nz2;
+    if (nz > 65535) return ORG_EINVAL;
+    dim3 g((a.M + FG_BM - 1) / FG_BM, (a.N + FG_BN - 1) / FG_BN, (unsigned)nz);
+    k_fno_gemm<true, false, true><<<g, dim3(256), 0, n->st>>>(a, s2, 1);
     return launch_ok();
 }
 
@@ -384,7 +527,6 @@ int prep_weights(fno_net *n) {
         hipLaunchKernelGGL(k_cvt_bf16, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->par + off, dst, (int64_t)cnt);
         return launch_ok();
     };
-    TRY(cvt(n->oP, (size_t)n->w * 4, n->wP));
     TRY(cvt(n->oQ1, (size_t)n->proj * n->w, n->wQ1));
     TRY(cvt(n->oQ2, (size_t)2 * n->proj, n->wQ2));
     for (int l = 0; l < n->L; l++) {
@@ -465,40 +607,70 @@ int spectral_fwd(fno_net *n, int B, int l) {
     a.B = n->Yh2; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)4 * m * m;
     a.C = n->Zp; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)M * 2 * m;
     a.M = 2 * M; a.N = m; a.K = 4 * m; a.nz1 = B * w;
-    TRY(gemm(n, a));
-    /* inverse y: u[row][y] = sum_(ri,ky) Zp[row][(ri,ky)] Fi[(ri,ky)][y] */
-    a = G0();
-    a.A = n->Zp; a.sAm = 2 * m; a.sAk = 1; a.a_f32 = 1;
-    a.B = n->Fi; a.sBk = M; a.sBn = 1;
-    a.C = n->u[l]; a.sCm = M; a.sCn = 1;
-    a.M = (int)rows; a.N = M; a.K = 2 * m;
     return gemm(n, a);
 }
 
-int forward(fno_net *n, const float *in, int B) {
-    TRY(prep_weights(n));
+/* The rest of layer l, one GEMM per (b, x) row with two K segments:
+ *   u[b][o][x][y] = sum_(ri,ky) Zp[b][o][x][(ri,ky)] Fi[(ri,ky)][y]      (inverse y)
+ *                 + sum_c W[o][c] v[b][c][x][y] + b[o]                  (pointwise)
+ * v[l+1] = GELU(u) (identity after the last layer); train: u kept in FP32. */
+int layer_out(fno_net *n, int B, int l, bool train) {
+    const int w = n->w, m = n->m, M = n->M;
+    const int64_t P = n->P;
+    fno_gemm_args a = G0();
+    a.A = n->Zp; a.sAm = (int64_t)M * 2 * m; a.sAk = 1; a.sA1 = (int64_t)w * M * 2 * m; a.sA2 = 2 * m; a.a_f32 = 1;
+    a.B = n->Fi; a.sBk = M; a.sBn = 1;
+    a.M = w; a.N = M; a.K = 2 * m; a.nz1 = B; a.nz2 = M;
+    a.sCm = P; a.sCn = 1; a.sC1 = (int64_t)w * P; a.sC2 = M;
+    a.bias = n->par + n->ob[l];
+    a.act = (l < n->L - 1);
+    if (train) { a.C = n->u[l]; a.c_mode = 0; a.C2 = n->v[l + 1]; }
+    else { a.C = n->v[l + 1]; a.c_mode = 2; }
+    GemmSeg s2{};
+    s2.A = n->wW[l]; s2.sAm = w; s2.sAk = 1;
+    s2.B = n->v[l]; s2.sBk = P; s2.sBn = 1; s2.sB1 = (int64_t)w * P; s2.sB2 = M;
+    s2.K = w;
+    return gemm2(n, a, s2);
+}
+
+/* Forward pass.  train: keep what backward needs (the BF16 input, the FP32
+ * projection pre-activations) and leave the unscaled output in n->out;
+ * otherwise write the scaled velocity to vout (fused projection when the
+ * network has width 32 and 128 hidden channels). */
+int forward(fno_net *n, const float *in, int B, bool train, float *vout) {
+    if (!n->weights_fresh) {
+        TRY(prep_weights(n));
+        n->weights_fresh = true;
+    }
     const int w = n->w;
-    hipLaunchKernelGGL(k_lift_in, dim3(grid_for((int64_t)B * n->P)), dim3(256), 0, n->st, in, n->xin, B, n->M,
-                       n->in_scale[0], n->in_scale[1]);
+    hipLaunchKernelGGL(k_lift, dim3(grid_for((int64_t)B * n->P)), dim3(256), 0, n->st, in, n->par + n->oP,
+                       n->par + n->obP, n->v[0], train ? n->xin : (bf16 *)nullptr, B, n->M, w, n->in_scale[0],
+                       n->in_scale[1]);
     TRY(launch_ok());
-    /* lift: v0[b][c][p] = sum_i P[c][i] x[b][i][p] + bP[c] */
-    fno_gemm_args a = pointwise(n, B, n->xin, 0, 4, n->wP, 1, 4, n->v[0], w);
-    a.c_mode = 2; a.bias = n->par + n->obP;
-    TRY(gemm(n, a));
+    fno_gemm_args a;
     for (int l = 0; l < n->L; l++) {
         TRY(spectral_fwd(n, B, l));
-        /* u[l] += W_l v[l] + b_l;  v[l+1] = GELU(u[l]) (identity after the last layer) */
-        a = pointwise(n, B, n->v[l], 0, w, n->wW[l], 1, w, n->u[l], w);
-        a.c_mode = 1; a.bias = n->par + n->ob[l]; a.C2 = n->v[l + 1]; a.act = (l < n->L - 1);
-        TRY(gemm(n, a));
+        TRY(layer_out(n, B, l, train));
     }
     /* projection */
+    if (!train && w == 32 && n->proj == 128) {
+        hipLaunchKernelGGL(k_proj_fused, dim3((unsigned)((n->P + 63) / 64), B), dim3(256), 0, n->st, n->v[n->L],
+                           n->wQ1, n->par + n->oc1, n->par + n->oQ2, n->par + n->oc2, vout, n->P, n->out_scale[0],
+                           n->out_scale[1]);
+        return launch_ok();
+    }
     a = pointwise(n, B, n->v[n->L], 0, w, n->wQ1, 1, w, n->qpre, n->proj);
     a.bias = n->par + n->oc1; a.C2 = n->q; a.act = 1;
     TRY(gemm(n, a));
-    a = pointwise(n, B, n->q, 0, n->proj, n->wQ2, 1, n->proj, n->out, 2);
+    a = pointwise(n, B, n->q, 0, n->proj, n->wQ2, 1, n->proj, train ? n->out : vout, 2);
     a.bias = n->par + n->oc2;
-    return gemm(n, a);
+    TRY(gemm(n, a));
+    if (!train) {
+        hipLaunchKernelGGL(k_scale_out, dim3(grid_for((int64_t)B * 2 * n->P)), dim3(256), 0, n->st, vout, B, n->P,
+                           n->out_scale[0], n->out_scale[1]);
+        return launch_ok();
+    }
+    return ORG_OK;
 }
 
 /* Backward of the spectral part of layer l: du (f32 [B][w][P]) -> gradient
@@ -675,7 +847,7 @@ int fno_create(const fno_config *cfg, int device, fno_net **out) {
     int rc = ORG_OK;
     auto A = [&](auto **p, size_t cnt) { if (!rc) rc = dal(n, p, cnt); };
     A(&n->par, n->npar); A(&n->grad, n->npar); A(&n->am, n->npar); A(&n->av, n->npar);
-    A(&n->wP, (size_t)w * 4); A(&n->wQ1, (size_t)n->proj * w); A(&n->wQ2, (size_t)2 * n->proj);
+    A(&n->wQ1, (size_t)n->proj * w); A(&n->wQ2, (size_t)2 * n->proj);
     n->wW.assign(L, nullptr); n->wX.assign(L, nullptr); n->u.assign(L, nullptr); n->xh.assign(L, nullptr);
     n->v.assign(L + 1, nullptr);
     for (int l = 0; l < L; l++) {
@@ -728,6 +900,7 @@ int fno_set_params(fno_net *n, const float *host, size_t cnt) {
     if (!n || !host || cnt != n->npar) return ORG_EINVAL;
     TRY(setdev(n));
     if (hipStreamSynchronize(n->st) != hipSuccess) return ORG_EHIP;
+    n->weights_fresh = false;
     return hipMemcpy(n->par, host, cnt * sizeof(float), hipMemcpyHostToDevice) == hipSuccess ? ORG_OK : ORG_EHIP;
 }
 
@@ -769,19 +942,13 @@ int fno_get_config(const fno_net *n, fno_config *out) {
 int fno_forward(fno_net *n, const float *in, int B, float *out) {
     if (!n || !in || !out || B < 1 || B > n->Bmax) return ORG_EINVAL;
     TRY(setdev(n));
-    TRY(forward(n, in, B));
-    if (hipMemcpyAsync(out, n->out, (size_t)B * 2 * n->P * sizeof(float), hipMemcpyDeviceToDevice, n->st) !=
-        hipSuccess)
-        return ORG_EHIP;
-    hipLaunchKernelGGL(k_scale_out, dim3(grid_for((int64_t)B * 2 * n->P)), dim3(256), 0, n->st, out, B, n->P,
-                       n->out_scale[0], n->out_scale[1]);
-    return launch_ok();
+    return forward(n, in, B, false, out);
 }
 
 int fno_loss_grad(fno_net *n, const float *in, const float *target, int B, double *loss, float *grad_host) {
     if (!n || !in || !target || B < 1 || B > n->Bmax) return ORG_EINVAL;
     TRY(setdev(n));
-    TRY(forward(n, in, B));
+    TRY(forward(n, in, B, true, nullptr));
     hipLaunchKernelGGL(k_loss, dim3(LOSS_BLOCKS), dim3(256), 0, n->st, n->out, target, n->dout, n->lpart, B, n->P,
                        1.0f / n->out_scale[0], 1.0f / n->out_scale[1]);
     TRY(launch_ok());
@@ -807,6 +974,7 @@ int fno_adam_step(fno_net *n, float lr, float b1, float b2, float eps, int t) {
     const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
     hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)n->npar)), dim3(256), 0, n->st, n->par, n->grad, n->am, n->av,
                        (int64_t)n->npar, lr, b1, b2, eps, c1, c2);
+    n->weights_fresh = false;
     return launch_ok();
 }
 

@@ -41,10 +41,17 @@ def _dev(a, dtype):
     return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
 
 
-@pytest.mark.parametrize("case", ["rowmajor", "transposed_f32", "bf16_gelu_out", "two_level_batch"])
+CASES = ["rowmajor", "transposed_f32", "bf16_gelu_out", "two_level_batch", "aligned_vector"]
+
+
+@pytest.mark.parametrize("case", CASES)
 def test_mfma_gemm_matches_numpy(small_net, case):
-    rng = np.random.default_rng(["rowmajor", "transposed_f32", "bf16_gelu_out", "two_level_batch"].index(case))
+    """Ragged shapes take the element-wise staging, aligned ones the 16-byte
+    vector loads (aligned_vector: BF16 k-contiguous A, n-contiguous B)."""
+    rng = np.random.default_rng(CASES.index(case))
     M, N, K, nz1, nz2 = 100, 45, 70, 3, 1
+    if case == "aligned_vector":
+        M, N, K = 96, 72, 80
     if case == "two_level_batch":
         nz1, nz2 = 2, 3
     nz = nz1 * nz2

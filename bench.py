@@ -81,6 +81,29 @@ def surrogate_rates(host, org, nc: int, steps: int) -> dict:
                     "every 10th step; untrained weights (timing only)"}
 
 
+def fno_forward_only(n: int) -> int:
+    """n forward passes of the FNO surrogate on 64 random 128^2 crops (the
+    organoid64_1024 batch), for kernel traces; prints the mean time."""
+    import torch
+
+    from multicellular_phase_field_active_gel_organoids_amd import fno
+    cfg = fno.FnoConfig(batch_max=64)
+    net = fno.FnoNet(cfg)
+    net.set_params(fno.init_params(cfg, 0))
+    x = torch.rand((64, 2, cfg.M, cfg.M), device="cuda")
+    out = torch.empty_like(x)
+    net.forward(x, out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        net.forward(x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    print(json.dumps({"fno_forward_ms": round(e0.elapsed_time(e1) / n, 4), "cells": 64, "passes": n}))
+    net.close()
+    return 0
+
+
 def measured_traffic(path: str = PMC_SUMMARY):
     """HBM bytes per k_step launch from the committed rocprofv3 --pmc summary
     (line 'traffic_bytes_per_launch: <int>'), or None."""
@@ -168,11 +191,15 @@ def main() -> int:
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")
     ap.add_argument("--surrogate-steps", type=int, default=40,
                     help="steps timed with the FNO velocity surrogate (0: skip)")
+    ap.add_argument("--fno-forward", type=int, default=0, metavar="N",
+                    help="only run N forward passes of the FNO surrogate on 64 random crops (profiling)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.calib_copy:
         return calib_copy(args.calib_copy)
+    if args.fno_forward:
+        return fno_forward_only(args.fno_forward)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), --gpus must be 1"}))

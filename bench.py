@@ -189,6 +189,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=160,
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape instead of autotuning it first")
     ap.add_argument("--surrogate-steps", type=int, default=40,
                     help="steps timed with the FNO velocity surrogate (0: skip)")
     ap.add_argument("--fno-forward", type=int, default=0, metavar="N",
@@ -219,9 +221,21 @@ def main() -> int:
     org.load_workload(wl)
     org.set_option(host.OPT_WRITE_VELOCITY, 0)
 
-    # warmup (also builds the graph), then the timed region
+    # warmup (also builds the graph), the launch-shape autotuner (results do
+    # not depend on it), warmup of the tuned graph, then the timed region
     org.step(max(args.warmup, 1), graph=True)
     org.sync()
+    tuning = {"tuning": org.get_tuning(), "source": "default"}
+    if not args.no_tune:
+        from multicellular_phase_field_active_gel_organoids_amd import autotune
+        r = autotune.tuned(org)
+        tuning = {"tuning": r["tuning"], "source": r["source"]}
+        if r["source"] == "tuned":
+            tuning.update(default=r["default"], default_ms_per_step=round(r["default_ms"], 5),
+                          tuned_ms_per_step=round(r["best_ms"], 5), candidates_timed=len(r["measured"]))
+        print(f"bench: tuning {tuning}", file=sys.stderr, flush=True)
+        org.step(max(args.warmup, 1), graph=True)
+        org.sync()
     t0 = time.perf_counter()
     org.step(args.steps, graph=True)
     org.sync()
@@ -297,12 +311,12 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl.name, "cells": nc, "grid": p.N, "zone_size": p.zone_size,
                    "form": "batched: k_step + k_mid per step for all cells (k_pre / k_post at the ends of a "
-                           "replay), hipGraphs of 8 steps (FNO surrogate not used)",
+                           f"replay), hipGraphs of {tuning['tuning']['graph_steps']} steps (FNO surrogate not used)",
                    "plain_dropin_cell_steps_per_s": round(plain_rate, 2) if plain_rate else None,
                    "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
                    "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it},
-                   "sweep8": sweep, "surrogate": surrogate},
+                   "launch_tuning": tuning, "sweep8": sweep, "surrogate": surrogate},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

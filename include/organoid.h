@@ -192,6 +192,25 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
  * and replays it.                                                            */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
 
+/* ---- launch tuning of the batched step ---------------------------------------
+ * No counterpart in the reference (its loop bounds are fixed by N_s and the
+ * OpenACC defaults).  lum_blocks: lumen/ECM workgroups of the step's main
+ * launch (1..min(128, compute units); they wait on each other, so all must
+ * fit on the device at once); roles: workgroups of each non-cell role of the
+ * launches before and between steps (psi tiles, h(lum)/h(ECM) parts,
+ * lumen/ECM copy-back; 1..512); graph_steps: steps per replayed hipGraph
+ * (1..64).  The defaults are sized from the device's compute-unit count;
+ * autotune.py picks them by timing.  None of them changes results: the
+ * partitions feed only pointwise updates, per-tile sums in a fixed order and
+ * box unions.  Setting a tuning drops the captured graphs.                    */
+typedef struct org_tuning {
+    int32_t lum_blocks;
+    int32_t roles;
+    int32_t graph_steps;
+} org_tuning;
+int org_get_tuning(org_ctx *ctx, org_tuning *out);
+int org_set_tuning(org_ctx *ctx, const org_tuning *tuning);
+
 /* ---- the velocity surrogate in the batched step ------------------------------
  * No counterpart in the reference (its velocity always comes from the
  * spectral solve inside update_all_cortices, rho_functions.cpp:134-190);

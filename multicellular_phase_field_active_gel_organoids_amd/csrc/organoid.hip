@@ -32,6 +32,8 @@ int note_error(int code) {
 /* steps per replay of the batched step's graph (the launch of a graph costs
  * more than the gaps between its kernels) */
 #define ORG_GRAPH_STEPS 8
+/* largest workgroup count of a non-cell role of k_pre / k_mid / k_post */
+#define ORG_ROLE_MAX 512
 
 }  // namespace
 
@@ -67,7 +69,9 @@ struct org_ctx {
     int nh = 0;                            /* h(lum), h(ECM) workgroups of k_pre */
     int nlp = 0;                           /* lumen/ECM copy-back workgroups of k_mid and k_post */
     int nlp_cap = 0;                       /* partial boxes allocated per field (nlp never exceeds it) */
-    int nlum = ORG_LUM_BLOCKS;             /* lumen/ECM workgroups of k_step in a batch */
+    int nlum = ORG_LUM_BLOCKS;             /* lumen/ECM workgroups of k_step (tuning, or the batch's share) */
+    int graph_steps = ORG_GRAPH_STEPS;     /* steps per replayed graph (tuning) */
+    int ncu = 256;                         /* compute units of the device */
     bool in_batch = false;                 /* an org_batch holds this context */
     int nparts_h = 0;                      /* parts of the sum of h(ECM): 64 x 64 tiles + the ring */
     int *live = nullptr, *live_part = nullptr;  /* live boxes of lum and ECM, k_post's partial boxes */
@@ -322,7 +326,7 @@ int L_pre(org_ctx *c) {
 /* the rest of the step up to the copy-back, one launch (see k_step) */
 int L_step(org_ctx *c) {
     Prof pr(c, ORG_K_STEP);
-    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, step_args(c));
+    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + c->nlum), dim3(1024), 0, c->stream, c->dp, step_args(c));
     return launch_check();
 }
 /* copy-back of one step and the first part of the next (see k_mid) */
@@ -332,7 +336,7 @@ int L_step_v(org_ctx *c, const float *vg, const int *use) {
     StepArgs a = step_args(c);
     a.vgiven = vg;
     a.vuse = use;
-    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + ORG_LUM_BLOCKS), dim3(1024), 0, c->stream, c->dp, a);
+    hipLaunchKernelGGL(k_step, dim3(2 * c->nc + c->nlum), dim3(1024), 0, c->stream, c->dp, a);
     return launch_check();
 }
 int L_mid(org_ctx *c) {
@@ -489,10 +493,11 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
             ncu = 256;
         const int role = std::max(32, std::max(64, ncu - c->nc) / 2);
+        c->ncu = ncu;
         c->npsi = role;
         c->nh = std::min(c->nparts_h, role);
         c->nlp = role;
-        c->nlp_cap = role;
+        c->nlp_cap = std::max(role, ORG_ROLE_MAX);
     }
     int nt = (c->N + 15) / 16;
     c->ntiles = nt * nt;
@@ -509,12 +514,12 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_BLOCKS); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_MAX); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
     A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);
     A(&c->xp, c->nc); A(&c->yp, c->nc);
-    A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->nlp); A(&c->counts, c->nc); A(&c->err, 1);
+    A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->nlp_cap); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, (size_t)c->nparts_h); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
@@ -677,6 +682,35 @@ int org_prepare(org_ctx *c) {
     return ORG_OK;
 }
 
+int org_get_tuning(org_ctx *c, org_tuning *t) {
+    TRY(check_ctx(c));
+    if (!t) return note_error(ORG_EINVAL);
+    t->lum_blocks = c->nlum;
+    t->roles = c->npsi;
+    t->graph_steps = c->graph_steps;
+    return ORG_OK;
+}
+
+int org_set_tuning(org_ctx *c, const org_tuning *t) {
+    TRY(check_ctx(c));
+    if (!t || c->in_batch) return note_error(ORG_EINVAL);
+    if (t->lum_blocks < 1 || t->lum_blocks > std::min(ORG_LUM_MAX, c->ncu) || t->roles < 1 ||
+        t->roles > c->nlp_cap || t->graph_steps < 1 || t->graph_steps > 64)
+        return note_error(ORG_EINVAL);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nlum = t->lum_blocks;
+    c->npsi = t->roles;
+    c->nh = std::min(c->nparts_h, t->roles);
+    if (c->nlp != t->roles) c->next_ready = false;   /* partial live boxes are re-initialised for the new nlp */
+    c->nlp = t->roles;
+    c->graph_steps = t->graph_steps;
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+        if (c->graph[k]) { (void)hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }
+    }
+    return ORG_OK;
+}
+
 int org_set_option(org_ctx *c, int option, int value) {
     TRY(check_ctx(c));
     switch (option) {
@@ -793,10 +827,10 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
         TRY(enqueue_steps(c, nsteps));
         return harvest_device_error(c);
     }
-    /* graphs of ORG_GRAPH_STEPS steps and of one step (the remainder) */
+    /* graphs of graph_steps steps and of one step (the remainder) */
     for (int k = 0; k < 2; k++) {
         if (c->gexec[k]) continue;
-        const int nst = k ? 1 : ORG_GRAPH_STEPS;
+        const int nst = k ? 1 : c->graph_steps;
         bool prof = c->prof;
         c->prof = false;                 /* no event records inside the graph */
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -812,7 +846,7 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
         HIPCHK(hipGraphInstantiate(&c->gexec[k], c->graph[k], nullptr, nullptr, 0));
     }
     int s = 0;
-    for (; s + ORG_GRAPH_STEPS <= nsteps; s += ORG_GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
+    for (; s + c->graph_steps <= nsteps; s += c->graph_steps) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
     for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec[1], c->stream));
     return harvest_device_error(c);
 }

@@ -1296,6 +1296,7 @@ static __device__ __forceinline__ float reshape_flux_j(float un, const float *u,
  * the deviation sum |X_{k+1}-X_k| dx^2.  The same decomposition is used by
  * the stand-alone and the fused loop so their sums agree bit for bit. */
 #define ORG_LUM_BLOCKS 64
+#define ORG_LUM_MAX 128      /* lumen/ECM workgroups of k_step a tuning may ask for */
 static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *__restrict__ src,
                                                    float *__restrict__ dst, const float *__restrict__ jx,
                                                    const float *__restrict__ jy, const int4 jb,
@@ -2232,7 +2233,7 @@ struct StepArgs {
     int npsi;                /* psi workgroups of k_pre */
     int *live, *live_part;   /* live boxes of lum and ECM; per-workgroup boxes of live pixels from k_post */
     int nlp;                 /* k_post's lumen/ECM workgroups (partial boxes per field) */
-    int nlum;                /* lumen/ECM workgroups of k_step in a batch (ORG_LUM_BLOCKS alone) */
+    int nlum;                /* lumen/ECM workgroups of k_step (at most ORG_LUM_MAX) */
     /* lumen / ECM loops */
     float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
     int *jbox_part;
@@ -2447,8 +2448,8 @@ __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
     const int nc = P.nc, b = blockIdx.x;
     if (b < nc) step_spectral(P, a, b, lds);
-    else if (b < nc + ORG_LUM_BLOCKS) step_lumecm(P, a, b - nc, ORG_LUM_BLOCKS, lds);
-    else step_cells(P, a, b - nc - ORG_LUM_BLOCKS, lds);
+    else if (b < nc + a.nlum) step_lumecm(P, a, b - nc, a.nlum, lds);
+    else step_cells(P, a, b - nc - a.nlum, lds);
 }
 
 /* rho <- rhon on the FFT box and phi's new window (copy_cell_tables_1_by_1,

@@ -83,6 +83,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_batch_step.argtypes = [vp, i32, i32]
     lib.org_batch_sync.argtypes = [vp]
     lib.org_batch_destroy.argtypes = [vp]
+    lib.org_get_tuning.argtypes = [vp, vp]
+    lib.org_set_tuning.argtypes = [vp, vp]
     lib.org_surrogate_samples.argtypes = [vp, i32, i32, vp, vp, vp]
     lib.org_step_surrogate.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, vp, vp]
     lib.org_error_string.restype = ctypes.c_char_p
@@ -99,6 +101,14 @@ def _p(a: np.ndarray):
 def _chk(rc: int, what: str) -> None:
     if rc:
         raise OrganoidError(rc, what)
+
+
+class Tuning(ctypes.Structure):
+    """org_tuning (include/organoid.h): launch shape of the batched step."""
+    _fields_ = [("lum_blocks", ctypes.c_int32), ("roles", ctypes.c_int32), ("graph_steps", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"lum_blocks": self.lum_blocks, "roles": self.roles, "graph_steps": self.graph_steps}
 
 
 class Organoid:
@@ -185,6 +195,15 @@ class Organoid:
         _chk(self.lib.org_get_iterations(self.h, _p(cells), ctypes.byref(lum), ctypes.byref(ecm)),
              "org_get_iterations")
         return cells, lum.value, ecm.value
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _chk(self.lib.org_get_tuning(self.h, ctypes.byref(t)), "org_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, lum_blocks: int, roles: int, graph_steps: int) -> None:
+        t = Tuning(lum_blocks, roles, graph_steps)
+        _chk(self.lib.org_set_tuning(self.h, ctypes.byref(t)), "org_set_tuning")
 
     def surrogate_samples(self, nsteps: int, it0: int = 0, residuals: bool = False):
         """Run nsteps exact batched steps and return, per step and cell, the

@@ -1,0 +1,70 @@
+"""Launch tuning of the batched step (org_set_tuning, autotune.py) on the
+GPU: every tuning gives the default tuning's trajectory bit for bit (also
+when it changes between steps), out-of-range tunings are refused, and the
+autotuner leaves the organoid's state as it found it and caches its pick."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from multicellular_phase_field_active_gel_organoids_amd import autotune, host  # noqa: E402
+from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload  # noqa: E402
+
+FIELDS = (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM)
+
+
+def _org(wl):
+    g = host.Organoid(wl.params, wl.s0_cell)
+    g.load_workload(wl)
+    return g
+
+
+def test_tunings_give_the_same_trajectory():
+    wl = make_workload("organoid16_512")
+    print("case organoid16_512: 24 steps under three tunings", flush=True)
+    ref = _org(wl)
+    ref.step(24, graph=True)
+    for t in ({"lum_blocks": 24, "roles": 40, "graph_steps": 3}, {"lum_blocks": 128, "roles": 256, "graph_steps": 16}):
+        g = _org(wl)
+        g.set_tuning(**t)
+        assert g.get_tuning() == t
+        g.step(24, graph=True)
+        for f in FIELDS:
+            assert np.array_equal(g.get_field(f), ref.get_field(f)), (t, f)
+    g = _org(wl)                       # tuning changed between steps (new copy-back partition)
+    g.step(8, graph=True)
+    g.set_tuning(lum_blocks=40, roles=72, graph_steps=5)
+    g.step(16, graph=True)
+    for f in FIELDS:
+        assert np.array_equal(g.get_field(f), ref.get_field(f)), f
+
+
+def test_out_of_range_tunings_are_refused():
+    wl = make_workload("two_cells_256")
+    g = _org(wl)
+    t0 = g.get_tuning()
+    for bad in ({"lum_blocks": 0}, {"lum_blocks": 129}, {"roles": 0}, {"roles": 513}, {"graph_steps": 0},
+                {"graph_steps": 65}):
+        with pytest.raises(host.OrganoidError):
+            g.set_tuning(**dict(t0, **bad))
+    assert g.get_tuning() == t0
+
+
+def test_autotune_restores_state_and_caches(tmp_path):
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: autotune, one pass of 8-step timings", flush=True)
+    g = _org(wl)
+    g.step(4)
+    before = {f: g.get_field(f) for f in FIELDS}
+    path = str(tmp_path / "tune.json")
+    r = autotune.tuned(g, cache_path=path, steps=8, warmup=2, passes=1)
+    assert r["source"] == "tuned" and os.path.exists(path)
+    assert g.get_tuning() == r["tuning"]
+    for f in FIELDS:
+        assert np.array_equal(g.get_field(f), before[f]), f
+    assert len(r["measured"]) >= 3 and r["best_ms"] <= r["default_ms"]
+    g2 = _org(wl)
+    r2 = autotune.tuned(g2, cache_path=path)
+    assert r2["source"] == "cache" and r2["tuning"] == r["tuning"] == g2.get_tuning()

@@ -81,6 +81,39 @@ def surrogate_rates(host, org, nc: int, steps: int) -> dict:
                     "every 10th step; untrained weights (timing only)"}
 
 
+def large_rates(host, make_workload, steps: int) -> dict:
+    """organoid256_2048 (256 cells, 2048^2) in the batched form (8-step
+    graphs, default launch shape) and in the plain per-function form."""
+    wl = make_workload("organoid256_2048")
+    p = wl.params
+    nc = p.ncell
+    org = host.Organoid(p, wl.s0_cell)
+    org.load_workload(wl)
+    org.set_option(host.OPT_WRITE_VELOCITY, 0)
+    org.step(24, graph=True)
+    org.sync()
+    t0 = time.perf_counter()
+    org.step(steps, graph=True)
+    org.sync()
+    dt = time.perf_counter() - t0
+    org.close()
+    plain = host.Organoid(p, wl.s0_cell)
+    plain.load_workload(wl)
+    plain.plain_step()
+    plain.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        plain.plain_step()
+    plain.sync()
+    pdt = time.perf_counter() - t0
+    plain.close()
+    rate, prate = nc * steps / dt, nc * 3 / pdt
+    print(f"bench: organoid256_2048 {1e3 * dt / steps:.4f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": wl.name, "cells": nc, "grid": p.N, "steps": steps, "ms_per_step": round(1e3 * dt / steps, 4),
+            "cell_steps_per_s": round(rate, 2), "plain_dropin_cell_steps_per_s": round(prate, 2),
+            "speedup_vs_plain": round(rate / prate, 3)}
+
+
 def fno_forward_only(n: int) -> int:
     """n forward passes of the FNO surrogate on 64 random 128^2 crops (the
     organoid64_1024 batch), for kernel traces; prints the mean time."""
@@ -189,6 +222,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=160,
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=100,
+                    help="steps of the 256-cell 2048^2 workload timed beside the main one (0: skip)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of autotuning it first")
     ap.add_argument("--surrogate-steps", type=int, default=40,
@@ -285,6 +320,10 @@ def main() -> int:
     if args.sweep_steps > 0:
         sweep = sweep_rates(host, args.sweep_steps)
 
+    large = None
+    if args.large_steps > 0:
+        large = large_rates(host, make_workload, args.large_steps)
+
     surrogate = None
     if args.surrogate_steps > 0:
         surrogate = surrogate_rates(host, org, nc, args.surrogate_steps)
@@ -316,7 +355,7 @@ def main() -> int:
                    "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
                    "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it},
-                   "launch_tuning": tuning, "sweep8": sweep, "surrogate": surrogate},
+                   "launch_tuning": tuning, "sweep8": sweep, "large": large, "surrogate": surrogate},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -16,17 +16,25 @@ restatement of the reference's time loop, main_code/run_with_active_gels.cpp:
 It then runs the same steps with the oracle built in FP32 (the
 reference's own float arithmetic, with its double-literal promotions) and
 stores that build's drift against the FP64 fields (``f32_<field>_s<step>``
-relative L2 over all cells x N^2, ``f32_vx`` / ``f32_vy``) and the number of
+relative L2 over all cells x N^2, ``f32_vx`` / ``f32_vy`` with each run's
+boxes placed on the grid at its own centres) and the number of
 steps whose centres (``f32_centre_mismatch``) or cell / lumen / ECM
 iteration counts (``f32_cells_mismatch``, ``f32_lum_mismatch``,
 ``f32_ecm_mismatch``) differ from the FP64 run's: the drift that FP32
 arithmetic alone produces, which the GPU's drift is compared with.
 
-Usage:  python tests/golden/make_drift_fixtures.py [--f32-only] [case ...]
+Finally an ensemble of FP32 runs whose initial phi differs from the
+workload's by one ulp (float32) at 10 interface pixels (seeds 1..k) measures
+how far FP32 runs that differ only in rounding separate from the FP64 one:
+``f32e_<field>_s<step>``, ``f32e_vx``, ``f32e_vy`` and ``f32e_<k>_mismatch``
+are the largest values over the ensemble and the unperturbed FP32 run.
+
+Usage:  python tests/golden/make_drift_fixtures.py [--f32-only | --ensemble] [case ...]
 (cases: two_cells_256, organoid16_512; all by default, run in parallel;
---f32-only adds the FP32 leg to existing fixtures).  Runtime on one core:
-about 5 + 5 min (two_cells_256, 10^4 steps) and 18 + 25 min
-(organoid16_512, 5000 steps).
+--f32-only adds the FP32 leg, --ensemble the ensemble, to existing
+fixtures).  Runtime on one core: about 5 + 5 + 4 x 5 min (two_cells_256,
+10^4 steps) and 18 + 25 + 3 x 20 min (organoid16_512, 5000 steps), the
+ensemble members in parallel.
 """
 from __future__ import annotations
 
@@ -81,6 +89,29 @@ def reconstruct(fx, key: str, init: np.ndarray) -> np.ndarray:
     return a
 
 
+def on_grid(win: np.ndarray, centres, N: int, zs: int) -> np.ndarray:
+    """Per-cell FFT-box windows placed on the N x N grid at their boxes'
+    origins (centre - zone_size + 1, useful_functions.cpp:605-625), zero
+    elsewhere, so runs whose integer centres differ are compared where their
+    boxes lie."""
+    nc, M = win.shape[0], win.shape[1]
+    out = np.zeros((nc, N, N))
+    for n in range(nc):
+        i0, j0 = int(centres[n][0]) - zs + 1, int(centres[n][1]) - zs + 1
+        out[n, i0:i0 + M, j0:j0 + M] = win[n]
+    return out
+
+
+def velocity_drift(o, fx, wl, last: int) -> dict:
+    """Relative L2 of the oracle's last-step (vx, vy) against the fixture's,
+    both on the grid at their own centres."""
+    xc, yc = o.centres()
+    mine = np.stack([xc, yc], axis=1)
+    N, zs = wl.params.N, wl.params.zone_size
+    return {k: rel_l2(on_grid(o.get(k), mine, N, zs), on_grid(fx[k].astype(np.float64), fx["centres"][last], N, zs))
+            for k in ("vx", "vy")}
+
+
 def rel_l2(a, b) -> float:
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / (nb if nb > 0 else 1.0))
@@ -112,11 +143,79 @@ def run_f32(case: str) -> str:
                 fx[f"f32_{name}_s{s}"] = np.float64(rel_l2(o.get(name), reconstruct(fx, f"s{s}_{name}",
                                                                                    getattr(wl, name))))
             if "vx" in fx and s == max(checkpoints):
-                fx["f32_vx"] = np.float64(rel_l2(o.get("vx"), fx["vx"].astype(np.float64)))
-                fx["f32_vy"] = np.float64(rel_l2(o.get("vy"), fx["vy"].astype(np.float64)))
+                for k, v in velocity_drift(o, fx, wl, it).items():
+                    fx[f"f32_{k}"] = np.float64(v)
             print(f"{case}: FP32 leg step {s} ({time.time() - t0:.0f} s)", flush=True)
     for k, v in bad.items():
         fx[f"f32_{k}_mismatch"] = np.int32(v)
+    np.savez_compressed(fixture_path(case), **fx)
+    return fixture_path(case)
+
+
+# ensemble members per case (seeds 1..k)
+ENSEMBLE = {"two_cells_256": 4, "organoid16_512": 3}
+
+
+def perturbed_workload(case: str, seed: int):
+    """The case's workload with phi moved by one float32 ulp (up or down) at
+    10 pixels of its interface band 0.05 < phi < 0.95."""
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+    wl = make_workload(case)
+    rng = np.random.default_rng(seed)
+    phi = np.asarray(wl.phi, np.float32)
+    idx = np.argwhere((phi > 0.05) & (phi < 0.95))
+    for p in idx[rng.choice(len(idx), 10, replace=False)]:
+        t = tuple(p)
+        phi[t] = np.nextafter(phi[t], np.float32(2.0) if rng.random() < 0.5 else np.float32(-1.0))
+    wl.phi = phi.astype(np.float64)
+    return wl
+
+
+def run_member(job) -> dict:
+    """One ensemble member: drift and mismatch counts of the FP32 oracle from
+    the perturbed initial state against the FP64 fixture."""
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+    from oracle.oracle import Oracle
+    case, seed = job
+    fx = dict(np.load(fixture_path(case)))
+    wl0 = make_workload(case)
+    wl = perturbed_workload(case, seed)
+    o = Oracle(wl.params, wl.s0_cell, precision="f32")
+    o.load_workload(wl)
+    checkpoints = [int(c) for c in fx["checkpoints"]]
+    res = dict(centre=0, cells=0, lum=0, ecm=0)
+    out = {}
+    for it in range(max(checkpoints)):
+        o.step(it)
+        xc, yc = o.centres()
+        res["centre"] += int((xc != fx["centres"][it, :, 0]).any() or (yc != fx["centres"][it, :, 1]).any())
+        c, lum, ecm = o.iterations()
+        res["cells"] += int((c != fx["it_cells"][it]).any())
+        res["lum"] += int(lum != fx["it_lum"][it])
+        res["ecm"] += int(ecm != fx["it_ecm"][it])
+        if it + 1 in checkpoints:
+            s = it + 1
+            for name in ("phi", "rho", "lum", "ecm"):
+                out[f"{name}_s{s}"] = rel_l2(o.get(name), reconstruct(fx, f"s{s}_{name}", getattr(wl0, name)))
+            if "vx" in fx and s == max(checkpoints):
+                out.update(velocity_drift(o, fx, wl0, it))
+    o.close()
+    for k, v in res.items():
+        out[f"{k}_mismatch"] = v
+    print(f"{case} member {seed}: " + ", ".join(f"{k} {v:.3e}" if isinstance(v, float) else f"{k} {v}"
+                                               for k, v in out.items()), flush=True)
+    return {"case": case, **out}
+
+
+def add_ensemble(case: str, members: list) -> str:
+    """Store the ensemble maxima (with the unperturbed FP32 run's values)."""
+    fx = dict(np.load(fixture_path(case)))
+    for key in members[0]:
+        if key == "case":
+            continue
+        f32 = fx[f"f32_{key}"]
+        vals = [m[key] for m in members] + [f32.item()]
+        fx[f"f32e_{key}"] = np.asarray(max(vals), f32.dtype)
     np.savez_compressed(fixture_path(case), **fx)
     return fixture_path(case)
 
@@ -165,8 +264,16 @@ def run_case(case: str) -> str:
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    f32_only = "--f32-only" in args
-    cases = [a for a in args if a != "--f32-only"] or list(CASES)
-    with mp.Pool(len(cases)) as pool:
-        for p in pool.imap_unordered(run_f32 if f32_only else run_both, cases):
+    f32_only, ens_only = "--f32-only" in args, "--ensemble" in args
+    cases = [a for a in args if not a.startswith("--")] or list(CASES)
+    if not ens_only:
+        with mp.Pool(len(cases)) as pool:
+            for p in pool.imap_unordered(run_f32 if f32_only else run_both, cases):
+                print("wrote", p, os.path.getsize(p), "bytes", flush=True)
+    if ens_only or not f32_only:
+        jobs = [(c, k) for c in cases for k in range(1, ENSEMBLE[c] + 1)]
+        with mp.Pool(len(jobs)) as pool:
+            members = pool.map(run_member, jobs)
+        for c in cases:
+            p = add_ensemble(c, [m for m in members if m["case"] == c])
             print("wrote", p, os.path.getsize(p), "bytes", flush=True)

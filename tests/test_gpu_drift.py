@@ -13,15 +13,16 @@ integer centres or iteration counts differ.
 
 Tolerances: phi relative L2 <= 1e-3 at every checkpoint (up to 10^4 steps),
 the bound the port is specified to.  Every field, the velocity and the
-mismatch counts: at most five times what the FP32 build of the oracle shows
-against the FP64 one (plus 1e-6, and 1 % of the compared steps for the
-counts).  FP32 trajectories separate from the FP64 one on their own (the
-reshaping stop tests and the integer centroids are threshold decisions, and
-each flip moves the trajectory), and how far they separate varies between
-FP32 implementations that differ only in rounding order: re-associating the
-lumen/ECM terms of the cell update's bracket alone took the two-cell phi
-drift at 10^4 steps from 3.7e-4 to 7.6e-4.  The factor 5 covers that spread
-and still catches a wrong term, which drifts by orders of magnitude more.
+mismatch counts: at most five times what FP32 arithmetic itself produces
+(plus 1e-6, and 1 % of the compared steps for the counts), taken as the
+largest value over the FP32 build of the oracle and an ensemble of FP32 runs
+whose initial phi differs by one ulp at 10 pixels (``f32e_*``).  FP32
+trajectories separate from the FP64 one on their own (the reshaping stop
+tests and the integer centroids are threshold decisions, and each flip moves
+the trajectory), and how far varies widely with rounding alone: over that
+ensemble the two-cell phi drift at 10^4 steps spans 3.3e-5 .. 8.8e-4, the
+16-cell one at 5000 steps 2.5e-5 .. 3.3e-5 (the unperturbed FP32 run gives
+9.6e-6).  A wrong term drifts by orders of magnitude more.
 """
 import os
 
@@ -45,6 +46,13 @@ def reconstruct(fx, key, init):
     for n, (i0, i1, j0, j1) in enumerate(fx[key + "_box"]):
         a3[n, i0:i1, j0:j1] = fx[f"{key}_{n}"]
     return a
+
+
+def f32_ref(fx, key):
+    """FP32 arithmetic's own drift for key: the unperturbed FP32 run's, or
+    the ensemble maximum where the fixture holds it."""
+    v = float(fx[f"f32_{key}"])
+    return max(v, float(fx[f"f32e_{key}"])) if f"f32e_{key}" in fx else v
 
 
 def rel_l2(a, b):
@@ -100,7 +108,7 @@ def test_long_trajectory_drift(case, chunk):
                 want = reconstruct(fx, f"s{s}_{name}", getattr(wl, name))
                 got = g.get_field(f)
                 r, rc = rel_l2(got, want), per_cell_max(got, want)
-                ref = float(fx[f"f32_{name}_s{s}"])
+                ref = f32_ref(fx, f"{name}_s{s}")
                 print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e}  max per cell {rc:.3e}  (FP32 oracle {ref:.3e})",
                       flush=True)
                 assert r <= 5.0 * ref + 1e-6, (s, name, r, ref)
@@ -113,13 +121,13 @@ def test_long_trajectory_drift(case, chunk):
                     got = on_grid(g.get_window(w), cg, wl.params.N, wl.params.zone_size)
                     want = on_grid(fx[name].astype(np.float64), co, wl.params.N, wl.params.zone_size)
                     r = rel_l2(got, want)
-                    ref = float(fx[f"f32_{name}"])
+                    ref = f32_ref(fx, name)
                     print(f"  step {s:6d} {name:3s}: rel L2 {r:.3e} (last step, on the grid; centres "
                           f"{'equal' if (cg == co).all() else 'differ'}; FP32 oracle {ref:.3e})", flush=True)
                     assert r <= 5.0 * ref + 1e-6, (s, name, r, ref)
     print(f"  steps compared {compared}: centre mismatches {bad['centre']}, cell iteration mismatches "
           f"{bad['cells']}, lumen {bad['lum']}, ECM {bad['ecm']}", flush=True)
     for k, v in bad.items():
-        ref = int(fx[f"f32_{k}_mismatch"])
+        ref = int(f32_ref(fx, f"{k}_mismatch"))
         print(f"    {k}: FP32 oracle {ref} of {nsteps} steps", flush=True)
         assert v <= 5.0 * ref * compared / nsteps + 0.01 * compared, (k, v, ref)

@@ -54,10 +54,11 @@ def surrogate_rates(host, org, nc: int, steps: int) -> dict:
     time does not depend on them."""
     import torch
 
-    from multicellular_phase_field_active_gel_organoids_amd import fno
+    from multicellular_phase_field_active_gel_organoids_amd import autotune, fno
     cfg = fno.FnoConfig(batch_max=nc)
     net = fno.FnoNet(cfg)
     net.set_params(fno.init_params(cfg, 0))
+    tiles = autotune.tuned_fno(net, nc, reps=3, passes=1, train=False)
     org.step_surrogate(10, net=net, k_exact=10, threshold=1e30, it0=1)
     org.sync()
     t0 = time.perf_counter()
@@ -76,7 +77,8 @@ def surrogate_rates(host, org, nc: int, steps: int) -> dict:
     fwd = e0.elapsed_time(e1) / 10
     net.close()
     return {"cell_steps_per_s": round(nc * steps / dt, 2), "ms_per_step": round(1e3 * dt / steps, 4),
-            "fno_forward_ms": round(fwd, 4), "exact_cell_solves": int(exact.sum()), "cell_steps": nc * steps,
+            "fno_forward_ms": round(fwd, 4), "fno_tiles": tiles["tiles"], "fno_tiles_source": tiles["source"],
+            "exact_cell_solves": int(exact.sum()), "cell_steps": nc * steps,
             "form": "eager steps; FNO (4 layers, width 32, 12 modes, BF16 MFMA) for the velocity, exact solve "
                     "every 10th step; untrained weights (timing only)"}
 

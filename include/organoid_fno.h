@@ -93,8 +93,32 @@ int fno_loss_grad(fno_net *net, const float *in, const float *target, int batch,
  * step t >= 1 enters the bias corrections. */
 int fno_adam_step(fno_net *net, float lr, float beta1, float beta2, float eps, int t);
 
+/* GEMM tile shapes (rows x columns of C per 256-thread workgroup) and the
+ * stages of the network whose GEMMs share one tile choice.  The default is
+ * 64 x 64 everywhere; autotune.py times the variants per stage and sets
+ * them.  The tile changes no result: every variant accumulates the same
+ * products in the same K order. */
+enum {
+    FNO_TILE_64x64 = 0, FNO_TILE_32x64 = 1, FNO_TILE_64x32 = 2, FNO_TILE_32x32 = 3, FNO_TILE_COUNT = 4
+};
+enum {
+    FNO_STAGE_Y = 0,             /* forward transform along y                        */
+    FNO_STAGE_X = 1,             /* forward transform along x                        */
+    FNO_STAGE_MIX = 2,           /* per-mode channel mix                             */
+    FNO_STAGE_IX = 3,            /* inverse transform along x                        */
+    FNO_STAGE_LAYER = 4,         /* inverse y transform + pointwise mix (2 segments) */
+    FNO_STAGE_PROJ = 5,          /* projection (training form)                       */
+    FNO_STAGE_BWD_SPECTRAL = 6,  /* backward of the spectral convolution             */
+    FNO_STAGE_BWD_POINTWISE = 7, /* backward of the pointwise products               */
+    FNO_STAGE_WGRAD = 8,         /* weight gradients (K-chunked partials)            */
+    FNO_STAGE_API = 9,           /* fno_gemm                                         */
+    FNO_STAGE_COUNT = 10
+};
+int fno_set_tiles(fno_net *net, const int32_t tiles[FNO_STAGE_COUNT]);
+int fno_get_tiles(const fno_net *net, int32_t tiles[FNO_STAGE_COUNT]);
+
 /* The batched BF16 matrix-core GEMM every layer is built from, exposed for
- * its unit tests:
+ * its unit tests (tile: the FNO_STAGE_API entry):
  *   C[z][m][n] = epilogue(alpha * sum_k A[z][m][k] B[z][k][n])
  * with z = z1 * nz2 + z2 and element offsets
  *   A: z1*sA1 + z2*sA2 + m*sAm + k*sAk   (FP32 if a_f32, else BF16)

@@ -1,6 +1,8 @@
-"""Autotuner of the batched step's launch shape (org_tuning, include/organoid.h).
+"""Autotuner of the batched step's launch shape (org_tuning,
+include/organoid.h) and of the velocity surrogate's GEMM tiles
+(include/organoid_fno.h).
 
-The knobs are the lumen/ECM workgroups of the step's main launch
+Launch shape: the lumen/ECM workgroups of the step's main launch
 (lum_blocks), the workgroups of each non-cell role of the launches before
 and between steps (roles: psi tiles, h(lum)/h(ECM) parts, lumen/ECM
 copy-back) and the steps per replayed hipGraph (graph_steps).  None of them
@@ -8,13 +10,19 @@ changes results (the library's partitions feed only pointwise updates,
 per-tile sums in a fixed order and box unions); they change how the
 launches fill the device.  The defaults come from the compute-unit count.
 
-Tuning times each candidate on the same state (re-uploaded before every
+GEMM tiles: 64 x 64, 32 x 64, 64 x 32 or 32 x 32 per stage of the network,
+chosen by timing forward passes (inference stages) and training steps
+(backward stages); every tile accumulates the same products in the same
+order, so results do not change either.
+
+Launch tuning times each candidate on the same state (re-uploaded before every
 candidate, so each sees the same reshaping work), after a warm-up past the
 slower first steps that follow an upload, by
 coordinate descent over the three knobs, and keeps the fastest.  Results are
 cached in a JSON file keyed by the device and the workload's shape
 (grid side, cell count, zone size), so a later run of the same shape on the
-same device reuses them.
+same device reuses them; the tile choices likewise per network shape and
+batch.
 """
 from __future__ import annotations
 
@@ -25,7 +33,7 @@ import time
 from . import host
 
 __all__ = ["CACHE_ENV", "default_cache_path", "cache_key", "load_cache", "save_cache", "candidates", "tune",
-           "tuned"]
+           "tuned", "tune_fno", "tuned_fno"]
 
 CACHE_ENV = "ORGANOID_TUNE_CACHE"
 
@@ -150,3 +158,84 @@ def tuned(org, cache_path: str | None = None, retune: bool = False, log=None, **
     cache[key] = {"tuning": r["best"], "ms_per_step": r["best_ms"], "default_ms_per_step": r["default_ms"]}
     save_cache(path, cache)
     return dict(r, tuning=r["best"], source="tuned", key=key, path=path)
+
+
+def tune_fno(net, batch: int, reps: int = 5, train: bool = True, passes: int = 2, log=None) -> dict:
+    """Coordinate descent over the tile shape of each network stage: the
+    forward stages timed by fno_forward, the training stages by
+    fno_loss_grad, on random inputs of the given batch.  Sets and returns the
+    best tiles ({"tiles", "forward_ms", "train_ms", "default_forward_ms",
+    "default_train_ms"})."""
+    import torch
+
+    from . import fno as fnomod
+    M = net.cfg.M
+    x = torch.rand((batch, 2, M, M), device="cuda")
+    y = torch.rand((batch, 2, M, M), device="cuda") * 1e-3
+    out = torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn) -> float:
+        fn()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def fwd():
+        net.forward(x, out)
+
+    def trn():
+        net.loss_grad(x, y)
+
+    tiles = net.get_tiles()
+    fwd_stages = [fnomod.STAGES.index(s) for s in ("y", "x", "mix", "ix", "layer")]
+    trn_stages = [fnomod.STAGES.index(s) for s in ("proj", "bwd_spectral", "bwd_pointwise", "wgrad")]
+    res = {}
+    for kind, stages, fn in (("forward", fwd_stages, fwd), ("train", trn_stages if train else [], trn)):
+        if not stages:
+            continue
+        net.set_tiles(tiles)
+        best = timed(fn)
+        res[f"default_{kind}_ms"] = best
+        for _ in range(passes):
+            changed = False
+            for st in stages:
+                for t in range(len(fnomod.TILES)):
+                    if t == tiles[st]:
+                        continue
+                    cand = list(tiles)
+                    cand[st] = t
+                    net.set_tiles(cand)
+                    ms = timed(fn)
+                    if log is not None:
+                        log(f"tune fno {kind} stage {fnomod.STAGES[st]} tile {fnomod.TILES[t]}: {ms:.4f} ms")
+                    if ms < best:
+                        best, tiles, changed = ms, cand, True
+            if not changed:
+                break
+        res[f"{kind}_ms"] = best
+    net.set_tiles(tiles)
+    res["tiles"] = {fnomod.STAGES[k]: fnomod.TILES[v] for k, v in enumerate(tiles)}
+    res["tile_ids"] = tiles
+    return res
+
+
+def tuned_fno(net, batch: int, cache_path: str | None = None, retune: bool = False, log=None, **kw) -> dict:
+    """Set the network's tiles from the cache, or tune and store them (keyed
+    by device, network shape and batch)."""
+    path = cache_path or default_cache_path()
+    dev, _ = device_name()
+    c = net.cfg
+    key = f"{dev}|fno M={c.M} w={c.width} m={c.modes} L={c.layers} p={c.proj}|batch={batch}"
+    cache = load_cache(path)
+    if not retune and key in cache:
+        net.set_tiles(cache[key]["tile_ids"])
+        return dict(cache[key], source="cache", key=key, path=path)
+    r = tune_fno(net, batch, log=log, **kw)
+    cache = load_cache(path)
+    cache[key] = {k: v for k, v in r.items()}
+    save_cache(path, cache)
+    return dict(r, source="tuned", key=key, path=path)

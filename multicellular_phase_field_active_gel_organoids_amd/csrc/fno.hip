@@ -32,8 +32,6 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-#define FG_BM 64
-#define FG_BN 64
 #define FG_BK 32
 #define FG_LD (FG_BK + 8)     /* LDS row stride (bf16): 80 B, keeps 16-B fragment reads aligned */
 
@@ -47,8 +45,9 @@ static __device__ __forceinline__ float ld_op(const void *p, int64_t o) {
     return AF ? ((const float *)p)[o] : (float)((const bf16 *)p)[o];
 }
 
-/* C = epilogue(alpha A B), one 64 x 64 tile of C per 256-thread workgroup,
- * 4 waves of 32 x 32 (2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
+/* C = epilogue(alpha A B), one BM x BN tile of C (64 or 32 each) per
+ * 256-thread workgroup, the 4 waves laid out by TileShape (64 x 64: 2 x 2
+ * waves of 32 x 32, i.e. 2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
  * LDS.  Tiles are staged with the contiguous dimension across threads
  * (k if the operand's k-stride is 1, else m or n), converted to BF16, and
  * stored k-contiguous so each lane reads its 8-element fragment as one
@@ -77,17 +76,18 @@ static __device__ __forceinline__ void ld8(const void *p, int64_t o, float (&v)[
     }
 }
 
-/* Stage one 64 x 32 operand tile (rows r0.., k0..) into LDS as [row][k]:
- * thread t takes 8 elements consecutive along the operand's unit-stride
- * dimension (k if kc, else the row), as one vector load when they are in
- * range and 16-byte aligned (vec: the offsets are multiples of 8 BF16 / 4
- * FP32 elements), and stores them k-contiguous. */
-template <bool F32>
+/* Stage one ROWS x 32 operand tile (rows r0.., k0..) into LDS as [row][k]:
+ * thread t < ROWS * 4 takes 8 elements consecutive along the operand's
+ * unit-stride dimension (k if kc, else the row), as one vector load when they
+ * are in range and 16-byte aligned (vec: the offsets are multiples of 8 BF16
+ * / 4 FP32 elements), and stores them k-contiguous. */
+template <bool F32, int ROWS>
 static __device__ __forceinline__ void stage_tile(const void *p, const int64_t o0, const int64_t s_row,
                                                   const int64_t s_k, const bool kc, const bool vec, const int r0,
                                                   const int nrow, const int k0, const int K, bf16 *S) {
     const int e0 = threadIdx.x * 8;
-    const int r = kc ? (e0 >> 5) : (e0 & 63), k = kc ? (e0 & 31) : (e0 >> 6);
+    if (e0 >= ROWS * FG_BK) return;
+    const int r = kc ? (e0 >> 5) : (e0 & (ROWS - 1)), k = kc ? (e0 & 31) : (e0 / ROWS);
     const int gr = r0 + r, gk = k0 + k;
     float v[8];
     const bool inb = kc ? (gr < nrow && gk + 8 <= K) : (gk < K && gr + 8 <= nrow);
@@ -111,10 +111,21 @@ static __device__ __forceinline__ void stage_tile(const void *p, const int64_t o
     }
 }
 
-template <bool AF, bool BF>
+/* Wave layout of a BM x BN tile over the 4 waves of a workgroup: WGM x WGN
+ * waves of (BM / WGM) x (BN / WGN), each MI x NI MFMA 16 x 16 tiles. */
+template <int BM, int BN>
+struct TileShape {
+    static constexpr int WGM = (BM == 64 && BN == 32) ? 4 : ((BM == 32 && BN == 64) ? 1 : 2);
+    static constexpr int WGN = 4 / WGM;
+    static constexpr int TM = BM / WGM, TN = BN / WGN;
+    static constexpr int MI = TM / 16, NI = TN / 16;
+};
+
+template <bool AF, bool BF, int BM, int BN>
 static __device__ __forceinline__ void gemm_seg(const GemmSeg &g, const int M, const int N, const int z1,
                                                 const int z2, const int m0, const int n0, bf16 *As, bf16 *Bs,
-                                                f32x4 (&acc)[2][2]) {
+                                                f32x4 (&acc)[TileShape<BM, BN>::MI][TileShape<BM, BN>::NI]) {
+    using T = TileShape<BM, BN>;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t oA = (int64_t)z1 * g.sA1 + (int64_t)z2 * g.sA2;
     const int64_t oB = (int64_t)z1 * g.sB1 + (int64_t)z2 * g.sB2;
@@ -122,27 +133,30 @@ static __device__ __forceinline__ void gemm_seg(const GemmSeg &g, const int M, c
     const int va = AF ? 4 : 8, vb = BF ? 4 : 8;
     const bool veca = oA % va == 0 && (akc ? g.sAm : g.sAk) % va == 0;
     const bool vecb = oB % vb == 0 && (bkc ? g.sBn : g.sBk) % vb == 0;
-    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    const int wm = (w % T::WGM) * T::TM, wn = (w / T::WGM) * T::TN;
     for (int k0 = 0; k0 < g.K; k0 += FG_BK) {
-        stage_tile<AF>(g.A, oA, g.sAm, g.sAk, akc, veca, m0, M, k0, g.K, As);
-        stage_tile<BF>(g.B, oB, g.sBn, g.sBk, bkc, vecb, n0, N, k0, g.K, Bs);
+        stage_tile<AF, BM>(g.A, oA, g.sAm, g.sAk, akc, veca, m0, M, k0, g.K, As);
+        stage_tile<BF, BN>(g.B, oB, g.sBn, g.sBk, bkc, vecb, n0, N, k0, g.K, Bs);
         __syncthreads();
-        bf16x8 a[2], b[2];
+        bf16x8 a[T::MI], b[T::NI];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+        for (int i = 0; i < T::MI; i++)
             a[i] = *(const bf16x8 *)&As[(wm + i * 16 + (lane & 15)) * FG_LD + (lane >> 4) * 8];
-            b[i] = *(const bf16x8 *)&Bs[(wn + i * 16 + (lane & 15)) * FG_LD + (lane >> 4) * 8];
-        }
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < T::NI; j++)
+            b[j] = *(const bf16x8 *)&Bs[(wn + j * 16 + (lane & 15)) * FG_LD + (lane >> 4) * 8];
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < T::MI; i++)
+#pragma unroll
+            for (int j = 0; j < T::NI; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
 }
 
-/* C = epilogue(alpha A B), one 64 x 64 tile of C per 256-thread workgroup,
- * 4 waves of 32 x 32 (2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
+/* C = epilogue(alpha A B), one BM x BN tile of C (64 or 32 each) per
+ * 256-thread workgroup, the 4 waves laid out by TileShape (64 x 64: 2 x 2
+ * waves of 32 x 32, i.e. 2 x 2 MFMA 16x16x32 tiles), K in steps of 32 through
  * LDS.  Tiles are staged with the contiguous dimension across threads
  * (k if the operand's k-stride is 1, else m or n), converted to BF16, and
  * stored k-contiguous so each lane reads its 8-element fragment as one
@@ -151,27 +165,28 @@ static __device__ __forceinline__ void gemm_seg(const GemmSeg &g, const int M, c
  * column lane % 16.  SEG2: a second K segment (BF16 operands, s2) adds into
  * the same accumulators before the epilogue; bias_m: the bias is indexed by
  * the row instead of the column. */
-template <bool AF, bool BF, bool SEG2>
+template <bool AF, bool BF, bool SEG2, int BM, int BN>
 __global__ __launch_bounds__(256) void k_fno_gemm(fno_gemm_args p, GemmSeg s2, int bias_m) {
-    __shared__ __attribute__((aligned(16))) bf16 As[FG_BM * FG_LD];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[FG_BN * FG_LD];
+    using T = TileShape<BM, BN>;
+    __shared__ __attribute__((aligned(16))) bf16 As[BM * FG_LD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[BN * FG_LD];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = blockIdx.z, z1 = z / p.nz2, z2 = z - z1 * p.nz2;
-    const int m0 = blockIdx.x * FG_BM, n0 = blockIdx.y * FG_BN;
-    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-    f32x4 acc[2][2];
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int wm = (w % T::WGM) * T::TM, wn = (w / T::WGM) * T::TN;
+    f32x4 acc[T::MI][T::NI];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < T::MI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < T::NI; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const GemmSeg s1 = {p.A, p.sAm, p.sAk, p.sA1, p.sA2, p.B, p.sBk, p.sBn, p.sB1, p.sB2, p.K};
-    gemm_seg<AF, BF>(s1, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
-    if (SEG2) gemm_seg<false, false>(s2, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
+    gemm_seg<AF, BF, BM, BN>(s1, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
+    if (SEG2) gemm_seg<false, false, BM, BN>(s2, p.M, p.N, z1, z2, m0, n0, As, Bs, acc);
     const int64_t oC = (int64_t)z1 * p.sC1 + (int64_t)z2 * p.sC2;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < T::MI; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < T::NI; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r, n = n0 + wn + j * 16 + (lane & 15);
@@ -437,6 +452,7 @@ struct fno_net {
     double *lpart = nullptr, *lpart_h = nullptr;
     std::vector<void *> allocs;
     bool weights_fresh = false;
+    int tile[FNO_STAGE_COUNT] = {};   /* GEMM tile variant per stage (FNO_TILE_*) */
 };
 
 namespace {
@@ -458,30 +474,43 @@ int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); 
 
 int launch_ok() { return hipGetLastError() == hipSuccess ? ORG_OK : ORG_EHIP; }
 
-int gemm(fno_net *n, const fno_gemm_args &a) {
+/* One GEMM launch with tile variant t (FNO_TILE_*). */
+template <bool AF, bool BF, bool SEG2>
+void launch_tiled(fno_net *n, int t, const fno_gemm_args &a, const GemmSeg &s2, int bias_m, int64_t nz) {
+    const int bm = (t == FNO_TILE_32x64 || t == FNO_TILE_32x32) ? 32 : 64;
+    const int bn = (t == FNO_TILE_64x32 || t == FNO_TILE_32x32) ? 32 : 64;
+    dim3 g((a.M + bm - 1) / bm, (a.N + bn - 1) / bn, (unsigned)nz);
+    switch (t) {
+    case FNO_TILE_32x64: k_fno_gemm<AF, BF, SEG2, 32, 64><<<g, dim3(256), 0, n->st>>>(a, s2, bias_m); break;
+    case FNO_TILE_64x32: k_fno_gemm<AF, BF, SEG2, 64, 32><<<g, dim3(256), 0, n->st>>>(a, s2, bias_m); break;
+    case FNO_TILE_32x32: k_fno_gemm<AF, BF, SEG2, 32, 32><<<g, dim3(256), 0, n->st>>>(a, s2, bias_m); break;
+    default: k_fno_gemm<AF, BF, SEG2, 64, 64><<<g, dim3(256), 0, n->st>>>(a, s2, bias_m); break;
+    }
+}
+
+int gemm(fno_net *n, const fno_gemm_args &a, int stage = FNO_STAGE_API) {
     if (a.M <= 0 || a.N <= 0 || a.K <= 0 || a.nz1 <= 0 || a.nz2 <= 0 || !a.A || !a.B || !a.C) return ORG_EINVAL;
     if (a.c_mode < 0 || a.c_mode > 2) return ORG_EINVAL;
     const int64_t nz = (int64_t)a.nz1 * a.nz2;
     if (nz > 65535) return ORG_EINVAL;
-    dim3 g((a.M + FG_BM - 1) / FG_BM, (a.N + FG_BN - 1) / FG_BN, (unsigned)nz);
     const GemmSeg none{};
-    if (a.a_f32 && a.b_f32) k_fno_gemm<true, true, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
-    else if (a.a_f32) k_fno_gemm<true, false, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
-    else if (a.b_f32) k_fno_gemm<false, true, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
-    else k_fno_gemm<false, false, false><<<g, dim3(256), 0, n->st>>>(a, none, 0);
+    const int t = n->tile[stage];
+    if (a.a_f32 && a.b_f32) launch_tiled<true, true, false>(n, t, a, none, 0, nz);
+    else if (a.a_f32) launch_tiled<true, false, false>(n, t, a, none, 0, nz);
+    else if (a.b_f32) launch_tiled<false, true, false>(n, t, a, none, 0, nz);
+    else launch_tiled<false, false, false>(n, t, a, none, 0, nz);
     return launch_ok();
 }
 
 /* A GEMM with a second K segment (FP32 A, BF16 B in the first; BF16 both in
  * the second) and the bias per row. */
-int gemm2(fno_net *n, const fno_gemm_args &a, const GemmSeg &s2) {
+int gemm2(fno_net *n, const fno_gemm_args &a, const GemmSeg &s2, int stage) {
     if (a.M <= 0 || a.N <= 0 || a.K <= 0 || s2.K <= 0 || a.nz1 <= 0 || a.nz2 <= 0 || !a.A || !a.B || !a.C ||
         !s2.A || !s2.B || !a.a_f32 || a.b_f32 || a.c_mode == 1)
         return ORG_EINVAL;
     const int64_t nz = (int64_t)a.nz1 * a.nz2;
     if (nz > 65535) return ORG_EINVAL;
-    dim3 g((a.M + FG_BM - 1) / FG_BM, (a.N + FG_BN - 1) / FG_BN, (unsigned)nz);
-    k_fno_gemm<true, false, true><<<g, dim3(256), 0, n->st>>>(a, s2, 1);
+    launch_tiled<true, false, true>(n, n->tile[stage], a, s2, 1, nz);
     return launch_ok();
 }
 
@@ -562,7 +591,7 @@ int wgrad(fno_net *n, int B, const void *X, int x_f32, int Mo, const void *Y, in
     a.C = n->part; a.sCm = No; a.sCn = 1; a.sC1 = (int64_t)nch * Mo * No; a.sC2 = (int64_t)Mo * No;
     a.M = Mo; a.N = No; a.K = (int)std::min<int64_t>(KC, n->P); a.nz1 = B; a.nz2 = nch;
     if (n->P % KC != 0 && n->P > KC) return ORG_EINVAL;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_WGRAD));
     const int64_t cnt = (int64_t)Mo * No;
     hipLaunchKernelGGL(k_zsum, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->part, g, B * nch, cnt);
     return launch_ok();
@@ -584,14 +613,14 @@ int spectral_fwd(fno_net *n, int B, int l) {
     a.B = n->Fy; a.sBk = 1; a.sBn = M;
     a.C = n->Z; a.sCm = 2 * m; a.sCn = 1;
     a.M = (int)rows; a.N = 2 * m; a.K = M;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_Y));
     /* x: Zh[b,c][(kx,ri')][ky] = sum_(x,ri) T[(kx,ri')][(x,ri)] Z[b,c][(x,ri)][ky] */
     a = G0();
     a.A = n->T; a.sAm = 2 * M; a.sAk = 1;
     a.B = n->Z; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)M * 2 * m; a.b_f32 = 1;
     a.C = n->Zh; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)4 * m * m;
     a.M = 4 * m; a.N = m; a.K = 2 * M; a.nz1 = B * w;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_X));
     TRY(to_modes(n, B, n->Zh, 1, n->xh[l], 1));
     /* per mode: Yh[mode][b][(o,ri)] = sum_(c,ri) Xh[mode][b][(c,ri)] Wx[mode][(c,ri)][(o,ri)] */
     a = G0();
@@ -599,7 +628,7 @@ int spectral_fwd(fno_net *n, int B, int l) {
     a.B = n->wX[l]; a.sBk = 2 * w; a.sBn = 1; a.sB1 = (int64_t)4 * w * w;
     a.C = n->Yh; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)B * 2 * w;
     a.M = B; a.N = 2 * w; a.K = 2 * w; a.nz1 = n->nm;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_MIX));
     TRY(from_modes(n, B, n->Yh, 1, n->Yh2, 1));
     /* inverse x: Zp[b,o][(x,ri)][ky] = sum_(kx,ri') Ti[(x,ri)][(kx,ri')] Yh2[b,o][(kx,ri')][ky] */
     a = G0();
@@ -607,7 +636,7 @@ int spectral_fwd(fno_net *n, int B, int l) {
     a.B = n->Yh2; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)4 * m * m;
     a.C = n->Zp; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)M * 2 * m;
     a.M = 2 * M; a.N = m; a.K = 4 * m; a.nz1 = B * w;
-    return gemm(n, a);
+    return gemm(n, a, FNO_STAGE_IX);
 }
 
 /* The rest of layer l, one GEMM per (b, x) row with two K segments:
@@ -630,7 +659,7 @@ int layer_out(fno_net *n, int B, int l, bool train) {
     s2.A = n->wW[l]; s2.sAm = w; s2.sAk = 1;
     s2.B = n->v[l]; s2.sBk = P; s2.sBn = 1; s2.sB1 = (int64_t)w * P; s2.sB2 = M;
     s2.K = w;
-    return gemm2(n, a, s2);
+    return gemm2(n, a, s2, FNO_STAGE_LAYER);
 }
 
 /* Forward pass.  train: keep what backward needs (the BF16 input, the FP32
@@ -661,10 +690,10 @@ int forward(fno_net *n, const float *in, int B, bool train, float *vout) {
     }
     a = pointwise(n, B, n->v[n->L], 0, w, n->wQ1, 1, w, n->qpre, n->proj);
     a.bias = n->par + n->oc1; a.C2 = n->q; a.act = 1;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_PROJ));
     a = pointwise(n, B, n->q, 0, n->proj, n->wQ2, 1, n->proj, train ? n->out : vout, 2);
     a.bias = n->par + n->oc2;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_PROJ));
     if (!train) {
         hipLaunchKernelGGL(k_scale_out, dim3(grid_for((int64_t)B * 2 * n->P)), dim3(256), 0, n->st, vout, B, n->P,
                            n->out_scale[0], n->out_scale[1]);
@@ -684,14 +713,14 @@ int spectral_bwd(fno_net *n, int B, int l, const float *du, float *dv) {
     a.B = n->Fi; a.sBk = 1; a.sBn = M;
     a.C = n->Zp; a.sCm = 2 * m; a.sCn = 1;
     a.M = (int)rows; a.N = 2 * m; a.K = M;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_SPECTRAL));
     /* dYh2[b,o][(kx,ri')][ky] = sum_(x,ri) Ti[(x,ri)][(kx,ri')] dZp[b,o][(x,ri)][ky] */
     a = G0();
     a.A = n->Ti; a.sAm = 1; a.sAk = 4 * m;
     a.B = n->Zp; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)M * 2 * m; a.b_f32 = 1;
     a.C = n->Zh; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)4 * m * m;
     a.M = 4 * m; a.N = m; a.K = 2 * M; a.nz1 = B * w;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_SPECTRAL));
     TRY(to_modes(n, B, n->Zh, 1, n->dYh, 1));
     /* dWx[mode] = Xh[mode]^T dYh[mode] */
     a = G0();
@@ -699,7 +728,7 @@ int spectral_bwd(fno_net *n, int B, int l, const float *du, float *dv) {
     a.B = n->dYh; a.sBk = 2 * w; a.sBn = 1; a.sB1 = (int64_t)B * 2 * w;
     a.C = n->gexp; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)4 * w * w;
     a.M = 2 * w; a.N = 2 * w; a.K = B; a.nz1 = n->nm;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_SPECTRAL));
     const int64_t cnt = (int64_t)n->nm * w * w;
     hipLaunchKernelGGL(k_fold_dR, dim3(grid_for(cnt)), dim3(256), 0, n->st, n->gexp, n->grad + n->oRr[l],
                        n->grad + n->oRi[l], n->nm, w);
@@ -710,7 +739,7 @@ int spectral_bwd(fno_net *n, int B, int l, const float *du, float *dv) {
     a.B = n->wX[l]; a.sBk = 1; a.sBn = 2 * w; a.sB1 = (int64_t)4 * w * w;
     a.C = n->dXh; a.sCm = 2 * w; a.sCn = 1; a.sC1 = (int64_t)B * 2 * w;
     a.M = B; a.N = 2 * w; a.K = 2 * w; a.nz1 = n->nm;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_SPECTRAL));
     TRY(from_modes(n, B, n->dXh, 1, n->dZh, 1));
     /* dZ[b,c][(x,ri)][ky] = sum_(kx,ri') T[(kx,ri')][(x,ri)] dZh[b,c][(kx,ri')][ky] */
     a = G0();
@@ -718,14 +747,14 @@ int spectral_bwd(fno_net *n, int B, int l, const float *du, float *dv) {
     a.B = n->dZh; a.sBk = m; a.sBn = 1; a.sB1 = (int64_t)4 * m * m;
     a.C = n->Z; a.sCm = m; a.sCn = 1; a.sC1 = (int64_t)M * 2 * m;
     a.M = 2 * M; a.N = m; a.K = 4 * m; a.nz1 = B * w;
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_SPECTRAL));
     /* dv[row][y] += sum_(ri,ky) dZ[row][(ri,ky)] Fy[(ri,ky)][y] */
     a = G0();
     a.A = n->Z; a.sAm = 2 * m; a.sAk = 1; a.a_f32 = 1;
     a.B = n->Fy; a.sBk = M; a.sBn = 1;
     a.C = dv; a.sCm = M; a.sCn = 1; a.c_mode = 1;
     a.M = (int)rows; a.N = M; a.K = 2 * m;
-    return gemm(n, a);
+    return gemm(n, a, FNO_STAGE_BWD_SPECTRAL);
 }
 
 int backward(fno_net *n, int B) {
@@ -736,7 +765,7 @@ int backward(fno_net *n, int B) {
     TRY(wgrad(n, B, n->dout, 1, 2, n->q, 0, proj, n->grad + n->oQ2));
     TRY(rowsum(n, B, n->dout, 2, n->grad + n->oc2));
     fno_gemm_args a = pointwise(n, B, n->dout, 1, 2, n->wQ2, proj, 1, n->dq, proj);
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_POINTWISE));
     const int64_t nq = (int64_t)B * proj * P;
     hipLaunchKernelGGL(k_gelu_bwd, dim3(grid_for(nq)), dim3(256), 0, n->st, n->dq, n->qpre, n->dq, nq);
     TRY(launch_ok());
@@ -745,7 +774,7 @@ int backward(fno_net *n, int B) {
     TRY(rowsum(n, B, n->dq, proj, n->grad + n->oc1));
     float *dv = n->dv0, *dvn = n->dv1;
     a = pointwise(n, B, n->dq, 1, proj, n->wQ1, w, 1, dv, w);
-    TRY(gemm(n, a));
+    TRY(gemm(n, a, FNO_STAGE_BWD_POINTWISE));
     const int64_t nv = (int64_t)B * w * P;
     for (int l = n->L - 1; l >= 0; l--) {
         if (l < n->L - 1) {
@@ -757,7 +786,7 @@ int backward(fno_net *n, int B) {
         TRY(rowsum(n, B, du, w, n->grad + n->ob[l]));
         /* dv_l = W_l^T du (pointwise), then the spectral part's transpose */
         a = pointwise(n, B, du, 1, w, n->wW[l], w, 1, dvn, w);
-        TRY(gemm(n, a));
+        TRY(gemm(n, a, FNO_STAGE_BWD_POINTWISE));
         TRY(spectral_bwd(n, B, l, du, dvn));
         std::swap(dv, dvn);
     }
@@ -976,6 +1005,20 @@ int fno_adam_step(fno_net *n, float lr, float b1, float b2, float eps, int t) {
                        (int64_t)n->npar, lr, b1, b2, eps, c1, c2);
     n->weights_fresh = false;
     return launch_ok();
+}
+
+int fno_set_tiles(fno_net *n, const int32_t tiles[FNO_STAGE_COUNT]) {
+    if (!n || !tiles) return ORG_EINVAL;
+    for (int k = 0; k < FNO_STAGE_COUNT; k++)
+        if (tiles[k] < 0 || tiles[k] >= FNO_TILE_COUNT) return ORG_EINVAL;
+    for (int k = 0; k < FNO_STAGE_COUNT; k++) n->tile[k] = tiles[k];
+    return ORG_OK;
+}
+
+int fno_get_tiles(const fno_net *n, int32_t tiles[FNO_STAGE_COUNT]) {
+    if (!n || !tiles) return ORG_EINVAL;
+    for (int k = 0; k < FNO_STAGE_COUNT; k++) tiles[k] = n->tile[k];
+    return ORG_OK;
 }
 
 int fno_gemm(fno_net *n, const fno_gemm_args *a) {

@@ -17,7 +17,13 @@ import numpy as np
 
 from . import host
 
-__all__ = ["FnoConfig", "FnoNet", "layout", "init_params", "fit_scales", "train"]
+__all__ = ["FnoConfig", "FnoNet", "layout", "init_params", "fit_scales", "train", "TILES", "STAGES"]
+
+
+# GEMM tile shapes and network stages (FNO_TILE_*, FNO_STAGE_* in include/organoid_fno.h)
+TILES = ("64x64", "32x64", "64x32", "32x32")
+STAGES = ("y", "x", "mix", "ix", "layer", "proj", "bwd_spectral", "bwd_pointwise", "wgrad", "api")
+STAGE_COUNT = len(STAGES)
 
 
 @dataclass
@@ -66,6 +72,8 @@ def _lib():
     lib.fno_loss_grad.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), vp]
     lib.fno_adam_step.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     lib.fno_gemm.argtypes = [vp, ctypes.POINTER(GemmArgs)]
+    lib.fno_set_tiles.argtypes = [vp, vp]
+    lib.fno_get_tiles.argtypes = [vp, vp]
     lib._fno_ready = True
     return lib
 
@@ -180,6 +188,17 @@ class FnoNet:
     def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         self.t += 1
         host._chk(self.lib.fno_adam_step(self.h, lr, beta1, beta2, eps, self.t), "fno_adam_step")
+
+    def get_tiles(self) -> list:
+        t = np.zeros(STAGE_COUNT, np.int32)
+        host._chk(self.lib.fno_get_tiles(self.h, t.ctypes.data_as(ctypes.c_void_p)), "fno_get_tiles")
+        return [int(v) for v in t]
+
+    def set_tiles(self, tiles) -> None:
+        t = np.ascontiguousarray(tiles, np.int32)
+        if t.size != STAGE_COUNT:
+            raise ValueError(f"{STAGE_COUNT} tile choices expected")
+        host._chk(self.lib.fno_set_tiles(self.h, t.ctypes.data_as(ctypes.c_void_p)), "fno_set_tiles")
 
     def gemm(self, args: GemmArgs) -> None:
         host._chk(self.lib.fno_gemm(self.h, ctypes.byref(args)), "fno_gemm")

@@ -44,10 +44,13 @@ def _dev(a, dtype):
 CASES = ["rowmajor", "transposed_f32", "bf16_gelu_out", "two_level_batch", "aligned_vector"]
 
 
+@pytest.mark.parametrize("tile", range(len(fno.TILES)))
 @pytest.mark.parametrize("case", CASES)
-def test_mfma_gemm_matches_numpy(small_net, case):
+def test_mfma_gemm_matches_numpy(small_net, case, tile):
     """Ragged shapes take the element-wise staging, aligned ones the 16-byte
-    vector loads (aligned_vector: BF16 k-contiguous A, n-contiguous B)."""
+    vector loads (aligned_vector: BF16 k-contiguous A, n-contiguous B); every
+    tile shape (64x64, 32x64, 64x32, 32x32)."""
+    small_net.set_tiles([0] * (fno.STAGE_COUNT - 1) + [tile])
     rng = np.random.default_rng(CASES.index(case))
     M, N, K, nz1, nz2 = 100, 45, 70, 3, 1
     if case == "aligned_vector":
@@ -175,6 +178,40 @@ def test_gradient_matches_oracle(full):
         worst = max(worst, r)
         print(f"  {name:4s} rel L2 {r:.3e}")
         assert r < 2.5e-2, (name, r)
+
+
+def test_tile_shapes_give_identical_results(full):
+    """Every tile shape accumulates the same products in the same K order:
+    forward output and gradient are bit for bit those of the 64x64 tiles."""
+    net, flat, ref = full
+    x = torch.as_tensor(_inputs(2, 128, 14)).cuda()
+    t = torch.as_tensor(_targets(_inputs(2, 128, 14))).cuda()
+    net.set_tiles([0] * fno.STAGE_COUNT)
+    out0 = net.forward(x).cpu().numpy()
+    _, g0 = net.loss_grad(x, t, want_grad=True)
+    for tile in range(1, len(fno.TILES)):
+        net.set_tiles([tile] * fno.STAGE_COUNT)
+        assert np.array_equal(net.forward(x).cpu().numpy(), out0), fno.TILES[tile]
+        _, g = net.loss_grad(x, t, want_grad=True)
+        assert np.array_equal(g, g0), fno.TILES[tile]
+    with pytest.raises(Exception):
+        net.set_tiles([4] * fno.STAGE_COUNT)
+    net.set_tiles([0] * fno.STAGE_COUNT)
+
+
+def test_tile_autotuner_picks_and_caches(full, tmp_path):
+    from multicellular_phase_field_active_gel_organoids_amd import autotune
+    net, flat, ref = full
+    path = str(tmp_path / "tune.json")
+    r = autotune.tuned_fno(net, 2, cache_path=path, reps=1, passes=1)
+    assert r["source"] == "tuned" and r["forward_ms"] <= r["default_forward_ms"]
+    assert net.get_tiles() == r["tile_ids"]
+    print(f"tiles {r['tiles']}: forward {r['default_forward_ms']:.3f} -> {r['forward_ms']:.3f} ms, "
+          f"train {r['default_train_ms']:.3f} -> {r['train_ms']:.3f} ms", flush=True)
+    net.set_tiles([0] * fno.STAGE_COUNT)
+    r2 = autotune.tuned_fno(net, 2, cache_path=path)
+    assert r2["source"] == "cache" and net.get_tiles() == r["tile_ids"]
+    net.set_tiles([0] * fno.STAGE_COUNT)
 
 
 def test_training_reduces_loss():

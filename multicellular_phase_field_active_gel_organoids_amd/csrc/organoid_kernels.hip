@@ -2323,12 +2323,26 @@ static __device__ __forceinline__ void pre_hlum(DevP P, StepArgs a, const int hb
         if (threadIdx.x == 0) a.part_h[k] = tot;
     }
 }
+/* Logical index of block b within a role of C blocks that starts at block
+ * B0.  The hardware deals the workgroups of a launch to the 8 XCDs round
+ * robin (block b runs on XCD b % 8), each XCD with its own L2; this mapping
+ * gives the role's blocks on one XCD a contiguous range of indices, so
+ * neighbouring cells (cells are ordered by angle) and neighbouring row bands,
+ * which read the same psi, h and lumen/ECM pixels, share an L2.  Identity
+ * unless C is a multiple of 8.  Results do not depend on it: every index is
+ * still taken by exactly one block. */
+#define ORG_NXCD 8
+static __device__ __forceinline__ int xcd_index(const int b, const int B0, const int C) {
+    if (C % ORG_NXCD) return b - B0;
+    return (b % ORG_NXCD) * (C / ORG_NXCD) + (b - B0) / ORG_NXCD;
+}
+
 __global__ __launch_bounds__(1024) void k_pre(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_PRE];
-    const int b = blockIdx.x;
-    if (b < P.nc) pre_cell(P, a, b, scr);
-    else if (b < P.nc + a.npsi) pre_psi(P, a, b - P.nc, scr);
-    else pre_hlum(P, a, b - P.nc - a.npsi, scr);
+    const int b = blockIdx.x, nc = P.nc;
+    if (b < nc) pre_cell(P, a, xcd_index(b, 0, nc), scr);
+    else if (b < nc + a.npsi) pre_psi(P, a, xcd_index(b, nc, a.npsi), scr);
+    else pre_hlum(P, a, xcd_index(b, nc + a.npsi, a.nh), scr);
 }
 
 /* k_step: the rest of the timestep up to the copy-back
@@ -2447,9 +2461,9 @@ static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int 
 __global__ __launch_bounds__(1024) void k_step(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[ORG_LDS_FUSED];
     const int nc = P.nc, b = blockIdx.x;
-    if (b < nc) step_spectral(P, a, b, lds);
-    else if (b < nc + a.nlum) step_lumecm(P, a, b - nc, a.nlum, lds);
-    else step_cells(P, a, b - nc - a.nlum, lds);
+    if (b < nc) step_spectral(P, a, xcd_index(b, 0, nc), lds);
+    else if (b < nc + a.nlum) step_lumecm(P, a, xcd_index(b, nc, a.nlum), a.nlum, lds);
+    else step_cells(P, a, xcd_index(b, nc + a.nlum, nc), lds);
 }
 
 /* rho <- rhon on the FFT box and phi's new window (copy_cell_tables_1_by_1,
@@ -2506,9 +2520,9 @@ static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int
 }
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
     __shared__ long long red[48];
-    const int b = blockIdx.x;
-    if (b < P.nc) post_cell(P, a, b);
-    else post_lumecm(P, a, b - P.nc, a.nlp, (int *)red);
+    const int b = blockIdx.x, nc = P.nc;
+    if (b < nc) post_cell(P, a, xcd_index(b, 0, nc));
+    else post_lumecm(P, a, xcd_index(b, nc, a.nlp), a.nlp, (int *)red);
 }
 
 /* k_mid: the copy-back of one timestep and the first part of the next
@@ -2568,10 +2582,10 @@ static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int 
 }
 __global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];
-    const int b = blockIdx.x;
-    if (b < P.nc) mid_cell(P, a, b, scr);
-    else if (b < P.nc + a.npsi) mid_psi(P, a, b - P.nc, scr);
-    else mid_lumecm(P, a, b - P.nc - a.npsi, scr);
+    const int b = blockIdx.x, nc = P.nc;
+    if (b < nc) mid_cell(P, a, xcd_index(b, 0, nc), scr);
+    else if (b < nc + a.npsi) mid_psi(P, a, xcd_index(b, nc, a.npsi), scr);
+    else mid_lumecm(P, a, xcd_index(b, nc + a.npsi, a.nlp), scr);
 }
 
 /* ---- batches: every organoid of a set in one launch per stage -----------

@@ -33,3 +33,21 @@ def test_fixture_matches_the_oracle(case, steps):
             for n, (i0, i1, j0, j1) in enumerate(boxes):
                 assert 0 <= i0 <= i1 <= N and 0 <= j0 <= j1 <= N
                 assert fx[f"s{s}_{name}_{n}"].shape == (i1 - i0, j1 - j0)
+
+
+@pytest.mark.parametrize("case", ["two_cells_256", "organoid16_512"])
+def test_fp32_drift_legs_are_consistent(case):
+    """Every FP32 drift the GPU test compares with is present, finite and
+    small, and the ensemble maximum is at least the unperturbed run's."""
+    fx = np.load(os.path.join(GOLDEN, f"drift_{case}.npz"))
+    keys = [k[4:] for k in fx.files if k.startswith("f32_")]
+    assert keys
+    for k in keys:
+        v = float(fx[f"f32_{k}"])
+        assert np.isfinite(v) and v >= 0.0, k
+        if f"f32e_{k}" in fx:
+            assert float(fx[f"f32e_{k}"]) >= v, k
+        if not k.endswith("_mismatch"):
+            assert max(v, float(fx.get(f"f32e_{k}", v))) < 0.05, k
+    for s in fx["checkpoints"]:
+        assert f"f32_phi_s{s}" in fx.files and f"f32e_phi_s{s}" in fx.files

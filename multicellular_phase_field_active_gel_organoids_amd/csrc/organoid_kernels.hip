@@ -172,6 +172,12 @@ static __device__ __forceinline__ float ldb(const float *__restrict__ base, unsi
 static __device__ __forceinline__ float2 ldb2(const float2 *__restrict__ base, unsigned byte_off) {
     return *(const float2 *)((const char *)base + byte_off);
 }
+/* four consecutive floats at a 4-byte-aligned offset: one global_load_dwordx4
+ * (multi-dword global loads need dword alignment only) */
+typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+static __device__ __forceinline__ f4a ldb4(const float *__restrict__ base, unsigned byte_off) {
+    return *(const f4a *)((const char *)base + byte_off);
+}
 
 /* Hardware reciprocal, square root and 2^x (v_rcp_f32, v_sqrt_f32,
  * v_exp_f32, about 1 ulp each) for the per-pixel and per-mode arithmetic,
@@ -1202,10 +1208,11 @@ static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4
 }
 
 /* Box mode: the boundary ring of both fields in both buffers (lumen 0,
- * ECM 1, useful_functions.cpp:439-462); its normals are zero. */
+ * ECM 1, useful_functions.cpp:439-462), by workgroup blk of nblk; its
+ * normals are zero. */
 static __device__ void lumecm_ring(const DevP &P, float *__restrict__ L0, float *__restrict__ L1,
-                                   float *__restrict__ E0, float *__restrict__ E1) {
-    for (int q = threadIdx.x; q < 4 * P.N - 4; q += blockDim.x) {
+                                   float *__restrict__ E0, float *__restrict__ E1, const int blk, const int nblk) {
+    for (int q = blk * blockDim.x + threadIdx.x; q < 4 * P.N - 4; q += nblk * blockDim.x) {
         int i, j;
         ring_px(q, P.N, i, j);
         const size_t e = (size_t)i * P.N + j;
@@ -1300,7 +1307,7 @@ static __device__ __forceinline__ float reshape_flux_j(float un, const float *u,
 static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *__restrict__ src,
                                                    float *__restrict__ dst, const float *__restrict__ jx,
                                                    const float *__restrict__ jy, const int4 jb,
-                                                   float side, int blk, int nblk) {
+                                                   float side, int blk, int nblk, bool ring = true) {
     const int N = P.N;
     const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
     const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
@@ -1319,6 +1326,10 @@ static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *_
         dst[e] = v;
         acc += (double)(fabsf(v - u) * dx2);
     }
+    /* the boundary ring: corners 0, the rest `side`.  With ring false the
+     * caller has put `side` on the whole ring of both buffers, so outside the
+     * corners the sweep would rewrite the same value and add exactly 0 to the
+     * deviation; only the corners are visited (by the same threads) */
     for (int k = blk * 1024 + (int)threadIdx.x; k < 4 * N - 4; k += stride) {
         int i, j;
         if (k < N) { i = 0; j = k; }
@@ -1327,6 +1338,7 @@ static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *_
         else { i = k - (3 * N - 2) + 1; j = N - 1; }
         const size_t e = (size_t)i * N + j;
         const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
+        if (!ring && !corner) continue;
         float v = corner ? 0.0f : side;
         dst[e] = v;
         acc += (double)(fabsf(v - src[e]) * dx2);
@@ -1433,7 +1445,7 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
                                      float *bufE0, float *bufE1, const float *jlx, const float *jly,
                                      const float *jex, const float *jey, const int *jbox_part, int nparts,
                                      LumState *lst, double *part, unsigned *bar, int &epoch, int *abort_flag,
-                                     int *err, long long limit, int *cur_out) {
+                                     int *err, long long limit, int *cur_out, bool ring) {
     LumState s[2] = {lst[0], lst[1]};
     const int4 jb[2] = {reduce_jbox(jbox_part, nparts, 0, P.N), reduce_jbox(jbox_part, nparts, 1, P.N)};
     for (int k = 0; !(s[0].done && s[1].done); k++) {
@@ -1444,7 +1456,8 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
                 float *b0 = f ? bufE0 : bufL0, *b1 = f ? bufE1 : bufL1;
                 const float *src = (s[f].ri & 1) ? b1 : b0;
                 float *dst = (s[f].ri & 1) ? b0 : b1;
-                acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jb[f], f ? 1.0f : 0.0f, blk, nblk);
+                acc = lum_sweep(P, src, dst, f ? jex : jlx, f ? jey : jly, jb[f], f ? 1.0f : 0.0f, blk, nblk,
+                                ring);
             }
             double tot = block_sum_d(acc, red);
             if (threadIdx.x == 0) pk[f * nblk + blk] = tot;
@@ -1674,41 +1687,54 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float MMf = (float)MM;
 
     /* P1: active stress (rho_functions.cpp:32-115), reaction (:193-209), in
-     * batches of 4 pixels per thread: the 40 loads of a batch issue before
-     * any use (byte offsets from uniform bases, scheduling barrier) */
+     * quads of 4 pixels along a row: thread t takes columns 4 (t & 31) ..
+     * +3 of rows (t >> 5) + 32 q, q = 0..3, and a quad's loads are 8 four-
+     * wide ones plus the two row neighbours at its ends, issued before any
+     * use (byte offsets from uniform bases, scheduling barrier) */
     const unsigned rN = 4u * (unsigned)N;
+    const int jq = 4 * (t & 31);
 #pragma unroll 1
-    for (int qb = 0; qb < 16; qb += 4) {
-        float fv[4], fxp[4], fxm[4], fyp[4], fym[4], fl[4], fe[4], f1[4], f2[4], fr[4];
+    for (int qb = 0; qb < 4; qb += 2) {
+        f4a fv[2], fxp[2], fxm[2], fl[2], fe[2], f1[2], f2[2], fr[2];
+        float fym[2], fyp[2];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int e = t + 1024 * (qb + k), i = e >> 7, j = e & 127;
-            const unsigned g = 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + j));
-            fv[k] = ldb(ph, g); fxp[k] = ldb(ph, g + rN); fxm[k] = ldb(ph, g - rN);
-            fyp[k] = ldb(ph, g + 4); fym[k] = ldb(ph, g - 4);
-            fl[k] = ldb(lum, g); fe[k] = ldb(ecm, g); f1[k] = ldb(ph1, g); f2[k] = ldb(ph2, g);
-            fr[k] = ldb(rh, g);
+        for (int k = 0; k < 2; k++) {
+            const int i = (t >> 5) + 32 * (qb + k);
+            const unsigned g = 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + jq));
+            fv[k] = ldb4(ph, g); fxp[k] = ldb4(ph, g + rN); fxm[k] = ldb4(ph, g - rN);
+            fym[k] = ldb(ph, g - 4); fyp[k] = ldb(ph, g + 16);
+            fl[k] = ldb4(lum, g); fe[k] = ldb4(ecm, g); f1[k] = ldb4(ph1, g); f2[k] = ldb4(ph2, g);
+            fr[k] = ldb4(rh, g);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int e = t + 1024 * (qb + k), i = e >> 7, j = e & 127;
-            const float v = fv[k];
-            const float dxp = fxp[k] - fxm[k], dyp = fyp[k] - fym[k];
-            const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
-            const int idx = turnover_label(v, fl[k], fe[k], f1[k], f2[k]);
-            lab[e] = (unsigned char)idx;
-            const float r = fr[k];
-            if (!GIVEN) {
-                float pxx, pxy, pyv;
-                active_stress(P, idx, r, ng, dxp, dyp, inv4, pxx, pxy, pyv);
-                SB[e] = pyv;
-                sm[i * ORG_LS + j] = make_float2(pxx, pxy);
+        for (int k = 0; k < 2; k++) {
+            const int i = (t >> 5) + 32 * (qb + k), e0 = i * ORG_MB + jq;
+            unsigned labs = 0;
+            f4a pyq, rq;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const float v = fv[k][m];
+                const float dxp = fxp[k][m] - fxm[k][m];
+                const float dyp = (m < 3 ? fv[k][m + 1] : fyp[k]) - (m > 0 ? fv[k][m - 1] : fym[k]);
+                const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+                const int idx = turnover_label(v, fl[k][m], fe[k][m], f1[k][m], f2[k][m]);
+                labs |= (unsigned)idx << (8 * m);
+                const float r = fr[k][m];
+                if (!GIVEN) {
+                    float pxx, pxy, pyv;
+                    active_stress(P, idx, r, ng, dxp, dyp, inv4, pxx, pxy, pyv);
+                    pyq[m] = pyv;
+                    sm[i * ORG_LS + jq + m] = make_float2(pxx, pxy);
+                }
+                const float d = v - 0.5f;
+                const float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
+                const float react = -by_label(P.deg, idx) * r + by_label(P.gen, idx) * pnd;
+                rq[m] = r + P.dt * react;
             }
-            const float d = v - 0.5f;
-            const float pnd = ng * hw_exp(-d * d * isig2) * dnorm;
-            const float react = -by_label(P.deg, idx) * r + by_label(P.gen, idx) * pnd;
-            rbox[e] = r + P.dt * react;
+            *(unsigned *)(lab + e0) = labs;
+            if (!GIVEN) *(float4 *)(SB + e0) = make_float4(pyq[0], pyq[1], pyq[2], pyq[3]);
+            *(float4 *)(rbox + e0) = make_float4(rq[0], rq[1], rq[2], rq[3]);
         }
     }
     if (GIVEN) {          /* given velocity: rho v straight into the transform buffer */
@@ -1727,7 +1753,10 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     __syncthreads();
     fft2d_128<-1>(sm, tw);
     {
-        float yy[16];                      /* this thread wrote SB[e] in P1 */
+        /* one pixel per lane along a row (no LDS bank conflicts); SB was
+         * written by other threads of this workgroup in P1, ordered before
+         * these loads by the transform's workgroup barriers */
+        float yy[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) yy[q] = ldb(SB, 4u * (unsigned)(t + 1024 * q));
         __builtin_amdgcn_sched_barrier(0);
@@ -2258,31 +2287,29 @@ struct StepArgs {
 /* Live box of field f merged with the partial boxes of live pixels that the
  * last copy-back found (wave 0 reduces them); valid in every thread.  sb: 4
  * ints of LDS. */
-static __device__ int4 merged_live(const DevP &P, const StepArgs &a, const int f, int *sb) {
-    const int N = P.N;
-    if (threadIdx.x < 64) {
-        int c0 = N, c1 = -1, c2 = N, c3 = -1;
-        for (int k = threadIdx.x; k < a.nlp; k += 64) {
-            const int *q = a.live_part + ((size_t)f * a.nlp + k) * 4;
-            if (q[1] >= 0) { c0 = min(c0, q[0]); c1 = max(c1, q[1]); c2 = min(c2, q[2]); c3 = max(c3, q[3]); }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            c0 = min(c0, __shfl_xor(c0, o, 64)); c1 = max(c1, __shfl_xor(c1, o, 64));
-            c2 = min(c2, __shfl_xor(c2, o, 64)); c3 = max(c3, __shfl_xor(c3, o, 64));
-        }
-        if (threadIdx.x == 0) {
-            int4 r = ld_box(a.live, f);
-            if (c1 >= 0) {
-                const int4 q = make_int4(c0, c1 + 1, c2, c3 + 1);
-                r = box_empty(r) ? q : make_int4(min(r.x, q.x), max(r.y, q.y), min(r.z, q.z), max(r.w, q.w));
-            }
-            sb[0] = r.x; sb[1] = r.y; sb[2] = r.z; sb[3] = r.w;
-        }
+/* The live boxes of lumen (to sb[0..3]) and ECM (sb[4..7]): the stored ones
+ * merged with the copy-back's partial boxes, wave f reducing field f.  No
+ * barrier inside: the caller's next workgroup barrier publishes sb. */
+static __device__ void merged_live_both(const DevP &P, const StepArgs &a, int *sb) {
+    const int N = P.N, f = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (f > 1) return;
+    int c0 = N, c1 = -1, c2 = N, c3 = -1;
+    for (int k = lane; k < a.nlp; k += 64) {
+        const int *q = a.live_part + ((size_t)f * a.nlp + k) * 4;
+        if (q[1] >= 0) { c0 = min(c0, q[0]); c1 = max(c1, q[1]); c2 = min(c2, q[2]); c3 = max(c3, q[3]); }
     }
-    __syncthreads();
-    const int4 r = make_int4(sb[0], sb[1], sb[2], sb[3]);
-    __syncthreads();
-    return r;
+    for (int o = 32; o > 0; o >>= 1) {
+        c0 = min(c0, __shfl_xor(c0, o, 64)); c1 = max(c1, __shfl_xor(c1, o, 64));
+        c2 = min(c2, __shfl_xor(c2, o, 64)); c3 = max(c3, __shfl_xor(c3, o, 64));
+    }
+    if (lane == 0) {
+        int4 r = ld_box(a.live, f);
+        if (c1 >= 0) {
+            const int4 q = make_int4(c0, c1 + 1, c2, c3 + 1);
+            r = box_empty(r) ? q : make_int4(min(r.x, q.x), max(r.y, q.y), min(r.z, q.z), max(r.w, q.w));
+        }
+        sb[4 * f] = r.x; sb[4 * f + 1] = r.y; sb[4 * f + 2] = r.z; sb[4 * f + 3] = r.w;
+    }
 }
 
 /* k_pre: the first part of a timestep (run_with_active_gels.cpp:123-142),
@@ -2382,21 +2409,28 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
     double *red = (double *)lds;
     int *sb = (int *)(lds + 128);
     int epoch = 0;
-    const double se = sum_parts(a.part_h, hl_ntiles(P.N) + 1, red);
+    /* the prologue's three reductions overlap: the parts of the h(ECM) sum
+     * (every thread's loads, then the fixed-order block sum) and the live
+     * boxes of this step, the stored ones plus what the last copy-back found
+     * (waves 0 and 1; every workgroup merges the same parts, the first
+     * stores the result, and merging again gives the same boxes) */
+    double hv = 0.0;
+    for (int b = threadIdx.x; b < hl_ntiles(P.N) + 1; b += blockDim.x)
+        hv += __hip_atomic_load(&a.part_h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    merged_live_both(P, a, sb);
+    const double se = block_sum_d(hv, red);      /* its barriers publish sb */
+    const int4 BL = make_int4(sb[0], sb[1], sb[2], sb[3]), BE = make_int4(sb[4], sb[5], sb[6], sb[7]);
+    __syncthreads();      /* sb is reused below */
     if (blk == 0) {
         if (threadIdx.x == 0) *a.sum_ecm = se;
         lum_reset_states(P, a.lst);
     }
-    /* live boxes of this step: the stored ones plus what the last
-     * copy-back found (every workgroup merges the same parts; the first
-     * stores the result, and merging again gives the same boxes) */
-    const int4 BL = merged_live(P, a, 0, sb), BE = merged_live(P, a, 1, sb);
     if (blk == 0 && threadIdx.x == 0) {
         a.live[0] = BL.x; a.live[1] = BL.y; a.live[2] = BL.z; a.live[3] = BL.w;
         a.live[4] = BE.x; a.live[5] = BE.y; a.live[6] = BE.z; a.live[7] = BE.w;
     }
     if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
-        if (blk == 0) lumecm_ring(P, a.L0, a.L1, a.E0, a.E1);
+        lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk);
         lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
                            a.jlx, a.jly, a.jbox_part, sb);
         lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
@@ -2408,13 +2442,15 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
     if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
     int cur[2];
     if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
-                       a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur))
+                       a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur,
+                       !(box_mode(BL, P.N) && box_mode(BE, P.N))))
         return;
     /* the lumen/ECM part of the next step's cell-update bracket from the
      * final iterates (the fields the copy-back commits), into the buffer this
      * step's phase-field role does not read; on the parts that can have
      * changed: the live boxes grown by two (the bracket's stencil reaches one
-     * pixel past the copied region), all of them in full mode, and the ring */
+     * pixel past the copied region), all of them in full mode, and the ring
+     * (its stencil reads the corners, which the sweeps rewrite) */
     const float *FL = cur[0] ? a.L1 : a.L0, *FE = cur[1] ? a.E1 : a.E0;
     double *nxt = *a.lbrk_par ? a.lbrk0 : a.lbrk1;
     const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;

@@ -294,8 +294,23 @@ static __device__ __forceinline__ void dft_reg(float2 *v) {
  * take consecutive lines with the same sub-index: along columns that is
  * contiguous, along rows it strides by the odd row pitch, so neither pass
  * has LDS bank conflicts, and the twiddle index is uniform in a wave. */
-template <int S>
-static __device__ __forceinline__ void fft128_lines(float2 *sm, const float2 *tw, int es, int ls) {
+/* Position along its line of output q (0..15) of thread sub in the final
+ * stage of fft128_lines: X[k2 + 16 k1] with k2 = 2 sub + q / 8, k1 = q % 8. */
+static __device__ __forceinline__ int fft_out_pos(const int sub, const int q) {
+    return 2 * sub + (q >> 3) + 16 * (q & 7);
+}
+/* The default last stage: the transformed lines back into the array. */
+struct FftStore {
+    float2 *sm;
+    int es, ls;
+    __device__ void operator()(const float2 *u, const int sub, const int line) const {
+        float2 *base = sm + line * ls;
+#pragma unroll
+        for (int q = 0; q < 16; q++) base[fft_out_pos(sub, q) * es] = u[q];
+    }
+};
+template <int S, class Epi>
+static __device__ __forceinline__ void fft128_lines(float2 *sm, const float2 *tw, int es, int ls, const Epi &epi) {
     const int t = threadIdx.x, line = t & 127, sub = t >> 7;
     float2 *base = sm + line * ls;
     float2 v[16];
@@ -319,19 +334,25 @@ static __device__ __forceinline__ void fft128_lines(float2 *sm, const float2 *tw
     __syncthreads();
     dft_reg<8, S>(u);          /* W8 = W16^2: dft_reg<8> uses w16(k * 2) */
     dft_reg<8, S>(u + 8);
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-        for (int k1 = 0; k1 < 8; k1++) base[((2 * sub + h) + 16 * k1) * es] = u[8 * h + k1];
+    /* every read of this pass is done: the last stage may write anywhere */
+    epi(u, sub, line);
     __syncthreads();
 }
 
 /* Unnormalised 2D DFT of the 128 x 128 array in LDS (row stride ORG_LS),
- * sign S as cufftExecC2C (useful_functions.cpp:650, 658). */
+ * sign S as cufftExecC2C (useful_functions.cpp:650, 658).  The column pass
+ * hands its outputs to epi(u, sub, col): u[q] is X[row][col] with row =
+ * fft_out_pos(sub, q); the default stores them in place, a fused consumer
+ * may instead use them directly (its own stores to LDS go to its own
+ * outputs' places). */
+template <int S, class Epi>
+static __device__ __forceinline__ void fft2d_128(float2 *sm, const float2 *tw, const Epi &epi) {
+    fft128_lines<S>(sm, tw, 1, ORG_LS, FftStore{sm, 1, ORG_LS});    /* rows */
+    fft128_lines<S>(sm, tw, ORG_LS, 1, epi);                         /* columns */
+}
 template <int S>
 static __device__ __forceinline__ void fft2d_128(float2 *sm, const float2 *tw) {
-    fft128_lines<S>(sm, tw, 1, ORG_LS);    /* rows */
-    fft128_lines<S>(sm, tw, ORG_LS, 1);    /* columns */
+    fft2d_128<S>(sm, tw, FftStore{sm, ORG_LS, 1});
 }
 
 /* ---- per-mode friction-viscosity solve, rho_functions.cpp:164-186 -------- */
@@ -723,26 +744,47 @@ static __device__ __forceinline__ double lumecm_bracket(const DevP &P, const flo
 /* The lumen/ECM bracket of part k of the h pass into lbrk, from the fields
  * lsrc, esrc (h = hfun of them, the values the h planes hold): tile k's
  * pixels in rows/cols 1..N-2, with h of the tile grown by one staged in LDS
- * (lds: ORG_LDS_LBRK bytes); the ring part (k = number of tiles): the first
- * interior rows and columns, whose stencils reach the ring.  The bracket
+ * (lds: ORG_LDS_LBRK bytes); the ring parts (k = number of tiles + s, s <
+ * ring_parts): slice s of the first interior rows and columns, whose
+ * stencils reach the ring (along a column every lane reads its own cache
+ * lines, so the ring is split to spread that over workgroups).  The bracket
  * changes only where lum or ECM changed or next to such a pixel. */
 #define ORG_LDS_LBRK (2 * (ORG_HT + 2) * (ORG_HT + 2) * 4)
+#define ORG_RING_PARTS 16
 static __device__ void lbrk_part(const DevP &P, const int k, const float *__restrict__ lsrc,
-                                 const float *__restrict__ esrc, double *__restrict__ lbrk, float *lds) {
+                                 const float *__restrict__ esrc, double *__restrict__ lbrk, float *lds,
+                                 const int ring_parts = 1) {
     const int N = P.N;
     const double inv4 = 1.0 / (4.0 * (double)P.dx * (double)P.dx);
-    if (k == hl_ntiles(N)) {
-        const int nr = 2 * (N - 2), nq = nr + 2 * (N - 4);
-        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-            int i, j;
-            if (q < nr) { i = (q < N - 2) ? 1 : N - 2; j = 1 + (q < N - 2 ? q : q - (N - 2)); }
-            else { const int w = q - nr; j = (w < N - 4) ? 1 : N - 2; i = 2 + (w < N - 4 ? w : w - (N - 4)); }
-            const size_t g = (size_t)i * N + j;
-            const size_t o[9] = {g, g + N, g - N, g + 1, g - 1, g + N + 1, g + N - 1, g - N - 1, g - N + 1};
-            float l[9], e[9];
+    if (k >= hl_ntiles(N)) {
+        /* two pixels per thread and batch, their 36 loads issued before any
+         * use (a slot past the slice loads its last pixel and is not stored) */
+        const int nr = 2 * (N - 2), nall = nr + 2 * (N - 4), sl = k - hl_ntiles(N);
+        const int qa = (int)(((long long)sl * nall) / ring_parts), nq = (int)(((long long)(sl + 1) * nall) / ring_parts);
+        if (nq <= qa) return;
+        for (int q0 = qa + threadIdx.x; q0 < nq; q0 += 2 * blockDim.x) {
+            float l[2][9], e[2][9];
+            size_t gs[2];
 #pragma unroll
-            for (int m = 0; m < 9; m++) { l[m] = hfun(lsrc[o[m]]); e[m] = hfun(esrc[o[m]]); }
-            lbrk[g] = lumecm_bracket(P, l, e, inv4);
+            for (int b = 0; b < 2; b++) {
+                const int q = min(q0 + b * (int)blockDim.x, nq - 1);
+                int i, j;
+                if (q < nr) { i = (q < N - 2) ? 1 : N - 2; j = 1 + (q < N - 2 ? q : q - (N - 2)); }
+                else { const int w = q - nr; j = (w < N - 4) ? 1 : N - 2; i = 2 + (w < N - 4 ? w : w - (N - 4)); }
+                const size_t g = (size_t)i * N + j;
+                gs[b] = g;
+                const size_t o[9] = {g, g + N, g - N, g + 1, g - 1, g + N + 1, g + N - 1, g - N - 1, g - N + 1};
+#pragma unroll
+                for (int m = 0; m < 9; m++) { l[b][m] = lsrc[o[m]]; e[b][m] = esrc[o[m]]; }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                if (q0 + b * (int)blockDim.x >= nq) break;
+#pragma unroll
+                for (int m = 0; m < 9; m++) { l[b][m] = hfun(l[b][m]); e[b][m] = hfun(e[b][m]); }
+                lbrk[gs[b]] = lumecm_bracket(P, l[b], e[b], inv4);
+            }
         }
         return;
     }
@@ -750,13 +792,28 @@ static __device__ void lbrk_part(const DevP &P, const int k, const float *__rest
     float *hl = lds, *he = lds + TW * TW;
     const int nt = (N + ORG_HT - 1) / ORG_HT;
     const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
-    for (int q = threadIdx.x; q < TW * TW; q += blockDim.x) {
-        const int i = ti - 1 + q / TW, j = tj - 1 + q % TW;
-        if (i >= 0 && i < N && j >= 0 && j < N) {
-            const size_t g = (size_t)i * N + j;
-            hl[q] = hfun(lsrc[g]);
-            he[q] = hfun(esrc[g]);
+    /* h of the tile grown by one into LDS, up to SQ pixels per thread and
+     * batch with their loads issued before any use (a slot off the tile or
+     * the grid loads element 0 and is not stored) */
+    constexpr int SQ = 5;
+    for (int q0 = threadIdx.x; q0 < TW * TW; q0 += SQ * blockDim.x) {
+        float lv[SQ], ev[SQ];
+        bool ok[SQ];
+#pragma unroll
+        for (int b = 0; b < SQ; b++) {
+            const int q = q0 + b * (int)blockDim.x, i = ti - 1 + q / TW, j = tj - 1 + q % TW;
+            ok[b] = q < TW * TW && i >= 0 && i < N && j >= 0 && j < N;
+            const size_t g = ok[b] ? (size_t)i * N + j : 0;
+            lv[b] = lsrc[g];
+            ev[b] = esrc[g];
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int b = 0; b < SQ; b++)
+            if (ok[b]) {
+                hl[q0 + b * blockDim.x] = hfun(lv[b]);
+                he[q0 + b * blockDim.x] = hfun(ev[b]);
+            }
     }
     __syncthreads();
 #pragma unroll 2
@@ -1751,23 +1808,24 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         }
     } else {
     __syncthreads();
-    fft2d_128<-1>(sm, tw);
-    {
-        /* one pixel per lane along a row (no LDS bank conflicts); SB was
-         * written by other threads of this workgroup in P1, ordered before
-         * these loads by the transform's workgroup barriers */
-        float yy[16];
+    /* F(pi_xx + i pi_xy) straight from the last stage to SA, and pi_yy (SB,
+     * written by other threads in P1, ordered by the transform's barriers)
+     * into its place for the next transform */
+    fft2d_128<-1>(sm, tw, [&](const float2 *u, const int sub, const int j) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) yy[q] = ldb(SB, 4u * (unsigned)(t + 1024 * q));
-        __builtin_amdgcn_sched_barrier(0);
+        for (int q0 = 0; q0 < 16; q0 += 8) {      /* in halves: 8 loads in flight */
+            float yy[8];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = t + 1024 * q, l = (e >> 7) * ORG_LS + (e & 127);
-            SA[e] = sm[l];
-            sm[l] = make_float2(yy[q], 0.0f);
+            for (int q = 0; q < 8; q++) yy[q] = SB[fft_out_pos(sub, q0 + q) * ORG_MB + j];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int i = fft_out_pos(sub, q0 + q);
+                SA[i * ORG_MB + j] = u[q0 + q];
+                sm[i * ORG_LS + j] = make_float2(yy[q], 0.0f);
+            }
         }
-    }
-    __syncthreads();
+    });
     fft2d_128<-1>(sm, tw);
     /* P4: per pair {k, -k}: split F(pi_xx + i pi_xy), solve both modes
      * (rho_functions.cpp:133-190), keep the part of v whose inverse is real,
@@ -1824,19 +1882,28 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         }
     }
     __syncthreads();
-    fft2d_128<1>(sm, tw);
-    /* P5: v = Re(.)/M^2 ; rho * v (rho_functions.cpp:267-284) */
-#pragma unroll 4
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q, i = e >> 7, j = e & 127;
-        float2 z = sm[i * ORG_LS + j];
-        const float r = ldb(rh, 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + j)));
-        if (write_v) {
-            vxbox[(size_t)n * MM + e] = z.x / MMf;
-            vybox[(size_t)n * MM + e] = z.y / MMf;
+    /* P5, fused into the inverse transform's last stage: v = Re(.)/M^2 ;
+     * rho * v (rho_functions.cpp:267-284) */
+    fft2d_128<1>(sm, tw, [&](const float2 *u, const int sub, const int j) {
+#pragma unroll
+        for (int q0 = 0; q0 < 16; q0 += 8) {      /* in halves: 8 loads in flight */
+            float rr[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                rr[q] = ldb(rh, 4u * ((unsigned)(x0 + fft_out_pos(sub, q0 + q)) * (unsigned)N + (unsigned)(y0 + j)));
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int i = fft_out_pos(sub, q0 + q), e = i * ORG_MB + j;
+                const float2 z = u[q0 + q];
+                if (write_v) {
+                    vxbox[(size_t)n * MM + e] = z.x / MMf;
+                    vybox[(size_t)n * MM + e] = z.y / MMf;
+                }
+                sm[i * ORG_LS + j] = make_float2(rr[q] * z.x / MMf, rr[q] * z.y / MMf);
+            }
         }
-        sm[i * ORG_LS + j] = make_float2(r * z.x / MMf, r * z.y / MMf);
-    }
+    });
     }                     /* the solve (exact velocity) */
     __syncthreads();
     fft2d_128<-1>(sm, tw);
@@ -1864,13 +1931,20 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         if (lm != lk) sm[lm] = make_float2(H.x, -H.y);
     }
     __syncthreads();
-    fft2d_128<1>(sm, tw);
-    /* P7: rho^{n+1} on the box (copy_to_rhon, rho_functions.cpp:253-265) */
-#pragma unroll 4
-    for (int q = 0; q < 16; q++) {
-        const int e = t + 1024 * q;
-        rbox[e] = rbox[e] + sm[(e >> 7) * ORG_LS + (e & 127)].x / MMf;
-    }
+    /* P7, fused into the inverse transform's last stage: rho^{n+1} on the
+     * box (copy_to_rhon, rho_functions.cpp:253-265) */
+    fft2d_128<1>(sm, tw, [&](const float2 *u, const int sub, const int j) {
+#pragma unroll
+        for (int q0 = 0; q0 < 16; q0 += 8) {      /* in halves: 8 loads in flight */
+            float rb[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) rb[q] = rbox[fft_out_pos(sub, q0 + q) * ORG_MB + j];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                rbox[fft_out_pos(sub, q0 + q) * ORG_MB + j] = rb[q] + u[q0 + q].x / MMf;
+        }
+    });
 }
 
 __global__ __launch_bounds__(1024) void k_spectral(DevP P, const float *__restrict__ phi,
@@ -2455,8 +2529,8 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
     double *nxt = *a.lbrk_par ? a.lbrk0 : a.lbrk1;
     const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
     const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-    for (int k = blk; k <= ntl; k += nblk) {
-        bool need = full || k == ntl;
+    for (int k = blk; k < ntl + ORG_RING_PARTS; k += nblk) {
+        bool need = full || k >= ntl;
         if (!need) {
             const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
             for (int f = 0; f < 2 && !need; f++) {
@@ -2465,7 +2539,7 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
                        tj + ORG_HT > B.z - 2;
             }
         }
-        if (need) lbrk_part(P, k, FL, FE, nxt, (float *)(lds + 256));
+        if (need) lbrk_part(P, k, FL, FE, nxt, (float *)(lds + 256), ORG_RING_PARTS);
     }
 }
 static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int n, unsigned char *lds) {

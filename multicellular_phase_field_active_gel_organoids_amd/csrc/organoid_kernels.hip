@@ -1625,8 +1625,9 @@ static __device__ void commit_field(const DevP &P, const int f, const int4 B, co
  * fields, normals <- 0, every part of the h(lum)/h(ECM) pass and both
  * buffers of the lumen/ECM bracket.  Workgroup
  * k < ntiles: tile k; the last: the ring. */
-__global__ void k_live_reset(int *live, int *live_part, int nparts, int N) {
+__global__ void k_live_reset(int *live, int *live_part, int nparts, int N, int *ringh) {
     if (threadIdx.x < 8) live[threadIdx.x] = (threadIdx.x & 1) ? 0 : N;
+    if (threadIdx.x == 0) *ringh = 0;
     for (int q = threadIdx.x; q < 8 * nparts; q += blockDim.x) live_part[q] = (q & 1) ? -1 : N;
 }
 __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restrict__ lum,
@@ -2331,6 +2332,9 @@ struct StepArgs {
      * step's into lbrk[1 - par]; the copy-back flips par */
     double *lbrk0, *lbrk1;
     int *lbrk_par;
+    /* 1 once the ring part of the h(lum)/h(ECM) pass has been taken from a
+     * committed batched step (k_live_reset clears it) */
+    int *ringh;
     double *part_h, *sum_ecm;
     int nh;                  /* h(lum), h(ECM) workgroups of k_pre */
     int npsi;                /* psi workgroups of k_pre */
@@ -2672,11 +2676,16 @@ static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int 
     int *sb = (int *)scr;
     commit_field(P, 0, BL, blk, nblk, a.lum, FL, a.live_part, sb);
     commit_field(P, 1, BE, blk, nblk, a.ecm, FE, a.live_part, sb);
-    /* h of the new fields on every part the step may have changed */
+    /* h of the new fields on every part the step may have changed.  The
+     * ring: every batched step leaves the same values there (the boundary
+     * values, with the corners the sweeps write, all fixed by the
+     * parameters; box mode never follows full mode, as boxes only grow), so
+     * after the first copy-back since set-up its h values and its part of
+     * the sum stand */
     const int nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
     const bool full = !box_mode(BL, N) || !box_mode(BE, N);
     for (int k = blk; k <= ntl; k += nblk) {
-        bool need = full || k == ntl;
+        bool need = full || (k == ntl && *a.ringh == 0);
         if (!need) {
             const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
             for (int f = 0; f < 2 && !need; f++) {
@@ -2687,7 +2696,10 @@ static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int 
         }
         if (!need) continue;
         const double tot = block_sum_d(hlum_part(P, k, FL, FE, a.hlum, a.hecm), (double *)(scr + 64));
-        if (threadIdx.x == 0) a.part_h[k] = tot;
+        if (threadIdx.x == 0) {
+            a.part_h[k] = tot;
+            if (k == ntl) *a.ringh = 1;     /* read above before the sum's barriers */
+        }
     }
 }
 __global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {

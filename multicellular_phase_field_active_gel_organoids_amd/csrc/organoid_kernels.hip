@@ -1703,8 +1703,9 @@ static __device__ __forceinline__ void active_stress(const DevP &P, const int id
 /* k_spectral: update_all_cortices (rho_functions.cpp:409-471) for every cell.
  *
  * One workgroup (1024 threads) per cell, one 128 x 128 complex field in LDS,
- * a 128 x 128 complex and a 128 x 128 real scratch plane per cell in global
- * memory (L2 resident): F(pi_xx + i pi_xy), needed at k and -k, and pi_yy.
+ * a 128 x 128 complex and two 128 x 128 real scratch planes per cell in
+ * global memory (L2 resident): F(pi_xx + i pi_xy), needed at k and -k,
+ * pi_yy, and rho on the box.
  * By linearity of the reference's spectral update, Re(F^-1[F(rho) +
  * dt F(react) + D]) / M^2 = rho + dt*react + Re(F^-1 D) / M^2, so the
  * transforms of rho and of the reaction term (rho_functions.cpp:415-422,
@@ -1737,6 +1738,10 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
     float2 *SA = scratch + (size_t)n * 2 * MM;
     float *SB = (float *)(SA + MM);
+    /* rho on the box, dense (row pitch 128), written by P1 beside pi_yy:
+     * the rho * v epilogue reads it from here (recently written, in L2)
+     * rather than from the plane */
+    float *SR = SB + MM;
     float *rbox = rho_box + (size_t)n * MM;
     const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
     const float isig2 = 1.0f / (2.0f * P.sigma * P.sigma);
@@ -1791,7 +1796,10 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
                 rq[m] = r + P.dt * react;
             }
             *(unsigned *)(lab + e0) = labs;
-            if (!GIVEN) *(float4 *)(SB + e0) = make_float4(pyq[0], pyq[1], pyq[2], pyq[3]);
+            if (!GIVEN) {
+                *(float4 *)(SB + e0) = make_float4(pyq[0], pyq[1], pyq[2], pyq[3]);
+                *(float4 *)(SR + e0) = make_float4(fr[k][0], fr[k][1], fr[k][2], fr[k][3]);
+            }
             *(float4 *)(rbox + e0) = make_float4(rq[0], rq[1], rq[2], rq[3]);
         }
     }
@@ -1884,14 +1892,13 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
     }
     __syncthreads();
     /* P5, fused into the inverse transform's last stage: v = Re(.)/M^2 ;
-     * rho * v (rho_functions.cpp:267-284) */
+     * rho * v (rho_functions.cpp:267-284), rho from SR (P1) */
     fft2d_128<1>(sm, tw, [&](const float2 *u, const int sub, const int j) {
 #pragma unroll
         for (int q0 = 0; q0 < 16; q0 += 8) {      /* in halves: 8 loads in flight */
             float rr[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                rr[q] = ldb(rh, 4u * ((unsigned)(x0 + fft_out_pos(sub, q0 + q)) * (unsigned)N + (unsigned)(y0 + j)));
+            for (int q = 0; q < 8; q++) rr[q] = SR[fft_out_pos(sub, q0 + q) * ORG_MB + j];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < 8; q++) {

@@ -220,8 +220,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="organoid64_1024")
     ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=6,
-                    help="timesteps of the FP64 CPU oracle timed as cpu_baseline (about 2 s each)")
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="timesteps of the FP64 CPU oracle timed as cpu_baseline (1.4-2 s each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=160,
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")

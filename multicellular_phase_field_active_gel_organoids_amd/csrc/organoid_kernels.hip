@@ -460,21 +460,25 @@ static __device__ void centroid_sums_win(const float *__restrict__ ph, int N, in
     const int npx = (bw > 0 && bh > 0) ? bw * bh : 0;
     long long sx = 0, sy = 0, c = 0;
     if (npx > 0) {
-        /* 8 pixels per thread at a time: the plane loads unconditional
-         * (inside the box) and issued before any use; window pixels then
-         * take the LDS value instead */
+        /* 8 pixels per thread at a time: the plane loads only outside the
+         * window (the support box normally lies inside it, so most waves
+         * issue none) and issued before any use; window pixels take the
+         * LDS value */
         const int di = nt / bw, dj = nt - di * bw;
         int i = t / bw, j = t - i * bw;
-        const unsigned b0 = 4u * ((unsigned)i0 * (unsigned)N + (unsigned)j0);
 #pragma unroll 1
         for (int e0 = t; e0 < npx; e0 += 8 * nt) {
             int pi[8], pj[8];
+            bool in[8];
             float vg[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 pi[k] = i0 + i; pj[k] = j0 + j;
-                const unsigned g = 4u * ((unsigned)pi[k] * (unsigned)N + (unsigned)pj[k]);
-                vg[k] = ldb(ph, e0 + k * nt < npx ? g : b0);
+                const int wi = pi[k] - x0, wj = pj[k] - y0;
+                in[k] = wi >= 0 && wi < W && wj >= 0 && wj < W;
+                vg[k] = 0.0f;
+                if (!in[k] && e0 + k * nt < npx)
+                    vg[k] = ldb(ph, 4u * ((unsigned)pi[k] * (unsigned)N + (unsigned)pj[k]));
                 i += di; j += dj;
                 if (j >= bw) { j -= bw; i++; }
             }
@@ -483,8 +487,7 @@ static __device__ void centroid_sums_win(const float *__restrict__ ph, int N, in
             for (int k = 0; k < 8; k++) {
                 if (e0 + k * nt >= npx) break;
                 const int wi = pi[k] - x0, wj = pj[k] - y0;
-                const bool in = wi >= 0 && wi < W && wj >= 0 && wj < W;
-                const float v = in ? win[(wi + 1) * WH + (wj + 1)] : vg[k];
+                const float v = in[k] ? win[(wi + 1) * WH + (wj + 1)] : vg[k];
                 if (v > 0.5f) { sx += pi[k]; sy += pj[k]; c += 1; }
             }
         }

@@ -24,6 +24,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef ORC_REAL
 #define ORC_REAL double
@@ -35,6 +38,16 @@ typedef struct { real x, y; } cplx;
 #define M_PI 3.14159265358979323846
 #endif
 
+/* The eleven FFT-box buffers of declare_fft_tables.cpp:10-41 plus the
+ * transform's double work array.  The reference holds one set and loops over
+ * cells serially (rho_functions.cpp:413-467); here each OpenMP thread owns a
+ * set, so cells run in parallel with the same per-cell arithmetic. */
+typedef struct {
+    cplx *temp_fft, *temp_fft_real, *fft_rho, *fft_react, *fft_pi_xx, *fft_pi_xy,
+         *fft_pi_yy, *fft_vx, *fft_vy, *fft_rho_vx, *fft_rho_vy;
+    double *fftwork;
+} orc_spec;
+
 struct orc_state {
     org_params p;
     int N, nc, zs, M;
@@ -45,10 +58,9 @@ struct orc_state {
     /* shared planes */
     real *lum, *lumn, *ECM, *ECMn, *psi, *hlum, *hECM, *nglum, *Jlx, *Jly;
     /* FFT-box buffers (declare_fft_tables.cpp:10-41) */
-    cplx *temp_fft, *temp_fft_real, *fft_rho, *fft_react, *fft_pi_xx, *fft_pi_xy,
-         *fft_pi_yy, *fft_vx, *fft_vy, *fft_rho_vx, *fft_rho_vy;
+    orc_spec *spec;          /* one FFT-box workspace per OpenMP thread */
+    int nspec;
     real *vx_box, *vy_box;
-    double *fftwork;
     real *f0;
     double *s, *s0_cell;
     int32_t *xc, *yc, *count_ri;
@@ -168,9 +180,9 @@ int orc_fft2d(int M, double *data, int sign) {
 /* cufftExecC2C(plan, in, out, dir) as called by compute_fft_forward /
  * compute_fft_backward (useful_functions.cpp:646-660): out-of-place,
  * unnormalised.  Computed in double, stored in the buffer type. */
-static void exec_c2c(orc_state *st, const cplx *in, cplx *out, int sign) {
+static void exec_c2c(const orc_state *st, orc_spec *sp, const cplx *in, cplx *out, int sign) {
     int M = st->M;
-    double *w = st->fftwork;
+    double *w = sp->fftwork;
     for (int k = 0; k < M * M; k++) { w[2 * k] = in[k].x; w[2 * k + 1] = in[k].y; }
     fft2d_inplace(M, w, sign);
     for (int k = 0; k < M * M; k++) { out[k].x = (real)w[2 * k]; out[k].y = (real)w[2 * k + 1]; }
@@ -196,13 +208,22 @@ orc_state *orc_create(const org_params *p, const double *s0_cell) {
     real **gp[] = {&st->lum, &st->lumn, &st->ECM, &st->ECMn, &st->psi, &st->hlum, &st->hECM,
                    &st->nglum, &st->Jlx, &st->Jly};
     for (size_t k = 0; k < sizeof(gp) / sizeof(gp[0]); k++) *gp[k] = (real *)xcalloc(NN, sizeof(real));
-    cplx **fp[] = {&st->temp_fft, &st->temp_fft_real, &st->fft_rho, &st->fft_react, &st->fft_pi_xx,
-                   &st->fft_pi_xy, &st->fft_pi_yy, &st->fft_vx, &st->fft_vy, &st->fft_rho_vx,
-                   &st->fft_rho_vy};
-    for (size_t k = 0; k < sizeof(fp) / sizeof(fp[0]); k++) *fp[k] = (cplx *)xcalloc(MM, sizeof(cplx));
+#ifdef _OPENMP
+    st->nspec = omp_get_max_threads();
+#else
+    st->nspec = 1;
+#endif
+    st->spec = (orc_spec *)xcalloc((size_t)st->nspec, sizeof(orc_spec));
+    for (int t = 0; t < st->nspec; t++) {
+        orc_spec *sp = &st->spec[t];
+        cplx **fp[] = {&sp->temp_fft, &sp->temp_fft_real, &sp->fft_rho, &sp->fft_react, &sp->fft_pi_xx,
+                       &sp->fft_pi_xy, &sp->fft_pi_yy, &sp->fft_vx, &sp->fft_vy, &sp->fft_rho_vx,
+                       &sp->fft_rho_vy};
+        for (size_t k = 0; k < sizeof(fp) / sizeof(fp[0]); k++) *fp[k] = (cplx *)xcalloc(MM, sizeof(cplx));
+        sp->fftwork = (double *)xcalloc(2 * MM, sizeof(double));
+    }
     st->vx_box = (real *)xcalloc(MM * st->nc, sizeof(real));
     st->vy_box = (real *)xcalloc(MM * st->nc, sizeof(real));
-    st->fftwork = (double *)xcalloc(2 * MM, sizeof(double));
     st->f0 = (real *)xcalloc(st->nc, sizeof(real));
     st->s = (double *)xcalloc(st->nc, sizeof(double));
     st->s0_cell = (double *)xcalloc(st->nc, sizeof(double));
@@ -222,10 +243,14 @@ void orc_destroy(orc_state *st) {
                      st->psi, st->hlum, st->hECM, st->nglum, st->Jlx, st->Jly, st->vx_box,
                      st->vy_box, st->f0};
     for (size_t k = 0; k < sizeof(cellp) / sizeof(cellp[0]); k++) free(cellp[k]);
-    cplx *fp[] = {st->temp_fft, st->temp_fft_real, st->fft_rho, st->fft_react, st->fft_pi_xx,
-                  st->fft_pi_xy, st->fft_pi_yy, st->fft_vx, st->fft_vy, st->fft_rho_vx, st->fft_rho_vy};
-    for (size_t k = 0; k < sizeof(fp) / sizeof(fp[0]); k++) free(fp[k]);
-    free(st->fftwork); free(st->s); free(st->s0_cell); free(st->xc); free(st->yc); free(st->count_ri);
+    for (int t = 0; t < st->nspec; t++) {
+        orc_spec *sp = &st->spec[t];
+        cplx *fp[] = {sp->temp_fft, sp->temp_fft_real, sp->fft_rho, sp->fft_react, sp->fft_pi_xx,
+                      sp->fft_pi_xy, sp->fft_pi_yy, sp->fft_vx, sp->fft_vy, sp->fft_rho_vx, sp->fft_rho_vy};
+        for (size_t k = 0; k < sizeof(fp) / sizeof(fp[0]); k++) free(fp[k]);
+        free(sp->fftwork);
+    }
+    free(st->spec); free(st->s); free(st->s0_cell); free(st->xc); free(st->yc); free(st->count_ri);
     free(st);
 }
 
@@ -331,16 +356,18 @@ static int check_windows(orc_state *st) {
 /* find_centres_cells, useful_functions.cpp:193-227: integer sums of the row
  * and column indices of phi>0.5 pixels, truncating division. */
 int orc_find_centres_cells(orc_state *st) {
-    int N = st->N;
+    int N = st->N, empty = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(|: empty)
     for (int n = 0; n < st->nc; n++) {
         long long xs = 0, ys = 0, cnt = 0;
         for (int i = 0; i < N; i++)
             for (int j = 0; j < N; j++)
                 if (st->phi[PL(n, i, j)] > 0.5) { xs += i; ys += j; cnt += 1; }
-        if (cnt == 0) return set_err(st, ORG_EEMPTY_CELL);
+        if (cnt == 0) { empty = 1; continue; }
         st->xc[n] = (int32_t)(xs / cnt);
         st->yc[n] = (int32_t)(ys / cnt);
     }
+    if (empty) return set_err(st, ORG_EEMPTY_CELL);
     return check_windows(st);
 }
 
@@ -348,6 +375,7 @@ int orc_find_centres_cells(orc_state *st) {
 int orc_compute_hfunc_cells(orc_state *st) {
     if (check_windows(st)) return st->err;
     int zs = st->zs;
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < st->nc; n++)
         for (int i = st->xc[n] - zs; i < st->xc[n] + zs; i++)
             for (int j = st->yc[n] - zs; j < st->yc[n] + zs; j++) {
@@ -359,6 +387,7 @@ int orc_compute_hfunc_cells(orc_state *st) {
 
 /* compute_all_hfunc_2d_tables_kernels, useful_functions.cpp:272-285 */
 int orc_compute_hfunc_lum_ecm(orc_state *st) {
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < st->NN; k++) {
         real a = st->lum[k], e = st->ECM[k];
         st->hlum[k] = a * a * (3.0 - 2.0 * a);
@@ -371,6 +400,7 @@ int orc_compute_hfunc_lum_ecm(orc_state *st) {
  * ring with period N-2 on every cell plane. */
 static void bc_cells(int N, int nplanes, real *t) {
     size_t NN = (size_t)N * N;
+#pragma omp parallel for schedule(static) if (nplanes > 1)
     for (int n = 0; n < nplanes; n++) {
         real *q = t + (size_t)n * NN;
         for (int i = 1; i < N - 1; i++) {
@@ -401,6 +431,7 @@ int orc_bc_cells_planes(int N, int nplanes, double *planes) {
 int orc_compute_normal_grad_phi(orc_state *st) {
     int N = st->N;
     real dx = (real)st->p.dx;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int n = 0; n < st->nc; n++)
         for (int i = 1; i < N - 1; i++)
             for (int j = 1; j < N - 1; j++) {
@@ -414,6 +445,7 @@ int orc_compute_normal_grad_phi(orc_state *st) {
 
 /* compute_psi_kernels, useful_functions.cpp:305-322 */
 int orc_compute_psi(orc_state *st) {
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < st->NN; k++) {
         real acc = 0.0;
         for (int n = 0; n < st->nc; n++) acc += st->hfunc[(size_t)n * st->NN + k];
@@ -426,6 +458,7 @@ int orc_compute_psi(orc_state *st) {
  * the whole plane (double accumulation), f0 = alpha*(s0 - s). */
 int orc_sum_s(orc_state *st) {
     real dx = (real)st->p.dx;
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < st->nc; n++) {
         double acc = 0.0;
         const real *h = st->hfunc + (size_t)n * st->NN;
@@ -441,6 +474,7 @@ int orc_sum_s(orc_state *st) {
 int orc_compute_normal_to_phi(orc_state *st) {
     int N = st->N;
     real dx = (real)st->p.dx;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int n = 0; n < st->nc; n++)
         for (int i = 1; i < N - 1; i++)
             for (int j = 1; j < N - 1; j++) {
@@ -467,6 +501,7 @@ int orc_create_turnover_rates_tensions(orc_state *st) {
     if (check_windows(st)) return st->err;
     const org_params *p = &st->p;
     int zs = st->zs, nc = st->nc;
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < nc; n++) {
         int n1, n2;
         if (n == 0) { n1 = nc - 1; n2 = n + 1; }
@@ -507,6 +542,7 @@ int orc_compute_prod_norm_delta(orc_state *st) {
     if (check_windows(st)) return st->err;
     int zs = st->zs;
     real sigma = (real)st->p.sigma;
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < st->nc; n++)
         for (int i = st->xc[n] - zs; i < st->xc[n] + zs; i++)
             for (int j = st->yc[n] - zs; j < st->yc[n] + zs; j++) {
@@ -591,7 +627,7 @@ static void mode_wavenumbers(int M, real dx, int i, int j, real *kx, real *ky) {
 
 /* update_rho_velocity, rho_functions.cpp:133-190.  Viscosity of mode (i,j)
  * is read at the real-space pixel (i+1+xc-zs, j+1+yc-zs) (:149-150). */
-static void update_rho_velocity(orc_state *st, int n) {
+static void update_rho_velocity(orc_state *st, orc_spec *sp, int n) {
     int M = st->M, zs = st->zs;
     real dx = (real)st->p.dx, xi = (real)st->p.friction;
     for (int i = 0; i < M; i++)
@@ -599,8 +635,8 @@ static void update_rho_velocity(orc_state *st, int n) {
             real kx, ky;
             mode_wavenumbers(M, dx, i, j, &kx, &ky);
             real eta = st->visc[PL(n, i + 1 + st->xc[n] - zs, j + 1 + st->yc[n] - zs)];
-            velocity_mode(eta, xi, kx, ky, st->fft_pi_xx[i * M + j], st->fft_pi_xy[i * M + j],
-                          st->fft_pi_yy[i * M + j], &st->fft_vx[i * M + j], &st->fft_vy[i * M + j]);
+            velocity_mode(eta, xi, kx, ky, sp->fft_pi_xx[i * M + j], sp->fft_pi_xy[i * M + j],
+                          sp->fft_pi_yy[i * M + j], &sp->fft_vx[i * M + j], &sp->fft_vy[i * M + j]);
         }
 }
 
@@ -635,20 +671,20 @@ static void load_fft_rho_v(orc_state *st, int n, cplx *t, const cplx *treal, rea
 }
 
 /* update_fft_rho, rho_functions.cpp:213-251 (forward Euler, no diffusion). */
-static void update_fft_rho(orc_state *st) {
+static void update_fft_rho(orc_state *st, orc_spec *sp) {
     int M = st->M;
     real dt = (real)st->p.dt, dx = (real)st->p.dx;
     for (int k = 0; k < M * M; k++) {
-        st->fft_rho[k].x += dt * st->fft_react[k].x;                  /* :222 */
-        st->fft_rho[k].y += dt * st->fft_react[k].y;                  /* :223 */
+        sp->fft_rho[k].x += dt * sp->fft_react[k].x;                  /* :222 */
+        sp->fft_rho[k].y += dt * sp->fft_react[k].y;                  /* :223 */
     }
     for (int i = 0; i < M; i++)
         for (int j = 0; j < M; j++) {
             int ir = (i < M / 2) ? i : i - M;
             int jr = (j < M / 2) ? j : j - M;
             int k = i * M + j;
-            st->fft_rho[k].x += dt * (2.0 * M_PI / (M * dx)) * (ir * st->fft_rho_vx[k].y + jr * st->fft_rho_vy[k].y);  /* :243 */
-            st->fft_rho[k].y -= dt * (2.0 * M_PI / (M * dx)) * (ir * st->fft_rho_vx[k].x + jr * st->fft_rho_vy[k].x);  /* :247 */
+            sp->fft_rho[k].x += dt * (2.0 * M_PI / (M * dx)) * (ir * sp->fft_rho_vx[k].y + jr * sp->fft_rho_vy[k].y);  /* :243 */
+            sp->fft_rho[k].y -= dt * (2.0 * M_PI / (M * dx)) * (ir * sp->fft_rho_vx[k].x + jr * sp->fft_rho_vy[k].x);  /* :247 */
         }
 }
 
@@ -663,28 +699,36 @@ static void copy_to_rhon(orc_state *st, int n, const cplx *t) {
 /* update_all_cortices, rho_functions.cpp:409-471 */
 int orc_update_all_cortices(orc_state *st) {
     if (check_windows(st)) return st->err;
+    /* cells are independent: each thread runs whole cells on its own set of
+     * FFT-box buffers, in the reference's per-cell order */
+#pragma omp parallel for schedule(dynamic, 1)
     for (int nc = 0; nc < st->nc; nc++) {
-        load_temp_fft(st, nc, st->rho, st->temp_fft);                 /* :415 */
-        exec_c2c(st, st->temp_fft, st->fft_rho, -1);                   /* :417 */
-        create_deg_gen(st, nc, st->temp_fft);                          /* :420 */
-        exec_c2c(st, st->temp_fft, st->fft_react, -1);                 /* :422 */
+#ifdef _OPENMP
+        orc_spec *sp = &st->spec[omp_get_thread_num()];
+#else
+        orc_spec *sp = &st->spec[0];
+#endif
+        load_temp_fft(st, nc, st->rho, sp->temp_fft);                  /* :415 */
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_rho, -1);               /* :417 */
+        create_deg_gen(st, nc, sp->temp_fft);                          /* :420 */
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_react, -1);             /* :422 */
         compute_pi(st, nc);                                            /* :428-430 */
-        load_temp_fft(st, nc, st->pixx, st->temp_fft);                 /* :434 */
-        exec_c2c(st, st->temp_fft, st->fft_pi_xx, -1);
-        load_temp_fft(st, nc, st->pixy, st->temp_fft);                 /* :437 */
-        exec_c2c(st, st->temp_fft, st->fft_pi_xy, -1);
-        load_temp_fft(st, nc, st->piyy, st->temp_fft);                 /* :440 */
-        exec_c2c(st, st->temp_fft, st->fft_pi_yy, -1);
-        update_rho_velocity(st, nc);                                   /* :445 */
-        exec_c2c(st, st->fft_vx, st->temp_fft_real, +1);               /* :449 */
-        load_fft_rho_v(st, nc, st->temp_fft, st->temp_fft_real, st->vx_box);
-        exec_c2c(st, st->temp_fft, st->fft_rho_vx, -1);                /* :451 */
-        exec_c2c(st, st->fft_vy, st->temp_fft_real, +1);               /* :455 */
-        load_fft_rho_v(st, nc, st->temp_fft, st->temp_fft_real, st->vy_box);
-        exec_c2c(st, st->temp_fft, st->fft_rho_vy, -1);                /* :457 */
-        update_fft_rho(st);                                            /* :462 */
-        exec_c2c(st, st->fft_rho, st->temp_fft, +1);                   /* :464 */
-        copy_to_rhon(st, nc, st->temp_fft);                            /* :466 */
+        load_temp_fft(st, nc, st->pixx, sp->temp_fft);                 /* :434 */
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_pi_xx, -1);
+        load_temp_fft(st, nc, st->pixy, sp->temp_fft);                 /* :437 */
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_pi_xy, -1);
+        load_temp_fft(st, nc, st->piyy, sp->temp_fft);                 /* :440 */
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_pi_yy, -1);
+        update_rho_velocity(st, sp, nc);                               /* :445 */
+        exec_c2c(st, sp, sp->fft_vx, sp->temp_fft_real, +1);           /* :449 */
+        load_fft_rho_v(st, nc, sp->temp_fft, sp->temp_fft_real, st->vx_box);
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_rho_vx, -1);            /* :451 */
+        exec_c2c(st, sp, sp->fft_vy, sp->temp_fft_real, +1);           /* :455 */
+        load_fft_rho_v(st, nc, sp->temp_fft, sp->temp_fft_real, st->vy_box);
+        exec_c2c(st, sp, sp->temp_fft, sp->fft_rho_vy, -1);            /* :457 */
+        update_fft_rho(st, sp);                                        /* :462 */
+        exec_c2c(st, sp, sp->fft_rho, sp->temp_fft, +1);               /* :464 */
+        copy_to_rhon(st, nc, sp->temp_fft);                            /* :466 */
     }
     bc_cells(st->N, st->nc, st->rhon);                                 /* :469 */
     return ORG_OK;
@@ -711,6 +755,7 @@ int orc_update_lumen_and_ECM(orc_state *st) {
     real tau_E = (real)p->tau_ECM, alpha_E = (real)p->alpha_ECM, xi_E = (real)p->xi_ECM;
     real beta_ec = (real)p->beta_ECM_cells, eta_ec = (real)p->eta_ECM_cells;
     const real *psi = st->psi;
+#pragma omp parallel for schedule(static)
     for (int i = 1; i < N - 1; i++)
         for (int j = 1; j < N - 1; j++) {
             size_t q = G(i, j);
@@ -795,12 +840,14 @@ int orc_reshaping_lumen_or_ECM(orc_state *st, int which_ecm) {
     real *ng = st->nglum, *jx = st->Jlx, *jy = st->Jly;
     double rdev = (double)(p->rdev_ini * rtol * rdt);                  /* :720 */
     double thr = (double)(p->rdev_threshold * rtol * rdt);             /* :774 */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i < N - 1; i++)
         for (int j = 1; j < N - 1; j++) {
             real a = lum[G(i + 1, j)] - lum[G(i - 1, j)], c = lum[G(i, j + 1)] - lum[G(i, j - 1)];
             ng[G(i, j)] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));  /* :734-736 */
         }
     bc_periodic(N, ng);                                                /* :741 */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i < N - 1; i++)
         for (int j = 1; j < N - 1; j++) {
             real g = ng[G(i, j)];
@@ -814,6 +861,7 @@ int orc_reshaping_lumen_or_ECM(orc_state *st, int which_ecm) {
     int ri = 0, count = 0;
     while (rdev > thr) {                                               /* :774 */
         memcpy(lum, lumn, st->NN * sizeof(real));                      /* :777-785 */
+#pragma omp parallel for schedule(static)
         for (int i = 1; i < N - 1; i++)
             for (int j = 1; j < N - 1; j++)
                 lumn[G(i, j)] = reshape_update(lumn[G(i, j)], lum, jx, jy, N, i, j, wi, rdt, dx);
@@ -838,6 +886,7 @@ int orc_update_all_cells(orc_state *st) {
     real cpl = (real)p->coupling_PF_AG, beta = (real)p->beta, eta = (real)p->eta;
     real beta_lc = (real)p->beta_lum_cells, eta_lc = (real)p->eta_lum_cells;
     real beta_ec = (real)p->beta_ECM_cells, eta_ec = (real)p->eta_ECM_cells;
+#pragma omp parallel for schedule(dynamic, 1)
     for (int n = 0; n < st->nc; n++) {
         real *tp = st->temp_phase + (size_t)n * st->NN;
         const real *ph = st->phi + (size_t)n * st->NN;
@@ -883,6 +932,8 @@ int orc_reshaping_cells(orc_state *st) {
     int N = st->N, zs = st->zs;
     real dx = (real)p->dx, wi = (real)p->wi, rdt = (real)p->rdt, rtol = (real)p->rtol, R = (real)p->R;
     double thr = (double)(rtol * 2.0 * M_PI * R * wi * rdt);           /* :475 */
+    int relax_fail = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(|: relax_fail)
     for (int n = 0; n < st->nc; n++) {
         real *ph = st->phi + (size_t)n * st->NN;
         real *pn = st->phin + (size_t)n * st->NN;
@@ -901,11 +952,12 @@ int orc_reshaping_cells(orc_state *st) {
                 for (int j = y0; j < y1; j++)
                     rdev += (double)(fabs(pn[G(i, j)] - ph[G(i, j)]) * dx * dx / p->n_current);
             count++;
-            if (ri > p->rimax) { set_err(st, ORG_ERELAX); break; }      /* :565-568 */
+            if (ri > p->rimax) { relax_fail = 1; break; }             /* :565-568 */
             ri++;
         }
         st->count_ri[n] = count;
     }
+    if (relax_fail) set_err(st, ORG_ERELAX);
     return st->err;
 }
 

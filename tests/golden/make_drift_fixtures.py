@@ -30,11 +30,13 @@ how far FP32 runs that differ only in rounding separate from the FP64 one:
 are the largest values over the ensemble and the unperturbed FP32 run.
 
 Usage:  python tests/golden/make_drift_fixtures.py [--f32-only | --ensemble] [case ...]
-(cases: two_cells_256, organoid16_512; all by default, run in parallel;
---f32-only adds the FP32 leg, --ensemble the ensemble, to existing
-fixtures).  Runtime on one core: about 5 + 5 + 4 x 5 min (two_cells_256,
-10^4 steps) and 18 + 25 + 3 x 20 min (organoid16_512, 5000 steps), the
-ensemble members in parallel.
+(cases: two_cells_256, organoid16_512, organoid64_1024; all by default, run
+in parallel; --f32-only adds the FP32 leg, --ensemble the ensemble, to
+existing fixtures).  Runtime on one core: about 5 + 5 + 4 x 5 min
+(two_cells_256, 10^4 steps) and 18 + 25 + 3 x 20 min (organoid16_512, 5000
+steps), the ensemble members in parallel.  organoid64_1024 (64 cells on
+1024^2, 10^4 steps, no ensemble) needs the oracle's OpenMP threads
+(OMP_NUM_THREADS): about 1 h for each leg on 8 cores.
 """
 from __future__ import annotations
 
@@ -53,6 +55,7 @@ sys.path.insert(0, ROOT)
 CASES = {
     "two_cells_256": ((1000, 10000), True),
     "organoid16_512": ((5000,), False),
+    "organoid64_1024": ((1000, 10000), False),
 }
 
 
@@ -153,7 +156,7 @@ def run_f32(case: str) -> str:
 
 
 # ensemble members per case (seeds 1..k)
-ENSEMBLE = {"two_cells_256": 4, "organoid16_512": 3}
+ENSEMBLE = {"two_cells_256": 4, "organoid16_512": 3, "organoid64_1024": 0}
 
 
 def perturbed_workload(case: str, seed: int):
@@ -272,8 +275,12 @@ if __name__ == "__main__":
                 print("wrote", p, os.path.getsize(p), "bytes", flush=True)
     if ens_only or not f32_only:
         jobs = [(c, k) for c in cases for k in range(1, ENSEMBLE[c] + 1)]
-        with mp.Pool(len(jobs)) as pool:
-            members = pool.map(run_member, jobs)
+        members = []
+        if jobs:
+            with mp.Pool(len(jobs)) as pool:
+                members = pool.map(run_member, jobs)
         for c in cases:
+            if not ENSEMBLE[c]:
+                continue
             p = add_ensemble(c, [m for m in members if m["case"] == c])
             print("wrote", p, os.path.getsize(p), "bytes", flush=True)

@@ -42,7 +42,11 @@ enum {
                              interior; the reference reads out of range
                              ("careful here", rho_functions.cpp:43)             */
     ORG_ERELAX = 5,       /* a reshaping loop passed rimax ("Relaxation
-                             failure!", cell_phase_functions.cpp:565, 862)     */
+                             failure!", cell_phase_functions.cpp:565, 862).
+                             A warning, as in the reference: the loop stops,
+                             stepping goes on, the call returns ORG_OK and
+                             org_get_relax_failures counts it.  Only the
+                             oracle and org_error_string use the code.       */
     ORG_ENOTBUILT = 6,    /* kernels for this build are missing                 */
     ORG_ESTATE = 7,       /* call order / state error                           */
     ORG_ETIMEOUT = 8      /* a wait between workgroups of one launch passed its
@@ -134,8 +138,11 @@ int org_get_iterations(org_ctx *ctx, int32_t *cells, int32_t *lum, int32_t *ecm)
 
 /* Scan the uploaded phi/hfunc once to seed each cell's support box (the
  * rectangle outside which phi<=0.5 and hfunc==0; both fields only change
- * inside the cell's windows afterwards).  Also zeroes psi, copies
- * rho->rhon, lum->lumn, ECM->ECMn (run_with_active_gels.cpp:92-98). */
+ * inside the cell's windows afterwards).  Also zeroes psi and copies
+ * lum->lumn, ECM->ECMn (run_with_active_gels.cpp:92-95).  rhon and phin
+ * need no copy (run_with_active_gels.cpp:96-98): rhon lives only on each
+ * cell's FFT box and is committed over rho there, phin only on each
+ * cell's window, so outside them both equal rho and phi by construction. */
 int org_prepare(org_ctx *ctx);
 
 /* ---- per-function entry points (the plain drop-in form) -------------------
@@ -287,6 +294,11 @@ int org_set_option(org_ctx *ctx, int option, int value);
  * 2D DFT through the LDS transform used by org_update_all_cortices.  data:
  * 128*128 interleaved (re, im) floats on the host, sign -1 or +1. */
 int org_debug_fft2d(float *data, int sign);
+
+/* Calls on this context in which a reshaping loop passed rimax (the
+ * reference prints "Relaxation failure!" and goes on,
+ * cell_phase_functions.cpp:565-568, 862-865). */
+int org_get_relax_failures(org_ctx *ctx, int64_t *count);
 
 /* Sticky first error and a readable message for any code. */
 int org_last_error(void);

@@ -152,6 +152,44 @@ def _hex_workload(name, N, ncell, r_lo, r_hi, seed, steps, eps=2.0, disc=None, s
     return Workload(name, p, phi, rho, lum, ecm, centres, radii, s0_cell, steps)
 
 
+def label_constants(p: OrgParams) -> OrgParams:
+    """Distinct apical / basal / lateral constants (each label's own deg, gen,
+    a_ort, a_tan != a_ort, b and viscosity) around the template values, so
+    that the turnover labels of create_turnover_rates_tensions
+    (rho_functions.cpp:286-405) and the viscosity read at the mode's pixel
+    (rho_functions.cpp:149-150, 170-186) change the results."""
+    p.visc_ap, p.visc_bas, p.visc_lat = 0.5, 1.0, 2.0
+    p.a_ort_ap, p.a_ort_bas, p.a_ort_lat = 80.0, 100.0, 120.0
+    p.a_tan_ap, p.a_tan_bas, p.a_tan_lat = 80.0 * 1.25, 100.0 * 0.8, 120.0 * 1.5
+    p.b_ap, p.b_bas, p.b_lat = 0.8, 1.0, 1.3
+    p.gen_ap, p.gen_bas, p.gen_lat = 1.0, 1.5, 0.7
+    p.deg_ap, p.deg_bas, p.deg_lat = 100.0, 120.0, 80.0
+    return p
+
+
+def _cyst_workload(name, N, ncell, R, seed, steps, eps=2.0):
+    """Cells on a ring around a lumen, as the reference's initial state is
+    built: ring radius rcyst = ncell * 0.9 * R / (pi dx) about (N/2, N/2) at
+    angles 2 pi n / ncell (initialisation_functions.cpp:117-159, so ring
+    neighbours are index neighbours), and where 1 - sum phi > 0 the lumen
+    takes 1 - sum phi inside rcyst and the ECM outside
+    (initialisation_functions.cpp:74-114).  The cell profile is this
+    module's tanh disc (eps = 2) with rho on the band."""
+    rng = SplitMix64(seed)
+    rc = ncell * 0.9 * R / math.pi
+    ang = 2.0 * math.pi * np.arange(ncell) / ncell
+    centres = np.stack([N // 2 + rc * np.cos(ang), N // 2 + rc * np.sin(ang)], axis=1)
+    radii = np.full(ncell, float(R))
+    phi, rho, _, _ = _disc_fields(N, centres, radii, eps, rng)
+    free = 1.0 - phi.sum(axis=0)
+    ii, jj = np.mgrid[0:N, 0:N]
+    d = np.sqrt((ii - N // 2) ** 2.0 + (jj - N // 2) ** 2.0)
+    lum = np.where((free > 0.0) & (d < rc), free, 0.0)
+    ecm = np.where((free > 0.0) & (d >= rc), free, 0.0)
+    p = grid_units(N, ncell, float(R), eps=eps)
+    return Workload(name, p, phi, rho, lum, ecm, centres, radii, None, steps)
+
+
 def make_workload(name: str, N: int | None = None, ncell: int | None = None, steps: int | None = None) -> Workload:
     """Build one of the named workloads.
 
@@ -161,6 +199,9 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
     ``organoid256_2048``  256 cells, radii U(14,26), jittered hex packing with
                           spacing 2 x 26 (no overlap at the start), 2048^2
     ``hex``               custom N / ncell hex packing with R=20 (tests)
+    ``cyst6_256``         6 cells R=20 on a ring around a lumen, 256^2, with
+                          distinct per-label constants (``label_constants``)
+                          and lumen-cell coupling eta_lum_cells = 0.002
     """
     if name == "two_cells_256":
         n_ = N or 256
@@ -184,6 +225,11 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
         return _hex_workload(name, N or 2048, ncell or 256, 14.0, 26.0, 1004, steps or 20000, spacing=52.0)
     if name == "hex":
         return _hex_workload(name, N or 256, ncell or 4, 20.0, 20.0, 1099, steps or 10)
+    if name == "cyst6_256":
+        wl = _cyst_workload(name, N or 256, ncell or 6, 20.0, 1005, steps or 100)
+        label_constants(wl.params)
+        wl.params.eta_lum_cells = 0.002
+        return wl
     raise ValueError(f"unknown workload {name!r}")
 
 

@@ -83,6 +83,7 @@ struct org_ctx {
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
+    int64_t relax_failures = 0;            /* calls in which a reshaping loop passed rimax */
     /* velocity surrogate: network input and output per cell, per-cell switch,
      * per-step judge residuals [cap][ncell] and exact-cell counts [cap] */
     float *sur_x = nullptr, *sur_v = nullptr, *sur_res = nullptr;
@@ -198,12 +199,27 @@ int need_prepared(org_ctx *c) {
     return ORG_OK;
 }
 
-/* Read the device error bitmask; map the lowest set code. */
+/* A reshaping loop that passed rimax is a warning, as in the reference: it
+ * prints "Relaxation failure!", leaves the loop and keeps simulating
+ * (cell_phase_functions.cpp:565-568, 862-865).  Counted per context. */
+void note_relax_failure(org_ctx *c) {
+    c->relax_failures++;
+    fprintf(stderr, "organoid: Relaxation failure! (a reshaping loop passed rimax = %d; %lld so far)\n",
+            c->p.rimax, (long long)c->relax_failures);
+}
+
+/* Read the device error bitmask, clear it for the next call, and map the
+ * lowest set code other than ORG_ERELAX (a warning, see above). */
 int harvest_device_error(org_ctx *c) {
     int bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!bits) return ORG_OK;
+    HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int), c->stream));
+    if (bits & (1 << ORG_ERELAX)) {
+        note_relax_failure(c);
+        bits &= ~(1 << ORG_ERELAX);
+    }
     for (int code = 1; code < 32; code++)
         if (bits & (1 << code)) {
             fprintf(stderr, "organoid: device reported: %s\n", org_error_string(code));
@@ -370,7 +386,7 @@ int run_lum_loop(org_ctx *c, int fbase, int nf, int chunk) {
         bool done = true;
         for (int f = fbase; f < fbase + nf; f++) done = done && st[f].done;
         if (done) return ORG_OK;
-        if (it > (long long)c->p.rimax + 4) return note_error(ORG_ERELAX);
+        if (it > (long long)c->p.rimax + 4) { note_relax_failure(c); return ORG_OK; }
     }
 }
 
@@ -447,6 +463,13 @@ int batch_harvest(org_batch *b) {
 extern "C" {
 
 int org_last_error(void) { return g_last_error.load(); }
+
+int org_get_relax_failures(org_ctx *c, int64_t *count) {
+    TRY(check_ctx(c));
+    if (!count) return note_error(ORG_EINVAL);
+    *count = c->relax_failures;
+    return ORG_OK;
+}
 
 const char *org_error_string(int code) {
     switch (code) {
@@ -671,6 +694,15 @@ int org_prepare(org_ctx *c) {
     HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
     HIPCHK(hipMemsetAsync(c->lum_bar, 0, sizeof(unsigned), c->stream));
     HIPCHK(hipMemsetAsync(c->lum_abort, 0, sizeof(int), c->stream));
+    /* lumn <- lum, ECMn <- ECM (run_with_active_gels.cpp:92-95): both
+     * ping-pong buffers of each field start as the uploaded field, and the
+     * loop state points at the first, so ORG_F_LUMN / ORG_F_ECMN read before
+     * the first update return the uploaded fields */
+    HIPCHK(hipMemsetAsync(c->lst, 0, 2 * sizeof(LumState), c->stream));
+    HIPCHK(hipMemcpyAsync(c->bufL0, c->lum, sizeof(float) * c->NN, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bufL1, c->lum, sizeof(float) * c->NN, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bufE0, c->ecm, sizeof(float) * c->NN, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bufE1, c->ecm, sizeof(float) * c->NN, hipMemcpyDeviceToDevice, c->stream));
     {
         const int u[4] = {c->N, 0, c->N, 0};
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_set_tuning.argtypes = [vp, vp]
     lib.org_surrogate_samples.argtypes = [vp, i32, i32, vp, vp, vp]
     lib.org_step_surrogate.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, vp, vp]
+    lib.org_get_relax_failures.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.org_error_string.restype = ctypes.c_char_p
     lib.org_error_string.argtypes = [i32]
     lib.org_build_info.restype = ctypes.c_char_p
@@ -195,6 +196,13 @@ class Organoid:
         _chk(self.lib.org_get_iterations(self.h, _p(cells), ctypes.byref(lum), ctypes.byref(ecm)),
              "org_get_iterations")
         return cells, lum.value, ecm.value
+
+    def relax_failures(self) -> int:
+        """Calls in which a reshaping loop passed rimax (a warning, as the
+        reference's "Relaxation failure!", cell_phase_functions.cpp:565, 862)."""
+        n = ctypes.c_int64()
+        _chk(self.lib.org_get_relax_failures(self.h, ctypes.byref(n)), "org_get_relax_failures")
+        return n.value
 
     def get_tuning(self) -> dict:
         t = Tuning()

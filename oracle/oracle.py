@@ -47,6 +47,8 @@ def load(precision: str = "f64") -> ctypes.CDLL:
                  "orc_update_lumen_and_ECM", "orc_compute_normal_to_phi", "orc_update_all_cells",
                  "orc_reshaping_cells", "orc_commit_step", "orc_last_error"):
         getattr(lib, name).argtypes = [vp]
+    lib.orc_relax_failures.restype = ctypes.c_int64
+    lib.orc_relax_failures.argtypes = [vp]
     lib.orc_fft2d.argtypes = [i32, vp, i32]
     lib.orc_fft1d.argtypes = [i32, vp, i32]
     lib.orc_bc_cells_planes.argtypes = [i32, i32, vp]
@@ -132,6 +134,9 @@ class Oracle:
 
     def step(self, it: int = 0) -> int:
         return self._check(self.lib.orc_step(self.h, it))
+
+    def relax_failures(self) -> int:
+        return int(self.lib.orc_relax_failures(self.h))
 
     def last_error(self) -> int:
         return self.lib.orc_last_error(self.h)

@@ -65,6 +65,7 @@ struct orc_state {
     double *s, *s0_cell;
     int32_t *xc, *yc, *count_ri;
     int32_t count_lum, count_ecm;
+    int64_t relax_failures;   /* reshaping loops that passed rimax */
     int err;
 };
 
@@ -328,6 +329,7 @@ int orc_get_iterations(const orc_state *st, int32_t *cells, int32_t *lum, int32_
     return ORG_OK;
 }
 int orc_last_error(const orc_state *st) { return st->err; }
+int64_t orc_relax_failures(const orc_state *st) { return st->relax_failures; }
 
 static int set_err(orc_state *st, int e) { if (!st->err) st->err = e; return e; }
 
@@ -869,7 +871,7 @@ int orc_reshaping_lumen_or_ECM(orc_state *st, int which_ecm) {
         rdev = 0.0;                                                    /* :848-855 */
         for (size_t k = 0; k < st->NN; k++) rdev += (double)(fabs(lumn[k] - lum[k]) * dx * dx);
         count++;
-        if (ri > p->rimax) { set_err(st, ORG_ERELAX); break; }          /* :862-865 */
+        if (ri > p->rimax) { st->relax_failures++; break; }             /* :862-865 */
         ri++;
     }
     if (which_ecm) st->count_ecm = count; else st->count_lum = count;
@@ -933,7 +935,7 @@ int orc_reshaping_cells(orc_state *st) {
     real dx = (real)p->dx, wi = (real)p->wi, rdt = (real)p->rdt, rtol = (real)p->rtol, R = (real)p->R;
     double thr = (double)(rtol * 2.0 * M_PI * R * wi * rdt);           /* :475 */
     int relax_fail = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(|: relax_fail)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+: relax_fail)
     for (int n = 0; n < st->nc; n++) {
         real *ph = st->phi + (size_t)n * st->NN;
         real *pn = st->phin + (size_t)n * st->NN;
@@ -957,7 +959,7 @@ int orc_reshaping_cells(orc_state *st) {
         }
         st->count_ri[n] = count;
     }
-    if (relax_fail) set_err(st, ORG_ERELAX);
+    st->relax_failures += relax_fail;
     return st->err;
 }
 

@@ -55,6 +55,9 @@ int orc_set_centres(orc_state *st, const int32_t *xc, const int32_t *yc);
 int orc_get_areas(const orc_state *st, double *s, double *f0);
 int orc_get_iterations(const orc_state *st, int32_t *cells, int32_t *lum, int32_t *ecm);
 int orc_last_error(const orc_state *st);
+/* Reshaping loops that passed rimax and stopped there, as the reference's
+ * "Relaxation failure!" break does (the step goes on). */
+int64_t orc_relax_failures(const orc_state *st);
 
 /* run_with_active_gels.cpp:92-98: lumn<-lum, ECMn<-ECM, phin<-phi, rhon<-rho.
  * hfunc is left as uploaded (the driver zeroes it, run_with_active_gels.cpp:74). */

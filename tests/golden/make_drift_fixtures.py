@@ -63,6 +63,49 @@ def fixture_path(case: str) -> str:
     return os.path.join(HERE, f"drift_{case}.npz")
 
 
+# Long runs save the oracle's state every RESUME_EVERY steps outside the
+# repository ($DRIFT_RESUME_DIR, default /tmp) and continue from the last
+# save when restarted.  The state that carries across a step boundary is
+# phi, rho, lum, ECM and the stale hfunc outside the windows (rhon, lumn and
+# ECMn equal rho, lum and ECM after the copy-back; phin is only read inside
+# the window it is written in), so a resumed run is bit for bit the
+# uninterrupted one.
+RESUME_EVERY = 250
+RESUME_FIELDS = ("phi", "rho", "lum", "ecm", "hfunc")
+
+
+def _resume_path(tag: str) -> str:
+    return os.path.join(os.environ.get("DRIFT_RESUME_DIR", "/tmp"), f"drift_resume_{tag}.npz")
+
+
+def save_resume(tag: str, o, it: int, record: dict) -> None:
+    path = _resume_path(tag)
+    tmp = path + ".part.npz"
+    np.savez(tmp, it=np.int64(it), **{f"state_{k}": o.get(k) for k in RESUME_FIELDS},
+             **{f"rec_{k}": np.asarray(v) for k, v in record.items()})
+    os.replace(tmp, path)
+
+
+def load_resume(tag: str, o):
+    """Restore a saved state into the oracle; returns (next step, record) or
+    (0, None) when there is no save."""
+    path = _resume_path(tag)
+    if not os.path.exists(path):
+        return 0, None
+    z = np.load(path)
+    for k in RESUME_FIELDS:
+        o.set(k, z[f"state_{k}"])
+    o.call("init_like_driver")
+    rec = {k[4:]: z[k] for k in z.files if k.startswith("rec_")}
+    print(f"{tag}: resuming at step {int(z['it'])}", flush=True)
+    return int(z["it"]), rec
+
+
+def drop_resume(tag: str) -> None:
+    if os.path.exists(_resume_path(tag)):
+        os.remove(_resume_path(tag))
+
+
 def changed_box(a: np.ndarray, a0: np.ndarray):
     """Bounding box [i0, i1) x [j0, j1) of the pixels where a != a0 (empty:
     a zero-size box at the origin)."""
@@ -132,7 +175,17 @@ def run_f32(case: str) -> str:
     checkpoints = [int(c) for c in fx["checkpoints"]]
     bad = dict(centre=0, cells=0, lum=0, ecm=0)
     t0 = time.time()
-    for it in range(max(checkpoints)):
+    tag = f"{case}_f32"
+    start, rec = load_resume(tag, o)
+    if rec is not None:
+        bad = {k: int(rec[f"bad_{k}"]) for k in bad}
+        for k in rec:
+            if k.startswith("fx_"):
+                fx[k[3:]] = rec[k]
+    for it in range(start, max(checkpoints)):
+        if it > start and it % RESUME_EVERY == 0:
+            save_resume(tag, o, it, {**{f"bad_{k}": v for k, v in bad.items()},
+                                     **{f"fx_{k}": fx[k] for k in fx if k.startswith("f32_")}})
         o.step(it)
         xc, yc = o.centres()
         bad["centre"] += int((xc != fx["centres"][it, :, 0]).any() or (yc != fx["centres"][it, :, 1]).any())
@@ -152,6 +205,7 @@ def run_f32(case: str) -> str:
     for k, v in bad.items():
         fx[f"f32_{k}_mismatch"] = np.int32(v)
     np.savez_compressed(fixture_path(case), **fx)
+    drop_resume(tag)
     return fixture_path(case)
 
 
@@ -224,7 +278,9 @@ def add_ensemble(case: str, members: list) -> str:
 
 
 def run_both(case: str) -> str:
-    run_case(case)
+    done64 = os.path.exists(fixture_path(case)) and "centres" in np.load(fixture_path(case)).files
+    if os.path.exists(_resume_path(case)) or not (done64 or os.path.exists(_resume_path(f"{case}_f32"))):
+        run_case(case)
     return run_f32(case)
 
 
@@ -244,7 +300,14 @@ def run_case(case: str) -> str:
     it_ecm = np.zeros(nsteps, np.int16)
     out = {"checkpoints": np.array(checkpoints, np.int32)}
     t0 = time.time()
-    for it in range(nsteps):
+    start, rec = load_resume(case, o)
+    if rec is not None:
+        centres, it_cells, it_lum, it_ecm = rec["centres"], rec["it_cells"], rec["it_lum"], rec["it_ecm"]
+        out.update({k[4:]: v for k, v in rec.items() if k.startswith("out_")})
+    for it in range(start, nsteps):
+        if it > start and it % RESUME_EVERY == 0:
+            save_resume(case, o, it, dict(centres=centres, it_cells=it_cells, it_lum=it_lum, it_ecm=it_ecm,
+                                          **{f"out_{k}": v for k, v in out.items()}))
         o.step(it)
         xc, yc = o.centres()
         centres[it, :, 0], centres[it, :, 1] = xc, yc
@@ -262,6 +325,7 @@ def run_case(case: str) -> str:
             print(f"{case}: step {it + 1} stored ({time.time() - t0:.0f} s)", flush=True)
     out.update(centres=centres, it_cells=it_cells, it_lum=it_lum, it_ecm=it_ecm)
     np.savez_compressed(fixture_path(case), **out)
+    drop_resume(case)
     return fixture_path(case)
 
 

@@ -9,13 +9,23 @@ centroids, turnover labels, h, psi, areas, the spectral cortex update of
 every cell, lumen/ECM update + reshaping, phase-field update + reshaping of
 every cell, copy-back.  The timed form is the batched one (for all cells:
 k_pre, then k_step + k_mid per step, k_step + k_post for the last step of a
-replay; hipGraphs of 8 steps).  The plain per-function drop-in form (the reference's call sequence
-with its host round trips) is timed beside it on a shorter run.
+replay; a replay of n steps is n / L hipGraphs of L steps plus one graph of
+the n % L remaining steps, L autotuned).  The plain per-function drop-in
+form (the reference's call sequence with its host round trips) is timed
+beside it on a shorter run.
+
+The timed window is steps [2 W, 2 W + K) of the trajectory (W warm-up steps,
+the autotuner, which restores the state, W more warm-up steps).  In the
+first few dozen steps the lumen/ECM reshaping loop runs many more sweeps
+than later (its stop test is an absolute threshold on the whole grid's
+change), so short windows cost more per step; the line reports the window
+and the sweeps executed in it.
 
 Printed: one JSON line (see the repository's benchmark contract) with a
 ``roofline`` object for the dominant kernel (algorithmic bytes per launch /
-HIP-event launch time on the library's stream) and a ``cpu_baseline``
-object (the FP64 CPU oracle on a bounded sample of the same workload).
+HIP-event launch time on the library's stream, both over a replay of the
+timed window) and a ``cpu_baseline`` object (the FP64 CPU oracle on a
+bounded sample of the same workload, OpenMP threads).
 """
 from __future__ import annotations
 
@@ -31,17 +41,23 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-timesteps/s, 64 cells @1024^2: FP32+batch+FNO vs plain, phi L2err<=1e-3"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per cell of the dominant launch, k_step
-# (FP32 = 4 B; M = FFT box side, W = window side):
-#   spectral role:    read phi_n, phi_n-1, phi_n+1, rho_n, lum, ECM on the box,
-#                     write rho_n on the box                   -> 7 x 4 B x M^2
-#   phase-field role: read phi_n, rho_n, hfunc_n, psi, h(lum), h(ECM) on the
-#                     window, write phi_n and phin, read phi_n for the next
-#                     centroid (support box >= window)           -> 9 x 4 B x W^2
-# The lumen/ECM role of the same launch is not counted (its traffic depends
-# on the live box and the sweep count), so `achieved` is a lower bound.
-def alg_bytes_per_cell(M: int, W: int) -> int:
-    return 28 * M * M + 36 * W * W
+# Algorithmic HBM bytes of the dominant launch, k_step (FP32 = 4 B; M = FFT
+# box side, W = window side, N = grid side; R_c, R_lum, R_ecm = reshaping
+# sweeps executed):
+#   spectral role, per cell:    read phi_n, phi_n-1, phi_n+1, rho_n, lum, ECM
+#                               on the box, write rho_n on the box  -> 28 B x M^2
+#   phase-field role, per cell: read phi_n, rho_n, hfunc_n, psi, h(lum), h(ECM)
+#                               on the window, write phi_n and phin, read phi_n
+#                               for the next centroid (support box >= window)
+#                                                                   -> 36 B x W^2
+#                               each sweep after the first: read phi, phi_old,
+#                               write phi                      -> 12 B x W^2 (R_c - 1)
+#   lumen/ECM role, per grid:   read lum, ECM, psi, write lumn, ECMn -> 20 B x N^2
+#                               each sweep: read, read old, write  -> 12 B x N^2 (R_lum + R_ecm)
+# These are the per-cell-step bytes B_alg of the reference's data flow
+# restricted to windows and boxes, minus the psi pass (k_mid / k_pre).
+def alg_bytes_k_step(nc: int, M: int, W: int, N: int, rc: float, rl: float, re: float) -> float:
+    return nc * (28 * M * M + 36 * W * W + 12 * W * W * max(rc - 1.0, 0.0)) + N * N * (20 + 12 * (rl + re))
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_organoid64_1024_pmc.md")
@@ -140,8 +156,10 @@ def fno_forward_only(n: int) -> int:
 
 
 def measured_traffic(path: str = PMC_SUMMARY):
-    """HBM bytes per k_step launch from the committed rocprofv3 --pmc summary
-    (line 'traffic_bytes_per_launch: <int>'), or None."""
+    """HBM bytes per k_step launch read from the committed rocprofv3 --pmc
+    summary (line 'traffic_bytes_per_launch: <int>'), or None.  It is the
+    profile's figure for its own window (named in that file), not a count
+    taken in this run."""
     try:
         with open(path) as f:
             for line in f:
@@ -221,7 +239,7 @@ def main() -> int:
     ap.add_argument("--workload", default="organoid64_1024")
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=8,
-                    help="timesteps of the FP64 CPU oracle timed as cpu_baseline (1.4-2 s each)")
+                    help="timesteps of the FP64 CPU oracle timed as cpu_baseline (~0.3-2.6 s each on 8 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=160,
                     help="steps of the uneven 8-organoid sweep batch (0: skip)")
@@ -274,38 +292,60 @@ def main() -> int:
         print(f"bench: tuning {tuning}", file=sys.stderr, flush=True)
         org.step(max(args.warmup, 1), graph=True)
         org.sync()
+    gs = org.get_tuning()["graph_steps"]
+    w0 = 2 * max(args.warmup, 1)
     t0 = time.perf_counter()
-    org.step(args.steps, graph=True)
+    org.step(args.steps, graph=True, it0=w0)
     org.sync()
     dt = time.perf_counter() - t0
     value = nc * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
     print(f"bench: batched {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
+    form = (f"batched: k_step + k_mid per step for all cells (k_pre / k_post at the ends of a replay); "
+            f"{args.steps // gs} hipGraph(s) of {gs} steps + {1 if args.steps % gs else 0} of "
+            f"{args.steps % gs} steps; exact velocity solve (FNO surrogate not used in the timed region; "
+            f"phi error not checked in it: see tests/test_gpu_drift.py)")
 
-    # per-kernel times with HIP events on the library stream (eager replay of
-    # the same step sequence)
+    # the same window replayed eagerly, one step at a time, on a second
+    # organoid: per-kernel times with HIP events on the library stream and
+    # the reshaping sweeps of every step
     nprof = min(200, args.steps)
-    org.profile(True)
-    org.step(nprof, graph=False)
-    org.sync()
-    kt = org.kernel_times()
-    org.profile(False)
+    rep_org = host.Organoid(p, wl.s0_cell)
+    rep_org.load_workload(wl)
+    rep_org.set_option(host.OPT_WRITE_VELOCITY, 0)
+    rep_org.set_tuning(**org.get_tuning())
+    rep_org.step(w0, graph=True)
+    rep_org.sync()
+    rep_org.profile(True)
+    sweeps = {"cells_mean": 0.0, "lum": 0, "ecm": 0}
+    for k in range(nprof):
+        rep_org.step(1, graph=False, it0=w0 + k)
+        ci, li, ei = rep_org.iterations()
+        sweeps["cells_mean"] += float(np.mean(ci)) / nprof
+        sweeps["lum"] += li
+        sweeps["ecm"] += ei
+    kt = rep_org.kernel_times()
+    rep_org.profile(False)
+    rep_org.close()
     per_kernel = {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}
     avg_ms = per_kernel["step"][0]              # profiling slot of k_step
-    bytes_launch = alg_bytes_per_cell(p.M, p.W) * nc
+    bytes_launch = alg_bytes_k_step(nc, p.M, p.W, p.N, sweeps["cells_mean"], sweeps["lum"] / nprof,
+                                    sweeps["ecm"] / nprof)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic() if wl.name == "organoid64_1024" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "profiles/k_step_organoid64_1024_pmc.md (rocprofv3 --pmc, its own window)"
+                                  if traffic else None,
                 "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5),
-                "alg_bytes_per_launch": bytes_launch}
+                "alg_bytes_per_launch": int(bytes_launch),
+                "window": f"steps {w0}..{w0 + nprof - 1}, eager replay, HIP events on the library stream"}
     # profiling slots of the batched step's launches (the other slots are
     # used by the per-function entry points)
     names = {"centres": "k_pre", "prep": "k_prep", "psi": "k_psi", "lumecm": "k_lumecm_update",
              "lumreshape": "k_lum_normals", "spectral": "k_spectral", "cells": "k_cells",
              "commit": "k_post", "step": "k_step", "mid": "k_mid"}
     kernel_ms = {names[k]: round(v[0], 5) for k, v in per_kernel.items() if v[1]}
-    cells_it, lum_it, ecm_it = org.iterations()
 
     # plain drop-in form: the reference's per-function call sequence
     plain = host.Organoid(p, wl.s0_cell)
@@ -333,17 +373,20 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        from oracle.oracle import Oracle
+        from oracle.oracle import Oracle, threads
         o = Oracle(p, wl.s0_cell)
         o.load_workload(wl)
         t0 = time.perf_counter()
         for it in range(args.cpu_steps):
             o.step(it)
         cdt = time.perf_counter() - t0
-        cpu = {"value": round(nc * args.cpu_steps / cdt, 3), "unit": "cell-timesteps/s", "cores": 1,
+        nthr = threads()
+        cpu = {"value": round(nc * args.cpu_steps / cdt, 3), "unit": "cell-timesteps/s", "cores": nthr,
                "kind": "port",
-               "sample": f"{args.cpu_steps} full timesteps of {wl.name} (64 cells, 1024^2) on the FP64 "
-                         f"CPU oracle (full-plane restatement of the reference), 1 thread, {cdt:.1f} s"}
+               "sample": f"steps 0..{args.cpu_steps - 1} of {wl.name} ({nc} cells, {p.N}^2) on the FP64 CPU "
+                         f"oracle (full-plane restatement of the reference, OpenMP over cells / rows), "
+                         f"{nthr} threads, {cdt:.1f} s; these first steps run more lumen/ECM sweeps "
+                         f"(281 at step 0) than the GPU's window"}
         o.close()
 
     out = {
@@ -352,12 +395,13 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "cells": nc, "grid": p.N, "zone_size": p.zone_size,
-                   "form": "batched: k_step + k_mid per step for all cells (k_pre / k_post at the ends of a "
-                           f"replay), hipGraphs of {tuning['tuning']['graph_steps']} steps (FNO surrogate not used)",
+                   "form": form,
+                   "timed_steps": [w0, w0 + args.steps - 1],
                    "plain_dropin_cell_steps_per_s": round(plain_rate, 2) if plain_rate else None,
                    "speedup_vs_plain": round(value / plain_rate, 3) if plain_rate else None,
                    "kernel_ms": kernel_ms,
-                   "reshape_iters_last_step": {"cells_mean": float(np.mean(cells_it)), "lum": lum_it, "ecm": ecm_it},
+                   "reshape_sweeps_in_window": {"steps": nprof, "cells_mean_per_step": round(sweeps["cells_mean"], 3),
+                                                "lum_total": sweeps["lum"], "ecm_total": sweeps["ecm"]},
                    "launch_tuning": tuning, "sweep8": sweep, "large": large, "surrogate": surrogate},
         "roofline": roofline,
         "cpu_baseline": cpu,

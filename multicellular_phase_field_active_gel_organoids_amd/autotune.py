@@ -82,14 +82,15 @@ def device_name() -> tuple:
 
 
 def _snapshot(org) -> dict:
-    return {f: org.get_field(f) for f in (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM)}
+    """The state that carries across a step boundary: phi, rho, lum, ECM and
+    hfunc (its values outside the current windows are stale and enter psi
+    and the areas), so a restored organoid continues as the original."""
+    return {f: org.get_field(f) for f in (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM, host.F_HFUNC)}
 
 
 def _restore(org, state: dict) -> None:
-    import numpy as np
     for f, a in state.items():
         org.set_field(f, a)
-    org.set_field(host.F_HFUNC, np.zeros_like(state[host.F_PHI]))
     org.prepare()
 
 

@@ -89,8 +89,9 @@ struct org_ctx {
     float *sur_x = nullptr, *sur_v = nullptr, *sur_res = nullptr;
     int *sur_use = nullptr, *sur_nexact = nullptr;
     int sur_cap = 0;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; one step */
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* graph_steps steps; the remainder */
     hipGraph_t graph[2] = {nullptr, nullptr};
+    int graph_len[2] = {0, 0};                      /* steps captured in each */
     /* profiling */
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[ORG_K_COUNT];
@@ -104,8 +105,9 @@ struct org_batch {
     BatchEnt *ent = nullptr;               /* one entry per organoid (device) */
     int *bmap[4] = {nullptr, nullptr, nullptr, nullptr};   /* block maps of k_pre, k_step, k_mid, k_post */
     int grid[4] = {0, 0, 0, 0};
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; one step */
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   /* ORG_GRAPH_STEPS steps; the remainder */
     hipGraph_t graph[2] = {nullptr, nullptr};
+    int graph_len[2] = {0, 0};                      /* steps captured in each */
 };
 
 namespace {
@@ -445,6 +447,31 @@ void batch_drop_graphs(org_batch *b) {
         if (b->gexec[k]) { (void)hipGraphExecDestroy(b->gexec[k]); b->gexec[k] = nullptr; }
         if (b->graph[k]) { (void)hipGraphDestroy(b->graph[k]); b->graph[k] = nullptr; }
     }
+}
+
+/* The replayable graph of `len` steps in slot (g, ge, have): captured from
+ * `enqueue(len)` on stream st, or reused when the slot already holds that
+ * length.  A replay of n steps is (n / L) graphs of L steps and one graph of
+ * the n % L remaining steps, so every replay runs k_step + k_mid between
+ * steps and k_pre / k_post only at its two ends. */
+template <typename F>
+int ensure_graph(hipStream_t st, hipGraph_t &g, hipGraphExec_t &ge, int &have, int len, F enqueue) {
+    if (ge && have == len) return ORG_OK;
+    if (ge) { (void)hipGraphExecDestroy(ge); ge = nullptr; }
+    if (g) { (void)hipGraphDestroy(g); g = nullptr; }
+    have = 0;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(len);
+    hipGraph_t cap = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &cap);
+    if (rc || e != hipSuccess) {
+        if (cap) (void)hipGraphDestroy(cap);
+        return rc ? rc : note_error(ORG_EHIP);
+    }
+    g = cap;
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    have = len;
+    return ORG_OK;
 }
 
 int batch_harvest(org_batch *b) {
@@ -860,27 +887,18 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
         TRY(enqueue_steps(c, nsteps));
         return harvest_device_error(c);
     }
-    /* graphs of graph_steps steps and of one step (the remainder) */
-    for (int k = 0; k < 2; k++) {
-        if (c->gexec[k]) continue;
-        const int nst = k ? 1 : c->graph_steps;
-        bool prof = c->prof;
-        c->prof = false;                 /* no event records inside the graph */
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(c, nst);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        c->prof = prof;
-        if (rc || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc ? rc : note_error(ORG_EHIP);
-        }
-        c->graph[k] = g;
-        HIPCHK(hipGraphInstantiate(&c->gexec[k], c->graph[k], nullptr, nullptr, 0));
-    }
-    int s = 0;
-    for (; s + c->graph_steps <= nsteps; s += c->graph_steps) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
-    for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec[1], c->stream));
+    /* graphs of graph_steps steps and of the remaining nsteps % graph_steps */
+    const bool prof = c->prof;
+    c->prof = false;                     /* no event records inside the graphs */
+    auto enq = [c](int n) { return enqueue_steps(c, n); };
+    const int full = nsteps / c->graph_steps, rem = nsteps % c->graph_steps;
+    int rc = ORG_OK;
+    if (full) rc = ensure_graph(c->stream, c->graph[0], c->gexec[0], c->graph_len[0], c->graph_steps, enq);
+    if (!rc && rem) rc = ensure_graph(c->stream, c->graph[1], c->gexec[1], c->graph_len[1], rem, enq);
+    c->prof = prof;
+    TRY(rc);
+    for (int s = 0; s < full; s++) HIPCHK(hipGraphLaunch(c->gexec[0], c->stream));
+    if (rem) HIPCHK(hipGraphLaunch(c->gexec[1], c->stream));
     return harvest_device_error(c);
 }
 
@@ -1163,22 +1181,12 @@ int org_batch_step(org_batch *b, int nsteps, int use_graph) {
         TRY(enqueue_batch_steps(b, nsteps));
         return batch_harvest(b);
     }
-    for (int k = 0; k < 2; k++) {
-        if (b->gexec[k]) continue;
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_batch_steps(b, k ? 1 : ORG_GRAPH_STEPS);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc ? rc : note_error(ORG_EHIP);
-        }
-        b->graph[k] = g;
-        HIPCHK(hipGraphInstantiate(&b->gexec[k], b->graph[k], nullptr, nullptr, 0));
-    }
-    int s = 0;
-    for (; s + ORG_GRAPH_STEPS <= nsteps; s += ORG_GRAPH_STEPS) HIPCHK(hipGraphLaunch(b->gexec[0], b->stream));
-    for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(b->gexec[1], b->stream));
+    auto enq = [b](int n) { return enqueue_batch_steps(b, n); };
+    const int full = nsteps / ORG_GRAPH_STEPS, rem = nsteps % ORG_GRAPH_STEPS;
+    if (full) TRY(ensure_graph(b->stream, b->graph[0], b->gexec[0], b->graph_len[0], ORG_GRAPH_STEPS, enq));
+    if (rem) TRY(ensure_graph(b->stream, b->graph[1], b->gexec[1], b->graph_len[1], rem, enq));
+    for (int s = 0; s < full; s++) HIPCHK(hipGraphLaunch(b->gexec[0], b->stream));
+    if (rem) HIPCHK(hipGraphLaunch(b->gexec[1], b->stream));
     return batch_harvest(b);
 }
 

@@ -47,6 +47,7 @@ def load(precision: str = "f64") -> ctypes.CDLL:
                  "orc_update_lumen_and_ECM", "orc_compute_normal_to_phi", "orc_update_all_cells",
                  "orc_reshaping_cells", "orc_commit_step", "orc_last_error"):
         getattr(lib, name).argtypes = [vp]
+    lib.orc_threads.restype = ctypes.c_int
     lib.orc_relax_failures.restype = ctypes.c_int64
     lib.orc_relax_failures.argtypes = [vp]
     lib.orc_fft2d.argtypes = [i32, vp, i32]
@@ -146,6 +147,11 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle error code {rc}")
         return rc
+
+
+def threads(precision: str = "f64") -> int:
+    """OpenMP threads of the oracle's loops (cells, rows, pixels)."""
+    return int(load(precision).orc_threads())
 
 
 def fft2d(a: np.ndarray, sign: int, precision: str = "f64") -> np.ndarray:

@@ -256,6 +256,13 @@ void orc_destroy(orc_state *st) {
 }
 
 int orc_real_bytes(void) { return (int)sizeof(real); }
+int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 int orc_params_bytes(void) { return (int)sizeof(org_params); }
 
 static real *field_ptr(const orc_state *st, int f, size_t *n) {

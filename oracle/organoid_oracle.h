@@ -55,6 +55,8 @@ int orc_set_centres(orc_state *st, const int32_t *xc, const int32_t *yc);
 int orc_get_areas(const orc_state *st, double *s, double *f0);
 int orc_get_iterations(const orc_state *st, int32_t *cells, int32_t *lum, int32_t *ecm);
 int orc_last_error(const orc_state *st);
+/* OpenMP threads the oracle's loops use (OMP_NUM_THREADS); 1 without OpenMP. */
+int orc_threads(void);
 /* Reshaping loops that passed rimax and stopped there, as the reference's
  * "Relaxation failure!" break does (the step goes on). */
 int64_t orc_relax_failures(const orc_state *st);

@@ -27,8 +27,8 @@ STATE = (("phi", host.F_PHI), ("rho", host.F_RHO), ("lum", host.F_LUM), ("ecm", 
 
 
 def rel(a, b) -> float:
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 

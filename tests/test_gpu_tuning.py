@@ -41,6 +41,27 @@ def test_tunings_give_the_same_trajectory():
         assert np.array_equal(g.get_field(f), ref.get_field(f)), f
 
 
+def test_remainder_graphs_match_eager():
+    """A replay shorter than graph_steps, or one that does not divide by it,
+    runs its last n % graph_steps steps as one graph of that length (k_step +
+    k_mid between steps): 20 steps under 32-step graphs, then 45 = 32 + 13,
+    then 20 again (the cached 13-step graph is recaptured), each equal to
+    eager steps bit for bit."""
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: remainder graphs (20, 45, 20 steps under graph_steps=32)", flush=True)
+    g, e = _org(wl), _org(wl)
+    t = g.get_tuning()
+    g.set_tuning(lum_blocks=t["lum_blocks"], roles=t["roles"], graph_steps=32)
+    done = 0
+    for n in (20, 45, 20):
+        g.step(n, graph=True, it0=done)
+        e.step(n, graph=False, it0=done)
+        done += n
+        for f in FIELDS + (host.F_HFUNC, host.F_PSI):
+            assert np.array_equal(g.get_field(f), e.get_field(f)), (n, f)
+        assert all((a == b).all() for a, b in zip(g.centres(), e.centres())), n
+
+
 def test_out_of_range_tunings_are_refused():
     wl = make_workload("two_cells_256")
     g = _org(wl)

@@ -41,7 +41,8 @@ struct org_ctx {
     org_params p{};
     DevP dp{};
     hipStream_t stream = nullptr;
-    double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][blocks] */
+    double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][2][ORG_LUM_MAX] */
+    float *xch = nullptr;                /* rows exchanged by k_step's LDS loops [2][2][N][N] */
     unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier: count, generation */
     int *lum_abort = nullptr;
     long long wait_limit = 0;            /* k_step's wait timeout, wall-clock ticks */
@@ -330,7 +331,7 @@ StepArgs step_args(org_ctx *c) {
     a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp; a.nlum = c->nlum;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
     a.jlx = c->jlx; a.jly = c->jly; a.jex = c->jex; a.jey = c->jey;
-    a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc;
+    a.jbox_part = c->jbox; a.lst = c->lst; a.part_lc = c->part_lc; a.xch = c->xch;
     a.spec = c->spec; a.rho_box = c->rho_box; a.vxbox = c->vxbox; a.vybox = c->vybox; a.write_v = c->write_v;
     a.phin_win = c->phin_win; a.phi_win = c->phi_win; a.jwin = c->jwin;
     a.bar = c->lum_bar; a.abort_flag = c->lum_abort; a.err = c->err; a.limit = c->wait_limit;
@@ -565,7 +566,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_MAX); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 8 * ORG_LUM_MAX); A(&c->xch, 4 * c->NN); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
     A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);

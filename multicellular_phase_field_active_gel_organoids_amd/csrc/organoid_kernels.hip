@@ -1364,6 +1364,9 @@ static __device__ __forceinline__ float reshape_flux_j(float un, const float *u,
  * the stand-alone and the fused loop so their sums agree bit for bit. */
 #define ORG_LUM_BLOCKS 64
 #define ORG_LUM_MAX 128      /* lumen/ECM workgroups of k_step a tuning may ask for */
+#ifndef ORG_LUM_LDS_LOOP
+#define ORG_LUM_LDS_LOOP 1   /* k_step's loops with the field in LDS where it fits (box mode) */
+#endif
 static __device__ __forceinline__ double lum_sweep(const DevP &P, const float *__restrict__ src,
                                                    float *__restrict__ dst, const float *__restrict__ jx,
                                                    const float *__restrict__ jy, const int4 jb,
@@ -1533,6 +1536,315 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
     cur_out[0] = s[0].cur;
     cur_out[1] = s[1].cur;
     return true;
+}
+
+/* ---- the reshaping loops with the field in LDS ------------------------------
+ * The same sweeps as lum_loop_body (box mode), but each loop workgroup keeps
+ * its band of rows of the sweep domain -- both iterates and the normals,
+ * with two halo rows above and below and one halo column on each side -- in
+ * LDS for the whole loop, and the workgroups exchange data only every second
+ * sweep:
+ *   - a span is two sweeps; the first also recomputes the row just outside
+ *     the band on each side (the neighbours' edge rows: the same arithmetic
+ *     on the same operands, so the same values), which the second needs;
+ *   - after a span each workgroup publishes its first two and last two rows
+ *     (at their grid coordinates, in the exchange plane of the span's
+ *     parity) and the deviation of its own rows for each of the two sweeps;
+ *   - after the barrier every workgroup reads the neighbours' rows into its
+ *     halo and takes the stop test of the first sweep, then, if that loop
+ *     goes on, of the second (cell_phase_functions.cpp:774, 862-868).  When
+ *     the first test stops the loop the second sweep was speculative: its
+ *     result is dropped and the first sweep's iterate, still in the other
+ *     LDS buffer, is the last one.
+ * The exchanged values go through relaxed agent-scope atomic stores and
+ * loads (coherent across XCDs by themselves) and the arrival counter is a
+ * relaxed atomic issued once every thread's stores have completed
+ * (lum_barrier_light): no L2 write-back or invalidation per span, so the
+ * other roles' cached data on the XCD stay valid.  Rows and columns outside
+ * the domain never change during the loop (both buffers hold the same
+ * values there) and are loaded once.  The last iterate of the band goes to
+ * the buffer `cur` at the end (the other buffer is rewritten by the next
+ * step's update before any read), then one full barrier publishes it.
+ * Each pixel's update is the same reshape_flux arithmetic on the same
+ * operands, so the fields equal lum_loop_body's bit for bit; only the
+ * summation order of the deviation differs (it feeds the stop test alone). */
+/* LDS of the loop workgroup: [0, 128) red, [128, 160) the prologue's boxes,
+ * [256, 768) per-wave deviations, the bands from ORG_LUM_LDS_BASE */
+#define ORG_LUM_LDS_BASE 1024
+#define ORG_LUM_HALO 2            /* sweeps per exchange = halo rows per side */
+struct LumBand {
+    int r0, r1, c0, c1;           /* sweep domain, inclusive */
+    int a, b;                     /* this workgroup's rows [a, b) */
+    int S;                        /* LDS row stride: domain width + 2 */
+    float *u[2], *jx, *jy;        /* LDS row 0 = grid row a - 2, column 0 = column c0 - 1 */
+};
+
+/* Sweep domain of a field: its normals' box grown by one, inside rows and
+ * columns 1 .. N-2 (as lum_sweep); empty when r1 < r0 or c1 < c0. */
+static __device__ __forceinline__ void lum_domain(const int4 jb, int N, int &r0, int &r1, int &c0, int &c1) {
+    r0 = max(jb.x - 1, 1); r1 = min(jb.y + 1, N - 2);
+    c0 = max(jb.z - 1, 1); c1 = min(jb.w + 1, N - 2);
+}
+
+/* LDS bytes of one field's band arrays when its domain is split over nblk
+ * workgroups (0 for an empty domain). */
+static __device__ __forceinline__ int lum_band_bytes(const int4 jb, int N, int nblk) {
+    int r0, r1, c0, c1;
+    lum_domain(jb, N, r0, r1, c0, c1);
+    const int bh = r1 - r0 + 1, bw = c1 - c0 + 1;
+    if (bh <= 0 || bw <= 0) return 0;
+    return 4 * 4 * ((bh + nblk - 1) / nblk + 2 * ORG_LUM_HALO) * (bw + 2);
+}
+
+/* Whether the LDS form fits: both fields' bands beside the reduction
+ * scratch in k_step's shared memory. */
+static __device__ __forceinline__ bool lum_lds_fits(const int4 jb0, const int4 jb1, int N, int nblk, int lds_bytes) {
+    return ORG_LUM_LDS_BASE + lum_band_bytes(jb0, N, nblk) + lum_band_bytes(jb1, N, nblk) <= lds_bytes;
+}
+
+/* Barrier among the loop workgroups without release or acquire: every
+ * thread waits for its own memory operations to complete (the exchanged
+ * values were written with agent-scope atomic stores, which are then
+ * visible at agent scope), then thread 0 arrives with a relaxed atomic and
+ * polls as lum_barrier does, with the same time limit and abort flag. */
+static __device__ bool lum_barrier_light(unsigned *bar, int nblk, int &epoch, int *abort_flag, int *err,
+                                         long long limit) {
+    __shared__ int ok_l;
+    epoch++;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int good = 1;
+        const unsigned target = (unsigned)epoch * (unsigned)nblk;
+        unsigned seen = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        const long long t0 = wall_clock64();
+        while (seen < target) {
+            __builtin_amdgcn_s_sleep(1);
+            seen = __hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen >= target) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > limit) {
+                atomicExch(abort_flag, 1);
+                set_err(err, ORG_ETIMEOUT);
+                good = 0;
+                break;
+            }
+        }
+        ok_l = good;
+    }
+    __syncthreads();
+    return ok_l != 0;
+}
+
+static __device__ __forceinline__ void xstore(float *p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ float xload(const float *p) {
+    return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Copy grid rows [i0, i1) x columns [c0 - 1, c1 + 1] of a plane into LDS
+ * rows starting at lrow (plain loads: the plane was published by a full
+ * barrier). */
+static __device__ __forceinline__ void lum_band_load(const float *__restrict__ g, int N, int i0, int i1, int c0,
+                                                     int c1, float *l, int S, int lrow) {
+    const int w = c1 - c0 + 3, n = (i1 - i0) * w;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int i = i0 + q / w, j = c0 - 1 + q % w;
+        l[(lrow + q / w) * S + q % w] = g[(size_t)i * N + j];
+    }
+}
+
+/* One sweep of rows [i0, i1) of a band in LDS (src -> dst); returns this
+ * thread's deviation over the rows within [a, b). */
+static __device__ __forceinline__ double lum_band_sweep(const LumBand &L, const float *src, float *dst, int i0,
+                                                        int i1, float ka, float kc, float dx2) {
+    const int bw = L.c1 - L.c0 + 1, npx = (i1 - i0) * bw;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < npx; q += blockDim.x) {
+        const int i = i0 + q / bw;
+        const int e = (i - L.a + ORG_LUM_HALO) * L.S + 1 + q % bw;
+        const float u = src[e];
+        const float v = reshape_flux(u, src + e, L.S, L.jx + e, L.jy + e, L.S, ka, kc);
+        dst[e] = v;
+        if (i >= L.a && i < L.b) acc += (double)(fabsf(v - u) * dx2);
+    }
+    return acc;
+}
+
+static __device__ bool lum_loop_lds(const DevP &P, int blk, int nblk, double *red, unsigned char *lds,
+                                    float *bufL0, float *bufL1, float *bufE0, float *bufE1, const float *jlx,
+                                    const float *jly, const float *jex, const float *jey, const int4 *jb,
+                                    float *xch, LumState *lst, double *part, unsigned *bar, int &epoch,
+                                    int *abort_flag, int *err, long long limit, int *cur_out) {
+    constexpr int H = ORG_LUM_HALO;
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
+    const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
+    const float dx2 = P.dx * P.dx;
+    LumState s[2] = {lst[0], lst[1]};
+    LumBand B[2];
+    /* buffer q of field f, selected without an indexed array (which would
+     * live in scratch) */
+    auto glob = [=](int f, int q) { return f ? (q ? bufE1 : bufE0) : (q ? bufL1 : bufL0); };
+    float *lp = (float *)(lds + ORG_LUM_LDS_BASE);
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        LumBand &L = B[f];
+        lum_domain(jb[f], N, L.r0, L.r1, L.c0, L.c1);
+        const int bh = L.r1 - L.r0 + 1, bw = L.c1 - L.c0 + 1;
+        if (bh <= 0 || bw <= 0) { L.a = L.b = 0; L.S = 0; continue; }
+        L.a = L.r0 + (int)(((long long)blk * bh) / nblk);
+        L.b = L.r0 + (int)(((long long)(blk + 1) * bh) / nblk);
+        L.S = bw + 2;
+        const int rows = (bh + nblk - 1) / nblk + 2 * H;
+        L.u[0] = lp; L.u[1] = lp + rows * L.S; L.jx = lp + 2 * rows * L.S; L.jy = lp + 3 * rows * L.S;
+        lp += 4 * rows * L.S;
+        if (L.b > L.a) {
+            /* grid rows max(a-2, r0-1) .. min(b+1, r1+1); both iterates start
+             * as the updated field (the two buffers agree there) */
+            const int i0 = max(L.a - H, L.r0 - 1), i1 = min(L.b + H, L.r1 + 2);
+            const int lr = i0 - (L.a - H);
+            lum_band_load(glob(f, 0), N, i0, i1, L.c0, L.c1, L.u[0], L.S, lr);
+            lum_band_load(glob(f, 0), N, i0, i1, L.c0, L.c1, L.u[1], L.S, lr);
+            lum_band_load(f ? jex : jlx, N, i0, i1, L.c0, L.c1, L.jx, L.S, lr);
+            lum_band_load(f ? jey : jly, N, i0, i1, L.c0, L.c1, L.jy, L.S, lr);
+        }
+    }
+    /* the four corners of the boundary ring: every sweep writes 0 there (also
+     * for the ECM, useful_functions.cpp:432-435) and counts the change;
+     * workgroup 0, thread k handles corner k of both fields, tracking the
+     * value each buffer holds */
+    const bool corner_thread = blk == 0 && threadIdx.x < 4;
+    size_t ce = 0;
+    float cval[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+    if (corner_thread) {
+        const int ci = (threadIdx.x & 1) ? N - 1 : 0, cj = (threadIdx.x & 2) ? N - 1 : 0;
+        ce = (size_t)ci * N + cj;
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) cval[f][q] = glob(f, q)[ce];
+    }
+    __syncthreads();
+    double *rdv = red;                       /* red[0..3]: the span's deviations, [f][t] */
+    for (int k = 0; !(s[0].done && s[1].done); k++) {
+        const int xp = k & 1;
+        double *pk = part + (size_t)xp * 4 * nblk;    /* [f][t][nblk] of this span */
+        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            if (s[f].done) continue;
+            const LumBand &L = B[f];
+            const int par = s[f].ri & 1;
+#pragma unroll
+            for (int t = 0; t < H; t++) {
+                const int p = (par + t) & 1;
+                const float *src = p ? L.u[1] : L.u[0];
+                float *dst = p ? L.u[0] : L.u[1];
+                if (L.b > L.a) {
+                    /* sweep t of the span covers the band grown by H-1-t rows */
+                    const int i0 = max(L.a - (H - 1 - t), L.r0), i1 = min(L.b + (H - 1 - t), L.r1 + 1);
+                    acc[f][t] = lum_band_sweep(L, src, dst, i0, i1, ka, kc, dx2);
+                }
+                if (corner_thread) {
+                    const float cv = p ? cval[f][1] : cval[f][0];
+                    acc[f][t] += (double)(fabsf(0.0f - cv) * dx2);
+                    if (p) cval[f][0] = 0.0f; else cval[f][1] = 0.0f;
+                }
+                __syncthreads();
+            }
+            if (L.b > L.a) {
+                /* the latest iterate's first H and last H rows */
+                const float *lat = (par & 1) ? L.u[1] : L.u[0];
+                const int bw = L.c1 - L.c0 + 1, nr = min(2 * H, L.b - L.a);
+                float *x = xch + (size_t)(f * 2 + xp) * NN;
+                for (int q = threadIdx.x; q < nr * bw; q += blockDim.x) {
+                    const int r = q / bw, j = q % bw;
+                    const int gi = (r < H || L.b - L.a <= 2 * H) ? L.a + r : L.b - (nr - r);
+                    xstore(&x[(size_t)gi * N + L.c0 + j], lat[(gi - L.a + H) * L.S + 1 + j]);
+                }
+            }
+        }
+        /* the four deviations of this workgroup, one fixed-order block sum */
+        {
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            double v[4] = {acc[0][0], acc[0][1], acc[1][0], acc[1][1]};
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = wave_sum_d(v[i]);
+            double *wred = (double *)(lds + 256);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) wred[4 * w + i] = v[i];
+            }
+            __syncthreads();
+            if (threadIdx.x < 4) {
+                double t = 0.0;
+                for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[4 * ww + threadIdx.x];
+                __hip_atomic_store(&pk[threadIdx.x * nblk + blk], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!lum_barrier_light(bar, nblk, epoch, abort_flag, err, limit)) return false;
+        /* wave 0: the span's deviations summed over the workgroups in a
+         * fixed order; the other waves: the neighbours' rows into the halo */
+        if (threadIdx.x < 64) {
+            double v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int f = i >> 1;
+                double x = 0.0;
+                if (!s[f].done)
+                    for (int b2 = threadIdx.x; b2 < nblk; b2 += 64)
+                        x += __hip_atomic_load(&pk[i * nblk + b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v[i] = wave_sum_d(x);
+            }
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) rdv[i] = v[i];
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const LumBand &L = B[f];
+                if (s[f].done || !(L.b > L.a)) continue;
+                float *lat = (s[f].ri & 1) ? L.u[1] : L.u[0];
+                const float *x = xch + (size_t)(f * 2 + xp) * NN;
+                const int bw = L.c1 - L.c0 + 1;
+                for (int q = (int)threadIdx.x - 64; q < 2 * H * bw; q += (int)blockDim.x - 64) {
+                    const int r = q / bw, j = q % bw;
+                    const int gi = r < H ? L.a - H + r : L.b + (r - H);
+                    if (gi >= L.r0 && gi <= L.r1)
+                        lat[(gi - L.a + H) * L.S + 1 + j] = xload(&x[(size_t)gi * N + L.c0 + j]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            if (s[f].done) continue;
+            lum_stop(P, s[f], rdv[2 * f], err);
+            if (!s[f].done) lum_stop(P, s[f], rdv[2 * f + 1], err);
+        }
+        __syncthreads();      /* rdv is rewritten by the next span */
+    }
+    /* the band's last iterate into the buffer that holds it, and the corners
+     * the sweeps zeroed */
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        const LumBand &L = B[f];
+        float *g = glob(f, s[f].cur);
+        if (corner_thread && s[f].ri > 0) g[ce] = 0.0f;
+        if (!(L.b > L.a)) continue;
+        const int bw = L.c1 - L.c0 + 1, npx = (L.b - L.a) * bw;
+        const float *src = s[f].cur ? L.u[1] : L.u[0];
+        for (int q = threadIdx.x; q < npx; q += blockDim.x)
+            g[(size_t)(L.a + q / bw) * N + L.c0 + q % bw] = src[(H + q / bw) * L.S + 1 + q % bw];
+    }
+    if (blk == 0 && threadIdx.x < 2) lst[threadIdx.x] = s[threadIdx.x];
+    cur_out[0] = s[0].cur;
+    cur_out[1] = s[1].cur;
+    return lum_barrier(bar, nblk, epoch, abort_flag, err, limit);
 }
 
 /* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
@@ -2355,7 +2667,8 @@ struct StepArgs {
     float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
     int *jbox_part;
     LumState *lst;
-    double *part_lc;
+    double *part_lc;         /* loop deviations: [2 spans][2 fields][2 sweeps][ORG_LUM_MAX] */
+    float *xch;              /* exchange planes of the LDS loops: [2 fields][2 span parities][N][N] */
     /* cortex */
     float2 *spec;
     float *rho_box, *vxbox, *vybox;
@@ -2529,10 +2842,16 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
     }
     if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
     int cur[2];
-    if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, nblk,
-                       a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur,
-                       !(box_mode(BL, P.N) && box_mode(BE, P.N))))
+    const bool boxes = box_mode(BL, P.N) && box_mode(BE, P.N);
+    const int4 jb[2] = {reduce_jbox(a.jbox_part, nblk, 0, P.N), reduce_jbox(a.jbox_part, nblk, 1, P.N)};
+    if (boxes && ORG_LUM_LDS_LOOP && lum_lds_fits(jb[0], jb[1], P.N, nblk, ORG_LDS_FUSED)) {
+        if (!lum_loop_lds(P, blk, nblk, red, lds, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, jb, a.xch,
+                          a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur))
+            return;
+    } else if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part,
+                              nblk, a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur, !boxes)) {
         return;
+    }
     /* the lumen/ECM part of the next step's cell-update bracket from the
      * final iterates (the fields the copy-back commits), into the buffer this
      * step's phase-field role does not read; on the parts that can have

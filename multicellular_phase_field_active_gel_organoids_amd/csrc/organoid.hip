@@ -41,7 +41,7 @@ struct org_ctx {
     org_params p{};
     DevP dp{};
     hipStream_t stream = nullptr;
-    double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][2][ORG_LUM_MAX] */
+    double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][ORG_LUM_SPAN][ORG_LUM_MAX] */
     float *xch = nullptr;                /* rows exchanged by k_step's LDS loops [2][2][N][N] */
     unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier: count, generation */
     int *lum_abort = nullptr;
@@ -566,7 +566,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 8 * ORG_LUM_MAX); A(&c->xch, 4 * c->NN); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_SPAN * ORG_LUM_MAX); A(&c->xch, 4 * c->NN); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
     A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);
@@ -598,7 +598,11 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         hipMemset(c->hecm, 0, sizeof(float) * c->NN) != hipSuccess ||
         hipMemset(c->sum_ecm, 0, sizeof(double)) != hipSuccess ||
         hipMemset(c->lum_bar, 0, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(c->lum_abort, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(c->lbrk_par, 0, 2 * sizeof(int)) != hipSuccess) {
+        hipMemset(c->lbrk_par, 0, 2 * sizeof(int)) != hipSuccess ||
+        /* the loops read normals one pixel outside the live box, where none
+         * is ever computed in box mode: they must be zero */
+        hipMemset(c->jlx, 0, sizeof(float) * c->NN) != hipSuccess || hipMemset(c->jly, 0, sizeof(float) * c->NN) != hipSuccess ||
+        hipMemset(c->jex, 0, sizeof(float) * c->NN) != hipSuccess || hipMemset(c->jey, 0, sizeof(float) * c->NN) != hipSuccess) {
         org_destroy(c);
         return note_error(ORG_EHIP);
     }

@@ -1271,10 +1271,12 @@ static __device__ void lumecm_normals_box(const DevP &P, const int f, const int4
  * ECM 1, useful_functions.cpp:439-462), by workgroup blk of nblk; its
  * normals are zero. */
 static __device__ void lumecm_ring(const DevP &P, float *__restrict__ L0, float *__restrict__ L1,
-                                   float *__restrict__ E0, float *__restrict__ E1, const int blk, const int nblk) {
+                                   float *__restrict__ E0, float *__restrict__ E1, const int blk, const int nblk,
+                                   const bool corners = true) {
     for (int q = blk * blockDim.x + threadIdx.x; q < 4 * P.N - 4; q += nblk * blockDim.x) {
         int i, j;
         ring_px(q, P.N, i, j);
+        if (!corners && (i == 0 || i == P.N - 1) && (j == 0 || j == P.N - 1)) continue;
         const size_t e = (size_t)i * P.N + j;
         L0[e] = 0.0f; L1[e] = 0.0f;
         E0[e] = 1.0f; E1[e] = 1.0f;
@@ -1538,68 +1540,75 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
     return true;
 }
 
-/* ---- the reshaping loops with the field in LDS ------------------------------
- * The same sweeps as lum_loop_body (box mode), but each loop workgroup keeps
- * its band of rows of the sweep domain -- both iterates and the normals,
- * with two halo rows above and below and one halo column on each side -- in
- * LDS for the whole loop, and the workgroups exchange data only every second
- * sweep:
- *   - a span is two sweeps; the first also recomputes the row just outside
- *     the band on each side (the neighbours' edge rows: the same arithmetic
- *     on the same operands, so the same values), which the second needs;
- *   - after a span each workgroup publishes its first two and last two rows
- *     (at their grid coordinates, in the exchange plane of the span's
- *     parity) and the deviation of its own rows for each of the two sweeps;
- *   - after the barrier every workgroup reads the neighbours' rows into its
- *     halo and takes the stop test of the first sweep, then, if that loop
- *     goes on, of the second (cell_phase_functions.cpp:774, 862-868).  When
- *     the first test stops the loop the second sweep was speculative: its
- *     result is dropped and the first sweep's iterate, still in the other
- *     LDS buffer, is the last one.
- * The exchanged values go through relaxed agent-scope atomic stores and
- * loads (coherent across XCDs by themselves) and the arrival counter is a
- * relaxed atomic issued once every thread's stores have completed
- * (lum_barrier_light): no L2 write-back or invalidation per span, so the
- * other roles' cached data on the XCD stay valid.  Rows and columns outside
- * the domain never change during the loop (both buffers hold the same
- * values there) and are loaded once.  The last iterate of the band goes to
- * the buffer `cur` at the end (the other buffer is rewritten by the next
- * step's update before any read), then one full barrier publishes it.
- * Each pixel's update is the same reshape_flux arithmetic on the same
- * operands, so the fields equal lum_loop_body's bit for bit; only the
- * summation order of the deviation differs (it feeds the stop test alone). */
+/* ---- the lumen/ECM role with the fields in LDS ------------------------------
+ * The batched step's lumen/ECM update, normals and reshaping loops (box mode)
+ * with each loop workgroup keeping one band of rows of each field's sweep
+ * domain in LDS from the update to the last sweep.  The sweep domain is the
+ * live box grown by one (the update's region): outside the normals' box
+ * grown by one a sweep leaves a pixel unchanged exactly (every flux term
+ * carries a face average of normals, all zero there), so this equals
+ * sweeping only that box.
+ *   - Set-up, no exchange: each workgroup computes the update (lumecm_px)
+ *     on its band plus ORG_LUM_SPAN + 1 rows on each side and the normals on
+ *     its band plus ORG_LUM_SPAN rows (zero outside the live box, from the
+ *     old field and the LDS update).
+ *   - Spans of ORG_LUM_SPAN sweeps: sweep t of a span covers the band grown
+ *     by SPAN - 1 - t rows (the neighbours' rows, recomputed with the same
+ *     arithmetic on the same operands, so the same values), so a span needs
+ *     the neighbours' rows only once, at its start.  After a span each
+ *     workgroup publishes its first and last SPAN rows (at their grid
+ *     coordinates, in the exchange plane of the span's parity) and the
+ *     deviation of its own rows for each sweep; after the barrier it reads
+ *     the neighbours' rows into its halo and takes the sweeps' stop tests in
+ *     order (cell_phase_functions.cpp:774, 862-868).  Sweeps after the one
+ *     that stops the loop were speculative and are dropped; the stopping
+ *     sweep's iterate is still in LDS (its own buffer, or a copy of the
+ *     band's rows for the sweeps the span overwrites later).
+ *   - The exchanged values go through relaxed agent-scope atomic stores and
+ *     loads (coherent across XCDs by themselves) and the arrival counter is
+ *     a relaxed atomic issued once every thread's stores have completed
+ *     (lum_barrier_light): no L2 write-back or invalidation per span, so the
+ *     other roles' cached data on the XCD stay valid.
+ *   - At the end the band's last iterate goes to the buffer `cur` (the
+ *     other buffer is rewritten before any read by the next step's update),
+ *     workgroup 0 writes the four corners of both buffers (the ring writes
+ *     of the other workgroups leave them out, so no two workgroups store to
+ *     the same address), and one full barrier publishes everything.
+ * Each pixel's update, normal and flux is the same arithmetic on the same
+ * operands as lumecm_normals_box and lum_sweep, so the fields equal the
+ * global-memory path bit for bit; only the summation order of the
+ * deviation differs (it feeds the stop test alone). */
+#ifndef ORG_LUM_SPAN
+#define ORG_LUM_SPAN 2             /* sweeps per exchange (even) */
+#endif
 /* LDS of the loop workgroup: [0, 128) red, [128, 160) the prologue's boxes,
- * [256, 768) per-wave deviations, the bands from ORG_LUM_LDS_BASE */
-#define ORG_LUM_LDS_BASE 1024
-#define ORG_LUM_HALO 2            /* sweeps per exchange = halo rows per side */
+ * [256, 1280) per-wave deviations, the bands from ORG_LUM_LDS_BASE */
+#define ORG_LUM_LDS_BASE 2048
 struct LumBand {
-    int r0, r1, c0, c1;           /* sweep domain, inclusive */
+    int r0, r1, c0, c1;           /* sweep domain (the live box grown by one), inclusive */
     int a, b;                     /* this workgroup's rows [a, b) */
-    int S;                        /* LDS row stride: domain width + 2 */
-    float *u[2], *jx, *jy;        /* LDS row 0 = grid row a - 2, column 0 = column c0 - 1 */
+    int S, bw;                    /* LDS row stride (bw + 2), domain width */
+    float *u[2], *jx, *jy;        /* LDS row 0 = grid row a - SPAN - 1, column 0 = column c0 - 1 */
+    float *hist;                  /* the band's rows after sweeps 0 .. SPAN-3 of a span */
 };
 
-/* Sweep domain of a field: its normals' box grown by one, inside rows and
- * columns 1 .. N-2 (as lum_sweep); empty when r1 < r0 or c1 < c0. */
-static __device__ __forceinline__ void lum_domain(const int4 jb, int N, int &r0, int &r1, int &c0, int &c1) {
-    r0 = max(jb.x - 1, 1); r1 = min(jb.y + 1, N - 2);
-    c0 = max(jb.z - 1, 1); c1 = min(jb.w + 1, N - 2);
+static __device__ __forceinline__ int lum_own_max(int bh, int nblk) { return (bh + nblk - 1) / nblk; }
+static __device__ __forceinline__ int lum_lds_rows(int bh, int nblk) {
+    return lum_own_max(bh, nblk) + 2 * ORG_LUM_SPAN + 2;
 }
 
-/* LDS bytes of one field's band arrays when its domain is split over nblk
- * workgroups (0 for an empty domain). */
-static __device__ __forceinline__ int lum_band_bytes(const int4 jb, int N, int nblk) {
-    int r0, r1, c0, c1;
-    lum_domain(jb, N, r0, r1, c0, c1);
-    const int bh = r1 - r0 + 1, bw = c1 - c0 + 1;
-    if (bh <= 0 || bw <= 0) return 0;
-    return 4 * 4 * ((bh + nblk - 1) / nblk + 2 * ORG_LUM_HALO) * (bw + 2);
+/* LDS bytes of one field's band arrays for live box B (half open) split
+ * over nblk workgroups (0 for an empty box). */
+static __device__ __forceinline__ int lum_band_bytes(const int4 B, int nblk) {
+    if (box_empty(B)) return 0;
+    const int bh = B.y - B.x + 2, bw = B.w - B.z + 2;
+    return 4 * (4 * lum_lds_rows(bh, nblk) * (bw + 2) + (ORG_LUM_SPAN - 2) * lum_own_max(bh, nblk) * bw);
 }
 
 /* Whether the LDS form fits: both fields' bands beside the reduction
  * scratch in k_step's shared memory. */
-static __device__ __forceinline__ bool lum_lds_fits(const int4 jb0, const int4 jb1, int N, int nblk, int lds_bytes) {
-    return ORG_LUM_LDS_BASE + lum_band_bytes(jb0, N, nblk) + lum_band_bytes(jb1, N, nblk) <= lds_bytes;
+static __device__ __forceinline__ bool lum_lds_fits(const int4 BL, const int4 BE, int nblk, int lds_bytes) {
+    return ORG_LUM_LDS_BASE + lum_band_bytes(BL, nblk) + lum_band_bytes(BE, nblk) <= lds_bytes;
 }
 
 /* Barrier among the loop workgroups without release or acquire: every
@@ -1643,208 +1652,52 @@ static __device__ __forceinline__ float xload(const float *p) {
     return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Copy grid rows [i0, i1) x columns [c0 - 1, c1 + 1] of a plane into LDS
- * rows starting at lrow (plain loads: the plane was published by a full
- * barrier). */
-static __device__ __forceinline__ void lum_band_load(const float *__restrict__ g, int N, int i0, int i1, int c0,
-                                                     int c1, float *l, int S, int lrow) {
-    const int w = c1 - c0 + 3, n = (i1 - i0) * w;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int i = i0 + q / w, j = c0 - 1 + q % w;
-        l[(lrow + q / w) * S + q % w] = g[(size_t)i * N + j];
+/* The normal of one pixel (as normals_row and lumecm_normals_box): |grad|
+ * of the old field at g, the new field's central differences given. */
+static __device__ __forceinline__ void lum_normal_px(const DevP &P, const float *__restrict__ old, const size_t g,
+                                                     const int N, const float np, const float nm, const float ep,
+                                                     const float em, float &vx, float &vy) {
+    const float c = 1.0f / P.dx / P.dx / 4.0f, h = 1.0f / (2.0f * P.dx);
+    const float a = old_diff(old[g + N], old[g - N]), b = old_diff(old[g + 1], old[g - 1]);
+    const float ng = hw_sqrt(c * (a * a + b * b));
+    vx = 0.0f; vy = 0.0f;
+    if (ng != 0.0f) {
+        const float rng = hw_rcp(ng);
+        vx = h * (np - nm) * rng;
+        vy = h * (ep - em) * rng;
     }
+    if (vx * vx + vy * vy > ORG_JMAX * ORG_JMAX) { vx = 0.0f; vy = 0.0f; }
 }
 
-/* One sweep of rows [i0, i1) of a band in LDS (src -> dst); returns this
- * thread's deviation over the rows within [a, b). */
+/* The pixels q = r * w + c (0 <= r < rows) of a block of rows taken by the
+ * workgroup's threads at q = threadIdx.x, + blockDim.x, ...: (r, c)
+ * advance by additions, without a division per pixel. */
+struct RowWalk {
+    int r, c, dr, dc, w;
+    __device__ __forceinline__ explicit RowWalk(int w_) : w(w_) {
+        r = (int)threadIdx.x / w; c = (int)threadIdx.x - r * w;
+        dr = (int)blockDim.x / w; dc = (int)blockDim.x - dr * w;
+    }
+    __device__ __forceinline__ void next() {
+        r += dr; c += dc;
+        if (c >= w) { c -= w; r++; }
+    }
+};
+
+/* One sweep of grid rows [i0, i1) of a band in LDS (src -> dst); returns
+ * this thread's deviation over the rows within [a, b). */
 static __device__ __forceinline__ double lum_band_sweep(const LumBand &L, const float *src, float *dst, int i0,
                                                         int i1, float ka, float kc, float dx2) {
-    const int bw = L.c1 - L.c0 + 1, npx = (i1 - i0) * bw;
+    const int base = L.a - ORG_LUM_SPAN - 1, own0 = L.a - i0, own1 = L.b - i0;
     double acc = 0.0;
-    for (int q = threadIdx.x; q < npx; q += blockDim.x) {
-        const int i = i0 + q / bw;
-        const int e = (i - L.a + ORG_LUM_HALO) * L.S + 1 + q % bw;
+    for (RowWalk q(L.bw); q.r < i1 - i0; q.next()) {
+        const int e = (i0 + q.r - base) * L.S + 1 + q.c;
         const float u = src[e];
         const float v = reshape_flux(u, src + e, L.S, L.jx + e, L.jy + e, L.S, ka, kc);
         dst[e] = v;
-        if (i >= L.a && i < L.b) acc += (double)(fabsf(v - u) * dx2);
+        if (q.r >= own0 && q.r < own1) acc += (double)(fabsf(v - u) * dx2);
     }
     return acc;
-}
-
-static __device__ bool lum_loop_lds(const DevP &P, int blk, int nblk, double *red, unsigned char *lds,
-                                    float *bufL0, float *bufL1, float *bufE0, float *bufE1, const float *jlx,
-                                    const float *jly, const float *jex, const float *jey, const int4 *jb,
-                                    float *xch, LumState *lst, double *part, unsigned *bar, int &epoch,
-                                    int *abort_flag, int *err, long long limit, int *cur_out) {
-    constexpr int H = ORG_LUM_HALO;
-    const int N = P.N;
-    const size_t NN = (size_t)N * N;
-    const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
-    const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
-    const float dx2 = P.dx * P.dx;
-    LumState s[2] = {lst[0], lst[1]};
-    LumBand B[2];
-    /* buffer q of field f, selected without an indexed array (which would
-     * live in scratch) */
-    auto glob = [=](int f, int q) { return f ? (q ? bufE1 : bufE0) : (q ? bufL1 : bufL0); };
-    float *lp = (float *)(lds + ORG_LUM_LDS_BASE);
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
-        LumBand &L = B[f];
-        lum_domain(jb[f], N, L.r0, L.r1, L.c0, L.c1);
-        const int bh = L.r1 - L.r0 + 1, bw = L.c1 - L.c0 + 1;
-        if (bh <= 0 || bw <= 0) { L.a = L.b = 0; L.S = 0; continue; }
-        L.a = L.r0 + (int)(((long long)blk * bh) / nblk);
-        L.b = L.r0 + (int)(((long long)(blk + 1) * bh) / nblk);
-        L.S = bw + 2;
-        const int rows = (bh + nblk - 1) / nblk + 2 * H;
-        L.u[0] = lp; L.u[1] = lp + rows * L.S; L.jx = lp + 2 * rows * L.S; L.jy = lp + 3 * rows * L.S;
-        lp += 4 * rows * L.S;
-        if (L.b > L.a) {
-            /* grid rows max(a-2, r0-1) .. min(b+1, r1+1); both iterates start
-             * as the updated field (the two buffers agree there) */
-            const int i0 = max(L.a - H, L.r0 - 1), i1 = min(L.b + H, L.r1 + 2);
-            const int lr = i0 - (L.a - H);
-            lum_band_load(glob(f, 0), N, i0, i1, L.c0, L.c1, L.u[0], L.S, lr);
-            lum_band_load(glob(f, 0), N, i0, i1, L.c0, L.c1, L.u[1], L.S, lr);
-            lum_band_load(f ? jex : jlx, N, i0, i1, L.c0, L.c1, L.jx, L.S, lr);
-            lum_band_load(f ? jey : jly, N, i0, i1, L.c0, L.c1, L.jy, L.S, lr);
-        }
-    }
-    /* the four corners of the boundary ring: every sweep writes 0 there (also
-     * for the ECM, useful_functions.cpp:432-435) and counts the change;
-     * workgroup 0, thread k handles corner k of both fields, tracking the
-     * value each buffer holds */
-    const bool corner_thread = blk == 0 && threadIdx.x < 4;
-    size_t ce = 0;
-    float cval[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
-    if (corner_thread) {
-        const int ci = (threadIdx.x & 1) ? N - 1 : 0, cj = (threadIdx.x & 2) ? N - 1 : 0;
-        ce = (size_t)ci * N + cj;
-#pragma unroll
-        for (int f = 0; f < 2; f++)
-#pragma unroll
-            for (int q = 0; q < 2; q++) cval[f][q] = glob(f, q)[ce];
-    }
-    __syncthreads();
-    double *rdv = red;                       /* red[0..3]: the span's deviations, [f][t] */
-    for (int k = 0; !(s[0].done && s[1].done); k++) {
-        const int xp = k & 1;
-        double *pk = part + (size_t)xp * 4 * nblk;    /* [f][t][nblk] of this span */
-        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-        for (int f = 0; f < 2; f++) {
-            if (s[f].done) continue;
-            const LumBand &L = B[f];
-            const int par = s[f].ri & 1;
-#pragma unroll
-            for (int t = 0; t < H; t++) {
-                const int p = (par + t) & 1;
-                const float *src = p ? L.u[1] : L.u[0];
-                float *dst = p ? L.u[0] : L.u[1];
-                if (L.b > L.a) {
-                    /* sweep t of the span covers the band grown by H-1-t rows */
-                    const int i0 = max(L.a - (H - 1 - t), L.r0), i1 = min(L.b + (H - 1 - t), L.r1 + 1);
-                    acc[f][t] = lum_band_sweep(L, src, dst, i0, i1, ka, kc, dx2);
-                }
-                if (corner_thread) {
-                    const float cv = p ? cval[f][1] : cval[f][0];
-                    acc[f][t] += (double)(fabsf(0.0f - cv) * dx2);
-                    if (p) cval[f][0] = 0.0f; else cval[f][1] = 0.0f;
-                }
-                __syncthreads();
-            }
-            if (L.b > L.a) {
-                /* the latest iterate's first H and last H rows */
-                const float *lat = (par & 1) ? L.u[1] : L.u[0];
-                const int bw = L.c1 - L.c0 + 1, nr = min(2 * H, L.b - L.a);
-                float *x = xch + (size_t)(f * 2 + xp) * NN;
-                for (int q = threadIdx.x; q < nr * bw; q += blockDim.x) {
-                    const int r = q / bw, j = q % bw;
-                    const int gi = (r < H || L.b - L.a <= 2 * H) ? L.a + r : L.b - (nr - r);
-                    xstore(&x[(size_t)gi * N + L.c0 + j], lat[(gi - L.a + H) * L.S + 1 + j]);
-                }
-            }
-        }
-        /* the four deviations of this workgroup, one fixed-order block sum */
-        {
-            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-            double v[4] = {acc[0][0], acc[0][1], acc[1][0], acc[1][1]};
-#pragma unroll
-            for (int i = 0; i < 4; i++) v[i] = wave_sum_d(v[i]);
-            double *wred = (double *)(lds + 256);
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) wred[4 * w + i] = v[i];
-            }
-            __syncthreads();
-            if (threadIdx.x < 4) {
-                double t = 0.0;
-                for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[4 * ww + threadIdx.x];
-                __hip_atomic_store(&pk[threadIdx.x * nblk + blk], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        if (!lum_barrier_light(bar, nblk, epoch, abort_flag, err, limit)) return false;
-        /* wave 0: the span's deviations summed over the workgroups in a
-         * fixed order; the other waves: the neighbours' rows into the halo */
-        if (threadIdx.x < 64) {
-            double v[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int f = i >> 1;
-                double x = 0.0;
-                if (!s[f].done)
-                    for (int b2 = threadIdx.x; b2 < nblk; b2 += 64)
-                        x += __hip_atomic_load(&pk[i * nblk + b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                v[i] = wave_sum_d(x);
-            }
-            if (threadIdx.x == 0) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) rdv[i] = v[i];
-            }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 2; f++) {
-                const LumBand &L = B[f];
-                if (s[f].done || !(L.b > L.a)) continue;
-                float *lat = (s[f].ri & 1) ? L.u[1] : L.u[0];
-                const float *x = xch + (size_t)(f * 2 + xp) * NN;
-                const int bw = L.c1 - L.c0 + 1;
-                for (int q = (int)threadIdx.x - 64; q < 2 * H * bw; q += (int)blockDim.x - 64) {
-                    const int r = q / bw, j = q % bw;
-                    const int gi = r < H ? L.a - H + r : L.b + (r - H);
-                    if (gi >= L.r0 && gi <= L.r1)
-                        lat[(gi - L.a + H) * L.S + 1 + j] = xload(&x[(size_t)gi * N + L.c0 + j]);
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int f = 0; f < 2; f++) {
-            if (s[f].done) continue;
-            lum_stop(P, s[f], rdv[2 * f], err);
-            if (!s[f].done) lum_stop(P, s[f], rdv[2 * f + 1], err);
-        }
-        __syncthreads();      /* rdv is rewritten by the next span */
-    }
-    /* the band's last iterate into the buffer that holds it, and the corners
-     * the sweeps zeroed */
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
-        const LumBand &L = B[f];
-        float *g = glob(f, s[f].cur);
-        if (corner_thread && s[f].ri > 0) g[ce] = 0.0f;
-        if (!(L.b > L.a)) continue;
-        const int bw = L.c1 - L.c0 + 1, npx = (L.b - L.a) * bw;
-        const float *src = s[f].cur ? L.u[1] : L.u[0];
-        for (int q = threadIdx.x; q < npx; q += blockDim.x)
-            g[(size_t)(L.a + q / bw) * N + L.c0 + q % bw] = src[(H + q / bw) * L.S + 1 + q % bw];
-    }
-    if (blk == 0 && threadIdx.x < 2) lst[threadIdx.x] = s[threadIdx.x];
-    cur_out[0] = s[0].cur;
-    cur_out[1] = s[1].cur;
-    return lum_barrier(bar, nblk, epoch, abort_flag, err, limit);
 }
 
 /* k_commit_lumecm: lum <- lumn, ECM <- ECMn (useful_functions.cpp:695-709,
@@ -2667,7 +2520,7 @@ struct StepArgs {
     float *L0, *L1, *E0, *E1, *jlx, *jly, *jex, *jey;
     int *jbox_part;
     LumState *lst;
-    double *part_lc;         /* loop deviations: [2 spans][2 fields][2 sweeps][ORG_LUM_MAX] */
+    double *part_lc;         /* loop deviations: [2 spans][2 fields][ORG_LUM_SPAN sweeps][ORG_LUM_MAX] */
     float *xch;              /* exchange planes of the LDS loops: [2 fields][2 span parities][N][N] */
     /* cortex */
     float2 *spec;
@@ -2805,6 +2658,227 @@ static __device__ __forceinline__ void step_spectral(DevP P, StepArgs a, const i
         spectral_body<false>(P, b, sm, sm + ORG_MB * ORG_LS, lab, a.phi, a.rho, a.lum, a.ecm, x0, y0, a.spec,
                              a.rho_box, a.vxbox, a.vybox, a.write_v, nullptr);
 }
+/* The batched step's lumen/ECM role in LDS (see the block comment above
+ * ORG_LUM_SPAN); false if a barrier timed out. */
+static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int blk, const int nblk, const float se,
+                                    const int4 BL, const int4 BE, unsigned char *lds, int &epoch, int *cur_out) {
+    constexpr int K = ORG_LUM_SPAN;
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    const float ka = P.wi * P.wi * (P.rdt / P.dx / P.dx);
+    const float kc = P.rdt * P.wi * 1.41421356237f / P.dx;
+    const float dx2 = P.dx * P.dx;
+    double *red = (double *)lds;
+    /* loop states as lum_reset_states sets them (workgroup 0 stores them) */
+    LumState s[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        s[f].done = P.rdev0_lum > P.thr_lum ? 0 : 1;
+        s[f].ri = 0; s[f].cur = 0; s[f].pad = 0;
+    }
+    /* buffer q of field f, selected without an indexed array (which would
+     * live in scratch) */
+    auto glob = [=](int f, int q) { return f ? (q ? A.E1 : A.E0) : (q ? A.L1 : A.L0); };
+    LumBand B[2];
+    float *lp = (float *)(lds + ORG_LUM_LDS_BASE);
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        LumBand &L = B[f];
+        const int4 Bf = f ? BE : BL;
+        if (box_empty(Bf)) { L.a = L.b = 0; L.r0 = 1; L.r1 = 0; L.c0 = 1; L.c1 = 0; L.S = 0; L.bw = 0; continue; }
+        L.r0 = Bf.x - 1; L.r1 = Bf.y; L.c0 = Bf.z - 1; L.c1 = Bf.w;
+        const int bh = L.r1 - L.r0 + 1;
+        L.bw = L.c1 - L.c0 + 1;
+        L.a = L.r0 + (int)(((long long)blk * bh) / nblk);
+        L.b = L.r0 + (int)(((long long)(blk + 1) * bh) / nblk);
+        L.S = L.bw + 2;
+        const int rows = lum_lds_rows(bh, nblk);
+        L.u[0] = lp; L.u[1] = lp + rows * L.S; L.jx = lp + 2 * rows * L.S; L.jy = lp + 3 * rows * L.S;
+        L.hist = lp + 4 * rows * L.S;
+        lp += 4 * rows * L.S + (K - 2) * lum_own_max(bh, nblk) * L.bw;
+        if (L.b <= L.a) continue;
+        const float *X = f ? A.ecm : A.lum;
+        const int base = L.a - K - 1, w = L.bw + 2;
+        /* the update on grid rows [a-K-1, b+K+1) within the domain and its
+         * halo; outside the domain the committed field (both buffers hold it) */
+        {
+            const int i0 = max(base, L.r0 - 1), i1 = min(L.b + K + 1, L.r1 + 2);
+#pragma unroll 4
+            for (RowWalk q(w); q.r < i1 - i0; q.next()) {
+                const int i = i0 + q.r, j = L.c0 - 1 + q.c;
+                const size_t e = (size_t)i * N + j;
+                const bool in = i >= L.r0 && i <= L.r1 && j >= L.c0 && j <= L.c1;
+                const float v = in ? lumecm_px(P, f, e, se, A.lum, A.ecm, A.psi, A.hlum, A.hecm) : X[e];
+                const int l = (i - base) * L.S + q.c;
+                L.u[0][l] = v; L.u[1][l] = v;
+            }
+        }
+        __syncthreads();
+        /* normals on grid rows [a-K, b+K): inside the live box from the old
+         * field and the update, zero outside it */
+        {
+            const int i0 = max(L.a - K, L.r0 - 1), i1 = min(L.b + K, L.r1 + 2);
+            const float *un = L.u[0];
+#pragma unroll 4
+            for (RowWalk q(w); q.r < i1 - i0; q.next()) {
+                const int i = i0 + q.r, j = L.c0 - 1 + q.c;
+                const int l = (i - base) * L.S + q.c;
+                float vx = 0.0f, vy = 0.0f;
+                if (i >= Bf.x && i < Bf.y && j >= Bf.z && j < Bf.w)
+                    lum_normal_px(P, X, (size_t)i * N + j, N, un[l + L.S], un[l - L.S], un[l + 1], un[l - 1], vx, vy);
+                L.jx[l] = vx; L.jy[l] = vy;
+            }
+        }
+    }
+    /* the four corners of the boundary ring: every sweep writes 0 there (also
+     * for the ECM, useful_functions.cpp:432-435) and counts the change;
+     * workgroup 0, thread k owns corner k of both fields and tracks the value
+     * each buffer holds, starting from the ring values (lumen 0, ECM 1) */
+    const bool corner_thread = blk == 0 && threadIdx.x < 4;
+    size_t ce = 0;
+    float cval[2][2] = {{0.0f, 0.0f}, {1.0f, 1.0f}};
+    if (corner_thread) ce = (size_t)((threadIdx.x & 1) ? N - 1 : 0) * N + ((threadIdx.x & 2) ? N - 1 : 0);
+    int stop_t[2] = {K - 1, K - 1}, span_par[2] = {0, 0};
+    __syncthreads();
+    for (int k = 0; !(s[0].done && s[1].done); k++) {
+        const int xp = k & 1;
+        double *pk = A.part_lc + (size_t)xp * 2 * K * nblk;    /* [f][t][nblk] of this span */
+        double acc[2][K];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int t = 0; t < K; t++) acc[f][t] = 0.0;
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            if (s[f].done) continue;
+            const LumBand &L = B[f];
+            const int par = s[f].ri & 1;
+            span_par[f] = par;
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                const int p = (par + t) & 1;
+                const float *src = p ? L.u[1] : L.u[0];
+                float *dst = p ? L.u[0] : L.u[1];
+                if (L.b > L.a) {
+                    const int i0 = max(L.a - (K - 1 - t), L.r0), i1 = min(L.b + (K - 1 - t), L.r1 + 1);
+                    acc[f][t] = lum_band_sweep(L, src, dst, i0, i1, ka, kc, dx2);
+                }
+                if (corner_thread) {
+                    const float cv = p ? cval[f][1] : cval[f][0];
+                    acc[f][t] += (double)(fabsf(0.0f - cv) * dx2);
+                    if (p) cval[f][0] = 0.0f; else cval[f][1] = 0.0f;
+                }
+                __syncthreads();
+                if (t <= K - 3 && L.b > L.a) {
+                    /* keep the band's rows of this iterate: sweep t + 2 overwrites them */
+                    float *h = L.hist + (size_t)t * (L.b - L.a) * L.bw;
+                    for (RowWalk q(L.bw); q.r < L.b - L.a; q.next())
+                        h[q.r * L.bw + q.c] = dst[(q.r + K + 1) * L.S + 1 + q.c];
+                }
+            }
+            if (L.b > L.a) {
+                /* the latest iterate's first K and last K rows */
+                const float *lat = par ? L.u[1] : L.u[0];
+                const int own = L.b - L.a, nr = min(2 * K, own), base = L.a - K - 1;
+                float *x = A.xch + (size_t)(f * 2 + xp) * NN;
+                for (int q = threadIdx.x; q < nr * L.bw; q += blockDim.x) {
+                    const int r = q / L.bw, j = q % L.bw;
+                    const int gi = (r < K || own <= 2 * K) ? L.a + r : L.b - (nr - r);
+                    xstore(&x[(size_t)gi * N + L.c0 + j], lat[(gi - base) * L.S + 1 + j]);
+                }
+            }
+        }
+        /* the 2K deviations of this workgroup, one fixed-order block sum */
+        {
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            double *wred = (double *)(lds + 256);
+            double v[2 * K];
+#pragma unroll
+            for (int i = 0; i < 2 * K; i++) v[i] = wave_sum_d(acc[i / K][i % K]);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < 2 * K; i++) wred[2 * K * w + i] = v[i];
+            }
+            __syncthreads();
+            if (threadIdx.x < 2 * K) {
+                double t = 0.0;
+                for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[2 * K * ww + threadIdx.x];
+                __hip_atomic_store(&pk[threadIdx.x * nblk + blk], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!lum_barrier_light(A.bar, nblk, epoch, A.abort_flag, A.err, A.limit)) return false;
+        /* wave 0: the span's deviations summed over the workgroups in a fixed
+         * order; the other waves: the neighbours' rows into the halo */
+        if (threadIdx.x < 64) {
+            double v[2 * K];
+#pragma unroll
+            for (int i = 0; i < 2 * K; i++) {
+                double x = 0.0;
+                if (!s[i / K].done)
+                    for (int b2 = threadIdx.x; b2 < nblk; b2 += 64)
+                        x += __hip_atomic_load(&pk[i * nblk + b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v[i] = wave_sum_d(x);
+            }
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int i = 0; i < 2 * K; i++) red[i] = v[i];
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const LumBand &L = B[f];
+                if (s[f].done || !(L.b > L.a)) continue;
+                float *lat = (s[f].ri & 1) ? L.u[1] : L.u[0];
+                const float *x = A.xch + (size_t)(f * 2 + xp) * NN;
+                const int base = L.a - K - 1;
+                for (int q = (int)threadIdx.x - 64; q < 2 * K * L.bw; q += (int)blockDim.x - 64) {
+                    const int r = q / L.bw, j = q % L.bw;
+                    const int gi = r < K ? L.a - K + r : L.b + (r - K);
+                    if (gi >= L.r0 && gi <= L.r1)
+                        lat[(gi - base) * L.S + 1 + j] = xload(&x[(size_t)gi * N + L.c0 + j]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            if (s[f].done) continue;
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                lum_stop(P, s[f], red[f * K + t], A.err);
+                if (s[f].done) { stop_t[f] = t; break; }
+            }
+        }
+        __syncthreads();      /* red is rewritten by the next span */
+    }
+    /* the band's last iterate into the buffer that holds it; the corners */
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        const LumBand &L = B[f];
+        float *g = glob(f, s[f].cur);
+        if (corner_thread) {
+            glob(f, 0)[ce] = cval[f][0];
+            glob(f, 1)[ce] = cval[f][1];
+        }
+        if (!(L.b > L.a)) continue;
+        const int own = L.b - L.a, base = L.a - K - 1, t = stop_t[f];
+        const float *src = nullptr;
+        bool hist = false;
+        if (s[f].ri == 0) src = L.u[0];                        /* no sweep ran: the update */
+        else if (t <= K - 3) hist = true;
+        else src = ((span_par[f] + t + 1) & 1) ? L.u[1] : L.u[0];
+        for (RowWalk q(L.bw); q.r < own; q.next()) {
+            const float v = hist ? L.hist[(size_t)t * own * L.bw + q.r * L.bw + q.c]
+                                 : src[(L.a + q.r - base) * L.S + 1 + q.c];
+            g[(size_t)(L.a + q.r) * N + L.c0 + q.c] = v;
+        }
+    }
+    if (blk == 0 && threadIdx.x < 2) A.lst[threadIdx.x] = s[threadIdx.x];
+    cur_out[0] = s[0].cur;
+    cur_out[1] = s[1].cur;
+    return lum_barrier(A.bar, nblk, epoch, A.abort_flag, A.err, A.limit);
+}
+
 static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
                                                    unsigned char *lds) {
     double *red = (double *)lds;
@@ -2830,27 +2904,26 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
         a.live[0] = BL.x; a.live[1] = BL.y; a.live[2] = BL.z; a.live[3] = BL.w;
         a.live[4] = BE.x; a.live[5] = BE.y; a.live[6] = BE.z; a.live[7] = BE.w;
     }
-    if (box_mode(BL, P.N) && box_mode(BE, P.N)) {
-        lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk);
-        lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
-                           a.jlx, a.jly, a.jbox_part, sb);
-        lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
-                           a.jex, a.jey, a.jbox_part, sb);
-    } else {
-        lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0,
-                            a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
-    }
-    if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
-    int cur[2];
     const bool boxes = box_mode(BL, P.N) && box_mode(BE, P.N);
-    const int4 jb[2] = {reduce_jbox(a.jbox_part, nblk, 0, P.N), reduce_jbox(a.jbox_part, nblk, 1, P.N)};
-    if (boxes && ORG_LUM_LDS_LOOP && lum_lds_fits(jb[0], jb[1], P.N, nblk, ORG_LDS_FUSED)) {
-        if (!lum_loop_lds(P, blk, nblk, red, lds, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, jb, a.xch,
-                          a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur))
+    int cur[2];
+    if (boxes && ORG_LUM_LDS_LOOP && lum_lds_fits(BL, BE, nblk, ORG_LDS_FUSED)) {
+        lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk, false);
+        if (!lum_role_lds(P, a, blk, nblk, (float)se, BL, BE, lds, epoch, cur)) return;
+    } else {
+        if (boxes) {
+            lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk);
+            lumecm_normals_box(P, 0, BL, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1,
+                               a.jlx, a.jly, a.jbox_part, sb);
+            lumecm_normals_box(P, 1, BE, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.E0, a.E1,
+                               a.jex, a.jey, a.jbox_part, sb);
+        } else {
+            lumecm_normals_band(P, blk, nblk, (float)se, a.lum, a.ecm, a.psi, a.hlum, a.hecm, a.L0, a.L1, a.E0,
+                                a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part, sb);
+        }
+        if (!lum_barrier(a.bar, nblk, epoch, a.abort_flag, a.err, a.limit)) return;
+        if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part,
+                           nblk, a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur, !boxes))
             return;
-    } else if (!lum_loop_body(P, blk, nblk, red, a.L0, a.L1, a.E0, a.E1, a.jlx, a.jly, a.jex, a.jey, a.jbox_part,
-                              nblk, a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur, !boxes)) {
-        return;
     }
     /* the lumen/ECM part of the next step's cell-update bracket from the
      * final iterates (the fields the copy-back commits), into the buffer this

@@ -290,6 +290,24 @@ enum {
 enum { ORG_OPT_WRITE_VELOCITY = 0 };
 int org_set_option(org_ctx *ctx, int option, int value);
 
+/* ---- allocation-free form -----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the work of org_compute_hfunc_lum_ecm
+ * (compute_all_hfunc_2d_tables_kernels, useful_functions.cpp:272-285, and
+ * the FP64 sum of h(ECM) dx^2 that update_lumen_and_ECM takes,
+ * cell_phase_functions.cpp:376-386) on caller-owned device planes, with no
+ * org_ctx.  lum, ecm: N x N device planes (read); hlum, hecm: N x N device
+ * planes (written, h(u) = u^2 (3 - 2u)); sum_ecm: one device double
+ * (written).  stream: a hipStream_t, or NULL for the default stream.  The
+ * results equal org_compute_hfunc_lum_ecm's bit for bit.  It calls no
+ * allocating routine: its partial sums and counter live in a static device
+ * array (about 33 KB per GPU, held until the process ends), so N must be
+ * 8 .. 4096 (ORG_EINVAL with a message otherwise), and calls that overlap
+ * on one GPU (other streams, other host threads) race on that array. */
+int org_compute_hfunc_lum_ecm_noalloc(int N, double dx, const float *lum, const float *ecm, float *hlum,
+                                      float *hecm, double *sum_ecm, void *stream);
+
 /* Test hook (no counterpart in the reference): one unnormalised 128 x 128
  * 2D DFT through the LDS transform used by org_update_all_cortices.  data:
  * 128*128 interleaved (re, im) floats on the host, sign -1 or +1. */

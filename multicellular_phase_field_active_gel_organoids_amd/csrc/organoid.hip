@@ -4,6 +4,8 @@
  * One org_ctx owns the device state of one organoid and one HIP stream
  * (default priority).  All kernels come from organoid_kernels.hip.
  */
+#include <mutex>
+
 #include "organoid_kernels.hip"
 #include "organoid_fno.h"
 
@@ -1199,6 +1201,71 @@ int org_batch_sync(org_batch *b) {
     if (!b) return note_error(ORG_EINVAL);
     HIPCHK(hipStreamSynchronize(b->stream));
     return ORG_OK;
+}
+
+/* ---- allocation-free h(lum), h(ECM) and sum of h(ECM) -------------------
+ * The partial sums (one per 64 x 64 tile of the grid and one for the
+ * boundary ring) and the last-workgroup counter of k_hlum live in a static
+ * device array: each GPU holds its own copy from the library's load to the
+ * process end.  Host code reaches a device's copy only through the address
+ * hipGetSymbolAddress returns with that device current (the array's name is
+ * a host-side stand-in, not a device address), looked up once per device. */
+#define ORG_NOALLOC_MAX_N 4096
+#define ORG_NOALLOC_PARTS ((ORG_NOALLOC_MAX_N / 64) * (ORG_NOALLOC_MAX_N / 64) + 1)
+#define ORG_NOALLOC_BYTES (8 * ORG_NOALLOC_PARTS + 256)
+static __device__ __align__(256) unsigned char scratch[ORG_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+/* This device's copy of `scratch` (looked up on the first call there). */
+int scratch_address(void **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return note_error(ORG_EINVAL);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "organoid: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return note_error(ORG_EHIP);
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return ORG_OK;
+}
+
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+}  // namespace
+
+int org_compute_hfunc_lum_ecm_noalloc(int N, double dx, const float *lum, const float *ecm, float *hlum, float *hecm,
+                                      double *sum_ecm, void *stream) {
+    if (!lum || !ecm || !hlum || !hecm || !sum_ecm || !(dx > 0.0)) return note_error(ORG_EINVAL);
+    if (N < 8 || N > ORG_NOALLOC_MAX_N) {
+        fprintf(stderr, "organoid: org_compute_hfunc_lum_ecm_noalloc: N = %d outside 8..%d (its scratch holds "
+                        "the partial sums of at most %d x %d tiles)\n", N, ORG_NOALLOC_MAX_N, ORG_NOALLOC_MAX_N / 64,
+                ORG_NOALLOC_MAX_N / 64);
+        return note_error(ORG_EINVAL);
+    }
+    void *base = nullptr;
+    TRY(scratch_address(&base));
+    const int t = (N + 63) / 64, nparts = t * t + 1;
+    double *part = (double *)base;
+    unsigned *counter = (unsigned *)((unsigned char *)base + 8 * ORG_NOALLOC_PARTS);
+    DevP d{};
+    d.N = N;
+    d.dx = (float)dx;
+    hipStream_t st = (hipStream_t)stream;
+    /* the counter must start at 0: a one-thread kernel rather than a memset,
+     * so the call can be captured into a graph */
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, counter);
+    TRY(launch_check());
+    hipLaunchKernelGGL(k_hlum, dim3(nparts), dim3(1024), 0, st, d, lum, ecm, hlum, hecm, part, counter, sum_ecm);
+    return launch_check();
 }
 
 int org_debug_fft2d(float *data, int sign) {

@@ -127,3 +127,49 @@ def test_ic_program_order_matches_relax_initial_state(tmp_path):
     for name, f, shape in (("phi", host.F_PHI, (nc, N, N)), ("rho", host.F_RHO, (nc, N, N)),
                            ("lum", host.F_LUM, (N, N)), ("ecm", host.F_ECM, (N, N))):
         assert np.array_equal(ld(f"out_{name}.f32", shape), g.get_field(f)), name
+
+
+REF_DRIVER = "/root/reference/main_code/run_with_active_gels.cpp"
+REF_FUNCTIONS = "/root/reference/functions"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="the reference sources are not on this machine")
+def test_unmodified_reference_driver_links(tmp_path):
+    """The reference's own driver (main_code/run_with_active_gels.cpp),
+    unmodified and compiled where it lies with the reference's own headers,
+    links against liborganoid_dropin.so / liborganoid_hip.so with the two
+    one-line headers INTEGRATION.md tells a maintainer to add (cufft.h
+    including organoid_dropin.h, an empty cublas.h) and a rendered
+    constants.cpp: every function and table it references resolves in the
+    drop-in, none in a stand-in."""
+    (tmp_path / "cufft.h").write_text('/* signature types of the organoid drop-in library */\n'
+                                      '#include "organoid_dropin.h"\n')
+    (tmp_path / "cublas.h").write_text("/* unused by the reference's timestep */\n")
+    obj = tmp_path / "driver.o"
+    inc = ["-I", str(tmp_path), "-I", REF_FUNCTIONS, "-I", os.path.join(ROOT, "include")]
+    subprocess.run(["g++", "-O1", "-std=c++14", "-Drestrict=__restrict", "-w"] + inc + ["-c", REF_DRIVER, "-o", str(obj)],
+                   check=True)
+    # every function and global the driver leaves undefined, except libc /
+    # libstdc++ / libm and the constants.h globals (defined by constants.cpp)
+    undef = set(re.findall(r"^\s+U (\S+)$", subprocess.run(["nm", "-C", str(obj)], check=True, capture_output=True,
+                                                            text=True).stdout, re.M))
+    consts = set(re.findall(r"^(?:const )?(?:int|float) (\w+)", open(os.path.join(
+        ROOT, "tests", "dropin", "constants_two_cells.cpp")).read(), re.M))
+    lib_syms = _exports(DROPIN, demangle=True)
+    # the reference's tables (extern declarations of declare_tables.h and
+    # declare_fft_tables.h, read as text) and its C++ functions
+    tables = set()
+    for h in ("declare_tables.h", "declare_fft_tables.h"):
+        for decl in re.findall(r"^\s*extern\s+([^;]+);", open(os.path.join(REF_FUNCTIONS, h)).read(), re.M):
+            tables.update(re.findall(r"(\w+)\s*(?:,|$)", decl))
+    system = ("std::", "vtable for", "VTT for", "typeinfo", "_")
+    ours = sorted(u for u in undef if not u.startswith(system) and u not in consts and ("(" in u or u in tables))
+    assert len(ours) >= 30, ours           # the timestep's functions and the tables
+    for u in ours:
+        assert re.search(rf" [TBD] {re.escape(u)}$", lib_syms, re.M), u
+    exe = tmp_path / "run_with_active_gels"
+    libdir = os.path.join(PKG, "csrc")
+    subprocess.run(["g++", "-o", str(exe), str(obj), os.path.join(ROOT, "tests", "dropin", "constants_two_cells.cpp"),
+                    "-I", os.path.join(ROOT, "include"), "-L", os.path.join(libdir, "dropin"), "-lorganoid_dropin",
+                    "-L", libdir, "-lorganoid_hip", "-Wl,--no-undefined"], check=True)
+    assert exe.exists()

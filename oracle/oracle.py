@@ -86,13 +86,13 @@ class Oracle:
             pass
 
     def _shape(self, name):
-        n = self.lib.orc_field_size(self.h, F[name])
         N, M = self.N, self.params.M
+        if name in ("vx", "vy"):                  # per-cell FFT boxes
+            return (self.nc, M, M)
+        n = self.lib.orc_field_size(self.h, F[name])
         if n == self.nc * N * N:
             return (self.nc, N, N)
-        if n == N * N:
-            return (N, N)
-        return (self.nc, M, M)
+        return (N, N)
 
     def set(self, name: str, arr) -> None:
         a = np.ascontiguousarray(arr, np.float64)

@@ -119,9 +119,25 @@ typedef struct org_ctx org_ctx;
 /* Allocate device state for one organoid.  s0_cell (ncell doubles) is used
  * when p->use_s0_cell is 1, else ignored.  Replaces malloc_all
  * (declare_tables.cpp:89-143) and ini_fft_all (declare_fft_tables.cpp:26-41);
- * device memory is laid out once for the whole run.                          */
+ * device memory is laid out once for the whole run.
+ *
+ * org_create holds the state in FP32, as the reference's float build
+ * (constants.h, `float` tables).  With zone_size 65 (FFT box 128) every
+ * entry point below runs the LDS-resident kernels; any other zone_size
+ * (3..513: boxes M = 2*zone_size-2 up to 1024, powers of two or not, e.g.
+ * the reference template's zone_size 200, M = 398) runs a per-function
+ * path on full planes in global memory (one launch per reference function,
+ * 2D transforms in double by radix-2 or Bluestein).  org_create_f64 holds
+ * the state in FP64 (the reference with `float` retyped to `double`) on
+ * that same full-plane path, for any zone_size.  Contexts of the full-plane
+ * path step host-driven (org_step runs the per-function sequence; the
+ * reshaping loops' stop tests read back each sweep), ignore org_set_tuning,
+ * and cannot join a batch or run the surrogate (ORG_EINVAL).                */
 int org_create(const org_params *p, const double *s0_cell, org_ctx **out);
+int org_create_f64(const org_params *p, const double *s0_cell, org_ctx **out);
 int org_destroy(org_ctx *ctx);
+/* 4 (FP32 state) or 8 (FP64 state). */
+int org_get_precision(org_ctx *ctx, int32_t *real_bytes);
 
 /* Upload / download a whole field (all planes).  Upload plays the role of
  * the driver's `#pragma acc data copyin` (run_with_active_gels.cpp:83).  After
@@ -130,6 +146,11 @@ int org_set_field(org_ctx *ctx, int field, const float *host);
 int org_get_field(org_ctx *ctx, int field, float *host);
 /* Window-local debug outputs of the last step (see ORG_W_*). */
 int org_get_window(org_ctx *ctx, int which, float *host);
+/* The same with double host buffers: unchanged for FP64 contexts, rounded
+ * to float (set) or widened (get) for FP32 ones. */
+int org_set_field_f64(org_ctx *ctx, int field, const double *host);
+int org_get_field_f64(org_ctx *ctx, int field, double *host);
+int org_get_window_f64(org_ctx *ctx, int which, double *host);
 /* Centres (ncell ints each), areas s and f0 (ncell doubles each). */
 int org_get_centres(org_ctx *ctx, int32_t *xc, int32_t *yc);
 int org_get_areas(org_ctx *ctx, double *s, double *f0);
@@ -312,6 +333,13 @@ int org_compute_hfunc_lum_ecm_noalloc(int N, double dx, const float *lum, const 
  * 2D DFT through the LDS transform used by org_update_all_cortices.  data:
  * 128*128 interleaved (re, im) floats on the host, sign -1 or +1. */
 int org_debug_fft2d(float *data, int sign);
+/* Test hook (no counterpart in the reference): one unnormalised M x M 2D DFT
+ * (M = 2..1024) through the full-plane path's transform: rows then
+ * columns, each line in double (radix-2 for powers of two, Bluestein's
+ * chirp-z otherwise), the input and output stored as float (f64 = 0) or
+ * double (f64 = 1) as that path's contexts store them.  data: M*M
+ * interleaved (re, im) doubles on the host, sign -1 or +1. */
+int org_debug_fft2d_generic(int M, double *data, int sign, int f64);
 
 /* Calls on this context in which a reshaping loop passed rimax (the
  * reference prints "Relaxation failure!" and goes on,

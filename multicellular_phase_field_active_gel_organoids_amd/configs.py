@@ -33,7 +33,7 @@ import math
 
 import numpy as np
 
-from .params import OrgParams, grid_units
+from .params import OrgParams, grid_units, template_defaults
 
 __all__ = ["SplitMix64", "Workload", "make_workload", "make_sweep", "WORKLOADS"]
 
@@ -167,14 +167,15 @@ def label_constants(p: OrgParams) -> OrgParams:
     return p
 
 
-def _cyst_workload(name, N, ncell, R, seed, steps, eps=2.0):
+def _cyst_workload(name, N, ncell, R, seed, steps, eps=2.0, params=None):
     """Cells on a ring around a lumen, as the reference's initial state is
     built: ring radius rcyst = ncell * 0.9 * R / (pi dx) about (N/2, N/2) at
     angles 2 pi n / ncell (initialisation_functions.cpp:117-159, so ring
     neighbours are index neighbours), and where 1 - sum phi > 0 the lumen
     takes 1 - sum phi inside rcyst and the ECM outside
     (initialisation_functions.cpp:74-114).  The cell profile is this
-    module's tanh disc (eps = 2) with rho on the band."""
+    module's tanh disc (R and eps in pixels) with rho on the band.  params:
+    the constants (default: grid units for R and eps)."""
     rng = SplitMix64(seed)
     rc = ncell * 0.9 * R / math.pi
     ang = 2.0 * math.pi * np.arange(ncell) / ncell
@@ -186,8 +187,21 @@ def _cyst_workload(name, N, ncell, R, seed, steps, eps=2.0):
     d = np.sqrt((ii - N // 2) ** 2.0 + (jj - N // 2) ** 2.0)
     lum = np.where((free > 0.0) & (d < rc), free, 0.0)
     ecm = np.where((free > 0.0) & (d >= rc), free, 0.0)
-    p = grid_units(N, ncell, float(R), eps=eps)
+    p = params if params is not None else grid_units(N, ncell, float(R), eps=eps)
     return Workload(name, p, phi, rho, lum, ecm, centres, radii, None, steps)
+
+
+def _template_cyst(name, seed, steps, N=None):
+    """The reference template's own configuration (constants_template.cpp
+    with setup_simulation.py's defaults, params.template_defaults): 6 cells
+    of R = 1 (100 pixels at dx = 0.01) on the ring of ini_ECM_lumen around a
+    lumen, zone_size 200 (FFT box 398), interface width wi = 0.07 (the tanh
+    profile's eps = sqrt(2) wi / dx pixels), on the template's 1200^2 grid
+    unless N is given."""
+    p = template_defaults()
+    if N:
+        p.N = int(N)
+    return _cyst_workload(name, p.N, p.ncell, p.R / p.dx, seed, steps, eps=math.sqrt(2.0) * p.wi / p.dx, params=p)
 
 
 def make_workload(name: str, N: int | None = None, ncell: int | None = None, steps: int | None = None) -> Workload:
@@ -202,6 +216,12 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
     ``cyst6_256``         6 cells R=20 on a ring around a lumen, 256^2, with
                           distinct per-label constants (``label_constants``)
                           and lumen-cell coupling eta_lum_cells = 0.002
+    ``two_cells_512_zone200``  two_cells_256's cells on 512^2 with the
+                          template's zone_size 200 (FFT box 398)
+    ``template6_1200``    the reference template's configuration: 6 cells of
+                          radius 100 pixels on the ini_ECM_lumen ring,
+                          template constants (dx 0.01, zone_size 200, FFT
+                          box 398), 1200^2
     """
     if name == "two_cells_256":
         n_ = N or 256
@@ -225,6 +245,13 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
         return _hex_workload(name, N or 2048, ncell or 256, 14.0, 26.0, 1004, steps or 20000, spacing=52.0)
     if name == "hex":
         return _hex_workload(name, N or 256, ncell or 4, 20.0, 20.0, 1099, steps or 10)
+    if name == "two_cells_512_zone200":
+        wl = make_workload("two_cells_256", N=512, steps=steps)
+        wl.name = name
+        wl.params.zone_size = 200
+        return wl
+    if name == "template6_1200":
+        return _template_cyst(name, 1006, steps or 100, N=N)
     if name == "cyst6_256":
         wl = _cyst_workload(name, N or 256, ncell or 6, 20.0, 1005, steps or 100)
         label_constants(wl.params)

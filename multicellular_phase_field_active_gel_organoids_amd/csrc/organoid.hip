@@ -2,12 +2,16 @@
  * organoid.hip -- host side of the C-ABI declared in include/organoid.h.
  *
  * One org_ctx owns the device state of one organoid and one HIP stream
- * (default priority).  All kernels come from organoid_kernels.hip.
+ * (default priority).  FP32 contexts with the 128-point FFT box
+ * (zone_size 65) run the kernels of organoid_kernels.hip; FP64 contexts and
+ * other box sizes run the per-function full-plane path of
+ * organoid_generic.hip (ctx->gen), behind the same entry points.
  */
 #include <mutex>
 
 #include "organoid_kernels.hip"
 #include "organoid_fno.h"
+#include "organoid_generic.h"
 
 namespace {
 
@@ -43,6 +47,7 @@ struct org_ctx {
     org_params p{};
     DevP dp{};
     hipStream_t stream = nullptr;
+    gen::Generic *gen = nullptr;         /* the full-plane path (FP64, or FFT box != 128); null: LDS kernels */
     double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][ORG_LUM_SPAN][ORG_LUM_MAX] */
     float *xch = nullptr;                /* rows exchanged by k_step's LDS loops [2][2][N][N] */
     unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier: count, generation */
@@ -518,14 +523,49 @@ const char *org_error_string(int code) {
 
 const char *org_build_info(void) { return "organoid_hip gfx950"; }
 
+/* A context of the full-plane path: FP64 (f64 = 1), or FP32 with an FFT box
+ * other than the LDS kernels' 128. */
+static int create_generic_ctx(const org_params *p, const double *s0_cell, int f64, org_ctx **out) {
+    if (p->ncell < 2 || p->ncell > 1024 || p->zone_size < 3 || p->zone_size > 513 || p->N < 2 * p->zone_size + 6 ||
+        p->n_current <= 0) {
+        fprintf(stderr, "organoid: need 2 <= ncell <= 1024, 3 <= zone_size <= 513, N >= 2*zone_size+6, "
+                        "n_current > 0\n");
+        return note_error(ORG_EINVAL);
+    }
+    org_ctx *c = new org_ctx();
+    c->p = *p;
+    c->N = p->N; c->nc = p->ncell; c->zs = p->zone_size; c->W = 2 * c->zs; c->M = 2 * c->zs - 2;
+    c->NN = (size_t)c->N * c->N; c->P = c->NN * c->nc;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        org_destroy(c);
+        return note_error(ORG_EHIP);
+    }
+    const int rc = gen::create_generic(p, s0_cell, f64, c->stream, &c->gen);
+    if (rc) {
+        org_destroy(c);
+        return note_error(rc);
+    }
+    *out = c;
+    return ORG_OK;
+}
+
+int org_create_f64(const org_params *p, const double *s0_cell, org_ctx **out) {
+    if (!p || !out) return note_error(ORG_EINVAL);
+    *out = nullptr;
+    return create_generic_ctx(p, s0_cell, 1, out);
+}
+
+int org_get_precision(org_ctx *c, int32_t *real_bytes) {
+    TRY(check_ctx(c));
+    if (!real_bytes) return note_error(ORG_EINVAL);
+    *real_bytes = c->gen ? c->gen->real_bytes() : (int32_t)sizeof(float);
+    return ORG_OK;
+}
+
 int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     if (!p || !out) return note_error(ORG_EINVAL);
     *out = nullptr;
-    if (p->zone_size != ORG_ZS_MAX) {
-        fprintf(stderr, "organoid: zone_size %d unsupported (the LDS FFT box is %d, zone_size %d)\n",
-                p->zone_size, ORG_MB, ORG_ZS_MAX);
-        return note_error(ORG_EINVAL);
-    }
+    if (p->zone_size != ORG_ZS_MAX) return create_generic_ctx(p, s0_cell, 0, out);
     if (p->ncell < 2 || p->ncell > 1024 || p->N < 2 * p->zone_size + 6 || p->n_current <= 0) {
         fprintf(stderr, "organoid: need 2 <= ncell <= 1024, N >= 2*zone_size+6, n_current > 0\n");
         return note_error(ORG_EINVAL);
@@ -615,6 +655,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
 int org_destroy(org_ctx *c) {
     if (!c) return ORG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->gen;
     for (int k = 0; k < 2; k++) {
         if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
         if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
@@ -643,8 +684,20 @@ static float *field_ptr(org_ctx *c, int f, size_t *n) {
     }
 }
 
+static int gen_rc(int rc) { return rc ? note_error(rc) : ORG_OK; }
+
+static void unprepare_on(org_ctx *c, int field) {
+    if (field == ORG_F_PHI || field == ORG_F_HFUNC || field == ORG_F_RHO) c->prepared = false;
+    c->next_ready = false;
+}
+
 int org_set_field(org_ctx *c, int field, const float *host) {
     TRY(check_ctx(c));
+    if (c->gen) {
+        TRY(gen_rc(c->gen->set_field(field, host, 0)));
+        unprepare_on(c, field);
+        return ORG_OK;
+    }
     size_t n;
     float *d = field_ptr(c, field, &n);
     if (!d || !host) return note_error(ORG_EINVAL);
@@ -657,6 +710,7 @@ int org_set_field(org_ctx *c, int field, const float *host) {
 
 int org_get_field(org_ctx *c, int field, float *host) {
     TRY(check_ctx(c));
+    if (c->gen) return gen_rc(c->gen->get_field(field, host, 0));
     size_t n;
     float *d = field_ptr(c, field, &n);
     if (!d || !host) return note_error(ORG_EINVAL);
@@ -676,6 +730,7 @@ int org_get_field(org_ctx *c, int field, float *host) {
 int org_get_window(org_ctx *c, int which, float *host) {
     TRY(check_ctx(c));
     if (!host) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_window(which, host, 0));
     const float *d = nullptr;
     size_t n = 0;
     switch (which) {
@@ -693,6 +748,7 @@ int org_get_window(org_ctx *c, int which, float *host) {
 int org_get_centres(org_ctx *c, int32_t *xc, int32_t *yc) {
     TRY(check_ctx(c));
     if (!xc || !yc) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_centres(xc, yc));
     HIPCHK(hipMemcpyAsync(xc, c->xc, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(yc, c->yc, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -702,6 +758,7 @@ int org_get_centres(org_ctx *c, int32_t *xc, int32_t *yc) {
 int org_get_areas(org_ctx *c, double *s, double *f0) {
     TRY(check_ctx(c));
     if (!s || !f0) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_areas(s, f0));
     std::vector<float> f(c->nc);
     HIPCHK(hipMemcpyAsync(s, c->s, sizeof(double) * c->nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(f.data(), c->f0, sizeof(float) * c->nc, hipMemcpyDeviceToHost, c->stream));
@@ -713,6 +770,7 @@ int org_get_areas(org_ctx *c, double *s, double *f0) {
 int org_get_iterations(org_ctx *c, int32_t *cells, int32_t *lum, int32_t *ecm) {
     TRY(check_ctx(c));
     if (!cells || !lum || !ecm) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_iterations(cells, lum, ecm));
     LumState st[2];
     HIPCHK(hipMemcpyAsync(cells, c->counts, sizeof(int) * c->nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(st, c->lst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
@@ -723,6 +781,11 @@ int org_get_iterations(org_ctx *c, int32_t *cells, int32_t *lum, int32_t *ecm) {
 
 int org_prepare(org_ctx *c) {
     TRY(check_ctx(c));
+    if (c->gen) {
+        TRY(gen_rc(c->gen->prepare()));
+        c->prepared = true;
+        return ORG_OK;
+    }
     hipLaunchKernelGGL(k_init_box, dim3(c->nc), dim3(256), 0, c->stream, c->dp, c->phi, c->hfunc, c->box);
     TRY(launch_check());
     HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(float) * c->NN, c->stream));
@@ -761,6 +824,7 @@ int org_get_tuning(org_ctx *c, org_tuning *t) {
 int org_set_tuning(org_ctx *c, const org_tuning *t) {
     TRY(check_ctx(c));
     if (!t || c->in_batch) return note_error(ORG_EINVAL);
+    if (c->gen) return ORG_OK;               /* the full-plane path has no tunable launch shape */
     if (t->lum_blocks < 1 || t->lum_blocks > std::min(ORG_LUM_MAX, c->ncu) || t->roles < 1 ||
         t->roles > c->nlp_cap || t->graph_steps < 1 || t->graph_steps > 64)
         return note_error(ORG_EINVAL);
@@ -780,6 +844,7 @@ int org_set_tuning(org_ctx *c, const org_tuning *t) {
 
 int org_set_option(org_ctx *c, int option, int value) {
     TRY(check_ctx(c));
+    if (c->gen) return option == ORG_OPT_WRITE_VELOCITY ? ORG_OK : note_error(ORG_EINVAL);
     switch (option) {
     case ORG_OPT_WRITE_VELOCITY: c->write_v = value ? 1 : 0; break;
 
@@ -795,15 +860,31 @@ int org_set_option(org_ctx *c, int option, int value) {
 /* ---- per-function entry points -----------------------------------------
  * Each may change phi or the centres, so the next batched step recomputes
  * its centres first. */
-int org_find_centres_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_centroid(c)); return harvest_device_error(c); }
-int org_compute_hfunc_cells(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 1); }
-int org_sum_s(org_ctx *c) { TRY(need_prepared(c)); return L_prep(c, 2); }
-int org_compute_hfunc_lum_ecm(org_ctx *c) { TRY(need_prepared(c)); return L_hlum(c); }
-int org_compute_psi(org_ctx *c) { TRY(need_prepared(c)); return L_psi(c); }
-int org_update_all_cortices(org_ctx *c) { TRY(need_prepared(c)); return L_spectral(c); }
-int org_update_lumen_and_ECM(org_ctx *c) { TRY(need_prepared(c)); return L_lumecm_update(c); }
+#define GEN_ROUTE(call) do { if (c->gen) return gen_rc(c->gen->call); } while (0)
+int org_find_centres_cells(org_ctx *c) {
+    TRY(need_prepared(c));
+    GEN_ROUTE(find_centres());
+    TRY(L_centroid(c));
+    return harvest_device_error(c);
+}
+int org_compute_hfunc_cells(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(hfunc_cells()); return L_prep(c, 1); }
+int org_sum_s(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(sum_s()); return L_prep(c, 2); }
+int org_compute_hfunc_lum_ecm(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(hfunc_lum_ecm()); return L_hlum(c); }
+int org_compute_psi(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(psi()); return L_psi(c); }
+int org_update_all_cortices(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(cortices()); return L_spectral(c); }
+int org_update_lumen_and_ECM(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(lumecm_update()); return L_lumecm_update(c); }
+/* the full-plane path's reshaping calls, with the rimax warning */
+static int gen_reshape(org_ctx *c, int what) {
+    int failed = 0;
+    const int rc = what == 2 ? c->gen->reshape_cells(0, &failed)
+                 : what == 3 ? c->gen->reshape_cells(1, &failed)
+                             : c->gen->reshape_lumecm(what, &failed);
+    if (failed) note_relax_failure(c);
+    return gen_rc(rc);
+}
 int org_reshaping_lumen_or_ECM(org_ctx *c, int which_ecm) {
     TRY(need_prepared(c));
+    if (c->gen) return gen_reshape(c, which_ecm ? 1 : 0);
     int f = which_ecm ? 1 : 0;
     TRY(L_lum_normals(c, f, 1));
     TRY(run_lum_loop(c, f, 1, 1));
@@ -821,15 +902,22 @@ int org_reshaping_lumen_or_ECM(org_ctx *c, int which_ecm) {
     }
     return ORG_OK;
 }
-int org_update_all_cells(org_ctx *c) { TRY(need_prepared(c)); return L_cells(c, 1); }
-int org_reshaping_cells(org_ctx *c) { TRY(need_prepared(c)); TRY(L_cells(c, 2)); return harvest_device_error(c); }
+int org_update_all_cells(org_ctx *c) { TRY(need_prepared(c)); GEN_ROUTE(update_cells()); return L_cells(c, 1); }
+int org_reshaping_cells(org_ctx *c) {
+    TRY(need_prepared(c));
+    if (c->gen) return gen_reshape(c, 2);
+    TRY(L_cells(c, 2));
+    return harvest_device_error(c);
+}
 int org_reshaping_cells_from_phi(org_ctx *c) {
     TRY(need_prepared(c));
+    if (c->gen) return gen_reshape(c, 3);
     TRY(L_cells(c, 2 | 4));
     return harvest_device_error(c);
 }
 int org_update_rho_to_shape(org_ctx *c, float rho0, float tol) {
     TRY(need_prepared(c));
+    GEN_ROUTE(rho_to_shape(rho0, tol));
     hipLaunchKernelGGL(k_rho_to_shape, dim3(1024), dim3(256), 0, c->stream, c->phi, c->rho, c->P, rho0, tol);
     TRY(launch_check());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -844,6 +932,15 @@ int org_update_rho_to_shape(org_ctx *c, float rho0, float tol) {
  * program's saved frame (:108). */
 int org_relax_initial_state(org_ctx *c, float rho0, float tol) {
     TRY(need_prepared(c));
+    if (c->gen) {
+        TRY(gen_rc(c->gen->find_centres()));
+        TRY(gen_reshape(c, 3));
+        TRY(gen_rc(c->gen->rho_to_shape(rho0, tol)));
+        TRY(gen_rc(c->gen->zero_gel_inputs()));
+        TRY(gen_rc(c->gen->lumecm_update()));
+        TRY(gen_reshape(c, 0));
+        return gen_reshape(c, 1);
+    }
     TRY(L_centroid(c));
     TRY(harvest_device_error(c));
     TRY(org_reshaping_cells_from_phi(c));
@@ -860,6 +957,10 @@ int org_relax_initial_state(org_ctx *c, float rho0, float tol) {
 }
 int org_commit_step(org_ctx *c) {
     TRY(need_prepared(c));
+    if (c->gen) {
+        TRY(gen_rc(c->gen->commit_rho()));
+        return gen_rc(c->gen->commit_lumecm());
+    }
     TRY(L_commit_cells(c));
     TRY(L_ring_if_pending(c));
     TRY(L_commit_lumecm(c));
@@ -867,15 +968,47 @@ int org_commit_step(org_ctx *c) {
 }
 int org_commit_rho(org_ctx *c) {
     TRY(need_prepared(c));
+    GEN_ROUTE(commit_rho());
     TRY(L_commit_cells(c));
     return L_ring_if_pending(c);
 }
-int org_commit_lumecm(org_ctx *c) { TRY(need_prepared(c)); TRY(L_commit_lumecm(c)); return harvest_device_error(c); }
+int org_commit_lumecm(org_ctx *c) {
+    TRY(need_prepared(c));
+    GEN_ROUTE(commit_lumecm());
+    TRY(L_commit_lumecm(c));
+    return harvest_device_error(c);
+}
+#undef GEN_ROUTE
 
 /* ---- batched form ------------------------------------------------------- */
+/* One timestep of the full-plane path in the driver's order
+ * (run_with_active_gels.cpp:118-184). */
+static int gen_step(org_ctx *c) {
+    gen::Generic *g = c->gen;
+    TRY(gen_rc(g->find_centres()));                      /* :123 */
+    TRY(gen_rc(g->hfunc_cells()));                       /* :133 */
+    TRY(gen_rc(g->hfunc_lum_ecm()));                     /* :134 */
+    TRY(gen_rc(g->psi()));                               /* :139 */
+    TRY(gen_rc(g->sum_s()));                             /* :142 */
+    TRY(gen_rc(g->cortices()));                          /* :130-154 */
+    TRY(gen_rc(g->lumecm_update()));                     /* :157-158 */
+    TRY(gen_reshape(c, 0));                              /* :160 */
+    TRY(gen_reshape(c, 1));                              /* :161 */
+    TRY(gen_rc(g->update_cells()));                      /* :169-170 */
+    TRY(gen_reshape(c, 2));                              /* :164-166, :172 */
+    TRY(gen_rc(g->commit_rho()));                        /* :175-178 */
+    return gen_rc(g->commit_lumecm());                   /* :181-182 */
+}
+
 int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     (void)it0;
     if (!c) return note_error(ORG_EINVAL);
+    if (c->gen) {                          /* host-driven: the reshaping loops' stop tests read back */
+        TRY(need_prepared(c));
+        if (nsteps < 0) return note_error(ORG_EINVAL);
+        for (int s = 0; s < nsteps; s++) TRY(gen_step(c));
+        return ORG_OK;
+    }
     const bool ready = c->next_ready;
     TRY(need_prepared(c));
     if (nsteps < 0) return note_error(ORG_EINVAL);
@@ -959,7 +1092,7 @@ int step_epilogue(org_ctx *c, bool last, bool &need_pre) {
 
 int org_surrogate_samples(org_ctx *c, int it0, int nsteps, float *x, float *v, float *res) {
     (void)it0;
-    if (!c || !x || !v || nsteps < 1) return note_error(ORG_EINVAL);
+    if (!c || !x || !v || nsteps < 1 || c->gen) return note_error(ORG_EINVAL);
     TRY(need_prepared(c));
     TRY(sur_buffers(c, 1));
     TRY(step_prologue(c));
@@ -990,7 +1123,7 @@ int org_surrogate_samples(org_ctx *c, int it0, int nsteps, float *x, float *v, f
 
 int org_step_surrogate(org_ctx *c, fno_net *net, const float *v_given, int it0, int nsteps, int k_exact, float thr,
                        int32_t *exact_cells, float *residuals) {
-    if (!c || (!net && !v_given) || nsteps < 1 || k_exact < 1 || !(thr >= 0.0f)) return note_error(ORG_EINVAL);
+    if (!c || c->gen || (!net && !v_given) || nsteps < 1 || k_exact < 1 || !(thr >= 0.0f)) return note_error(ORG_EINVAL);
     if (net) {
         fno_config fc;
         if (fno_get_config(net, &fc) || fc.M != ORG_MB || fc.batch_max < c->nc) return note_error(ORG_EINVAL);
@@ -1072,6 +1205,10 @@ int org_batch_create(org_ctx *const *ctxs, int n, org_batch **out) {
     *out = nullptr;
     for (int o = 0; o < n; o++) {
         if (!ctxs[o] || ctxs[o]->in_batch) return note_error(ORG_EINVAL);
+        if (ctxs[o]->gen) {
+            fprintf(stderr, "organoid: batches take FP32 contexts with zone_size %d only\n", ORG_ZS_MAX);
+            return note_error(ORG_EINVAL);
+        }
         for (int q = 0; q < o; q++)
             if (ctxs[q] == ctxs[o]) return note_error(ORG_EINVAL);
     }
@@ -1282,6 +1419,61 @@ int org_debug_fft2d(float *data, int sign) {
     if (!rc && hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = note_error(ORG_EHIP);
     (void)hipFree(d);
     return rc;
+}
+
+int org_debug_fft2d_generic(int M, double *data, int sign, int f64) {
+    return gen_rc(gen::debug_fft2d(M, data, sign, f64));
+}
+
+/* ---- double-precision host buffers ----------------------------------------
+ * FP64 contexts move doubles unchanged; FP32 contexts round to float on the
+ * way in and widen on the way out. */
+static size_t field_count(org_ctx *c, int field) {
+    switch (field) {
+    case ORG_F_PHI: case ORG_F_RHO: case ORG_F_HFUNC: return c->P;
+    case ORG_F_LUM: case ORG_F_ECM: case ORG_F_PSI: case ORG_F_HLUM: case ORG_F_HECM:
+    case ORG_F_LUMN: case ORG_F_ECMN: return c->NN;
+    default: return 0;
+    }
+}
+
+int org_set_field_f64(org_ctx *c, int field, const double *host) {
+    TRY(check_ctx(c));
+    if (!host) return note_error(ORG_EINVAL);
+    if (c->gen) {
+        TRY(gen_rc(c->gen->set_field(field, host, 1)));
+        unprepare_on(c, field);
+        return ORG_OK;
+    }
+    const size_t n = field_count(c, field);
+    if (!n) return note_error(ORG_EINVAL);
+    std::vector<float> t(n);
+    for (size_t k = 0; k < n; k++) t[k] = (float)host[k];
+    return org_set_field(c, field, t.data());
+}
+
+int org_get_field_f64(org_ctx *c, int field, double *host) {
+    TRY(check_ctx(c));
+    if (!host) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_field(field, host, 1));
+    const size_t n = field_count(c, field);
+    if (!n) return note_error(ORG_EINVAL);
+    std::vector<float> t(n);
+    TRY(org_get_field(c, field, t.data()));
+    for (size_t k = 0; k < n; k++) host[k] = t[k];
+    return ORG_OK;
+}
+
+int org_get_window_f64(org_ctx *c, int which, double *host) {
+    TRY(check_ctx(c));
+    if (!host) return note_error(ORG_EINVAL);
+    if (c->gen) return gen_rc(c->gen->get_window(which, host, 1));
+    if (which < 0 || which >= ORG_W_COUNT) return note_error(ORG_EINVAL);
+    const size_t n = (size_t)c->nc * (which == ORG_W_PHIN ? (size_t)c->W * c->W : (size_t)c->M * c->M);
+    std::vector<float> t(n);
+    TRY(org_get_window(c, which, t.data()));
+    for (size_t k = 0; k < n; k++) host[k] = t[k];
+    return ORG_OK;
 }
 
 }  /* extern "C" */

@@ -1,0 +1,1262 @@
+/*
+ * organoid_generic.hip -- the per-function timestep on full planes in global
+ * memory, for the forms the LDS kernels of organoid_kernels.hip do not cover:
+ * the FP64 drop-in (the reference retyped to double) and any FFT box
+ * (M = 2*zone_size - 2; the reference's template has zone_size 200, M = 398).
+ *
+ * Each kernel is one function of the reference's time loop
+ * (run_with_active_gels.cpp:118-184) for all cells in one launch, templated
+ * on the stored real type R, with every expression written with the
+ * reference's own literal types: for R = float the double literals promote
+ * intermediate expressions exactly as the reference's float build does
+ * (e.g. rho_functions.cpp:25), for R = double everything is double.  FMA
+ * contraction is off, so each operation rounds as written.  Reductions
+ * (areas, the sum of h(ECM), reshaping deviations) accumulate in double in
+ * a fixed order.  The 2D transforms of update_all_cortices (cufftExecC2C,
+ * useful_functions.cpp:646-660: unnormalised, out of place) run in double
+ * whatever R is, rows then columns, and store R: radix-2 Cooley-Tukey with
+ * the bit-reversed load for powers of two, Bluestein's chirp-z transform
+ * (1970) through power-of-two transforms otherwise; twiddles and chirps are
+ * computed on the host from the definition in double.
+ *
+ * Performance is not the point of this path (one launch per reference
+ * function, full-plane passes as the reference makes them); the batched
+ * LDS form is the fast one.
+ */
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "organoid_generic.h"
+
+#pragma clang fp contract(off)
+
+namespace gen {
+
+template <typename R>
+struct Cx { R x, y; };
+
+/* ---- FFT plan ----------------------------------------------------------- */
+/* Twiddles of the power-of-two transform of length n: tw[k] = (cos, sin) of
+ * 2 pi k / n, k < n/2 (the sign is applied by the kernel). */
+static void host_twiddles(int n, std::vector<double> &tw) {
+    tw.resize(n);
+    for (int k = 0; k < n / 2; k++) {
+        tw[2 * k] = cos(2.0 * M_PI * k / n);
+        tw[2 * k + 1] = sin(2.0 * M_PI * k / n);
+    }
+}
+
+/* In-place radix-2 DFT on the host (definition X_k = sum x_j e^{s 2 pi i jk/n}),
+ * used only to build Bluestein's kernel spectrum. */
+static void host_fft_pow2(double *re, double *im, int n, int sign) {
+    for (int i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+    }
+    std::vector<double> cw(n / 2 + 1), sw(n / 2 + 1);
+    for (int k = 0; k < n / 2; k++) {
+        cw[k] = cos(2.0 * M_PI * k / n);
+        sw[k] = sign * sin(2.0 * M_PI * k / n);
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len / 2, step = n / len;
+        for (int i = 0; i < n; i += len)
+            for (int k = 0; k < half; k++) {
+                const double wr = cw[k * step], wi = sw[k * step];
+                const int a = i + k, b = i + k + half;
+                const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - xr; im[b] = im[a] - xi;
+                re[a] += xr; im[a] += xi;
+            }
+    }
+}
+
+struct FftPlan {
+    int M = 0, L = 0;          /* line length; power-of-two length used (L = M, or Bluestein's) */
+    bool pow2 = true;
+    double *tw = nullptr;      /* device: L/2 (cos, sin) pairs of length L */
+    double *chirp[2] = {nullptr, nullptr};   /* device, sign -1 / +1: w_k (M pairs) */
+    double *khat[2] = {nullptr, nullptr};    /* device, sign -1 / +1: FFT_L of the conjugate chirp (L pairs) */
+};
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* ---- line transforms in LDS (one workgroup per line) --------------------- */
+/* The oracle's radix-2 transform (oracle/organoid_oracle.c fft_pow2): the
+ * input in bit-reversed order, then log2(n) butterfly stages with the
+ * twiddle of stage `len` at k * (n / len); the same operations in the same
+ * order per butterfly, so the values match it. */
+static __device__ void lds_fft_pow2(double *re, double *im, int n, int sign, const double *__restrict__ tw) {
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len / 2, step = n / len;
+        for (int t = threadIdx.x; t < n / 2; t += blockDim.x) {
+            const int grp = t / half, k = t - grp * half;
+            const int a = grp * len + k, b = a + half;
+            const double wr = tw[2 * (k * step)], wi = sign * tw[2 * (k * step) + 1];
+            const double xr = re[b] * wr - im[b] * wi;
+            const double xi = re[b] * wi + im[b] * wr;
+            re[b] = re[a] - xr; im[b] = im[a] - xi;
+            re[a] += xr; im[a] += xi;
+        }
+        __syncthreads();
+    }
+}
+
+static __device__ __forceinline__ int bitrev(int i, int logn) { return (int)(__brev((unsigned)i) >> (32 - logn)); }
+
+/* nlines transforms of length M: line l of plane p = l / lines_per_plane
+ * starts at p * plane + (l % lines_per_plane) * line_step, elements
+ * elem_step apart, read from src and written to dst (which may be src).
+ * Computed in double; TI, TO: the stored types.  LDS: 2 L doubles. */
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void g_fft_lines(const Cx<TI> *src, Cx<TO> *dst,
+                                                   int nlines, int lines_per_plane, long plane, int line_step,
+                                                   int elem_step, int M, int L, int sign, const double *__restrict__ tw,
+                                                   const double *__restrict__ chirp, const double *__restrict__ khat) {
+    extern __shared__ double sh[];
+    double *re = sh, *im = sh + L;
+    const int logL = 31 - __clz(L);
+    for (int line = blockIdx.x; line < nlines; line += gridDim.x) {
+        const long base = (long)(line / lines_per_plane) * plane + (long)(line % lines_per_plane) * line_step;
+        if (!chirp) {
+            for (int k = threadIdx.x; k < M; k += blockDim.x) {
+                const Cx<TI> v = src[base + (long)k * elem_step];
+                const int r = bitrev(k, logL);
+                re[r] = (double)v.x; im[r] = (double)v.y;
+            }
+            __syncthreads();
+            lds_fft_pow2(re, im, L, sign, tw);
+            for (int k = threadIdx.x; k < M; k += blockDim.x) {
+                Cx<TO> o; o.x = (TO)re[k]; o.y = (TO)im[k];
+                dst[base + (long)k * elem_step] = o;
+            }
+        } else {
+            /* Bluestein (the oracle's fft_any): a_k = x_k w_k zero-padded to L,
+             * A = F_-(a), C = A * K, c = F_+(C), X_k = w_k c_k / L */
+            for (int k = threadIdx.x; k < L; k += blockDim.x) {
+                double ar = 0.0, ai = 0.0;
+                if (k < M) {
+                    const Cx<TI> v = src[base + (long)k * elem_step];
+                    const double xr = (double)v.x, xi = (double)v.y, wr = chirp[2 * k], wi = chirp[2 * k + 1];
+                    ar = xr * wr - xi * wi;
+                    ai = xr * wi + xi * wr;
+                }
+                const int r = bitrev(k, logL);
+                re[r] = ar; im[r] = ai;
+            }
+            __syncthreads();
+            lds_fft_pow2(re, im, L, -1, tw);
+            double pr[8], pi_[8];      /* this thread's products, then the bit-reversed reload */
+            int cnt = 0;
+            for (int k = threadIdx.x; k < L; k += blockDim.x, cnt++) {
+                const double kr = khat[2 * k], ki = khat[2 * k + 1];
+                pr[cnt] = re[k] * kr - im[k] * ki;
+                pi_[cnt] = re[k] * ki + im[k] * kr;
+            }
+            __syncthreads();
+            cnt = 0;
+            for (int k = threadIdx.x; k < L; k += blockDim.x, cnt++) {
+                const int r = bitrev(k, logL);
+                re[r] = pr[cnt]; im[r] = pi_[cnt];
+            }
+            __syncthreads();
+            lds_fft_pow2(re, im, L, +1, tw);
+            for (int k = threadIdx.x; k < M; k += blockDim.x) {
+                const double cr = re[k] / L, ci = im[k] / L, wr = chirp[2 * k], wi = chirp[2 * k + 1];
+                Cx<TO> o;
+                o.x = (TO)(cr * wr - ci * wi);
+                o.y = (TO)(cr * wi + ci * wr);
+                dst[base + (long)k * elem_step] = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
+/* ---- the timestep's functions ---------------------------------------------- */
+/* Constants as the reference holds them: float globals for R = float
+ * (constants_template.cpp), double for the FP64 form. */
+template <typename R>
+struct GP {
+    int N, nc, zs, M, rimax, ncur;
+    R dt, dx, sigma, cpl, tau, alpha, beta, eta;
+    R tau_lum, beta_lc, eta_lc, xi_lum, alpha_E, tau_E, beta_ec, eta_ec, xi_E, beta_le;
+    R wi, rdt, friction;
+    R visc[3], aort[3], atan_[3], bb[3], gen[3], deg[3];   /* apical, basal, lateral */
+};
+
+#define GPL(n, i, j) ((size_t)(n) * NN + (size_t)(i) * N + (size_t)(j))
+#define GG(i, j) ((size_t)(i) * N + (size_t)(j))
+
+/* Fixed-order double sum over the block (blockDim a multiple of 64, <= 1024). */
+static __device__ double gblock_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+/* find_centres_cells, useful_functions.cpp:193-227, then the window check
+ * (window plus one pixel inside rows/columns 1..N-2). */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_centroid(GP<R> P, const R *__restrict__ phi, int *__restrict__ xc,
+                                                   int *__restrict__ yc, int *__restrict__ err) {
+    __shared__ long long red[48];
+    const int N = P.N, n = blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    long long xs = 0, ys = 0, cnt = 0;
+    for (size_t k = threadIdx.x; k < NN; k += blockDim.x)
+        if (phi[n * NN + k] > 0.5) { xs += (long long)(k / N); ys += (long long)(k % N); cnt += 1; }
+    for (int o = 32; o > 0; o >>= 1) {
+        xs += __shfl_down(xs, o, 64); ys += __shfl_down(ys, o, 64); cnt += __shfl_down(cnt, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w] = xs; red[16 + w] = ys; red[32 + w] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long X = 0, Y = 0, C = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { X += red[k]; Y += red[16 + k]; C += red[32 + k]; }
+        if (C == 0) { atomicCAS(err, 0, ORG_EEMPTY_CELL); return; }
+        const int x = (int)(X / C), y = (int)(Y / C);
+        xc[n] = x; yc[n] = y;
+        const int x0 = x - P.zs, y0 = y - P.zs;
+        if (x0 - 1 < 1 || x0 + 2 * P.zs > N - 2 || y0 - 1 < 1 || y0 + 2 * P.zs > N - 2) atomicCAS(err, 0, ORG_EWINDOW);
+    }
+}
+
+/* compute_hfunc_3d_tables_kernels_loop_n, useful_functions.cpp:233-251:
+ * block (r, n) does row r of cell n's window. */
+template <typename R>
+__global__ void g_hfunc_cells(GP<R> P, const R *__restrict__ phi, R *__restrict__ hf, const int *__restrict__ xc,
+                              const int *__restrict__ yc) {
+    const int N = P.N, n = blockIdx.y, zs = P.zs;
+    const size_t NN = (size_t)N * N;
+    const int i = xc[n] - zs + blockIdx.x;
+    for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x) {
+        const R v = phi[GPL(n, i, j)];
+        hf[GPL(n, i, j)] = v * v * (3.0 - 2.0 * v);
+    }
+}
+
+/* compute_all_hfunc_2d_tables_kernels, useful_functions.cpp:272-285 */
+template <typename R>
+__global__ void g_hfunc_lumecm(GP<R> P, const R *__restrict__ lum, const R *__restrict__ ecm, R *__restrict__ hl,
+                               R *__restrict__ he) {
+    const size_t NN = (size_t)P.N * P.N;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < NN; k += (size_t)gridDim.x * blockDim.x) {
+        const R a = lum[k], e = ecm[k];
+        hl[k] = a * a * (3.0 - 2.0 * a);
+        he[k] = e * e * (3.0 - 2.0 * e);
+    }
+}
+
+/* boundary_conditions_cells, useful_functions.cpp:391-420: periodic ghost
+ * ring (period N-2) of `planes` planes; block p does plane p. */
+template <typename R>
+__global__ void g_bc_cells(int N, R *__restrict__ t) {
+    const size_t NN = (size_t)N * N;
+    R *q = t + (size_t)blockIdx.x * NN;
+    for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
+        q[(size_t)i * N] = q[(size_t)i * N + N - 2];
+        q[(size_t)i * N + N - 1] = q[(size_t)i * N + 1];
+        q[i] = q[(size_t)N * (N - 2) + i];
+        q[(size_t)N * (N - 1) + i] = q[(size_t)N + i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q[0] = q[(size_t)N * (N - 2) + N - 2];
+        q[N - 1] = q[(size_t)N * (N - 2) + 1];
+        q[(size_t)N * (N - 1)] = q[(size_t)N + N - 2];
+        q[(size_t)N * (N - 1) + N - 1] = q[(size_t)N + 1];
+    }
+}
+
+/* compute_normal_grad_phi, useful_functions.cpp:326-354, interior of every
+ * plane (grid-stride over nc * N * N). */
+template <typename R>
+__global__ void g_norm_grad(GP<R> P, const R *__restrict__ phi, R *__restrict__ ngp) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N, tot = NN * P.nc;
+    const R dx = P.dx;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t n = k / NN, r = k % NN;
+        const int i = (int)(r / N), j = (int)(r % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
+        const R a = phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)];
+        const R c = phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)];
+        ngp[k] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
+    }
+}
+
+/* compute_psi_kernels, useful_functions.cpp:305-322 (cells summed in order) */
+template <typename R>
+__global__ void g_psi(GP<R> P, const R *__restrict__ hf, R *__restrict__ psi) {
+    const size_t NN = (size_t)P.N * P.N;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < NN; k += (size_t)gridDim.x * blockDim.x) {
+        R acc = 0.0;
+        for (int n = 0; n < P.nc; n++) acc += hf[(size_t)n * NN + k];
+        psi[k] = acc;
+    }
+}
+
+/* sum_s_kernels, useful_functions.cpp:168-190, one block per cell: the area
+ * sum_k h dx^2 in double, f0 = alpha (s0 - s). */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ hf, const double *__restrict__ s0,
+                                                double *__restrict__ s, R *__restrict__ f0) {
+    __shared__ double red[16];
+    const size_t NN = (size_t)P.N * P.N;
+    const int n = blockIdx.x;
+    const R dx = P.dx;
+    double acc = 0.0;
+    for (size_t k = threadIdx.x; k < NN; k += blockDim.x) acc += (double)(hf[(size_t)n * NN + k] * dx * dx);
+    const double tot = gblock_sum(acc, red);
+    if (threadIdx.x == 0) {
+        s[n] = tot;
+        f0[n] = (R)(P.alpha * ((R)s0[n] - (R)tot));
+    }
+}
+
+/* compute_normal_to_phi, useful_functions.cpp:356-388 (interior; the ring
+ * by g_bc_cells) */
+template <typename R>
+__global__ void g_normals(GP<R> P, const R *__restrict__ phi, const R *__restrict__ ngp, R *__restrict__ jx,
+                          R *__restrict__ jy) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N, tot = NN * P.nc;
+    const R dx = P.dx;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t n = k / NN, r = k % NN;
+        const int i = (int)(r / N), j = (int)(r % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
+        const R g = ngp[k];
+        if (g > 0.00) {
+            jx[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)]) / g;
+            jy[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)]) / g;
+        } else {
+            jx[k] = 0.0;
+            jy[k] = 0.0;
+        }
+    }
+}
+
+/* create_turnover_rates_tensions, rho_functions.cpp:286-405, at one pixel:
+ * 0 apical, 1 basal, 2 lateral (ring neighbours n-1, n+1, strict '>'). */
+template <typename R>
+static __device__ __forceinline__ int g_label(const GP<R> &P, const R *__restrict__ phi, const R *__restrict__ lum,
+                                              const R *__restrict__ ecm, int n, size_t g) {
+    const size_t NN = (size_t)P.N * P.N;
+    const int nc = P.nc;
+    const int n1 = n == 0 ? nc - 1 : n - 1, n2 = n == nc - 1 ? 0 : n + 1;
+    const R ph = phi[(size_t)n * NN + g];
+    const R c1l = ph * lum[g], c1e = ph * ecm[g];
+    int index;
+    R mx;
+    if (c1l > c1e) { index = 0; mx = c1l; } else { index = 1; mx = c1e; }
+    const R c1c2 = ph * phi[(size_t)n1 * NN + g];
+    if (c1c2 > mx) { index = 2; mx = c1c2; }
+    const R c1c3 = ph * phi[(size_t)n2 * NN + g];
+    if (c1c3 > mx) { index = 3; }
+    return index == 3 ? 2 : index;
+}
+
+/* update_all_cortices, first part (rho_functions.cpp:413-440): per cell and
+ * FFT-box pixel, the five transform inputs load_temp_fft puts in temp_fft
+ * (useful_functions.cpp:605-625): rho, the reaction term of create_deg_gen
+ * (rho_functions.cpp:193-209) with prod_norm_delta (rho_functions.cpp:13-30),
+ * and pi_xx, pi_xy, pi_yy (rho_functions.cpp:32-115); the turnover maps at
+ * the pixel from its label.  in5: [5][nc][M][M]. */
+template <typename R>
+__global__ void g_spec_load(GP<R> P, const R *__restrict__ phi, const R *__restrict__ rho,
+                            const R *__restrict__ ngp, const R *__restrict__ lum, const R *__restrict__ ecm,
+                            const int *__restrict__ xc, const int *__restrict__ yc, Cx<R> *__restrict__ in5) {
+    const int N = P.N, M = P.M, zs = P.zs, n = blockIdx.y;
+    const size_t NN = (size_t)N * N, MM = (size_t)M * M, PL5 = MM * P.nc;
+    const R dx = P.dx, sigma = P.sigma;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < M * M; k += gridDim.x * blockDim.x) {
+        const int i = k / M, j = k % M;
+        const int gi = i + 1 + xc[n] - zs, gj = j + 1 + yc[n] - zs;
+        const size_t g = GG(gi, gj), q = (size_t)n * NN + g;
+        const int lab = g_label(P, phi, lum, ecm, n, g);
+        const R ph = phi[q], r = rho[q], gr = ngp[q];
+        const R pnd = gr * exp(-(ph - 0.5) * (ph - 0.5) / 2.0 / sigma / sigma) / sqrt(2.0 * M_PI) / sigma;
+        const R deg = P.deg[lab], gen = P.gen[lab], at = P.atan_[lab], ao = P.aort[lab], bb = P.bb[lab];
+        const R dxp = phi[q + N] - phi[q - N];
+        const R dyp = phi[q + 1] - phi[q - 1];
+        const R base = -at * r * r * r + bb * r * r * r * r;
+        R pxx = base, pxy = 0.0, pyy = base;
+        if (gr > 0.0) {
+            pxx += (at - ao) * r * r * r * (1.0 / (4.0 * dx * dx)) * dxp * dxp / gr / gr;
+            pxy += (at - ao) * r * r * r * (1.0 / (4.0 * dx * dx)) * dxp * dyp / gr / gr;
+            pyy += (at - ao) * r * r * r * (1.0 / (4.0 * dx * dx)) * dyp * dyp / gr / gr;
+        }
+        const size_t o = (size_t)n * MM + k;
+        Cx<R> v;
+        v.y = 0.0;
+        v.x = r; in5[o] = v;
+        v.x = -deg * r + gen * pnd; in5[PL5 + o] = v;
+        v.x = pxx; in5[2 * PL5 + o] = v;
+        v.x = pxy; in5[3 * PL5 + o] = v;
+        v.x = pyy; in5[4 * PL5 + o] = v;
+    }
+}
+
+/* update_rho_velocity, rho_functions.cpp:133-190: per mode, the 2x2 force
+ * balance solved for (v_x, v_y); the viscosity is the turnover map's value at
+ * the real-space pixel with the mode's indices (:149-150).  f5 as in5
+ * transformed; v2: [2][nc][M][M]. */
+template <typename R>
+__global__ void g_spec_solve(GP<R> P, const R *__restrict__ phi, const R *__restrict__ lum,
+                             const R *__restrict__ ecm, const int *__restrict__ xc, const int *__restrict__ yc,
+                             const Cx<R> *__restrict__ f5, Cx<R> *__restrict__ v2) {
+    const int N = P.N, M = P.M, zs = P.zs, n = blockIdx.y;
+    const size_t MM = (size_t)M * M, PL5 = MM * P.nc;
+    const R dx = P.dx, xi = P.friction;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < M * M; k += gridDim.x * blockDim.x) {
+        const int i = k / M, j = k % M;
+        const int ir = (i < M / 2) ? i : i - M, jr = (j < M / 2) ? j : j - M;      /* :159-162 */
+        const R kx = (2.0 * M_PI / (M * dx)) * ir;                                  /* :164 */
+        const R ky = (2.0 * M_PI / (M * dx)) * jr;                                  /* :165 */
+        const size_t g = GG(i + 1 + xc[n] - zs, j + 1 + yc[n] - zs);
+        const R eta = P.visc[g_label(P, phi, lum, ecm, n, g)];
+        const size_t o = (size_t)n * MM + k;
+        const Cx<R> pxx = f5[2 * PL5 + o], pxy = f5[3 * PL5 + o], pyy = f5[4 * PL5 + o];
+        const R D = -(4.0 * eta * eta * kx * kx * ky * ky + xi * xi + 3.0 * eta * xi * (kx * kx + ky * ky)
+                      + 2.0 * eta * eta * (kx * kx * kx * kx + ky * ky * ky * ky));
+        const R A = xi + 2.0 * eta * (kx * kx + ky * ky / 2.0);
+        Cx<R> y, x;
+        y.x = 0.0; y.y = 0.0; x.x = 0.0; x.y = 0.0;
+        y.x += eta * kx * ky * (kx * pxx.y + ky * pxy.y);                     /* :170 */
+        y.x -= A * (ky * pyy.y + kx * pxy.y);                                  /* :171 */
+        y.x /= D;                                                              /* :172 */
+        y.y -= eta * kx * ky * (kx * pxx.x + ky * pxy.x);                     /* :175 */
+        y.y += A * (ky * pyy.x + kx * pxy.x);                                  /* :176 */
+        y.y /= D;                                                              /* :177 */
+        x.x += kx * pxx.y + ky * pxy.y - eta * kx * ky * y.x;                 /* :181 */
+        x.x /= A;                                                              /* :182 */
+        x.y += -kx * pxx.x - ky * pxy.x - eta * kx * ky * y.y;                /* :185 */
+        x.y /= A;                                                              /* :186 */
+        v2[o] = x;
+        v2[PL5 + o] = y;
+    }
+}
+
+/* load_fft_rho_v, rho_functions.cpp:267-284, for one component:
+ * out = rho Re(F^-1 v)/M^2 (Im 0); the velocity on the box to vbox. */
+template <typename R>
+__global__ void g_spec_rhov(GP<R> P, const R *__restrict__ rho, const int *__restrict__ xc,
+                            const int *__restrict__ yc, const Cx<R> *__restrict__ treal, Cx<R> *__restrict__ out,
+                            R *__restrict__ vbox) {
+    const int N = P.N, M = P.M, zs = P.zs, n = blockIdx.y;
+    const size_t NN = (size_t)N * N, MM = (size_t)M * M;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < M * M; k += gridDim.x * blockDim.x) {
+        const int i = k / M, j = k % M;
+        const size_t q = (size_t)n * NN + GG(i + 1 + xc[n] - zs, j + 1 + yc[n] - zs);
+        const size_t o = (size_t)n * MM + k;
+        Cx<R> t;
+        t.x = rho[q] * treal[o].x / (M * M);
+        t.y = 0.0;
+        out[o] = t;
+        if (vbox) vbox[o] = treal[o].x / (M * M);
+    }
+}
+
+/* update_fft_rho, rho_functions.cpp:213-251 (forward Euler, no diffusion) */
+template <typename R>
+__global__ void g_spec_update(GP<R> P, Cx<R> *__restrict__ frho, const Cx<R> *__restrict__ freact,
+                              const Cx<R> *__restrict__ frvx, const Cx<R> *__restrict__ frvy) {
+    const int M = P.M;
+    const size_t MM = (size_t)M * M, tot = MM * P.nc;
+    const R dt = P.dt, dx = P.dx;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < tot; o += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(o % MM), i = k / M, j = k % M;
+        Cx<R> f = frho[o];
+        f.x += dt * freact[o].x;                                               /* :222 */
+        f.y += dt * freact[o].y;                                               /* :223 */
+        const int ir = (i < M / 2) ? i : i - M, jr = (j < M / 2) ? j : j - M;
+        f.x += dt * (2.0 * M_PI / (M * dx)) * (ir * frvx[o].y + jr * frvy[o].y);   /* :243 */
+        f.y -= dt * (2.0 * M_PI / (M * dx)) * (ir * frvx[o].x + jr * frvy[o].x);   /* :247 */
+        frho[o] = f;
+    }
+}
+
+/* copy_to_rhon, rho_functions.cpp:253-265: rhon = Re(F^-1 rho^)/M^2 on the box */
+template <typename R>
+__global__ void g_spec_commit(GP<R> P, const int *__restrict__ xc, const int *__restrict__ yc,
+                              const Cx<R> *__restrict__ t, R *__restrict__ rhon) {
+    const int N = P.N, M = P.M, zs = P.zs, n = blockIdx.y;
+    const size_t NN = (size_t)N * N, MM = (size_t)M * M;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < M * M; k += gridDim.x * blockDim.x) {
+        const int i = k / M, j = k % M;
+        rhon[(size_t)n * NN + GG(i + 1 + xc[n] - zs, j + 1 + yc[n] - zs)] = t[(size_t)n * MM + k].x / (M * M);
+    }
+}
+
+/* 9-point Laplacian numerator of cell_phase_functions.cpp (e.g. :198) */
+#define GL9(t, g) (-12.0 * (t)[(g)] + 2.0 * ((t)[(g) + N] + (t)[(g) - N] + (t)[(g) + 1] + (t)[(g) - 1]) \
+                   + (t)[(g) + N + 1] + (t)[(g) + N - 1] + (t)[(g) - N - 1] + (t)[(g) - N + 1])
+
+/* Fixed-order double partial sums of f(k) over k < n: block b sums the
+ * contiguous range [b * per, (b + 1) * per), part[b]. */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_sum_parts(const R *__restrict__ a, const R *__restrict__ b, R dx,
+                                                    size_t n, size_t per, int mode, double *__restrict__ part) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    const size_t k0 = (size_t)blockIdx.x * per, k1 = k0 + per < n ? k0 + per : n;
+    for (size_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        if (mode == 0) acc += (double)(a[k] * dx * dx);                      /* sum of h(ECM) dx^2 */
+        else acc += fabs((double)(a[k] - b[k])) * dx * dx;                   /* reshaping deviation */
+    }
+    const double t = gblock_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ void g_sum_final(const double *__restrict__ part, int nparts, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < nparts; k++) s += part[k];
+        *out = s;
+    }
+}
+
+/* update_lumen_and_ECM, cell_phase_functions.cpp:372-444 (interior), then
+ * boundary_conditions_lumen_and_ECM, useful_functions.cpp:439-462. */
+template <typename R>
+__global__ void g_lumecm_update(GP<R> P, const R *__restrict__ lum, const R *__restrict__ ecm,
+                                const R *__restrict__ psi, const R *__restrict__ hl, const R *__restrict__ he,
+                                const double *__restrict__ sum_ecm, R *__restrict__ lumn, R *__restrict__ ecmn) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx, dt = P.dt;
+    const R sum_e = (R)(*sum_ecm);
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / N), j = (int)(q % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
+            lumn[q] = 0.0;                                                  /* edges and corners */
+            ecmn[q] = 1.0;
+            continue;
+        }
+        const R l = lum[q], e = ecm[q];
+        const R lap = GL9(psi, q);
+        R ln = l;                                                          /* :408 */
+        ln += (1.0 / P.tau_lum) * l * (1.0 - l) * dt * (                   /* :409 */
+            P.xi_lum - P.beta_lc * psi[q] + P.eta_lc * lap / (4.0 * dx * dx) - P.beta_le * he[q]);
+        R en = e;                                                          /* :424 */
+        en += (1.0 / P.tau_E) * e * (1.0 - e) * dt * (                     /* :425 */
+            P.alpha_E * (N * N * dx * dx - sum_e) + P.xi_E - P.beta_ec * psi[q]
+            + P.eta_ec * lap / (4.0 * dx * dx) - P.beta_le * hl[q]);
+        lumn[q] = ln;
+        ecmn[q] = en;
+    }
+}
+
+/* reshaping_lumen_or_ECM's |grad| of the old field (cell_phase_functions.cpp:
+ * 734-736) and normals from the new one over it (:757-763), interior; the
+ * rings by g_bc_cells. */
+template <typename R>
+__global__ void g_lum_grad(GP<R> P, const R *__restrict__ old, R *__restrict__ ng) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / N), j = (int)(q % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
+        const R a = old[q + N] - old[q - N], c = old[q + 1] - old[q - 1];
+        ng[q] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
+    }
+}
+template <typename R>
+__global__ void g_lum_normals(GP<R> P, const R *__restrict__ nw, const R *__restrict__ ng, R *__restrict__ jx,
+                              R *__restrict__ jy) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / N), j = (int)(q % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
+        const R g = ng[q];
+        if (g != 0.0) {
+            jx[q] = (1.0 / (2.0 * dx)) * (nw[q + N] - nw[q - N]) / g;
+            jy[q] = (1.0 / (2.0 * dx)) * (nw[q + 1] - nw[q - 1]) / g;
+        } else {
+            jx[q] = 0.0;
+            jy[q] = 0.0;
+        }
+    }
+}
+
+/* The conservative interface flux of both reshaping loops
+ * (cell_phase_functions.cpp:511-547 and :802-838) at (i, j): the new value
+ * of un given the relaxed field u and the normals, row stride N. */
+template <typename R>
+static __device__ __forceinline__ R g_reshape(R un, const R *__restrict__ u, const R *__restrict__ jx,
+                                              const R *__restrict__ jy, int N, size_t q, R wi, R rdt, R dx) {
+#define U(a, b) u[q + (long)(a) * N + (b)]
+#define JX(a, b) jx[q + (long)(a) * N + (b)]
+#define JY(a, b) jy[q + (long)(a) * N + (b)]
+    un += 0.5 * (JX(1, 0) + JX(0, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(1, 0) - U(0, 0)) * 0.5 * (JX(1, 0) + JX(0, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(0, 1) - U(1, -1) - U(0, -1)) * 0.5 * (JY(1, 0) + JY(0, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(1, 0) * (1.0 - U(1, 0)) + U(0, 0) * (1.0 - U(0, 0))));
+    un += 0.5 * (JY(0, 1) + JY(0, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 1) - U(0, 0)) * 0.5 * (JY(0, 1) + JY(0, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(1, 0) - U(-1, 1) - U(-1, 0)) * 0.5 * (JX(0, 1) + JX(0, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 1) * (1.0 - U(0, 1)) + U(0, 0) * (1.0 - U(0, 0))));
+    un -= 0.5 * (JX(0, 0) + JX(-1, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(-1, 0)) * 0.5 * (JX(0, 0) + JX(-1, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(0, 1) + U(-1, 1) - U(0, -1) - U(-1, -1)) * 0.5 * (JY(0, 0) + JY(-1, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(-1, 0) * (1.0 - U(-1, 0))));
+    un -= 0.5 * (JY(0, 0) + JY(0, -1)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(0, -1)) * 0.5 * (JY(0, 0) + JY(0, -1))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 0) + U(1, -1) - U(-1, 0) - U(-1, -1)) * 0.5 * (JX(0, 0) + JX(0, -1))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(0, -1) * (1.0 - U(0, -1))));
+    return un;
+#undef U
+#undef JX
+#undef JY
+}
+
+/* One sweep of reshaping_lumen_or_ECM's loop (cell_phase_functions.cpp:
+ * 777-844): the reference copies lum <- lumn and updates lumn from its own
+ * value; here u holds the previous iterate and un the buffer of the one
+ * before, so the update starts from u[q] (the value the copy would leave in
+ * lumn).  Then the Dirichlet ring of boundary_conditions_dirichlet
+ * (useful_functions.cpp:422-437: edges `side`, corners 0). */
+template <typename R>
+__global__ void g_lum_sweep(GP<R> P, const R *__restrict__ u, R *__restrict__ un, const R *__restrict__ jx,
+                            const R *__restrict__ jy, R side) {
+    const int N = P.N;
+    const size_t NN = (size_t)N * N;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / N), j = (int)(q % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
+            const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
+            un[q] = corner ? (R)0.0 : side;
+            continue;
+        }
+        un[q] = g_reshape(u[q], u, jx, jy, N, q, P.wi, P.rdt, P.dx);
+    }
+}
+
+/* update_all_cells, cell_phase_functions.cpp:126-217, on every cell's window
+ * (block (r, n): window row r of cell n); temp_phase = psi - hfunc_n
+ * (:141-153) is formed where the Laplacian reads it. */
+template <typename R>
+__global__ void g_update_cells(GP<R> P, const R *__restrict__ phi, const R *__restrict__ rho,
+                               const R *__restrict__ ngp, const R *__restrict__ psi, const R *__restrict__ hf,
+                               const R *__restrict__ hl, const R *__restrict__ he, const R *__restrict__ f0,
+                               const int *__restrict__ xc, const int *__restrict__ yc, R *__restrict__ phin) {
+    const int N = P.N, zs = P.zs, n = blockIdx.y;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx, dt = P.dt, tau = P.tau, sigma = P.sigma;
+    const R *ph = phi + (size_t)n * NN, *rh = rho + (size_t)n * NN, *ng = ngp + (size_t)n * NN;
+    const R *hn = hf + (size_t)n * NN;
+    const int i = xc[n] - zs + blockIdx.x;
+    const R f0n = f0[n];
+    for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x) {
+        const size_t q = GG(i, j);
+        R tp[9];                                   /* temp_phase on the 3 x 3 stencil */
+        {
+            int k = 0;
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++, k++) {
+                    const size_t g = q + (long)a * N + b;
+                    tp[k] = psi[g] - hn[g];
+                }
+        }
+        const double l9tp = -12.0 * tp[4] + 2.0 * (tp[7] + tp[1] + tp[5] + tp[3]) + tp[8] + tp[6] + tp[0] + tp[2];
+        const R v = ph[q];
+        R pn = v;                                                                      /* :170 */
+        if (ng[q] > 0.0) {                                                             /* :177-186 */
+            pn -= P.cpl / tau * (dt / dx / dx / 4.0) * exp(-(v - 0.5) * (v - 0.5) / 2.0 / sigma / sigma)
+                  / sqrt(2.0 * M_PI) / sigma
+                  * ((rh[q + N] - rh[q - N]) * (ph[q + N] - ph[q - N]) + (rh[q + 1] - rh[q - 1]) * (ph[q + 1] - ph[q - 1]))
+                  / ng[q];
+        }
+        pn += (1.0 / tau) * v * (1.0 - v) * dt * (                                     /* :190-211 */
+            f0n
+            - P.beta * tp[4]
+            + P.eta * l9tp / (4.0 * dx * dx)
+            - P.beta_lc * hl[q]
+            + P.eta_lc * GL9(hl, q) / (4.0 * dx * dx)
+            - P.beta_ec * he[q]
+            + P.eta_ec * GL9(he, q) / (4.0 * dx * dx));
+        phin[(size_t)n * NN + q] = pn;
+    }
+}
+
+/* reshaping_cells, cell_phase_functions.cpp:447-580: one block per cell runs
+ * the cell's whole loop on its window (copy phase <- phasen, flux update,
+ * deviation sum_window |phasen - phase| dx^2 / N_current in double, stop
+ * when <= rtol 2 pi R wi rdt); phase ends at the second-to-last iterate. */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_relax_cells(GP<R> P, R *__restrict__ phi, R *__restrict__ phin,
+                                                      const R *__restrict__ jx, const R *__restrict__ jy,
+                                                      const int *__restrict__ xc, const int *__restrict__ yc,
+                                                      double thr, double rdev0, int *__restrict__ counts,
+                                                      int *__restrict__ relax_flag) {
+    __shared__ double red[16];
+    const int N = P.N, zs = P.zs, n = blockIdx.x, W = 2 * zs;
+    const size_t NN = (size_t)N * N;
+    R *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
+    const R *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
+    const int x0 = xc[n] - zs, y0 = yc[n] - zs;
+    const R dx = P.dx;
+    double rdev = rdev0;
+    int ri = 0, count = 0;
+    while (rdev > thr) {
+        for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+            const size_t q = GG(x0 + k / W, y0 + k % W);
+            ph[q] = pn[q];
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+            const size_t q = GG(x0 + k / W, y0 + k % W);
+            pn[q] = g_reshape(pn[q], ph, jxn, jyn, N, q, P.wi, P.rdt, dx);
+        }
+        __syncthreads();
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+            const size_t q = GG(x0 + k / W, y0 + k % W);
+            acc += fabs((double)(pn[q] - ph[q])) * dx * dx / P.ncur;
+        }
+        rdev = gblock_sum(acc, red);
+        count++;
+        if (ri > P.rimax) { if (threadIdx.x == 0) *relax_flag = 1; break; }   /* :565-568 */
+        ri++;
+    }
+    if (threadIdx.x == 0) counts[n] = count;
+}
+
+template <typename R>
+__global__ void g_copy(const R *__restrict__ src, R *__restrict__ dst, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
+#undef GPL
+#undef GG
+}  // namespace gen
+
+namespace gen {
+
+/* update_rho_to_shape (cell_phase_functions.cpp:42-62) on every plane, in
+ * double as the reference's double literals make it */
+template <typename R>
+__global__ void g_rho_to_shape(const R *__restrict__ phi, R *__restrict__ rho, size_t count, double rho0,
+                               double tol) {
+    const double floor_v = rho0 * exp(-1.0 / (2.0 * tol) + 1.0 / (1.0 + tol));
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        const R p = phi[e];
+        R r = rho[e];
+        if (p != 0.0) {
+            const double pd = p, h = pd * pd * (3.0 - 2.0 * pd);
+            r = (R)(rho0 * exp(-1.0 / (4.0 * h * (1.0 - h) + tol) + 1.0 / (1.0 + tol)));
+        }
+        if ((double)r < floor_v) r = 0.0;
+        rho[e] = r;
+    }
+}
+
+/* side x side squares of every cell's planes, anchored at (xc + off, yc + off) */
+template <typename R>
+__global__ void g_gather(int N, const R *__restrict__ src, const int *__restrict__ xc, const int *__restrict__ yc,
+                         int off, int side, R *__restrict__ out) {
+    const int n = blockIdx.y;
+    const size_t NN = (size_t)N * N;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < side * side; k += gridDim.x * blockDim.x)
+        out[(size_t)n * side * side + k] =
+            src[(size_t)n * NN + (size_t)(xc[n] + off + k / side) * N + (yc[n] + off + k % side)];
+}
+
+/* ---- host side ----------------------------------------------------------- */
+
+#define GHIP(expr)                                                                                        \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess) {                                                                           \
+            fprintf(stderr, "organoid (generic path): %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                                                  \
+            return ORG_EHIP;                                                                              \
+        }                                                                                                 \
+    } while (0)
+#define GTRY(x) do { int _r = (x); if (_r) return _r; } while (0)
+#define GLAUNCH(kern, grid, block, shm, ...)                                                             \
+    do {                                                                                                  \
+        hipLaunchKernelGGL(kern, grid, block, shm, s, __VA_ARGS__);                                       \
+        GHIP(hipGetLastError());                                                                          \
+    } while (0)
+
+static int dev_alloc(std::vector<void *> &allocs, void **p, size_t bytes) {
+    GHIP(hipMalloc(p, bytes ? bytes : 1));
+    allocs.push_back(*p);
+    return ORG_OK;
+}
+
+/* Bluestein tables of one sign: the chirp w_k = exp(i sign pi k^2 / M) with
+ * k^2 reduced mod 2M first (exact), and the L-point transform (sign -1) of
+ * the conjugate chirp wrapped around (b_k = b_{L-k} = conj w_k). */
+static void bluestein_tables(int M, int L, int sign, std::vector<double> &chirp, std::vector<double> &khat) {
+    chirp.assign(2 * M, 0.0);
+    for (int k = 0; k < M; k++) {
+        const long long k2 = ((long long)k * k) % (2LL * M);
+        const double ang = sign * M_PI * (double)k2 / M;
+        chirp[2 * k] = cos(ang);
+        chirp[2 * k + 1] = sin(ang);
+    }
+    std::vector<double> br(L, 0.0), bi(L, 0.0);
+    br[0] = chirp[0];
+    bi[0] = -chirp[1];
+    for (int k = 1; k < M; k++) {
+        br[k] = chirp[2 * k]; bi[k] = -chirp[2 * k + 1];
+        br[L - k] = chirp[2 * k]; bi[L - k] = -chirp[2 * k + 1];
+    }
+    host_fft_pow2(br.data(), bi.data(), L, -1);
+    khat.resize(2 * L);
+    for (int k = 0; k < L; k++) { khat[2 * k] = br[k]; khat[2 * k + 1] = bi[k]; }
+}
+
+static int make_plan(int M, FftPlan &pl, std::vector<void *> &allocs) {
+    pl.M = M;
+    pl.pow2 = is_pow2(M);
+    pl.L = M;
+    if (!pl.pow2) {
+        pl.L = 1;
+        while (pl.L < 2 * M - 1) pl.L <<= 1;
+    }
+    if (pl.L > 2048) {
+        fprintf(stderr, "organoid (generic path): FFT box %d too large (line transforms up to 2048 points)\n", M);
+        return ORG_EINVAL;
+    }
+    std::vector<double> tw;
+    host_twiddles(pl.L, tw);
+    GTRY(dev_alloc(allocs, (void **)&pl.tw, sizeof(double) * tw.size()));
+    GHIP(hipMemcpy(pl.tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    if (!pl.pow2) {
+        for (int si = 0; si < 2; si++) {
+            std::vector<double> ch, kh;
+            bluestein_tables(M, pl.L, si ? +1 : -1, ch, kh);
+            GTRY(dev_alloc(allocs, (void **)&pl.chirp[si], sizeof(double) * ch.size()));
+            GTRY(dev_alloc(allocs, (void **)&pl.khat[si], sizeof(double) * kh.size()));
+            GHIP(hipMemcpy(pl.chirp[si], ch.data(), sizeof(double) * ch.size(), hipMemcpyHostToDevice));
+            GHIP(hipMemcpy(pl.khat[si], kh.data(), sizeof(double) * kh.size(), hipMemcpyHostToDevice));
+        }
+    }
+    return ORG_OK;
+}
+
+/* nplanes M x M transforms src -> dst (unnormalised, sign -1 or +1): rows
+ * into the double work buffer, then columns out of it (cufftExecC2C on each
+ * plane, useful_functions.cpp:646-660, with the double intermediate the
+ * oracle's transform keeps). */
+template <typename R>
+static int fft2d_planes(hipStream_t s, const FftPlan &pl, const Cx<R> *src, Cx<R> *dst, Cx<double> *work,
+                        int nplanes, int sign) {
+    const int M = pl.M, L = pl.L, si = sign > 0 ? 1 : 0;
+    const long plane = (long)M * M;
+    const int nlines = nplanes * M;
+    const dim3 grid(std::min(nlines, 8192)), block(256);
+    const size_t shm = 2 * sizeof(double) * L;
+    const double *ch = pl.pow2 ? nullptr : pl.chirp[si], *kh = pl.pow2 ? nullptr : pl.khat[si];
+    GLAUNCH((g_fft_lines<R, double>), grid, block, shm, src, work, nlines, M, plane, M, 1, M, L, sign, pl.tw, ch, kh);
+    GLAUNCH((g_fft_lines<double, R>), grid, block, shm, (const Cx<double> *)work, dst, nlines, M, plane, 1, M, M, L,
+            sign, pl.tw, ch, kh);
+    return ORG_OK;
+}
+
+template <typename R>
+struct GenCtx final : Generic {
+    org_params p{};
+    GP<R> P{};
+    hipStream_t s = nullptr;
+    int N = 0, nc = 0, zs = 0, W = 0, M = 0;
+    size_t NN = 0, PP = 0, MM = 0;
+    std::vector<void *> allocs;
+    FftPlan plan;
+    /* per-cell planes (declare_tables.cpp:97-213) */
+    R *phi = nullptr, *phin = nullptr, *rho = nullptr, *rhon = nullptr, *hf = nullptr, *ngp = nullptr;
+    R *jx = nullptr, *jy = nullptr;
+    /* shared planes; the loop's pairs swap roles between sweeps */
+    R *lum = nullptr, *lumn = nullptr, *ecm = nullptr, *ecmn = nullptr, *psi_ = nullptr, *hl = nullptr, *he = nullptr;
+    R *ngl = nullptr, *jlx = nullptr, *jly = nullptr;
+    /* FFT boxes: five transforms' inputs / spectra ([5][nc][M][M]), the velocity modes ([2][nc][M][M]) */
+    Cx<R> *A = nullptr, *B = nullptr, *V = nullptr;
+    Cx<double> *work = nullptr;
+    R *vxb = nullptr, *vyb = nullptr, *win = nullptr;
+    R *f0 = nullptr;
+    double *s_ = nullptr, *s0 = nullptr, *sum_e = nullptr, *part = nullptr, *dev_sum = nullptr;
+    int *xc = nullptr, *yc = nullptr, *err = nullptr, *counts = nullptr, *relax = nullptr;
+    int nparts = 1;
+    int count_lum = 0, count_ecm = 0;
+    int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
+
+    ~GenCtx() override {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void *q : allocs) (void)hipFree(q);
+    }
+    int real_bytes() const override { return (int)sizeof(R); }
+
+    template <typename T>
+    int A_(T **ptr, size_t count) { return dev_alloc(allocs, (void **)ptr, count * sizeof(T)); }
+
+    int init(const org_params &pp, const double *s0_cell, hipStream_t stream) {
+        p = pp;
+        s = stream;
+        N = p.N; nc = p.ncell; zs = p.zone_size; W = 2 * zs; M = 2 * zs - 2;
+        NN = (size_t)N * N; PP = NN * nc; MM = (size_t)M * M;
+        P.N = N; P.nc = nc; P.zs = zs; P.M = M; P.rimax = p.rimax; P.ncur = p.n_current;
+        P.dt = (R)p.dt; P.dx = (R)p.dx; P.sigma = (R)p.sigma; P.cpl = (R)p.coupling_PF_AG;
+        P.tau = (R)p.tau; P.alpha = (R)p.alpha; P.beta = (R)p.beta; P.eta = (R)p.eta;
+        P.tau_lum = (R)p.tau_lum; P.beta_lc = (R)p.beta_lum_cells; P.eta_lc = (R)p.eta_lum_cells;
+        P.xi_lum = (R)p.xi_lum; P.alpha_E = (R)p.alpha_ECM; P.tau_E = (R)p.tau_ECM;
+        P.beta_ec = (R)p.beta_ECM_cells; P.eta_ec = (R)p.eta_ECM_cells; P.xi_E = (R)p.xi_ECM;
+        P.beta_le = (R)p.beta_lum_ECM; P.wi = (R)p.wi; P.rdt = (R)p.rdt; P.friction = (R)p.friction;
+        const double tab[6][3] = {{p.visc_ap, p.visc_bas, p.visc_lat}, {p.a_ort_ap, p.a_ort_bas, p.a_ort_lat},
+                                  {p.a_tan_ap, p.a_tan_bas, p.a_tan_lat}, {p.b_ap, p.b_bas, p.b_lat},
+                                  {p.gen_ap, p.gen_bas, p.gen_lat}, {p.deg_ap, p.deg_bas, p.deg_lat}};
+        for (int l = 0; l < 3; l++) {
+            P.visc[l] = (R)tab[0][l]; P.aort[l] = (R)tab[1][l]; P.atan_[l] = (R)tab[2][l];
+            P.bb[l] = (R)tab[3][l]; P.gen[l] = (R)tab[4][l]; P.deg[l] = (R)tab[5][l];
+        }
+        GTRY(make_plan(M, plan, allocs));
+        for (R **f : {&phi, &phin, &rho, &rhon, &hf, &ngp, &jx, &jy}) GTRY(A_(f, PP));
+        for (R **f : {&lum, &lumn, &ecm, &ecmn, &psi_, &hl, &he, &ngl, &jlx, &jly}) GTRY(A_(f, NN));
+        GTRY(A_(&A, 5 * nc * MM)); GTRY(A_(&B, 5 * nc * MM)); GTRY(A_(&V, 2 * nc * MM));
+        GTRY(A_(&work, 5 * nc * MM));
+        GTRY(A_(&vxb, nc * MM)); GTRY(A_(&vyb, nc * MM));
+        GTRY(A_(&win, (size_t)nc * std::max(W * W, M * M)));
+        nparts = (int)std::min<size_t>(256, (NN + 4095) / 4096);
+        GTRY(A_(&f0, nc)); GTRY(A_(&s_, nc)); GTRY(A_(&s0, nc)); GTRY(A_(&sum_e, 1));
+        GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
+        GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&err, 1)); GTRY(A_(&counts, nc)); GTRY(A_(&relax, 1));
+        std::vector<double> s0v(nc);
+        for (int n = 0; n < nc; n++) s0v[n] = (p.use_s0_cell && s0_cell) ? s0_cell[n] : p.s0;
+        std::vector<int> cen(nc, N / 2);
+        GHIP(hipMemcpy(s0, s0v.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+        GHIP(hipMemcpy(xc, cen.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+        GHIP(hipMemcpy(yc, cen.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+        for (R *f : {phi, phin, rho, rhon, hf, ngp, jx, jy}) GHIP(hipMemset(f, 0, sizeof(R) * PP));
+        for (R *f : {lum, lumn, ecm, ecmn, psi_, hl, he, ngl, jlx, jly}) GHIP(hipMemset(f, 0, sizeof(R) * NN));
+        GHIP(hipMemset(vxb, 0, sizeof(R) * nc * MM)); GHIP(hipMemset(vyb, 0, sizeof(R) * nc * MM));
+        GHIP(hipMemset(f0, 0, sizeof(R) * nc)); GHIP(hipMemset(s_, 0, sizeof(double) * nc));
+        GHIP(hipMemset(err, 0, sizeof(int))); GHIP(hipMemset(counts, 0, sizeof(int) * nc));
+        GHIP(hipMemset(relax, 0, sizeof(int)));
+        GHIP(hipDeviceSynchronize());          /* the memsets above ran on the null stream */
+        return ORG_OK;
+    }
+
+    R *field(int f, size_t *n) {
+        switch (f) {
+        case ORG_F_PHI: *n = PP; return phi;
+        case ORG_F_RHO: *n = PP; return rho;
+        case ORG_F_HFUNC: *n = PP; return hf;
+        case ORG_F_LUM: *n = NN; return lum;
+        case ORG_F_ECM: *n = NN; return ecm;
+        case ORG_F_PSI: *n = NN; return psi_;
+        case ORG_F_HLUM: *n = NN; return hl;
+        case ORG_F_HECM: *n = NN; return he;
+        case ORG_F_LUMN: *n = NN; return lumn;
+        case ORG_F_ECMN: *n = NN; return ecmn;
+        default: *n = 0; return nullptr;
+        }
+    }
+
+    /* host <-> device in the caller's precision */
+    int upload(R *d, const void *host, size_t n, int f64) {
+        if ((sizeof(R) == 8) == (f64 != 0)) {
+            GHIP(hipMemcpyAsync(d, host, n * sizeof(R), hipMemcpyHostToDevice, s));
+            GHIP(hipStreamSynchronize(s));
+            return ORG_OK;
+        }
+        std::vector<R> t(n);
+        if (f64) for (size_t k = 0; k < n; k++) t[k] = (R)((const double *)host)[k];
+        else for (size_t k = 0; k < n; k++) t[k] = (R)((const float *)host)[k];
+        GHIP(hipMemcpyAsync(d, t.data(), n * sizeof(R), hipMemcpyHostToDevice, s));
+        GHIP(hipStreamSynchronize(s));
+        return ORG_OK;
+    }
+    int download(const R *d, void *host, size_t n, int f64) {
+        if ((sizeof(R) == 8) == (f64 != 0)) {
+            GHIP(hipMemcpyAsync(host, d, n * sizeof(R), hipMemcpyDeviceToHost, s));
+            GHIP(hipStreamSynchronize(s));
+            return ORG_OK;
+        }
+        std::vector<R> t(n);
+        GHIP(hipMemcpyAsync(t.data(), d, n * sizeof(R), hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        if (f64) for (size_t k = 0; k < n; k++) ((double *)host)[k] = (double)t[k];
+        else for (size_t k = 0; k < n; k++) ((float *)host)[k] = (float)t[k];
+        return ORG_OK;
+    }
+
+    int set_field(int f, const void *host, int f64) override {
+        size_t n;
+        R *d = field(f, &n);
+        if (!d || !host) return ORG_EINVAL;
+        return upload(d, host, n, f64);
+    }
+    int get_field(int f, void *host, int f64) override {
+        size_t n;
+        R *d = field(f, &n);
+        if (!d || !host) return ORG_EINVAL;
+        return download(d, host, n, f64);
+    }
+    int get_window(int which, void *host, int f64) override {
+        if (!host) return ORG_EINVAL;
+        const dim3 blk(256);
+        switch (which) {
+        case ORG_W_PHIN:
+            GLAUNCH((g_gather<R>), dim3((W * W + 255) / 256, nc), blk, 0, N, (const R *)phin, xc, yc, -zs, W, win);
+            return download(win, host, (size_t)nc * W * W, f64);
+        case ORG_W_RHON:
+            GLAUNCH((g_gather<R>), dim3((M * M + 255) / 256, nc), blk, 0, N, (const R *)rhon, xc, yc, 1 - zs, M, win);
+            return download(win, host, (size_t)nc * MM, f64);
+        case ORG_W_VX: return download(vxb, host, (size_t)nc * MM, f64);
+        case ORG_W_VY: return download(vyb, host, (size_t)nc * MM, f64);
+        default: return ORG_EINVAL;
+        }
+    }
+    int get_centres(int32_t *x, int32_t *y) override {
+        GHIP(hipMemcpyAsync(x, xc, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+        GHIP(hipMemcpyAsync(y, yc, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        return ORG_OK;
+    }
+    int get_areas(double *sv, double *f0v) override {
+        std::vector<R> f(nc);
+        GHIP(hipMemcpyAsync(sv, s_, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+        GHIP(hipMemcpyAsync(f.data(), f0, sizeof(R) * nc, hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        for (int n = 0; n < nc; n++) f0v[n] = (double)f[n];
+        return ORG_OK;
+    }
+    int get_iterations(int32_t *cells, int32_t *l, int32_t *e) override {
+        GHIP(hipMemcpyAsync(cells, counts, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        *l = count_lum;
+        *e = count_ecm;
+        return ORG_OK;
+    }
+
+    int copy(R *dst, const R *src, size_t n) {
+        GHIP(hipMemcpyAsync(dst, src, n * sizeof(R), hipMemcpyDeviceToDevice, s));
+        return ORG_OK;
+    }
+    int grid_for(size_t n) const { return (int)std::min<size_t>(4096, (n + 255) / 256); }
+    int bc(R *planes, int nplanes) {
+        GLAUNCH((g_bc_cells<R>), dim3(nplanes), dim3(256), 0, N, planes);
+        return ORG_OK;
+    }
+    int window_check() const { return win_err; }
+
+    /* run_with_active_gels.cpp:92-98: lumn <- lum, ECMn <- ECM, phin <- phi, rhon <- rho */
+    int prepare() override {
+        GTRY(copy(ecmn, ecm, NN));
+        GTRY(copy(lumn, lum, NN));
+        GTRY(copy(phin, phi, PP));
+        GTRY(copy(rhon, rho, PP));
+        GHIP(hipMemsetAsync(psi_, 0, sizeof(R) * NN, s));
+        GHIP(hipStreamSynchronize(s));
+        count_lum = count_ecm = 0;
+        return ORG_OK;
+    }
+    int find_centres() override {
+        GHIP(hipMemsetAsync(err, 0, sizeof(int), s));
+        GLAUNCH((g_centroid<R>), dim3(nc), dim3(1024), 0, P, (const R *)phi, xc, yc, err);
+        int e = 0;
+        GHIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        if (e == ORG_EEMPTY_CELL) return e;
+        win_err = e;
+        return e;
+    }
+    int hfunc_cells() override {
+        GTRY(window_check());
+        GLAUNCH((g_hfunc_cells<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, hf, (const int *)xc, (const int *)yc);
+        return ORG_OK;
+    }
+    int hfunc_lum_ecm() override {
+        GLAUNCH((g_hfunc_lumecm<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)lum, (const R *)ecm, hl, he);
+        return ORG_OK;
+    }
+    int psi() override {
+        GLAUNCH((g_psi<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)hf, psi_);
+        return ORG_OK;
+    }
+    int sum_s() override {
+        GLAUNCH((g_sum_s<R>), dim3(nc), dim3(1024), 0, P, (const R *)hf, (const double *)s0, s_, f0);
+        return ORG_OK;
+    }
+    /* compute_normal_grad_phi + boundary_conditions_cells (run_with_active_gels.cpp:135-136) */
+    int norm_grad() {
+        GLAUNCH((g_norm_grad<R>), dim3(grid_for(PP)), dim3(256), 0, P, (const R *)phi, ngp);
+        return bc(ngp, nc);
+    }
+    /* run_with_active_gels.cpp:130-154: turnover maps, prod_norm_delta and
+     * the whole per-cell spectral loop of update_all_cortices
+     * (rho_functions.cpp:409-471) for all cells at once */
+    int cortices() override {
+        GTRY(window_check());
+        GTRY(norm_grad());
+        const size_t pl = (size_t)nc * MM;
+        const dim3 g(std::max(1, std::min(64, (int)((MM + 255) / 256))), nc), b(256);
+        GLAUNCH((g_spec_load<R>), g, b, 0, P, (const R *)phi, (const R *)rho, (const R *)ngp, (const R *)lum,
+                (const R *)ecm, (const int *)xc, (const int *)yc, A);
+        GTRY(fft2d_planes<R>(s, plan, A, B, work, 5 * nc, -1));                       /* :417-440 */
+        GLAUNCH((g_spec_solve<R>), g, b, 0, P, (const R *)phi, (const R *)lum, (const R *)ecm, (const int *)xc,
+                (const int *)yc, (const Cx<R> *)B, V);                                 /* :445 */
+        GTRY(fft2d_planes<R>(s, plan, V, A, work, 2 * nc, +1));                       /* :449, :455 */
+        GLAUNCH((g_spec_rhov<R>), g, b, 0, P, (const R *)rho, (const int *)xc, (const int *)yc, (const Cx<R> *)A,
+                A + 2 * pl, vxb);
+        GLAUNCH((g_spec_rhov<R>), g, b, 0, P, (const R *)rho, (const int *)xc, (const int *)yc,
+                (const Cx<R> *)(A + pl), A + 3 * pl, vyb);
+        GTRY(fft2d_planes<R>(s, plan, A + 2 * pl, A, work, 2 * nc, -1));              /* :451, :457 */
+        GLAUNCH((g_spec_update<R>), dim3(grid_for(pl)), b, 0, P, B, (const Cx<R> *)(B + pl), (const Cx<R> *)A,
+                (const Cx<R> *)(A + pl));                                              /* :462 */
+        GTRY(fft2d_planes<R>(s, plan, B, A + 4 * pl, work, nc, +1));                  /* :464 */
+        GLAUNCH((g_spec_commit<R>), g, b, 0, P, (const int *)xc, (const int *)yc, (const Cx<R> *)(A + 4 * pl), rhon);
+        return bc(rhon, nc);                                                           /* :469 */
+    }
+    /* fixed-order double sum (mode 0: h dx^2 over a; 1: |a - b| dx^2) into dev_sum */
+    int plane_sum(const R *a, const R *b2, int mode) {
+        const size_t per = (NN + nparts - 1) / nparts;
+        GLAUNCH((g_sum_parts<R>), dim3(nparts), dim3(1024), 0, a, b2, P.dx, NN, per, mode, part);
+        GLAUNCH(g_sum_final, dim3(1), dim3(64), 0, (const double *)part, nparts, dev_sum);
+        return ORG_OK;
+    }
+    int lumecm_update() override {
+        GTRY(plane_sum(he, nullptr, 0));                                               /* :376-386 */
+        GHIP(hipMemcpyAsync(sum_e, dev_sum, sizeof(double), hipMemcpyDeviceToDevice, s));
+        GLAUNCH((g_lumecm_update<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)lum, (const R *)ecm,
+                (const R *)psi_, (const R *)hl, (const R *)he, (const double *)sum_e, lumn, ecmn);
+        return ORG_OK;
+    }
+    int reshape_lumecm(int which, int *failed) override {
+        R *&u = which ? ecm : lum;
+        R *&un = which ? ecmn : lumn;
+        const R side = which ? (R)1.0 : (R)0.0;
+        const R rtol = (R)p.rtol, rdt = (R)p.rdt;
+        GLAUNCH((g_lum_grad<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)u, ngl);   /* :734-736 */
+        GTRY(bc(ngl, 1));                                                                     /* :741 */
+        GLAUNCH((g_lum_normals<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx, jly);
+        GTRY(bc(jlx, 1));                                                                     /* :770-771 */
+        GTRY(bc(jly, 1));
+        double rdev = (double)(p.rdev_ini * rtol * rdt);                                      /* :720 */
+        const double thr = (double)(p.rdev_threshold * rtol * rdt);                           /* :774 */
+        int ri = 0, count = 0;
+        *failed = 0;
+        while (rdev > thr) {
+            std::swap(u, un);                       /* u: the previous iterate; un: rewritten */
+            GLAUNCH((g_lum_sweep<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)u, un, (const R *)jlx,
+                    (const R *)jly, side);
+            GTRY(plane_sum(un, u, 1));                                                        /* :848-855 */
+            GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
+            GHIP(hipStreamSynchronize(s));
+            count++;
+            if (ri > p.rimax) { *failed = 1; break; }                                         /* :862-865 */
+            ri++;
+        }
+        (which ? count_ecm : count_lum) = count;
+        return ORG_OK;
+    }
+    int update_cells() override {
+        GTRY(window_check());
+        GTRY(norm_grad());
+        GLAUNCH((g_update_cells<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, (const R *)rho, (const R *)ngp,
+                (const R *)psi_, (const R *)hf, (const R *)hl, (const R *)he, (const R *)f0, (const int *)xc,
+                (const int *)yc, phin);
+        return bc(phin, nc);                                          /* run_with_active_gels.cpp:170 */
+    }
+    int reshape_cells(int from_phi, int *failed) override {
+        GTRY(window_check());
+        if (from_phi) GTRY(copy(phin, phi, PP));
+        GTRY(norm_grad());
+        GLAUNCH((g_normals<R>), dim3(grid_for(PP)), dim3(256), 0, P, (const R *)phi, (const R *)ngp, jx, jy);
+        GTRY(bc(jx, nc));                                             /* run_with_active_gels.cpp:164-166 */
+        GTRY(bc(jy, nc));
+        const R rtol = (R)p.rtol, Rr = (R)p.R, wi = (R)p.wi, rdt = (R)p.rdt;
+        const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
+        const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
+        GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+        GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
+                (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
+        int f = 0;
+        GHIP(hipMemcpyAsync(&f, relax, sizeof(int), hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        *failed = f;
+        return ORG_OK;
+    }
+    int commit_rho() override { return copy(rho, rhon, PP); }
+    int commit_lumecm() override {
+        GTRY(copy(ecm, ecmn, NN));
+        return copy(lum, lumn, NN);
+    }
+    int rho_to_shape(double rho0, double tol) override {
+        GLAUNCH((g_rho_to_shape<R>), dim3(grid_for(PP)), dim3(256), 0, (const R *)phi, rho, PP, rho0, tol);
+        GHIP(hipStreamSynchronize(s));
+        return ORG_OK;
+    }
+    int zero_gel_inputs() override {
+        GHIP(hipMemsetAsync(psi_, 0, sizeof(R) * NN, s));
+        GHIP(hipMemsetAsync(hl, 0, sizeof(R) * NN, s));
+        GHIP(hipMemsetAsync(he, 0, sizeof(R) * NN, s));
+        return ORG_OK;
+    }
+};
+
+int create_generic(const org_params *p, const double *s0_cell, int f64, hipStream_t stream, Generic **out) {
+    *out = nullptr;
+    if (2 * p->zone_size - 2 < 2 || p->N < 2 * p->zone_size + 6) return ORG_EINVAL;
+    int rc;
+    if (f64) {
+        auto *g = new GenCtx<double>();
+        rc = g->init(*p, s0_cell, stream);
+        if (rc) { delete g; return rc; }
+        *out = g;
+    } else {
+        auto *g = new GenCtx<float>();
+        rc = g->init(*p, s0_cell, stream);
+        if (rc) { delete g; return rc; }
+        *out = g;
+    }
+    return ORG_OK;
+}
+
+template <typename R>
+static int debug_fft2d_t(int M, double *data, int sign) {
+    std::vector<void *> allocs;
+    FftPlan pl;
+    Cx<R> *buf = nullptr;
+    Cx<double> *work = nullptr;
+    hipStream_t s = nullptr;
+    int rc = make_plan(M, pl, allocs);
+    const size_t MM = (size_t)M * M;
+    std::vector<Cx<R>> h(MM);
+    for (size_t k = 0; k < MM; k++) { h[k].x = (R)data[2 * k]; h[k].y = (R)data[2 * k + 1]; }
+    if (!rc) rc = dev_alloc(allocs, (void **)&buf, sizeof(Cx<R>) * MM);
+    if (!rc) rc = dev_alloc(allocs, (void **)&work, sizeof(Cx<double>) * MM);
+    if (!rc && hipMemcpy(buf, h.data(), sizeof(Cx<R>) * MM, hipMemcpyHostToDevice) != hipSuccess) rc = ORG_EHIP;
+    if (!rc) rc = fft2d_planes<R>(s, pl, buf, buf, work, 1, sign);
+    if (!rc && hipMemcpy(h.data(), buf, sizeof(Cx<R>) * MM, hipMemcpyDeviceToHost) != hipSuccess) rc = ORG_EHIP;
+    for (void *q : allocs) (void)hipFree(q);
+    if (rc) return rc;
+    for (size_t k = 0; k < MM; k++) { data[2 * k] = (double)h[k].x; data[2 * k + 1] = (double)h[k].y; }
+    return ORG_OK;
+}
+
+int debug_fft2d(int M, double *data, int sign, int f64) {
+    if (M < 2 || M > 1024 || !data || (sign != 1 && sign != -1)) return ORG_EINVAL;
+    return f64 ? debug_fft2d_t<double>(M, data, sign) : debug_fft2d_t<float>(M, data, sign);
+}
+
+}  // namespace gen

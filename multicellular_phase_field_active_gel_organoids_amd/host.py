@@ -60,8 +60,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib = ctypes.CDLL(path)
     vp, i32 = ctypes.c_void_p, ctypes.c_int
     lib.org_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    lib.org_create_f64.argtypes = [vp, vp, ctypes.POINTER(vp)]
     lib.org_destroy.argtypes = [vp]
-    for n in ("org_set_field", "org_get_field", "org_get_window"):
+    lib.org_get_precision.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    for n in ("org_set_field", "org_get_field", "org_get_window", "org_set_field_f64", "org_get_field_f64",
+              "org_get_window_f64"):
         getattr(lib, n).argtypes = [vp, i32, vp]
     lib.org_get_centres.argtypes = [vp, vp, vp]
     lib.org_get_areas.argtypes = [vp, vp, vp]
@@ -79,6 +82,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_kernel_time.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.org_set_option.argtypes = [vp, i32, i32]
     lib.org_debug_fft2d.argtypes = [vp, i32]
+    lib.org_debug_fft2d_generic.argtypes = [i32, vp, i32, i32]
     lib.org_batch_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
     lib.org_batch_step.argtypes = [vp, i32, i32]
     lib.org_batch_sync.argtypes = [vp]
@@ -113,18 +117,31 @@ class Tuning(ctypes.Structure):
 
 
 class Organoid:
-    """One organoid resident on the GPU (library-owned device memory)."""
+    """One organoid resident on the GPU (library-owned device memory).
 
-    def __init__(self, params: OrgParams, s0_cell=None):
+    precision "f32" (the reference's float build; zone_size 65 runs the LDS
+    kernels, any other zone_size the full-plane path) or "f64" (the
+    reference retyped to double, full-plane path).  Fields come back in the
+    context's precision."""
+
+    def __init__(self, params: OrgParams, s0_cell=None, precision: str = "f32"):
+        if precision not in ("f32", "f64"):
+            raise ValueError(f"precision {precision!r}: 'f32' or 'f64'")
         self.lib = load_library()
         self.params = params.copy()
         self.nc, self.N = params.ncell, params.N
         self.W, self.M = params.W, params.M
         self._s0 = None if s0_cell is None else np.ascontiguousarray(s0_cell, np.float64)
         h = ctypes.c_void_p()
-        _chk(self.lib.org_create(ctypes.byref(self.params), None if self._s0 is None else _p(self._s0),
-                                 ctypes.byref(h)), "org_create")
+        create = self.lib.org_create_f64 if precision == "f64" else self.lib.org_create
+        _chk(create(ctypes.byref(self.params), None if self._s0 is None else _p(self._s0), ctypes.byref(h)),
+             "org_create_f64" if precision == "f64" else "org_create")
         self.h = h
+        rb = ctypes.c_int32()
+        _chk(self.lib.org_get_precision(self.h, ctypes.byref(rb)), "org_get_precision")
+        self.dtype = np.float64 if rb.value == 8 else np.float32
+        # the per-function full-plane path (FP64, or an FFT box other than 128)
+        self.generic = self.dtype == np.float64 or params.zone_size != 65
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -149,20 +166,30 @@ class Organoid:
         return (self.nc, self.N, self.N) if per_cell else (self.N, self.N)
 
     def set_field(self, field: int, arr) -> None:
-        a = np.ascontiguousarray(arr, np.float32)
+        f64 = self.dtype == np.float64
+        a = np.ascontiguousarray(arr, self.dtype)
         if a.shape != self._size(field):
             raise ValueError(f"field {field}: shape {a.shape} != {self._size(field)}")
-        _chk(self.lib.org_set_field(self.h, field, _p(a)), "org_set_field")
+        if f64:
+            _chk(self.lib.org_set_field_f64(self.h, field, _p(a)), "org_set_field_f64")
+        else:
+            _chk(self.lib.org_set_field(self.h, field, _p(a)), "org_set_field")
 
     def get_field(self, field: int) -> np.ndarray:
-        out = np.empty(self._size(field), np.float32)
-        _chk(self.lib.org_get_field(self.h, field, _p(out)), "org_get_field")
+        out = np.empty(self._size(field), self.dtype)
+        if self.dtype == np.float64:
+            _chk(self.lib.org_get_field_f64(self.h, field, _p(out)), "org_get_field_f64")
+        else:
+            _chk(self.lib.org_get_field(self.h, field, _p(out)), "org_get_field")
         return out
 
     def get_window(self, which: int) -> np.ndarray:
         side = self.W if which == W_PHIN else self.M
-        out = np.empty((self.nc, side, side), np.float32)
-        _chk(self.lib.org_get_window(self.h, which, _p(out)), "org_get_window")
+        out = np.empty((self.nc, side, side), self.dtype)
+        if self.dtype == np.float64:
+            _chk(self.lib.org_get_window_f64(self.h, which, _p(out)), "org_get_window_f64")
+        else:
+            _chk(self.lib.org_get_window(self.h, which, _p(out)), "org_get_window")
         return out
 
     def load_workload(self, wl) -> None:
@@ -403,6 +430,17 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def debug_fft2d_generic(a: np.ndarray, sign: int, f64: bool) -> np.ndarray:
+    """M x M unnormalised 2D DFT through the full-plane path's transform
+    (radix-2 or Bluestein lines in double; input and output stored as
+    float64 when f64 else float32)."""
+    lib = load_library()
+    M = a.shape[0]
+    buf = np.ascontiguousarray(np.stack([a.real, a.imag], -1), np.float64)
+    _chk(lib.org_debug_fft2d_generic(M, _p(buf), sign, 1 if f64 else 0), "org_debug_fft2d_generic")
+    return buf[..., 0] + 1j * buf[..., 1]
 
 
 def debug_fft2d(a: np.ndarray, sign: int) -> np.ndarray:

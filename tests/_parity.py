@@ -70,13 +70,16 @@ def iterations_agree(gi, oi, what):
     return int((d > 0).sum())
 
 
-def compare_driver_order(g, o, wl, tag: str) -> None:
+def compare_driver_order(g, o, wl, tag: str, scale: float = 1.0) -> None:
     """One timestep through the per-function entry points on the GPU and the
     oracle, in the driver's order (run_with_active_gels.cpp:123-182), each
     output compared after its call.  Tolerances (relative L2, FP32 vs FP64):
     stencil / pointwise fields 2e-6, spectral outputs 2e-5 (four FP32
-    transforms), areas 1e-6; centres and lumen/ECM counts exact."""
+    transforms), areas 1e-6; centres and lumen/ECM counts exact.  `scale`
+    multiplies every tolerance (1e-5 for an FP64 context, which must also
+    match every reshaping count)."""
     zs, W, M = wl.params.zone_size, wl.params.W, wl.params.M
+    t = lambda v: v * scale  # noqa: E731
     print(f"case {tag}: per-function step", flush=True)
     g.find_centres_cells(); o.call("find_centres_cells")
     xc, yc = g.centres()
@@ -85,42 +88,47 @@ def compare_driver_order(g, o, wl, tag: str) -> None:
 
     o.call("create_turnover_rates_tensions")
     g.compute_hfunc_3d_tables_kernels_loop_n(); o.call("compute_hfunc_cells")
-    assert rel(g.get_field(host.F_HFUNC), o.get("hfunc")) < 2e-6, tag
+    assert rel(g.get_field(host.F_HFUNC), o.get("hfunc")) < t(2e-6), tag
     g.compute_all_hfunc_2d_tables_kernels(); o.call("compute_hfunc_lum_ecm")
-    assert rel(g.get_field(host.F_HECM), o.get("hecm")) < 2e-6, tag
-    assert rel(g.get_field(host.F_HLUM), o.get("hlum")) < 2e-6, tag
+    assert rel(g.get_field(host.F_HECM), o.get("hecm")) < t(2e-6), tag
+    assert rel(g.get_field(host.F_HLUM), o.get("hlum")) < t(2e-6), tag
     o.call("compute_normal_grad_phi"); o.call("compute_prod_norm_delta")
     g.compute_psi_kernels(); o.call("compute_psi")
-    assert rel(g.get_field(host.F_PSI), o.get("psi")) < 2e-6, tag
+    assert rel(g.get_field(host.F_PSI), o.get("psi")) < t(2e-6), tag
     s = g.sum_s_kernels(); o.call("sum_s")
     os_, of0 = o.areas()
-    assert np.allclose(s, os_, rtol=1e-6), tag
+    assert np.allclose(s, os_, rtol=t(1e-6)), tag
     _, f0 = g.areas()
-    assert np.allclose(f0, of0, rtol=1e-4, atol=1e-3), tag
+    assert np.allclose(f0, of0, rtol=t(1e-4), atol=t(1e-3)), tag
 
     g.update_all_cortices(); o.call("update_all_cortices")
     r = {k: rel(g.get_window(w), o.get(k)) for k, w in (("vx", host.W_VX), ("vy", host.W_VY))}
     r["rhon"] = rel(g.get_window(host.W_RHON), windows(o.get("rhon"), xc, yc, zs, 1, M))
     print(f"  spectral: " + ", ".join(f"{k} {v:.2e}" for k, v in r.items()), flush=True)
-    assert max(r.values()) < 2e-5, (tag, r)
+    assert max(r.values()) < t(2e-5), (tag, r)
 
     g.update_lumen_and_ECM(); o.call("update_lumen_and_ECM")
     g.reshaping_lumen_or_ECM(0); o.call("reshaping_lumen_or_ECM", 0)
     g.reshaping_lumen_or_ECM(1); o.call("reshaping_lumen_or_ECM", 1)
-    assert rel(g.get_field(host.F_ECMN), o.get("ecmn")) < 2e-6, tag
-    assert rel(g.get_field(host.F_LUMN), o.get("lumn")) < 2e-6, tag
+    assert rel(g.get_field(host.F_ECMN), o.get("ecmn")) < t(2e-6), tag
+    assert rel(g.get_field(host.F_LUMN), o.get("lumn")) < t(2e-6), tag
 
     o.call("compute_normal_to_phi")
     g.update_all_cells(); o.call("update_all_cells")
-    assert rel(g.get_window(host.W_PHIN), windows(o.get("phin"), xc, yc, zs, 0, W)) < 2e-6, tag
+    assert rel(g.get_window(host.W_PHIN), windows(o.get("phin"), xc, yc, zs, 0, W)) < t(2e-6), tag
     g.reshaping_cells(); o.call("reshaping_cells")
-    ndiff = iterations_agree(g.iterations(), o.iterations(), tag)
+    if scale < 1.0:
+        gi, oi = g.iterations(), o.iterations()
+        assert gi[1:] == oi[1:] and (gi[0] == oi[0]).all(), (tag, gi, oi)
+        ndiff = 0
+    else:
+        ndiff = iterations_agree(g.iterations(), o.iterations(), tag)
     # one extra sweep on one cell moves its phi by ~1e-5 of the whole field
-    assert rel(g.get_field(host.F_PHI), o.get("phi")) < (2e-6 if ndiff == 0 else 5e-5), tag
+    assert rel(g.get_field(host.F_PHI), o.get("phi")) < (t(2e-6) if ndiff == 0 else 5e-5), tag
     g.copy_tables(); o.call("commit_step")
-    assert rel(g.get_field(host.F_RHO), o.get("rho")) < 2e-5, tag
-    assert rel(g.get_field(host.F_ECM), o.get("ecm")) < 2e-6, tag
-    assert rel(g.get_field(host.F_LUM), o.get("lum")) < 2e-6, tag
+    assert rel(g.get_field(host.F_RHO), o.get("rho")) < t(2e-5), tag
+    assert rel(g.get_field(host.F_ECM), o.get("ecm")) < t(2e-6), tag
+    assert rel(g.get_field(host.F_LUM), o.get("lum")) < t(2e-6), tag
 
 
 def f32_separation(wl, steps: int, fields=("phi", "rho", "ecm", "lum", "vx", "vy")):

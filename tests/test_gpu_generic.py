@@ -1,0 +1,126 @@
+"""The full-plane path of the library (organoid_generic.hip): FP64 contexts
+(org_create_f64, the reference retyped to double) and FP32 contexts whose
+FFT box is not 128 (any zone_size; the reference template's zone_size 200
+gives M = 398, a Bluestein length).
+
+* The 2D transform against the oracle's (same algorithm and twiddles:
+  equal to rounding, ~1e-16) and against numpy's FFT.
+* FP64 contexts against the FP64 oracle: whole steps (centres and every
+  reshaping count equal, fields within 1e-12 relative L2; observed
+  ~3e-15), and one step
+  through the per-function entry points with each output compared after its
+  call at 1e-5 of the FP32 tolerances.
+* FP32 contexts with the 398 box against the oracle: steps within 1e-6 of
+  the oracle's float build (the reference's own arithmetic; the kernels
+  keep its literal types and rounding order) and within the float build's
+  own separation from FP64, and the per-function step at the FP32
+  tolerances.
+"""
+import numpy as np
+import pytest
+
+from multicellular_phase_field_active_gel_organoids_amd import host
+from multicellular_phase_field_active_gel_organoids_amd.configs import make_workload
+from oracle import oracle as orc
+from oracle.oracle import Oracle
+
+from _parity import compare_driver_order, f32_separation, handoff_oracle, rel, round32
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (("phi", host.F_PHI), ("rho", host.F_RHO), ("ecm", host.F_ECM), ("lum", host.F_LUM))
+
+
+@pytest.mark.parametrize("M", [64, 100, 128, 398])
+@pytest.mark.parametrize("f64", [True, False], ids=["f64", "f32"])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_generic_fft2d(M, f64, sign):
+    rng = np.random.default_rng(1000 + M)
+    a = rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))
+    if not f64:
+        a = a.astype(np.complex64).astype(np.complex128)
+    got = host.debug_fft2d_generic(a, sign, f64)
+    want = orc.fft2d(a, sign)
+    if not f64:
+        want = want.astype(np.complex64).astype(np.complex128)
+    ref = np.fft.fft2(a) if sign < 0 else np.fft.ifft2(a) * (M * M)
+    r_o, r_np = rel(got, want), rel(got, ref)
+    print(f"M {M} {'f64' if f64 else 'f32'} sign {sign}: vs oracle {r_o:.2e} (equal: {np.array_equal(got, want)}), "
+          f"vs numpy {r_np:.2e}", flush=True)
+    assert r_o <= (1e-15 if f64 else 1e-7)
+    assert r_np <= (1e-13 if f64 else 1e-6)
+
+
+def _trajectory(name, precision, steps):
+    wl = make_workload(name)
+    if precision == "f32":
+        round32(wl)
+    g = host.Organoid(wl.params, wl.s0_cell, precision=precision)
+    assert g.generic
+    g.load_workload(wl)
+    return wl, g
+
+
+@pytest.mark.parametrize("name", ["two_cells_256", "two_cells_512_zone200", "cyst6_256"])
+def test_f64_steps_match_oracle(name):
+    steps = 3
+    wl, g = _trajectory(name, "f64", steps)
+    o = Oracle(wl.params, wl.s0_cell)
+    o.load_workload(wl)
+    tol = 1e-12
+    for it in range(steps):
+        print(f"case {name} f64: step {it}", flush=True)
+        g.step(1)
+        o.step(it)
+        xc, yc = g.centres()
+        oxc, oyc = o.centres()
+        assert (xc == oxc).all() and (yc == oyc).all()
+        gi, oi = g.iterations(), o.iterations()
+        assert gi[1:] == oi[1:] and (gi[0] == oi[0]).all(), (gi, oi)
+        r = {k: rel(g.get_field(f), o.get(k)) for k, f in FIELDS}
+        print("  " + ", ".join(f"{k} {v:.2e}" for k, v in r.items()), f"counts {gi[0]} {gi[1:]}", flush=True)
+        assert max(r.values()) <= tol, r
+    s, f0 = g.areas()
+    os_, of0 = o.areas()
+    assert np.allclose(s, os_, rtol=1e-12)
+    o.close()
+    compare_driver_order(g, handoff_oracle(g, wl), wl, f"{name} f64 handoff", scale=1e-5)
+    g.close()
+
+
+def test_f32_zone200_steps_within_float_build_separation():
+    steps = 3
+    wl, g = _trajectory("two_cells_512_zone200", "f32", steps)
+    a, sep = f32_separation(wl, steps, fields=("phi", "rho", "ecm", "lum"))
+    b = Oracle(wl.params, wl.s0_cell, precision="f32")
+    b.load_workload(wl)
+    for it in range(steps):
+        b.step(it)
+    print(f"case two_cells_512_zone200 f32: {steps} steps", flush=True)
+    g.step(steps)
+    gi, oi = g.iterations(), b.iterations()
+    print(f"  counts GPU {gi} float build {oi}", flush=True)
+    assert gi[1:] == oi[1:] and (gi[0] == oi[0]).all()
+    for k, f in FIELDS:
+        got = g.get_field(f)
+        r, r32 = rel(got, a.get(k)), rel(got, b.get(k))
+        print(f"  {k}: GPU vs FP64 {r:.2e} (float build vs FP64 {sep[k]:.2e}), GPU vs float build {r32:.2e}",
+              flush=True)
+        assert r <= 5 * sep[k] + 1e-6, (k, r, sep[k])
+        assert r32 <= 1e-6, (k, r32)
+    a.close()
+    b.close()
+    compare_driver_order(g, handoff_oracle(g, wl), wl, "two_cells_512_zone200 f32 handoff")
+    g.close()
+
+
+def test_f64_fields_roundtrip_and_precision():
+    wl = make_workload("two_cells_256")
+    g = host.Organoid(wl.params, wl.s0_cell, precision="f64")
+    assert g.dtype == np.float64
+    g.set_field(host.F_PHI, wl.phi)
+    back = g.get_field(host.F_PHI)
+    assert back.dtype == np.float64 and np.array_equal(back, wl.phi)
+    with pytest.raises(host.OrganoidError):
+        host.Batch([g])
+    g.close()

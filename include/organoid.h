@@ -203,6 +203,9 @@ int org_reshaping_cells(org_ctx *ctx);             /* cell_phase_functions.cpp:4
  * program saves them. */
 int org_reshaping_cells_from_phi(org_ctx *ctx);
 int org_update_rho_to_shape(org_ctx *ctx, float rho0, float tol);
+/* The same with double constants (the FP64 form's double rho0 and tol;
+ * an FP32 context rounds them to float). */
+int org_update_rho_to_shape_f64(org_ctx *ctx, double rho0, double tol);
 int org_relax_initial_state(org_ctx *ctx, float rho0, float tol);
 int org_commit_step(org_ctx *ctx);                 /* useful_functions.cpp:679-709 as
                                                       called at run_with_active_gels.cpp:175-182 */

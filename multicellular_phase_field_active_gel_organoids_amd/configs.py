@@ -193,15 +193,53 @@ def _cyst_workload(name, N, ncell, R, seed, steps, eps=2.0, params=None):
 
 def _template_cyst(name, seed, steps, N=None):
     """The reference template's own configuration (constants_template.cpp
-    with setup_simulation.py's defaults, params.template_defaults): 6 cells
-    of R = 1 (100 pixels at dx = 0.01) on the ring of ini_ECM_lumen around a
-    lumen, zone_size 200 (FFT box 398), interface width wi = 0.07 (the tanh
-    profile's eps = sqrt(2) wi / dx pixels), on the template's 1200^2 grid
-    unless N is given."""
+    with setup_simulation.py's defaults, params.template_defaults: 6 cells,
+    dx = 0.01, R0 = R = 1, wi = 0.07, zone_size 200 so the FFT box is 398) in
+    the analytic initial state of its IC program
+    (prepare_initial_files.cpp:56-66), on the template's 1200^2 grid unless N
+    is given:
+
+    * phi_n = 1 / (1 + exp((dx |x - c_n| - R0) / wi)) with the centres on the
+      ring of radius rcyst = N_ini 0.9 R / (pi dx) at angles 2 pi n / N_ini
+      (initialisation_functions.cpp:117-159);
+    * rho_n = rho0 exp(-1 / (4 h (1 - h) + tol) + 1 / (1 + tol)),
+      h = phi_n^2 (3 - 2 phi_n), zero below rho0 exp(-1 / (2 tol) + 1 / (1 + tol)),
+      rho0 = 0.001, tol = 0.01 (initialisation_functions.cpp:64-73;
+      constants_template.cpp:110-111);
+    * lum / ECM = 1 - sum phi where positive, inside / outside rcyst
+      (initialisation_functions.cpp:82-114).
+
+    Evaluated in float64 (the IC program's float rounding is not
+    reproduced: the state is an input handed to both sides).  No random
+    numbers: `seed` is unused."""
+    del seed
     p = template_defaults()
     if N:
         p.N = int(N)
-    return _cyst_workload(name, p.N, p.ncell, p.R / p.dx, seed, steps, eps=math.sqrt(2.0) * p.wi / p.dx, params=p)
+    N, nc = p.N, p.ncell
+    rc = nc * 0.9 * p.R / (math.pi * p.dx)
+    ang = 2.0 * math.pi * np.arange(nc) / nc
+    centres = np.stack([N // 2 + rc * np.cos(ang), N // 2 + rc * np.sin(ang)], axis=1)
+    ii = np.arange(N, dtype=np.float64)[:, None]
+    jj = np.arange(N, dtype=np.float64)[None, :]
+    phi = np.empty((nc, N, N))
+    rho = np.empty((nc, N, N))
+    rho0, tol = 0.001, 0.01
+    floor = rho0 * math.exp(-1.0 / (2.0 * tol) + 1.0 / (1.0 + tol))
+    for n in range(nc):
+        r = np.sqrt((ii - centres[n, 0]) ** 2 + (jj - centres[n, 1]) ** 2)
+        with np.errstate(over="ignore"):
+            phi[n] = 1.0 / (1.0 + np.exp((p.dx * r - 1.0) / p.wi))
+        h = phi[n] * phi[n] * (3.0 - 2.0 * phi[n])
+        with np.errstate(divide="ignore"):
+            v = rho0 * np.exp(-1.0 / (4.0 * h * (1.0 - h) + tol) + 1.0 / (1.0 + tol))
+        rho[n] = np.where(v < floor, 0.0, v)
+    free = 1.0 - phi.sum(axis=0)
+    d = np.sqrt((ii - N // 2) ** 2 + (jj - N // 2) ** 2)
+    lum = np.where((free > 0.0) & (d < rc), free, 0.0)
+    ecm = np.where((free > 0.0) & (d >= rc), free, 0.0)
+    radii = np.full(nc, p.R / p.dx)
+    return Workload(name, p, phi, rho, lum, ecm, centres, radii, None, steps)
 
 
 def make_workload(name: str, N: int | None = None, ncell: int | None = None, steps: int | None = None) -> Workload:
@@ -218,10 +256,11 @@ def make_workload(name: str, N: int | None = None, ncell: int | None = None, ste
                           and lumen-cell coupling eta_lum_cells = 0.002
     ``two_cells_512_zone200``  two_cells_256's cells on 512^2 with the
                           template's zone_size 200 (FFT box 398)
-    ``template6_1200``    the reference template's configuration: 6 cells of
-                          radius 100 pixels on the ini_ECM_lumen ring,
-                          template constants (dx 0.01, zone_size 200, FFT
-                          box 398), 1200^2
+    ``template6_1200``    the reference template's configuration: template
+                          constants (6 cells, dx 0.01, zone_size 200, FFT box
+                          398) in its IC program's analytic state (cells of
+                          radius 100 pixels on the ring, rho from
+                          ini_rho), 1200^2
     """
     if name == "two_cells_256":
         n_ = N or 256

@@ -12,6 +12,10 @@
  * Host arrays the driver reads back are written back eagerly: the output
  * slabs (saving_out_functions) and the areas s every ndisplay steps
  * (run_with_active_gels.cpp:144-151).
+ *
+ * Compiled twice: org_real_t float (liborganoid_dropin.so, FP32 device
+ * state) and, with ORGANOID_DROPIN_F64, double (liborganoid_dropin_f64.so,
+ * FP64 device state through org_create_f64 and the _f64 field I/O).
  */
 #include "organoid_dropin.h"
 
@@ -21,18 +25,18 @@
 #include <vector>
 
 /* ---- declare_tables.cpp:97-157 ------------------------------------------- */
-float *__restrict phi, *__restrict phin, *__restrict rho, *__restrict rho_to_compare, *__restrict rhon;
-float *__restrict lum, *__restrict ECM, *__restrict lumn, *__restrict ECMn;
-float *__restrict psi, *__restrict temp_phase;
-float *__restrict hfunc_cell, *__restrict hfunc_lum, *__restrict hfunc_ECM;
-float *__restrict norm_grad_phi, *__restrict prod_norm_delta, *__restrict norm_grad_lum, *__restrict norm_grad_ecm;
-float *__restrict a_tan_1, *__restrict a_ort_1, *__restrict b, *__restrict viscosity;
-float *__restrict pi_xx, *__restrict pi_yy, *__restrict pi_xy;
-float *__restrict phiout, *__restrict rhoout, *__restrict lumout, *__restrict ECMout;
-float *__restrict Jxout, *__restrict Jyout, *__restrict vol_out;
-float *__restrict J_x, *__restrict J_y;
+org_real_t *__restrict phi, *__restrict phin, *__restrict rho, *__restrict rho_to_compare, *__restrict rhon;
+org_real_t *__restrict lum, *__restrict ECM, *__restrict lumn, *__restrict ECMn;
+org_real_t *__restrict psi, *__restrict temp_phase;
+org_real_t *__restrict hfunc_cell, *__restrict hfunc_lum, *__restrict hfunc_ECM;
+org_real_t *__restrict norm_grad_phi, *__restrict prod_norm_delta, *__restrict norm_grad_lum, *__restrict norm_grad_ecm;
+org_real_t *__restrict a_tan_1, *__restrict a_ort_1, *__restrict b, *__restrict viscosity;
+org_real_t *__restrict pi_xx, *__restrict pi_yy, *__restrict pi_xy;
+org_real_t *__restrict phiout, *__restrict rhoout, *__restrict lumout, *__restrict ECMout;
+org_real_t *__restrict Jxout, *__restrict Jyout, *__restrict vol_out;
+org_real_t *__restrict J_x, *__restrict J_y;
 int *__restrict xcenters, *__restrict ycenters;
-float *__restrict matgenrates, *__restrict degrates, *__restrict matout, *__restrict degout;
+org_real_t *__restrict matgenrates, *__restrict degrates, *__restrict matout, *__restrict degout;
 
 /* ---- declare_fft_tables.cpp:10-22 ---------------------------------------- */
 cufftComplex *__restrict temp_fft, *__restrict temp_fft_real, *__restrict fft_pi_xx, *__restrict fft_pi_yy;
@@ -53,7 +57,17 @@ void chk(int rc, const char *what) {
     if (rc) die(what, rc);
 }
 
-float *fmalloc(size_t n) { return (float *)malloc(n * sizeof(float)); }
+org_real_t *fmalloc(size_t n) { return (org_real_t *)malloc(n * sizeof(org_real_t)); }
+
+#ifdef ORGANOID_DROPIN_F64
+int create_ctx(const org_params *p, org_ctx **c) { return org_create_f64(p, nullptr, c); }
+int put(org_ctx *c, int f, const double *h) { return org_set_field_f64(c, f, h); }
+int get(org_ctx *c, int f, double *h) { return org_get_field_f64(c, f, h); }
+#else
+int create_ctx(const org_params *p, org_ctx **c) { return org_create(p, nullptr, c); }
+int put(org_ctx *c, int f, const float *h) { return org_set_field(c, f, h); }
+int get(org_ctx *c, int f, float *h) { return org_get_field(c, f, h); }
+#endif
 
 size_t planes() { return (size_t)Ntot * N_s * N_s; }
 size_t plane() { return (size_t)N_s * N_s; }
@@ -85,12 +99,12 @@ void release() {
 org_ctx *ctx() {
     if (g_ctx) return g_ctx;
     org_params p = params_from_globals();
-    chk(org_create(&p, nullptr, &g_ctx), "org_create");
-    chk(org_set_field(g_ctx, ORG_F_PHI, phi), "copyin phi");
-    chk(org_set_field(g_ctx, ORG_F_RHO, rho), "copyin rho");
-    chk(org_set_field(g_ctx, ORG_F_LUM, lum), "copyin lum");
-    chk(org_set_field(g_ctx, ORG_F_ECM, ECM), "copyin ECM");
-    chk(org_set_field(g_ctx, ORG_F_HFUNC, hfunc_cell), "copyin hfunc_cell");
+    chk(create_ctx(&p, &g_ctx), "org_create");
+    chk(put(g_ctx, ORG_F_PHI, phi), "copyin phi");
+    chk(put(g_ctx, ORG_F_RHO, rho), "copyin rho");
+    chk(put(g_ctx, ORG_F_LUM, lum), "copyin lum");
+    chk(put(g_ctx, ORG_F_ECM, ECM), "copyin ECM");
+    chk(put(g_ctx, ORG_F_HFUNC, hfunc_cell), "copyin hfunc_cell");
     chk(org_prepare(g_ctx), "org_prepare");
     std::atexit(release);
     return g_ctx;
@@ -101,7 +115,12 @@ org_ctx *ctx() {
 org_ctx *organoid_dropin_context(void) { return ctx(); }
 
 int cufftPlan2d(cufftHandle *plan, int nx, int ny, int type) {
-    if (!plan || nx != 2 * zone_size - 2 || ny != nx || type != CUFFT_C2C) {
+#ifdef ORGANOID_DROPIN_F64
+    const bool type_ok = type == CUFFT_Z2Z || type == CUFFT_C2C;
+#else
+    const bool type_ok = type == CUFFT_C2C;
+#endif
+    if (!plan || nx != 2 * zone_size - 2 || ny != nx || !type_ok) {
         fprintf(stderr, "organoid drop-in: plan %dx%d type %d unsupported (box is %d)\n", nx, ny, type,
                 2 * zone_size - 2);
         std::exit(1);
@@ -136,82 +155,86 @@ void ini_fft_all(void) {
 
 /* useful_functions.cpp:66-99: snapshot into slab it/nsaving of the host
  * output arrays, which the driver reads after the loop. */
-void saving_out_functions(int it_, float *phi_, float *rho_, float *ECM_, float *lum_, float *phiout_,
-                          float *rhoout_, float *lumout_, float *ECMout_) {
+void saving_out_functions(int it_, org_real_t *phi_, org_real_t *rho_, org_real_t *ECM_, org_real_t *lum_, org_real_t *phiout_,
+                          org_real_t *rhoout_, org_real_t *lumout_, org_real_t *ECMout_) {
     org_ctx *c = ctx();
     size_t slab = (size_t)(it_ / nsaving);
-    chk(org_get_field(c, ORG_F_PHI, phiout_ + slab * planes()), "saving phi");
-    chk(org_get_field(c, ORG_F_RHO, rhoout_ + slab * planes()), "saving rho");
-    chk(org_get_field(c, ORG_F_LUM, lumout_ + slab * plane()), "saving lum");
-    chk(org_get_field(c, ORG_F_ECM, ECMout_ + slab * plane()), "saving ECM");
+    chk(get(c, ORG_F_PHI, phiout_ + slab * planes()), "saving phi");
+    chk(get(c, ORG_F_RHO, rhoout_ + slab * planes()), "saving rho");
+    chk(get(c, ORG_F_LUM, lumout_ + slab * plane()), "saving lum");
+    chk(get(c, ORG_F_ECM, ECMout_ + slab * plane()), "saving ECM");
     (void)phi_; (void)rho_; (void)ECM_; (void)lum_;
 }
 
-void find_centres_cells(float *, int *, int *) { chk(org_find_centres_cells(ctx()), "find_centres_cells"); }
+void find_centres_cells(org_real_t *, int *, int *) { chk(org_find_centres_cells(ctx()), "find_centres_cells"); }
 
 /* Labels are evaluated where they are consumed (update_all_cortices). */
-void create_turnover_rates_tensions(float *, float *, float *, float *, float *, float *, float *, float *, float *,
+void create_turnover_rates_tensions(org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *,
                                     int *, int *) {}
 
-void compute_hfunc_3d_tables_kernels_loop_n(float *, float *, int *, int *) {
+void compute_hfunc_3d_tables_kernels_loop_n(org_real_t *, org_real_t *, int *, int *) {
     chk(org_compute_hfunc_cells(ctx()), "compute_hfunc_3d_tables_kernels_loop_n");
 }
-void compute_all_hfunc_2d_tables_kernels(float *, float *, float *, float *) {
+void compute_all_hfunc_2d_tables_kernels(org_real_t *, org_real_t *, org_real_t *, org_real_t *) {
     chk(org_compute_hfunc_lum_ecm(ctx()), "compute_all_hfunc_2d_tables_kernels");
 }
 /* |grad phi|, prod_norm_delta, the normals and their ghost rings are only
  * read on the windows; the consuming kernels recompute them there. */
-void compute_normal_grad_phi(float *, float *) {}
-void compute_prod_norm_delta(float *, float *, float *, int *, int *) {}
-void compute_normal_to_phi(float *, float *, float *, float *) {}
-void boundary_conditions_cells(float *) {}
-void boundary_conditions_lumen_and_ECM(float *, float *) {}
+void compute_normal_grad_phi(org_real_t *, org_real_t *) {}
+void compute_prod_norm_delta(org_real_t *, org_real_t *, org_real_t *, int *, int *) {}
+void compute_normal_to_phi(org_real_t *, org_real_t *, org_real_t *, org_real_t *) {}
+void boundary_conditions_cells(org_real_t *) {}
+void boundary_conditions_lumen_and_ECM(org_real_t *, org_real_t *) {}
 
-void compute_psi_kernels(float *, float *) { chk(org_compute_psi(ctx()), "compute_psi_kernels"); }
+void compute_psi_kernels(org_real_t *, org_real_t *) { chk(org_compute_psi(ctx()), "compute_psi_kernels"); }
 
-void sum_s_kernels(float *, float *s, float *) {
+void sum_s_kernels(org_real_t *, org_real_t *s, org_real_t *) {
     org_ctx *c = ctx();
     chk(org_sum_s(c), "sum_s_kernels");
     if (ndisplay > 0 && it % ndisplay == 0) {      /* acc update host(s), run_with_active_gels.cpp:146 */
         std::vector<double> sv(Ntot), fv(Ntot);
         chk(org_get_areas(c, sv.data(), fv.data()), "sum_s_kernels readback");
-        for (int q = 0; q < Ntot; q++) s[q] = (float)sv[q];
+        for (int q = 0; q < Ntot; q++) s[q] = (org_real_t)sv[q];
     }
 }
 
-void update_all_cortices(float *, float *, float *, int *, int *, float *, float *, float *, float *, float *, float *,
-                         float *, float *, cufftHandle, cufftHandle, cufftComplex *, cufftComplex *, cufftComplex *,
+void update_all_cortices(org_real_t *, org_real_t *, org_real_t *, int *, int *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *,
+                         org_real_t *, org_real_t *, cufftHandle, cufftHandle, cufftComplex *, cufftComplex *, cufftComplex *,
                          cufftComplex *, cufftComplex *, cufftComplex *, cufftComplex *, cufftComplex *,
                          cufftComplex *, cufftComplex *, cufftComplex *) {
     chk(org_update_all_cortices(ctx()), "update_all_cortices");
 }
 
-void update_lumen_and_ECM(float *, float *, float *, float *, float *, float *, float *) {
+void update_lumen_and_ECM(org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *) {
     chk(org_update_lumen_and_ECM(ctx()), "update_lumen_and_ECM");
 }
 
-void reshaping_lumen_or_ECM(int, float *field, float *, float) {
+void reshaping_lumen_or_ECM(int, org_real_t *field, org_real_t *, org_real_t) {
     chk(org_reshaping_lumen_or_ECM(ctx(), field == ECM ? 1 : 0), "reshaping_lumen_or_ECM");
 }
 
-void update_all_cells(int, float *, float *, float *, float *, float *, float *, float *, int *, int *, float *,
-                      float *) {
+void update_all_cells(int, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, int *, int *, org_real_t *,
+                      org_real_t *) {
     g_phin_is_phi = false;
     chk(org_update_all_cells(ctx()), "update_all_cells");
 }
 
-void reshaping_cells(int, float *, float *, float *, float *, float *, int *, int *) {
+void reshaping_cells(int, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, int *, int *) {
     if (g_phin_is_phi) chk(org_reshaping_cells_from_phi(ctx()), "reshaping_cells");
     else chk(org_reshaping_cells(ctx()), "reshaping_cells");
 }
 
 /* cell_phase_functions.cpp:42-62 */
-void update_rho_to_shape(float *, float *, float rho0_, float tol_) {
+void update_rho_to_shape(org_real_t *, org_real_t *, org_real_t rho0_, org_real_t tol_) {
+#ifdef ORGANOID_DROPIN_F64
+    chk(org_update_rho_to_shape_f64(ctx(), rho0_, tol_), "update_rho_to_shape");
+#else
     chk(org_update_rho_to_shape(ctx(), rho0_, tol_), "update_rho_to_shape");
+#endif
 }
 
 /* useful_functions.cpp:11-25: NaNs in the first two cell planes (host) */
-int check_nan(float *table) {
+int check_nan(org_real_t *table) {
     int c = 0;
     const size_t pl = plane();
     for (size_t k = 0; k < pl; k++) c += std::isnan(table[k]) + std::isnan(table[pl + k]);
@@ -225,7 +248,7 @@ int check_nan(float *table) {
  * prepare_initial_files.cpp:87-100).  In the loop the driver copies
  * rho<-rhon cell by cell (:175-178); the device commits all cells with the
  * last one. */
-void copy_cell_tables_1_by_1(int n_, float *phase, float *phasen) {
+void copy_cell_tables_1_by_1(int n_, org_real_t *phase, org_real_t *phasen) {
     if (phase == phin && phasen == phi && n_ == Ntot - 1) g_phin_is_phi = true;
     if (!g_ctx) return;
     if (phase == rho && n_ == Ntot - 1) chk(org_commit_rho(g_ctx), "copy rho<-rhon");
@@ -234,7 +257,7 @@ void copy_cell_tables_1_by_1(int n_, float *phase, float *phasen) {
 /* useful_functions.cpp:695-709: ECM<-ECMn then lum<-lumn at the end of a
  * step (:181-182) commit the lumen/ECM iterate; the initial copies in the
  * other direction (:92-93) need nothing. */
-void copy_tables_ECM_or_lumen(float *phase, float *) {
+void copy_tables_ECM_or_lumen(org_real_t *phase, org_real_t *) {
     if (!g_ctx) return;
     if (phase == ECM) chk(org_commit_lumecm(g_ctx), "copy ECM/lum <- ECMn/lumn");
 }

@@ -930,6 +930,11 @@ int org_update_rho_to_shape(org_ctx *c, float rho0, float tol) {
  * zero.  No copy-back follows, so lum and ECM end at their second-to-last
  * iterates and phi at its second-to-last reshaping iterate, as in the
  * program's saved frame (:108). */
+int org_update_rho_to_shape_f64(org_ctx *c, double rho0, double tol) {
+    TRY(need_prepared(c));
+    if (c->gen) return gen_rc(c->gen->rho_to_shape(rho0, tol));
+    return org_update_rho_to_shape(c, (float)rho0, (float)tol);
+}
 int org_relax_initial_state(org_ctx *c, float rho0, float tol) {
     TRY(need_prepared(c));
     if (c->gen) {

@@ -738,6 +738,218 @@ __global__ __launch_bounds__(1024) void g_relax_cells(GP<R> P, R *__restrict__ p
     if (threadIdx.x == 0) counts[n] = count;
 }
 
+/* ---- loops split over several workgroups ------------------------------------
+ * The reshaping loops on big windows and on the whole grid run on the
+ * device in one launch each: a group of workgroups shares the work of every
+ * sweep and meets at a barrier after it.  The barrier is a monotonic arrival
+ * counter per group (the e-th barrier of the launch releases at e * members),
+ * every wave drains its stores, thread 0 releases at agent scope (the
+ * workgroups sit on different XCDs, each with its own L2), arrives, polls
+ * with a sleep and acquires.  All workgroups of the launch must be resident
+ * together: the host launches at most one per compute unit.  A wait longer
+ * than `limit` wall-clock ticks, or one that finds the abort flag raised,
+ * raises the flag and every workgroup leaves its loop (ORG_ETIMEOUT). */
+static __device__ bool g_barrier(unsigned *ctr, unsigned target, int *abort_flag, long long limit) {
+    __shared__ int ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int good = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned seen = atomicAdd(ctr, 1u) + 1u;
+        const long long t0 = wall_clock64();
+        while (seen < target) {
+            __builtin_amdgcn_s_sleep(1);
+            seen = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (seen >= target) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > limit) {
+                atomicExch(abort_flag, 1);
+                good = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+/* The interface flux of g_reshape with the relaxed field u on its own row
+ * stride su (a window buffer) and the normals on the grid (stride sj). */
+template <typename R>
+static __device__ __forceinline__ R g_reshape2(R un, const R *__restrict__ u, size_t qu, int su,
+                                               const R *__restrict__ jx, const R *__restrict__ jy, size_t qj, int sj,
+                                               R wi, R rdt, R dx) {
+#define U(a, b) u[qu + (long)(a) * su + (b)]
+#define JX(a, b) jx[qj + (long)(a) * sj + (b)]
+#define JY(a, b) jy[qj + (long)(a) * sj + (b)]
+    un += 0.5 * (JX(1, 0) + JX(0, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(1, 0) - U(0, 0)) * 0.5 * (JX(1, 0) + JX(0, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(0, 1) - U(1, -1) - U(0, -1)) * 0.5 * (JY(1, 0) + JY(0, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(1, 0) * (1.0 - U(1, 0)) + U(0, 0) * (1.0 - U(0, 0))));
+    un += 0.5 * (JY(0, 1) + JY(0, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 1) - U(0, 0)) * 0.5 * (JY(0, 1) + JY(0, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(1, 0) - U(-1, 1) - U(-1, 0)) * 0.5 * (JX(0, 1) + JX(0, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 1) * (1.0 - U(0, 1)) + U(0, 0) * (1.0 - U(0, 0))));
+    un -= 0.5 * (JX(0, 0) + JX(-1, 0)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(-1, 0)) * 0.5 * (JX(0, 0) + JX(-1, 0))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(0, 1) + U(-1, 1) - U(0, -1) - U(-1, -1)) * 0.5 * (JY(0, 0) + JY(-1, 0))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(-1, 0) * (1.0 - U(-1, 0))));
+    un -= 0.5 * (JY(0, 0) + JY(0, -1)) * (
+        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(0, -1)) * 0.5 * (JY(0, 0) + JY(0, -1))
+        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 0) + U(1, -1) - U(-1, 0) - U(-1, -1)) * 0.5 * (JX(0, 0) + JX(0, -1))
+        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(0, -1) * (1.0 - U(0, -1))));
+    return un;
+#undef U
+#undef JX
+#undef JY
+}
+
+/* reshaping_cells (cell_phase_functions.cpp:447-580) with K workgroups per
+ * cell (blockIdx.x = n K + b), workgroup b on window rows [b W / K,
+ * (b + 1) W / K).  The iterates live in two (W + 2)^2 window buffers per
+ * cell whose one-pixel ring holds phi around the window (what the
+ * reference's stencil reads there); the first holds phin's window.  Sweep
+ * k reads one buffer and writes the other, so the reference's copy
+ * phase <- phasen before each sweep (:480-492) is the change of roles, and
+ * one barrier per sweep suffices.  The deviation is each workgroup's FP64
+ * sum over its band, the K parts added in order by every workgroup (so all
+ * take the same stop decision).  At the end phin's window gets the last
+ * iterate and phi's the second-to-last.  S: [nc][2][W + 2][W + 2];
+ * part: [nc][2 parities][K]; ctr: [nc] arrival counters, zero at launch. */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict__ phi, R *__restrict__ phin,
+                                                         const R *__restrict__ jx, const R *__restrict__ jy,
+                                                         const int *__restrict__ xc, const int *__restrict__ yc,
+                                                         double thr, double rdev0, R *__restrict__ S,
+                                                         double *__restrict__ part, unsigned *__restrict__ ctr,
+                                                         int *__restrict__ counts, int *__restrict__ relax_flag,
+                                                         int *__restrict__ abort_flag, long long limit, int K) {
+    __shared__ double red[16];
+    const int N = P.N, zs = P.zs, W = 2 * zs, W2 = W + 2;
+    const int n = blockIdx.x / K, b = blockIdx.x % K;
+    const size_t NN = (size_t)N * N, SS = (size_t)W2 * W2;
+    R *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
+    const R *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
+    R *S0 = S + (size_t)n * 2 * SS, *S1 = S0 + SS;
+    const int x0 = xc[n] - zs, y0 = yc[n] - zs;              /* window origin on the grid */
+    const int r0 = b * W / K, r1 = (b + 1) * W / K;          /* this workgroup's window rows */
+    const int rows = r1 - r0;
+    /* buffer rows r0+1 .. r1 (window rows r0 .. r1-1) and, for the first and
+     * last workgroup, the ring rows 0 and W+1: ring columns from phi, the
+     * window from phin */
+    const int f0 = (b == 0) ? 0 : r0 + 1, f1 = (b == K - 1) ? W2 : r1 + 1;
+    for (int k = threadIdx.x; k < (f1 - f0) * W2; k += blockDim.x) {
+        const int sr = f0 + k / W2, sc = k % W2;
+        const size_t g = (size_t)(x0 - 1 + sr) * N + (y0 - 1 + sc);
+        const bool ring = sr == 0 || sr == W + 1 || sc == 0 || sc == W + 1;
+        const R v = ring ? ph[g] : pn[g];
+        S0[(size_t)sr * W2 + sc] = v;
+        if (ring) S1[(size_t)sr * W2 + sc] = v;
+    }
+    unsigned epoch = 1;
+    if (!g_barrier(&ctr[n], epoch * K, abort_flag, limit)) return;
+    const R dx = P.dx;
+    double rdev = rdev0;
+    int ri = 0, count = 0, cur = 0;
+    while (rdev > thr) {
+        const R *src = cur ? S1 : S0;
+        R *dst = cur ? S0 : S1;
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
+            const int wr = r0 + k / W, wc = k % W;
+            const size_t qs = (size_t)(wr + 1) * W2 + (wc + 1);
+            const size_t qj = (size_t)(x0 + wr) * N + (y0 + wc);
+            const R v = g_reshape2(src[qs], src, qs, W2, jxn, jyn, qj, N, P.wi, P.rdt, dx);
+            dst[qs] = v;
+            acc += fabs((double)(v - src[qs])) * dx * dx / P.ncur;
+        }
+        const double t = gblock_sum(acc, red);
+        const int par = count & 1;
+        if (threadIdx.x == 0) part[((size_t)n * 2 + par) * K + b] = t;
+        epoch++;
+        if (!g_barrier(&ctr[n], epoch * K, abort_flag, limit)) return;
+        rdev = 0.0;
+        for (int q = 0; q < K; q++) rdev += part[((size_t)n * 2 + par) * K + q];
+        count++;
+        cur ^= 1;
+        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :565-568 */
+        ri++;
+    }
+    /* phin <- the last iterate, phi <- the one before (the reference's copy
+     * at the start of the last sweep) */
+    const R *last = cur ? S1 : S0, *prev = cur ? S0 : S1;
+    for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
+        const int wr = r0 + k / W, wc = k % W;
+        const size_t qs = (size_t)(wr + 1) * W2 + (wc + 1);
+        const size_t g = (size_t)(x0 + wr) * N + (y0 + wc);
+        pn[g] = last[qs];
+        if (count > 0) ph[g] = prev[qs];
+    }
+    if (threadIdx.x == 0 && b == 0) counts[n] = count;
+}
+
+/* reshaping_lumen_or_ECM's loop (cell_phase_functions.cpp:774-866) with K
+ * workgroups, workgroup b on grid rows [b N / K, (b + 1) N / K): sweep k
+ * reads bufB (k odd) or bufA (k even) and writes the other, the reference's
+ * copy lum <- lumn being the change of roles; the Dirichlet ring as
+ * g_lum_sweep.  out[0]: sweeps done (the last iterate is in bufA when odd,
+ * in bufB when even).  part: [2][K]; ctr: one arrival counter, zero at
+ * launch. */
+template <typename R>
+__global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ bufA, R *__restrict__ bufB,
+                                                      const R *__restrict__ jx, const R *__restrict__ jy, R side,
+                                                      double thr, double rdev0, double *__restrict__ part,
+                                                      unsigned *__restrict__ ctr, int *__restrict__ out,
+                                                      int *__restrict__ relax_flag, int *__restrict__ abort_flag,
+                                                      long long limit, int K) {
+    __shared__ double red[16];
+    const int N = P.N, b = blockIdx.x;
+    const int r0 = b * N / K, r1 = (b + 1) * N / K;
+    const R dx = P.dx;
+    double rdev = rdev0;
+    int ri = 0, count = 0;
+    unsigned epoch = 0;
+    while (rdev > thr) {
+        const R *u = (count & 1) ? bufA : bufB;      /* sweep count+1: odd reads B */
+        R *un = (count & 1) ? bufB : bufA;
+        double acc = 0.0;
+        for (size_t k = threadIdx.x; k < (size_t)(r1 - r0) * N; k += blockDim.x) {
+            const int i = r0 + (int)(k / N), j = (int)(k % N);
+            const size_t q = (size_t)i * N + j;
+            R v;
+            if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
+                const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
+                v = corner ? (R)0.0 : side;
+            } else {
+                v = g_reshape(u[q], u, jx, jy, N, q, P.wi, P.rdt, dx);
+            }
+            un[q] = v;
+            acc += fabs((double)(v - u[q])) * dx * dx;
+        }
+        const double t = gblock_sum(acc, red);
+        const int par = count & 1;
+        if (threadIdx.x == 0) part[par * K + b] = t;
+        epoch++;
+        if (!g_barrier(ctr, epoch * K, abort_flag, limit)) return;
+        rdev = 0.0;
+        for (int q = 0; q < K; q++) rdev += part[par * K + q];
+        count++;
+        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :862-865 */
+        ri++;
+    }
+    if (threadIdx.x == 0 && b == 0) out[0] = count;
+}
+
+/* a launch's arrival counters and abort flag start at zero (a fill kernel,
+ * not a memset, so the sequence can be captured) */
+__global__ void g_zero_u32(unsigned *p, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0u;
+}
+
 template <typename R>
 __global__ void g_copy(const R *__restrict__ src, R *__restrict__ dst, size_t n) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
@@ -897,6 +1109,15 @@ struct GenCtx final : Generic {
     double *s_ = nullptr, *s0 = nullptr, *sum_e = nullptr, *part = nullptr, *dev_sum = nullptr;
     int *xc = nullptr, *yc = nullptr, *err = nullptr, *counts = nullptr, *relax = nullptr;
     int nparts = 1;
+    /* the loops split over several workgroups: workgroups per cell / for
+     * the lumen-ECM loop (1: the cell loop runs one workgroup per cell),
+     * window buffers, per-workgroup deviations, arrival counters + abort flag */
+    int kc = 1, kl = 1;
+    R *swin = nullptr;
+    double *mb_part = nullptr;
+    unsigned *mb_ctr = nullptr;      /* [nc] cell groups, [nc] the lumen/ECM group, [nc + 1] abort */
+    int *mb_out = nullptr;
+    long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
 
@@ -936,9 +1157,27 @@ struct GenCtx final : Generic {
         GTRY(A_(&vxb, nc * MM)); GTRY(A_(&vyb, nc * MM));
         GTRY(A_(&win, (size_t)nc * std::max(W * W, M * M)));
         nparts = (int)std::min<size_t>(256, (NN + 4095) / 4096);
+        {
+            int dev = 0, ncu = 0, khz = 0;
+            GHIP(hipGetDevice(&dev));
+            GHIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            GHIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            if (ncu <= 0 || khz <= 0) return ORG_EHIP;
+            wait_limit = 2000LL * khz;                 /* 2 s */
+            /* one workgroup per compute unit at most (all must be resident),
+             * about 4096 pixels or more each */
+            kc = std::min(std::min(ncu / nc, (W * W + 4095) / 4096), W / 4);
+            if (kc < 2) kc = 1;
+            kl = (int)std::min<size_t>(std::min<size_t>((size_t)ncu, (NN + 4095) / 4096), (size_t)(N / 4));
+            if (kl < 1) kl = 1;
+        }
         GTRY(A_(&f0, nc)); GTRY(A_(&s_, nc)); GTRY(A_(&s0, nc)); GTRY(A_(&sum_e, 1));
         GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
         GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&err, 1)); GTRY(A_(&counts, nc)); GTRY(A_(&relax, 1));
+        if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
+        GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
+        GTRY(A_(&mb_ctr, (size_t)nc + 2));
+        GTRY(A_(&mb_out, 1));
         std::vector<double> s0v(nc);
         for (int n = 0; n < nc; n++) s0v[n] = (p.use_s0_cell && s0_cell) ? s0_cell[n] : p.s0;
         std::vector<int> cen(nc, N / 2);
@@ -1151,21 +1390,28 @@ struct GenCtx final : Generic {
         GLAUNCH((g_lum_normals<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx, jly);
         GTRY(bc(jlx, 1));                                                                     /* :770-771 */
         GTRY(bc(jly, 1));
-        double rdev = (double)(p.rdev_ini * rtol * rdt);                                      /* :720 */
+        const double rdev0 = (double)(p.rdev_ini * rtol * rdt);                               /* :720 */
         const double thr = (double)(p.rdev_threshold * rtol * rdt);                           /* :774 */
-        int ri = 0, count = 0;
         *failed = 0;
-        while (rdev > thr) {
-            std::swap(u, un);                       /* u: the previous iterate; un: rewritten */
-            GLAUNCH((g_lum_sweep<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)u, un, (const R *)jlx,
-                    (const R *)jly, side);
-            GTRY(plane_sum(un, u, 1));                                                        /* :848-855 */
-            GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
-            GHIP(hipStreamSynchronize(s));
-            count++;
-            if (ri > p.rimax) { *failed = 1; break; }                                         /* :862-865 */
-            ri++;
+        /* the loop on the device: sweep 1 reads lumn (the update's output)
+         * and writes lum's buffer, and so on (g_lum_loop_mb) */
+        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+        GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+        GLAUNCH((g_lum_loop_mb<R>), dim3(kl), dim3(1024), 0, P, u, un, (const R *)jlx, (const R *)jly, side, thr,
+                rdev0, mb_part, mb_ctr + nc, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kl);
+        int h[3] = {0, 0, 0};
+        GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
+        GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+        GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        if (h[2]) {
+            fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each other "
+                            "for more than 2 s (not all resident?)\n");
+            return ORG_ETIMEOUT;
         }
+        const int count = h[0];
+        if (count & 1) std::swap(u, un);            /* an odd count leaves the last iterate in lum's buffer */
+        *failed = h[1];
         (which ? count_ecm : count_lum) = count;
         return ORG_OK;
     }
@@ -1188,12 +1434,25 @@ struct GenCtx final : Generic {
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
         GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
-        GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
-                (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
-        int f = 0;
-        GHIP(hipMemcpyAsync(&f, relax, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (kc > 1) {
+            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+            GLAUNCH((g_relax_cells_mb<R>), dim3(nc * kc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
+                    (const int *)xc, (const int *)yc, thr, rdev0, swin, mb_part, mb_ctr, counts, relax,
+                    (int *)(mb_ctr + nc + 1), wait_limit, kc);
+        } else {
+            GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
+                    (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
+        }
+        int f[2] = {0, 0};
+        GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (kc > 1) GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         GHIP(hipStreamSynchronize(s));
-        *failed = f;
+        if (f[1]) {
+            fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
+                            "for more than 2 s (not all resident?)\n");
+            return ORG_ETIMEOUT;
+        }
+        *failed = f[0];
         return ORG_OK;
     }
     int commit_rho() override { return copy(rho, rhon, PP); }

@@ -1,22 +1,26 @@
 /* Test program: runs timesteps through the drop-in library in the order of
  * the reference driver's loop (main_code/run_with_active_gels.cpp:92-184),
- * reading the initial fields from raw float32 files and writing the final
- * ones.  Usage: driver_two_cells <dir> <steps>;  reads <dir>/{phi,rho,lum,ecm}.f32,
- * writes <dir>/out_{phi,rho,lum,ecm}.f32 and <dir>/out_s.f32. */
+ * reading the initial fields from raw files and writing the final ones.
+ * Usage: driver_two_cells <dir> <steps>;  reads <dir>/{phi,rho,lum,ecm}.f32,
+ * writes <dir>/out_{phi,rho,lum,ecm}.f32 and <dir>/out_s.f32.  Built with
+ * ORGANOID_DROPIN_F64 (driver_two_cells_f64, the FP64 form) the files are
+ * .f64. */
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 
 #include "organoid_dropin.h"
 
-static void read_raw(const std::string &path, float *dst, size_t n) {
+static const char *EXT = sizeof(org_real_t) == 8 ? ".f64" : ".f32";
+
+static void read_raw(const std::string &path, org_real_t *dst, size_t n) {
     FILE *f = fopen(path.c_str(), "rb");
-    if (!f || fread(dst, sizeof(float), n, f) != n) { fprintf(stderr, "cannot read %s\n", path.c_str()); exit(2); }
+    if (!f || fread(dst, sizeof(org_real_t), n, f) != n) { fprintf(stderr, "cannot read %s\n", path.c_str()); exit(2); }
     fclose(f);
 }
-static void write_raw(const std::string &path, const float *src, size_t n) {
+static void write_raw(const std::string &path, const org_real_t *src, size_t n) {
     FILE *f = fopen(path.c_str(), "wb");
-    if (!f || fwrite(src, sizeof(float), n, f) != n) { fprintf(stderr, "cannot write %s\n", path.c_str()); exit(2); }
+    if (!f || fwrite(src, sizeof(org_real_t), n, f) != n) { fprintf(stderr, "cannot write %s\n", path.c_str()); exit(2); }
     fclose(f);
 }
 
@@ -27,12 +31,12 @@ int main(int argc, char **argv) {
     size_t P = (size_t)Ntot * N_s * N_s, G = (size_t)N_s * N_s;
     malloc_all();
     ini_fft_all();
-    float *s = (float *)malloc(Ntot * sizeof(float)), *f0 = (float *)malloc(Ntot * sizeof(float));
-    read_raw(dir + "/phi.f32", phi, P);
-    read_raw(dir + "/rho.f32", rho, P);
-    read_raw(dir + "/lum.f32", lum, G);
-    read_raw(dir + "/ecm.f32", ECM, G);
-    for (size_t q = 0; q < P; q++) { phin[q] = 0.0f; rhon[q] = 0.0f; hfunc_cell[q] = 0.0f; }
+    org_real_t *s = (org_real_t *)malloc(Ntot * sizeof(org_real_t)), *f0 = (org_real_t *)malloc(Ntot * sizeof(org_real_t));
+    read_raw(dir + "/phi" + EXT, phi, P);
+    read_raw(dir + "/rho" + EXT, rho, P);
+    read_raw(dir + "/lum" + EXT, lum, G);
+    read_raw(dir + "/ecm" + EXT, ECM, G);
+    for (size_t q = 0; q < P; q++) { phin[q] = 0.0; rhon[q] = 0.0; hfunc_cell[q] = 0.0; }
     cufftPlan2d(&planf, 2 * zone_size - 2, 2 * zone_size - 2, CUFFT_C2C);
     cufftPlan2d(&planr, 2 * zone_size - 2, 2 * zone_size - 2, CUFFT_C2C);
     copy_tables_ECM_or_lumen(ECMn, ECM);
@@ -70,11 +74,11 @@ int main(int argc, char **argv) {
     }
     /* final state: the snapshot path reads it back into slab 0 */
     saving_out_functions(0, phi, rho, ECM, lum, phiout, rhoout, lumout, ECMout);
-    write_raw(dir + "/out_phi.f32", phiout, P);
-    write_raw(dir + "/out_rho.f32", rhoout, P);
-    write_raw(dir + "/out_lum.f32", lumout, G);
-    write_raw(dir + "/out_ecm.f32", ECMout, G);
-    write_raw(dir + "/out_s.f32", s, Ntot);
+    write_raw(dir + "/out_phi" + EXT, phiout, P);
+    write_raw(dir + "/out_rho" + EXT, rhoout, P);
+    write_raw(dir + "/out_lum" + EXT, lumout, G);
+    write_raw(dir + "/out_ecm" + EXT, ECMout, G);
+    write_raw(dir + "/out_s" + EXT, s, Ntot);
     printf("driver_two_cells: %d steps done\n", steps);
     return 0;
 }

@@ -740,24 +740,26 @@ __global__ __launch_bounds__(1024) void g_relax_cells(GP<R> P, R *__restrict__ p
 
 /* ---- loops split over several workgroups ------------------------------------
  * The reshaping loops on big windows and on the whole grid run on the
- * device in one launch each: a group of workgroups shares the work of every
- * sweep and meets at a barrier after it.  The barrier is a monotonic arrival
- * counter per group (the e-th barrier of the launch releases at e * members),
- * every wave drains its stores, thread 0 releases at agent scope (the
- * workgroups sit on different XCDs, each with its own L2), arrives, polls
- * with a sleep and acquires.  All workgroups of the launch must be resident
- * together: the host launches at most one per compute unit.  A wait longer
- * than `limit` wall-clock ticks, or one that finds the abort flag raised,
- * raises the flag and every workgroup leaves its loop (ORG_ETIMEOUT). */
+ * device in one launch each: a group of workgroups shares every sweep, each
+ * on a band of rows, and meets at a barrier after it.  Only the band's
+ * first and last rows (and the grid's boundary ring) are read by other
+ * workgroups: they are stored and loaded with agent-scope atomics, which
+ * reach every XCD's view without a cache flush, everything else with plain
+ * stores and loads that only the same workgroup reads back.  So the barrier
+ * needs no release or acquire: every thread waits for its own stores to
+ * complete, then thread 0 arrives on a monotonic counter (the e-th barrier
+ * of the launch releases at e * members) and polls it with a sleep.  All
+ * workgroups of the launch must be resident together: the host launches at
+ * most one per compute unit.  A wait longer than `limit` wall-clock ticks,
+ * or one that finds the abort flag raised, raises the flag and every
+ * workgroup leaves its loop (ORG_ETIMEOUT). */
 static __device__ bool g_barrier(unsigned *ctr, unsigned target, int *abort_flag, long long limit) {
     __shared__ int ok;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         int good = 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned seen = atomicAdd(ctr, 1u) + 1u;
+        unsigned seen = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         const long long t0 = wall_clock64();
         while (seen < target) {
             __builtin_amdgcn_s_sleep(1);
@@ -770,20 +772,40 @@ static __device__ bool g_barrier(unsigned *ctr, unsigned target, int *abort_flag
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ok = good;
     }
     __syncthreads();
     return ok != 0;
 }
 
-/* The interface flux of g_reshape with the relaxed field u on its own row
- * stride su (a window buffer) and the normals on the grid (stride sj). */
-template <typename R>
-static __device__ __forceinline__ R g_reshape2(R un, const R *__restrict__ u, size_t qu, int su,
-                                               const R *__restrict__ jx, const R *__restrict__ jy, size_t qj, int sj,
-                                               R wi, R rdt, R dx) {
-#define U(a, b) u[qu + (long)(a) * su + (b)]
+/* The group partials at p (n <= 1024, agent-scope loads, one per thread)
+ * into LDS; after a __syncthreads, g_lds_sum adds them in order: the same
+ * value in every workgroup of the group. */
+static __device__ __forceinline__ void g_group_load(const double *p, int n, double *lds) {
+    if ((int)threadIdx.x < n)
+        lds[threadIdx.x] = __hip_atomic_load(const_cast<double *>(&p[threadIdx.x]), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ double g_lds_sum(const double *lds, int n) {
+    double r = 0.0;
+    for (int q = 0; q < n; q++) r += lds[q];
+    return r;
+}
+
+template <typename T>
+static __device__ __forceinline__ void g_xs(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+static __device__ __forceinline__ T g_xl(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* The interface flux of g_reshape with the relaxed field read through U(a, b)
+ * (offsets from the pixel) and the normals on the grid (row stride sj). */
+template <typename R, typename FU>
+static __device__ __forceinline__ R g_flux(R un, FU U, const R *__restrict__ jx, const R *__restrict__ jy, size_t qj,
+                                           int sj, R wi, R rdt, R dx) {
 #define JX(a, b) jx[qj + (long)(a) * sj + (b)]
 #define JY(a, b) jy[qj + (long)(a) * sj + (b)]
     un += 0.5 * (JX(1, 0) + JX(0, 0)) * (
@@ -803,9 +825,20 @@ static __device__ __forceinline__ R g_reshape2(R un, const R *__restrict__ u, si
         + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 0) + U(1, -1) - U(-1, 0) - U(-1, -1)) * 0.5 * (JX(0, 0) + JX(0, -1))
         - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(0, -1) * (1.0 - U(0, -1))));
     return un;
-#undef U
 #undef JX
 #undef JY
+}
+
+/* One pixel of a banded sweep: the relaxed field's rows above, at and below
+ * the pixel (up, mid, dn: each the pixel's column in that row; a
+ * neighbour's row comes from its copy in LDS) and the normals on the grid;
+ * u0 returns the centre value. */
+template <typename R>
+static __device__ __forceinline__ R g_flux_rows(const R *up, const R *mid, const R *dn, const R *__restrict__ jx,
+                                                const R *__restrict__ jy, size_t qj, int sj, R wi, R rdt, R dx, R &u0) {
+    auto U = [&](int a, int b) { return a < 0 ? up[b] : (a > 0 ? dn[b] : mid[b]); };
+    u0 = mid[0];
+    return g_flux(u0, U, jx, jy, qj, sj, wi, rdt, dx);
 }
 
 /* reshaping_cells (cell_phase_functions.cpp:447-580) with K workgroups per
@@ -815,72 +848,113 @@ static __device__ __forceinline__ R g_reshape2(R un, const R *__restrict__ u, si
  * reference's stencil reads there); the first holds phin's window.  Sweep
  * k reads one buffer and writes the other, so the reference's copy
  * phase <- phasen before each sweep (:480-492) is the change of roles, and
- * one barrier per sweep suffices.  The deviation is each workgroup's FP64
- * sum over its band, the K parts added in order by every workgroup (so all
- * take the same stop decision).  At the end phin's window gets the last
- * iterate and phi's the second-to-last.  S: [nc][2][W + 2][W + 2];
- * part: [nc][2 parities][K]; ctr: [nc] arrival counters, zero at launch. */
+ * one barrier per sweep suffices.  A workgroup reads and writes the window
+ * buffers only on its own rows (plain accesses); the first and last rows
+ * of every sweep's result also go to an exchange buffer with agent-scope
+ * stores, from which the neighbours load them into LDS with agent-scope
+ * loads after the barrier (the two rows next to the first and last band
+ * are the static ring rows).  The deviation is each workgroup's FP64 sum
+ * over its band, the K parts added in order by every workgroup (so all take
+ * the same stop decision).  At the end phin's window gets the last iterate
+ * and phi's the second-to-last.  S: [nc][2][W + 2][W + 2]; X: [nc][2
+ * parities][K][2][W + 2]; part: [nc][2][K]; ctr: [nc] arrival counters,
+ * zero at launch.  Dynamic LDS: 2 (W + 2) reals. */
 template <typename R>
 __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict__ phi, R *__restrict__ phin,
                                                          const R *__restrict__ jx, const R *__restrict__ jy,
                                                          const int *__restrict__ xc, const int *__restrict__ yc,
                                                          double thr, double rdev0, R *__restrict__ S,
-                                                         double *__restrict__ part, unsigned *__restrict__ ctr,
-                                                         int *__restrict__ counts, int *__restrict__ relax_flag,
-                                                         int *__restrict__ abort_flag, long long limit, int K) {
+                                                         R *__restrict__ X, double *__restrict__ part,
+                                                         unsigned *__restrict__ ctr, int *__restrict__ counts,
+                                                         int *__restrict__ relax_flag, int *__restrict__ abort_flag,
+                                                         long long limit, int K) {
     __shared__ double red[16];
+    __shared__ double sums[1024];
+    extern __shared__ char g_dyn[];
     const int N = P.N, zs = P.zs, W = 2 * zs, W2 = W + 2;
+    R *hu = (R *)g_dyn, *hd = hu + W2;                       /* the rows above and below the band */
     const int n = blockIdx.x / K, b = blockIdx.x % K;
     const size_t NN = (size_t)N * N, SS = (size_t)W2 * W2;
     R *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
     const R *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
     R *S0 = S + (size_t)n * 2 * SS, *S1 = S0 + SS;
+    R *Xn = X + (size_t)n * 2 * K * 2 * W2;                  /* [2][K][2][W2] */
+    auto xrow = [&](int par, int blk, int e) { return Xn + (((size_t)par * K + blk) * 2 + e) * W2; };
     const int x0 = xc[n] - zs, y0 = yc[n] - zs;              /* window origin on the grid */
     const int r0 = b * W / K, r1 = (b + 1) * W / K;          /* this workgroup's window rows */
     const int rows = r1 - r0;
     /* buffer rows r0+1 .. r1 (window rows r0 .. r1-1) and, for the first and
      * last workgroup, the ring rows 0 and W+1: ring columns from phi, the
-     * window from phin */
+     * window from phin; the band's first and last rows also to the exchange */
     const int f0 = (b == 0) ? 0 : r0 + 1, f1 = (b == K - 1) ? W2 : r1 + 1;
     for (int k = threadIdx.x; k < (f1 - f0) * W2; k += blockDim.x) {
         const int sr = f0 + k / W2, sc = k % W2;
         const size_t g = (size_t)(x0 - 1 + sr) * N + (y0 - 1 + sc);
         const bool ring = sr == 0 || sr == W + 1 || sc == 0 || sc == W + 1;
         const R v = ring ? ph[g] : pn[g];
-        S0[(size_t)sr * W2 + sc] = v;
-        if (ring) S1[(size_t)sr * W2 + sc] = v;
+        const size_t q = (size_t)sr * W2 + sc;
+        S0[q] = v;
+        if (ring) S1[q] = v;
+        if (sr == r0 + 1) g_xs(&xrow(0, b, 0)[sc], v);
+        if (sr == r1) g_xs(&xrow(0, b, 1)[sc], v);
     }
     unsigned epoch = 1;
     if (!g_barrier(&ctr[n], epoch * K, abort_flag, limit)) return;
     const R dx = P.dx;
     double rdev = rdev0;
     int ri = 0, count = 0, cur = 0;
+    /* halo rows of the sweep to come (and, from the second sweep on, the
+     * previous sweep's partials) are loaded together after each barrier */
+    auto load_halo = [&](const R *src, int par) {
+        for (int k = threadIdx.x; k < 2 * W2; k += blockDim.x) {
+            const int e = k / W2, c = k % W2;
+            R v;
+            if (e == 0) v = b == 0 ? src[c] : g_xl(&xrow(par, b - 1, 1)[c]);
+            else v = b == K - 1 ? src[(size_t)(W + 1) * W2 + c] : g_xl(&xrow(par, b + 1, 0)[c]);
+            hu[k] = v;
+        }
+    };
+    load_halo(S0, 0);
+    __syncthreads();
     while (rdev > thr) {
         const R *src = cur ? S1 : S0;
         R *dst = cur ? S0 : S1;
+        const int par = count & 1;
         double acc = 0.0;
         for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
             const int wr = r0 + k / W, wc = k % W;
             const size_t qs = (size_t)(wr + 1) * W2 + (wc + 1);
             const size_t qj = (size_t)(x0 + wr) * N + (y0 + wc);
-            const R v = g_reshape2(src[qs], src, qs, W2, jxn, jyn, qj, N, P.wi, P.rdt, dx);
+            const R *up = wr == r0 ? hu + (wc + 1) : src + (qs - W2);
+            const R *dn = wr == r1 - 1 ? hd + (wc + 1) : src + (qs + W2);
+            R o;
+            const R v = g_flux_rows(up, src + qs, dn, jxn, jyn, qj, N, P.wi, P.rdt, dx, o);
             dst[qs] = v;
-            acc += fabs((double)(v - src[qs])) * dx * dx / P.ncur;
+            if (wr == r0) g_xs(&xrow(par ^ 1, b, 0)[wc + 1], v);
+            if (wr == r1 - 1) g_xs(&xrow(par ^ 1, b, 1)[wc + 1], v);
+            acc += fabs((double)(v - o)) * dx * dx / P.ncur;
+        }
+        /* the exchanged rows' ring columns (static) */
+        if (threadIdx.x < 4) {
+            const int e = threadIdx.x >> 1, c = (threadIdx.x & 1) ? W + 1 : 0;
+            const size_t qs = (size_t)(e ? r1 : r0 + 1) * W2 + c;
+            g_xs(&xrow(par ^ 1, b, e)[c], src[qs]);
         }
         const double t = gblock_sum(acc, red);
-        const int par = count & 1;
-        if (threadIdx.x == 0) part[((size_t)n * 2 + par) * K + b] = t;
+        if (threadIdx.x == 0) g_xs(&part[((size_t)n * 2 + par) * K + b], t);
         epoch++;
         if (!g_barrier(&ctr[n], epoch * K, abort_flag, limit)) return;
-        rdev = 0.0;
-        for (int q = 0; q < K; q++) rdev += part[((size_t)n * 2 + par) * K + q];
+        g_group_load(&part[((size_t)n * 2 + par) * K], K, sums);
+        load_halo(dst, par ^ 1);                     /* for the next sweep (unused if the loop stops) */
+        __syncthreads();
+        rdev = g_lds_sum(sums, K);
         count++;
         cur ^= 1;
         if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :565-568 */
         ri++;
     }
     /* phin <- the last iterate, phi <- the one before (the reference's copy
-     * at the start of the last sweep) */
+     * at the start of the last sweep); own rows only */
     const R *last = cur ? S1 : S0, *prev = cur ? S0 : S1;
     for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
         const int wr = r0 + k / W, wc = k % W;
@@ -892,51 +966,97 @@ __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict_
     if (threadIdx.x == 0 && b == 0) counts[n] = count;
 }
 
-/* reshaping_lumen_or_ECM's loop (cell_phase_functions.cpp:774-866) with K
- * workgroups, workgroup b on grid rows [b N / K, (b + 1) N / K): sweep k
- * reads bufB (k odd) or bufA (k even) and writes the other, the reference's
- * copy lum <- lumn being the change of roles; the Dirichlet ring as
- * g_lum_sweep.  out[0]: sweeps done (the last iterate is in bufA when odd,
- * in bufB when even).  part: [2][K]; ctr: one arrival counter, zero at
- * launch. */
+/* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop
+ * (cell_phase_functions.cpp:774-866) with K workgroups; the caller ran
+ * sweep 1 on the whole grid (lumn <- F(lum), the reference's copy lum <-
+ * lumn being the change of roles: X_1 is in bufA, X_0 in bufB) and wrote
+ * the Dirichlet ring of boundary_conditions_dirichlet into bufB too, which
+ * sweep 2 would write, so both buffers now hold the same constant ring and
+ * the ring adds 0 to every later deviation.  Sweep k + 1 reads X_k (bufA
+ * for odd k) and writes the other buffer.  A sweep leaves a pixel whose
+ * 3 x 3 neighbourhood has zero normals unchanged exactly (every flux term
+ * carries a face average of the normals), and after sweep 1 both buffers
+ * agree outside box = [box[0], box[1]) x [box[2], box[3]) (the nonzero
+ * normals grown by one, inside rows / columns 1..N-2), so only the box is
+ * swept, workgroup b on box rows [r0, r1).  A workgroup reads and writes
+ * the buffers only on its rows and on static pixels; its first and last
+ * rows also go to an exchange buffer (agent-scope stores), from which the
+ * neighbours load them into LDS (agent-scope loads) after each barrier.
+ * count / ri: sweeps done and the rimax counter on entry; rdev: the last
+ * deviation.  out[0]: sweeps done in all (the last iterate is in bufA when
+ * odd, in bufB when even).  X: [2][K][2][box width + 2]; part: [2][K]; ctr:
+ * one arrival counter, zero at launch.  Dynamic LDS: 2 (box width + 2)
+ * reals. */
 template <typename R>
 __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ bufA, R *__restrict__ bufB,
-                                                      const R *__restrict__ jx, const R *__restrict__ jy, R side,
-                                                      double thr, double rdev0, double *__restrict__ part,
-                                                      unsigned *__restrict__ ctr, int *__restrict__ out,
-                                                      int *__restrict__ relax_flag, int *__restrict__ abort_flag,
-                                                      long long limit, int K) {
+                                                      const R *__restrict__ jx, const R *__restrict__ jy,
+                                                      const int *__restrict__ box, double thr, double rdev,
+                                                      int count, int ri, R *__restrict__ X,
+                                                      double *__restrict__ part, unsigned *__restrict__ ctr,
+                                                      int *__restrict__ out, int *__restrict__ relax_flag,
+                                                      int *__restrict__ abort_flag, long long limit, int K) {
     __shared__ double red[16];
+    __shared__ double sums[1024];
+    extern __shared__ char g_dyn[];
     const int N = P.N, b = blockIdx.x;
-    const int r0 = b * N / K, r1 = (b + 1) * N / K;
+    const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
+    const int bw = bj1 - bj0, bh = bi1 - bi0, L = bw + 2;    /* the caller launches only for a nonempty box */
+    R *hu = (R *)g_dyn, *hd = hu + L;                        /* grid rows r0-1 and r1, columns bj0-1 .. bj1 */
+    auto xrow = [&](int par, int blk, int e) { return X + (((size_t)par * K + blk) * 2 + e) * L; };
+    const int r0 = bi0 + b * bh / K, r1 = bi0 + (b + 1) * bh / K;     /* this workgroup's box rows */
     const R dx = P.dx;
-    double rdev = rdev0;
-    int ri = 0, count = 0;
-    unsigned epoch = 0;
-    while (rdev > thr) {
-        const R *u = (count & 1) ? bufA : bufB;      /* sweep count+1: odd reads B */
-        R *un = (count & 1) ? bufB : bufA;
-        double acc = 0.0;
-        for (size_t k = threadIdx.x; k < (size_t)(r1 - r0) * N; k += blockDim.x) {
-            const int i = r0 + (int)(k / N), j = (int)(k % N);
-            const size_t q = (size_t)i * N + j;
+    /* the exchange rows of X_count (in the buffer the next sweep reads) */
+    {
+        const R *u = (count & 1) ? bufA : bufB;
+        for (int k = threadIdx.x; k < 2 * L; k += blockDim.x) {
+            const int e = k / L, c = k % L;
+            g_xs(&xrow(count & 1, b, e)[c], u[(size_t)(e ? r1 - 1 : r0) * N + (bj0 - 1 + c)]);
+        }
+    }
+    unsigned epoch = 1;
+    if (!g_barrier(ctr, epoch * K, abort_flag, limit)) return;
+    auto load_halo = [&](const R *u, int par) {
+        for (int k = threadIdx.x; k < 2 * L; k += blockDim.x) {
+            const int e = k / L, c = k % L;
             R v;
-            if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
-                const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
-                v = corner ? (R)0.0 : side;
-            } else {
-                v = g_reshape(u[q], u, jx, jy, N, q, P.wi, P.rdt, dx);
-            }
+            if (e == 0) v = b == 0 ? u[(size_t)(r0 - 1) * N + (bj0 - 1 + c)] : g_xl(&xrow(par, b - 1, 1)[c]);
+            else v = b == K - 1 ? u[(size_t)r1 * N + (bj0 - 1 + c)] : g_xl(&xrow(par, b + 1, 0)[c]);
+            hu[k] = v;
+        }
+    };
+    load_halo((count & 1) ? bufA : bufB, count & 1);
+    __syncthreads();
+    while (rdev > thr) {
+        const R *u = (count & 1) ? bufA : bufB;
+        R *un = (count & 1) ? bufB : bufA;
+        const int par = count & 1;
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < (r1 - r0) * bw; k += blockDim.x) {
+            const int i = r0 + k / bw, j = bj0 + k % bw;
+            const size_t q = (size_t)i * N + j;
+            const int c = j - bj0 + 1;                            /* column in the exchange rows */
+            const R *up = i == r0 ? hu + c : u + (q - N);
+            const R *dn = i == r1 - 1 ? hd + c : u + (q + N);
+            R o;
+            const R v = g_flux_rows(up, u + q, dn, jx, jy, q, N, P.wi, P.rdt, dx, o);
             un[q] = v;
-            acc += fabs((double)(v - u[q])) * dx * dx;
+            if (i == r0) g_xs(&xrow(par ^ 1, b, 0)[c], v);
+            if (i == r1 - 1) g_xs(&xrow(par ^ 1, b, 1)[c], v);
+            acc += fabs((double)(v - o)) * dx * dx;
+        }
+        /* the exchanged rows' columns bj0-1 and bj1 (static) */
+        if (threadIdx.x < 4) {
+            const int e = threadIdx.x >> 1, c = (threadIdx.x & 1) ? L - 1 : 0;
+            g_xs(&xrow(par ^ 1, b, e)[c], u[(size_t)(e ? r1 - 1 : r0) * N + (bj0 - 1 + c)]);
         }
         const double t = gblock_sum(acc, red);
-        const int par = count & 1;
-        if (threadIdx.x == 0) part[par * K + b] = t;
+        if (threadIdx.x == 0) g_xs(&part[par * K + b], t);
         epoch++;
         if (!g_barrier(ctr, epoch * K, abort_flag, limit)) return;
-        rdev = 0.0;
-        for (int q = 0; q < K; q++) rdev += part[par * K + q];
+        g_group_load(&part[par * K], K, sums);
+        load_halo(un, par ^ 1);                      /* for the next sweep (unused if the loop stops) */
+        __syncthreads();
+        rdev = g_lds_sum(sums, K);
         count++;
         if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :862-865 */
         ri++;
@@ -944,10 +1064,49 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
     if (threadIdx.x == 0 && b == 0) out[0] = count;
 }
 
-/* a launch's arrival counters and abort flag start at zero (a fill kernel,
- * not a memset, so the sequence can be captured) */
+/* the Dirichlet ring of boundary_conditions_dirichlet (useful_functions.cpp:
+ * 422-437): edges `side`, corners 0 */
+template <typename R>
+__global__ void g_ring_dirichlet(int N, R *__restrict__ t, R side) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 4 * N; k += gridDim.x * blockDim.x) {
+        const int e = k / N, c = k % N;
+        size_t q;
+        int i, j;
+        if (e == 0) { i = 0; j = c; } else if (e == 1) { i = N - 1; j = c; } else if (e == 2) { i = c; j = 0; }
+        else { i = c; j = N - 1; }
+        q = (size_t)i * N + j;
+        t[q] = ((i == 0 || i == N - 1) && (j == 0 || j == N - 1)) ? (R)0.0 : side;
+    }
+}
+
+/* The box of nonzero normals (interior), grown by one and clipped to rows /
+ * columns 1..N-2; box = {N, 0, N, 0} on entry (empty). */
+template <typename R>
+__global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ jy, int *__restrict__ box) {
+    int i0 = N, i1 = -1, j0 = N, j1 = -1;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)N * N;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / N), j = (int)(q % N);
+        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
+        if (jx[q] != (R)0.0 || jy[q] != (R)0.0) {
+            i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
+        }
+    }
+    if (i1 >= 0) {
+        atomicMin(&box[0], max(1, i0 - 1));
+        atomicMax(&box[1], min(N - 1, i1 + 2));
+        atomicMin(&box[2], max(1, j0 - 1));
+        atomicMax(&box[3], min(N - 1, j1 + 2));
+    }
+}
+
+/* a launch's arrival counters and abort flag start at zero, the normals'
+ * box empty (fill kernels, not memsets, so the sequence can be captured) */
 __global__ void g_zero_u32(unsigned *p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0u;
+}
+__global__ void g_box_empty(int *box, int N) {
+    if (threadIdx.x == 0) { box[0] = N; box[1] = 0; box[2] = N; box[3] = 0; }
 }
 
 template <typename R>
@@ -1116,7 +1275,8 @@ struct GenCtx final : Generic {
     R *swin = nullptr;
     double *mb_part = nullptr;
     unsigned *mb_ctr = nullptr;      /* [nc] cell groups, [nc] the lumen/ECM group, [nc + 1] abort */
-    int *mb_out = nullptr;
+    int *mb_out = nullptr, *mb_box = nullptr;
+    R *mb_xch = nullptr;             /* exchanged band rows: cells [nc][2][kc][2][W + 2]; lumen/ECM [2][kl][2][N] */
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
@@ -1165,11 +1325,20 @@ struct GenCtx final : Generic {
             if (ncu <= 0 || khz <= 0) return ORG_EHIP;
             wait_limit = 2000LL * khz;                 /* 2 s */
             /* one workgroup per compute unit at most (all must be resident),
-             * about 4096 pixels or more each */
-            kc = std::min(std::min(ncu / nc, (W * W + 4095) / 4096), W / 4);
+             * about 8192 pixels or more each: a sweep's share then takes
+             * ~13 us against ~5 us for the barrier (measured on the 400^2
+             * windows of the template configuration: 4096 pixels per
+             * workgroup left the FP64-bound flux at ~12 us and doubled the
+             * barrier's wait for the slowest workgroup) */
+            kc = std::min(std::min(ncu / nc, (W * W + 8191) / 8192), W / 8);
             if (kc < 2) kc = 1;
-            kl = (int)std::min<size_t>(std::min<size_t>((size_t)ncu, (NN + 4095) / 4096), (size_t)(N / 4));
+            kl = (int)std::min<size_t>(std::min<size_t>((size_t)ncu, (NN + 4095) / 4096), (size_t)(N / 8));
             if (kl < 1) kl = 1;
+            if (2 * sizeof(R) * (size_t)(N + 2) > 65536 || 2 * sizeof(R) * (size_t)(W + 2) > 65536) {
+                fprintf(stderr, "organoid (generic path): grid side %d / window %d too large for the loops' LDS rows\n",
+                        N, W);
+                return ORG_EINVAL;
+            }
         }
         GTRY(A_(&f0, nc)); GTRY(A_(&s_, nc)); GTRY(A_(&s0, nc)); GTRY(A_(&sum_e, 1));
         GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
@@ -1177,7 +1346,8 @@ struct GenCtx final : Generic {
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
-        GTRY(A_(&mb_out, 1));
+        GTRY(A_(&mb_out, 1)); GTRY(A_(&mb_box, 4));
+        GTRY(A_(&mb_xch, std::max((size_t)nc * 2 * kc * 2 * (W + 2), (size_t)2 * kl * 2 * N)));
         std::vector<double> s0v(nc);
         for (int n = 0; n < nc; n++) s0v[n] = (p.use_s0_cell && s0_cell) ? s0_cell[n] : p.s0;
         std::vector<int> cen(nc, N / 2);
@@ -1390,28 +1560,60 @@ struct GenCtx final : Generic {
         GLAUNCH((g_lum_normals<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx, jly);
         GTRY(bc(jlx, 1));                                                                     /* :770-771 */
         GTRY(bc(jly, 1));
-        const double rdev0 = (double)(p.rdev_ini * rtol * rdt);                               /* :720 */
+        double rdev = (double)(p.rdev_ini * rtol * rdt);                                      /* :720 */
         const double thr = (double)(p.rdev_threshold * rtol * rdt);                           /* :774 */
         *failed = 0;
-        /* the loop on the device: sweep 1 reads lumn (the update's output)
-         * and writes lum's buffer, and so on (g_lum_loop_mb) */
-        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
-        GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
-        GLAUNCH((g_lum_loop_mb<R>), dim3(kl), dim3(1024), 0, P, u, un, (const R *)jlx, (const R *)jly, side, thr,
-                rdev0, mb_part, mb_ctr + nc, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kl);
-        int h[3] = {0, 0, 0};
-        GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
-        GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-        GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        GHIP(hipStreamSynchronize(s));
-        if (h[2]) {
-            fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each other "
-                            "for more than 2 s (not all resident?)\n");
-            return ORG_ETIMEOUT;
+        int count = 0, ri = 0;
+        if (rdev > thr) {
+            /* sweep 1 on the whole grid: lum <- lumn (the role change), lumn <- F(lum) into lum's buffer */
+            GLAUNCH((g_lum_sweep<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, u, (const R *)jlx,
+                    (const R *)jly, side);
+            GTRY(plane_sum(u, un, 1));                                                        /* :848-855 */
+            GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
+            GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, mb_box, N);
+            GLAUNCH((g_nz_box<R>), dim3(grid_for(NN)), dim3(256), 0, N, (const R *)jlx, (const R *)jly, mb_box);
+            int bx[4];
+            GHIP(hipMemcpyAsync(bx, mb_box, sizeof(bx), hipMemcpyDeviceToHost, s));
+            GHIP(hipStreamSynchronize(s));
+            count = 1;
+            if (ri > p.rimax) *failed = 1;                                                    /* :862-865 */
+            else ri++;
+            const int bh = bx[1] - bx[0], bw = bx[3] - bx[2];
+            if (!*failed && rdev > thr) {
+                /* the ring sweep 2 writes into lumn's buffer, now: it is then
+                 * the same constant ring in both buffers */
+                GLAUNCH((g_ring_dirichlet<R>), dim3((4 * N + 255) / 256), dim3(256), 0, N, un, side);
+                if (bh <= 0 || bw <= 0) {
+                    /* no nonzero normals: sweep 2 changes nothing (both buffers already agree) */
+                    count = 2;
+                    if (ri > p.rimax) *failed = 1;
+                } else {
+                    /* sweeps 2, 3, ... on the device, bands of at least 8 box rows
+                     * and about 8192 pixels */
+                    const int k = std::max(1, std::min(std::min(kl, (bh + 7) / 8),
+                                                       (int)(((size_t)bh * bw + 8191) / 8192)));
+                    const size_t shm = 2 * sizeof(R) * (size_t)(bw + 2);
+                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+                    GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+                    GLAUNCH((g_lum_loop_mb<R>), dim3(k), dim3(1024), shm, P, u, un, (const R *)jlx,
+                            (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_xch, mb_part,
+                            mb_ctr + nc, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, k);
+                    int h[3] = {0, 0, 0};
+                    GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipStreamSynchronize(s));
+                    if (h[2]) {
+                        fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
+                                        "other for more than 2 s (not all resident?)\n");
+                        return ORG_ETIMEOUT;
+                    }
+                    count = h[0];
+                    *failed = h[1];
+                }
+            }
         }
-        const int count = h[0];
         if (count & 1) std::swap(u, un);            /* an odd count leaves the last iterate in lum's buffer */
-        *failed = h[1];
         (which ? count_ecm : count_lum) = count;
         return ORG_OK;
     }
@@ -1436,9 +1638,9 @@ struct GenCtx final : Generic {
         GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
         if (kc > 1) {
             GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
-            GLAUNCH((g_relax_cells_mb<R>), dim3(nc * kc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
-                    (const int *)xc, (const int *)yc, thr, rdev0, swin, mb_part, mb_ctr, counts, relax,
-                    (int *)(mb_ctr + nc + 1), wait_limit, kc);
+            GLAUNCH((g_relax_cells_mb<R>), dim3(nc * kc), dim3(1024), 2 * sizeof(R) * (W + 2), P, phi, phin,
+                    (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, swin, mb_xch,
+                    mb_part, mb_ctr, counts, relax, (int *)(mb_ctr + nc + 1), wait_limit, kc);
         } else {
             GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
                     (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);

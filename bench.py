@@ -21,6 +21,13 @@ than later (its stop test is an absolute threshold on the whole grid's
 change), so short windows cost more per step; the line reports the window
 and the sweeps executed in it.
 
+Timed beside it (``config``): the plain form, the uneven 8-organoid sweep
+batch, the 256-cell 2048^2 workload, the FNO surrogate, the FP64 form on
+the same workload (``fp64_form``) and the reference template's own
+configuration (``template``: 6 cells, 1200^2, FFT box 398, on the
+full-plane path in the float build's arithmetic) against the reference's
+published steps/s for it.
+
 Printed: one JSON line (see the repository's benchmark contract) with a
 ``roofline`` object for the dominant kernel (algorithmic bytes per launch /
 HIP-event launch time on the library's stream, both over a replay of the
@@ -130,6 +137,65 @@ def large_rates(host, make_workload, steps: int) -> dict:
     return {"workload": wl.name, "cells": nc, "grid": p.N, "steps": steps, "ms_per_step": round(1e3 * dt / steps, 4),
             "cell_steps_per_s": round(rate, 2), "plain_dropin_cell_steps_per_s": round(prate, 2),
             "speedup_vs_plain": round(rate / prate, 3)}
+
+
+# The reference's own throughput figure: 100001 steps of its template
+# configuration (6 cells, 1200^2, zone_size 200: FFT box 398) "runned about
+# 25 times in 24 hours" on an unstated NVIDIA GPU, saving every 10000 steps
+# (README.md:61): about 28.9 steps/s.
+REFERENCE_TEMPLATE_STEPS_PER_S = 100001 * 25 / 86400.0
+
+
+def template_rates(host, make_workload, steps: int) -> dict:
+    """template6_1200 -- the reference template's own configuration -- on
+    the full-plane path in FP32 (the reference's float arithmetic, bit for
+    bit): the IC program's relaxation (org_relax_initial_state, untimed),
+    20 warm-up steps, then `steps` timed steps."""
+    wl = make_workload("template6_1200")
+    p = wl.params
+    org = host.Organoid(p, wl.s0_cell)
+    org.load_workload(wl)
+    t0 = time.perf_counter()
+    org.relax_initial_state(0.001, 0.01)
+    org.sync()
+    relax_s = time.perf_counter() - t0
+    org.prepare()
+    org.step(20)
+    org.sync()
+    t0 = time.perf_counter()
+    org.step(steps)
+    org.sync()
+    dt = time.perf_counter() - t0
+    cells, lum, ecm = org.iterations()
+    org.close()
+    sps = steps / dt
+    print(f"bench: template6_1200 {1e3 * dt / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": wl.name, "cells": p.ncell, "grid": p.N, "zone_size": p.zone_size, "fft_box": p.M,
+            "form": "full-plane path, FP32 (the float build's arithmetic), host-driven steps",
+            "ic_relaxation_s": round(relax_s, 3), "timed_steps": [20, 20 + steps - 1],
+            "ms_per_step": round(1e3 * dt / steps, 4), "steps_per_s": round(sps, 2),
+            "cell_steps_per_s": round(p.ncell * sps, 2),
+            "last_step_reshape_sweeps": {"cells": int(cells.max()), "lum": lum, "ecm": ecm},
+            "reference_published_steps_per_s": round(REFERENCE_TEMPLATE_STEPS_PER_S, 2),
+            "vs_reference_published": round(sps / REFERENCE_TEMPLATE_STEPS_PER_S, 3)}
+
+
+def fp64_rates(host, wl, steps: int) -> dict:
+    """The FP64 form (org_create_f64: the reference retyped to double, the
+    full-plane path) on the bench workload, steps 5..5+steps-1."""
+    p = wl.params
+    org = host.Organoid(p, wl.s0_cell, precision="f64")
+    org.load_workload(wl)
+    org.step(5)
+    org.sync()
+    t0 = time.perf_counter()
+    org.step(steps)
+    org.sync()
+    dt = time.perf_counter() - t0
+    org.close()
+    print(f"bench: FP64 form {1e3 * dt / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"timed_steps": [5, 5 + steps - 1], "ms_per_step": round(1e3 * dt / steps, 4),
+            "cell_steps_per_s": round(p.ncell * steps / dt, 2)}
 
 
 def fno_forward_only(n: int) -> int:
@@ -249,6 +315,11 @@ def main() -> int:
                     help="keep the default launch shape instead of autotuning it first")
     ap.add_argument("--surrogate-steps", type=int, default=40,
                     help="steps timed with the FNO velocity surrogate (0: skip)")
+    ap.add_argument("--template-steps", type=int, default=300,
+                    help="steps of the reference template's configuration (6 cells, 1200^2, FFT box 398) on the "
+                         "full-plane path (0: skip)")
+    ap.add_argument("--fp64-steps", type=int, default=20,
+                    help="steps of the bench workload in the FP64 form (0: skip)")
     ap.add_argument("--fno-forward", type=int, default=0, metavar="N",
                     help="only run N forward passes of the FNO surrogate on 64 random crops (profiling)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -371,6 +442,14 @@ def main() -> int:
     if args.surrogate_steps > 0:
         surrogate = surrogate_rates(host, org, nc, args.surrogate_steps)
 
+    template = None
+    if args.template_steps > 0:
+        template = template_rates(host, make_workload, args.template_steps)
+
+    fp64 = None
+    if args.fp64_steps > 0:
+        fp64 = fp64_rates(host, wl, args.fp64_steps)
+
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import Oracle, threads
@@ -402,7 +481,8 @@ def main() -> int:
                    "kernel_ms": kernel_ms,
                    "reshape_sweeps_in_window": {"steps": nprof, "cells_mean_per_step": round(sweeps["cells_mean"], 3),
                                                 "lum_total": sweeps["lum"], "ecm_total": sweeps["ecm"]},
-                   "launch_tuning": tuning, "sweep8": sweep, "large": large, "surrogate": surrogate},
+                   "launch_tuning": tuning, "sweep8": sweep, "large": large, "surrogate": surrogate,
+                   "fp64_form": fp64, "template": template},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

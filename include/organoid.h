@@ -340,9 +340,21 @@ int org_debug_fft2d(float *data, int sign);
  * (M = 2..1024) through the full-plane path's transform: rows then
  * columns, each line in double (radix-2 for powers of two, Bluestein's
  * chirp-z otherwise), the input and output stored as float (f64 = 0) or
- * double (f64 = 1) as that path's contexts store them.  data: M*M
- * interleaved (re, im) doubles on the host, sign -1 or +1. */
-int org_debug_fft2d_generic(int M, double *data, int sign, int f64);
+ * double (f64 = 1) as that path's contexts store them, with the schedule
+ * (fuse, lines_per_wg) of org_set_fft_schedule.  data: M*M interleaved
+ * (re, im) doubles on the host, sign -1 or +1. */
+int org_debug_fft2d_generic(int M, double *data, int sign, int f64, int fuse, int lines_per_wg);
+
+/* The full-plane path's transform schedule (no counterpart in the reference,
+ * whose transforms are cuFFT plans): radix-2 stages fused per LDS round
+ * trip (fuse 1, 2, 3: radix 2, 4, 8 passes) and lines per workgroup (1, 2,
+ * 4; lines_per_wg x the power-of-two line length <= 2048).  Every schedule
+ * gives the same values (each butterfly is the same operation on the same
+ * operands); org_create / org_create_f64 time all of them on the step's
+ * largest batch of transforms and keep the fastest.  Contexts on the LDS
+ * path (one 16 x 8 schedule) report 0, 0 and refuse the setter. */
+int org_get_fft_schedule(org_ctx *ctx, int32_t *fuse, int32_t *lines_per_wg);
+int org_set_fft_schedule(org_ctx *ctx, int fuse, int lines_per_wg);
 
 /* Calls on this context in which a reshaping loop passed rimax (the
  * reference prints "Relaxation failure!" and goes on,

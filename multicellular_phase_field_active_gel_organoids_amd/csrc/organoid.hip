@@ -1426,8 +1426,25 @@ int org_debug_fft2d(float *data, int sign) {
     return rc;
 }
 
-int org_debug_fft2d_generic(int M, double *data, int sign, int f64) {
-    return gen_rc(gen::debug_fft2d(M, data, sign, f64));
+int org_debug_fft2d_generic(int M, double *data, int sign, int f64, int fuse, int lines_per_wg) {
+    return gen_rc(gen::debug_fft2d(M, data, sign, f64, fuse, lines_per_wg));
+}
+
+int org_get_fft_schedule(org_ctx *c, int32_t *fuse, int32_t *lines_per_wg) {
+    TRY(check_ctx(c));
+    if (!fuse || !lines_per_wg) return note_error(ORG_EINVAL);
+    if (!c->gen) {                 /* the LDS transform: 16 x 8, one 128^2 plane per workgroup */
+        *fuse = 0;
+        *lines_per_wg = 0;
+        return ORG_OK;
+    }
+    return gen_rc(c->gen->get_fft_schedule(fuse, lines_per_wg));
+}
+
+int org_set_fft_schedule(org_ctx *c, int fuse, int lines_per_wg) {
+    TRY(check_ctx(c));
+    if (!c->gen) return note_error(ORG_EINVAL);
+    return gen_rc(c->gen->set_fft_schedule(fuse, lines_per_wg));
 }
 
 /* ---- double-precision host buffers ----------------------------------------

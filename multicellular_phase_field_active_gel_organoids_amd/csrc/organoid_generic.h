@@ -43,6 +43,9 @@ struct Generic {
     virtual int rho_to_shape(double rho0, double tol) = 0;
     /* psi, h(lum), h(ECM) set to zero (the initial-condition program never computes them) */
     virtual int zero_gel_inputs() = 0;
+    /* the 2D transforms' schedule: stages fused per LDS round trip (1..3) and lines per workgroup (1, 2, 4) */
+    virtual int get_fft_schedule(int32_t *fuse, int32_t *lines_per_wg) = 0;
+    virtual int set_fft_schedule(int fuse, int lines_per_wg) = 0;
 };
 
 /* A context on `stream` (borrowed: the caller owns and destroys it). */
@@ -50,8 +53,8 @@ int create_generic(const org_params *p, const double *s0_cell, int f64, hipStrea
 
 /* One unnormalised 2D DFT of an M x M plane (interleaved re, im doubles on
  * the host) through the generic path's transform, stored as float (f64 = 0)
- * or double between calls, as its contexts do. */
-int debug_fft2d(int M, double *data, int sign, int f64);
+ * or double between calls, as its contexts do, with the given schedule. */
+int debug_fft2d(int M, double *data, int sign, int f64, int fuse, int lines_per_wg);
 
 }  // namespace gen
 

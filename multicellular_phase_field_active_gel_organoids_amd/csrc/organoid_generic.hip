@@ -81,65 +81,106 @@ struct FftPlan {
     double *tw = nullptr;      /* device: L/2 (cos, sin) pairs of length L */
     double *chirp[2] = {nullptr, nullptr};   /* device, sign -1 / +1: w_k (M pairs) */
     double *khat[2] = {nullptr, nullptr};    /* device, sign -1 / +1: FFT_L of the conjugate chirp (L pairs) */
+    /* schedule: radix-2 stages fused per LDS round trip (1, 2 or 3: radix
+     * 2, 4, 8) and lines per workgroup (1, 2 or 4) */
+    int fuse = 1, lpw = 1;
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-/* ---- line transforms in LDS (one workgroup per line) --------------------- */
+/* ---- line transforms in LDS ---------------------------------------------- */
 /* The oracle's radix-2 transform (oracle/organoid_oracle.c fft_pow2): the
  * input in bit-reversed order, then log2(n) butterfly stages with the
- * twiddle of stage `len` at k * (n / len); the same operations in the same
- * order per butterfly, so the values match it. */
-static __device__ void lds_fft_pow2(double *re, double *im, int n, int sign, const double *__restrict__ tw) {
-    for (int len = 2; len <= n; len <<= 1) {
-        const int half = len / 2, step = n / len;
-        for (int t = threadIdx.x; t < n / 2; t += blockDim.x) {
-            const int grp = t / half, k = t - grp * half;
-            const int a = grp * len + k, b = a + half;
-            const double wr = tw[2 * (k * step)], wi = sign * tw[2 * (k * step) + 1];
-            const double xr = re[b] * wr - im[b] * wi;
-            const double xi = re[b] * wi + im[b] * wr;
-            re[b] = re[a] - xr; im[b] = im[a] - xi;
-            re[a] += xr; im[a] += xi;
+ * twiddle of stage `len` at k * (n / len).  The schedule groups F
+ * consecutive stages per LDS round trip: each of nt threads owns groups of
+ * 2^F elements (indices base + k + m half0, m < 2^F) and runs the F stages'
+ * butterflies among them in registers.  Every butterfly is the same
+ * operation on the same operands in any schedule, so the values are the
+ * oracle's for F = 1, 2, 3 alike. */
+template <int F>
+static __device__ __forceinline__ void lds_fft_pow2(double *re, double *im, int n, int sign,
+                                                    const double *__restrict__ tw, int t0, int nt) {
+    int half0 = 1;
+    while (half0 < n) {
+        /* stages half0, 2 half0, ... up to F of them (fewer at the end) */
+        int f = 0;
+        while (f < F && (half0 << f) < n) f++;
+        const int g = 1 << f;                          /* elements per group */
+        const int span = half0 << f;                   /* indices one group covers */
+        for (int t = t0; t < n / g; t += nt) {
+            const int blk = t / half0, k = t - blk * half0;
+            const int base = blk * span + k;
+            double xr[8], xi[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                if (m < g) { xr[m] = re[base + m * half0]; xi[m] = im[base + m * half0]; }
+#pragma unroll
+            for (int sft = 0; sft < 3; sft++) {
+                if (sft >= f) break;
+                const int half = half0 << sft, step = n / (2 * half);
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    if (m >= g || (m & (1 << sft))) continue;
+                    const int kk = k + (m & ((1 << sft) - 1)) * half0;     /* index within its len-block */
+                    const double wr = tw[2 * (kk * step)], wi = sign * tw[2 * (kk * step) + 1];
+                    const int mb = m + (1 << sft);
+                    const double br = xr[mb], bi = xi[mb];
+                    const double pr = br * wr - bi * wi;
+                    const double pi_ = br * wi + bi * wr;
+                    xr[mb] = xr[m] - pr; xi[mb] = xi[m] - pi_;
+                    xr[m] += pr; xi[m] += pi_;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                if (m < g) { re[base + m * half0] = xr[m]; im[base + m * half0] = xi[m]; }
         }
         __syncthreads();
+        half0 <<= f;
     }
 }
 
 static __device__ __forceinline__ int bitrev(int i, int logn) { return (int)(__brev((unsigned)i) >> (32 - logn)); }
 
-/* nlines transforms of length M: line l of plane p = l / lines_per_plane
- * starts at p * plane + (l % lines_per_plane) * line_step, elements
- * elem_step apart, read from src and written to dst (which may be src).
- * Computed in double; TI, TO: the stored types.  LDS: 2 L doubles. */
-template <typename TI, typename TO>
+/* nlines transforms of length M, LPW lines per workgroup pass (256 / LPW
+ * threads each): line l of plane p = l / lines_per_plane starts at
+ * p * plane + (l % lines_per_plane) * line_step, elements elem_step apart,
+ * read from src and written to dst (which may be src).  Computed in double;
+ * TI, TO: the stored types.  LDS: 2 L doubles per line. */
+template <typename TI, typename TO, int F>
 __global__ __launch_bounds__(256) void g_fft_lines(const Cx<TI> *src, Cx<TO> *dst,
                                                    int nlines, int lines_per_plane, long plane, int line_step,
                                                    int elem_step, int M, int L, int sign, const double *__restrict__ tw,
-                                                   const double *__restrict__ chirp, const double *__restrict__ khat) {
+                                                   const double *__restrict__ chirp, const double *__restrict__ khat,
+                                                   int lpw) {
     extern __shared__ double sh[];
-    double *re = sh, *im = sh + L;
+    const int nt = blockDim.x / lpw, sub = threadIdx.x / nt, t0 = threadIdx.x - sub * nt;
+    double *re = sh + (size_t)sub * 2 * L, *im = re + L;
     const int logL = 31 - __clz(L);
-    for (int line = blockIdx.x; line < nlines; line += gridDim.x) {
-        const long base = (long)(line / lines_per_plane) * plane + (long)(line % lines_per_plane) * line_step;
+    for (int line0 = blockIdx.x * lpw; line0 < nlines; line0 += gridDim.x * lpw) {
+        const int line = line0 + sub;
+        const bool act = line < nlines;
+        const long base = act ? (long)(line / lines_per_plane) * plane + (long)(line % lines_per_plane) * line_step : 0;
         if (!chirp) {
-            for (int k = threadIdx.x; k < M; k += blockDim.x) {
-                const Cx<TI> v = src[base + (long)k * elem_step];
-                const int r = bitrev(k, logL);
-                re[r] = (double)v.x; im[r] = (double)v.y;
-            }
+            if (act)
+                for (int k = t0; k < M; k += nt) {
+                    const Cx<TI> v = src[base + (long)k * elem_step];
+                    const int r = bitrev(k, logL);
+                    re[r] = (double)v.x; im[r] = (double)v.y;
+                }
             __syncthreads();
-            lds_fft_pow2(re, im, L, sign, tw);
-            for (int k = threadIdx.x; k < M; k += blockDim.x) {
-                Cx<TO> o; o.x = (TO)re[k]; o.y = (TO)im[k];
-                dst[base + (long)k * elem_step] = o;
-            }
+            lds_fft_pow2<F>(re, im, L, sign, tw, t0, nt);
+            if (act)
+                for (int k = t0; k < M; k += nt) {
+                    Cx<TO> o; o.x = (TO)re[k]; o.y = (TO)im[k];
+                    dst[base + (long)k * elem_step] = o;
+                }
         } else {
             /* Bluestein (the oracle's fft_any): a_k = x_k w_k zero-padded to L,
              * A = F_-(a), C = A * K, c = F_+(C), X_k = w_k c_k / L */
-            for (int k = threadIdx.x; k < L; k += blockDim.x) {
+            for (int k = t0; k < L; k += nt) {
                 double ar = 0.0, ai = 0.0;
-                if (k < M) {
+                if (act && k < M) {
                     const Cx<TI> v = src[base + (long)k * elem_step];
                     const double xr = (double)v.x, xi = (double)v.y, wr = chirp[2 * k], wi = chirp[2 * k + 1];
                     ar = xr * wr - xi * wi;
@@ -149,29 +190,30 @@ __global__ __launch_bounds__(256) void g_fft_lines(const Cx<TI> *src, Cx<TO> *ds
                 re[r] = ar; im[r] = ai;
             }
             __syncthreads();
-            lds_fft_pow2(re, im, L, -1, tw);
+            lds_fft_pow2<F>(re, im, L, -1, tw, t0, nt);
             double pr[8], pi_[8];      /* this thread's products, then the bit-reversed reload */
             int cnt = 0;
-            for (int k = threadIdx.x; k < L; k += blockDim.x, cnt++) {
+            for (int k = t0; k < L; k += nt, cnt++) {
                 const double kr = khat[2 * k], ki = khat[2 * k + 1];
                 pr[cnt] = re[k] * kr - im[k] * ki;
                 pi_[cnt] = re[k] * ki + im[k] * kr;
             }
             __syncthreads();
             cnt = 0;
-            for (int k = threadIdx.x; k < L; k += blockDim.x, cnt++) {
+            for (int k = t0; k < L; k += nt, cnt++) {
                 const int r = bitrev(k, logL);
                 re[r] = pr[cnt]; im[r] = pi_[cnt];
             }
             __syncthreads();
-            lds_fft_pow2(re, im, L, +1, tw);
-            for (int k = threadIdx.x; k < M; k += blockDim.x) {
-                const double cr = re[k] / L, ci = im[k] / L, wr = chirp[2 * k], wi = chirp[2 * k + 1];
-                Cx<TO> o;
-                o.x = (TO)(cr * wr - ci * wi);
-                o.y = (TO)(cr * wi + ci * wr);
-                dst[base + (long)k * elem_step] = o;
-            }
+            lds_fft_pow2<F>(re, im, L, +1, tw, t0, nt);
+            if (act)
+                for (int k = t0; k < M; k += nt) {
+                    const double cr = re[k] / L, ci = im[k] / L, wr = chirp[2 * k], wi = chirp[2 * k + 1];
+                    Cx<TO> o;
+                    o.x = (TO)(cr * wr - ci * wi);
+                    o.y = (TO)(cr * wi + ci * wr);
+                    dst[base + (long)k * elem_step] = o;
+                }
         }
         __syncthreads();
     }
@@ -1230,18 +1272,85 @@ static int make_plan(int M, FftPlan &pl, std::vector<void *> &allocs) {
  * into the double work buffer, then columns out of it (cufftExecC2C on each
  * plane, useful_functions.cpp:646-660, with the double intermediate the
  * oracle's transform keeps). */
+template <typename TI, typename TO>
+static int fft_lines_launch(hipStream_t s, const FftPlan &pl, const Cx<TI> *src, Cx<TO> *dst, int nlines,
+                            long plane, int line_step, int elem_step, int sign) {
+    const int M = pl.M, L = pl.L, si = sign > 0 ? 1 : 0;
+    const dim3 grid(std::max(1, std::min((nlines + pl.lpw - 1) / pl.lpw, 8192))), block(256);
+    const size_t shm = 2 * sizeof(double) * L * pl.lpw;
+    const double *ch = pl.pow2 ? nullptr : pl.chirp[si], *kh = pl.pow2 ? nullptr : pl.khat[si];
+    switch (pl.fuse) {
+    case 1:
+        GLAUNCH((g_fft_lines<TI, TO, 1>), grid, block, shm, src, dst, nlines, M, plane, line_step, elem_step, M, L,
+                sign, pl.tw, ch, kh, pl.lpw);
+        break;
+    case 2:
+        GLAUNCH((g_fft_lines<TI, TO, 2>), grid, block, shm, src, dst, nlines, M, plane, line_step, elem_step, M, L,
+                sign, pl.tw, ch, kh, pl.lpw);
+        break;
+    default:
+        GLAUNCH((g_fft_lines<TI, TO, 3>), grid, block, shm, src, dst, nlines, M, plane, line_step, elem_step, M, L,
+                sign, pl.tw, ch, kh, pl.lpw);
+        break;
+    }
+    return ORG_OK;
+}
+
+/* nplanes M x M transforms src -> dst (unnormalised, sign -1 or +1): rows
+ * into the double work buffer, then columns out of it (cufftExecC2C on each
+ * plane, useful_functions.cpp:646-660, with the double intermediate the
+ * oracle's transform keeps). */
 template <typename R>
 static int fft2d_planes(hipStream_t s, const FftPlan &pl, const Cx<R> *src, Cx<R> *dst, Cx<double> *work,
                         int nplanes, int sign) {
-    const int M = pl.M, L = pl.L, si = sign > 0 ? 1 : 0;
+    const int M = pl.M;
     const long plane = (long)M * M;
-    const int nlines = nplanes * M;
-    const dim3 grid(std::min(nlines, 8192)), block(256);
-    const size_t shm = 2 * sizeof(double) * L;
-    const double *ch = pl.pow2 ? nullptr : pl.chirp[si], *kh = pl.pow2 ? nullptr : pl.khat[si];
-    GLAUNCH((g_fft_lines<R, double>), grid, block, shm, src, work, nlines, M, plane, M, 1, M, L, sign, pl.tw, ch, kh);
-    GLAUNCH((g_fft_lines<double, R>), grid, block, shm, (const Cx<double> *)work, dst, nlines, M, plane, 1, M, M, L,
-            sign, pl.tw, ch, kh);
+    GTRY((fft_lines_launch<R, double>(s, pl, src, work, nplanes * M, plane, M, 1, sign)));
+    return fft_lines_launch<double, R>(s, pl, (const Cx<double> *)work, dst, nplanes * M, plane, 1, M, sign);
+}
+
+/* A schedule is valid when every thread holds at most 8 elements of a
+ * Bluestein line and the lines fit the workgroup's LDS. */
+static bool schedule_ok(const FftPlan &pl, int fuse, int lpw) {
+    if (fuse < 1 || fuse > 3 || (lpw != 1 && lpw != 2 && lpw != 4)) return false;
+    if ((size_t)pl.L * lpw > 2048) return false;                 /* 256 / lpw threads, <= 8 elements each */
+    return 2 * sizeof(double) * (size_t)pl.L * lpw <= 65536;
+}
+
+/* The autotuner of the transform schedule: times every valid (fuse, lines
+ * per workgroup) on nplanes forward transforms of the given buffers (two
+ * runs each, the faster kept) and leaves the fastest in the plan.  All
+ * schedules give the same values. */
+template <typename R>
+static int tune_fft(hipStream_t s, FftPlan &pl, Cx<R> *a, Cx<R> *b, Cx<double> *work, int nplanes,
+                    std::vector<float> *times) {
+    hipEvent_t e0, e1;
+    GHIP(hipEventCreate(&e0));
+    GHIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bf = 1, bl = 1;
+    for (int fuse = 1; fuse <= 3; fuse++)
+        for (int lpw = 1; lpw <= 4; lpw *= 2) {
+            if (!schedule_ok(pl, fuse, lpw)) continue;
+            pl.fuse = fuse;
+            pl.lpw = lpw;
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                GHIP(hipEventRecord(e0, s));
+                GTRY(fft2d_planes<R>(s, pl, a, b, work, nplanes, -1));
+                GHIP(hipEventRecord(e1, s));
+                GHIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                GHIP(hipEventElapsedTime(&ms, e0, e1));
+                tmin = std::min(tmin, ms);
+            }
+            if (times) times->push_back(tmin);
+            if (tmin < best) { best = tmin; bf = fuse; bl = lpw; }
+        }
+    pl.fuse = bf;
+    pl.lpw = bl;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return ORG_OK;
 }
 
@@ -1360,7 +1469,24 @@ struct GenCtx final : Generic {
         GHIP(hipMemset(f0, 0, sizeof(R) * nc)); GHIP(hipMemset(s_, 0, sizeof(double) * nc));
         GHIP(hipMemset(err, 0, sizeof(int))); GHIP(hipMemset(counts, 0, sizeof(int) * nc));
         GHIP(hipMemset(relax, 0, sizeof(int)));
+        GHIP(hipMemset(A, 0, sizeof(Cx<R>) * 5 * nc * MM));
+        GHIP(hipMemset(B, 0, sizeof(Cx<R>) * 5 * nc * MM));
         GHIP(hipDeviceSynchronize());          /* the memsets above ran on the null stream */
+        /* the transform schedule, timed on the step's largest batch (5 ncell planes) */
+        GTRY(tune_fft<R>(s, plan, A, B, work, 5 * nc, &fft_times));
+        return ORG_OK;
+    }
+    std::vector<float> fft_times;
+    int get_fft_schedule(int32_t *fuse, int32_t *lpw) override {
+        *fuse = plan.fuse;
+        *lpw = plan.lpw;
+        return ORG_OK;
+    }
+    int set_fft_schedule(int fuse, int lpw) override {
+        if (!schedule_ok(plan, fuse, lpw)) return ORG_EINVAL;
+        GHIP(hipStreamSynchronize(s));
+        plan.fuse = fuse;
+        plan.lpw = lpw;
         return ORG_OK;
     }
 
@@ -1694,13 +1820,16 @@ int create_generic(const org_params *p, const double *s0_cell, int f64, hipStrea
 }
 
 template <typename R>
-static int debug_fft2d_t(int M, double *data, int sign) {
+static int debug_fft2d_t(int M, double *data, int sign, int fuse, int lpw) {
     std::vector<void *> allocs;
     FftPlan pl;
     Cx<R> *buf = nullptr;
     Cx<double> *work = nullptr;
     hipStream_t s = nullptr;
     int rc = make_plan(M, pl, allocs);
+    if (!rc && !schedule_ok(pl, fuse, lpw)) rc = ORG_EINVAL;
+    pl.fuse = fuse;
+    pl.lpw = lpw;
     const size_t MM = (size_t)M * M;
     std::vector<Cx<R>> h(MM);
     for (size_t k = 0; k < MM; k++) { h[k].x = (R)data[2 * k]; h[k].y = (R)data[2 * k + 1]; }
@@ -1715,9 +1844,9 @@ static int debug_fft2d_t(int M, double *data, int sign) {
     return ORG_OK;
 }
 
-int debug_fft2d(int M, double *data, int sign, int f64) {
+int debug_fft2d(int M, double *data, int sign, int f64, int fuse, int lpw) {
     if (M < 2 || M > 1024 || !data || (sign != 1 && sign != -1)) return ORG_EINVAL;
-    return f64 ? debug_fft2d_t<double>(M, data, sign) : debug_fft2d_t<float>(M, data, sign);
+    return f64 ? debug_fft2d_t<double>(M, data, sign, fuse, lpw) : debug_fft2d_t<float>(M, data, sign, fuse, lpw);
 }
 
 }  // namespace gen

@@ -82,7 +82,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_kernel_time.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.org_set_option.argtypes = [vp, i32, i32]
     lib.org_debug_fft2d.argtypes = [vp, i32]
-    lib.org_debug_fft2d_generic.argtypes = [i32, vp, i32, i32]
+    lib.org_debug_fft2d_generic.argtypes = [i32, vp, i32, i32, i32, i32]
+    lib.org_get_fft_schedule.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    lib.org_set_fft_schedule.argtypes = [vp, i32, i32]
     lib.org_batch_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
     lib.org_batch_step.argtypes = [vp, i32, i32]
     lib.org_batch_sync.argtypes = [vp]
@@ -230,6 +232,17 @@ class Organoid:
         n = ctypes.c_int64()
         _chk(self.lib.org_get_relax_failures(self.h, ctypes.byref(n)), "org_get_relax_failures")
         return n.value
+
+    def fft_schedule(self):
+        """(stages fused per LDS pass, lines per workgroup) of the full-plane
+        path's transforms, as the autotuner at creation picked them; (0, 0)
+        on the LDS path."""
+        f, l = ctypes.c_int32(), ctypes.c_int32()
+        _chk(self.lib.org_get_fft_schedule(self.h, ctypes.byref(f), ctypes.byref(l)), "org_get_fft_schedule")
+        return f.value, l.value
+
+    def set_fft_schedule(self, fuse: int, lines_per_wg: int) -> None:
+        _chk(self.lib.org_set_fft_schedule(self.h, int(fuse), int(lines_per_wg)), "org_set_fft_schedule")
 
     def get_tuning(self) -> dict:
         t = Tuning()
@@ -432,14 +445,15 @@ class Batch:
         self.close()
 
 
-def debug_fft2d_generic(a: np.ndarray, sign: int, f64: bool) -> np.ndarray:
+def debug_fft2d_generic(a: np.ndarray, sign: int, f64: bool, fuse: int = 1, lines_per_wg: int = 1) -> np.ndarray:
     """M x M unnormalised 2D DFT through the full-plane path's transform
     (radix-2 or Bluestein lines in double; input and output stored as
-    float64 when f64 else float32)."""
+    float64 when f64 else float32) with the given schedule."""
     lib = load_library()
     M = a.shape[0]
     buf = np.ascontiguousarray(np.stack([a.real, a.imag], -1), np.float64)
-    _chk(lib.org_debug_fft2d_generic(M, _p(buf), sign, 1 if f64 else 0), "org_debug_fft2d_generic")
+    _chk(lib.org_debug_fft2d_generic(M, _p(buf), sign, 1 if f64 else 0, fuse, lines_per_wg),
+         "org_debug_fft2d_generic")
     return buf[..., 0] + 1j * buf[..., 1]
 
 

@@ -51,6 +51,50 @@ def test_generic_fft2d(M, f64, sign):
     assert r_np <= (1e-13 if f64 else 1e-6)
 
 
+SCHEDULES = [(f, l) for f in (1, 2, 3) for l in (1, 2, 4)]
+
+
+@pytest.mark.parametrize("M", [64, 128, 398])
+def test_fft_schedules_give_identical_values(M):
+    """Every transform schedule (radix-2 stages fused 1, 2 or 3 per LDS pass;
+    1, 2 or 4 lines per workgroup) gives the same bits: each butterfly is
+    the same operation on the same operands."""
+    rng = np.random.default_rng(2000 + M)
+    a = rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))
+    L = M if M & (M - 1) == 0 else 1 << (2 * M - 2).bit_length()
+    ref = host.debug_fft2d_generic(a, -1, True, 1, 1)
+    ran = 0
+    for fuse, lpw in SCHEDULES:
+        if L * lpw > 2048:
+            continue
+        got = host.debug_fft2d_generic(a, -1, True, fuse, lpw)
+        assert np.array_equal(got, ref), (M, fuse, lpw)
+        ran += 1
+    print(f"M {M} (line length {L}): {ran} schedules, identical", flush=True)
+    assert ran >= 6
+
+
+def test_tuned_schedule_steps_identically():
+    """A context's autotuned schedule and the others step bit for bit alike
+    (two_cells_512_zone200, FP32, two steps)."""
+    wl = round32(make_workload("two_cells_512_zone200"))
+    out = {}
+    for sched in (None, (1, 1), (3, 1)):
+        g = host.Organoid(wl.params, wl.s0_cell)
+        tuned = g.fft_schedule()
+        assert tuned[0] in (1, 2, 3) and tuned[1] in (1, 2, 4)
+        if sched:
+            g.set_fft_schedule(*sched)
+        g.load_workload(wl)
+        g.step(2)
+        out[sched] = [g.get_field(f) for _, f in FIELDS]
+        print(f"schedule {sched or tuned} ({'tuned' if sched is None else 'set'})", flush=True)
+        g.close()
+    base = out[None]
+    for k, v in out.items():
+        assert all(np.array_equal(x, y) for x, y in zip(v, base)), k
+
+
 def _trajectory(name, precision, steps):
     wl = make_workload(name)
     if precision == "f32":

@@ -253,20 +253,20 @@ int org_set_tuning(org_ctx *ctx, const org_tuning *tuning);
  * cell's FFT box at the start of the step (x_dev, [.][2][M][M] FP32), the
  * solve's velocity (v_dev, [.][2][M][M], vx then vy) and, if res_dev is not
  * NULL, the judge's residual of that exact velocity (res_dev, [.] FP32: the
- * floor the finite-difference residual has on an exact solution).
+ * judge's floor on an exact solution).
  *
  * org_step_surrogate: nsteps steps in which the cortex velocity of every
  * cell comes from the network (net), or, with net NULL, from v_given
  * ([ncell][2][M][M] on the device, the same for every step: a caller's own
  * surrogate, and the test of this path), except that (a) every step with
  * (it0 + s) % k_exact == 0 solves every cell exactly, and (b) a cell whose
- * relative residual of the force balance (xi v - eta (lap v + grad div v) +
- * div pi, finite differences on the periodic box, over |div pi|) exceeds
- * `threshold` is solved exactly in that step.  exact_cells (nsteps int32,
- * may be NULL): cells solved exactly per step; residuals (nsteps*ncell
- * FP32, may be NULL): the judge's residual per step and cell (-1 where the
- * step was exact by (a)).  The network must have M = 128 and batch_max >=
- * ncell; it runs on the organoid's stream for the call. */
+ * judge residual (ORG_OPT_SURROGATE_JUDGE: spectral by default, or the
+ * finite-difference residual of the force balance) exceeds `threshold` is
+ * solved exactly in that step.  exact_cells (nsteps int32, may be NULL):
+ * cells solved exactly per step; residuals (nsteps*ncell FP32, may be
+ * NULL): the judge's residual per step and cell (-1 where the step was
+ * exact by (a)).  The network must have M = 128 and batch_max >= ncell; it
+ * runs on the organoid's stream for the call. */
 typedef struct fno_net fno_net;
 int org_surrogate_samples(org_ctx *ctx, int it0, int nsteps, float *x_dev, float *v_dev, float *res_dev);
 int org_step_surrogate(org_ctx *ctx, fno_net *net, const float *v_given, int it0, int nsteps, int k_exact,
@@ -310,8 +310,16 @@ enum {
 };
 
 /* Options.  ORG_OPT_WRITE_VELOCITY (default 1): keep the FFT-box velocity of
- * the last step for org_get_window. */
-enum { ORG_OPT_WRITE_VELOCITY = 0 };
+ * the last step for org_get_window.  ORG_OPT_SURROGATE_JUDGE: the judge of
+ * org_step_surrogate and of org_surrogate_samples' residuals,
+ * ORG_JUDGE_SPECTRAL (default): the candidate's relative L2 distance from
+ * the per-mode solution of update_rho_velocity's system (rho_functions.cpp:
+ * 133-190), computed from F(pi) and F(v) (three of the exact solve's five
+ * transforms; the exact solve scores ~1e-6), or ORG_JUDGE_FINITE_DIFF: the
+ * finite-difference residual of the force balance over |div pi| on the
+ * box (cheap, but the exact solve itself scores ~0.5 on sharp cortices). */
+enum { ORG_OPT_WRITE_VELOCITY = 0, ORG_OPT_SURROGATE_JUDGE = 1 };
+enum { ORG_JUDGE_SPECTRAL = 0, ORG_JUDGE_FINITE_DIFF = 1 };
 int org_set_option(org_ctx *ctx, int option, int value);
 
 /* ---- allocation-free form -----------------------------------------------

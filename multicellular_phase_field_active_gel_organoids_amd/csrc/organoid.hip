@@ -91,6 +91,7 @@ struct org_ctx {
     LumState *lst = nullptr;
     bool prepared = false;
     int write_v = 1;
+    int judge = ORG_JUDGE_SPECTRAL;        /* the surrogate's judge (ORG_OPT_SURROGATE_JUDGE) */
     int64_t relax_failures = 0;            /* calls in which a reshaping loop passed rimax */
     /* velocity surrogate: network input and output per cell, per-cell switch,
      * per-step judge residuals [cap][ncell] and exact-cell counts [cap] */
@@ -844,9 +845,14 @@ int org_set_tuning(org_ctx *c, const org_tuning *t) {
 
 int org_set_option(org_ctx *c, int option, int value) {
     TRY(check_ctx(c));
-    if (c->gen) return option == ORG_OPT_WRITE_VELOCITY ? ORG_OK : note_error(ORG_EINVAL);
+    if (c->gen) return option == ORG_OPT_WRITE_VELOCITY || option == ORG_OPT_SURROGATE_JUDGE ? ORG_OK
+                                                                                              : note_error(ORG_EINVAL);
     switch (option) {
     case ORG_OPT_WRITE_VELOCITY: c->write_v = value ? 1 : 0; break;
+    case ORG_OPT_SURROGATE_JUDGE:
+        if (value != ORG_JUDGE_SPECTRAL && value != ORG_JUDGE_FINITE_DIFF) return note_error(ORG_EINVAL);
+        c->judge = value;
+        return ORG_OK;        /* the batched step's graphs do not use it */
 
     default: return note_error(ORG_EINVAL);
     }
@@ -1114,9 +1120,9 @@ int org_surrogate_samples(org_ctx *c, int it0, int nsteps, float *x, float *v, f
         hipLaunchKernelGGL(k_pack_v, dim3(c->nc), dim3(1024), 0, c->stream, c->vxbox, c->vybox, v + s * per);
         if ((rc = launch_check())) break;
         if (res) {        /* phi, rho, lum, ECM and the centres are still the step's own until the copy-back */
-            hipLaunchKernelGGL(k_vresid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->lum,
-                               c->ecm, c->xc, c->yc, c->spec, v + s * per, INFINITY, res + (size_t)s * c->nc,
-                               c->sur_use, c->sur_nexact);
+            hipLaunchKernelGGL(c->judge == ORG_JUDGE_SPECTRAL ? k_vresid_spec : k_vresid, dim3(c->nc), dim3(1024), 0,
+                               c->stream, c->dp, c->phi, c->rho, c->lum, c->ecm, c->xc, c->yc, c->spec,
+                               v + s * per, INFINITY, res + (size_t)s * c->nc, c->sur_use, c->sur_nexact);
             if ((rc = launch_check())) break;
         }
         rc = step_epilogue(c, s + 1 == nsteps, need_pre);
@@ -1156,8 +1162,9 @@ int org_step_surrogate(org_ctx *c, fno_net *net, const float *v_given, int it0, 
                                    c->yc, c->sur_x);
                 if ((rc = launch_check()) || (rc = fno_forward(net, c->sur_x, c->nc, c->sur_v))) break;
             }
-            hipLaunchKernelGGL(k_vresid, dim3(c->nc), dim3(1024), 0, c->stream, c->dp, c->phi, c->rho, c->lum,
-                               c->ecm, c->xc, c->yc, c->spec, vsrc, thr, res, c->sur_use, c->sur_nexact + s);
+            hipLaunchKernelGGL(c->judge == ORG_JUDGE_SPECTRAL ? k_vresid_spec : k_vresid, dim3(c->nc), dim3(1024), 0,
+                               c->stream, c->dp, c->phi, c->rho, c->lum, c->ecm, c->xc, c->yc, c->spec, vsrc, thr,
+                               res, c->sur_use, c->sur_nexact + s);
             if ((rc = launch_check())) break;
         }
         if ((rc = L_step_v(c, vsrc, c->sur_use))) break;

@@ -2246,6 +2246,113 @@ __global__ __launch_bounds__(1024) void k_vresid(DevP P, const float *__restrict
     }
 }
 
+/* The spectral judge of a given velocity v ([ncell][2][M*M]): the relative
+ * L2 distance, in Fourier space, between the candidate and the solution of
+ * the per-mode friction-viscosity system update_rho_velocity solves
+ * (rho_functions.cpp:133-190), with the viscosity of the mode's pixel
+ * (:149-150) and the part whose inverse is real (the reference keeps Re,
+ * :279):
+ *   res = |F(v) - H(M^-1 b)| / |H(M^-1 b)|,  b = -i k . F(pi),
+ * summed over all modes of both components (by Parseval the real-space
+ * relative L2 error of v).  The exact solve's own velocity scores the
+ * rounding of its transforms (~1e-6), a zero velocity 1.  It takes the
+ * active stress pass, F(pi_xx + i pi_xy), F(pi_yy) and F(v_x + i v_y):
+ * three of the five transforms of the exact solve.  use[n] = res <= thr;
+ * cells over the threshold add one to *nfall. */
+__global__ __launch_bounds__(1024) void k_vresid_spec(DevP P, const float *__restrict__ phi,
+                                                      const float *__restrict__ rho, const float *__restrict__ lum,
+                                                      const float *__restrict__ ecm, const int *__restrict__ xc,
+                                                      const int *__restrict__ yc, float2 *__restrict__ scratch,
+                                                      const float *__restrict__ v, float thr, float *__restrict__ res,
+                                                      int *__restrict__ use, int *__restrict__ nfall) {
+    constexpr int MM = ORG_MB * ORG_MB;
+    __shared__ float2 sm[ORG_MB * ORG_LS];
+    __shared__ float2 tw[128];
+    __shared__ unsigned char lab[MM];
+    __shared__ double red[16];
+    const int n = blockIdx.x, t = threadIdx.x, N = P.N;
+    const size_t NN = (size_t)N * N;
+    const int n1 = (n == 0) ? P.nc - 1 : n - 1, n2 = (n == P.nc - 1) ? 0 : n + 1;
+    const int x0 = xc[n] - P.zs + 1, y0 = yc[n] - P.zs + 1;
+    const float *ph = phi + n * NN, *ph1 = phi + n1 * NN, *ph2 = phi + n2 * NN, *rh = rho + n * NN;
+    float2 *SA = scratch + (size_t)n * 2 * MM;         /* F(pi_xx + i pi_xy) */
+    float *SB = (float *)(SA + MM);                    /* pi_yy, then (as float2) F(pi_yy) */
+    float2 *SY = (float2 *)SB;
+    const float inv4 = 1.0f / (4.0f * P.dx * P.dx);
+    const float kscale = 6.28318530718f / (ORG_MB * P.dx);
+    if (t < 128) tw[t] = make_float2(kCos128[t], kSin128[t]);
+    for (int e = t; e < MM; e += 1024) {
+        const int i = e >> 7, j = e & 127;
+        const size_t g = (size_t)(x0 + i) * N + (y0 + j);
+        const float vv = ph[g], dxp = ph[g + N] - ph[g - N], dyp = ph[g + 1] - ph[g - 1];
+        const float ng = hw_sqrt(inv4 * (dxp * dxp + dyp * dyp));
+        const int idx = turnover_label(vv, lum[g], ecm[g], ph1[g], ph2[g]);
+        lab[e] = (unsigned char)idx;
+        float pxx, pxy, pyv;
+        active_stress(P, idx, rh[g], ng, dxp, dyp, inv4, pxx, pxy, pyv);
+        sm[i * ORG_LS + j] = make_float2(pxx, pxy);
+        SB[e] = pyv;
+    }
+    __syncthreads();
+    /* F(pi_xx + i pi_xy) to SA; pi_yy into the plane for the next transform */
+    fft2d_128<-1>(sm, tw, [&](const float2 *u, const int sub, const int j) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int i = fft_out_pos(sub, q);
+            SA[i * ORG_MB + j] = u[q];
+            sm[i * ORG_LS + j] = make_float2(SB[i * ORG_MB + j], 0.0f);
+        }
+    });
+    fft2d_128<-1>(sm, tw);
+    __syncthreads();
+    for (int e = t; e < MM; e += 1024) SY[e] = sm[(e >> 7) * ORG_LS + (e & 127)];
+    __syncthreads();
+    const float *vx = v + (size_t)n * 2 * MM, *vy = vx + MM;
+    for (int e = t; e < MM; e += 1024) sm[(e >> 7) * ORG_LS + (e & 127)] = make_float2(vx[e], vy[e]);
+    __syncthreads();
+    fft2d_128<-1>(sm, tw);
+    __syncthreads();
+    double err = 0.0, nrm = 0.0;
+    auto split = [](float2 a, float2 am, float2 &re_part, float2 &im_part) {
+        /* X = F(p + i q) at k and -k -> F(p)(k), F(q)(k) */
+        re_part = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+        im_part = make_float2(0.5f * (a.y + am.y), 0.5f * (am.x - a.x));
+    };
+    for (int e = t; e < MM; e += 1024) {
+        const int i = e >> 7, j = e & 127, im = (ORG_MB - i) & 127, jm = (ORG_MB - j) & 127;
+        const int em = im * ORG_MB + jm;
+        float2 pxx, pxy, pxxm, pxym, cx, cy;
+        split(SA[e], SA[em], pxx, pxy);
+        split(SA[em], SA[e], pxxm, pxym);
+        split(sm[i * ORG_LS + j], sm[im * ORG_LS + jm], cx, cy);
+        /* each mode with its own wave numbers (rho_functions.cpp:159-165) */
+        const int ir = (i < ORG_MB / 2) ? i : i - ORG_MB, jr = (j < ORG_MB / 2) ? j : j - ORG_MB;
+        const int irm = (im < ORG_MB / 2) ? im : im - ORG_MB, jrm = (jm < ORG_MB / 2) ? jm : jm - ORG_MB;
+        float2 vxk, vyk, vxm, vym;
+        velocity_mode(by_label(P.visc, lab[e]), P.friction, kscale * ir, kscale * jr, pxx, pxy, SY[e], vxk, vyk);
+        velocity_mode(by_label(P.visc, lab[em]), P.friction, kscale * irm, kscale * jrm, pxxm, pxym, SY[em], vxm,
+                      vym);
+        /* the spectrum of the exact solve's real velocity: H v(k) = (v(k) +
+         * conj v(-k)) / 2 (its inverse transform / M^2 is the velocity on the
+         * box, rho_functions.cpp:279) */
+        const float2 hx = make_float2(0.5f * (vxk.x + vxm.x), 0.5f * (vxk.y - vxm.y));
+        const float2 hy = make_float2(0.5f * (vyk.x + vym.x), 0.5f * (vyk.y - vym.y));
+        const double dxr = (double)cx.x - hx.x, dxi = (double)cx.y - hx.y;
+        const double dyr = (double)cy.x - hy.x, dyi = (double)cy.y - hy.y;
+        err += dxr * dxr + dxi * dxi + dyr * dyr + dyi * dyi;
+        nrm += (double)hx.x * hx.x + (double)hx.y * hx.y + (double)hy.x * hy.x + (double)hy.y * hy.y;
+    }
+    err = block_sum_d(err, red);
+    nrm = block_sum_d(nrm, red);
+    if (t == 0) {
+        const float q = nrm > 0.0 ? (float)sqrt(err / nrm) : (err > 0.0 ? INFINITY : 0.0f);
+        res[n] = q;
+        const int ok = q <= thr;
+        use[n] = ok;
+        if (!ok) atomicAdd(nfall, 1);
+    }
+}
+
 /* ======================================================================== */
 /* Phase field of one cell, one 1024-thread workgroup per cell:
  * update_all_cells (cell_phase_functions.cpp:126-217) and reshaping_cells

@@ -31,7 +31,8 @@ LIB_PATH = os.path.join(_HERE, "csrc", "liborganoid_hip.so")
 F_PHI, F_RHO, F_HFUNC, F_LUM, F_ECM, F_PSI, F_HLUM, F_HECM, F_LUMN, F_ECMN = range(10)
 W_PHIN, W_RHON, W_VX, W_VY = range(4)
 K_NAMES = ("centres", "prep", "psi", "lumecm", "lumreshape", "spectral", "cells", "commit", "step", "mid")
-OPT_WRITE_VELOCITY = 0
+OPT_WRITE_VELOCITY, OPT_SURROGATE_JUDGE = 0, 1
+JUDGE_SPECTRAL, JUDGE_FINITE_DIFF = 0, 1
 
 _lib = None
 

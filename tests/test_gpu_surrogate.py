@@ -2,7 +2,10 @@
 org_step_surrogate): the recorded training pairs are the exact step's own
 inputs and velocities, the judge's fallback and the every-k exact steps give
 the exact trajectory bit for bit, and a given velocity enters the cortex
-update in place of the solve's."""
+update in place of the solve's.  The spectral judge (the default) scores the
+exact solve's velocity at its transforms' rounding, a zero velocity at 1,
+twice the exact one at 1, and at the north star's threshold 1e-3 rejects a
+velocity 1 % off."""
 import numpy as np
 import pytest
 
@@ -44,8 +47,34 @@ def test_samples_are_the_exact_steps_inputs_and_velocities():
     for name, f in FIELDS:
         assert np.array_equal(g1.get_field(f), g2.get_field(f)), name
     rr = r.cpu().numpy()
-    print(f"  judge residual of the exact velocity: {np.array2string(rr, precision=4)}", flush=True)
-    assert np.isfinite(rr).all() and (rr > 0).all() and (rr < 1).all()
+    print(f"  spectral judge residual of the exact velocity: {np.array2string(rr, precision=4)}", flush=True)
+    assert np.isfinite(rr).all() and (rr < 1e-4).all()
+    g3 = _org(wl)
+    g3.set_option(host.OPT_SURROGATE_JUDGE, host.JUDGE_FINITE_DIFF)
+    _, _, r3 = g3.surrogate_samples(4, residuals=True)
+    r3 = r3.cpu().numpy()
+    print(f"  finite-difference judge residual of the exact velocity: {np.array2string(r3, precision=4)}", flush=True)
+    assert np.isfinite(r3).all() and (r3 > 0).all() and (r3 < 1).all()
+
+
+def test_spectral_judge_closed_forms_and_threshold():
+    """Given the exact velocity v: the judge scores v at the rounding of its
+    transforms, 2 v at 1 (|2 v - v| / |v|), and with the threshold 1e-3 it
+    accepts v and rejects 1.01 v (residual 0.01) in every cell."""
+    wl = make_workload("cyst6_256")
+    print("case cyst6_256: spectral judge on the exact velocity and multiples", flush=True)
+    gs = _org(wl)
+    _, v, r = gs.surrogate_samples(1, residuals=True)
+    nc = wl.params.ncell
+    res = {}
+    for scale in (1.0, 1.01, 2.0):
+        g = _org(wl)
+        exact, rr = g.step_surrogate(1, v_given=(v * scale).contiguous(), k_exact=1000, threshold=1e-3, it0=1)
+        res[scale] = (exact[0], rr[0])
+        print(f"  v x {scale}: residual {np.array2string(rr[0], precision=5)}, exact cells {exact[0]}", flush=True)
+    assert (res[1.0][1] < 1e-4).all() and res[1.0][0] == 0
+    assert np.allclose(res[1.01][1], 0.01, rtol=2e-3) and res[1.01][0] == nc
+    assert np.allclose(res[2.0][1], 1.0, rtol=1e-4) and res[2.0][0] == nc
 
 
 def test_fallback_and_exact_steps_give_the_exact_trajectory():
@@ -119,7 +148,7 @@ def test_trained_surrogate_ablation():
             return h
         NS = 30
         runs = {}
-        for name in ("exact", "fno", "zero"):
+        for name in ("exact", "fno", "fno_judged", "zero"):
             h = fresh()
             h.step(1, graph=False)
             h.sync()
@@ -129,6 +158,8 @@ def test_trained_surrogate_ablation():
                 ex = np.full(NS, nc)
             elif name == "fno":
                 ex, res = h.step_surrogate(NS, net=net, k_exact=10, threshold=1e30, it0=1)
+            elif name == "fno_judged":      # the north star's deployment: spectral judge at 1e-3
+                ex, res_j = h.step_surrogate(NS, net=net, k_exact=10, threshold=1e-3, it0=1)
             else:
                 zero = torch.zeros((nc, 2, wl.params.M, wl.params.M), dtype=torch.float32, device="cuda")
                 ex, _ = h.step_surrogate(NS, v_given=zero, k_exact=10, threshold=1e30, it0=1)
@@ -140,8 +171,8 @@ def test_trained_surrogate_ablation():
             return np.linalg.norm(h.get_field(f) - ref[f]) / np.linalg.norm(ref[f])
         print(f"  training loss {losses[0]:.3e} -> {np.mean(losses[-20:]):.3e}; validation rel L2 per cell "
               f"median {np.median(err):.3f}, 90% {np.percentile(err, 90):.3f}", flush=True)
-        print(f"  judge residual of exact velocities {np.median(floor.cpu().numpy()):.3f} (median), of the "
-              f"network's {np.median(res[res >= 0]):.3f}", flush=True)
+        print(f"  spectral judge residual of exact velocities {np.median(floor.cpu().numpy()):.2e} (median), of "
+              f"the network's {np.median(res[res >= 0]):.3f}", flush=True)
         for name, (h, t, ex) in runs.items():
             print(f"  {name:5s}: {1e3 * t:.3f} ms/step, exact cell solves {int(ex.sum())} of {NS * nc}, drift vs "
                   f"exact phi {drift(h, host.F_PHI):.3e} rho {drift(h, host.F_RHO):.3e}", flush=True)
@@ -153,5 +184,12 @@ def test_trained_surrogate_ablation():
             assert ex.sum() == 3 * nc
         d_fno, d_zero = drift(runs["fno"][0], host.F_RHO), drift(runs["zero"][0], host.F_RHO)
         assert np.isfinite(d_fno) and d_fno < d_zero
+        # at 1e-3 the judge sends every cell whose prediction is > 0.1 % off
+        # back to the exact solve: with this network, all of them
+        ex_j = runs["fno_judged"][2]
+        assert ex_j.sum() == (res_j > 1e-3).sum() + 3 * nc
+        if ex_j.sum() == NS * nc:
+            for f in (host.F_PHI, host.F_RHO):
+                assert np.array_equal(runs["fno_judged"][0].get_field(f), runs["exact"][0].get_field(f))
     finally:
         net.close()

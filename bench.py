@@ -106,6 +106,122 @@ def surrogate_rates(host, org, nc: int, steps: int) -> dict:
                     "every 10th step; untrained weights (timing only)"}
 
 
+def surrogate_spec(samples: int, train_steps: int, deploy_steps: int) -> int:
+    """The surrogate at the specification's scale (a separate run, not the
+    benchmark line): about `samples` per-cell training pairs (phi_n, rho_n) ->
+    (vx, vy) on 128^2 boxes from the exact solve of the uneven 8-organoid
+    sweep (steps 50.. of each member), training at batch 256 with Adam, the
+    network's error on held-out states of organoid64_1024 (steps 100..105),
+    then organoid64_1024 deployed for `deploy_steps` steps with the exact
+    solve every 10th step and the spectral judge at 1e-3, against the exact
+    steps of the same trajectory (and, at 10^4 steps, the FP64 oracle's
+    fields from tests/golden/drift_organoid64_1024.npz when present).
+    Prints one JSON line."""
+    import numpy as np
+    import torch
+
+    from multicellular_phase_field_active_gel_organoids_amd import fno, host
+    from multicellular_phase_field_active_gel_organoids_amd.configs import make_sweep, make_workload
+    t_all = time.perf_counter()
+    members = make_sweep()
+    ncells = sum(w.params.ncell for w in members)
+    per = max(1, -(-samples // ncells))
+    xs, vs = [], []
+    t0 = time.perf_counter()
+    for w in members:
+        g = host.Organoid(w.params, w.s0_cell)
+        g.load_workload(w)
+        g.step(50)
+        x, v = g.surrogate_samples(per, it0=50)
+        xs.append(x)
+        vs.append(v)
+        g.close()
+    x = torch.cat(xs)
+    v = torch.cat(vs)
+    del xs, vs
+    gen_s = time.perf_counter() - t0
+    print(f"bench: {x.shape[0]} training pairs in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    cfg = fno.FnoConfig(batch_max=256)
+    net = fno.FnoNet(cfg)
+    net.set_params(fno.init_params(cfg, 7))
+    sub = slice(None, None, max(1, x.shape[0] // 4096))
+    net.set_scales(*fno.fit_scales(x[sub].cpu().numpy(), v[sub].cpu().numpy()))
+    t0 = time.perf_counter()
+    losses = fno.train(net, x, v, steps=train_steps, batch=256, lr=2e-3, lr_final=1e-4,
+                       log=lambda m: print("bench: train " + m, file=sys.stderr, flush=True))
+    train_s = time.perf_counter() - t0
+    del x, v
+    wl = make_workload("organoid64_1024")
+    nc = wl.params.ncell
+    g = host.Organoid(wl.params, wl.s0_cell)
+    g.load_workload(wl)
+    g.step(100)
+    xv, vv = g.surrogate_samples(6, it0=100)
+    g.close()
+    pred = torch.cat([net.forward(xv[b:b + nc].contiguous()) for b in range(0, xv.shape[0], nc)])
+    err = ((pred - vv).double().flatten(1).norm(dim=1) / vv.double().flatten(1).norm(dim=1)).cpu().numpy()
+    del xv, vv, pred
+    print(f"bench: held-out error median {np.median(err):.4f}", file=sys.stderr, flush=True)
+
+    # deployment on organoid64_1024 from its initial state: exact every 10th
+    # step, the spectral judge at 1e-3, against the exact trajectory
+    ex_org = host.Organoid(wl.params, wl.s0_cell)
+    ex_org.load_workload(wl)
+    ex_org.set_option(host.OPT_WRITE_VELOCITY, 0)
+    t0 = time.perf_counter()
+    ex_org.step(deploy_steps, graph=True)
+    ex_org.sync()
+    exact_s = time.perf_counter() - t0
+    su_org = host.Organoid(wl.params, wl.s0_cell)
+    su_org.load_workload(wl)
+    chunk = 1000
+    exact_cells = 0
+    rejected = 0
+    judged = 0
+    t0 = time.perf_counter()
+    for it0 in range(0, deploy_steps, chunk):
+        k = min(chunk, deploy_steps - it0)
+        ex, res = su_org.step_surrogate(k, net=net, k_exact=10, threshold=1e-3, it0=it0)
+        exact_cells += int(ex.sum())
+        judged += int((res >= 0).sum())
+        rejected += int((res > 1e-3).sum())
+        print(f"bench: deployment step {it0 + k}: exact cell solves {exact_cells}", file=sys.stderr, flush=True)
+    su_org.sync()
+    sur_s = time.perf_counter() - t0
+    net.close()
+    fields = {}
+    for name, f in (("phi", host.F_PHI), ("rho", host.F_RHO)):
+        a = su_org.get_field(f).astype(np.float64)
+        b_ = ex_org.get_field(f).astype(np.float64)
+        fields[name] = {"vs_exact_gpu": float(np.linalg.norm(a - b_) / np.linalg.norm(b_))}
+        fx_path = os.path.join(ROOT, "tests", "golden", "drift_organoid64_1024.npz")
+        key = f"s{deploy_steps}_{name}"
+        if os.path.exists(fx_path):
+            fx = np.load(fx_path)
+            if key + "_box" in fx:
+                want = np.array(getattr(wl, name), dtype=np.float64, copy=True)
+                for n, (i0, i1, j0, j1) in enumerate(fx[key + "_box"]):
+                    want[n, i0:i1, j0:j1] = fx[f"{key}_{n}"]
+                fields[name]["vs_fp64_oracle"] = float(np.linalg.norm(a - want) / np.linalg.norm(want))
+                fields[name]["exact_gpu_vs_fp64_oracle"] = float(np.linalg.norm(b_ - want) / np.linalg.norm(want))
+    ex_org.close()
+    su_org.close()
+    out = {"samples": int(per * ncells), "generation_s": round(gen_s, 1), "train_steps": train_steps, "batch": 256,
+           "train_s": round(train_s, 1), "loss_first": round(float(losses[0]), 5),
+           "loss_last_50": round(float(np.mean(losses[-50:])), 6),
+           "heldout": "organoid64_1024 steps 100..105 (384 cells)",
+           "heldout_rel_l2_median": round(float(np.median(err)), 5),
+           "heldout_rel_l2_p90": round(float(np.percentile(err, 90)), 5),
+           "deployment": {"workload": wl.name, "steps": deploy_steps, "k_exact": 10, "judge": "spectral",
+                          "threshold": 1e-3, "cell_steps": nc * deploy_steps, "exact_cell_solves": exact_cells,
+                          "judged": judged, "rejected_by_judge": rejected,
+                          "ms_per_step": round(1e3 * sur_s / deploy_steps, 4),
+                          "exact_ms_per_step": round(1e3 * exact_s / deploy_steps, 4), "fields": fields}}
+    print(json.dumps({"surrogate_spec": out}), flush=True)
+    print(f"bench: surrogate at spec {time.perf_counter() - t_all:.1f} s", file=sys.stderr, flush=True)
+    return 0
+
+
 def large_rates(host, make_workload, steps: int) -> dict:
     """organoid256_2048 (256 cells, 2048^2) in the batched form (8-step
     graphs, default launch shape) and in the plain per-function form."""
@@ -320,6 +436,11 @@ def main() -> int:
                          "full-plane path (0: skip)")
     ap.add_argument("--fp64-steps", type=int, default=20,
                     help="steps of the bench workload in the FP64 form (0: skip)")
+    ap.add_argument("--surrogate-spec", type=int, default=0, metavar="SAMPLES",
+                    help="only run the surrogate at the specification's scale: SAMPLES training pairs (e.g. 200000), "
+                         "batch 256, deployment on organoid64_1024 (see --spec-train-steps, --spec-deploy-steps)")
+    ap.add_argument("--spec-train-steps", type=int, default=800)
+    ap.add_argument("--spec-deploy-steps", type=int, default=10000)
     ap.add_argument("--fno-forward", type=int, default=0, metavar="N",
                     help="only run N forward passes of the FNO surrogate on 64 random crops (profiling)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -329,6 +450,8 @@ def main() -> int:
         return calib_copy(args.calib_copy)
     if args.fno_forward:
         return fno_forward_only(args.fno_forward)
+    if args.surrogate_spec:
+        return surrogate_spec(args.surrogate_spec, args.spec_train_steps, args.spec_deploy_steps)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), --gpus must be 1"}))

@@ -219,9 +219,27 @@ void update_all_cells(int, org_real_t *, org_real_t *, org_real_t *, org_real_t 
     chk(org_update_all_cells(ctx()), "update_all_cells");
 }
 
-void reshaping_cells(int, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, int *, int *) {
-    if (g_phin_is_phi) chk(org_reshaping_cells_from_phi(ctx()), "reshaping_cells");
-    else chk(org_reshaping_cells(ctx()), "reshaping_cells");
+void reshaping_cells(int it_, org_real_t *, org_real_t *, org_real_t *, org_real_t *, org_real_t *, int *, int *) {
+    org_ctx *c = ctx();
+    if (g_phin_is_phi) chk(org_reshaping_cells_from_phi(c), "reshaping_cells");
+    else chk(org_reshaping_cells(c), "reshaping_cells");
+    /* the reference's diagnostics every ndisplay steps: each cell's window
+     * bounds and its reshaping count (cell_phase_functions.cpp:463-471,
+     * 575-578) */
+    if (ndisplay > 0 && it_ % ndisplay == 0) {
+        std::vector<int32_t> xc(Ntot), yc(Ntot), cnt(Ntot);
+        int32_t l = 0, e = 0;
+        chk(org_get_centres(c, xc.data(), yc.data()), "reshaping_cells diagnostics");
+        chk(org_get_iterations(c, cnt.data(), &l, &e), "reshaping_cells diagnostics");
+        for (int q = 0; q < Ntot; q++) {
+            fprintf(stderr, "n = %d \n \n", q);
+            fprintf(stderr, "xcenters[%d]-%d = %d  \n", q, zone_size, xc[q] - zone_size);
+            fprintf(stderr, "xcenters[%d]+%d = %d  \n", q, zone_size, xc[q] + zone_size);
+            fprintf(stderr, "ycenters[%d]-%d = %d  \n", q, zone_size, yc[q] - zone_size);
+            fprintf(stderr, "ycenters[%d]+%d = %d  \n", q, zone_size, yc[q] + zone_size);
+            fprintf(stderr, "count_ri = %d \n \n", cnt[q]);
+        }
+    }
 }
 
 /* cell_phase_functions.cpp:42-62 */

@@ -104,7 +104,17 @@ def test_driver_order_program_matches_batched_step(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     g = host.Organoid(wl.params)
     g.load_workload(wl)
-    g.step(3)
+    g.step(1)
+    # the reference's diagnostics at step 0 (ndisplay = 5): window bounds and
+    # reshaping counts per cell (cell_phase_functions.cpp:463-471, 575-578)
+    xc, yc = g.centres()
+    cells, _, _ = g.iterations()
+    zs = wl.params.zone_size
+    printed = [int(v) for v in re.findall(r"count_ri = (\d+)", r.stderr)]
+    assert printed == [int(c) for c in cells], (printed, cells)
+    for n in range(wl.params.ncell):
+        assert f"xcenters[{n}]-{zs} = {xc[n] - zs}" in r.stderr and f"ycenters[{n}]+{zs} = {yc[n] + zs}" in r.stderr
+    g.step(2)
     nc, N = wl.params.ncell, wl.params.N
     for name, f, shape in (("phi", host.F_PHI, (nc, N, N)), ("rho", host.F_RHO, (nc, N, N)),
                            ("lum", host.F_LUM, (N, N)), ("ecm", host.F_ECM, (N, N))):

@@ -49,8 +49,8 @@ struct org_ctx {
     hipStream_t stream = nullptr;
     gen::Generic *gen = nullptr;         /* the full-plane path (FP64, or FFT box != 128); null: LDS kernels */
     double *part_lc = nullptr;           /* lumen/ECM loop partial deviations [2][2][ORG_LUM_SPAN][ORG_LUM_MAX] */
-    float *xch = nullptr;                /* rows exchanged by k_step's LDS loops [2][2][N][N] */
-    unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier: count, generation */
+    float *xch = nullptr;                /* tagged words exchanged by k_step's LDS loops (ORG_XCH_WORDS) */
+    unsigned *lum_bar = nullptr;         /* lumen/ECM workgroups' barrier count; tag base of the LDS loops' exchanges */
     int *lum_abort = nullptr;
     long long wait_limit = 0;            /* k_step's wait timeout, wall-clock ticks */
     float *phi_win = nullptr;
@@ -609,7 +609,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->vybox, (size_t)c->nc * c->M * c->M);
     A(&c->jwin, (size_t)c->nc * 2 * (c->W + 2) * (c->W + 2));
     A(&c->spec, (size_t)c->nc * 2 * c->M * c->M);
-    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_SPAN * ORG_LUM_MAX); A(&c->xch, 4 * c->NN); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
+    A(&c->uni, 4); A(&c->jbox, 2 * 4 * 1024); A(&c->part_lc, 4 * ORG_LUM_SPAN * ORG_LUM_MAX); A(&c->xch, 2 * ORG_XCH_WORDS(c->NN)); A(&c->lum_bar, 2); A(&c->lum_abort, 1);
     A(&c->phi_win, (size_t)c->nc * c->W * c->W);
     A(&c->xc, c->nc); A(&c->yc, c->nc); A(&c->box, 4 * (size_t)c->nc);
     A(&c->xn, c->nc); A(&c->yn, c->nc); A(&c->bn, 4 * (size_t)c->nc);
@@ -641,6 +641,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         hipMemset(c->hecm, 0, sizeof(float) * c->NN) != hipSuccess ||
         hipMemset(c->sum_ecm, 0, sizeof(double)) != hipSuccess ||
         hipMemset(c->lum_bar, 0, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(c->lum_abort, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(c->xch, 0, sizeof(unsigned long long) * ORG_XCH_WORDS(c->NN)) != hipSuccess ||
         hipMemset(c->lbrk_par, 0, 2 * sizeof(int)) != hipSuccess ||
         /* the loops read normals one pixel outside the live box, where none
          * is ever computed in box mode: they must be zero */

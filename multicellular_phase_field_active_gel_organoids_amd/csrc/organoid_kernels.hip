@@ -189,9 +189,30 @@ static __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdg
 static __device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504089f); }
 
 /* ---- block reductions (blockDim multiple of 64) --------------------------- */
+/* v of another lane of the same row of 16 (DPP control CTRL), by halves */
+template <int CTRL>
+static __device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+static __device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+/* Sum over the 64 lanes of a wave (every lane active), the same value in
+ * every lane and a fixed order: pairs, quads, halves and whole rows of 16
+ * through DPP lane moves (each step adds a lane's value to its partner's,
+ * and a + b == b + a, so all lanes of a row hold the same row sum), then
+ * (row0 + row1) + (row2 + row3).  No LDS traffic. */
 static __device__ __forceinline__ double wave_sum_d(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
+    v += dpp_d<0xB1>(v);      /* quad_perm [1, 0, 3, 2] */
+    v += dpp_d<0x4E>(v);      /* quad_perm [2, 3, 0, 1] */
+    v += dpp_d<0x141>(v);     /* row_half_mirror */
+    v += dpp_d<0x140>(v);     /* row_mirror */
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 static __device__ __forceinline__ long long wave_sum_ll(long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -1558,17 +1579,22 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
  *     the neighbours' rows only once, at its start.  After a span each
  *     workgroup publishes its first and last SPAN rows (at their grid
  *     coordinates, in the exchange plane of the span's parity) and the
- *     deviation of its own rows for each sweep; after the barrier it reads
- *     the neighbours' rows into its halo and takes the sweeps' stop tests in
- *     order (cell_phase_functions.cpp:774, 862-868).  Sweeps after the one
- *     that stops the loop were speculative and are dropped; the stopping
- *     sweep's iterate is still in LDS (its own buffer, or a copy of the
- *     band's rows for the sweeps the span overwrites later).
- *   - The exchanged values go through relaxed agent-scope atomic stores and
- *     loads (coherent across XCDs by themselves) and the arrival counter is
- *     a relaxed atomic issued once every thread's stores have completed
- *     (lum_barrier_light): no L2 write-back or invalidation per span, so the
- *     other roles' cached data on the XCD stay valid.
+ *     deviation of its own rows for each sweep; then, with no barrier, it
+ *     waits for the neighbours' rows and every workgroup's deviations of
+ *     the span, puts the rows into its halo and takes the sweeps' stop
+ *     tests in order (cell_phase_functions.cpp:774, 862-868).  Sweeps after
+ *     the one that stops the loop were speculative and are dropped; the
+ *     stopping sweep's iterate is still in LDS (its own buffer, or a copy of
+ *     the band's rows for the sweeps the span overwrites later).
+ *   - Every exchanged value travels in a 64-bit word with the tag of its
+ *     exchange (xput / xget / xwait), through relaxed agent-scope accesses
+ *     that are coherent across XCDs by themselves: a reader polls the words
+ *     it needs until they carry the span's tag, so a span's exchange costs
+ *     one store-to-load latency, with no wait for store completion, no
+ *     arrival counter and no L2 write-back or invalidation (the other
+ *     roles' cached data on the XCD stay valid).  Two planes suffice: a
+ *     workgroup overwrites span k's words only in span k + 2, after every
+ *     reader has consumed them (it needed their span k + 1 words first).
  *   - At the end the band's last iterate goes to the buffer `cur` (the
  *     other buffer is rewritten before any read by the next step's update),
  *     workgroup 0 writes the four corners of both buffers (the ring writes
@@ -1581,6 +1607,8 @@ static __device__ bool lum_loop_body(const DevP &P, int blk, int nblk, double *r
 #ifndef ORG_LUM_SPAN
 #define ORG_LUM_SPAN 2             /* sweeps per exchange (even) */
 #endif
+/* 64-bit words of the LDS loops' exchange buffer (rows and partial sums) */
+#define ORG_XCH_WORDS(NN) (4 * (size_t)(NN) + 2 * 2 * ORG_LUM_SPAN * 2 * ORG_LUM_MAX)
 /* LDS of the loop workgroup: [0, 128) red, [128, 160) the prologue's boxes,
  * [256, 1280) per-wave deviations, the bands from ORG_LUM_LDS_BASE */
 #define ORG_LUM_LDS_BASE 2048
@@ -1611,45 +1639,34 @@ static __device__ __forceinline__ bool lum_lds_fits(const int4 BL, const int4 BE
     return ORG_LUM_LDS_BASE + lum_band_bytes(BL, nblk) + lum_band_bytes(BE, nblk) <= lds_bytes;
 }
 
-/* Barrier among the loop workgroups without release or acquire: every
- * thread waits for its own memory operations to complete (the exchanged
- * values were written with agent-scope atomic stores, which are then
- * visible at agent scope), then thread 0 arrives with a relaxed atomic and
- * polls as lum_barrier does, with the same time limit and abort flag. */
-static __device__ bool lum_barrier_light(unsigned *bar, int nblk, int &epoch, int *abort_flag, int *err,
-                                         long long limit) {
-    __shared__ int ok_l;
-    epoch++;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int good = 1;
-        const unsigned target = (unsigned)epoch * (unsigned)nblk;
-        unsigned seen = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        const long long t0 = wall_clock64();
-        while (seen < target) {
-            __builtin_amdgcn_s_sleep(1);
-            seen = __hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (seen >= target) break;
-            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                wall_clock64() - t0 > limit) {
-                atomicExch(abort_flag, 1);
-                set_err(err, ORG_ETIMEOUT);
-                good = 0;
-                break;
-            }
-        }
-        ok_l = good;
+/* Tagged exchange words of the LDS loops: 32 bits of payload in the low
+ * half, the tag of the exchange in the high half, written and read as one
+ * 64-bit agent-scope access (single-copy atomic, coherent across XCDs).  A
+ * reader that finds its tag holds the payload of that exchange, so no
+ * barrier and no ordering between words is needed.  Tags grow by one per
+ * exchange of a context and never repeat within 2^32 exchanges; the
+ * buffer is zeroed at creation and no exchange uses tag 0. */
+static __device__ __forceinline__ void xput(unsigned long long *p, unsigned v, unsigned tag) {
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | (unsigned long long)v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ unsigned long long xget(const unsigned long long *p) {
+    return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+/* Poll *p until it carries tag (w: the word as first read, updated); false
+ * once the wait passes `limit` wall-clock ticks or the abort flag is up. */
+static __device__ bool xwait(const unsigned long long *p, unsigned tag, unsigned long long &w, int *abort_flag,
+                             long long limit) {
+    if ((unsigned)(w >> 32) == tag) return true;
+    const long long t0 = wall_clock64();
+    for (int n = 1;; n++) {
+        __builtin_amdgcn_s_sleep(1);
+        w = xget(p);
+        if ((unsigned)(w >> 32) == tag) return true;
+        if ((n & 15) == 0 && (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                              wall_clock64() - t0 > limit))
+            return false;
     }
-    __syncthreads();
-    return ok_l != 0;
-}
-
-static __device__ __forceinline__ void xstore(float *p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-static __device__ __forceinline__ float xload(const float *p) {
-    return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The normal of one pixel (as normals_row and lumecm_normals_box): |grad|
@@ -2628,7 +2645,7 @@ struct StepArgs {
     int *jbox_part;
     LumState *lst;
     double *part_lc;         /* loop deviations: [2 spans][2 fields][ORG_LUM_SPAN sweeps][ORG_LUM_MAX] */
-    float *xch;              /* exchange planes of the LDS loops: [2 fields][2 span parities][N][N] */
+    float *xch;              /* tagged exchange words of the LDS loops (ORG_XCH_WORDS, 64-bit) */
     /* cortex */
     float2 *spec;
     float *rho_box, *vxbox, *vybox;
@@ -2846,10 +2863,18 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
     float cval[2][2] = {{0.0f, 0.0f}, {1.0f, 1.0f}};
     if (corner_thread) ce = (size_t)((threadIdx.x & 1) ? N - 1 : 0) * N + ((threadIdx.x & 2) ? N - 1 : 0);
     int stop_t[2] = {K - 1, K - 1}, span_par[2] = {0, 0};
+    /* the exchange words: [2 fields][2 span parities][N][N] rows, then
+     * [2 span parities][2K deviations][2 halves][ORG_LUM_MAX] partial sums */
+    unsigned long long *xrow = (unsigned long long *)A.xch;
+    unsigned long long *xdev = xrow + 4 * NN;
+    __shared__ int xfail;
+    const unsigned tag0 = __hip_atomic_load(&A.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int k = 0;
     __syncthreads();
-    for (int k = 0; !(s[0].done && s[1].done); k++) {
+    for (; !(s[0].done && s[1].done); k++) {
         const int xp = k & 1;
-        double *pk = A.part_lc + (size_t)xp * 2 * K * nblk;    /* [f][t][nblk] of this span */
+        const unsigned tag = tag0 + (unsigned)k + 1u;
+        unsigned long long *pk = xdev + (size_t)xp * 2 * K * 2 * ORG_LUM_MAX;
         double acc[2][K];
 #pragma unroll
         for (int f = 0; f < 2; f++)
@@ -2884,48 +2909,73 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
                 }
             }
             if (L.b > L.a) {
-                /* the latest iterate's first K and last K rows */
+                /* the latest iterate's first K and last K rows, tagged */
                 const float *lat = par ? L.u[1] : L.u[0];
                 const int own = L.b - L.a, nr = min(2 * K, own), base = L.a - K - 1;
-                float *x = A.xch + (size_t)(f * 2 + xp) * NN;
+                unsigned long long *x = xrow + (size_t)(f * 2 + xp) * NN;
                 for (int q = threadIdx.x; q < nr * L.bw; q += blockDim.x) {
                     const int r = q / L.bw, j = q % L.bw;
                     const int gi = (r < K || own <= 2 * K) ? L.a + r : L.b - (nr - r);
-                    xstore(&x[(size_t)gi * N + L.c0 + j], lat[(gi - base) * L.S + 1 + j]);
+                    xput(&x[(size_t)gi * N + L.c0 + j], __float_as_uint(lat[(gi - base) * L.S + 1 + j]), tag);
                 }
             }
         }
-        /* the 2K deviations of this workgroup, one fixed-order block sum */
+        /* the 2K deviations of this workgroup, one fixed-order block sum,
+         * published as tagged halves */
         {
             const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
             double *wred = (double *)(lds + 256);
             double v[2 * K];
 #pragma unroll
-            for (int i = 0; i < 2 * K; i++) v[i] = wave_sum_d(acc[i / K][i % K]);
+            for (int i = 0; i < 2 * K; i++) v[i] = s[i / K].done ? 0.0 : wave_sum_d(acc[i / K][i % K]);
             if (lane == 0) {
 #pragma unroll
                 for (int i = 0; i < 2 * K; i++) wred[2 * K * w + i] = v[i];
             }
+            if (threadIdx.x == 0) xfail = 0;
             __syncthreads();
             if (threadIdx.x < 2 * K) {
                 double t = 0.0;
                 for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[2 * K * ww + threadIdx.x];
-                __hip_atomic_store(&pk[threadIdx.x * nblk + blk], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+                xput(&pk[(threadIdx.x * 2 + 0) * ORG_LUM_MAX + blk], (unsigned)tb, tag);
+                xput(&pk[(threadIdx.x * 2 + 1) * ORG_LUM_MAX + blk], (unsigned)(tb >> 32), tag);
             }
         }
-        if (!lum_barrier_light(A.bar, nblk, epoch, A.abort_flag, A.err, A.limit)) return false;
-        /* wave 0: the span's deviations summed over the workgroups in a fixed
-         * order; the other waves: the neighbours' rows into the halo */
+        /* no barrier: wave 0 waits for every workgroup's deviations of this
+         * span and sums them in a fixed order; the other waves wait for the
+         * neighbours' rows and put them into the halo.  Every word carries
+         * the tag of the exchange it belongs to. */
+        bool ok = true;
         if (threadIdx.x < 64) {
-            double v[2 * K];
+            double x[2 * K][2], v[2 * K];
 #pragma unroll
             for (int i = 0; i < 2 * K; i++) {
-                double x = 0.0;
-                if (!s[i / K].done)
-                    for (int b2 = threadIdx.x; b2 < nblk; b2 += 64)
-                        x += __hip_atomic_load(&pk[i * nblk + b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                v[i] = wave_sum_d(x);
+                unsigned long long wv[2][2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int b2 = (int)threadIdx.x + 64 * h;
+                    const bool want = !s[i / K].done && b2 < nblk;
+#pragma unroll
+                    for (int half = 0; half < 2; half++)
+                        wv[h][half] = want ? xget(&pk[(i * 2 + half) * ORG_LUM_MAX + b2]) : 0ull;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int b2 = (int)threadIdx.x + 64 * h;
+                    x[i][h] = 0.0;
+                    if (!s[i / K].done && b2 < nblk) {
+#pragma unroll
+                        for (int half = 0; half < 2; half++)
+                            ok = ok && xwait(&pk[(i * 2 + half) * ORG_LUM_MAX + b2], tag, wv[h][half], A.abort_flag,
+                                             A.limit);
+                        x[i][h] = __longlong_as_double((long long)(((wv[h][1] & 0xffffffffull) << 32) |
+                                                                   (wv[h][0] & 0xffffffffull)));
+                    }
+                }
             }
+#pragma unroll
+            for (int i = 0; i < 2 * K; i++) v[i] = s[i / K].done ? 0.0 : wave_sum_d(x[i][0] + x[i][1]);
             if (threadIdx.x == 0) {
 #pragma unroll
                 for (int i = 0; i < 2 * K; i++) red[i] = v[i];
@@ -2936,17 +2986,48 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
                 const LumBand &L = B[f];
                 if (s[f].done || !(L.b > L.a)) continue;
                 float *lat = (s[f].ri & 1) ? L.u[1] : L.u[0];
-                const float *x = A.xch + (size_t)(f * 2 + xp) * NN;
-                const int base = L.a - K - 1;
-                for (int q = (int)threadIdx.x - 64; q < 2 * K * L.bw; q += (int)blockDim.x - 64) {
-                    const int r = q / L.bw, j = q % L.bw;
-                    const int gi = r < K ? L.a - K + r : L.b + (r - K);
-                    if (gi >= L.r0 && gi <= L.r1)
-                        lat[(gi - base) * L.S + 1 + j] = xload(&x[(size_t)gi * N + L.c0 + j]);
+                const unsigned long long *x = xrow + (size_t)(f * 2 + xp) * NN;
+                const int base = L.a - K - 1, step = (int)blockDim.x - 64, nq = 2 * K * L.bw;
+                /* four words per thread in flight, then the waits */
+                for (int q0 = (int)threadIdx.x - 64; q0 < nq; q0 += 4 * step) {
+                    unsigned long long wv[4];
+                    size_t ga[4];
+                    int la[4];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const int q = q0 + m * step;
+                        la[m] = -1; ga[m] = 0; wv[m] = 0ull;
+                        if (q < nq) {
+                            const int r = q / L.bw, j = q % L.bw;
+                            const int gi = r < K ? L.a - K + r : L.b + (r - K);
+                            if (gi >= L.r0 && gi <= L.r1) {
+                                la[m] = (gi - base) * L.S + 1 + j;
+                                ga[m] = (size_t)gi * N + L.c0 + j;
+                                wv[m] = xget(&x[ga[m]]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        if (la[m] < 0) continue;
+                        ok = ok && xwait(&x[ga[m]], tag, wv[m], A.abort_flag, A.limit);
+                        lat[la[m]] = __uint_as_float((unsigned)wv[m]);
+                    }
                 }
             }
         }
+        if (!ok) xfail = 1;
         __syncthreads();
+        if (xfail) {
+            /* a wait passed its limit or another workgroup gave up: stop,
+             * and move the tags of later steps past every one used here */
+            if (threadIdx.x == 0) {
+                atomicExch(A.abort_flag, 1);
+                set_err(A.err, ORG_ETIMEOUT);
+                atomicAdd(&A.bar[1], 1u << 20);
+            }
+            return false;
+        }
 #pragma unroll
         for (int f = 0; f < 2; f++) {
             if (s[f].done) continue;
@@ -2958,6 +3039,8 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
         }
         __syncthreads();      /* red is rewritten by the next span */
     }
+    /* the next step's exchanges use tags after this step's */
+    if (blk == 0 && threadIdx.x == 0) atomicAdd(&A.bar[1], (unsigned)k + 1u);
     /* the band's last iterate into the buffer that holds it; the corners */
 #pragma unroll
     for (int f = 0; f < 2; f++) {

@@ -202,6 +202,11 @@ static __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
+static __device__ __forceinline__ double readfirstlane_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 /* Sum over the 64 lanes of a wave (every lane active), the same value in
  * every lane and a fixed order: pairs, quads, halves and whole rows of 16
  * through DPP lane moves (each step adds a lane's value to its partner's,
@@ -3033,7 +3038,7 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
             if (s[f].done) continue;
 #pragma unroll
             for (int t = 0; t < K; t++) {
-                lum_stop(P, s[f], red[f * K + t], A.err);
+                lum_stop(P, s[f], readfirstlane_d(red[f * K + t]), A.err);
                 if (s[f].done) { stop_t[f] = t; break; }
             }
         }
@@ -3063,7 +3068,7 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
             g[(size_t)(L.a + q.r) * N + L.c0 + q.c] = v;
         }
     }
-    if (blk == 0 && threadIdx.x < 2) A.lst[threadIdx.x] = s[threadIdx.x];
+    if (blk == 0 && threadIdx.x == 0) { A.lst[0] = s[0]; A.lst[1] = s[1]; }   /* s stays in registers */
     cur_out[0] = s[0].cur;
     cur_out[1] = s[1].cur;
     return lum_barrier(A.bar, nblk, epoch, A.abort_flag, A.err, A.limit);
@@ -3084,7 +3089,11 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
         hv += __hip_atomic_load(&a.part_h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     merged_live_both(P, a, sb);
     const double se = block_sum_d(hv, red);      /* its barriers publish sb */
-    const int4 BL = make_int4(sb[0], sb[1], sb[2], sb[3]), BE = make_int4(sb[4], sb[5], sb[6], sb[7]);
+    /* the boxes are the same in every thread: readfirstlane keeps them, and
+     * everything the loops derive from them, in scalar registers */
+    auto u = [&](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    const int4 BL = make_int4(u(sb[0]), u(sb[1]), u(sb[2]), u(sb[3]));
+    const int4 BE = make_int4(u(sb[4]), u(sb[5]), u(sb[6]), u(sb[7]));
     __syncthreads();      /* sb is reused below */
     if (blk == 0) {
         if (threadIdx.x == 0) *a.sum_ecm = se;

@@ -488,6 +488,8 @@ def main() -> int:
         org.sync()
     gs = org.get_tuning()["graph_steps"]
     w0 = 2 * max(args.warmup, 1)
+    org.prepare_graph(args.steps)     # capture + instantiate the replayed graphs (set-up, state unchanged)
+    org.sync()
     t0 = time.perf_counter()
     org.step(args.steps, graph=True, it0=w0)
     org.sync()
@@ -497,7 +499,8 @@ def main() -> int:
     print(f"bench: batched {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
     form = (f"batched: k_step + k_mid per step for all cells (k_pre / k_post at the ends of a replay); "
             f"{args.steps // gs} hipGraph(s) of {gs} steps + {1 if args.steps % gs else 0} of "
-            f"{args.steps % gs} steps; exact velocity solve (FNO surrogate not used in the timed region; "
+            f"{args.steps % gs} steps, instantiated before the timed region; exact velocity solve (FNO surrogate "
+            f"not used in the timed region; "
             f"phi error not checked in it: see tests/test_gpu_drift.py)")
 
     # the same window replayed eagerly, one step at a time, on a second

@@ -222,6 +222,12 @@ int org_commit_lumecm(org_ctx *ctx);               /* ECM <- ECMn, lum <- lumn,
  * the device).  use_graph=1 captures the step into a hipGraph on first use
  * and replays it.                                                            */
 int org_step(org_ctx *ctx, int it0, int nsteps, int use_graph);
+/* Captures and instantiates the hipGraphs that org_step(ctx, ., nsteps, 1)
+ * replays, without running them (the state does not change), so that a
+ * later replay of that length starts at once.  No counterpart in the
+ * reference (like cufftPlan2d, set-up of launches to come).  A no-op on the
+ * host-driven full-plane path. */
+int org_graph_prepare(org_ctx *ctx, int nsteps);
 
 /* ---- launch tuning of the batched step ---------------------------------------
  * No counterpart in the reference (its loop bounds are fixed by N_s and the

@@ -1054,6 +1054,22 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     return harvest_device_error(c);
 }
 
+int org_graph_prepare(org_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return note_error(ORG_EINVAL);
+    if (c->gen) return ORG_OK;
+    TRY(need_prepared(c));
+    const int n = nsteps - (c->ring_pending && nsteps > 0 ? 1 : 0);   /* org_step runs that one eagerly */
+    const int full = n / c->graph_steps, rem = n % c->graph_steps;
+    const bool prof = c->prof;
+    c->prof = false;
+    auto enq = [c](int k) { return enqueue_steps(c, k); };
+    int rc = ORG_OK;
+    if (full) rc = ensure_graph(c->stream, c->graph[0], c->gexec[0], c->graph_len[0], c->graph_steps, enq);
+    if (!rc && rem) rc = ensure_graph(c->stream, c->graph[1], c->gexec[1], c->graph_len[1], rem, enq);
+    c->prof = prof;
+    return rc;
+}
+
 /* ---- velocity surrogate ----------------------------------------------------- */
 namespace {
 /* buffers of the surrogate for nsteps steps (per-step arrays regrown) */

@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.org_update_rho_to_shape.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     lib.org_relax_initial_state.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     lib.org_step.argtypes = [vp, i32, i32, i32]
+    lib.org_graph_prepare.argtypes = [vp, i32]
     lib.org_profile_enable.argtypes = [vp, i32]
     lib.org_kernel_time.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.org_set_option.argtypes = [vp, i32, i32]
@@ -387,6 +388,10 @@ class Organoid:
     # ---- batched form ---------------------------------------------------
     def step(self, nsteps: int = 1, graph: bool = False, it0: int = 0) -> None:
         _chk(self.lib.org_step(self.h, it0, nsteps, 1 if graph else 0), "org_step")
+
+    def prepare_graph(self, nsteps: int) -> None:
+        """Capture the hipGraphs a replay of nsteps steps uses, without running them."""
+        _chk(self.lib.org_graph_prepare(self.h, int(nsteps)), "org_graph_prepare")
 
     def sync(self) -> None:
         _chk(self.lib.org_sync(self.h), "org_sync")

@@ -89,3 +89,29 @@ def test_autotune_restores_state_and_caches(tmp_path):
     g2 = _org(wl)
     r2 = autotune.tuned(g2, cache_path=path)
     assert r2["source"] == "cache" and r2["tuning"] == r["tuning"] == g2.get_tuning()
+
+
+def test_graph_prepare_leaves_state_and_replays_identically():
+    """org_graph_prepare captures the graphs of an n-step replay without
+    running them: the state is unchanged (also right after org_prepare,
+    whose first step org_step runs eagerly), and the replay that follows
+    equals a replay without it bit for bit."""
+    wl = make_workload("two_cells_256")
+    print("case two_cells_256: org_graph_prepare then 20 + 13 graph steps", flush=True)
+    g, r = _org(wl), _org(wl)
+    t = g.get_tuning()
+    g.set_tuning(lum_blocks=t["lum_blocks"], roles=t["roles"], graph_steps=8)
+    r.set_tuning(lum_blocks=t["lum_blocks"], roles=t["roles"], graph_steps=8)
+    before = [g.get_field(f) for f in FIELDS + (host.F_HFUNC,)]
+    g.prepare_graph(20)
+    g.sync()
+    for f, a in zip(FIELDS + (host.F_HFUNC,), before):
+        assert np.array_equal(g.get_field(f), a), f
+    done = 0
+    for n in (20, 13):
+        g.prepare_graph(n)
+        g.step(n, graph=True, it0=done)
+        r.step(n, graph=True, it0=done)
+        done += n
+        for f in FIELDS + (host.F_HFUNC, host.F_PSI):
+            assert np.array_equal(g.get_field(f), r.get_field(f)), (n, f)

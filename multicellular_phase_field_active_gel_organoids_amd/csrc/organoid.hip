@@ -60,8 +60,8 @@ struct org_ctx {
     /* full planes */
     float *phi = nullptr, *rho = nullptr, *hfunc = nullptr;
     float *lum = nullptr, *ecm = nullptr, *psi = nullptr, *hlum = nullptr, *hecm = nullptr;
-    double *lbrk0 = nullptr, *lbrk1 = nullptr;   /* lumen/ECM part of the cell update's bracket (batched step) */
-    int *lbrk_par = nullptr;               /* [0] bracket buffer parity, [1] ring flag of k_mid */
+    double *lbrk = nullptr;                /* lumen/ECM part of the cell update's bracket (batched step) */
+    int *ringh = nullptr;                  /* ring flag of the h parts of k_mid */
     float *bufL0 = nullptr, *bufL1 = nullptr, *bufE0 = nullptr, *bufE1 = nullptr;
     float *jlx = nullptr, *jly = nullptr, *jex = nullptr, *jey = nullptr;
     /* window-local */
@@ -250,11 +250,11 @@ int L_centroid(org_ctx *c) {
  * k_live_init) */
 int L_live_init(org_ctx *c) {
     hipLaunchKernelGGL(k_live_reset, dim3(1), dim3(256), 0, c->stream, c->live, c->live_part, c->nlp, c->N,
-                       c->lbrk_par + 1);
+                       c->ringh);
     TRY(launch_check());
     hipLaunchKernelGGL(k_live_init, dim3(c->nparts_h), dim3(1024), 0, c->stream, c->dp, c->lum, c->ecm, c->hlum,
                        c->hecm, c->part_h, c->bufL0, c->bufL1, c->bufE0, c->bufE1, c->jlx, c->jly, c->jex, c->jey,
-                       c->live, c->lbrk0, c->lbrk1);
+                       c->live, c->lbrk);
     return launch_check();
 }
 /* centres and support boxes for the next batched step, from the current
@@ -334,7 +334,7 @@ StepArgs step_args(org_ctx *c) {
     a.xp = c->xp; a.yp = c->yp;
     a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
     a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
-    a.lbrk0 = c->lbrk0; a.lbrk1 = c->lbrk1; a.lbrk_par = c->lbrk_par; a.ringh = c->lbrk_par + 1;
+    a.lbrk = c->lbrk; a.ringh = c->ringh;
     a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nh = c->nh;
     a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp; a.nlum = c->nlum;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
@@ -600,7 +600,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     auto A = [&](auto **ptr, size_t n) { if (!rc) rc = dalloc(c, ptr, n); };
     A(&c->phi, c->P); A(&c->rho, c->P); A(&c->hfunc, c->P);
     A(&c->lum, c->NN); A(&c->ecm, c->NN); A(&c->psi, c->NN); A(&c->hlum, c->NN); A(&c->hecm, c->NN);
-    A(&c->lbrk0, c->NN); A(&c->lbrk1, c->NN); A(&c->lbrk_par, 2);
+    A(&c->lbrk, c->NN); A(&c->ringh, 1);
     A(&c->bufL0, c->NN); A(&c->bufL1, c->NN); A(&c->bufE0, c->NN); A(&c->bufE1, c->NN);
     A(&c->jlx, c->NN); A(&c->jly, c->NN); A(&c->jex, c->NN); A(&c->jey, c->NN);
     A(&c->phin_win, (size_t)c->nc * c->W * c->W);
@@ -642,7 +642,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
         hipMemset(c->sum_ecm, 0, sizeof(double)) != hipSuccess ||
         hipMemset(c->lum_bar, 0, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(c->lum_abort, 0, sizeof(int)) != hipSuccess ||
         hipMemset(c->xch, 0, sizeof(unsigned long long) * ORG_XCH_WORDS(c->NN)) != hipSuccess ||
-        hipMemset(c->lbrk_par, 0, 2 * sizeof(int)) != hipSuccess ||
+        hipMemset(c->ringh, 0, sizeof(int)) != hipSuccess ||
         /* the loops read normals one pixel outside the live box, where none
          * is ever computed in box mode: they must be zero */
         hipMemset(c->jlx, 0, sizeof(float) * c->NN) != hipSuccess || hipMemset(c->jly, 0, sizeof(float) * c->NN) != hipSuccess ||

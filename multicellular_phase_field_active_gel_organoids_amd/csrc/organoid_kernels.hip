@@ -1827,16 +1827,14 @@ __global__ __launch_bounds__(1024) void k_live_init(DevP P, const float *__restr
                                                     float *__restrict__ E0, float *__restrict__ E1,
                                                     float *__restrict__ jlx, float *__restrict__ jly,
                                                     float *__restrict__ jex, float *__restrict__ jey,
-                                                    int *__restrict__ live, double *__restrict__ lbrk0,
-                                                    double *__restrict__ lbrk1) {
+                                                    int *__restrict__ live, double *__restrict__ lbrk) {
     __shared__ double red[16];
     __shared__ int sb[8];
     __shared__ __attribute__((aligned(16))) float lds[ORG_LDS_LBRK / 4];
     const int k = blockIdx.x, N = P.N;
     const double tot = block_sum_d(hlum_part(P, k, lum, ecm, hlum, hecm), red);
     if (threadIdx.x == 0) part[k] = tot;
-    lbrk_part(P, k, lum, ecm, lbrk0, lds);
-    lbrk_part(P, k, lum, ecm, lbrk1, lds);
+    lbrk_part(P, k, lum, ecm, lbrk, lds);
     if (threadIdx.x < 8) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : N;
     __syncthreads();
     int bx[2][4] = {{N, -1, N, -1}, {N, -1, N, -1}};
@@ -1905,6 +1903,9 @@ static __device__ __forceinline__ void active_stress(const DevP &P, const int id
  * reference keeps only Re, rho_functions.cpp:279), F(rho v_x + i rho v_y),
  * and F^-1(H D).  The per-mode solve uses the viscosity of the real-space
  * pixel with the mode's indices (rho_functions.cpp:149-150, 170). */
+#ifndef ORG_P1_QUADS
+#define ORG_P1_QUADS 2      /* quads of the stress pass whose loads are issued together */
+#endif
 template <bool GIVEN>
 static __device__ __forceinline__ void spectral_body(const DevP &P, const int n, float2 *__restrict__ sm,
                                                      float2 *__restrict__ tw, unsigned char *__restrict__ lab,
@@ -1946,12 +1947,13 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
      * use (byte offsets from uniform bases, scheduling barrier) */
     const unsigned rN = 4u * (unsigned)N;
     const int jq = 4 * (t & 31);
+    constexpr int QB = ORG_P1_QUADS;
 #pragma unroll 1
-    for (int qb = 0; qb < 4; qb += 2) {
-        f4a fv[2], fxp[2], fxm[2], fl[2], fe[2], f1[2], f2[2], fr[2];
-        float fym[2], fyp[2];
+    for (int qb = 0; qb < 4; qb += QB) {
+        f4a fv[QB], fxp[QB], fxm[QB], fl[QB], fe[QB], f1[QB], f2[QB], fr[QB];
+        float fym[QB], fyp[QB];
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < QB; k++) {
             const int i = (t >> 5) + 32 * (qb + k);
             const unsigned g = 4u * ((unsigned)(x0 + i) * (unsigned)N + (unsigned)(y0 + jq));
             fv[k] = ldb4(ph, g); fxp[k] = ldb4(ph, g + rN); fxm[k] = ldb4(ph, g - rN);
@@ -1961,7 +1963,7 @@ static __device__ __forceinline__ void spectral_body(const DevP &P, const int n,
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < QB; k++) {
             const int i = (t >> 5) + 32 * (qb + k), e0 = i * ORG_MB + jq;
             unsigned labs = 0;
             f4a pyq, rq;
@@ -2631,11 +2633,10 @@ struct StepArgs {
     /* shared planes */
     float *lum, *ecm;
     float *psi, *hlum, *hecm;
-    /* lumen/ECM part of the cell update's bracket, two buffers: k_step's
-     * phase-field role reads lbrk[par], its lumen/ECM role writes the next
-     * step's into lbrk[1 - par]; the copy-back flips par */
-    double *lbrk0, *lbrk1;
-    int *lbrk_par;
+    /* lumen/ECM part of the cell update's bracket: k_step's phase-field role
+     * reads it, the copy-back launch after it (k_mid, k_post) writes the
+     * next step's */
+    double *lbrk;
     /* 1 once the ring part of the h(lum)/h(ECM) pass has been taken from a
      * committed batched step (k_live_reset clears it) */
     int *ringh;
@@ -3042,7 +3043,8 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
                 if (s[f].done) { stop_t[f] = t; break; }
             }
         }
-        __syncthreads();      /* red is rewritten by the next span */
+        /* red and xfail are next written after the next span's sweep
+         * barriers, so no barrier is needed here */
     }
     /* the next step's exchanges use tags after this step's */
     if (blk == 0 && threadIdx.x == 0) atomicAdd(&A.bar[1], (unsigned)k + 1u);
@@ -3071,7 +3073,7 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
     if (blk == 0 && threadIdx.x == 0) { A.lst[0] = s[0]; A.lst[1] = s[1]; }   /* s stays in registers */
     cur_out[0] = s[0].cur;
     cur_out[1] = s[1].cur;
-    return lum_barrier(A.bar, nblk, epoch, A.abort_flag, A.err, A.limit);
+    return true;          /* the copy-back launch reads the bands: no barrier here */
 }
 
 static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
@@ -3124,28 +3126,6 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
                            nblk, a.lst, a.part_lc, a.bar, epoch, a.abort_flag, a.err, a.limit, cur, !boxes))
             return;
     }
-    /* the lumen/ECM part of the next step's cell-update bracket from the
-     * final iterates (the fields the copy-back commits), into the buffer this
-     * step's phase-field role does not read; on the parts that can have
-     * changed: the live boxes grown by two (the bracket's stencil reaches one
-     * pixel past the copied region), all of them in full mode, and the ring
-     * (its stencil reads the corners, which the sweeps rewrite) */
-    const float *FL = cur[0] ? a.L1 : a.L0, *FE = cur[1] ? a.E1 : a.E0;
-    double *nxt = *a.lbrk_par ? a.lbrk0 : a.lbrk1;
-    const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
-    const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-    for (int k = blk; k < ntl + ORG_RING_PARTS; k += nblk) {
-        bool need = full || k >= ntl;
-        if (!need) {
-            const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
-            for (int f = 0; f < 2 && !need; f++) {
-                const int4 B = f ? BE : BL;
-                need = !box_empty(B) && ti < B.y + 2 && ti + ORG_HT > B.x - 2 && tj < B.w + 2 &&
-                       tj + ORG_HT > B.z - 2;
-            }
-        }
-        if (need) lbrk_part(P, k, FL, FE, nxt, (float *)(lds + 256), ORG_RING_PARTS);
-    }
 }
 static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int n, unsigned char *lds) {
     constexpr int WH = ORG_WH, NPX = ORG_W * ORG_W;
@@ -3156,7 +3136,7 @@ static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int 
     float *ph = a.phi + n * NN;
     float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
     cells_update<true>(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm,
-                       *a.lbrk_par ? a.lbrk1 : a.lbrk0, a.f0[n], pnl, nullptr, jx, jy);
+                       a.lbrk, a.f0[n], pnl, nullptr, jx, jy);
     cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
     __syncthreads();      /* the workgroup reads back its own stores */
     cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
@@ -3215,16 +3195,47 @@ __global__ __launch_bounds__(1024) void k_commit_cells(DevP P, float *__restrict
     commit_cell_body(P, blockIdx.x, rho, rho_box, phi, phi_win, xc, yc);
 }
 
+/* The lumen/ECM part of the next step's cell-update bracket into lbrk, from
+ * the loops' final iterates (the fields the copy-back commits), on the parts
+ * that can have changed: the live boxes grown by two (the bracket's stencil
+ * reaches one pixel past the copied region), all of them in full mode, and
+ * the ring (its stencil reads the corners, which the sweeps rewrite).  Run by
+ * the copy-back launches (k_mid, k_post), after the step's launch, whose
+ * phase-field role read the current bracket, has ended: one buffer serves
+ * both, and the step's lumen/ECM role ends with its loops (no barrier among
+ * its workgroups to read each other's bands).  lds: ORG_LDS_LBRK bytes. */
+static __device__ void lbrk_step(const DevP &P, const StepArgs &a, const int blk, const int nblk, float *lds) {
+    const LumState sl = a.lst[0], se = a.lst[1];
+    const float *FL = sl.cur ? a.L1 : a.L0, *FE = se.cur ? a.E1 : a.E0;
+    const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
+    const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
+    const bool full = !box_mode(BL, N) || !box_mode(BE, N);
+    for (int k = blk; k < ntl + ORG_RING_PARTS; k += nblk) {
+        bool need = full || k >= ntl;
+        if (!need) {
+            const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;
+            for (int f = 0; f < 2 && !need; f++) {
+                const int4 B = f ? BE : BL;
+                need = !box_empty(B) && ti < B.y + 2 && ti + ORG_HT > B.x - 2 && tj < B.w + 2 &&
+                       tj + ORG_HT > B.z - 2;
+            }
+        }
+        if (need) lbrk_part(P, k, FL, FE, a.lbrk, lds, ORG_RING_PARTS);
+    }
+}
+
 /* k_post: the copy-back of a timestep (run_with_active_gels.cpp:172-182):
  * rho box and phi window of every cell (workgroups 0..ncell-1); lum <- lumn,
- * ECM <- ECMn (the other workgroups, see commit_field).  Clears k_step's
- * barrier counter and abort flag. */
+ * ECM <- ECMn (the other workgroups, see commit_field), then the next
+ * step's lumen/ECM bracket.  Clears k_step's barrier counter and abort
+ * flag. */
 static __device__ __forceinline__ void post_cell(DevP P, StepArgs a, const int n) {
-    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; *a.lbrk_par ^= 1; }
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
     commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
 }
 static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int blk, const int nblk,
-                                                   int *sb) {
+                                                   unsigned char *scr) {
+    int *sb = (int *)scr;
     const LumState sl = a.lst[0], se = a.lst[1];
     if (blk == 0 && threadIdx.x == 0) {
         a.uni[0] = P.N; a.uni[1] = 0; a.uni[2] = P.N; a.uni[3] = 0;
@@ -3232,12 +3243,14 @@ static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int
     }
     commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
     commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
+    lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
 }
+#define ORG_LDS_POST (256 + ORG_LDS_LBRK)
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
-    __shared__ long long red[48];
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_POST];
     const int b = blockIdx.x, nc = P.nc;
     if (b < nc) post_cell(P, a, xcd_index(b, 0, nc));
-    else post_lumecm(P, a, xcd_index(b, nc, a.nlp), a.nlp, (int *)red);
+    else post_lumecm(P, a, xcd_index(b, nc, a.nlp), a.nlp, scr);
 }
 
 /* k_mid: the copy-back of one timestep and the first part of the next
@@ -3252,11 +3265,12 @@ __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
  *                  elsewhere (the preps write hfunc only inside the next
  *                  windows)
  *   last nlp       lum <- lumn, ECM <- ECMn with the live pixels (commit_field),
- *                  and h(lum), h(ECM) of the changed tiles from lumn, ECMn
+ *                  h(lum), h(ECM) of the changed tiles from lumn, ECMn, and
+ *                  the next step's lumen/ECM bracket (lbrk_step)
  * The previous centres come from (xp, yp), which the preps do not touch. */
-#define ORG_LDS_MID (4 * ORG_PSI_LDS_GROUP + 16)
+#define ORG_LDS_MID (4 * ORG_PSI_LDS_GROUP + 16 > 256 + ORG_LDS_LBRK ? 4 * ORG_PSI_LDS_GROUP + 16 : 256 + ORG_LDS_LBRK)
 static __device__ __forceinline__ void mid_cell(DevP P, StepArgs a, const int n, unsigned char *scr) {
-    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; *a.lbrk_par ^= 1; }
+    if (n == 0 && threadIdx.x == 0) { a.bar[0] = 0u; *a.abort_flag = 0; }
     commit_cell_body(P, n, a.rho, a.rho_box, a.phi, a.phi_win, a.xc, a.yc);
     __syncthreads();      /* the prep reads back this workgroup's phi window */
     cell_prep(P, n, a.phi, a.hfunc, a.xn, a.yn, a.bn, a.xc, a.yc, a.box, a.s0, a.s, a.f0, (double *)scr);
@@ -3302,6 +3316,7 @@ static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int 
             if (k == ntl) *a.ringh = 1;     /* read above before the sum's barriers */
         }
     }
+    lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
 }
 __global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];
@@ -3362,11 +3377,11 @@ __global__ __launch_bounds__(1024) void k_mid_batch(const ORG_CONST BatchEnt *en
 }
 
 __global__ __launch_bounds__(1024) void k_post_batch(const ORG_CONST BatchEnt *ent, const ORG_CONST int *bmap) {
-    __shared__ long long red[48];
+    __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_POST];
     const int v = bmap[blockIdx.x], r = ORG_BROLE(v), i = ORG_BIDX(v);
     const BatchEnt *E = (const BatchEnt *)(ent + ORG_BENT(v));
     if (r == 0) post_cell(E->P, E->a, i);
-    else post_lumecm(E->P, E->a, i, E->a.nlp, (int *)red);
+    else post_lumecm(E->P, E->a, i, E->a.nlp, scr);
 }
 
 /* update_rho_to_shape (cell_phase_functions.cpp:42-62) on every plane:

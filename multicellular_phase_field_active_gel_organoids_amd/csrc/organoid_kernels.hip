@@ -2881,11 +2881,10 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
         const int xp = k & 1;
         const unsigned tag = tag0 + (unsigned)k + 1u;
         unsigned long long *pk = xdev + (size_t)xp * 2 * K * 2 * ORG_LUM_MAX;
-        double acc[2][K];
-#pragma unroll
-        for (int f = 0; f < 2; f++)
-#pragma unroll
-            for (int t = 0; t < K; t++) acc[f][t] = 0.0;
+        /* per-wave sums of each sweep's deviation, [wave][2K] (the fixed
+         * order of the workgroup's sum: waves, then lanes by wave_sum_d) */
+        double *wred = (double *)(lds + 256);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
         for (int f = 0; f < 2; f++) {
             if (s[f].done) continue;
@@ -2897,15 +2896,23 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
                 const int p = (par + t) & 1;
                 const float *src = p ? L.u[1] : L.u[0];
                 float *dst = p ? L.u[0] : L.u[1];
+                double acc = 0.0;
                 if (L.b > L.a) {
                     const int i0 = max(L.a - (K - 1 - t), L.r0), i1 = min(L.b + (K - 1 - t), L.r1 + 1);
-                    acc[f][t] = lum_band_sweep(L, src, dst, i0, i1, ka, kc, dx2);
+                    acc = lum_band_sweep(L, src, dst, i0, i1, ka, kc, dx2);
                 }
                 if (corner_thread) {
                     const float cv = p ? cval[f][1] : cval[f][0];
-                    acc[f][t] += (double)(fabsf(0.0f - cv) * dx2);
+                    acc += (double)(fabsf(0.0f - cv) * dx2);
                     if (p) cval[f][0] = 0.0f; else cval[f][1] = 0.0f;
                 }
+                /* the wave's part of this sweep's deviation, before the
+                 * sweep's barrier (so the workgroup's sum needs none) */
+                const double ws = wave_sum_d(acc);
+                if (lane == 0) wred[2 * K * wv + f * K + t] = ws;
+                /* every thread read the last span's xfail before this
+                 * span's first barrier; the waits below come after the last */
+                if (t == K - 1 && threadIdx.x == 0) xfail = 0;
                 __syncthreads();
                 if (t <= K - 3 && L.b > L.a) {
                     /* keep the band's rows of this iterate: sweep t + 2 overwrites them */
@@ -2914,38 +2921,28 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
                         h[q.r * L.bw + q.c] = dst[(q.r + K + 1) * L.S + 1 + q.c];
                 }
             }
-            if (L.b > L.a) {
-                /* the latest iterate's first K and last K rows, tagged */
-                const float *lat = par ? L.u[1] : L.u[0];
-                const int own = L.b - L.a, nr = min(2 * K, own), base = L.a - K - 1;
-                unsigned long long *x = xrow + (size_t)(f * 2 + xp) * NN;
-                for (int q = threadIdx.x; q < nr * L.bw; q += blockDim.x) {
-                    const int r = q / L.bw, j = q % L.bw;
-                    const int gi = (r < K || own <= 2 * K) ? L.a + r : L.b - (nr - r);
-                    xput(&x[(size_t)gi * N + L.c0 + j], __float_as_uint(lat[(gi - base) * L.S + 1 + j]), tag);
-                }
-            }
         }
-        /* the 2K deviations of this workgroup, one fixed-order block sum,
-         * published as tagged halves */
-        {
-            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-            double *wred = (double *)(lds + 256);
-            double v[2 * K];
+        /* the 2K deviations of this workgroup (fixed-order sums of the wave
+         * parts), published first as tagged halves, then the latest
+         * iterate's first K and last K rows, tagged */
+        if (threadIdx.x < 2 * K && !((threadIdx.x >= K) ? s[1].done : s[0].done)) {
+            double t = 0.0;
+            for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[2 * K * ww + threadIdx.x];
+            const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+            xput(&pk[(threadIdx.x * 2 + 0) * ORG_LUM_MAX + blk], (unsigned)tb, tag);
+            xput(&pk[(threadIdx.x * 2 + 1) * ORG_LUM_MAX + blk], (unsigned)(tb >> 32), tag);
+        }
 #pragma unroll
-            for (int i = 0; i < 2 * K; i++) v[i] = s[i / K].done ? 0.0 : wave_sum_d(acc[i / K][i % K]);
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < 2 * K; i++) wred[2 * K * w + i] = v[i];
-            }
-            if (threadIdx.x == 0) xfail = 0;
-            __syncthreads();
-            if (threadIdx.x < 2 * K) {
-                double t = 0.0;
-                for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++) t += wred[2 * K * ww + threadIdx.x];
-                const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
-                xput(&pk[(threadIdx.x * 2 + 0) * ORG_LUM_MAX + blk], (unsigned)tb, tag);
-                xput(&pk[(threadIdx.x * 2 + 1) * ORG_LUM_MAX + blk], (unsigned)(tb >> 32), tag);
+        for (int f = 0; f < 2; f++) {
+            const LumBand &L = B[f];
+            if (s[f].done || !(L.b > L.a)) continue;
+            const float *lat = span_par[f] ? L.u[1] : L.u[0];
+            const int own = L.b - L.a, nr = min(2 * K, own), base = L.a - K - 1;
+            unsigned long long *x = xrow + (size_t)(f * 2 + xp) * NN;
+            for (int q = threadIdx.x; q < nr * L.bw; q += blockDim.x) {
+                const int r = q / L.bw, j = q % L.bw;
+                const int gi = (r < K || own <= 2 * K) ? L.a + r : L.b - (nr - r);
+                xput(&x[(size_t)gi * N + L.c0 + j], __float_as_uint(lat[(gi - base) * L.S + 1 + j]), tag);
             }
         }
         /* no barrier: wave 0 waits for every workgroup's deviations of this

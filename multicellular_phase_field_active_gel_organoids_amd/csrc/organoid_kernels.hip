@@ -2791,7 +2791,8 @@ static __device__ __forceinline__ void step_spectral(DevP P, StepArgs a, const i
 /* The batched step's lumen/ECM role in LDS (see the block comment above
  * ORG_LUM_SPAN); false if a barrier timed out. */
 static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int blk, const int nblk, const float se,
-                                    const int4 BL, const int4 BE, unsigned char *lds, int &epoch, int *cur_out) {
+                                    const int4 BL, const int4 BE, unsigned char *lds, int &epoch, int *cur_out,
+                                    const unsigned tag0) {
     constexpr int K = ORG_LUM_SPAN;
     const int N = P.N;
     const size_t NN = (size_t)N * N;
@@ -2874,7 +2875,6 @@ static __device__ bool lum_role_lds(const DevP &P, const StepArgs &A, const int 
     unsigned long long *xrow = (unsigned long long *)A.xch;
     unsigned long long *xdev = xrow + 4 * NN;
     __shared__ int xfail;
-    const unsigned tag0 = __hip_atomic_load(&A.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int k = 0;
     __syncthreads();
     for (; !(s[0].done && s[1].done); k++) {
@@ -3083,6 +3083,9 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
      * boxes of this step, the stored ones plus what the last copy-back found
      * (waves 0 and 1; every workgroup merges the same parts, the first
      * stores the result, and merging again gives the same boxes) */
+    /* the exchanges' tag base (a round trip to memory: read first, used
+     * after the update) */
+    const unsigned tag0 = __hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double hv = 0.0;
     for (int b = threadIdx.x; b < hl_ntiles(P.N) + 1; b += blockDim.x)
         hv += __hip_atomic_load(&a.part_h[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -3106,7 +3109,7 @@ static __device__ __forceinline__ void step_lumecm(DevP P, StepArgs a, const int
     int cur[2];
     if (boxes && ORG_LUM_LDS_LOOP && lum_lds_fits(BL, BE, nblk, ORG_LDS_FUSED)) {
         lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk, false);
-        if (!lum_role_lds(P, a, blk, nblk, (float)se, BL, BE, lds, epoch, cur)) return;
+        if (!lum_role_lds(P, a, blk, nblk, (float)se, BL, BE, lds, epoch, cur, tag0)) return;
     } else {
         if (boxes) {
             lumecm_ring(P, a.L0, a.L1, a.E0, a.E1, blk, nblk);

@@ -3210,7 +3210,10 @@ static __device__ void lbrk_step(const DevP &P, const StepArgs &a, const int blk
     const int4 BL = ld_box(a.live, 0), BE = ld_box(a.live, 1);
     const int N = P.N, nt = (N + ORG_HT - 1) / ORG_HT, ntl = nt * nt;
     const bool full = !box_mode(BL, N) || !box_mode(BE, N);
-    for (int k = blk; k < ntl + ORG_RING_PARTS; k += nblk) {
+    /* parts by residue blk + nblk/2: in k_mid the same workgroups' h parts
+     * go by residue blk, so the two loops' busy parts fall on different
+     * workgroups */
+    for (int k = (blk + nblk / 2) % nblk; k < ntl + ORG_RING_PARTS; k += nblk) {
         bool need = full || k >= ntl;
         if (!need) {
             const int ti = (k / nt) * ORG_HT, tj = (k % nt) * ORG_HT;

@@ -478,8 +478,10 @@ def main() -> int:
     tuning = {"tuning": org.get_tuning(), "source": "default"}
     if not args.no_tune:
         from multicellular_phase_field_active_gel_organoids_amd import autotune
-        r = autotune.tuned(org)
-        tuning = {"tuning": r["tuning"], "source": r["source"]}
+        r = autotune.tuned(org, steps=args.steps, warmup=0)     # on the steps about to be timed
+        tuning = {"tuning": r["tuning"], "source": r["source"],
+                  "tuned_on": f"the {args.steps} steps after the first warm-up (steps {max(args.warmup, 1)}.."
+                              f"{max(args.warmup, 1) + args.steps - 1}), from the same state for every candidate"}
         if r["source"] == "tuned":
             tuning.update(default=r["default"], default_ms_per_step=round(r["default_ms"], 5),
                           tuned_ms_per_step=round(r["best_ms"], 5), candidates_timed=len(r["measured"]))

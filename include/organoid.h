@@ -236,14 +236,20 @@ int org_graph_prepare(org_ctx *ctx, int nsteps);
  * fit on the device at once); roles: workgroups of each non-cell role of the
  * launches before and between steps (psi tiles, h(lum)/h(ECM) parts,
  * lumen/ECM copy-back; 1..512); graph_steps: steps per replayed hipGraph
- * (1..64).  The defaults are sized from the device's compute-unit count;
- * autotune.py picks them by timing.  None of them changes results: the
- * partitions feed only pointwise updates, per-tile sums in a fixed order and
- * box unions.  Setting a tuning drops the captured graphs.                    */
+ * (1..64); shared_bracket: 1 = the copy-back launch writes the lumen/ECM
+ * part of the cell update's bracket once per step into a shared FP64 plane
+ * that every cell's update reads (28 loads per pixel), 0 = each cell's
+ * update forms it from h(lum), h(ECM) itself (45 loads per pixel, a shorter
+ * copy-back launch).  The defaults are sized from the device's compute-unit
+ * count; autotune.py picks them by timing.  None of them changes results:
+ * the partitions feed only pointwise updates, per-tile sums in a fixed order
+ * and box unions, and both bracket forms evaluate the same expression on the
+ * same operands.  Setting a tuning drops the captured graphs.                */
 typedef struct org_tuning {
     int32_t lum_blocks;
     int32_t roles;
     int32_t graph_steps;
+    int32_t shared_bracket;
 } org_tuning;
 int org_get_tuning(org_ctx *ctx, org_tuning *out);
 int org_set_tuning(org_ctx *ctx, const org_tuning *tuning);

@@ -79,6 +79,7 @@ struct org_ctx {
     int nlp_cap = 0;                       /* partial boxes allocated per field (nlp never exceeds it) */
     int nlum = ORG_LUM_BLOCKS;             /* lumen/ECM workgroups of k_step (tuning, or the batch's share) */
     int graph_steps = ORG_GRAPH_STEPS;     /* steps per replayed graph (tuning) */
+    int shared_brk = 1;                    /* the bracket's shared plane (tuning) */
     int ncu = 256;                         /* compute units of the device */
     bool in_batch = false;                 /* an org_batch holds this context */
     int nparts_h = 0;                      /* parts of the sum of h(ECM): 64 x 64 tiles + the ring */
@@ -334,7 +335,7 @@ StepArgs step_args(org_ctx *c) {
     a.xp = c->xp; a.yp = c->yp;
     a.s0 = c->s0; a.s = c->s; a.f0 = c->f0;
     a.lum = c->lum; a.ecm = c->ecm; a.psi = c->psi; a.hlum = c->hlum; a.hecm = c->hecm;
-    a.lbrk = c->lbrk; a.ringh = c->ringh;
+    a.lbrk = c->lbrk; a.ringh = c->ringh; a.shared_brk = c->shared_brk;
     a.part_h = c->part_h; a.sum_ecm = c->sum_ecm; a.nh = c->nh;
     a.live = c->live; a.live_part = c->live_part; a.nlp = c->nlp; a.nlum = c->nlum;
     a.L0 = c->bufL0; a.L1 = c->bufL1; a.E0 = c->bufE0; a.E1 = c->bufE1;
@@ -820,6 +821,7 @@ int org_get_tuning(org_ctx *c, org_tuning *t) {
     t->lum_blocks = c->nlum;
     t->roles = c->npsi;
     t->graph_steps = c->graph_steps;
+    t->shared_bracket = c->shared_brk;
     return ORG_OK;
 }
 
@@ -828,7 +830,8 @@ int org_set_tuning(org_ctx *c, const org_tuning *t) {
     if (!t || c->in_batch) return note_error(ORG_EINVAL);
     if (c->gen) return ORG_OK;               /* the full-plane path has no tunable launch shape */
     if (t->lum_blocks < 1 || t->lum_blocks > std::min(ORG_LUM_MAX, c->ncu) || t->roles < 1 ||
-        t->roles > c->nlp_cap || t->graph_steps < 1 || t->graph_steps > 64)
+        t->roles > c->nlp_cap || t->graph_steps < 1 || t->graph_steps > 64 ||
+        (t->shared_bracket != 0 && t->shared_bracket != 1))
         return note_error(ORG_EINVAL);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->nlum = t->lum_blocks;
@@ -837,6 +840,10 @@ int org_set_tuning(org_ctx *c, const org_tuning *t) {
     if (c->nlp != t->roles) c->next_ready = false;   /* partial live boxes are re-initialised for the new nlp */
     c->nlp = t->roles;
     c->graph_steps = t->graph_steps;
+    /* the shared plane is not kept up to date without it: k_live_init
+     * recomputes it before the next batched step */
+    if (t->shared_bracket && !c->shared_brk) c->next_ready = false;
+    c->shared_brk = t->shared_bracket;
     for (int k = 0; k < 2; k++) {
         if (c->gexec[k]) { (void)hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
         if (c->graph[k]) { (void)hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }

@@ -2637,6 +2637,7 @@ struct StepArgs {
      * reads it, the copy-back launch after it (k_mid, k_post) writes the
      * next step's */
     double *lbrk;
+    int shared_brk;          /* 1: the copy-back launches write lbrk and the phase-field role reads it */
     /* 1 once the ring part of the h(lum)/h(ECM) pass has been taken from a
      * committed batched step (k_live_reset clears it) */
     int *ringh;
@@ -3135,8 +3136,12 @@ static __device__ __forceinline__ void step_cells(DevP P, StepArgs a, const int 
     const int x0 = a.xc[n] - P.zs, y0 = a.yc[n] - P.zs;
     float *ph = a.phi + n * NN;
     float *jx = a.jwin + (size_t)n * 2 * WH * WH, *jy = jx + WH * WH;
-    cells_update<true>(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm,
-                       a.lbrk, a.f0[n], pnl, nullptr, jx, jy);
+    if (a.shared_brk)
+        cells_update<true>(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, a.lbrk,
+                           a.f0[n], pnl, nullptr, jx, jy);
+    else
+        cells_update<false>(P, x0, y0, ph, a.rho + n * NN, a.hfunc + n * NN, a.psi, a.hlum, a.hecm, nullptr,
+                            a.f0[n], pnl, nullptr, jx, jy);
     cells_normals_ring(P, x0, y0, ph, jx, jy, phase);
     __syncthreads();      /* the workgroup reads back its own stores */
     cells_relax(P, n, x0, y0, ph, phase, pnl, jx, jy, red, a.phin_win + (size_t)n * NPX,
@@ -3246,7 +3251,7 @@ static __device__ __forceinline__ void post_lumecm(DevP P, StepArgs a, const int
     }
     commit_field(P, 0, ld_box(a.live, 0), blk, nblk, a.lum, sl.cur ? a.L1 : a.L0, a.live_part, sb);
     commit_field(P, 1, ld_box(a.live, 1), blk, nblk, a.ecm, se.cur ? a.E1 : a.E0, a.live_part, sb);
-    lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
+    if (a.shared_brk) lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
 }
 #define ORG_LDS_POST (256 + ORG_LDS_LBRK)
 __global__ __launch_bounds__(1024) void k_post(DevP P, StepArgs a) {
@@ -3319,7 +3324,7 @@ static __device__ __forceinline__ void mid_lumecm(DevP P, StepArgs a, const int 
             if (k == ntl) *a.ringh = 1;     /* read above before the sum's barriers */
         }
     }
-    lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
+    if (a.shared_brk) lbrk_step(P, a, blk, nblk, (float *)(scr + 256));
 }
 __global__ __launch_bounds__(1024) void k_mid(DevP P, StepArgs a) {
     __shared__ __attribute__((aligned(16))) unsigned char scr[ORG_LDS_MID];

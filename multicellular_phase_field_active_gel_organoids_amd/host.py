@@ -114,10 +114,12 @@ def _chk(rc: int, what: str) -> None:
 
 class Tuning(ctypes.Structure):
     """org_tuning (include/organoid.h): launch shape of the batched step."""
-    _fields_ = [("lum_blocks", ctypes.c_int32), ("roles", ctypes.c_int32), ("graph_steps", ctypes.c_int32)]
+    _fields_ = [("lum_blocks", ctypes.c_int32), ("roles", ctypes.c_int32), ("graph_steps", ctypes.c_int32),
+                ("shared_bracket", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
-        return {"lum_blocks": self.lum_blocks, "roles": self.roles, "graph_steps": self.graph_steps}
+        return {"lum_blocks": self.lum_blocks, "roles": self.roles, "graph_steps": self.graph_steps,
+                "shared_bracket": self.shared_bracket}
 
 
 class Organoid:
@@ -251,8 +253,8 @@ class Organoid:
         _chk(self.lib.org_get_tuning(self.h, ctypes.byref(t)), "org_get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, lum_blocks: int, roles: int, graph_steps: int) -> None:
-        t = Tuning(lum_blocks, roles, graph_steps)
+    def set_tuning(self, lum_blocks: int, roles: int, graph_steps: int, shared_bracket: int = 1) -> None:
+        t = Tuning(lum_blocks, roles, graph_steps, shared_bracket)
         _chk(self.lib.org_set_tuning(self.h, ctypes.byref(t)), "org_set_tuning")
 
     def surrogate_samples(self, nsteps: int, it0: int = 0, residuals: bool = False):

@@ -26,17 +26,21 @@ def test_tunings_give_the_same_trajectory():
     print("case organoid16_512: 24 steps under three tunings", flush=True)
     ref = _org(wl)
     ref.step(24, graph=True)
-    for t in ({"lum_blocks": 24, "roles": 40, "graph_steps": 3}, {"lum_blocks": 128, "roles": 256, "graph_steps": 16}):
+    for t in ({"lum_blocks": 24, "roles": 40, "graph_steps": 3, "shared_bracket": 1},
+              {"lum_blocks": 128, "roles": 256, "graph_steps": 16, "shared_bracket": 1},
+              {"lum_blocks": 64, "roles": 96, "graph_steps": 8, "shared_bracket": 0}):
         g = _org(wl)
         g.set_tuning(**t)
         assert g.get_tuning() == t
         g.step(24, graph=True)
         for f in FIELDS:
             assert np.array_equal(g.get_field(f), ref.get_field(f)), (t, f)
-    g = _org(wl)                       # tuning changed between steps (new copy-back partition)
+    g = _org(wl)                       # tuning changed between steps (new copy-back partition,
+    g.step(8, graph=True)              # the bracket's shared plane off, then on again)
+    g.set_tuning(lum_blocks=40, roles=72, graph_steps=5, shared_bracket=0)
     g.step(8, graph=True)
-    g.set_tuning(lum_blocks=40, roles=72, graph_steps=5)
-    g.step(16, graph=True)
+    g.set_tuning(lum_blocks=40, roles=72, graph_steps=5, shared_bracket=1)
+    g.step(8, graph=True)
     for f in FIELDS:
         assert np.array_equal(g.get_field(f), ref.get_field(f)), f
 
@@ -67,7 +71,7 @@ def test_out_of_range_tunings_are_refused():
     g = _org(wl)
     t0 = g.get_tuning()
     for bad in ({"lum_blocks": 0}, {"lum_blocks": 129}, {"roles": 0}, {"roles": 513}, {"graph_steps": 0},
-                {"graph_steps": 65}):
+                {"graph_steps": 65}, {"shared_bracket": 2}):
         with pytest.raises(host.OrganoidError):
             g.set_tuning(**dict(t0, **bad))
     assert g.get_tuning() == t0

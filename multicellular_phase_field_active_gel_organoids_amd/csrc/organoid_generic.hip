@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "organoid_generic.h"
@@ -1008,6 +1009,160 @@ __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict_
     if (threadIdx.x == 0 && b == 0) counts[n] = count;
 }
 
+/* Tagged exchange words (FP32 state): 32 bits of payload in the low half,
+ * the tag of the exchange in the high half, stored and loaded as one 64-bit
+ * agent-scope access (single-copy atomic and coherent across the XCDs'
+ * L2s): a reader that finds its tag holds that exchange's payload, so the
+ * workgroups of a loop meet without a barrier, a wait for store completion
+ * or a cache flush.  Tags grow by one per sweep; the host passes each
+ * launch a base above every tag used before (the buffers are zeroed at
+ * set-up and no exchange uses tag 0). */
+static __device__ __forceinline__ void g_tput(unsigned long long *p, unsigned v, unsigned tag) {
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | (unsigned long long)v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ unsigned long long g_tget(const unsigned long long *p) {
+    return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+/* poll *p until it carries tag (w: the word as first read); false once the
+ * wait passes `limit` wall-clock ticks or the abort flag is up */
+static __device__ bool g_twait(const unsigned long long *p, unsigned tag, unsigned long long &w, int *abort_flag,
+                               long long limit) {
+    if ((unsigned)(w >> 32) == tag) return true;
+    const long long t0 = wall_clock64();
+    for (int k = 1;; k++) {
+        __builtin_amdgcn_s_sleep(1);
+        w = g_tget(p);
+        if ((unsigned)(w >> 32) == tag) return true;
+        if ((k & 15) == 0 && (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                              wall_clock64() - t0 > limit))
+            return false;
+    }
+}
+
+/* reshaping_cells (cell_phase_functions.cpp:447-580) for FP32 state with K
+ * workgroups per cell (blockIdx.x = n K + b), workgroup b on window rows
+ * [b W / K, (b + 1) W / K) held in LDS with the row above and below:
+ * both iterates and the normals (the window's one-pixel ring, phi around
+ * the window, is static and in both iterates).  Each sweep is the flux of
+ * g_relax_cells_mb on the same operands in the same order, so the
+ * iterates, the per-workgroup deviations (same pixel-to-thread split, same
+ * block sum) and the in-order sum over the K parts equal that kernel's bit
+ * for bit.  After a sweep a workgroup publishes its first and last rows
+ * and its deviation as tagged words (tag base + sweep number), then waits
+ * for its neighbours' rows (into the new iterate's halo rows) and for the
+ * cell's K deviations.  At the end phin's window gets the last iterate and
+ * phi's the second-to-last (own rows).  xw: [nc][2 parities][K][2][W] row
+ * words, then [nc][2][K][2 halves] deviation words.  Dynamic LDS: 4 (rmax
+ * + 2)(W + 2) reals, rmax >= the largest band. */
+__global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__restrict__ phi,
+                                                          float *__restrict__ phin, const float *__restrict__ jx,
+                                                          const float *__restrict__ jy, const int *__restrict__ xc,
+                                                          const int *__restrict__ yc, double thr, double rdev0,
+                                                          unsigned long long *__restrict__ xw, unsigned tag0,
+                                                          int *__restrict__ counts, int *__restrict__ cmax,
+                                                          int *__restrict__ relax_flag, int *__restrict__ abort_flag,
+                                                          long long limit, int K, int rmax) {
+    __shared__ double red[16];
+    __shared__ double sums[64];
+    __shared__ int xfail;
+    extern __shared__ char g_dyn[];
+    const int N = P.N, zs = P.zs, W = 2 * zs, W2 = W + 2, LR = rmax + 2;
+    const int n = blockIdx.x / K, b = blockIdx.x % K;
+    const size_t NN = (size_t)N * N;
+    float *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
+    const float *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
+    const int x0 = xc[n] - zs, y0 = yc[n] - zs;              /* window origin on the grid */
+    const int r0 = b * W / K, r1 = (b + 1) * W / K, rows = r1 - r0;
+    /* LDS row l = window row r0 - 1 + l, LDS column c = window column c - 1 */
+    float *U0 = (float *)g_dyn, *U1 = U0 + (size_t)LR * W2, *JX = U1 + (size_t)LR * W2, *JY = JX + (size_t)LR * W2;
+    unsigned long long *xr = xw + (size_t)n * 2 * K * 2 * W;
+    unsigned long long *xd = xw + (size_t)P.nc * 2 * K * 2 * W + (size_t)n * 2 * K * 2;
+    for (int k = threadIdx.x; k < (rows + 2) * W2; k += blockDim.x) {
+        const int l = k / W2, c = k % W2, wr = r0 - 1 + l, wc = c - 1;
+        const size_t g = (size_t)(x0 + wr) * N + (y0 + wc);
+        const bool ring = wr < 0 || wr >= W || wc < 0 || wc >= W;
+        const float v = ring ? ph[g] : pn[g];
+        U0[k] = v; U1[k] = v;
+        JX[k] = jxn[g]; JY[k] = jyn[g];
+    }
+    __syncthreads();
+    const float dx = P.dx;
+    double rdev = rdev0;
+    int ri = 0, count = 0;
+    while (rdev > thr) {
+        const float *src = (count & 1) ? U1 : U0;
+        float *dst = (count & 1) ? U0 : U1;
+        const int par = count & 1;
+        const unsigned tag = tag0 + (unsigned)count + 1u;
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
+            const int q = (k / W + 1) * W2 + (k % W + 1);
+            float o;
+            const float v = g_flux_rows(src + q - W2, src + q, src + q + W2, JX, JY, (size_t)q, W2, P.wi, P.rdt,
+                                        dx, o);
+            dst[q] = v;
+            acc += fabs((double)(v - o)) * dx * dx / P.ncur;
+        }
+        __syncthreads();
+        /* every thread read the last sweep's xfail before the barrier above */
+        if (threadIdx.x == 0) xfail = 0;
+        for (int k = threadIdx.x; k < 2 * W; k += blockDim.x) {
+            const int e = k / W, c = k % W, l = e ? rows : 1;
+            g_tput(&xr[(((size_t)par * K + b) * 2 + e) * W + c], __float_as_uint(dst[l * W2 + c + 1]), tag);
+        }
+        const double t = gblock_sum(acc, red);
+        if (threadIdx.x == 0) {
+            const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+            g_tput(&xd[((size_t)par * K + b) * 2 + 0], (unsigned)tb, tag);
+            g_tput(&xd[((size_t)par * K + b) * 2 + 1], (unsigned)(tb >> 32), tag);
+        }
+        bool ok = true;
+        if (threadIdx.x < 64) {                   /* the cell's K deviations, one per lane */
+            const int b2 = threadIdx.x;
+            if (b2 < K) {
+                unsigned long long lo = g_tget(&xd[((size_t)par * K + b2) * 2 + 0]);
+                unsigned long long hi = g_tget(&xd[((size_t)par * K + b2) * 2 + 1]);
+                ok = g_twait(&xd[((size_t)par * K + b2) * 2 + 0], tag, lo, abort_flag, limit) &&
+                     g_twait(&xd[((size_t)par * K + b2) * 2 + 1], tag, hi, abort_flag, limit);
+                sums[b2] = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+            }
+        } else {                                  /* the neighbours' rows into the new iterate's halo rows */
+            const int nq = 2 * W;
+            for (int k = (int)threadIdx.x - 64; k < nq; k += (int)blockDim.x - 64) {
+                const int e = k / W, c = k % W;   /* e 0: the row above (b - 1's last), 1: below (b + 1's first) */
+                if ((e == 0 && b == 0) || (e == 1 && b == K - 1)) continue;   /* the static ring rows */
+                const unsigned long long *a = &xr[(((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * W + c];
+                unsigned long long w = g_tget(a);
+                ok = ok && g_twait(a, tag, w, abort_flag, limit);
+                dst[(e ? rows + 1 : 0) * W2 + c + 1] = __uint_as_float((unsigned)w);
+            }
+        }
+        if (!ok) xfail = 1;
+        __syncthreads();
+        if (xfail) {
+            if (threadIdx.x == 0) atomicExch(abort_flag, 1);
+            return;
+        }
+        rdev = g_lds_sum(sums, K);
+        count++;
+        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :565-568 */
+        ri++;
+    }
+    /* phin <- the last iterate, phi <- the one before; own rows only */
+    const float *last = (count & 1) ? U1 : U0, *prev = (count & 1) ? U0 : U1;
+    for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
+        const int q = (k / W + 1) * W2 + (k % W + 1);
+        const size_t g = (size_t)(x0 + r0 + k / W) * N + (y0 + k % W);
+        pn[g] = last[q];
+        if (count > 0) ph[g] = prev[q];
+    }
+    if (threadIdx.x == 0 && b == 0) {
+        counts[n] = count;
+        atomicMax(cmax, count);
+    }
+}
+
 /* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop
  * (cell_phase_functions.cpp:774-866) with K workgroups; the caller ran
  * sweep 1 on the whole grid (lumn <- F(lum), the reference's copy lum <-
@@ -1386,6 +1541,15 @@ struct GenCtx final : Generic {
     unsigned *mb_ctr = nullptr;      /* [nc] cell groups, [nc] the lumen/ECM group, [nc + 1] abort */
     int *mb_out = nullptr, *mb_box = nullptr;
     R *mb_xch = nullptr;             /* exchanged band rows: cells [nc][2][kc][2][W + 2]; lumen/ECM [2][kl][2][N] */
+    /* FP32 state: the cell loop with its bands in LDS and tagged exchanges
+     * (g_relax_cells_lds): workgroups per cell (0: not used), the largest
+     * band, its LDS bytes, the tagged words, the launch's largest sweep
+     * count, and the next launch's tag base */
+    int kcl = 0, rmax_l = 0;
+    size_t lds_l = 0;
+    unsigned long long *mb_tw = nullptr;
+    int *mb_cmax = nullptr;
+    unsigned tag_next = 1;
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
@@ -1457,6 +1621,29 @@ struct GenCtx final : Generic {
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
         GTRY(A_(&mb_out, 1)); GTRY(A_(&mb_box, 4));
         GTRY(A_(&mb_xch, std::max((size_t)nc * 2 * kc * 2 * (W + 2), (size_t)2 * kl * 2 * N)));
+        if (std::is_same<R, float>::value) {
+            int ncu = 0, dev = 0;
+            GHIP(hipGetDevice(&dev));
+            GHIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            /* one workgroup per compute unit at most (all resident), at
+             * least 4 window rows each, the deviations polled one per lane */
+            kcl = std::min(std::min(ncu / nc, W / 4), 64);
+            if (kcl >= 2) {
+                rmax_l = (W + kcl - 1) / kcl;
+                lds_l = 4 * (size_t)(rmax_l + 2) * (W + 2) * sizeof(float);
+                if (lds_l > 140 * 1024) kcl = 0;
+            } else {
+                kcl = 0;
+            }
+            if (kcl) {
+                const size_t words = (size_t)nc * 2 * kcl * 2 * W + (size_t)nc * 2 * kcl * 2;
+                GTRY(A_(&mb_tw, words));
+                GTRY(A_(&mb_cmax, 1));
+                GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
+                GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds_l));
+            }
+        }
         std::vector<double> s0v(nc);
         for (int n = 0; n < nc; n++) s0v[n] = (p.use_s0_cell && s0_cell) ? s0_cell[n] : p.s0;
         std::vector<int> cen(nc, N / 2);
@@ -1762,6 +1949,29 @@ struct GenCtx final : Generic {
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
         GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+        if constexpr (std::is_same<R, float>::value) {
+            if (kcl) {
+                GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+                GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)mb_cmax, 1);
+                GLAUNCH(g_relax_cells_lds, dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin, (const R *)jx,
+                        (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next, counts,
+                        mb_cmax, relax, (int *)(mb_ctr + nc + 1), wait_limit, kcl, rmax_l);
+                int f[3] = {0, 0, 0};
+                GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+                GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+                GHIP(hipMemcpyAsync(&f[2], mb_cmax, sizeof(int), hipMemcpyDeviceToHost, s));
+                GHIP(hipStreamSynchronize(s));
+                if (f[1]) {
+                    tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
+                    fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
+                                    "for more than 2 s (not all resident?)\n");
+                    return ORG_ETIMEOUT;
+                }
+                tag_next += (unsigned)f[2] + 2u;
+                *failed = f[0];
+                return ORG_OK;
+            }
+        }
         if (kc > 1) {
             GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
             GLAUNCH((g_relax_cells_mb<R>), dim3(nc * kc), dim3(1024), 2 * sizeof(R) * (W + 2), P, phi, phin,

@@ -1086,6 +1086,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
         U0[k] = v; U1[k] = v;
         JX[k] = jxn[g]; JY[k] = jyn[g];
     }
+    if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
     __syncthreads();
     const float dx = P.dx;
     double rdev = rdev0;
@@ -1105,8 +1106,6 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
             acc += fabs((double)(v - o)) * dx * dx / P.ncur;
         }
         __syncthreads();
-        /* every thread read the last sweep's xfail before the barrier above */
-        if (threadIdx.x == 0) xfail = 0;
         for (int k = threadIdx.x; k < 2 * W; k += blockDim.x) {
             const int e = k / W, c = k % W, l = e ? rows : 1;
             g_tput(&xr[(((size_t)par * K + b) * 2 + e) * W + c], __float_as_uint(dst[l * W2 + c + 1]), tag);
@@ -1257,6 +1256,113 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
         count++;
         if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :862-865 */
         ri++;
+    }
+    if (threadIdx.x == 0 && b == 0) out[0] = count;
+}
+
+/* the most dynamic LDS a banded FP32 loop takes (the static part adds a
+ * little over 1 KB; a compute unit has 160 KB) */
+#define ORG_GEN_LDS_MAX (140 * 1024)
+
+/* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop as g_lum_loop_mb, for
+ * FP32 state, with each workgroup's band of box rows held in LDS with the
+ * row above and below (both iterates and the normals; the rows and
+ * columns just outside the box are static and the same in both buffers).
+ * Same flux on the same operands, same per-thread pixel split and block
+ * sum, in-order sum over the K parts: the iterates equal g_lum_loop_mb's
+ * bit for bit.  After a sweep a workgroup publishes its first and last
+ * rows and its deviation as tagged words (tag base + sweeps of this
+ * launch), then waits for its neighbours' rows and the K deviations.  At
+ * the end both buffers get the band's rows of the iterates they hold, as
+ * the sweeps of g_lum_loop_mb leave them.  xw: [2 parities][K][2][N] row
+ * words, then [2][K][2 halves] deviation words.  Dynamic LDS: 4 (rmax + 2)
+ * (box width + 2) reals, rmax >= the largest band. */
+__global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__restrict__ bufA,
+                                                       float *__restrict__ bufB, const float *__restrict__ jx,
+                                                       const float *__restrict__ jy, const int *__restrict__ box,
+                                                       double thr, double rdev, int count, int ri,
+                                                       unsigned long long *__restrict__ xw, unsigned tag0,
+                                                       int *__restrict__ out, int *__restrict__ relax_flag,
+                                                       int *__restrict__ abort_flag, long long limit, int K, int rmax) {
+    __shared__ double red[16];
+    __shared__ double sums[128];
+    __shared__ int xfail;
+    extern __shared__ char g_dyn[];
+    const int N = P.N, b = blockIdx.x;
+    const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
+    const int bw = bj1 - bj0, bh = bi1 - bi0, L = bw + 2, LR = rmax + 2;
+    const int r0 = bi0 + b * bh / K, r1 = bi0 + (b + 1) * bh / K, rows = r1 - r0;
+    /* LDS row l = grid row r0 - 1 + l, LDS column c = grid column bj0 - 1 + c */
+    float *UA = (float *)g_dyn, *UB = UA + (size_t)LR * L, *JX = UB + (size_t)LR * L, *JY = JX + (size_t)LR * L;
+    unsigned long long *xr = xw, *xd = xw + (size_t)2 * K * 2 * N;
+    for (int k = threadIdx.x; k < (rows + 2) * L; k += blockDim.x) {
+        const size_t g = (size_t)(r0 - 1 + k / L) * N + (bj0 - 1 + k % L);
+        UA[k] = bufA[g]; UB[k] = bufB[g];
+        JX[k] = jx[g]; JY[k] = jy[g];
+    }
+    if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
+    __syncthreads();
+    const float dx = P.dx;
+    const int c0 = count;
+    while (rdev > thr) {
+        const float *u = (count & 1) ? UA : UB;      /* X_count: bufA for odd count */
+        float *un = (count & 1) ? UB : UA;
+        const int par = count & 1;
+        const unsigned tag = tag0 + (unsigned)(count - c0) + 1u;
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
+            const int q = (k / bw + 1) * L + (k % bw + 1);
+            float o;
+            const float v = g_flux_rows(u + q - L, u + q, u + q + L, JX, JY, (size_t)q, L, P.wi, P.rdt, dx, o);
+            un[q] = v;
+            acc += fabs((double)(v - o)) * dx * dx;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < 2 * bw; k += blockDim.x) {
+            const int e = k / bw, c = k % bw, l = e ? rows : 1;
+            g_tput(&xr[(((size_t)par * K + b) * 2 + e) * N + c], __float_as_uint(un[l * L + c + 1]), tag);
+        }
+        const double t = gblock_sum(acc, red);
+        if (threadIdx.x == 0) {
+            const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+            g_tput(&xd[((size_t)par * K + b) * 2 + 0], (unsigned)tb, tag);
+            g_tput(&xd[((size_t)par * K + b) * 2 + 1], (unsigned)(tb >> 32), tag);
+        }
+        bool ok = true;
+        if (threadIdx.x < 64) {                   /* the K deviations, lanes b2 and b2 + 64 */
+            for (int b2 = threadIdx.x; b2 < K; b2 += 64) {
+                unsigned long long lo = g_tget(&xd[((size_t)par * K + b2) * 2 + 0]);
+                unsigned long long hi = g_tget(&xd[((size_t)par * K + b2) * 2 + 1]);
+                ok = ok && g_twait(&xd[((size_t)par * K + b2) * 2 + 0], tag, lo, abort_flag, limit) &&
+                     g_twait(&xd[((size_t)par * K + b2) * 2 + 1], tag, hi, abort_flag, limit);
+                sums[b2] = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+            }
+        } else {                                  /* the neighbours' rows into the new iterate's halo rows */
+            for (int k = (int)threadIdx.x - 64; k < 2 * bw; k += (int)blockDim.x - 64) {
+                const int e = k / bw, c = k % bw;
+                if ((e == 0 && b == 0) || (e == 1 && b == K - 1)) continue;   /* static rows outside the box */
+                const unsigned long long *a = &xr[(((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * N + c];
+                unsigned long long w = g_tget(a);
+                ok = ok && g_twait(a, tag, w, abort_flag, limit);
+                un[(e ? rows + 1 : 0) * L + c + 1] = __uint_as_float((unsigned)w);
+            }
+        }
+        if (!ok) xfail = 1;
+        __syncthreads();
+        if (xfail) {
+            if (threadIdx.x == 0) atomicExch(abort_flag, 1);
+            return;
+        }
+        rdev = g_lds_sum(sums, K);
+        count++;
+        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :862-865 */
+        ri++;
+    }
+    for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
+        const int q = (k / bw + 1) * L + (k % bw + 1);
+        const size_t g = (size_t)(r0 + k / bw) * N + (bj0 + k % bw);
+        bufA[g] = UA[q];
+        bufB[g] = UB[q];
     }
     if (threadIdx.x == 0 && b == 0) out[0] = count;
 }
@@ -1548,6 +1654,10 @@ struct GenCtx final : Generic {
     int kcl = 0, rmax_l = 0;
     size_t lds_l = 0;
     unsigned long long *mb_tw = nullptr;
+    /* and the lumen/ECM loop's (g_lum_loop_lds): the most workgroups (0: not
+     * used) and its tagged words, [2][klm][2][N] rows + [2][klm][2] halves */
+    int klm = 0;
+    unsigned long long *mb_lw = nullptr;
     int *mb_cmax = nullptr;
     unsigned tag_next = 1;
     long long wait_limit = 0;
@@ -1631,7 +1741,7 @@ struct GenCtx final : Generic {
             if (kcl >= 2) {
                 rmax_l = (W + kcl - 1) / kcl;
                 lds_l = 4 * (size_t)(rmax_l + 2) * (W + 2) * sizeof(float);
-                if (lds_l > 140 * 1024) kcl = 0;
+                if (lds_l > ORG_GEN_LDS_MAX) kcl = 0;
             } else {
                 kcl = 0;
             }
@@ -1642,6 +1752,19 @@ struct GenCtx final : Generic {
                 GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
                 GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)lds_l));
+            }
+            /* the lumen/ECM loop: one band of box rows per workgroup, up to
+             * one workgroup per compute unit and two deviations per lane of
+             * the polling wave; each launch checks that its bands fit */
+            klm = std::min(std::min(ncu, 128), N - 2);
+            if (klm >= 2) {
+                const size_t words = (size_t)2 * klm * 2 * N + (size_t)2 * klm * 2;
+                GTRY(A_(&mb_lw, words));
+                GHIP(hipMemset(mb_lw, 0, sizeof(unsigned long long) * words));
+                GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         ORG_GEN_LDS_MAX));
+            } else {
+                klm = 0;
             }
         }
         std::vector<double> s0v(nc);
@@ -1863,6 +1986,18 @@ struct GenCtx final : Generic {
                 (const R *)psi_, (const R *)hl, (const R *)he, (const double *)sum_e, lumn, ecmn);
         return ORG_OK;
     }
+    /* dynamic LDS of g_lum_loop_lds for bands of up to `rows` rows of a box
+     * `bw` columns wide */
+    static size_t lum_lds_bytes(int rows, int bw) { return 4 * (size_t)(rows + 2) * (bw + 2) * sizeof(float); }
+    /* workgroups of g_lum_loop_lds for an bh x bw box: as many as allowed,
+     * a band of at least one row each; 0 (the exchange-buffer loop runs)
+     * for FP64 state or when the bands do not fit in LDS */
+    int lum_lds_blocks(int bh, int bw) const {
+        if (!std::is_same<R, float>::value || klm < 2) return 0;
+        const int kb = std::min(klm, bh);
+        if (kb < 2 || lum_lds_bytes((bh + kb - 1) / kb, bw) > ORG_GEN_LDS_MAX) return 0;
+        return kb;
+    }
     int reshape_lumecm(int which, int *failed) override {
         R *&u = which ? ecm : lum;
         R *&un = which ? ecmn : lumn;
@@ -1900,6 +2035,29 @@ struct GenCtx final : Generic {
                     /* no nonzero normals: sweep 2 changes nothing (both buffers already agree) */
                     count = 2;
                     if (ri > p.rimax) *failed = 1;
+                } else if (int kb = lum_lds_blocks(bh, bw)) {
+                    /* sweeps 2, 3, ... with the bands in LDS (FP32 state) */
+                    const int rows_max = (bh + kb - 1) / kb;
+                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr + nc, 2);
+                    GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+                    if constexpr (std::is_same<R, float>::value)
+                        GLAUNCH(g_lum_loop_lds, dim3(kb), dim3(1024), lum_lds_bytes(rows_max, bw), P, u, un,
+                                (const float *)jlx, (const float *)jly, (const int *)mb_box, thr, rdev, count, ri,
+                                mb_lw, tag_next, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
+                    int h[3] = {0, 0, 0};
+                    GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+                    GHIP(hipStreamSynchronize(s));
+                    if (h[2]) {
+                        tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
+                        fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
+                                        "other for more than 2 s (not all resident?)\n");
+                        return ORG_ETIMEOUT;
+                    }
+                    tag_next += (unsigned)h[0] + 2u;
+                    count = h[0];
+                    *failed = h[1];
                 } else {
                     /* sweeps 2, 3, ... on the device, bands of at least 8 box rows
                      * and about 8192 pixels */

@@ -1040,21 +1040,217 @@ static __device__ bool g_twait(const unsigned long long *p, unsigned tag, unsign
     }
 }
 
+/* The banded FP32 reshaping loops (g_relax_cells_lds, g_lum_loop_lds) run
+ * S = 1 or 2 sweeps per exchange (ORG_GEN_SPAN_CELLS, ORG_GEN_SPAN_LUM) and
+ * take at most ORG_GEN_LDS_MAX bytes of dynamic LDS (the static part adds
+ * up to 4 KB; a compute unit has 160 KB). */
+#ifndef ORG_GEN_SPAN_CELLS
+#define ORG_GEN_SPAN_CELLS 1
+#endif
+#ifndef ORG_GEN_SPAN_LUM
+#define ORG_GEN_SPAN_LUM 1
+#endif
+#define ORG_GEN_LDS_MAX (140 * 1024)
+
+/* One workgroup's band of a banded FP32 reshaping loop with spans of S
+ * sweeps: own rows [r0, r1) of a domain (the cell's window, the normals'
+ * box) `bw` columns wide whose one-pixel surround is static.  LDS row l
+ * holds domain row r0 - S + l, LDS column c domain column c - 1 (L = bw +
+ * 2).  X0 holds the span's input, X1 the other iterate (both with the
+ * static surround), JX / JY the normals, SV (S = 2; from row S on) the span
+ * input's own rows.  Every band has S rows or more. */
+struct GBand {
+    float *X0, *X1, *JX, *JY, *SV;
+    int L, bw, rows, b, K;
+    unsigned long long *xr;   /* [2 parities][K][2 ends][S][bw] tagged rows */
+    unsigned long long *xd;   /* [2 parities][K][S][2 halves] tagged deviations */
+};
+/* dynamic LDS of a band of up to `rows` rows, `bw` columns */
+static __host__ __device__ inline size_t g_band_lds(int S, int rows, int bw) {
+    return ((size_t)4 * (rows + 2 * S) + (S > 1 ? rows : 0)) * (bw + 2) * sizeof(float);
+}
+/* tagged words of a group of K bands `bw` columns wide */
+static __host__ __device__ inline size_t g_band_words(int S, int K, int bw) {
+    return (size_t)2 * K * 2 * S * bw + (size_t)2 * K * S * 2;
+}
+/* the dynamic LDS of the launch */
+static __device__ inline char *g_dyn_base() {
+    extern __shared__ char g_dyn[];
+    return g_dyn;
+}
+/* G's LDS arrays, for bands of up to rmax rows (G.L set) */
+static __device__ inline void g_band_lds_split(GBand &G, int S, int rmax) {
+    const size_t plane = (size_t)(rmax + 2 * S) * G.L;
+    G.X0 = (float *)g_dyn_base(); G.X1 = G.X0 + plane; G.JX = G.X1 + plane; G.JY = G.JX + plane;
+    G.SV = G.JY + plane - (size_t)S * G.L;          /* own rows only: from row S on */
+}
+
+/* The loop `while (rdev > thr) { sweep; rdev = deviation; count++; if (ri >
+ * rimax) { relax flag; break; } ri++; }` of reshaping_cells
+ * (cell_phase_functions.cpp:475-568) and reshaping_lumen_or_ECM (:774-866)
+ * on one band.  Sweep t of a span covers the band grown by S - 1 - t rows
+ * on each side that has a neighbouring band (those rows are the
+ * neighbour's, recomputed with the same arithmetic on the same operands,
+ * so the same values), so a span needs the neighbours' rows once, at its
+ * start.  A sweep's deviation is over the own rows, with the same
+ * pixel-to-thread split and block sum (each wave's tree sum, then the
+ * waves in order) as the exchange-buffer kernels, so the same value, over
+ * ncur for the cells (DIV).  After a span each workgroup publishes its
+ * first and last S rows and its S deviations as tagged words (tag tag0 +
+ * span number), then, with no barrier, waits for its neighbours' rows
+ * (into the span's last iterate's halo rows) and the K deviations of each
+ * sweep, and takes the sweeps' stop tests in order; sweeps after the one
+ * that stops the loop were speculative and are dropped.  Two planes
+ * suffice: a workgroup overwrites span k's words in span k + 2, after every
+ * reader has consumed them (it needed their span k + 1 deviations first).
+ * Returns the sweep t of the last span at which the loop stopped, with X0
+ * and X1 as at that span's start (the iterate: X1 for t = 0, X0 for t = 1;
+ * the one before: X0 (S = 1) or SV (S = 2) for t = 0, X1 for t = 1), -1 if
+ * the loop did not run, -2 if a wait timed out or another workgroup gave
+ * up (the abort flag is then up).  KMAX: the largest K. */
+template <int S, int KMAX, bool DIV>
+static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, double rdev, int &count, int &ri,
+                                  unsigned tag0, int *relax_flag, int *abort_flag, long long limit, double *red,
+                                  double *sums, int *xfail) {
+    static_assert(S == 1 || S == 2, "spans of one or two sweeps");
+    constexpr int MD = (S * KMAX + 63) / 64;        /* deviations per lane of the polling wave */
+    if (!(rdev > thr)) return -1;
+    const int L = G.L, bw = G.bw, rows = G.rows, b = G.b, K = G.K;
+    const bool up = b > 0, dn = b < K - 1;          /* neighbouring bands above / below */
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const float dx = P.dx;
+    for (unsigned span = 1;; span++) {
+        const unsigned tag = tag0 + span;
+        const int par = span & 1;
+        for (int t = 0; t < S; t++) {
+            const float *u = (t & 1) ? G.X1 : G.X0;
+            float *un = (t & 1) ? G.X0 : G.X1;
+            double acc = 0.0;
+            for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
+                const int q = (k / bw + S) * L + (k % bw + 1);
+                float o;
+                const float v = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, P.wi, P.rdt, dx,
+                                            o);
+                un[q] = v;
+                if (S > 1 && t == 0) G.SV[q] = o;
+                if constexpr (DIV) acc += fabs((double)(v - o)) * dx * dx / P.ncur;
+                else acc += fabs((double)(v - o)) * dx * dx;
+            }
+            if (S > 1) {
+                const int ga = up ? S - 1 - t : 0, gb = dn ? S - 1 - t : 0;
+                for (int k = threadIdx.x; k < (ga + gb) * bw; k += blockDim.x) {
+                    const int r = k / bw, l = r < ga ? S - ga + r : S + rows + (r - ga);
+                    const int q = l * L + (k % bw + 1);
+                    float o;
+                    un[q] = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, P.wi, P.rdt, dx, o);
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+            if (lane == 0) red[t * 16 + wv] = acc;
+            __syncthreads();
+        }
+        float *out = (S & 1) ? G.X1 : G.X0;         /* the span's last iterate */
+        for (int k = threadIdx.x; k < 2 * S * bw; k += blockDim.x) {
+            const int e = k / (S * bw), s = (k / bw) % S, c = k % bw;
+            const int l = e ? rows + s : S + s;     /* the first / last S own rows */
+            g_tput(&G.xr[((((size_t)par * K + b) * 2 + e) * S + s) * bw + c], __float_as_uint(out[l * L + c + 1]), tag);
+        }
+        if ((int)threadIdx.x < S) {
+            const int t = threadIdx.x;
+            double d = 0.0;
+            for (int k = 0; k < nw; k++) d += red[t * 16 + k];
+            const unsigned long long tb = (unsigned long long)__double_as_longlong(d);
+            g_tput(&G.xd[(((size_t)par * K + b) * S + t) * 2 + 0], (unsigned)tb, tag);
+            g_tput(&G.xd[(((size_t)par * K + b) * S + t) * 2 + 1], (unsigned)(tb >> 32), tag);
+        }
+        bool ok = true;
+        if (threadIdx.x < 64) {
+            /* the group's deviations, every first load in flight before the first wait */
+            unsigned long long lo[MD], hi[MD];
+#pragma unroll
+            for (int m = 0; m < MD; m++) {
+                const int j = lane + 64 * m;
+                lo[m] = hi[m] = 0ull;
+                if (j < S * K) {
+                    const unsigned long long *a = &G.xd[(((size_t)par * K + j % K) * S + j / K) * 2];
+                    lo[m] = g_tget(a);
+                    hi[m] = g_tget(a + 1);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MD; m++) {
+                const int j = lane + 64 * m;
+                if (j < S * K) {
+                    const unsigned long long *a = &G.xd[(((size_t)par * K + j % K) * S + j / K) * 2];
+                    ok = ok && g_twait(a, tag, lo[m], abort_flag, limit) && g_twait(a + 1, tag, hi[m], abort_flag, limit);
+                    sums[(j / K) * K + j % K] =
+                        __longlong_as_double((long long)(((hi[m] & 0xffffffffull) << 32) | (lo[m] & 0xffffffffull)));
+                }
+            }
+        } else {
+            /* the neighbours' rows into the halo rows of `out`, four words
+             * per thread in flight, then the waits */
+            const int nq = 2 * S * bw, step = (int)blockDim.x - 64;
+            for (int q0 = (int)threadIdx.x - 64; q0 < nq; q0 += 4 * step) {
+                unsigned long long w[4];
+                const unsigned long long *ap[4];
+                int la[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const int q = q0 + m * step;
+                    la[m] = -1; ap[m] = nullptr; w[m] = 0ull;
+                    if (q < nq) {
+                        const int e = q / (S * bw), s = (q / bw) % S, c = q % bw;
+                        if (e ? dn : up) {          /* else static rows there */
+                            /* above: b - 1's last S rows; below: b + 1's first */
+                            ap[m] = &G.xr[((((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * S + s) * bw + c];
+                            la[m] = (e ? S + rows + s : s) * L + c + 1;
+                            w[m] = g_tget(ap[m]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    if (la[m] < 0) continue;
+                    ok = ok && g_twait(ap[m], tag, w[m], abort_flag, limit);
+                    out[la[m]] = __uint_as_float((unsigned)w[m]);
+                }
+            }
+        }
+        if (!ok) *xfail = 1;
+        __syncthreads();
+        if (*xfail) {
+            if (threadIdx.x == 0) atomicExch(abort_flag, 1);
+            return -2;
+        }
+        for (int t = 0; t < S; t++) {
+            rdev = g_lds_sum(sums + t * K, K);
+            count++;
+            if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; return t; }
+            ri++;
+            if (!(rdev > thr)) return t;
+        }
+        if (S & 1) { float *x = G.X0; G.X0 = G.X1; G.X1 = x; }
+    }
+}
+/* the iterate the loop stopped at and the one before (g_band_loop's return t >= 0) */
+template <int S>
+static __device__ inline const float *g_band_last(const GBand &G, int t) { return (t & 1) ? G.X0 : G.X1; }
+template <int S>
+static __device__ inline const float *g_band_prev(const GBand &G, int t) {
+    return t == 0 ? (S == 1 ? G.X0 : G.SV) : G.X1;
+}
+
 /* reshaping_cells (cell_phase_functions.cpp:447-580) for FP32 state with K
  * workgroups per cell (blockIdx.x = n K + b), workgroup b on window rows
- * [b W / K, (b + 1) W / K) held in LDS with the row above and below:
- * both iterates and the normals (the window's one-pixel ring, phi around
- * the window, is static and in both iterates).  Each sweep is the flux of
- * g_relax_cells_mb on the same operands in the same order, so the
- * iterates, the per-workgroup deviations (same pixel-to-thread split, same
- * block sum) and the in-order sum over the K parts equal that kernel's bit
- * for bit.  After a sweep a workgroup publishes its first and last rows
- * and its deviation as tagged words (tag base + sweep number), then waits
- * for its neighbours' rows (into the new iterate's halo rows) and for the
- * cell's K deviations.  At the end phin's window gets the last iterate and
- * phi's the second-to-last (own rows).  xw: [nc][2 parities][K][2][W] row
- * words, then [nc][2][K][2 halves] deviation words.  Dynamic LDS: 4 (rmax
- * + 2)(W + 2) reals, rmax >= the largest band. */
+ * [b W / K, (b + 1) W / K) held in LDS (g_band_loop; the window's one-pixel
+ * ring, phi around the window, is the static surround).  The iterates, the
+ * per-workgroup deviations and their in-order sum equal g_relax_cells_mb's
+ * bit for bit.  At the end phin's window gets the last iterate and phi's
+ * the second-to-last (own rows).  xw: [nc] groups of g_band_words(S, K, W)
+ * words.  Dynamic LDS: g_band_lds(S, rmax, W), rmax >= the largest band;
+ * K <= 64. */
+template <int S>
 __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__restrict__ phi,
                                                           float *__restrict__ phin, const float *__restrict__ jx,
                                                           const float *__restrict__ jy, const int *__restrict__ xc,
@@ -1063,98 +1259,48 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
                                                           int *__restrict__ counts, int *__restrict__ cmax,
                                                           int *__restrict__ relax_flag, int *__restrict__ abort_flag,
                                                           long long limit, int K, int rmax) {
-    __shared__ double red[16];
-    __shared__ double sums[64];
+    __shared__ double red[S * 16];
+    __shared__ double sums[S * 64];
     __shared__ int xfail;
-    extern __shared__ char g_dyn[];
-    const int N = P.N, zs = P.zs, W = 2 * zs, W2 = W + 2, LR = rmax + 2;
+    const int N = P.N, zs = P.zs, W = 2 * zs, L = W + 2;
     const int n = blockIdx.x / K, b = blockIdx.x % K;
     const size_t NN = (size_t)N * N;
     float *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
     const float *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
     const int x0 = xc[n] - zs, y0 = yc[n] - zs;              /* window origin on the grid */
     const int r0 = b * W / K, r1 = (b + 1) * W / K, rows = r1 - r0;
-    /* LDS row l = window row r0 - 1 + l, LDS column c = window column c - 1 */
-    float *U0 = (float *)g_dyn, *U1 = U0 + (size_t)LR * W2, *JX = U1 + (size_t)LR * W2, *JY = JX + (size_t)LR * W2;
-    unsigned long long *xr = xw + (size_t)n * 2 * K * 2 * W;
-    unsigned long long *xd = xw + (size_t)P.nc * 2 * K * 2 * W + (size_t)n * 2 * K * 2;
-    for (int k = threadIdx.x; k < (rows + 2) * W2; k += blockDim.x) {
-        const int l = k / W2, c = k % W2, wr = r0 - 1 + l, wc = c - 1;
+    GBand G;
+    G.L = L; G.bw = W; G.rows = rows; G.b = b; G.K = K;
+    g_band_lds_split(G, S, rmax);
+    G.xr = xw + (size_t)n * g_band_words(S, K, W);
+    G.xd = G.xr + (size_t)2 * K * 2 * S * W;
+    /* the window rows r0 - S .. r1 + S - 1 there are, and the ring rows -1
+     * and W: ring pixels from phi, the window from phin */
+    const int l0 = b == 0 ? S - 1 : 0, l1 = b == K - 1 ? S + rows + 1 : rows + 2 * S;
+    for (int k = threadIdx.x; k < (l1 - l0) * L; k += blockDim.x) {
+        const int l = l0 + k / L, c = k % L, wr = r0 - S + l, wc = c - 1;
         const size_t g = (size_t)(x0 + wr) * N + (y0 + wc);
         const bool ring = wr < 0 || wr >= W || wc < 0 || wc >= W;
         const float v = ring ? ph[g] : pn[g];
-        U0[k] = v; U1[k] = v;
-        JX[k] = jxn[g]; JY[k] = jyn[g];
+        const int q = l * L + c;
+        G.X0[q] = v; G.X1[q] = v;
+        G.JX[q] = jxn[g]; G.JY[q] = jyn[g];
     }
     if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
     __syncthreads();
-    const float dx = P.dx;
-    double rdev = rdev0;
     int ri = 0, count = 0;
-    while (rdev > thr) {
-        const float *src = (count & 1) ? U1 : U0;
-        float *dst = (count & 1) ? U0 : U1;
-        const int par = count & 1;
-        const unsigned tag = tag0 + (unsigned)count + 1u;
-        double acc = 0.0;
+    const int last = g_band_loop<S, 64, true>(P, G, thr, rdev0, count, ri, tag0, relax_flag, abort_flag, limit, red,
+                                              sums, &xfail);
+    if (last == -2) return;
+    if (last >= 0) {
+        /* phin <- the last iterate, phi <- the one before; own rows only */
+        const float *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
         for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
-            const int q = (k / W + 1) * W2 + (k % W + 1);
-            float o;
-            const float v = g_flux_rows(src + q - W2, src + q, src + q + W2, JX, JY, (size_t)q, W2, P.wi, P.rdt,
-                                        dx, o);
-            dst[q] = v;
-            acc += fabs((double)(v - o)) * dx * dx / P.ncur;
+            const int q = (k / W + S) * L + (k % W + 1);
+            const size_t g = (size_t)(x0 + r0 + k / W) * N + (y0 + k % W);
+            pn[g] = vl[q];
+            ph[g] = vp[q];
         }
-        __syncthreads();
-        for (int k = threadIdx.x; k < 2 * W; k += blockDim.x) {
-            const int e = k / W, c = k % W, l = e ? rows : 1;
-            g_tput(&xr[(((size_t)par * K + b) * 2 + e) * W + c], __float_as_uint(dst[l * W2 + c + 1]), tag);
-        }
-        const double t = gblock_sum(acc, red);
-        if (threadIdx.x == 0) {
-            const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
-            g_tput(&xd[((size_t)par * K + b) * 2 + 0], (unsigned)tb, tag);
-            g_tput(&xd[((size_t)par * K + b) * 2 + 1], (unsigned)(tb >> 32), tag);
-        }
-        bool ok = true;
-        if (threadIdx.x < 64) {                   /* the cell's K deviations, one per lane */
-            const int b2 = threadIdx.x;
-            if (b2 < K) {
-                unsigned long long lo = g_tget(&xd[((size_t)par * K + b2) * 2 + 0]);
-                unsigned long long hi = g_tget(&xd[((size_t)par * K + b2) * 2 + 1]);
-                ok = g_twait(&xd[((size_t)par * K + b2) * 2 + 0], tag, lo, abort_flag, limit) &&
-                     g_twait(&xd[((size_t)par * K + b2) * 2 + 1], tag, hi, abort_flag, limit);
-                sums[b2] = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
-            }
-        } else {                                  /* the neighbours' rows into the new iterate's halo rows */
-            const int nq = 2 * W;
-            for (int k = (int)threadIdx.x - 64; k < nq; k += (int)blockDim.x - 64) {
-                const int e = k / W, c = k % W;   /* e 0: the row above (b - 1's last), 1: below (b + 1's first) */
-                if ((e == 0 && b == 0) || (e == 1 && b == K - 1)) continue;   /* the static ring rows */
-                const unsigned long long *a = &xr[(((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * W + c];
-                unsigned long long w = g_tget(a);
-                ok = ok && g_twait(a, tag, w, abort_flag, limit);
-                dst[(e ? rows + 1 : 0) * W2 + c + 1] = __uint_as_float((unsigned)w);
-            }
-        }
-        if (!ok) xfail = 1;
-        __syncthreads();
-        if (xfail) {
-            if (threadIdx.x == 0) atomicExch(abort_flag, 1);
-            return;
-        }
-        rdev = g_lds_sum(sums, K);
-        count++;
-        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :565-568 */
-        ri++;
-    }
-    /* phin <- the last iterate, phi <- the one before; own rows only */
-    const float *last = (count & 1) ? U1 : U0, *prev = (count & 1) ? U0 : U1;
-    for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
-        const int q = (k / W + 1) * W2 + (k % W + 1);
-        const size_t g = (size_t)(x0 + r0 + k / W) * N + (y0 + k % W);
-        pn[g] = last[q];
-        if (count > 0) ph[g] = prev[q];
     }
     if (threadIdx.x == 0 && b == 0) {
         counts[n] = count;
@@ -1260,23 +1406,17 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
     if (threadIdx.x == 0 && b == 0) out[0] = count;
 }
 
-/* the most dynamic LDS a banded FP32 loop takes (the static part adds a
- * little over 1 KB; a compute unit has 160 KB) */
-#define ORG_GEN_LDS_MAX (140 * 1024)
-
 /* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop as g_lum_loop_mb, for
- * FP32 state, with each workgroup's band of box rows held in LDS with the
- * row above and below (both iterates and the normals; the rows and
- * columns just outside the box are static and the same in both buffers).
- * Same flux on the same operands, same per-thread pixel split and block
- * sum, in-order sum over the K parts: the iterates equal g_lum_loop_mb's
- * bit for bit.  After a sweep a workgroup publishes its first and last
- * rows and its deviation as tagged words (tag base + sweeps of this
- * launch), then waits for its neighbours' rows and the K deviations.  At
- * the end both buffers get the band's rows of the iterates they hold, as
- * the sweeps of g_lum_loop_mb leave them.  xw: [2 parities][K][2][N] row
- * words, then [2][K][2 halves] deviation words.  Dynamic LDS: 4 (rmax + 2)
- * (box width + 2) reals, rmax >= the largest band. */
+ * FP32 state, with workgroup b's band of box rows [r0, r1) held in LDS
+ * (g_band_loop; the rows and columns just outside the box are the static
+ * surround, the same in both buffers).  The iterates, the per-workgroup
+ * deviations and their in-order sum equal g_lum_loop_mb's bit for bit.  At
+ * the end both buffers get the band's rows of the iterates they hold as
+ * the sweeps of g_lum_loop_mb leave them (X_count in bufA for odd count).
+ * xw: g_band_words(S, K, box width) words.  Dynamic LDS: g_band_lds(S,
+ * rmax, box width), rmax >= the largest band; every band has S rows or
+ * more; K <= 128. */
+template <int S>
 __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__restrict__ bufA,
                                                        float *__restrict__ bufB, const float *__restrict__ jx,
                                                        const float *__restrict__ jy, const int *__restrict__ box,
@@ -1284,85 +1424,44 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
                                                        unsigned long long *__restrict__ xw, unsigned tag0,
                                                        int *__restrict__ out, int *__restrict__ relax_flag,
                                                        int *__restrict__ abort_flag, long long limit, int K, int rmax) {
-    __shared__ double red[16];
-    __shared__ double sums[128];
+    __shared__ double red[S * 16];
+    __shared__ double sums[S * 128];
     __shared__ int xfail;
-    extern __shared__ char g_dyn[];
     const int N = P.N, b = blockIdx.x;
     const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
-    const int bw = bj1 - bj0, bh = bi1 - bi0, L = bw + 2, LR = rmax + 2;
+    const int bw = bj1 - bj0, bh = bi1 - bi0, L = bw + 2;
     const int r0 = bi0 + b * bh / K, r1 = bi0 + (b + 1) * bh / K, rows = r1 - r0;
-    /* LDS row l = grid row r0 - 1 + l, LDS column c = grid column bj0 - 1 + c */
-    float *UA = (float *)g_dyn, *UB = UA + (size_t)LR * L, *JX = UB + (size_t)LR * L, *JY = JX + (size_t)LR * L;
-    unsigned long long *xr = xw, *xd = xw + (size_t)2 * K * 2 * N;
-    for (int k = threadIdx.x; k < (rows + 2) * L; k += blockDim.x) {
-        const size_t g = (size_t)(r0 - 1 + k / L) * N + (bj0 - 1 + k % L);
-        UA[k] = bufA[g]; UB[k] = bufB[g];
-        JX[k] = jx[g]; JY[k] = jy[g];
+    GBand G;
+    G.L = L; G.bw = bw; G.rows = rows; G.b = b; G.K = K;
+    g_band_lds_split(G, S, rmax);
+    G.xr = xw;
+    G.xd = xw + (size_t)2 * K * 2 * S * bw;
+    /* X0: X_count (bufA for odd count), X1: the other buffer; grid row r0 -
+     * S + l, column bj0 - 1 + c, for the box rows there are and the static
+     * rows bi0 - 1 and bi1 */
+    const float *cur = (count & 1) ? bufA : bufB, *oth = (count & 1) ? bufB : bufA;
+    const int l0 = b == 0 ? S - 1 : 0, l1 = b == K - 1 ? S + rows + 1 : rows + 2 * S;
+    for (int k = threadIdx.x; k < (l1 - l0) * L; k += blockDim.x) {
+        const int l = l0 + k / L, c = k % L;
+        const size_t g = (size_t)(r0 - S + l) * N + (bj0 - 1 + c);
+        const int q = l * L + c;
+        G.X0[q] = cur[g]; G.X1[q] = oth[g];
+        G.JX[q] = jx[g]; G.JY[q] = jy[g];
     }
     if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
     __syncthreads();
-    const float dx = P.dx;
-    const int c0 = count;
-    while (rdev > thr) {
-        const float *u = (count & 1) ? UA : UB;      /* X_count: bufA for odd count */
-        float *un = (count & 1) ? UB : UA;
-        const int par = count & 1;
-        const unsigned tag = tag0 + (unsigned)(count - c0) + 1u;
-        double acc = 0.0;
+    const int last = g_band_loop<S, 128, false>(P, G, thr, rdev, count, ri, tag0, relax_flag, abort_flag, limit, red,
+                                                sums, &xfail);
+    if (last == -2) return;
+    if (last >= 0) {
+        const float *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
+        float *dl = (count & 1) ? bufA : bufB, *dp = (count & 1) ? bufB : bufA;
         for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
-            const int q = (k / bw + 1) * L + (k % bw + 1);
-            float o;
-            const float v = g_flux_rows(u + q - L, u + q, u + q + L, JX, JY, (size_t)q, L, P.wi, P.rdt, dx, o);
-            un[q] = v;
-            acc += fabs((double)(v - o)) * dx * dx;
+            const int q = (k / bw + S) * L + (k % bw + 1);
+            const size_t g = (size_t)(r0 + k / bw) * N + (bj0 + k % bw);
+            dl[g] = vl[q];
+            dp[g] = vp[q];
         }
-        __syncthreads();
-        for (int k = threadIdx.x; k < 2 * bw; k += blockDim.x) {
-            const int e = k / bw, c = k % bw, l = e ? rows : 1;
-            g_tput(&xr[(((size_t)par * K + b) * 2 + e) * N + c], __float_as_uint(un[l * L + c + 1]), tag);
-        }
-        const double t = gblock_sum(acc, red);
-        if (threadIdx.x == 0) {
-            const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
-            g_tput(&xd[((size_t)par * K + b) * 2 + 0], (unsigned)tb, tag);
-            g_tput(&xd[((size_t)par * K + b) * 2 + 1], (unsigned)(tb >> 32), tag);
-        }
-        bool ok = true;
-        if (threadIdx.x < 64) {                   /* the K deviations, lanes b2 and b2 + 64 */
-            for (int b2 = threadIdx.x; b2 < K; b2 += 64) {
-                unsigned long long lo = g_tget(&xd[((size_t)par * K + b2) * 2 + 0]);
-                unsigned long long hi = g_tget(&xd[((size_t)par * K + b2) * 2 + 1]);
-                ok = ok && g_twait(&xd[((size_t)par * K + b2) * 2 + 0], tag, lo, abort_flag, limit) &&
-                     g_twait(&xd[((size_t)par * K + b2) * 2 + 1], tag, hi, abort_flag, limit);
-                sums[b2] = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
-            }
-        } else {                                  /* the neighbours' rows into the new iterate's halo rows */
-            for (int k = (int)threadIdx.x - 64; k < 2 * bw; k += (int)blockDim.x - 64) {
-                const int e = k / bw, c = k % bw;
-                if ((e == 0 && b == 0) || (e == 1 && b == K - 1)) continue;   /* static rows outside the box */
-                const unsigned long long *a = &xr[(((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * N + c];
-                unsigned long long w = g_tget(a);
-                ok = ok && g_twait(a, tag, w, abort_flag, limit);
-                un[(e ? rows + 1 : 0) * L + c + 1] = __uint_as_float((unsigned)w);
-            }
-        }
-        if (!ok) xfail = 1;
-        __syncthreads();
-        if (xfail) {
-            if (threadIdx.x == 0) atomicExch(abort_flag, 1);
-            return;
-        }
-        rdev = g_lds_sum(sums, K);
-        count++;
-        if (ri > P.rimax) { if (threadIdx.x == 0 && b == 0) *relax_flag = 1; break; }   /* :862-865 */
-        ri++;
-    }
-    for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
-        const int q = (k / bw + 1) * L + (k % bw + 1);
-        const size_t g = (size_t)(r0 + k / bw) * N + (bj0 + k % bw);
-        bufA[g] = UA[q];
-        bufB[g] = UB[q];
     }
     if (threadIdx.x == 0 && b == 0) out[0] = count;
 }
@@ -1736,21 +1835,22 @@ struct GenCtx final : Generic {
             GHIP(hipGetDevice(&dev));
             GHIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
             /* one workgroup per compute unit at most (all resident), at
-             * least 4 window rows each, the deviations polled one per lane */
+             * least 4 window rows each, 64 deviations a sweep at most */
             kcl = std::min(std::min(ncu / nc, W / 4), 64);
             if (kcl >= 2) {
                 rmax_l = (W + kcl - 1) / kcl;
-                lds_l = 4 * (size_t)(rmax_l + 2) * (W + 2) * sizeof(float);
+                lds_l = g_band_lds(ORG_GEN_SPAN_CELLS, rmax_l, W);
                 if (lds_l > ORG_GEN_LDS_MAX) kcl = 0;
             } else {
                 kcl = 0;
             }
             if (kcl) {
-                const size_t words = (size_t)nc * 2 * kcl * 2 * W + (size_t)nc * 2 * kcl * 2;
+                const size_t words = (size_t)nc * g_band_words(ORG_GEN_SPAN_CELLS, kcl, W);
                 GTRY(A_(&mb_tw, words));
                 GTRY(A_(&mb_cmax, 1));
                 GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
-                GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds<ORG_GEN_SPAN_CELLS>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)lds_l));
             }
             /* the lumen/ECM loop: one band of box rows per workgroup, up to
@@ -1758,10 +1858,11 @@ struct GenCtx final : Generic {
              * the polling wave; each launch checks that its bands fit */
             klm = std::min(std::min(ncu, 128), N - 2);
             if (klm >= 2) {
-                const size_t words = (size_t)2 * klm * 2 * N + (size_t)2 * klm * 2;
+                const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N);
                 GTRY(A_(&mb_lw, words));
                 GHIP(hipMemset(mb_lw, 0, sizeof(unsigned long long) * words));
-                GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds<ORG_GEN_SPAN_LUM>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                          ORG_GEN_LDS_MAX));
             } else {
                 klm = 0;
@@ -1986,16 +2087,13 @@ struct GenCtx final : Generic {
                 (const R *)psi_, (const R *)hl, (const R *)he, (const double *)sum_e, lumn, ecmn);
         return ORG_OK;
     }
-    /* dynamic LDS of g_lum_loop_lds for bands of up to `rows` rows of a box
-     * `bw` columns wide */
-    static size_t lum_lds_bytes(int rows, int bw) { return 4 * (size_t)(rows + 2) * (bw + 2) * sizeof(float); }
-    /* workgroups of g_lum_loop_lds for an bh x bw box: as many as allowed,
-     * a band of at least one row each; 0 (the exchange-buffer loop runs)
-     * for FP64 state or when the bands do not fit in LDS */
+    /* workgroups of g_lum_loop_lds for a bh x bw box: as many as allowed,
+     * ORG_GEN_SPAN_LUM rows or more each; 0 (the exchange-buffer loop
+     * runs) for FP64 state or when the bands do not fit in LDS */
     int lum_lds_blocks(int bh, int bw) const {
         if (!std::is_same<R, float>::value || klm < 2) return 0;
-        const int kb = std::min(klm, bh);
-        if (kb < 2 || lum_lds_bytes((bh + kb - 1) / kb, bw) > ORG_GEN_LDS_MAX) return 0;
+        const int kb = std::min(klm, bh / ORG_GEN_SPAN_LUM);
+        if (kb < 2 || g_band_lds(ORG_GEN_SPAN_LUM, (bh + kb - 1) / kb, bw) > ORG_GEN_LDS_MAX) return 0;
         return kb;
     }
     int reshape_lumecm(int which, int *failed) override {
@@ -2041,7 +2139,8 @@ struct GenCtx final : Generic {
                     GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr + nc, 2);
                     GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
                     if constexpr (std::is_same<R, float>::value)
-                        GLAUNCH(g_lum_loop_lds, dim3(kb), dim3(1024), lum_lds_bytes(rows_max, bw), P, u, un,
+                        GLAUNCH(g_lum_loop_lds<ORG_GEN_SPAN_LUM>, dim3(kb), dim3(1024),
+                                g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw), P, u, un,
                                 (const float *)jlx, (const float *)jly, (const int *)mb_box, thr, rdev, count, ri,
                                 mb_lw, tag_next, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
                     int h[3] = {0, 0, 0};
@@ -2111,7 +2210,7 @@ struct GenCtx final : Generic {
             if (kcl) {
                 GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
                 GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)mb_cmax, 1);
-                GLAUNCH(g_relax_cells_lds, dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin, (const R *)jx,
+                GLAUNCH(g_relax_cells_lds<ORG_GEN_SPAN_CELLS>, dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin, (const R *)jx,
                         (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next, counts,
                         mb_cmax, relax, (int *)(mb_ctr + nc + 1), wait_limit, kcl, rmax_l);
                 int f[3] = {0, 0, 0};

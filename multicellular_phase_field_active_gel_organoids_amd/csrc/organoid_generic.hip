@@ -249,27 +249,43 @@ static __device__ double gblock_sum(double v, double *red) {
     return s;
 }
 
-/* find_centres_cells, useful_functions.cpp:193-227, then the window check
- * (window plus one pixel inside rows/columns 1..N-2). */
+/* find_centres_cells, useful_functions.cpp:193-227: the integer sums of i,
+ * j and the count over the pixels with phi > 0.5, workgroup (b, n) on rows
+ * b, b + gridDim.x, ... of cell n, added into acc[n] = {sum i, sum j,
+ * count} (integer sums: the same in any order).  acc zero on entry. */
 template <typename R>
-__global__ __launch_bounds__(1024) void g_centroid(GP<R> P, const R *__restrict__ phi, int *__restrict__ xc,
-                                                   int *__restrict__ yc, int *__restrict__ err) {
-    __shared__ long long red[48];
-    const int N = P.N, n = blockIdx.x;
-    const size_t NN = (size_t)N * N;
+__global__ __launch_bounds__(256) void g_centroid_part(GP<R> P, const R *__restrict__ phi,
+                                                       unsigned long long *__restrict__ acc) {
+    __shared__ long long red[3 * 4];
+    const int N = P.N, n = blockIdx.y;
+    const R *ph = phi + (size_t)n * N * N;
     long long xs = 0, ys = 0, cnt = 0;
-    for (size_t k = threadIdx.x; k < NN; k += blockDim.x)
-        if (phi[n * NN + k] > 0.5) { xs += (long long)(k / N); ys += (long long)(k % N); cnt += 1; }
+    for (int i = blockIdx.x; i < N; i += gridDim.x)
+        for (int j = threadIdx.x; j < N; j += blockDim.x)
+            if (ph[(size_t)i * N + j] > 0.5) { xs += i; ys += j; cnt += 1; }
     for (int o = 32; o > 0; o >>= 1) {
         xs += __shfl_down(xs, o, 64); ys += __shfl_down(ys, o, 64); cnt += __shfl_down(cnt, o, 64);
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { red[w] = xs; red[16 + w] = ys; red[32 + w] = cnt; }
+    if (lane == 0) { red[w] = xs; red[4 + w] = ys; red[8 + w] = cnt; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        long long X = 0, Y = 0, C = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { X += red[k]; Y += red[16 + k]; C += red[32 + k]; }
-        if (C == 0) { atomicCAS(err, 0, ORG_EEMPTY_CELL); return; }
+    if (threadIdx.x < 3) {
+        long long v = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) v += red[4 * threadIdx.x + k];
+        if (v) atomicAdd(&acc[3 * n + threadIdx.x], (unsigned long long)v);
+    }
+}
+
+/* the centres from g_centroid_part's sums (integer division as the
+ * reference's), then the window check (window plus one pixel inside rows /
+ * columns 1..N-2) */
+template <typename R>
+__global__ void g_centroid_fin(GP<R> P, const unsigned long long *__restrict__ acc, int *__restrict__ xc,
+                               int *__restrict__ yc, int *__restrict__ err) {
+    const int N = P.N;
+    for (int n = threadIdx.x; n < P.nc; n += blockDim.x) {
+        const long long X = (long long)acc[3 * n], Y = (long long)acc[3 * n + 1], C = (long long)acc[3 * n + 2];
+        if (C == 0) { atomicCAS(err, 0, ORG_EEMPTY_CELL); continue; }
         const int x = (int)(X / C), y = (int)(Y / C);
         xc[n] = x; yc[n] = y;
         const int x0 = x - P.zs, y0 = y - P.zs;
@@ -361,8 +377,20 @@ __global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ h
     const size_t NN = (size_t)P.N * P.N;
     const int n = blockIdx.x;
     const R dx = P.dx;
+    const R *h = hf + (size_t)n * NN;
     double acc = 0.0;
-    for (size_t k = threadIdx.x; k < NN; k += blockDim.x) acc += (double)(hf[(size_t)n * NN + k] * dx * dx);
+    /* each thread's pixels k = t, t + 1024, ... added in that order, the
+     * loads of 16 of them in flight together */
+    constexpr int U = 16;
+    size_t k = threadIdx.x;
+    for (; k + (size_t)(U - 1) * blockDim.x < NN; k += (size_t)U * blockDim.x) {
+        R v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = h[k + (size_t)u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += (double)(v[u] * dx * dx);
+    }
+    for (; k < NN; k += blockDim.x) acc += (double)(h[k] * dx * dx);
     const double tot = gblock_sum(acc, red);
     if (threadIdx.x == 0) {
         s[n] = tot;
@@ -1494,11 +1522,23 @@ __global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ 
             i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
         }
     }
-    if (i1 >= 0) {
-        atomicMin(&box[0], max(1, i0 - 1));
-        atomicMax(&box[1], min(N - 1, i1 + 2));
-        atomicMin(&box[2], max(1, j0 - 1));
-        atomicMax(&box[3], min(N - 1, j1 + 2));
+    /* the workgroup's bounds (min / max in any order), then one atomic each */
+    __shared__ int bb[4];
+    if (threadIdx.x == 0) { bb[0] = N; bb[1] = -1; bb[2] = N; bb[3] = -1; }
+    for (int o = 32; o > 0; o >>= 1) {
+        i0 = min(i0, __shfl_xor(i0, o, 64)); i1 = max(i1, __shfl_xor(i1, o, 64));
+        j0 = min(j0, __shfl_xor(j0, o, 64)); j1 = max(j1, __shfl_xor(j1, o, 64));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && i1 >= 0) {
+        atomicMin(&bb[0], i0); atomicMax(&bb[1], i1); atomicMin(&bb[2], j0); atomicMax(&bb[3], j1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bb[1] >= 0) {
+        atomicMin(&box[0], max(1, bb[0] - 1));
+        atomicMax(&box[1], min(N - 1, bb[1] + 2));
+        atomicMin(&box[2], max(1, bb[2] - 1));
+        atomicMax(&box[3], min(N - 1, bb[3] + 2));
     }
 }
 
@@ -1758,6 +1798,7 @@ struct GenCtx final : Generic {
     int klm = 0;
     unsigned long long *mb_lw = nullptr;
     int *mb_cmax = nullptr;
+    unsigned long long *cen_acc = nullptr;   /* the centroid sums, [nc][3] */
     unsigned tag_next = 1;
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
@@ -1825,6 +1866,7 @@ struct GenCtx final : Generic {
         GTRY(A_(&f0, nc)); GTRY(A_(&s_, nc)); GTRY(A_(&s0, nc)); GTRY(A_(&sum_e, 1));
         GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
         GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&err, 1)); GTRY(A_(&counts, nc)); GTRY(A_(&relax, 1));
+        GTRY(A_(&cen_acc, (size_t)3 * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
@@ -2018,7 +2060,10 @@ struct GenCtx final : Generic {
     }
     int find_centres() override {
         GHIP(hipMemsetAsync(err, 0, sizeof(int), s));
-        GLAUNCH((g_centroid<R>), dim3(nc), dim3(1024), 0, P, (const R *)phi, xc, yc, err);
+        GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
+        GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 512 / nc)), nc), dim3(256), 0, P, (const R *)phi,
+                cen_acc);
+        GLAUNCH((g_centroid_fin<R>), dim3(1), dim3(64), 0, P, (const unsigned long long *)cen_acc, xc, yc, err);
         int e = 0;
         GHIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
         GHIP(hipStreamSynchronize(s));

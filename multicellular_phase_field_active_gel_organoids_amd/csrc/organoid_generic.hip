@@ -666,35 +666,74 @@ __global__ void g_lum_normals(GP<R> P, const R *__restrict__ nw, const R *__rest
     }
 }
 
-/* The conservative interface flux of both reshaping loops
- * (cell_phase_functions.cpp:511-547 and :802-838) at (i, j): the new value
- * of un given the relaxed field u and the normals, row stride N. */
+/* The loop-invariant factors of the reshaping flux (cell_phase_functions.cpp:
+ * 511-547 and :802-838), each formed as the reference's expression forms it
+ * (left to right, FP contraction off): a = wi wi (rdt / dx / dx) in R, and
+ * in double a4 = a 0.25 and c = rdt wi sqrt(2) / dx 0.5.  Formed once per
+ * kernel: within a loop the compiler cannot hoist them itself (they are read
+ * through a reference the loop's stores might alias), and c alone is a
+ * double division. */
 template <typename R>
-static __device__ __forceinline__ R g_reshape(R un, const R *__restrict__ u, const R *__restrict__ jx,
-                                              const R *__restrict__ jy, int N, size_t q, R wi, R rdt, R dx) {
-#define U(a, b) u[q + (long)(a) * N + (b)]
-#define JX(a, b) jx[q + (long)(a) * N + (b)]
-#define JY(a, b) jy[q + (long)(a) * N + (b)]
-    un += 0.5 * (JX(1, 0) + JX(0, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(1, 0) - U(0, 0)) * 0.5 * (JX(1, 0) + JX(0, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(0, 1) - U(1, -1) - U(0, -1)) * 0.5 * (JY(1, 0) + JY(0, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(1, 0) * (1.0 - U(1, 0)) + U(0, 0) * (1.0 - U(0, 0))));
-    un += 0.5 * (JY(0, 1) + JY(0, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 1) - U(0, 0)) * 0.5 * (JY(0, 1) + JY(0, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(1, 0) - U(-1, 1) - U(-1, 0)) * 0.5 * (JX(0, 1) + JX(0, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 1) * (1.0 - U(0, 1)) + U(0, 0) * (1.0 - U(0, 0))));
-    un -= 0.5 * (JX(0, 0) + JX(-1, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(-1, 0)) * 0.5 * (JX(0, 0) + JX(-1, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(0, 1) + U(-1, 1) - U(0, -1) - U(-1, -1)) * 0.5 * (JY(0, 0) + JY(-1, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(-1, 0) * (1.0 - U(-1, 0))));
-    un -= 0.5 * (JY(0, 0) + JY(0, -1)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(0, -1)) * 0.5 * (JY(0, 0) + JY(0, -1))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 0) + U(1, -1) - U(-1, 0) - U(-1, -1)) * 0.5 * (JX(0, 0) + JX(0, -1))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(0, -1) * (1.0 - U(0, -1))));
-    return un;
-#undef U
+struct GFluxK {
+    R a;
+    double a4, c;
+};
+template <typename R>
+static __device__ __forceinline__ GFluxK<R> g_flux_k(R wi, R rdt, R dx) {
+    GFluxK<R> k;
+    k.a = wi * wi * (rdt / dx / dx);
+    k.a4 = k.a * 0.25;
+    k.c = rdt * wi * sqrt(2.0) / dx * 0.5;
+    return k;
+}
+
+/* The flux through the face between a pixel and its neighbour at row
+ * offset +1 (g_face_x) or column offset +1 (g_face_y), with the relaxed
+ * field read through U(a, b) (offsets from the pixel) and the normals at
+ * jx / jy + qj (row stride sj): one of the four face terms of the reshaping
+ * update (cell_phase_functions.cpp:511-547, :802-838).  The neighbour's
+ * opposite term takes the same operands in the same order, so it is the
+ * same value: the faces can be computed once and shared. */
+#define JX(a, b) jx[qj + (long)(a) * sj + (b)]
+#define JY(a, b) jy[qj + (long)(a) * sj + (b)]
+template <typename R, typename FU>
+static __device__ __forceinline__ double g_face_x(FU U, const R *__restrict__ jx, const R *__restrict__ jy, size_t qj,
+                                                  int sj, const GFluxK<R> &K) {
+    return 0.5 * (JX(1, 0) + JX(0, 0)) * (
+        K.a * (U(1, 0) - U(0, 0)) * 0.5 * (JX(1, 0) + JX(0, 0))
+        + K.a4 * (U(1, 1) + U(0, 1) - U(1, -1) - U(0, -1)) * 0.5 * (JY(1, 0) + JY(0, 0))
+        - K.c * (U(1, 0) * (1.0 - U(1, 0)) + U(0, 0) * (1.0 - U(0, 0))));
+}
+template <typename R, typename FU>
+static __device__ __forceinline__ double g_face_y(FU U, const R *__restrict__ jx, const R *__restrict__ jy, size_t qj,
+                                                  int sj, const GFluxK<R> &K) {
+    return 0.5 * (JY(0, 1) + JY(0, 0)) * (
+        K.a * (U(0, 1) - U(0, 0)) * 0.5 * (JY(0, 1) + JY(0, 0))
+        + K.a4 * (U(1, 1) + U(1, 0) - U(-1, 1) - U(-1, 0)) * 0.5 * (JX(0, 1) + JX(0, 0))
+        - K.c * (U(0, 1) * (1.0 - U(0, 1)) + U(0, 0) * (1.0 - U(0, 0))));
+}
 #undef JX
 #undef JY
+
+/* The interface flux at a pixel, the relaxed field read through U(a, b):
+ * the faces below and right added, above and left subtracted, each rounded
+ * to R as the reference's `+=` / `-=` do. */
+template <typename R, typename FU>
+static __device__ __forceinline__ R g_flux(R un, FU U, const R *__restrict__ jx, const R *__restrict__ jy, size_t qj,
+                                           int sj, const GFluxK<R> &K) {
+    un += g_face_x(U, jx, jy, qj, sj, K);
+    un += g_face_y(U, jx, jy, qj, sj, K);
+    un -= g_face_x([&](int a, int b) { return U(a - 1, b); }, jx, jy, qj - sj, sj, K);
+    un -= g_face_y([&](int a, int b) { return U(a, b - 1); }, jx, jy, qj - 1, sj, K);
+    return un;
+}
+
+/* The conservative interface flux of both reshaping loops at grid index q
+ * (row stride N): the new value of un given the relaxed field u. */
+template <typename R>
+static __device__ __forceinline__ R g_reshape(R un, const R *__restrict__ u, const R *__restrict__ jx,
+                                              const R *__restrict__ jy, int N, size_t q, const GFluxK<R> &K) {
+    return g_flux(un, [&](int a, int b) { return u[q + (long)a * N + b]; }, jx, jy, q, N, K);
 }
 
 /* One sweep of reshaping_lumen_or_ECM's loop (cell_phase_functions.cpp:
@@ -708,6 +747,7 @@ __global__ void g_lum_sweep(GP<R> P, const R *__restrict__ u, R *__restrict__ un
                             const R *__restrict__ jy, R side) {
     const int N = P.N;
     const size_t NN = (size_t)N * N;
+    const GFluxK<R> K = g_flux_k(P.wi, P.rdt, P.dx);
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(q / N), j = (int)(q % N);
         if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
@@ -715,7 +755,7 @@ __global__ void g_lum_sweep(GP<R> P, const R *__restrict__ u, R *__restrict__ un
             un[q] = corner ? (R)0.0 : side;
             continue;
         }
-        un[q] = g_reshape(u[q], u, jx, jy, N, q, P.wi, P.rdt, P.dx);
+        un[q] = g_reshape(u[q], u, jx, jy, N, q, K);
     }
 }
 
@@ -783,6 +823,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells(GP<R> P, R *__restrict__ p
     const R *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
     const int x0 = xc[n] - zs, y0 = yc[n] - zs;
     const R dx = P.dx;
+    const GFluxK<R> K = g_flux_k(P.wi, P.rdt, dx);
     double rdev = rdev0;
     int ri = 0, count = 0;
     while (rdev > thr) {
@@ -793,7 +834,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells(GP<R> P, R *__restrict__ p
         __syncthreads();
         for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
             const size_t q = GG(x0 + k / W, y0 + k % W);
-            pn[q] = g_reshape(pn[q], ph, jxn, jyn, N, q, P.wi, P.rdt, dx);
+            pn[q] = g_reshape(pn[q], ph, jxn, jyn, N, q, K);
         }
         __syncthreads();
         double acc = 0.0;
@@ -872,44 +913,17 @@ static __device__ __forceinline__ T g_xl(const T *p) {
     return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* The interface flux of g_reshape with the relaxed field read through U(a, b)
- * (offsets from the pixel) and the normals on the grid (row stride sj). */
-template <typename R, typename FU>
-static __device__ __forceinline__ R g_flux(R un, FU U, const R *__restrict__ jx, const R *__restrict__ jy, size_t qj,
-                                           int sj, R wi, R rdt, R dx) {
-#define JX(a, b) jx[qj + (long)(a) * sj + (b)]
-#define JY(a, b) jy[qj + (long)(a) * sj + (b)]
-    un += 0.5 * (JX(1, 0) + JX(0, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(1, 0) - U(0, 0)) * 0.5 * (JX(1, 0) + JX(0, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(0, 1) - U(1, -1) - U(0, -1)) * 0.5 * (JY(1, 0) + JY(0, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(1, 0) * (1.0 - U(1, 0)) + U(0, 0) * (1.0 - U(0, 0))));
-    un += 0.5 * (JY(0, 1) + JY(0, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 1) - U(0, 0)) * 0.5 * (JY(0, 1) + JY(0, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 1) + U(1, 0) - U(-1, 1) - U(-1, 0)) * 0.5 * (JX(0, 1) + JX(0, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 1) * (1.0 - U(0, 1)) + U(0, 0) * (1.0 - U(0, 0))));
-    un -= 0.5 * (JX(0, 0) + JX(-1, 0)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(-1, 0)) * 0.5 * (JX(0, 0) + JX(-1, 0))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(0, 1) + U(-1, 1) - U(0, -1) - U(-1, -1)) * 0.5 * (JY(0, 0) + JY(-1, 0))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(-1, 0) * (1.0 - U(-1, 0))));
-    un -= 0.5 * (JY(0, 0) + JY(0, -1)) * (
-        wi * wi * (rdt / dx / dx) * (U(0, 0) - U(0, -1)) * 0.5 * (JY(0, 0) + JY(0, -1))
-        + wi * wi * (rdt / dx / dx) * 0.25 * (U(1, 0) + U(1, -1) - U(-1, 0) - U(-1, -1)) * 0.5 * (JX(0, 0) + JX(0, -1))
-        - rdt * wi * sqrt(2.0) / dx * 0.5 * (U(0, 0) * (1.0 - U(0, 0)) + U(0, -1) * (1.0 - U(0, -1))));
-    return un;
-#undef JX
-#undef JY
-}
-
 /* One pixel of a banded sweep: the relaxed field's rows above, at and below
  * the pixel (up, mid, dn: each the pixel's column in that row; a
  * neighbour's row comes from its copy in LDS) and the normals on the grid;
  * u0 returns the centre value. */
 template <typename R>
 static __device__ __forceinline__ R g_flux_rows(const R *up, const R *mid, const R *dn, const R *__restrict__ jx,
-                                                const R *__restrict__ jy, size_t qj, int sj, R wi, R rdt, R dx, R &u0) {
+                                                const R *__restrict__ jy, size_t qj, int sj, const GFluxK<R> &K,
+                                                R &u0) {
     auto U = [&](int a, int b) { return a < 0 ? up[b] : (a > 0 ? dn[b] : mid[b]); };
     u0 = mid[0];
-    return g_flux(u0, U, jx, jy, qj, sj, wi, rdt, dx);
+    return g_flux(u0, U, jx, jy, qj, sj, K);
 }
 
 /* reshaping_cells (cell_phase_functions.cpp:447-580) with K workgroups per
@@ -972,6 +986,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict_
     unsigned epoch = 1;
     if (!g_barrier(&ctr[n], epoch * K, abort_flag, limit)) return;
     const R dx = P.dx;
+    const GFluxK<R> FK = g_flux_k(P.wi, P.rdt, dx);
     double rdev = rdev0;
     int ri = 0, count = 0, cur = 0;
     /* halo rows of the sweep to come (and, from the second sweep on, the
@@ -999,7 +1014,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict_
             const R *up = wr == r0 ? hu + (wc + 1) : src + (qs - W2);
             const R *dn = wr == r1 - 1 ? hd + (wc + 1) : src + (qs + W2);
             R o;
-            const R v = g_flux_rows(up, src + qs, dn, jxn, jyn, qj, N, P.wi, P.rdt, dx, o);
+            const R v = g_flux_rows(up, src + qs, dn, jxn, jyn, qj, N, FK, o);
             dst[qs] = v;
             if (wr == r0) g_xs(&xrow(par ^ 1, b, 0)[wc + 1], v);
             if (wr == r1 - 1) g_xs(&xrow(par ^ 1, b, 1)[wc + 1], v);
@@ -1071,14 +1086,14 @@ static __device__ bool g_twait(const unsigned long long *p, unsigned tag, unsign
 /* The banded FP32 reshaping loops (g_relax_cells_lds, g_lum_loop_lds) run
  * S = 1 or 2 sweeps per exchange (ORG_GEN_SPAN_CELLS, ORG_GEN_SPAN_LUM) and
  * take at most ORG_GEN_LDS_MAX bytes of dynamic LDS (the static part adds
- * up to 4 KB; a compute unit has 160 KB). */
+ * up to 5 KB; a compute unit has 160 KB). */
 #ifndef ORG_GEN_SPAN_CELLS
 #define ORG_GEN_SPAN_CELLS 1
 #endif
 #ifndef ORG_GEN_SPAN_LUM
 #define ORG_GEN_SPAN_LUM 1
 #endif
-#define ORG_GEN_LDS_MAX (140 * 1024)
+#define ORG_GEN_LDS_MAX (150 * 1024)
 
 /* One workgroup's band of a banded FP32 reshaping loop with spans of S
  * sweeps: own rows [r0, r1) of a domain (the cell's window, the normals'
@@ -1112,6 +1127,27 @@ static __device__ inline void g_band_lds_split(GBand &G, int S, int rmax) {
     G.X0 = (float *)g_dyn_base(); G.X1 = G.X0 + plane; G.JX = G.X1 + plane; G.JY = G.JX + plane;
     G.SV = G.JY + plane - (size_t)S * G.L;          /* own rows only: from row S on */
 }
+
+/* a / b rounded to nearest from y = RN(1 / b) (Markstein: q = RN(a y) is
+ * within an ulp of a / b, r = a - b q is exact in one fma, RN(q + r y) is
+ * RN(a / b) for normal operands): the IEEE quotient with three FP64
+ * operations instead of a division sequence */
+static __device__ __forceinline__ double g_div_rn(double a, double b, double y) {
+    const double q = a * y;
+    return fma(fma(-q, b, a), y, q);
+}
+
+/* pixel k = t, t + step, ... of a rows x bw block as (row, column), with no
+ * integer division per pixel */
+struct GRowCol {
+    int r, c, dr, dc, bw;
+    __device__ GRowCol(int k, int step, int bw_) : r(k / bw_), c(k % bw_), dr(step / bw_), dc(step % bw_), bw(bw_) {}
+    __device__ void next() {
+        r += dr;
+        c += dc;
+        if (c >= bw) { c -= bw; r++; }
+    }
+};
 
 /* The loop `while (rdev > thr) { sweep; rdev = deviation; count++; if (ri >
  * rimax) { relax flag; break; } ri++; }` of reshaping_cells
@@ -1147,6 +1183,8 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
     const bool up = b > 0, dn = b < K - 1;          /* neighbouring bands above / below */
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const float dx = P.dx;
+    const GFluxK<float> FK = g_flux_k(P.wi, P.rdt, dx);
+    const double ncd = (double)P.ncur, rnc = 1.0 / ncd;   /* the deviation's divisor and its reciprocal */
     for (unsigned span = 1;; span++) {
         const unsigned tag = tag0 + span;
         const int par = span & 1;
@@ -1154,14 +1192,14 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
             const float *u = (t & 1) ? G.X1 : G.X0;
             float *un = (t & 1) ? G.X0 : G.X1;
             double acc = 0.0;
-            for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
-                const int q = (k / bw + S) * L + (k % bw + 1);
+            GRowCol rc(threadIdx.x, blockDim.x, bw);
+            for (int k = threadIdx.x; k < rows * bw; k += blockDim.x, rc.next()) {
+                const int q = (rc.r + S) * L + (rc.c + 1);
                 float o;
-                const float v = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, P.wi, P.rdt, dx,
-                                            o);
+                const float v = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, FK, o);
                 un[q] = v;
                 if (S > 1 && t == 0) G.SV[q] = o;
-                if constexpr (DIV) acc += fabs((double)(v - o)) * dx * dx / P.ncur;
+                if constexpr (DIV) acc += g_div_rn(fabs((double)(v - o)) * dx * dx, ncd, rnc);
                 else acc += fabs((double)(v - o)) * dx * dx;
             }
             if (S > 1) {
@@ -1170,7 +1208,7 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
                     const int r = k / bw, l = r < ga ? S - ga + r : S + rows + (r - ga);
                     const int q = l * L + (k % bw + 1);
                     float o;
-                    un[q] = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, P.wi, P.rdt, dx, o);
+                    un[q] = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, FK, o);
                 }
             }
             for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -1375,6 +1413,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
     auto xrow = [&](int par, int blk, int e) { return X + (((size_t)par * K + blk) * 2 + e) * L; };
     const int r0 = bi0 + b * bh / K, r1 = bi0 + (b + 1) * bh / K;     /* this workgroup's box rows */
     const R dx = P.dx;
+    const GFluxK<R> FK = g_flux_k(P.wi, P.rdt, dx);
     /* the exchange rows of X_count (in the buffer the next sweep reads) */
     {
         const R *u = (count & 1) ? bufA : bufB;
@@ -1408,7 +1447,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
             const R *up = i == r0 ? hu + c : u + (q - N);
             const R *dn = i == r1 - 1 ? hd + c : u + (q + N);
             R o;
-            const R v = g_flux_rows(up, u + q, dn, jx, jy, q, N, P.wi, P.rdt, dx, o);
+            const R v = g_flux_rows(up, u + q, dn, jx, jy, q, N, FK, o);
             un[q] = v;
             if (i == r0) g_xs(&xrow(par ^ 1, b, 0)[c], v);
             if (i == r1 - 1) g_xs(&xrow(par ^ 1, b, 1)[c], v);
@@ -1443,7 +1482,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
  * the sweeps of g_lum_loop_mb leave them (X_count in bufA for odd count).
  * xw: g_band_words(S, K, box width) words.  Dynamic LDS: g_band_lds(S,
  * rmax, box width), rmax >= the largest band; every band has S rows or
- * more; K <= 128. */
+ * more; K <= 256. */
 template <int S>
 __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__restrict__ bufA,
                                                        float *__restrict__ bufB, const float *__restrict__ jx,
@@ -1453,7 +1492,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
                                                        int *__restrict__ out, int *__restrict__ relax_flag,
                                                        int *__restrict__ abort_flag, long long limit, int K, int rmax) {
     __shared__ double red[S * 16];
-    __shared__ double sums[S * 128];
+    __shared__ double sums[S * 256];
     __shared__ int xfail;
     const int N = P.N, b = blockIdx.x;
     const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
@@ -1478,7 +1517,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
     }
     if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
     __syncthreads();
-    const int last = g_band_loop<S, 128, false>(P, G, thr, rdev, count, ri, tag0, relax_flag, abort_flag, limit, red,
+    const int last = g_band_loop<S, 256, false>(P, G, thr, rdev, count, ri, tag0, relax_flag, abort_flag, limit, red,
                                                 sums, &xfail);
     if (last == -2) return;
     if (last >= 0) {
@@ -1514,14 +1553,13 @@ __global__ void g_ring_dirichlet(int N, R *__restrict__ t, R side) {
 template <typename R>
 __global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ jy, int *__restrict__ box) {
     int i0 = N, i1 = -1, j0 = N, j1 = -1;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)N * N;
-         q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / N), j = (int)(q % N);
-        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
-        if (jx[q] != (R)0.0 || jy[q] != (R)0.0) {
-            i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
+    for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
+        for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
+            const size_t q = (size_t)i * N + j;
+            if (jx[q] != (R)0.0 || jy[q] != (R)0.0) {
+                i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
+            }
         }
-    }
     /* the workgroup's bounds (min / max in any order), then one atomic each */
     __shared__ int bb[4];
     if (threadIdx.x == 0) { bb[0] = N; bb[1] = -1; bb[2] = N; bb[3] = -1; }
@@ -1892,20 +1930,18 @@ struct GenCtx final : Generic {
                 GTRY(A_(&mb_cmax, 1));
                 GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
                 GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds<ORG_GEN_SPAN_CELLS>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)lds_l));
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l));
             }
             /* the lumen/ECM loop: one band of box rows per workgroup, up to
-             * one workgroup per compute unit and two deviations per lane of
-             * the polling wave; each launch checks that its bands fit */
-            klm = std::min(std::min(ncu, 128), N - 2);
+             * one workgroup per compute unit (256 at most); each launch
+             * checks that its bands fit */
+            klm = std::min(std::min(ncu, 256), N - 2);
             if (klm >= 2) {
                 const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N);
                 GTRY(A_(&mb_lw, words));
                 GHIP(hipMemset(mb_lw, 0, sizeof(unsigned long long) * words));
                 GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds<ORG_GEN_SPAN_LUM>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         ORG_GEN_LDS_MAX));
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, ORG_GEN_LDS_MAX));
             } else {
                 klm = 0;
             }
@@ -2133,8 +2169,8 @@ struct GenCtx final : Generic {
         return ORG_OK;
     }
     /* workgroups of g_lum_loop_lds for a bh x bw box: as many as allowed,
-     * ORG_GEN_SPAN_LUM rows or more each; 0 (the exchange-buffer loop
-     * runs) for FP64 state or when the bands do not fit in LDS */
+     * ORG_GEN_SPAN_LUM rows or more each; 0 (the exchange-buffer loop runs)
+     * for FP64 state or when the bands do not fit in LDS */
     int lum_lds_blocks(int bh, int bw) const {
         if (!std::is_same<R, float>::value || klm < 2) return 0;
         const int kb = std::min(klm, bh / ORG_GEN_SPAN_LUM);
@@ -2162,7 +2198,7 @@ struct GenCtx final : Generic {
             GTRY(plane_sum(u, un, 1));                                                        /* :848-855 */
             GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
             GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, mb_box, N);
-            GLAUNCH((g_nz_box<R>), dim3(grid_for(NN)), dim3(256), 0, N, (const R *)jlx, (const R *)jly, mb_box);
+            GLAUNCH((g_nz_box<R>), dim3(std::max(1, N - 2)), dim3(256), 0, N, (const R *)jlx, (const R *)jly, mb_box);
             int bx[4];
             GHIP(hipMemcpyAsync(bx, mb_box, sizeof(bx), hipMemcpyDeviceToHost, s));
             GHIP(hipStreamSynchronize(s));
@@ -2185,9 +2221,9 @@ struct GenCtx final : Generic {
                     GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
                     if constexpr (std::is_same<R, float>::value)
                         GLAUNCH(g_lum_loop_lds<ORG_GEN_SPAN_LUM>, dim3(kb), dim3(1024),
-                                g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw), P, u, un,
-                                (const float *)jlx, (const float *)jly, (const int *)mb_box, thr, rdev, count, ri,
-                                mb_lw, tag_next, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
+                                g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw), P, u, un, (const float *)jlx,
+                                (const float *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_lw, tag_next,
+                                mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
                     int h[3] = {0, 0, 0};
                     GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
                     GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
  * the sweeps of g_lum_loop_mb leave them (X_count in bufA for odd count).
  * xw: g_band_words(S, K, box width) words.  Dynamic LDS: g_band_lds(S,
  * rmax, box width), rmax >= the largest band; every band has S rows or
- * more; K <= 256. */
+ * more; K <= 128. */
 template <int S>
 __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__restrict__ bufA,
                                                        float *__restrict__ bufB, const float *__restrict__ jx,
@@ -1492,7 +1492,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
                                                        int *__restrict__ out, int *__restrict__ relax_flag,
                                                        int *__restrict__ abort_flag, long long limit, int K, int rmax) {
     __shared__ double red[S * 16];
-    __shared__ double sums[S * 256];
+    __shared__ double sums[S * 128];
     __shared__ int xfail;
     const int N = P.N, b = blockIdx.x;
     const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
@@ -1517,7 +1517,7 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
     }
     if (threadIdx.x == 0) xfail = 0;      /* set once: after it goes to 1 every thread returns */
     __syncthreads();
-    const int last = g_band_loop<S, 256, false>(P, G, thr, rdev, count, ri, tag0, relax_flag, abort_flag, limit, red,
+    const int last = g_band_loop<S, 128, false>(P, G, thr, rdev, count, ri, tag0, relax_flag, abort_flag, limit, red,
                                                 sums, &xfail);
     if (last == -2) return;
     if (last >= 0) {
@@ -1933,9 +1933,11 @@ struct GenCtx final : Generic {
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l));
             }
             /* the lumen/ECM loop: one band of box rows per workgroup, up to
-             * one workgroup per compute unit (256 at most); each launch
-             * checks that its bands fit */
-            klm = std::min(std::min(ncu, 256), N - 2);
+             * one workgroup per compute unit and 128 in all (with 256, the
+             * polls of twice the deviations cost more than the smaller
+             * bands save: 1.38 against 1.17 ms per template step); each
+             * launch checks that its bands fit */
+            klm = std::min(std::min(ncu, 128), N - 2);
             if (klm >= 2) {
                 const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N);
                 GTRY(A_(&mb_lw, words));

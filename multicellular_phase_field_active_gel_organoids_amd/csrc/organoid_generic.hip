@@ -341,20 +341,18 @@ __global__ void g_bc_cells(int N, R *__restrict__ t) {
 }
 
 /* compute_normal_grad_phi, useful_functions.cpp:326-354, interior of every
- * plane (grid-stride over nc * N * N). */
+ * plane: workgroup (x, n) on rows x + 1, x + 1 + gridDim.x, ... of plane n. */
 template <typename R>
 __global__ void g_norm_grad(GP<R> P, const R *__restrict__ phi, R *__restrict__ ngp) {
-    const int N = P.N;
-    const size_t NN = (size_t)N * N, tot = NN * P.nc;
+    const int N = P.N, n = blockIdx.y;
+    const size_t NN = (size_t)N * N;
     const R dx = P.dx;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
-        const size_t n = k / NN, r = k % NN;
-        const int i = (int)(r / N), j = (int)(r % N);
-        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
-        const R a = phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)];
-        const R c = phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)];
-        ngp[k] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
-    }
+    for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
+        for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
+            const R a = phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)];
+            const R c = phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)];
+            ngp[GPL(n, i, j)] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
+        }
 }
 
 /* compute_psi_kernels, useful_functions.cpp:305-322 (cells summed in order) */
@@ -399,26 +397,26 @@ __global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ h
 }
 
 /* compute_normal_to_phi, useful_functions.cpp:356-388 (interior; the ring
- * by g_bc_cells) */
+ * by g_bc_cells): workgroup (x, n) on rows x + 1, x + 1 + gridDim.x, ... of
+ * plane n */
 template <typename R>
 __global__ void g_normals(GP<R> P, const R *__restrict__ phi, const R *__restrict__ ngp, R *__restrict__ jx,
                           R *__restrict__ jy) {
-    const int N = P.N;
-    const size_t NN = (size_t)N * N, tot = NN * P.nc;
+    const int N = P.N, n = blockIdx.y;
+    const size_t NN = (size_t)N * N;
     const R dx = P.dx;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
-        const size_t n = k / NN, r = k % NN;
-        const int i = (int)(r / N), j = (int)(r % N);
-        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
-        const R g = ngp[k];
-        if (g > 0.00) {
-            jx[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)]) / g;
-            jy[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)]) / g;
-        } else {
-            jx[k] = 0.0;
-            jy[k] = 0.0;
+    for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
+        for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
+            const size_t k = GPL(n, i, j);
+            const R g = ngp[k];
+            if (g > 0.00) {
+                jx[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)]) / g;
+                jy[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)]) / g;
+            } else {
+                jx[k] = 0.0;
+                jy[k] = 0.0;
+            }
         }
-    }
 }
 
 /* create_turnover_rates_tensions, rho_functions.cpp:286-405, at one pixel:
@@ -610,8 +608,10 @@ __global__ void g_lumecm_update(GP<R> P, const R *__restrict__ lum, const R *__r
     const size_t NN = (size_t)N * N;
     const R dx = P.dx, dt = P.dt;
     const R sum_e = (R)(*sum_ecm);
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / N), j = (int)(q % N);
+    (void)NN;
+    for (int i = blockIdx.x; i < N; i += gridDim.x)                        /* workgroup x on rows x, x + gridDim.x, ... */
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const size_t q = (size_t)i * N + j;
         if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
             lumn[q] = 0.0;                                                  /* edges and corners */
             ecmn[q] = 1.0;
@@ -637,33 +637,31 @@ __global__ void g_lumecm_update(GP<R> P, const R *__restrict__ lum, const R *__r
 template <typename R>
 __global__ void g_lum_grad(GP<R> P, const R *__restrict__ old, R *__restrict__ ng) {
     const int N = P.N;
-    const size_t NN = (size_t)N * N;
     const R dx = P.dx;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / N), j = (int)(q % N);
-        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
-        const R a = old[q + N] - old[q - N], c = old[q + 1] - old[q - 1];
-        ng[q] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
-    }
+    for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
+        for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
+            const size_t q = (size_t)i * N + j;
+            const R a = old[q + N] - old[q - N], c = old[q + 1] - old[q - 1];
+            ng[q] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
+        }
 }
 template <typename R>
 __global__ void g_lum_normals(GP<R> P, const R *__restrict__ nw, const R *__restrict__ ng, R *__restrict__ jx,
                               R *__restrict__ jy) {
     const int N = P.N;
-    const size_t NN = (size_t)N * N;
     const R dx = P.dx;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / N), j = (int)(q % N);
-        if (i < 1 || i > N - 2 || j < 1 || j > N - 2) continue;
-        const R g = ng[q];
-        if (g != 0.0) {
-            jx[q] = (1.0 / (2.0 * dx)) * (nw[q + N] - nw[q - N]) / g;
-            jy[q] = (1.0 / (2.0 * dx)) * (nw[q + 1] - nw[q - 1]) / g;
-        } else {
-            jx[q] = 0.0;
-            jy[q] = 0.0;
+    for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
+        for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
+            const size_t q = (size_t)i * N + j;
+            const R g = ng[q];
+            if (g != 0.0) {
+                jx[q] = (1.0 / (2.0 * dx)) * (nw[q + N] - nw[q - N]) / g;
+                jy[q] = (1.0 / (2.0 * dx)) * (nw[q + 1] - nw[q - 1]) / g;
+            } else {
+                jx[q] = 0.0;
+                jy[q] = 0.0;
+            }
         }
-    }
 }
 
 /* The loop-invariant factors of the reshaping flux (cell_phase_functions.cpp:
@@ -746,10 +744,10 @@ template <typename R>
 __global__ void g_lum_sweep(GP<R> P, const R *__restrict__ u, R *__restrict__ un, const R *__restrict__ jx,
                             const R *__restrict__ jy, R side) {
     const int N = P.N;
-    const size_t NN = (size_t)N * N;
     const GFluxK<R> K = g_flux_k(P.wi, P.rdt, P.dx);
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < NN; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / N), j = (int)(q % N);
+    for (int i = blockIdx.x; i < N; i += gridDim.x)                        /* workgroup x on rows x, x + gridDim.x, ... */
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const size_t q = (size_t)i * N + j;
         if (i < 1 || i > N - 2 || j < 1 || j > N - 2) {
             const bool corner = (i == 0 || i == N - 1) && (j == 0 || j == N - 1);
             un[q] = corner ? (R)0.0 : side;
@@ -2079,6 +2077,8 @@ struct GenCtx final : Generic {
         return ORG_OK;
     }
     int grid_for(size_t n) const { return (int)std::min<size_t>(4096, (n + 255) / 256); }
+    /* workgroups for a launch of one row per workgroup (at most 65535) */
+    static int row_blocks(int rows) { return std::max(1, std::min(rows, 65535)); }
     int bc(R *planes, int nplanes) {
         GLAUNCH((g_bc_cells<R>), dim3(nplanes), dim3(256), 0, N, planes);
         return ORG_OK;
@@ -2128,7 +2128,7 @@ struct GenCtx final : Generic {
     }
     /* compute_normal_grad_phi + boundary_conditions_cells (run_with_active_gels.cpp:135-136) */
     int norm_grad() {
-        GLAUNCH((g_norm_grad<R>), dim3(grid_for(PP)), dim3(256), 0, P, (const R *)phi, ngp);
+        GLAUNCH((g_norm_grad<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, ngp);
         return bc(ngp, nc);
     }
     /* run_with_active_gels.cpp:130-154: turnover maps, prod_norm_delta and
@@ -2166,7 +2166,7 @@ struct GenCtx final : Generic {
     int lumecm_update() override {
         GTRY(plane_sum(he, nullptr, 0));                                               /* :376-386 */
         GHIP(hipMemcpyAsync(sum_e, dev_sum, sizeof(double), hipMemcpyDeviceToDevice, s));
-        GLAUNCH((g_lumecm_update<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)lum, (const R *)ecm,
+        GLAUNCH((g_lumecm_update<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)lum, (const R *)ecm,
                 (const R *)psi_, (const R *)hl, (const R *)he, (const double *)sum_e, lumn, ecmn);
         return ORG_OK;
     }
@@ -2184,9 +2184,10 @@ struct GenCtx final : Generic {
         R *&un = which ? ecmn : lumn;
         const R side = which ? (R)1.0 : (R)0.0;
         const R rtol = (R)p.rtol, rdt = (R)p.rdt;
-        GLAUNCH((g_lum_grad<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)u, ngl);   /* :734-736 */
+        GLAUNCH((g_lum_grad<R>), dim3(row_blocks(N - 2)), dim3(256), 0, P, (const R *)u, ngl);   /* :734-736 */
         GTRY(bc(ngl, 1));                                                                     /* :741 */
-        GLAUNCH((g_lum_normals<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx, jly);
+        GLAUNCH((g_lum_normals<R>), dim3(row_blocks(N - 2)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx,
+                jly);
         GTRY(bc(jlx, 1));                                                                     /* :770-771 */
         GTRY(bc(jly, 1));
         double rdev = (double)(p.rdev_ini * rtol * rdt);                                      /* :720 */
@@ -2195,7 +2196,7 @@ struct GenCtx final : Generic {
         int count = 0, ri = 0;
         if (rdev > thr) {
             /* sweep 1 on the whole grid: lum <- lumn (the role change), lumn <- F(lum) into lum's buffer */
-            GLAUNCH((g_lum_sweep<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)un, u, (const R *)jlx,
+            GLAUNCH((g_lum_sweep<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)un, u, (const R *)jlx,
                     (const R *)jly, side);
             GTRY(plane_sum(u, un, 1));                                                        /* :848-855 */
             GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2282,7 +2283,7 @@ struct GenCtx final : Generic {
         GTRY(window_check());
         if (from_phi) GTRY(copy(phin, phi, PP));
         GTRY(norm_grad());
-        GLAUNCH((g_normals<R>), dim3(grid_for(PP)), dim3(256), 0, P, (const R *)phi, (const R *)ngp, jx, jy);
+        GLAUNCH((g_normals<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, (const R *)ngp, jx, jy);
         GTRY(bc(jx, nc));                                             /* run_with_active_gels.cpp:164-166 */
         GTRY(bc(jy, nc));
         const R rtol = (R)p.rtol, Rr = (R)p.R, wi = (R)p.wi, rdt = (R)p.rdt;

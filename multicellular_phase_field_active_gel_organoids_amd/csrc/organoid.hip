@@ -1002,7 +1002,7 @@ int org_commit_lumecm(org_ctx *c) {
 /* ---- batched form ------------------------------------------------------- */
 /* One timestep of the full-plane path in the driver's order
  * (run_with_active_gels.cpp:118-184). */
-static int gen_step(org_ctx *c) {
+static int gen_step_body(org_ctx *c) {
     gen::Generic *g = c->gen;
     TRY(gen_rc(g->find_centres()));                      /* :123 */
     TRY(gen_rc(g->hfunc_cells()));                       /* :133 */
@@ -1015,8 +1015,15 @@ static int gen_step(org_ctx *c) {
     TRY(gen_reshape(c, 1));                              /* :161 */
     TRY(gen_rc(g->update_cells()));                      /* :169-170 */
     TRY(gen_reshape(c, 2));                              /* :164-166, :172 */
+    g->step_scope(false);                                /* phi changes from here on */
     TRY(gen_rc(g->commit_rho()));                        /* :175-178 */
     return gen_rc(g->commit_lumecm());                   /* :181-182 */
+}
+static int gen_step(org_ctx *c) {
+    c->gen->step_scope(true);      /* |grad phi| formed once: phi is fixed until the copy-back */
+    const int rc = gen_step_body(c);
+    c->gen->step_scope(false);
+    return rc;
 }
 
 int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {

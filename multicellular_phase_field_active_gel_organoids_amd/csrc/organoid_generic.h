@@ -46,6 +46,12 @@ struct Generic {
     /* the 2D transforms' schedule: stages fused per LDS round trip (1..3) and lines per workgroup (1, 2, 4) */
     virtual int get_fft_schedule(int32_t *fuse, int32_t *lines_per_wg) = 0;
     virtual int set_fft_schedule(int fuse, int lines_per_wg) = 0;
+    /* Inside one whole step (from its first call to the end of the cell
+     * reshaping) phi does not change, so |grad phi| is formed once there
+     * instead of by each of cortices, update_cells and reshape_cells: on
+     * from the step's start, off before its copy-back.  The per-function
+     * entry points leave it off (each call forms its own, as the reference). */
+    virtual void step_scope(bool on) = 0;
 };
 
 /* A context on `stream` (borrowed: the caller owns and destroys it). */

@@ -378,8 +378,9 @@ __global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ h
     const R *h = hf + (size_t)n * NN;
     double acc = 0.0;
     /* each thread's pixels k = t, t + 1024, ... added in that order, the
-     * loads of 16 of them in flight together */
-    constexpr int U = 16;
+     * loads of 32 of them in flight together (one workgroup per cell: its
+     * loads in flight set the pace) */
+    constexpr int U = 32;
     size_t k = threadIdx.x;
     for (; k + (size_t)(U - 1) * blockDim.x < NN; k += (size_t)U * blockDim.x) {
         R v[U];
@@ -1838,6 +1839,7 @@ struct GenCtx final : Generic {
     unsigned tag_next = 1;
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
+    bool in_step = false, ngp_fresh = false;   /* step_scope: ngp formed in this step from the current phi */
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
 
     ~GenCtx() override {
@@ -2099,8 +2101,8 @@ struct GenCtx final : Generic {
     int find_centres() override {
         GHIP(hipMemsetAsync(err, 0, sizeof(int), s));
         GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
-        GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 512 / nc)), nc), dim3(256), 0, P, (const R *)phi,
-                cen_acc);
+        GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P,
+                (const R *)phi, cen_acc);
         GLAUNCH((g_centroid_fin<R>), dim3(1), dim3(64), 0, P, (const unsigned long long *)cen_acc, xc, yc, err);
         int e = 0;
         GHIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2128,8 +2130,15 @@ struct GenCtx final : Generic {
     }
     /* compute_normal_grad_phi + boundary_conditions_cells (run_with_active_gels.cpp:135-136) */
     int norm_grad() {
+        if (in_step && ngp_fresh) return ORG_OK;
         GLAUNCH((g_norm_grad<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, ngp);
-        return bc(ngp, nc);
+        GTRY(bc(ngp, nc));
+        ngp_fresh = in_step;
+        return ORG_OK;
+    }
+    void step_scope(bool on) override {
+        in_step = on;
+        ngp_fresh = false;
     }
     /* run_with_active_gels.cpp:130-154: turnover maps, prod_norm_delta and
      * the whole per-cell spectral loop of update_all_cortices

@@ -6,10 +6,11 @@ checkpoints and its centres and reshaping iteration counts at every step,
 plus the drift of the oracle built in FP32 (the reference's own float
 arithmetic) over the same steps.  The GPU runs the same seeded workload
 through the batched step (hipGraphs of one step for two_cells_256, of eight
-steps, i.e. the k_mid path, for organoid16_512) and the test reports, per
-field and checkpoint, the relative L2 error over all cells x N^2 and the
-largest per-cell relative L2 error, plus the number of compared steps whose
-integer centres or iteration counts differ.
+steps, i.e. the k_mid path, for organoid16_512, of 40 steps for
+organoid64_1024, the benchmark's workload: 64 cells on 1024^2, 10^4 steps)
+and the test reports, per field and checkpoint, the relative L2 error over
+all cells x N^2 and the largest per-cell relative L2 error, plus the number
+of compared steps whose integer centres or iteration counts differ.
 
 Tolerances: phi relative L2 <= 1e-3 at every checkpoint (up to 10^4 steps),
 the bound the port is specified to.  Every field, the velocity and the
@@ -22,7 +23,11 @@ tests and the integer centroids are threshold decisions, and each flip moves
 the trajectory), and how far varies widely with rounding alone: over that
 ensemble the two-cell phi drift at 10^4 steps spans 3.3e-5 .. 8.8e-4, the
 16-cell one at 5000 steps 2.5e-5 .. 3.3e-5 (the unperturbed FP32 run gives
-9.6e-6).  A wrong term drifts by orders of magnitude more.
+9.6e-6).  On organoid64_1024 the float build itself ends 1.27e-3 from FP64
+in phi after 10^4 steps (6.2e-6 at step 1000; its cell reshaping counts
+differ from FP64's at 8616 of the 10^4 steps): the 1e-3 bound is then
+tighter than FP32 arithmetic's own spread, and the GPU trajectory has to
+meet it as well.  A wrong term drifts by orders of magnitude more.
 
 The FP64 form (org_create_f64, the reference retyped to double) runs the
 same trajectories against the same fixtures, to the same bounds.  FP64
@@ -90,7 +95,7 @@ def per_cell_max(a, b):
     return max(rel_l2(a[n], b[n]) for n in range(b.shape[0]))
 
 
-@pytest.mark.parametrize("case,chunk", [("two_cells_256", 1), ("organoid16_512", 8)])
+@pytest.mark.parametrize("case,chunk", [("two_cells_256", 1), ("organoid16_512", 8), ("organoid64_1024", 40)])
 def test_long_trajectory_drift(case, chunk):
     run_drift(case, chunk, "f32")
 

@@ -38,3 +38,36 @@ def test_organoid256_2048_steps():
     d = np.hypot(xc - wl.centres[:, 0], yc - wl.centres[:, 1])
     print(f"  centre displacement: max {d.max():.2f} px", flush=True)
     assert d.max() < 4.0
+
+
+def test_organoid256_2048_full_length():
+    """The configuration's full length, 2 x 10^4 steps (replayed as graphs):
+    at every 5000 steps every field finite and in range, every cell's area
+    near its target and its centre near where it started, the reshaping
+    loops within their iteration limit (a limit reached raises)."""
+    wl = make_workload("organoid256_2048")
+    p = wl.params
+    print(f"case organoid256_2048: {p.ncell} cells, {p.N}^2, 2 x 10^4 steps in graphs", flush=True)
+    g = host.Organoid(p, wl.s0_cell)
+    g.load_workload(wl)
+    done = 0
+    for chunk in (5000, 5000, 5000, 5000):
+        g.step(chunk, graph=True, it0=done)
+        done += chunk
+        fields = {f: g.get_field(f) for f in (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM)}
+        ranges = {f: (float(a.min()), float(a.max())) for f, a in fields.items()}
+        s, _ = g.areas()
+        rel = np.abs(s - wl.s0_cell) / wl.s0_cell
+        xc, yc = g.centres()
+        d = np.hypot(xc - wl.centres[:, 0], yc - wl.centres[:, 1])
+        cells, lum, ecm = g.iterations()
+        print(f"  step {done}: ranges phi {ranges[host.F_PHI]} rho {ranges[host.F_RHO]} lum {ranges[host.F_LUM]} "
+              f"ecm {ranges[host.F_ECM]}; area dev max {rel.max():.3f}; centre displacement max {d.max():.2f} px; "
+              f"sweeps cells <= {int(cells.max())}, lum {lum}, ECM {ecm}", flush=True)
+        for f, a in fields.items():
+            assert np.isfinite(a).all(), (done, f)
+            if f != host.F_RHO:
+                assert ranges[f][0] > -0.05 and ranges[f][1] < 1.05, (done, f, ranges[f])
+        assert rel.max() < 0.1, (done, rel.max())
+        assert d.max() < 8.0, (done, d.max())
+    g.close()

@@ -3,7 +3,8 @@ U(14, 26) on a jittered hex packing, 2048^2): too large for the FP64 oracle
 (its state alone is ~100 GiB), so it is checked through properties that do
 not depend on size: the graph-replayed and the eager batched step agree bit
 for bit, every field stays finite and in range, every cell keeps its area
-within the reshaping tolerance of its target and stays near its centre."""
+within the reshaping tolerance of its target and stays near its centre:
+over 24 steps, and over the configuration's full 2 x 10^4 steps (~20 s)."""
 import numpy as np
 import pytest
 
@@ -68,6 +69,6 @@ def test_organoid256_2048_full_length():
             assert np.isfinite(a).all(), (done, f)
             if f != host.F_RHO:
                 assert ranges[f][0] > -0.05 and ranges[f][1] < 1.05, (done, f, ranges[f])
-        assert rel.max() < 0.1, (done, rel.max())
-        assert d.max() < 8.0, (done, d.max())
+        assert rel.max() < 0.02, (done, rel.max())         # observed 0.002 at every checkpoint
+        assert d.max() < 4.0, (done, d.max())              # observed 1.27 px
     g.close()

@@ -1051,7 +1051,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_mb(GP<R> P, R *__restrict_
     if (threadIdx.x == 0 && b == 0) counts[n] = count;
 }
 
-/* Tagged exchange words (FP32 state): 32 bits of payload in the low half,
+/* Tagged exchange words: 32 bits of payload in the low half,
  * the tag of the exchange in the high half, stored and loaded as one 64-bit
  * agent-scope access (single-copy atomic and coherent across the XCDs'
  * L2s): a reader that finds its tag holds that exchange's payload, so the
@@ -1082,10 +1082,10 @@ static __device__ bool g_twait(const unsigned long long *p, unsigned tag, unsign
     }
 }
 
-/* The banded FP32 reshaping loops (g_relax_cells_lds, g_lum_loop_lds) run
- * S = 1 or 2 sweeps per exchange (ORG_GEN_SPAN_CELLS, ORG_GEN_SPAN_LUM) and
- * take at most ORG_GEN_LDS_MAX bytes of dynamic LDS (the static part adds
- * up to 5 KB; a compute unit has 160 KB). */
+/* The banded reshaping loops (g_relax_cells_lds, g_lum_loop_lds; FP32 or
+ * FP64 state) run S = 1 or 2 sweeps per exchange (ORG_GEN_SPAN_CELLS,
+ * ORG_GEN_SPAN_LUM) and take at most ORG_GEN_LDS_MAX bytes of dynamic LDS
+ * (the static part adds up to 5 KB; a compute unit has 160 KB). */
 #ifndef ORG_GEN_SPAN_CELLS
 #define ORG_GEN_SPAN_CELLS 1
 #endif
@@ -1094,37 +1094,55 @@ static __device__ bool g_twait(const unsigned long long *p, unsigned tag, unsign
 #endif
 #define ORG_GEN_LDS_MAX (150 * 1024)
 
-/* One workgroup's band of a banded FP32 reshaping loop with spans of S
- * sweeps: own rows [r0, r1) of a domain (the cell's window, the normals'
- * box) `bw` columns wide whose one-pixel surround is static.  LDS row l
- * holds domain row r0 - S + l, LDS column c domain column c - 1 (L = bw +
- * 2).  X0 holds the span's input, X1 the other iterate (both with the
- * static surround), JX / JY the normals, SV (S = 2; from row S on) the span
- * input's own rows.  Every band has S rows or more. */
+/* One workgroup's band of a banded reshaping loop with spans of S sweeps
+ * and values of type V: own rows [r0, r1) of a domain (the cell's window,
+ * the normals' box) `bw` columns wide whose one-pixel surround is static.
+ * LDS row l holds domain row r0 - S + l, LDS column c domain column c - 1
+ * (L = bw + 2).  X0 holds the span's input, X1 the other iterate (both with
+ * the static surround), JX / JY the normals, SV (S = 2; from row S on) the
+ * span input's own rows.  Every band has S rows or more.  A value travels
+ * as sizeof(V) / 4 tagged words (its 32-bit halves, low first). */
+template <typename V>
 struct GBand {
-    float *X0, *X1, *JX, *JY, *SV;
+    V *X0, *X1, *JX, *JY, *SV;
     int L, bw, rows, b, K;
-    unsigned long long *xr;   /* [2 parities][K][2 ends][S][bw] tagged rows */
+    unsigned long long *xr;   /* [2 parities][K][2 ends][S][bw][words per value] tagged rows */
     unsigned long long *xd;   /* [2 parities][K][S][2 halves] tagged deviations */
 };
-/* dynamic LDS of a band of up to `rows` rows, `bw` columns */
-static __host__ __device__ inline size_t g_band_lds(int S, int rows, int bw) {
-    return ((size_t)4 * (rows + 2 * S) + (S > 1 ? rows : 0)) * (bw + 2) * sizeof(float);
+/* dynamic LDS of a band of up to `rows` rows, `bw` columns, values of vbytes */
+static __host__ __device__ inline size_t g_band_lds(int S, int rows, int bw, size_t vbytes) {
+    return ((size_t)4 * (rows + 2 * S) + (S > 1 ? rows : 0)) * (bw + 2) * vbytes;
 }
-/* tagged words of a group of K bands `bw` columns wide */
-static __host__ __device__ inline size_t g_band_words(int S, int K, int bw) {
-    return (size_t)2 * K * 2 * S * bw + (size_t)2 * K * S * 2;
+/* tagged words of a group of K bands `bw` columns wide, values of vbytes */
+static __host__ __device__ inline size_t g_band_words(int S, int K, int bw, size_t vbytes) {
+    return (size_t)2 * K * 2 * S * bw * (vbytes / 4) + (size_t)2 * K * S * 2;
 }
-/* the dynamic LDS of the launch */
+/* the dynamic LDS of the launch (16-byte aligned: FP64 arrays follow the
+ * static part) */
 static __device__ inline char *g_dyn_base() {
-    extern __shared__ char g_dyn[];
-    return g_dyn;
+    extern __shared__ __attribute__((aligned(16))) char g_band_dyn[];
+    return g_band_dyn;
 }
 /* G's LDS arrays, for bands of up to rmax rows (G.L set) */
-static __device__ inline void g_band_lds_split(GBand &G, int S, int rmax) {
+template <typename V>
+static __device__ inline void g_band_lds_split(GBand<V> &G, int S, int rmax) {
     const size_t plane = (size_t)(rmax + 2 * S) * G.L;
-    G.X0 = (float *)g_dyn_base(); G.X1 = G.X0 + plane; G.JX = G.X1 + plane; G.JY = G.JX + plane;
+    G.X0 = (V *)g_dyn_base(); G.X1 = G.X0 + plane; G.JX = G.X1 + plane; G.JY = G.JX + plane;
     G.SV = G.JY + plane - (size_t)S * G.L;          /* own rows only: from row S on */
+}
+/* word h of a value and a value from its words */
+static __device__ __forceinline__ unsigned g_word(float v, int) { return __float_as_uint(v); }
+static __device__ __forceinline__ unsigned g_word(double v, int h) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return h ? (unsigned)(b >> 32) : (unsigned)b;
+}
+template <typename V>
+static __device__ __forceinline__ V g_join(const unsigned long long *w);
+template <>
+__device__ __forceinline__ float g_join<float>(const unsigned long long *w) { return __uint_as_float((unsigned)w[0]); }
+template <>
+__device__ __forceinline__ double g_join<double>(const unsigned long long *w) {
+    return __longlong_as_double((long long)(((w[1] & 0xffffffffull) << 32) | (w[0] & 0xffffffffull)));
 }
 
 /* a / b rounded to nearest from y = RN(1 / b) (Markstein: q = RN(a y) is
@@ -1171,31 +1189,33 @@ struct GRowCol {
  * the one before: X0 (S = 1) or SV (S = 2) for t = 0, X1 for t = 1), -1 if
  * the loop did not run, -2 if a wait timed out or another workgroup gave
  * up (the abort flag is then up).  KMAX: the largest K. */
-template <int S, int KMAX, bool DIV>
-static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, double rdev, int &count, int &ri,
+template <int S, int KMAX, bool DIV, typename V>
+static __device__ int g_band_loop(const GP<V> &P, GBand<V> &G, double thr, double rdev, int &count, int &ri,
                                   unsigned tag0, int *relax_flag, int *abort_flag, long long limit, double *red,
                                   double *sums, int *xfail) {
     static_assert(S == 1 || S == 2, "spans of one or two sweeps");
     constexpr int MD = (S * KMAX + 63) / 64;        /* deviations per lane of the polling wave */
+    constexpr int WPV = (int)(sizeof(V) / 4);       /* tagged words per value */
     if (!(rdev > thr)) return -1;
     const int L = G.L, bw = G.bw, rows = G.rows, b = G.b, K = G.K;
     const bool up = b > 0, dn = b < K - 1;          /* neighbouring bands above / below */
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const float dx = P.dx;
-    const GFluxK<float> FK = g_flux_k(P.wi, P.rdt, dx);
+    const V dx = P.dx;
+    const GFluxK<V> FK = g_flux_k(P.wi, P.rdt, dx);
     const double ncd = (double)P.ncur, rnc = 1.0 / ncd;   /* the deviation's divisor and its reciprocal */
     for (unsigned span = 1;; span++) {
         const unsigned tag = tag0 + span;
         const int par = span & 1;
         for (int t = 0; t < S; t++) {
-            const float *u = (t & 1) ? G.X1 : G.X0;
-            float *un = (t & 1) ? G.X0 : G.X1;
+            const V *u = (t & 1) ? G.X1 : G.X0;
+            V *un = (t & 1) ? G.X0 : G.X1;
             double acc = 0.0;
             GRowCol rc(threadIdx.x, blockDim.x, bw);
             for (int k = threadIdx.x; k < rows * bw; k += blockDim.x, rc.next()) {
                 const int q = (rc.r + S) * L + (rc.c + 1);
-                float o;
-                const float v = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, FK, o);
+                V o;
+                const V v = g_flux_rows(u + q - L, u + q, u + q + L, (const V *)G.JX, (const V *)G.JY, (size_t)q, L,
+                                        FK, o);
                 un[q] = v;
                 if (S > 1 && t == 0) G.SV[q] = o;
                 if constexpr (DIV) acc += g_div_rn(fabs((double)(v - o)) * dx * dx, ncd, rnc);
@@ -1206,19 +1226,22 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
                 for (int k = threadIdx.x; k < (ga + gb) * bw; k += blockDim.x) {
                     const int r = k / bw, l = r < ga ? S - ga + r : S + rows + (r - ga);
                     const int q = l * L + (k % bw + 1);
-                    float o;
-                    un[q] = g_flux_rows(u + q - L, u + q, u + q + L, G.JX, G.JY, (size_t)q, L, FK, o);
+                    V o;
+                    un[q] = g_flux_rows(u + q - L, u + q, u + q + L, (const V *)G.JX, (const V *)G.JY, (size_t)q, L,
+                                        FK, o);
                 }
             }
             for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
             if (lane == 0) red[t * 16 + wv] = acc;
             __syncthreads();
         }
-        float *out = (S & 1) ? G.X1 : G.X0;         /* the span's last iterate */
-        for (int k = threadIdx.x; k < 2 * S * bw; k += blockDim.x) {
-            const int e = k / (S * bw), s = (k / bw) % S, c = k % bw;
+        V *out = (S & 1) ? G.X1 : G.X0;             /* the span's last iterate */
+        for (int k = threadIdx.x; k < 2 * S * bw * WPV; k += blockDim.x) {
+            const int h = k % WPV, kv = k / WPV;
+            const int e = kv / (S * bw), s = (kv / bw) % S, c = kv % bw;
             const int l = e ? rows + s : S + s;     /* the first / last S own rows */
-            g_tput(&G.xr[((((size_t)par * K + b) * 2 + e) * S + s) * bw + c], __float_as_uint(out[l * L + c + 1]), tag);
+            g_tput(&G.xr[(((((size_t)par * K + b) * 2 + e) * S + s) * bw + c) * WPV + h],
+                   g_word(out[l * L + c + 1], h), tag);
         }
         if ((int)threadIdx.x < S) {
             const int t = threadIdx.x;
@@ -1253,32 +1276,37 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
                 }
             }
         } else {
-            /* the neighbours' rows into the halo rows of `out`, four words
+            /* the neighbours' rows into the halo rows of `out`, four values
              * per thread in flight, then the waits */
             const int nq = 2 * S * bw, step = (int)blockDim.x - 64;
             for (int q0 = (int)threadIdx.x - 64; q0 < nq; q0 += 4 * step) {
-                unsigned long long w[4];
+                unsigned long long w[4][WPV];
                 const unsigned long long *ap[4];
                 int la[4];
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     const int q = q0 + m * step;
-                    la[m] = -1; ap[m] = nullptr; w[m] = 0ull;
+                    la[m] = -1; ap[m] = nullptr;
+#pragma unroll
+                    for (int h = 0; h < WPV; h++) w[m][h] = 0ull;
                     if (q < nq) {
                         const int e = q / (S * bw), s = (q / bw) % S, c = q % bw;
                         if (e ? dn : up) {          /* else static rows there */
                             /* above: b - 1's last S rows; below: b + 1's first */
-                            ap[m] = &G.xr[((((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * S + s) * bw + c];
+                            ap[m] = &G.xr[(((((size_t)par * K + (e ? b + 1 : b - 1)) * 2 + (e ? 0 : 1)) * S + s) * bw +
+                                           c) * WPV];
                             la[m] = (e ? S + rows + s : s) * L + c + 1;
-                            w[m] = g_tget(ap[m]);
+#pragma unroll
+                            for (int h = 0; h < WPV; h++) w[m][h] = g_tget(ap[m] + h);
                         }
                     }
                 }
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     if (la[m] < 0) continue;
-                    ok = ok && g_twait(ap[m], tag, w[m], abort_flag, limit);
-                    out[la[m]] = __uint_as_float((unsigned)w[m]);
+#pragma unroll
+                    for (int h = 0; h < WPV; h++) ok = ok && g_twait(ap[m] + h, tag, w[m][h], abort_flag, limit);
+                    out[la[m]] = g_join<V>(w[m]);
                 }
             }
         }
@@ -1295,30 +1323,30 @@ static __device__ int g_band_loop(const GP<float> &P, GBand &G, double thr, doub
             ri++;
             if (!(rdev > thr)) return t;
         }
-        if (S & 1) { float *x = G.X0; G.X0 = G.X1; G.X1 = x; }
+        if (S & 1) { V *x = G.X0; G.X0 = G.X1; G.X1 = x; }
     }
 }
 /* the iterate the loop stopped at and the one before (g_band_loop's return t >= 0) */
-template <int S>
-static __device__ inline const float *g_band_last(const GBand &G, int t) { return (t & 1) ? G.X0 : G.X1; }
-template <int S>
-static __device__ inline const float *g_band_prev(const GBand &G, int t) {
+template <int S, typename V>
+static __device__ inline const V *g_band_last(const GBand<V> &G, int t) { return (t & 1) ? G.X0 : G.X1; }
+template <int S, typename V>
+static __device__ inline const V *g_band_prev(const GBand<V> &G, int t) {
     return t == 0 ? (S == 1 ? G.X0 : G.SV) : G.X1;
 }
 
-/* reshaping_cells (cell_phase_functions.cpp:447-580) for FP32 state with K
- * workgroups per cell (blockIdx.x = n K + b), workgroup b on window rows
+/* reshaping_cells (cell_phase_functions.cpp:447-580) with K workgroups per
+ * cell (blockIdx.x = n K + b), workgroup b on window rows
  * [b W / K, (b + 1) W / K) held in LDS (g_band_loop; the window's one-pixel
- * ring, phi around the window, is the static surround).  The iterates, the
- * per-workgroup deviations and their in-order sum equal g_relax_cells_mb's
- * bit for bit.  At the end phin's window gets the last iterate and phi's
- * the second-to-last (own rows).  xw: [nc] groups of g_band_words(S, K, W)
- * words.  Dynamic LDS: g_band_lds(S, rmax, W), rmax >= the largest band;
- * K <= 64. */
-template <int S>
-__global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__restrict__ phi,
-                                                          float *__restrict__ phin, const float *__restrict__ jx,
-                                                          const float *__restrict__ jy, const int *__restrict__ xc,
+ * ring, phi around the window, is the static surround).  The iterates
+ * equal g_relax_cells_mb's bit for bit, and with the same K so do the
+ * per-workgroup deviations and their in-order sum.  At the end phin's
+ * window gets the last iterate and phi's the second-to-last (own rows).
+ * xw: [nc] groups of g_band_words(S, K, W, sizeof(V)) words.  Dynamic LDS: g_band_lds(S, rmax, W, sizeof(V)), rmax >= the
+ * largest band; K <= 64. */
+template <int S, typename V>
+__global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<V> P, V *__restrict__ phi, V *__restrict__ phin,
+                                                          const V *__restrict__ jx, const V *__restrict__ jy,
+                                                          const int *__restrict__ xc,
                                                           const int *__restrict__ yc, double thr, double rdev0,
                                                           unsigned long long *__restrict__ xw, unsigned tag0,
                                                           int *__restrict__ counts, int *__restrict__ cmax,
@@ -1330,15 +1358,15 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
     const int N = P.N, zs = P.zs, W = 2 * zs, L = W + 2;
     const int n = blockIdx.x / K, b = blockIdx.x % K;
     const size_t NN = (size_t)N * N;
-    float *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
-    const float *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
+    V *ph = phi + (size_t)n * NN, *pn = phin + (size_t)n * NN;
+    const V *jxn = jx + (size_t)n * NN, *jyn = jy + (size_t)n * NN;
     const int x0 = xc[n] - zs, y0 = yc[n] - zs;              /* window origin on the grid */
     const int r0 = b * W / K, r1 = (b + 1) * W / K, rows = r1 - r0;
-    GBand G;
+    GBand<V> G;
     G.L = L; G.bw = W; G.rows = rows; G.b = b; G.K = K;
     g_band_lds_split(G, S, rmax);
-    G.xr = xw + (size_t)n * g_band_words(S, K, W);
-    G.xd = G.xr + (size_t)2 * K * 2 * S * W;
+    G.xr = xw + (size_t)n * g_band_words(S, K, W, sizeof(V));
+    G.xd = G.xr + (size_t)2 * K * 2 * S * W * (sizeof(V) / 4);
     /* the window rows r0 - S .. r1 + S - 1 there are, and the ring rows -1
      * and W: ring pixels from phi, the window from phin */
     const int l0 = b == 0 ? S - 1 : 0, l1 = b == K - 1 ? S + rows + 1 : rows + 2 * S;
@@ -1346,7 +1374,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
         const int l = l0 + k / L, c = k % L, wr = r0 - S + l, wc = c - 1;
         const size_t g = (size_t)(x0 + wr) * N + (y0 + wc);
         const bool ring = wr < 0 || wr >= W || wc < 0 || wc >= W;
-        const float v = ring ? ph[g] : pn[g];
+        const V v = ring ? ph[g] : pn[g];
         const int q = l * L + c;
         G.X0[q] = v; G.X1[q] = v;
         G.JX[q] = jxn[g]; G.JY[q] = jyn[g];
@@ -1359,7 +1387,7 @@ __global__ __launch_bounds__(1024) void g_relax_cells_lds(GP<float> P, float *__
     if (last == -2) return;
     if (last >= 0) {
         /* phin <- the last iterate, phi <- the one before; own rows only */
-        const float *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
+        const V *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
         for (int k = threadIdx.x; k < rows * W; k += blockDim.x) {
             const int q = (k / W + S) * L + (k % W + 1);
             const size_t g = (size_t)(x0 + r0 + k / W) * N + (y0 + k % W);
@@ -1472,20 +1500,21 @@ __global__ __launch_bounds__(1024) void g_lum_loop_mb(GP<R> P, R *__restrict__ b
     if (threadIdx.x == 0 && b == 0) out[0] = count;
 }
 
-/* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop as g_lum_loop_mb, for
- * FP32 state, with workgroup b's band of box rows [r0, r1) held in LDS
+/* Sweeps 2, 3, ... of reshaping_lumen_or_ECM's loop as g_lum_loop_mb, with
+ * workgroup b's band of box rows [r0, r1) held in LDS
  * (g_band_loop; the rows and columns just outside the box are the static
- * surround, the same in both buffers).  The iterates, the per-workgroup
- * deviations and their in-order sum equal g_lum_loop_mb's bit for bit.  At
+ * surround, the same in both buffers).  The iterates equal g_lum_loop_mb's
+ * bit for bit, and with the same K so do the per-workgroup deviations and
+ * their in-order sum.  At
  * the end both buffers get the band's rows of the iterates they hold as
  * the sweeps of g_lum_loop_mb leave them (X_count in bufA for odd count).
- * xw: g_band_words(S, K, box width) words.  Dynamic LDS: g_band_lds(S,
- * rmax, box width), rmax >= the largest band; every band has S rows or
- * more; K <= 128. */
-template <int S>
-__global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__restrict__ bufA,
-                                                       float *__restrict__ bufB, const float *__restrict__ jx,
-                                                       const float *__restrict__ jy, const int *__restrict__ box,
+ * xw: g_band_words(S, K, box width, sizeof(V)) words.  Dynamic LDS:
+ * g_band_lds(S, rmax, box width, sizeof(V)), rmax >= the largest band;
+ * every band has S rows or more; K <= 128. */
+template <int S, typename V>
+__global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<V> P, V *__restrict__ bufA, V *__restrict__ bufB,
+                                                       const V *__restrict__ jx, const V *__restrict__ jy,
+                                                       const int *__restrict__ box,
                                                        double thr, double rdev, int count, int ri,
                                                        unsigned long long *__restrict__ xw, unsigned tag0,
                                                        int *__restrict__ out, int *__restrict__ relax_flag,
@@ -1497,15 +1526,15 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
     const int bi0 = box[0], bi1 = box[1], bj0 = box[2], bj1 = box[3];
     const int bw = bj1 - bj0, bh = bi1 - bi0, L = bw + 2;
     const int r0 = bi0 + b * bh / K, r1 = bi0 + (b + 1) * bh / K, rows = r1 - r0;
-    GBand G;
+    GBand<V> G;
     G.L = L; G.bw = bw; G.rows = rows; G.b = b; G.K = K;
     g_band_lds_split(G, S, rmax);
     G.xr = xw;
-    G.xd = xw + (size_t)2 * K * 2 * S * bw;
+    G.xd = xw + (size_t)2 * K * 2 * S * bw * (sizeof(V) / 4);
     /* X0: X_count (bufA for odd count), X1: the other buffer; grid row r0 -
      * S + l, column bj0 - 1 + c, for the box rows there are and the static
      * rows bi0 - 1 and bi1 */
-    const float *cur = (count & 1) ? bufA : bufB, *oth = (count & 1) ? bufB : bufA;
+    const V *cur = (count & 1) ? bufA : bufB, *oth = (count & 1) ? bufB : bufA;
     const int l0 = b == 0 ? S - 1 : 0, l1 = b == K - 1 ? S + rows + 1 : rows + 2 * S;
     for (int k = threadIdx.x; k < (l1 - l0) * L; k += blockDim.x) {
         const int l = l0 + k / L, c = k % L;
@@ -1520,8 +1549,8 @@ __global__ __launch_bounds__(1024) void g_lum_loop_lds(GP<float> P, float *__res
                                                 sums, &xfail);
     if (last == -2) return;
     if (last >= 0) {
-        const float *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
-        float *dl = (count & 1) ? bufA : bufB, *dp = (count & 1) ? bufB : bufA;
+        const V *vl = g_band_last<S>(G, last), *vp = g_band_prev<S>(G, last);
+        V *dl = (count & 1) ? bufA : bufB, *dp = (count & 1) ? bufB : bufA;
         for (int k = threadIdx.x; k < rows * bw; k += blockDim.x) {
             const int q = (k / bw + S) * L + (k % bw + 1);
             const size_t g = (size_t)(r0 + k / bw) * N + (bj0 + k % bw);
@@ -1823,7 +1852,7 @@ struct GenCtx final : Generic {
     unsigned *mb_ctr = nullptr;      /* [nc] cell groups, [nc] the lumen/ECM group, [nc + 1] abort */
     int *mb_out = nullptr, *mb_box = nullptr;
     R *mb_xch = nullptr;             /* exchanged band rows: cells [nc][2][kc][2][W + 2]; lumen/ECM [2][kl][2][N] */
-    /* FP32 state: the cell loop with its bands in LDS and tagged exchanges
+    /* the cell loop with its bands in LDS and tagged exchanges
      * (g_relax_cells_lds): workgroups per cell (0: not used), the largest
      * band, its LDS bytes, the tagged words, the launch's largest sweep
      * count, and the next launch's tag base */
@@ -1831,7 +1860,7 @@ struct GenCtx final : Generic {
     size_t lds_l = 0;
     unsigned long long *mb_tw = nullptr;
     /* and the lumen/ECM loop's (g_lum_loop_lds): the most workgroups (0: not
-     * used) and its tagged words, [2][klm][2][N] rows + [2][klm][2] halves */
+     * used) and its tagged words (g_band_words for klm bands of N columns) */
     int klm = 0;
     unsigned long long *mb_lw = nullptr;
     int *mb_cmax = nullptr;
@@ -1910,7 +1939,7 @@ struct GenCtx final : Generic {
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
         GTRY(A_(&mb_out, 1)); GTRY(A_(&mb_box, 4));
         GTRY(A_(&mb_xch, std::max((size_t)nc * 2 * kc * 2 * (W + 2), (size_t)2 * kl * 2 * N)));
-        if (std::is_same<R, float>::value) {
+        {
             int ncu = 0, dev = 0;
             GHIP(hipGetDevice(&dev));
             GHIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1919,17 +1948,17 @@ struct GenCtx final : Generic {
             kcl = std::min(std::min(ncu / nc, W / 4), 64);
             if (kcl >= 2) {
                 rmax_l = (W + kcl - 1) / kcl;
-                lds_l = g_band_lds(ORG_GEN_SPAN_CELLS, rmax_l, W);
+                lds_l = g_band_lds(ORG_GEN_SPAN_CELLS, rmax_l, W, sizeof(R));
                 if (lds_l > ORG_GEN_LDS_MAX) kcl = 0;
             } else {
                 kcl = 0;
             }
             if (kcl) {
-                const size_t words = (size_t)nc * g_band_words(ORG_GEN_SPAN_CELLS, kcl, W);
+                const size_t words = (size_t)nc * g_band_words(ORG_GEN_SPAN_CELLS, kcl, W, sizeof(R));
                 GTRY(A_(&mb_tw, words));
                 GTRY(A_(&mb_cmax, 1));
                 GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
-                GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds<ORG_GEN_SPAN_CELLS>,
+                GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l));
             }
             /* the lumen/ECM loop: one band of box rows per workgroup, up to
@@ -1939,10 +1968,10 @@ struct GenCtx final : Generic {
              * launch checks that its bands fit */
             klm = std::min(std::min(ncu, 128), N - 2);
             if (klm >= 2) {
-                const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N);
+                const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N, sizeof(R));
                 GTRY(A_(&mb_lw, words));
                 GHIP(hipMemset(mb_lw, 0, sizeof(unsigned long long) * words));
-                GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds<ORG_GEN_SPAN_LUM>,
+                GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, ORG_GEN_LDS_MAX));
             } else {
                 klm = 0;
@@ -2181,11 +2210,11 @@ struct GenCtx final : Generic {
     }
     /* workgroups of g_lum_loop_lds for a bh x bw box: as many as allowed,
      * ORG_GEN_SPAN_LUM rows or more each; 0 (the exchange-buffer loop runs)
-     * for FP64 state or when the bands do not fit in LDS */
+     * when the bands do not fit in LDS */
     int lum_lds_blocks(int bh, int bw) const {
-        if (!std::is_same<R, float>::value || klm < 2) return 0;
+        if (klm < 2) return 0;
         const int kb = std::min(klm, bh / ORG_GEN_SPAN_LUM);
-        if (kb < 2 || g_band_lds(ORG_GEN_SPAN_LUM, (bh + kb - 1) / kb, bw) > ORG_GEN_LDS_MAX) return 0;
+        if (kb < 2 || g_band_lds(ORG_GEN_SPAN_LUM, (bh + kb - 1) / kb, bw, sizeof(R)) > ORG_GEN_LDS_MAX) return 0;
         return kb;
     }
     int reshape_lumecm(int which, int *failed) override {
@@ -2227,15 +2256,14 @@ struct GenCtx final : Generic {
                     count = 2;
                     if (ri > p.rimax) *failed = 1;
                 } else if (int kb = lum_lds_blocks(bh, bw)) {
-                    /* sweeps 2, 3, ... with the bands in LDS (FP32 state) */
+                    /* sweeps 2, 3, ... with the bands in LDS */
                     const int rows_max = (bh + kb - 1) / kb;
                     GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr + nc, 2);
                     GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
-                    if constexpr (std::is_same<R, float>::value)
-                        GLAUNCH(g_lum_loop_lds<ORG_GEN_SPAN_LUM>, dim3(kb), dim3(1024),
-                                g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw), P, u, un, (const float *)jlx,
-                                (const float *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_lw, tag_next,
-                                mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
+                    GLAUNCH((g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>), dim3(kb), dim3(1024),
+                            g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw, sizeof(R)), P, u, un, (const R *)jlx,
+                            (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_lw, tag_next, mb_out, relax,
+                            (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
                     int h[3] = {0, 0, 0};
                     GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
                     GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2299,28 +2327,26 @@ struct GenCtx final : Generic {
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
         GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
-        if constexpr (std::is_same<R, float>::value) {
-            if (kcl) {
-                GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
-                GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)mb_cmax, 1);
-                GLAUNCH(g_relax_cells_lds<ORG_GEN_SPAN_CELLS>, dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin, (const R *)jx,
-                        (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next, counts,
-                        mb_cmax, relax, (int *)(mb_ctr + nc + 1), wait_limit, kcl, rmax_l);
-                int f[3] = {0, 0, 0};
-                GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-                GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-                GHIP(hipMemcpyAsync(&f[2], mb_cmax, sizeof(int), hipMemcpyDeviceToHost, s));
-                GHIP(hipStreamSynchronize(s));
-                if (f[1]) {
-                    tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
-                    fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
-                                    "for more than 2 s (not all resident?)\n");
-                    return ORG_ETIMEOUT;
-                }
-                tag_next += (unsigned)f[2] + 2u;
-                *failed = f[0];
-                return ORG_OK;
+        if (kcl) {
+            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)mb_cmax, 1);
+            GLAUNCH((g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>), dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin,
+                    (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next,
+                    counts, mb_cmax, relax, (int *)(mb_ctr + nc + 1), wait_limit, kcl, rmax_l);
+            int f[3] = {0, 0, 0};
+            GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
+            GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            GHIP(hipMemcpyAsync(&f[2], mb_cmax, sizeof(int), hipMemcpyDeviceToHost, s));
+            GHIP(hipStreamSynchronize(s));
+            if (f[1]) {
+                tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
+                fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
+                                "for more than 2 s (not all resident?)\n");
+                return ORG_ETIMEOUT;
             }
+            tag_next += (unsigned)f[2] + 2u;
+            *failed = f[0];
+            return ORG_OK;
         }
         if (kc > 1) {
             GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);

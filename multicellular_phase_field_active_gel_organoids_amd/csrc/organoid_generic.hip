@@ -320,19 +320,21 @@ __global__ void g_hfunc_lumecm(GP<R> P, const R *__restrict__ lum, const R *__re
 }
 
 /* boundary_conditions_cells, useful_functions.cpp:391-420: periodic ghost
- * ring (period N-2) of `planes` planes; block p does plane p. */
+ * ring (period N-2) of `planes` planes; workgroup (p, y) does plane p's
+ * ring indices 1 + y blockDim.x + t, 1 + (y + gridDim.y) blockDim.x + t,
+ * ... (every value read is an interior pixel, every one written a ring
+ * pixel: the workgroups need no order among them). */
 template <typename R>
 __global__ void g_bc_cells(int N, R *__restrict__ t) {
     const size_t NN = (size_t)N * N;
     R *q = t + (size_t)blockIdx.x * NN;
-    for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x) {
+    for (int i = 1 + blockIdx.y * blockDim.x + threadIdx.x; i < N - 1; i += gridDim.y * blockDim.x) {
         q[(size_t)i * N] = q[(size_t)i * N + N - 2];
         q[(size_t)i * N + N - 1] = q[(size_t)i * N + 1];
         q[i] = q[(size_t)N * (N - 2) + i];
         q[(size_t)N * (N - 1) + i] = q[(size_t)N + i];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (blockIdx.y == 0 && threadIdx.x == 0) {
         q[0] = q[(size_t)N * (N - 2) + N - 2];
         q[N - 1] = q[(size_t)N * (N - 2) + 1];
         q[(size_t)N * (N - 1)] = q[(size_t)N + N - 2];
@@ -2111,7 +2113,7 @@ struct GenCtx final : Generic {
     /* workgroups for a launch of one row per workgroup (at most 65535) */
     static int row_blocks(int rows) { return std::max(1, std::min(rows, 65535)); }
     int bc(R *planes, int nplanes) {
-        GLAUNCH((g_bc_cells<R>), dim3(nplanes), dim3(256), 0, N, planes);
+        GLAUNCH((g_bc_cells<R>), dim3(nplanes, std::max(1, (N - 2 + 63) / 64)), dim3(64), 0, N, planes);
         return ORG_OK;
     }
     int window_check() const { return win_err; }

@@ -961,7 +961,7 @@ int orc_reshaping_cells(orc_state *st) {
                 for (int j = y0; j < y1; j++)
                     rdev += (double)(fabs(pn[G(i, j)] - ph[G(i, j)]) * dx * dx / p->n_current);
             count++;
-            if (ri > p->rimax) { relax_fail = 1; break; }             /* :565-568 */
+            if (ri > p->rimax) { relax_fail++; break; }               /* :565-568, one report per cell */
             ri++;
         }
         st->count_ri[n] = count;

@@ -407,6 +407,27 @@ def test_cell_reshaping_keeps_second_to_last_iterate():
         assert dev > 0               # ... is not copied back into phi
 
 
+def test_cell_reshaping_counts_each_failed_loop():
+    """With rimax = 1 every cell loop passes its limit after 3 sweeps
+    (cell_phase_functions.cpp:565-568 reports each one): the oracle counts
+    one failure per cell and step, whichever threads ran the cells (more
+    cells than threads, so some thread runs several)."""
+    nc = orc.threads() + 3
+    wl = make_workload("hex", N=512, ncell=nc)
+    p = wl.params.copy()
+    p.rimax = 1
+    o = orc.Oracle(p, wl.s0_cell)
+    o.load_workload(wl)
+    expected = 0
+    for it in range(2):
+        o.step(it)
+        cells, lum, ecm = o.iterations()
+        assert (cells == 3).all(), it
+        # a lumen/ECM loop passes the limit too once it reaches its 3rd sweep (:862-865)
+        expected += nc + int(lum >= 3) + int(ecm >= 3)
+    assert o.relax_failures() == expected
+
+
 def test_lumen_reshaping_loop_rules():
     wl = make_workload("two_cells_256")
     p = wl.params

@@ -422,6 +422,46 @@ __global__ void g_normals(GP<R> P, const R *__restrict__ phi, const R *__restric
         }
 }
 
+/* g_norm_grad and g_normals on each cell's window grown by one pixel only
+ * (workgroup (r, n): row xc - zs - 1 + r of cell n, columns yc - zs - 1 ..
+ * yc + zs): the same arithmetic at every pixel there.  Inside one step
+ * these are the only pixels read (the spectral box and the update on the
+ * window, the reshaping's normals on the window and its ring), and the
+ * window check keeps them inside rows / columns 1..N-2. */
+template <typename R>
+__global__ void g_norm_grad_win(GP<R> P, const R *__restrict__ phi, const int *__restrict__ xc,
+                                const int *__restrict__ yc, R *__restrict__ ngp) {
+    const int N = P.N, n = blockIdx.y, zs = P.zs;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx;
+    const int i = xc[n] - zs - 1 + blockIdx.x;
+    for (int j = yc[n] - zs - 1 + threadIdx.x; j <= yc[n] + zs; j += blockDim.x) {
+        const R a = phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)];
+        const R c = phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)];
+        ngp[GPL(n, i, j)] = sqrt((1.0 / dx / dx / 4.0) * (a * a + c * c));
+    }
+}
+template <typename R>
+__global__ void g_normals_win(GP<R> P, const R *__restrict__ phi, const R *__restrict__ ngp,
+                              const int *__restrict__ xc, const int *__restrict__ yc, R *__restrict__ jx,
+                              R *__restrict__ jy) {
+    const int N = P.N, n = blockIdx.y, zs = P.zs;
+    const size_t NN = (size_t)N * N;
+    const R dx = P.dx;
+    const int i = xc[n] - zs - 1 + blockIdx.x;
+    for (int j = yc[n] - zs - 1 + threadIdx.x; j <= yc[n] + zs; j += blockDim.x) {
+        const size_t k = GPL(n, i, j);
+        const R g = ngp[k];
+        if (g > 0.00) {
+            jx[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i + 1, j)] - phi[GPL(n, i - 1, j)]) / g;
+            jy[k] = (1.0 / (2.0 * dx)) * (phi[GPL(n, i, j + 1)] - phi[GPL(n, i, j - 1)]) / g;
+        } else {
+            jx[k] = 0.0;
+            jy[k] = 0.0;
+        }
+    }
+}
+
 /* create_turnover_rates_tensions, rho_functions.cpp:286-405, at one pixel:
  * 0 apical, 1 basal, 2 lateral (ring neighbours n-1, n+1, strict '>'). */
 template <typename R>
@@ -2162,11 +2202,21 @@ struct GenCtx final : Generic {
     /* compute_normal_grad_phi + boundary_conditions_cells (run_with_active_gels.cpp:135-136) */
     int norm_grad() {
         if (in_step && ngp_fresh) return ORG_OK;
-        GLAUNCH((g_norm_grad<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, ngp);
-        GTRY(bc(ngp, nc));
+        if (windows_only()) {
+            GLAUNCH((g_norm_grad_win<R>), dim3(W + 2, nc), dim3(256), 0, P, (const R *)phi, (const int *)xc,
+                    (const int *)yc, ngp);
+        } else {
+            GLAUNCH((g_norm_grad<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, ngp);
+            GTRY(bc(ngp, nc));
+        }
         ngp_fresh = in_step;
         return ORG_OK;
     }
+    /* inside one step (step_scope) |grad phi| and the normals are read only
+     * on each window grown by one (the spectral box, M = W - 2, lies inside):
+     * they are formed there alone, without the planes' periodic rings; the
+     * per-function entry points form them on every cell's whole plane */
+    bool windows_only() const { return in_step && M <= W; }
     void step_scope(bool on) override {
         in_step = on;
         ngp_fresh = false;
@@ -2322,9 +2372,15 @@ struct GenCtx final : Generic {
         GTRY(window_check());
         if (from_phi) GTRY(copy(phin, phi, PP));
         GTRY(norm_grad());
-        GLAUNCH((g_normals<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, (const R *)ngp, jx, jy);
-        GTRY(bc(jx, nc));                                             /* run_with_active_gels.cpp:164-166 */
-        GTRY(bc(jy, nc));
+        if (windows_only()) {
+            GLAUNCH((g_normals_win<R>), dim3(W + 2, nc), dim3(256), 0, P, (const R *)phi, (const R *)ngp,
+                    (const int *)xc, (const int *)yc, jx, jy);
+        } else {
+            GLAUNCH((g_normals<R>), dim3(row_blocks(N - 2), nc), dim3(256), 0, P, (const R *)phi, (const R *)ngp, jx,
+                    jy);
+            GTRY(bc(jx, nc));                                         /* run_with_active_gels.cpp:164-166 */
+            GTRY(bc(jy, nc));
+        }
         const R rtol = (R)p.rtol, Rr = (R)p.R, wi = (R)p.wi, rdt = (R)p.rdt;
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */

@@ -399,6 +399,37 @@ __global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ h
     }
 }
 
+#define ORG_SUMS_PARTS 64
+
+/* sum_s_kernels for FP64 state on many workgroups: workgroup (b, n) adds
+ * cell n's pixels [b per, (b + 1) per) (each thread its strided share,
+ * then the block sum) into part[n][b]; g_sum_s_fin adds the B parts in
+ * order.  A fixed order, so the same value at every call; it differs from
+ * g_sum_s's order only in the last bits of the area (which the FP64 state
+ * carries on in double). */
+__global__ __launch_bounds__(1024) void g_sum_s_parts(GP<double> P, const double *__restrict__ hf, size_t per,
+                                                      double *__restrict__ part) {
+    __shared__ double red[16];
+    const size_t NN = (size_t)P.N * P.N;
+    const int n = blockIdx.y, b = blockIdx.x;
+    const double dx = P.dx;
+    const double *h = hf + (size_t)n * NN;
+    const size_t k0 = (size_t)b * per, k1 = min(NN, k0 + per);
+    double acc = 0.0;
+    for (size_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += (double)(h[k] * dx * dx);
+    const double tot = gblock_sum(acc, red);
+    if (threadIdx.x == 0) part[(size_t)n * gridDim.x + b] = tot;
+}
+__global__ void g_sum_s_fin(GP<double> P, const double *__restrict__ part, int nb, const double *__restrict__ s0,
+                            double *__restrict__ s, double *__restrict__ f0) {
+    for (int n = threadIdx.x; n < P.nc; n += blockDim.x) {
+        double tot = 0.0;
+        for (int b = 0; b < nb; b++) tot += part[(size_t)n * nb + b];
+        s[n] = tot;
+        f0[n] = (double)(P.alpha * ((double)s0[n] - (double)tot));
+    }
+}
+
 /* compute_normal_to_phi, useful_functions.cpp:356-388 (interior; the ring
  * by g_bc_cells): workgroup (x, n) on rows x + 1, x + 1 + gridDim.x, ... of
  * plane n */
@@ -1907,6 +1938,7 @@ struct GenCtx final : Generic {
     unsigned long long *mb_lw = nullptr;
     int *mb_cmax = nullptr;
     unsigned long long *cen_acc = nullptr;   /* the centroid sums, [nc][3] */
+    double *sparts = nullptr;                /* FP64 areas' parts, [nc][ORG_SUMS_PARTS] */
     unsigned tag_next = 1;
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
@@ -1976,6 +2008,7 @@ struct GenCtx final : Generic {
         GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
         GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&err, 1)); GTRY(A_(&counts, nc)); GTRY(A_(&relax, 1));
         GTRY(A_(&cen_acc, (size_t)3 * nc));
+        if (std::is_same<R, double>::value) GTRY(A_(&sparts, (size_t)ORG_SUMS_PARTS * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
@@ -2196,7 +2229,17 @@ struct GenCtx final : Generic {
         return ORG_OK;
     }
     int sum_s() override {
-        GLAUNCH((g_sum_s<R>), dim3(nc), dim3(1024), 0, P, (const R *)hf, (const double *)s0, s_, f0);
+        if constexpr (std::is_same<R, double>::value) {
+            /* FP64 state: 64 workgroups per cell (one per cell is bound by a
+             * CU's loads in flight); FP32 keeps g_sum_s's order, whose sum
+             * the float build's f0 rounds the same way */
+            const size_t per = ((size_t)N * N + ORG_SUMS_PARTS - 1) / ORG_SUMS_PARTS;
+            GLAUNCH(g_sum_s_parts, dim3(ORG_SUMS_PARTS, nc), dim3(1024), 0, P, (const double *)hf, per, sparts);
+            GLAUNCH(g_sum_s_fin, dim3(1), dim3(64), 0, P, (const double *)sparts, ORG_SUMS_PARTS, (const double *)s0,
+                    s_, f0);
+        } else {
+            GLAUNCH((g_sum_s<R>), dim3(nc), dim3(1024), 0, P, (const R *)hf, (const double *)s0, s_, f0);
+        }
         return ORG_OK;
     }
     /* compute_normal_grad_phi + boundary_conditions_cells (run_with_active_gels.cpp:135-136) */

@@ -1893,6 +1893,17 @@ static int tune_fft(hipStream_t s, FftPlan &pl, Cx<R> *a, Cx<R> *b, Cx<double> *
     return ORG_OK;
 }
 
+/* The values the host reads back at a sync point of the step, in one
+ * device block so that one copy (into pinned host memory) brings them all:
+ * each separate small copy costs ~25 us of idle GPU. */
+struct GCtl {
+    double rdev;               /* reshape_lumecm: the first sweep's deviation */
+    int box[4];                /* reshape_lumecm: the normals' box */
+    int relax, abrt, cmax, count;   /* a loop launch: relax flag, abort flag, largest count, count */
+    int err;                   /* find_centres: the window check */
+    int pad[3];
+};
+
 template <typename R>
 struct GenCtx final : Generic {
     org_params p{};
@@ -1913,7 +1924,7 @@ struct GenCtx final : Generic {
     Cx<double> *work = nullptr;
     R *vxb = nullptr, *vyb = nullptr, *win = nullptr;
     R *f0 = nullptr;
-    double *s_ = nullptr, *s0 = nullptr, *sum_e = nullptr, *part = nullptr, *dev_sum = nullptr;
+    double *s_ = nullptr, *s0 = nullptr, *sum_e = nullptr, *part = nullptr;
     int *xc = nullptr, *yc = nullptr, *err = nullptr, *counts = nullptr, *relax = nullptr;
     int nparts = 1;
     /* the loops split over several workgroups: workgroups per cell / for
@@ -1922,7 +1933,7 @@ struct GenCtx final : Generic {
     int kc = 1, kl = 1;
     R *swin = nullptr;
     double *mb_part = nullptr;
-    unsigned *mb_ctr = nullptr;      /* [nc] cell groups, [nc] the lumen/ECM group, [nc + 1] abort */
+    unsigned *mb_ctr = nullptr;      /* arrival counters: [nc] cell groups, [nc] the lumen/ECM group */
     int *mb_out = nullptr, *mb_box = nullptr;
     R *mb_xch = nullptr;             /* exchanged band rows: cells [nc][2][kc][2][W + 2]; lumen/ECM [2][kl][2][N] */
     /* the cell loop with its bands in LDS and tagged exchanges
@@ -1945,9 +1956,24 @@ struct GenCtx final : Generic {
     bool in_step = false, ngp_fresh = false;   /* step_scope: ngp formed in this step from the current phi */
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
 
+    GCtl *ctl = nullptr, *hctl = nullptr;    /* on the device; its pinned host mirror */
+    int *abrt = nullptr;                     /* &ctl->abrt: the loops' abort flag */
+
     ~GenCtx() override {
         if (s) (void)hipStreamSynchronize(s);
         for (void *q : allocs) (void)hipFree(q);
+        if (hctl) (void)hipHostFree(hctl);
+    }
+    /* ctl -> hctl and wait: the one readback of a sync point */
+    int read_ctl() {
+        GHIP(hipMemcpyAsync(hctl, ctl, sizeof(GCtl), hipMemcpyDeviceToHost, s));
+        GHIP(hipStreamSynchronize(s));
+        return ORG_OK;
+    }
+    /* a loop launch's relax flag, abort flag, largest count and count to zero (one fill kernel) */
+    int zero_loop_ctl() {
+        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)&ctl->relax, 4);
+        return ORG_OK;
     }
     int real_bytes() const override { return (int)sizeof(R); }
 
@@ -2005,14 +2031,18 @@ struct GenCtx final : Generic {
             }
         }
         GTRY(A_(&f0, nc)); GTRY(A_(&s_, nc)); GTRY(A_(&s0, nc)); GTRY(A_(&sum_e, 1));
-        GTRY(A_(&part, nparts)); GTRY(A_(&dev_sum, 1));
-        GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&err, 1)); GTRY(A_(&counts, nc)); GTRY(A_(&relax, 1));
+        GTRY(A_(&part, nparts));
+        GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&counts, nc));
+        GTRY(A_(&ctl, 1));
+        GHIP(hipHostMalloc((void **)&hctl, sizeof(GCtl)));
+        err = &ctl->err; relax = &ctl->relax; abrt = &ctl->abrt;
+        mb_out = &ctl->count; mb_box = ctl->box; mb_cmax = &ctl->cmax;
         GTRY(A_(&cen_acc, (size_t)3 * nc));
         if (std::is_same<R, double>::value) GTRY(A_(&sparts, (size_t)ORG_SUMS_PARTS * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
-        GTRY(A_(&mb_ctr, (size_t)nc + 2));
-        GTRY(A_(&mb_out, 1)); GTRY(A_(&mb_box, 4));
+        GTRY(A_(&mb_ctr, (size_t)nc + 1));
+
         GTRY(A_(&mb_xch, std::max((size_t)nc * 2 * kc * 2 * (W + 2), (size_t)2 * kl * 2 * N)));
         {
             int ncu = 0, dev = 0;
@@ -2031,7 +2061,6 @@ struct GenCtx final : Generic {
             if (kcl) {
                 const size_t words = (size_t)nc * g_band_words(ORG_GEN_SPAN_CELLS, kcl, W, sizeof(R));
                 GTRY(A_(&mb_tw, words));
-                GTRY(A_(&mb_cmax, 1));
                 GHIP(hipMemset(mb_tw, 0, sizeof(unsigned long long) * words));
                 GHIP(hipFuncSetAttribute((const void *)g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l));
@@ -2062,8 +2091,7 @@ struct GenCtx final : Generic {
         for (R *f : {lum, lumn, ecm, ecmn, psi_, hl, he, ngl, jlx, jly}) GHIP(hipMemset(f, 0, sizeof(R) * NN));
         GHIP(hipMemset(vxb, 0, sizeof(R) * nc * MM)); GHIP(hipMemset(vyb, 0, sizeof(R) * nc * MM));
         GHIP(hipMemset(f0, 0, sizeof(R) * nc)); GHIP(hipMemset(s_, 0, sizeof(double) * nc));
-        GHIP(hipMemset(err, 0, sizeof(int))); GHIP(hipMemset(counts, 0, sizeof(int) * nc));
-        GHIP(hipMemset(relax, 0, sizeof(int)));
+        GHIP(hipMemset(ctl, 0, sizeof(GCtl))); GHIP(hipMemset(counts, 0, sizeof(int) * nc));
         GHIP(hipMemset(A, 0, sizeof(Cx<R>) * 5 * nc * MM));
         GHIP(hipMemset(B, 0, sizeof(Cx<R>) * 5 * nc * MM));
         GHIP(hipDeviceSynchronize());          /* the memsets above ran on the null stream */
@@ -2203,14 +2231,14 @@ struct GenCtx final : Generic {
         return ORG_OK;
     }
     int find_centres() override {
-        GHIP(hipMemsetAsync(err, 0, sizeof(int), s));
+        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)err, 1);
         GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
         GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P,
                 (const R *)phi, cen_acc);
         GLAUNCH((g_centroid_fin<R>), dim3(1), dim3(64), 0, P, (const unsigned long long *)cen_acc, xc, yc, err);
         int e = 0;
-        GHIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
-        GHIP(hipStreamSynchronize(s));
+        GTRY(read_ctl());
+        e = hctl->err;
         if (e == ORG_EEMPTY_CELL) return e;
         win_err = e;
         return e;
@@ -2289,16 +2317,15 @@ struct GenCtx final : Generic {
         GLAUNCH((g_spec_commit<R>), g, b, 0, P, (const int *)xc, (const int *)yc, (const Cx<R> *)(A + 4 * pl), rhon);
         return bc(rhon, nc);                                                           /* :469 */
     }
-    /* fixed-order double sum (mode 0: h dx^2 over a; 1: |a - b| dx^2) into dev_sum */
-    int plane_sum(const R *a, const R *b2, int mode) {
+    /* fixed-order double sum (mode 0: h dx^2 over a; 1: |a - b| dx^2) into *out */
+    int plane_sum(const R *a, const R *b2, int mode, double *out) {
         const size_t per = (NN + nparts - 1) / nparts;
         GLAUNCH((g_sum_parts<R>), dim3(nparts), dim3(1024), 0, a, b2, P.dx, NN, per, mode, part);
-        GLAUNCH(g_sum_final, dim3(1), dim3(64), 0, (const double *)part, nparts, dev_sum);
+        GLAUNCH(g_sum_final, dim3(1), dim3(64), 0, (const double *)part, nparts, out);
         return ORG_OK;
     }
     int lumecm_update() override {
-        GTRY(plane_sum(he, nullptr, 0));                                               /* :376-386 */
-        GHIP(hipMemcpyAsync(sum_e, dev_sum, sizeof(double), hipMemcpyDeviceToDevice, s));
+        GTRY(plane_sum(he, nullptr, 0, sum_e));                                        /* :376-386 */
         GLAUNCH((g_lumecm_update<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)lum, (const R *)ecm,
                 (const R *)psi_, (const R *)hl, (const R *)he, (const double *)sum_e, lumn, ecmn);
         return ORG_OK;
@@ -2331,13 +2358,12 @@ struct GenCtx final : Generic {
             /* sweep 1 on the whole grid: lum <- lumn (the role change), lumn <- F(lum) into lum's buffer */
             GLAUNCH((g_lum_sweep<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)un, u, (const R *)jlx,
                     (const R *)jly, side);
-            GTRY(plane_sum(u, un, 1));                                                        /* :848-855 */
-            GHIP(hipMemcpyAsync(&rdev, dev_sum, sizeof(double), hipMemcpyDeviceToHost, s));
+            GTRY(plane_sum(u, un, 1, &ctl->rdev));                                            /* :848-855 */
             GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, mb_box, N);
             GLAUNCH((g_nz_box<R>), dim3(std::max(1, N - 2)), dim3(256), 0, N, (const R *)jlx, (const R *)jly, mb_box);
-            int bx[4];
-            GHIP(hipMemcpyAsync(bx, mb_box, sizeof(bx), hipMemcpyDeviceToHost, s));
-            GHIP(hipStreamSynchronize(s));
+            GTRY(read_ctl());                                                                 /* rdev and the box */
+            rdev = hctl->rdev;
+            const int bx[4] = {hctl->box[0], hctl->box[1], hctl->box[2], hctl->box[3]};
             count = 1;
             if (ri > p.rimax) *failed = 1;                                                    /* :862-865 */
             else ri++;
@@ -2353,17 +2379,13 @@ struct GenCtx final : Generic {
                 } else if (int kb = lum_lds_blocks(bh, bw)) {
                     /* sweeps 2, 3, ... with the bands in LDS */
                     const int rows_max = (bh + kb - 1) / kb;
-                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr + nc, 2);
-                    GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+                    GTRY(zero_loop_ctl());
                     GLAUNCH((g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>), dim3(kb), dim3(1024),
                             g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw, sizeof(R)), P, u, un, (const R *)jlx,
                             (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_lw, tag_next, mb_out, relax,
-                            (int *)(mb_ctr + nc + 1), wait_limit, kb, rows_max);
-                    int h[3] = {0, 0, 0};
-                    GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipStreamSynchronize(s));
+                            abrt, wait_limit, kb, rows_max);
+                    GTRY(read_ctl());
+                    const int h[3] = {hctl->count, hctl->relax, hctl->abrt};
                     if (h[2]) {
                         tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
                         fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
@@ -2379,16 +2401,13 @@ struct GenCtx final : Generic {
                     const int k = std::max(1, std::min(std::min(kl, (bh + 7) / 8),
                                                        (int)(((size_t)bh * bw + 8191) / 8192)));
                     const size_t shm = 2 * sizeof(R) * (size_t)(bw + 2);
-                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
-                    GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 1);
+                    GTRY(zero_loop_ctl());
                     GLAUNCH((g_lum_loop_mb<R>), dim3(k), dim3(1024), shm, P, u, un, (const R *)jlx,
                             (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_xch, mb_part,
-                            mb_ctr + nc, mb_out, relax, (int *)(mb_ctr + nc + 1), wait_limit, k);
-                    int h[3] = {0, 0, 0};
-                    GHIP(hipMemcpyAsync(&h[0], mb_out, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipMemcpyAsync(&h[1], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipMemcpyAsync(&h[2], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-                    GHIP(hipStreamSynchronize(s));
+                            mb_ctr + nc, mb_out, relax, abrt, wait_limit, k);
+                    GTRY(read_ctl());
+                    const int h[3] = {hctl->count, hctl->relax, hctl->abrt};
                     if (h[2]) {
                         fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
                                         "other for more than 2 s (not all resident?)\n");
@@ -2427,18 +2446,13 @@ struct GenCtx final : Generic {
         const R rtol = (R)p.rtol, Rr = (R)p.R, wi = (R)p.wi, rdt = (R)p.rdt;
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
-        GHIP(hipMemsetAsync(relax, 0, sizeof(int), s));
+        GTRY(zero_loop_ctl());
         if (kcl) {
-            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
-            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)mb_cmax, 1);
             GLAUNCH((g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>), dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin,
                     (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next,
-                    counts, mb_cmax, relax, (int *)(mb_ctr + nc + 1), wait_limit, kcl, rmax_l);
-            int f[3] = {0, 0, 0};
-            GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-            GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-            GHIP(hipMemcpyAsync(&f[2], mb_cmax, sizeof(int), hipMemcpyDeviceToHost, s));
-            GHIP(hipStreamSynchronize(s));
+                    counts, mb_cmax, relax, abrt, wait_limit, kcl, rmax_l);
+            GTRY(read_ctl());
+            const int f[3] = {hctl->relax, hctl->abrt, hctl->cmax};
             if (f[1]) {
                 tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
                 fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
@@ -2450,18 +2464,16 @@ struct GenCtx final : Generic {
             return ORG_OK;
         }
         if (kc > 1) {
-            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 2);
+            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 1);
             GLAUNCH((g_relax_cells_mb<R>), dim3(nc * kc), dim3(1024), 2 * sizeof(R) * (W + 2), P, phi, phin,
                     (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, swin, mb_xch,
-                    mb_part, mb_ctr, counts, relax, (int *)(mb_ctr + nc + 1), wait_limit, kc);
+                    mb_part, mb_ctr, counts, relax, abrt, wait_limit, kc);
         } else {
             GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
                     (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
         }
-        int f[2] = {0, 0};
-        GHIP(hipMemcpyAsync(&f[0], relax, sizeof(int), hipMemcpyDeviceToHost, s));
-        if (kc > 1) GHIP(hipMemcpyAsync(&f[1], mb_ctr + nc + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        GHIP(hipStreamSynchronize(s));
+        GTRY(read_ctl());
+        const int f[2] = {hctl->relax, hctl->abrt};
         if (f[1]) {
             fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
                             "for more than 2 s (not all resident?)\n");

@@ -207,9 +207,10 @@ __global__ __launch_bounds__(256) void g_fft_lines(const Cx<TI> *src, Cx<TO> *ds
             }
             __syncthreads();
             lds_fft_pow2<F>(re, im, L, +1, tw, t0, nt);
+            const double rL = 1.0 / L;     /* L a power of two: x * rL is x / L exactly */
             if (act)
                 for (int k = t0; k < M; k += nt) {
-                    const double cr = re[k] / L, ci = im[k] / L, wr = chirp[2 * k], wi = chirp[2 * k + 1];
+                    const double cr = re[k] * rL, ci = im[k] * rL, wr = chirp[2 * k], wi = chirp[2 * k + 1];
                     Cx<TO> o;
                     o.x = (TO)(cr * wr - ci * wi);
                     o.y = (TO)(cr * wi + ci * wr);

@@ -1011,8 +1011,13 @@ static int gen_step_body(org_ctx *c) {
     TRY(gen_rc(g->sum_s()));                             /* :142 */
     TRY(gen_rc(g->cortices()));                          /* :130-154 */
     TRY(gen_rc(g->lumecm_update()));                     /* :157-158 */
-    TRY(gen_reshape(c, 0));                              /* :160 */
-    TRY(gen_reshape(c, 1));                              /* :161 */
+    {                                                    /* :160-161 */
+        int failed[2] = {0, 0};
+        const int rc = g->reshape_lumecm_pair(failed);
+        for (int f = 0; f < 2; f++)
+            if (failed[f]) note_relax_failure(c);
+        TRY(gen_rc(rc));
+    }
     TRY(gen_rc(g->update_cells()));                      /* :169-170 */
     TRY(gen_reshape(c, 2));                              /* :164-166, :172 */
     g->step_scope(false);                                /* phi changes from here on */

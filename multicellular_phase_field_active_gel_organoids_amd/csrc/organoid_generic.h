@@ -36,6 +36,9 @@ struct Generic {
     virtual int lumecm_update() = 0;
     /* *relax_failed is set to 1 when the loop passed rimax */
     virtual int reshape_lumecm(int which_ecm, int *relax_failed) = 0;
+    /* the lumen's then the ECM's loop (neither reads the other's field),
+     * their host round trips shared; relax_failed[2] */
+    virtual int reshape_lumecm_pair(int *relax_failed) = 0;
     virtual int update_cells() = 0;
     virtual int reshape_cells(int from_phi, int *relax_failed) = 0;
     virtual int commit_rho() = 0;

@@ -1897,10 +1897,13 @@ static int tune_fft(hipStream_t s, FftPlan &pl, Cx<R> *a, Cx<R> *b, Cx<double> *
 /* The values the host reads back at a sync point of the step, in one
  * device block so that one copy (into pinned host memory) brings them all:
  * each separate small copy costs ~25 us of idle GPU. */
-struct GCtl {
-    double rdev;               /* reshape_lumecm: the first sweep's deviation */
-    int box[4];                /* reshape_lumecm: the normals' box */
+struct GLoopCtl {
     int relax, abrt, cmax, count;   /* a loop launch: relax flag, abort flag, largest count, count */
+};
+struct GCtl {
+    double rdev[2];            /* reshape_lumecm: each field's first-sweep deviation (0 lumen, 1 ECM) */
+    int box[2][4];             /* reshape_lumecm: each field's box of nonzero normals */
+    GLoopCtl loop[2];          /* [0]: the cell loop's or the lumen loop's, [1]: the ECM loop's */
     int err;                   /* find_centres: the window check */
     int pad[3];
 };
@@ -1919,7 +1922,7 @@ struct GenCtx final : Generic {
     R *jx = nullptr, *jy = nullptr;
     /* shared planes; the loop's pairs swap roles between sweeps */
     R *lum = nullptr, *lumn = nullptr, *ecm = nullptr, *ecmn = nullptr, *psi_ = nullptr, *hl = nullptr, *he = nullptr;
-    R *ngl = nullptr, *jlx = nullptr, *jly = nullptr;
+    R *ngl = nullptr, *jlx = nullptr, *jly = nullptr, *jex = nullptr, *jey = nullptr;   /* lumen's, ECM's normals */
     /* FFT boxes: five transforms' inputs / spectra ([5][nc][M][M]), the velocity modes ([2][nc][M][M]) */
     Cx<R> *A = nullptr, *B = nullptr, *V = nullptr;
     Cx<double> *work = nullptr;
@@ -1934,8 +1937,7 @@ struct GenCtx final : Generic {
     int kc = 1, kl = 1;
     R *swin = nullptr;
     double *mb_part = nullptr;
-    unsigned *mb_ctr = nullptr;      /* arrival counters: [nc] cell groups, [nc] the lumen/ECM group */
-    int *mb_out = nullptr, *mb_box = nullptr;
+    unsigned *mb_ctr = nullptr;      /* arrival counters: [nc] cell groups, [nc], [nc + 1] the lumen, ECM groups */
     R *mb_xch = nullptr;             /* exchanged band rows: cells [nc][2][kc][2][W + 2]; lumen/ECM [2][kl][2][N] */
     /* the cell loop with its bands in LDS and tagged exchanges
      * (g_relax_cells_lds): workgroups per cell (0: not used), the largest
@@ -1947,11 +1949,12 @@ struct GenCtx final : Generic {
     /* and the lumen/ECM loop's (g_lum_loop_lds): the most workgroups (0: not
      * used) and its tagged words (g_band_words for klm bands of N columns) */
     int klm = 0;
-    unsigned long long *mb_lw = nullptr;
+    unsigned long long *mb_lw[2] = {nullptr, nullptr};   /* the lumen loop's, the ECM loop's */
+    unsigned tag_lum[2] = {1, 1};                        /* each one's next tag base */
     int *mb_cmax = nullptr;
     unsigned long long *cen_acc = nullptr;   /* the centroid sums, [nc][3] */
     double *sparts = nullptr;                /* FP64 areas' parts, [nc][ORG_SUMS_PARTS] */
-    unsigned tag_next = 1;
+    unsigned tag_next = 1;                   /* the cell loop's next tag base */
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     bool in_step = false, ngp_fresh = false;   /* step_scope: ngp formed in this step from the current phi */
@@ -1972,8 +1975,8 @@ struct GenCtx final : Generic {
         return ORG_OK;
     }
     /* a loop launch's relax flag, abort flag, largest count and count to zero (one fill kernel) */
-    int zero_loop_ctl() {
-        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)&ctl->relax, 4);
+    int zero_loop_ctl(int slot) {
+        GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)&ctl->loop[slot], 4);
         return ORG_OK;
     }
     int real_bytes() const override { return (int)sizeof(R); }
@@ -2002,7 +2005,7 @@ struct GenCtx final : Generic {
         }
         GTRY(make_plan(M, plan, allocs));
         for (R **f : {&phi, &phin, &rho, &rhon, &hf, &ngp, &jx, &jy}) GTRY(A_(f, PP));
-        for (R **f : {&lum, &lumn, &ecm, &ecmn, &psi_, &hl, &he, &ngl, &jlx, &jly}) GTRY(A_(f, NN));
+        for (R **f : {&lum, &lumn, &ecm, &ecmn, &psi_, &hl, &he, &ngl, &jlx, &jly, &jex, &jey}) GTRY(A_(f, NN));
         GTRY(A_(&A, 5 * nc * MM)); GTRY(A_(&B, 5 * nc * MM)); GTRY(A_(&V, 2 * nc * MM));
         GTRY(A_(&work, 5 * nc * MM));
         GTRY(A_(&vxb, nc * MM)); GTRY(A_(&vyb, nc * MM));
@@ -2036,13 +2039,12 @@ struct GenCtx final : Generic {
         GTRY(A_(&xc, nc)); GTRY(A_(&yc, nc)); GTRY(A_(&counts, nc));
         GTRY(A_(&ctl, 1));
         GHIP(hipHostMalloc((void **)&hctl, sizeof(GCtl)));
-        err = &ctl->err; relax = &ctl->relax; abrt = &ctl->abrt;
-        mb_out = &ctl->count; mb_box = ctl->box; mb_cmax = &ctl->cmax;
+        err = &ctl->err; relax = &ctl->loop[0].relax; abrt = &ctl->loop[0].abrt; mb_cmax = &ctl->loop[0].cmax;
         GTRY(A_(&cen_acc, (size_t)3 * nc));
         if (std::is_same<R, double>::value) GTRY(A_(&sparts, (size_t)ORG_SUMS_PARTS * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
-        GTRY(A_(&mb_ctr, (size_t)nc + 1));
+        GTRY(A_(&mb_ctr, (size_t)nc + 2));
 
         GTRY(A_(&mb_xch, std::max((size_t)nc * 2 * kc * 2 * (W + 2), (size_t)2 * kl * 2 * N)));
         {
@@ -2074,8 +2076,10 @@ struct GenCtx final : Generic {
             klm = std::min(std::min(ncu, 128), N - 2);
             if (klm >= 2) {
                 const size_t words = g_band_words(ORG_GEN_SPAN_LUM, klm, N, sizeof(R));
-                GTRY(A_(&mb_lw, words));
-                GHIP(hipMemset(mb_lw, 0, sizeof(unsigned long long) * words));
+                for (int f = 0; f < 2; f++) {
+                    GTRY(A_(&mb_lw[f], words));
+                    GHIP(hipMemset(mb_lw[f], 0, sizeof(unsigned long long) * words));
+                }
                 GHIP(hipFuncSetAttribute((const void *)g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, ORG_GEN_LDS_MAX));
             } else {
@@ -2089,7 +2093,8 @@ struct GenCtx final : Generic {
         GHIP(hipMemcpy(xc, cen.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
         GHIP(hipMemcpy(yc, cen.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
         for (R *f : {phi, phin, rho, rhon, hf, ngp, jx, jy}) GHIP(hipMemset(f, 0, sizeof(R) * PP));
-        for (R *f : {lum, lumn, ecm, ecmn, psi_, hl, he, ngl, jlx, jly}) GHIP(hipMemset(f, 0, sizeof(R) * NN));
+        for (R *f : {lum, lumn, ecm, ecmn, psi_, hl, he, ngl, jlx, jly, jex, jey})
+            GHIP(hipMemset(f, 0, sizeof(R) * NN));
         GHIP(hipMemset(vxb, 0, sizeof(R) * nc * MM)); GHIP(hipMemset(vyb, 0, sizeof(R) * nc * MM));
         GHIP(hipMemset(f0, 0, sizeof(R) * nc)); GHIP(hipMemset(s_, 0, sizeof(double) * nc));
         GHIP(hipMemset(ctl, 0, sizeof(GCtl))); GHIP(hipMemset(counts, 0, sizeof(int) * nc));
@@ -2340,87 +2345,140 @@ struct GenCtx final : Generic {
         if (kb < 2 || g_band_lds(ORG_GEN_SPAN_LUM, (bh + kb - 1) / kb, bw, sizeof(R)) > ORG_GEN_LDS_MAX) return 0;
         return kb;
     }
-    int reshape_lumecm(int which, int *failed) override {
-        R *&u = which ? ecm : lum;
-        R *&un = which ? ecmn : lumn;
-        const R side = which ? (R)1.0 : (R)0.0;
+    /* One lumen/ECM loop (reshaping_lumen_or_ECM, cell_phase_functions.cpp:
+     * 709-871) in three phases around two readbacks of the control block:
+     * lum_pre (normals, sweep 1 on the whole grid, its deviation and the box
+     * of nonzero normals), lum_mid (the host's stop and rimax decisions, the
+     * ring, the loop launch), lum_post (the loop's count, relax and abort
+     * flags; the buffers' roles).  Each field has its own normals, control
+     * slots, exchange words and tags, so the lumen's and the ECM's loops can
+     * share their readbacks (reshape_lumecm_pair): neither reads the other's
+     * field. */
+    struct LumJob {
+        int f = 0;                 /* 0 lumen, 1 ECM */
+        bool started = false;      /* the start deviation was above the threshold */
+        double rdev = 0.0, thr = 0.0;
+        int count = 0, ri = 0, failed = 0;
+        int launched = 0;          /* 0 no loop launch, 1 banded (LDS), 2 exchange-buffer */
+    };
+    R *&fld(int f) { return f ? ecm : lum; }
+    R *&fldn(int f) { return f ? ecmn : lumn; }
+    int lum_pre(LumJob &j) {
+        const int f = j.f;
+        R *u = fld(f), *un = fldn(f), *nx = f ? jex : jlx, *ny = f ? jey : jly;
+        const R side = f ? (R)1.0 : (R)0.0;
         const R rtol = (R)p.rtol, rdt = (R)p.rdt;
         GLAUNCH((g_lum_grad<R>), dim3(row_blocks(N - 2)), dim3(256), 0, P, (const R *)u, ngl);   /* :734-736 */
         GTRY(bc(ngl, 1));                                                                     /* :741 */
-        GLAUNCH((g_lum_normals<R>), dim3(row_blocks(N - 2)), dim3(256), 0, P, (const R *)un, (const R *)ngl, jlx,
-                jly);
-        GTRY(bc(jlx, 1));                                                                     /* :770-771 */
-        GTRY(bc(jly, 1));
-        double rdev = (double)(p.rdev_ini * rtol * rdt);                                      /* :720 */
-        const double thr = (double)(p.rdev_threshold * rtol * rdt);                           /* :774 */
-        *failed = 0;
-        int count = 0, ri = 0;
-        if (rdev > thr) {
+        GLAUNCH((g_lum_normals<R>), dim3(row_blocks(N - 2)), dim3(256), 0, P, (const R *)un, (const R *)ngl, nx, ny);
+        GTRY(bc(nx, 1));                                                                      /* :770-771 */
+        GTRY(bc(ny, 1));
+        j.rdev = (double)(p.rdev_ini * rtol * rdt);                                           /* :720 */
+        j.thr = (double)(p.rdev_threshold * rtol * rdt);                                      /* :774 */
+        j.count = 0; j.ri = 0; j.failed = 0; j.launched = 0;
+        j.started = j.rdev > j.thr;
+        if (j.started) {
             /* sweep 1 on the whole grid: lum <- lumn (the role change), lumn <- F(lum) into lum's buffer */
-            GLAUNCH((g_lum_sweep<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)un, u, (const R *)jlx,
-                    (const R *)jly, side);
-            GTRY(plane_sum(u, un, 1, &ctl->rdev));                                            /* :848-855 */
-            GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, mb_box, N);
-            GLAUNCH((g_nz_box<R>), dim3(std::max(1, N - 2)), dim3(256), 0, N, (const R *)jlx, (const R *)jly, mb_box);
-            GTRY(read_ctl());                                                                 /* rdev and the box */
-            rdev = hctl->rdev;
-            const int bx[4] = {hctl->box[0], hctl->box[1], hctl->box[2], hctl->box[3]};
-            count = 1;
-            if (ri > p.rimax) *failed = 1;                                                    /* :862-865 */
-            else ri++;
-            const int bh = bx[1] - bx[0], bw = bx[3] - bx[2];
-            if (!*failed && rdev > thr) {
-                /* the ring sweep 2 writes into lumn's buffer, now: it is then
-                 * the same constant ring in both buffers */
-                GLAUNCH((g_ring_dirichlet<R>), dim3((4 * N + 255) / 256), dim3(256), 0, N, un, side);
-                if (bh <= 0 || bw <= 0) {
-                    /* no nonzero normals: sweep 2 changes nothing (both buffers already agree) */
-                    count = 2;
-                    if (ri > p.rimax) *failed = 1;
-                } else if (int kb = lum_lds_blocks(bh, bw)) {
-                    /* sweeps 2, 3, ... with the bands in LDS */
-                    const int rows_max = (bh + kb - 1) / kb;
-                    GTRY(zero_loop_ctl());
-                    GLAUNCH((g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>), dim3(kb), dim3(1024),
-                            g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw, sizeof(R)), P, u, un, (const R *)jlx,
-                            (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_lw, tag_next, mb_out, relax,
-                            abrt, wait_limit, kb, rows_max);
-                    GTRY(read_ctl());
-                    const int h[3] = {hctl->count, hctl->relax, hctl->abrt};
-                    if (h[2]) {
-                        tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
-                        fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
-                                        "other for more than 2 s (not all resident?)\n");
-                        return ORG_ETIMEOUT;
-                    }
-                    tag_next += (unsigned)h[0] + 2u;
-                    count = h[0];
-                    *failed = h[1];
-                } else {
-                    /* sweeps 2, 3, ... on the device, bands of at least 8 box rows
-                     * and about 8192 pixels */
-                    const int k = std::max(1, std::min(std::min(kl, (bh + 7) / 8),
-                                                       (int)(((size_t)bh * bw + 8191) / 8192)));
-                    const size_t shm = 2 * sizeof(R) * (size_t)(bw + 2);
-                    GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr, nc + 1);
-                    GTRY(zero_loop_ctl());
-                    GLAUNCH((g_lum_loop_mb<R>), dim3(k), dim3(1024), shm, P, u, un, (const R *)jlx,
-                            (const R *)jly, (const int *)mb_box, thr, rdev, count, ri, mb_xch, mb_part,
-                            mb_ctr + nc, mb_out, relax, abrt, wait_limit, k);
-                    GTRY(read_ctl());
-                    const int h[3] = {hctl->count, hctl->relax, hctl->abrt};
-                    if (h[2]) {
-                        fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
-                                        "other for more than 2 s (not all resident?)\n");
-                        return ORG_ETIMEOUT;
-                    }
-                    count = h[0];
-                    *failed = h[1];
-                }
-            }
+            GLAUNCH((g_lum_sweep<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)un, u, (const R *)nx,
+                    (const R *)ny, side);
+            GTRY(plane_sum(u, un, 1, &ctl->rdev[f]));                                         /* :848-855 */
+            GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, ctl->box[f], N);
+            GLAUNCH((g_nz_box<R>), dim3(std::max(1, N - 2)), dim3(256), 0, N, (const R *)nx, (const R *)ny,
+                    ctl->box[f]);
         }
-        if (count & 1) std::swap(u, un);            /* an odd count leaves the last iterate in lum's buffer */
-        (which ? count_ecm : count_lum) = count;
+        return ORG_OK;
+    }
+    /* after a readback of the control block */
+    int lum_mid(LumJob &j) {
+        if (!j.started) return ORG_OK;
+        const int f = j.f;
+        R *u = fld(f), *un = fldn(f), *nx = f ? jex : jlx, *ny = f ? jey : jly;
+        const R side = f ? (R)1.0 : (R)0.0;
+        j.rdev = hctl->rdev[f];
+        const int *bx = hctl->box[f];
+        j.count = 1;
+        if (j.ri > p.rimax) j.failed = 1;                                                     /* :862-865 */
+        else j.ri++;
+        const int bh = bx[1] - bx[0], bw = bx[3] - bx[2];
+        if (j.failed || !(j.rdev > j.thr)) return ORG_OK;
+        /* the ring sweep 2 writes into lumn's buffer, now: it is then the
+         * same constant ring in both buffers */
+        GLAUNCH((g_ring_dirichlet<R>), dim3((4 * N + 255) / 256), dim3(256), 0, N, un, side);
+        if (bh <= 0 || bw <= 0) {
+            /* no nonzero normals: sweep 2 changes nothing (both buffers already agree) */
+            j.count = 2;
+            if (j.ri > p.rimax) j.failed = 1;
+            return ORG_OK;
+        }
+        GLoopCtl *lc = &ctl->loop[f];
+        GTRY(zero_loop_ctl(f));
+        if (int kb = lum_lds_blocks(bh, bw)) {
+            /* sweeps 2, 3, ... with the bands in LDS */
+            const int rows_max = (bh + kb - 1) / kb;
+            GLAUNCH((g_lum_loop_lds<ORG_GEN_SPAN_LUM, R>), dim3(kb), dim3(1024),
+                    g_band_lds(ORG_GEN_SPAN_LUM, rows_max, bw, sizeof(R)), P, u, un, (const R *)nx, (const R *)ny,
+                    (const int *)ctl->box[f], j.thr, j.rdev, j.count, j.ri, mb_lw[f], tag_lum[f], &lc->count,
+                    &lc->relax, &lc->abrt, wait_limit, kb, rows_max);
+            j.launched = 1;
+        } else {
+            /* sweeps 2, 3, ... on the device, bands of at least 8 box rows
+             * and about 8192 pixels */
+            const int k = std::max(1, std::min(std::min(kl, (bh + 7) / 8), (int)(((size_t)bh * bw + 8191) / 8192)));
+            const size_t shm = 2 * sizeof(R) * (size_t)(bw + 2);
+            GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, mb_ctr + nc + f, 1);
+            GLAUNCH((g_lum_loop_mb<R>), dim3(k), dim3(1024), shm, P, u, un, (const R *)nx, (const R *)ny,
+                    (const int *)ctl->box[f], j.thr, j.rdev, j.count, j.ri, mb_xch, mb_part, mb_ctr + nc + f,
+                    &lc->count, &lc->relax, &lc->abrt, wait_limit, k);
+            j.launched = 2;
+        }
+        return ORG_OK;
+    }
+    /* after a readback of the control block, if lum_mid launched a loop */
+    int lum_post(LumJob &j) {
+        const int f = j.f;
+        if (j.launched) {
+            const GLoopCtl &h = hctl->loop[f];
+            if (h.abrt) {
+                if (j.launched == 1) tag_lum[f] += 1u << 20;   /* past every tag the aborted launch may have used */
+                fprintf(stderr, "organoid (generic path): the lumen/ECM loop's workgroups waited on each "
+                                "other for more than 2 s (not all resident?)\n");
+                return ORG_ETIMEOUT;
+            }
+            if (j.launched == 1) tag_lum[f] += (unsigned)h.count + 2u;
+            j.count = h.count;
+            j.failed = h.relax;
+        }
+        if (j.count & 1) std::swap(fld(f), fldn(f));   /* an odd count leaves the last iterate in lum's buffer */
+        (f ? count_ecm : count_lum) = j.count;
+        return ORG_OK;
+    }
+    int reshape_lumecm(int which, int *failed) override {
+        LumJob j;
+        j.f = which ? 1 : 0;
+        GTRY(lum_pre(j));
+        if (j.started) {
+            GTRY(read_ctl());
+            GTRY(lum_mid(j));
+            if (j.launched) GTRY(read_ctl());
+        }
+        GTRY(lum_post(j));
+        *failed = j.failed;
+        return ORG_OK;
+    }
+    int reshape_lumecm_pair(int *failed) override {
+        LumJob j[2];
+        j[0].f = 0;
+        j[1].f = 1;
+        GTRY(lum_pre(j[0]));
+        GTRY(lum_pre(j[1]));
+        if (j[0].started || j[1].started) GTRY(read_ctl());
+        GTRY(lum_mid(j[0]));
+        GTRY(lum_mid(j[1]));
+        if (j[0].launched || j[1].launched) GTRY(read_ctl());
+        GTRY(lum_post(j[0]));
+        GTRY(lum_post(j[1]));
+        failed[0] = j[0].failed;
+        failed[1] = j[1].failed;
         return ORG_OK;
     }
     int update_cells() override {
@@ -2447,13 +2505,13 @@ struct GenCtx final : Generic {
         const R rtol = (R)p.rtol, Rr = (R)p.R, wi = (R)p.wi, rdt = (R)p.rdt;
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
-        GTRY(zero_loop_ctl());
+        GTRY(zero_loop_ctl(0));
         if (kcl) {
             GLAUNCH((g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>), dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin,
                     (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next,
                     counts, mb_cmax, relax, abrt, wait_limit, kcl, rmax_l);
             GTRY(read_ctl());
-            const int f[3] = {hctl->relax, hctl->abrt, hctl->cmax};
+            const int f[3] = {hctl->loop[0].relax, hctl->loop[0].abrt, hctl->loop[0].cmax};
             if (f[1]) {
                 tag_next += 1u << 20;      /* past every tag the aborted launch may have used */
                 fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
@@ -2474,7 +2532,7 @@ struct GenCtx final : Generic {
                     (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
         }
         GTRY(read_ctl());
-        const int f[2] = {hctl->relax, hctl->abrt};
+        const int f[2] = {hctl->loop[0].relax, hctl->loop[0].abrt};
         if (f[1]) {
             fprintf(stderr, "organoid (generic path): a cell's reshaping workgroups waited on each other "
                             "for more than 2 s (not all resident?)\n");

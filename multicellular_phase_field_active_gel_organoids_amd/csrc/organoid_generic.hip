@@ -646,6 +646,18 @@ __global__ void g_spec_commit(GP<R> P, const int *__restrict__ xc, const int *__
     }
 }
 
+/* copy_cell_tables_1_by_1's rho <- rhon (useful_functions.cpp:679-693) on
+ * the spectral boxes g_spec_commit writes (workgroup (i, n): box row i of
+ * cell n), for a step whose rhon equalled rho elsewhere */
+template <typename R>
+__global__ void g_copy_boxes(GP<R> P, const int *__restrict__ xc, const int *__restrict__ yc,
+                             const R *__restrict__ rhon, R *__restrict__ rho) {
+    const int N = P.N, M = P.M, zs = P.zs, n = blockIdx.y;
+    const size_t NN = (size_t)N * N;
+    const size_t row = (size_t)n * NN + GG(blockIdx.x + 1 + xc[n] - zs, 1 + yc[n] - zs);
+    for (int j = threadIdx.x; j < M; j += blockDim.x) rho[row + j] = rhon[row + j];
+}
+
 /* 9-point Laplacian numerator of cell_phase_functions.cpp (e.g. :198) */
 #define GL9(t, g) (-12.0 * (t)[(g)] + 2.0 * ((t)[(g) + N] + (t)[(g) - N] + (t)[(g) + 1] + (t)[(g) - 1]) \
                    + (t)[(g) + N + 1] + (t)[(g) + N - 1] + (t)[(g) - N - 1] + (t)[(g) - N + 1])
@@ -1958,6 +1970,10 @@ struct GenCtx final : Generic {
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     bool in_step = false, ngp_fresh = false;   /* step_scope: ngp formed in this step from the current phi */
+    /* rho equals rhon on every pixel (both planes' writers keep this up to
+     * date), and this step's spectral update started from that state: its
+     * copy-back then needs only the boxes it wrote */
+    bool rho_synced = true, boxes_only = false;
     int win_err = 0;          /* the last centroid's window check (the oracle's sticky check) */
 
     GCtl *ctl = nullptr, *hctl = nullptr;    /* on the device; its pinned host mirror */
@@ -2167,6 +2183,7 @@ struct GenCtx final : Generic {
         size_t n;
         R *d = field(f, &n);
         if (!d || !host) return ORG_EINVAL;
+        if (d == rho) rho_synced = false;
         return upload(d, host, n, f64);
     }
     int get_field(int f, void *host, int f64) override {
@@ -2231,12 +2248,14 @@ struct GenCtx final : Generic {
         GTRY(copy(lumn, lum, NN));
         GTRY(copy(phin, phi, PP));
         GTRY(copy(rhon, rho, PP));
+        rho_synced = true;
         GHIP(hipMemsetAsync(psi_, 0, sizeof(R) * NN, s));
         GHIP(hipStreamSynchronize(s));
         count_lum = count_ecm = 0;
         return ORG_OK;
     }
     int find_centres() override {
+        boxes_only = false;          /* the boxes move with the centres */
         GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)err, 1);
         GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
         GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P,
@@ -2303,6 +2322,12 @@ struct GenCtx final : Generic {
      * (rho_functions.cpp:409-471) for all cells at once */
     int cortices() override {
         GTRY(window_check());
+        /* rhon is written on the boxes (and its ring, a copy of interior
+         * pixels no box reaches): inside a step the copy-back that follows
+         * restores rho == rhon from the boxes alone; otherwise (and if the
+         * step ends before its copy-back) a whole-plane copy-back does */
+        boxes_only = in_step && rho_synced;
+        rho_synced = false;
         GTRY(norm_grad());
         const size_t pl = (size_t)nc * MM;
         const dim3 g(std::max(1, std::min(64, (int)((MM + 255) / 256))), nc), b(256);
@@ -2541,12 +2566,24 @@ struct GenCtx final : Generic {
         *failed = f[0];
         return ORG_OK;
     }
-    int commit_rho() override { return copy(rho, rhon, PP); }
+    int commit_rho() override {
+        if (boxes_only) {
+            GLAUNCH((g_copy_boxes<R>), dim3(M, nc), dim3(256), 0, P, (const int *)xc, (const int *)yc,
+                    (const R *)rhon, rho);
+        } else {
+            GTRY(copy(rho, rhon, PP));
+        }
+        boxes_only = false;
+        rho_synced = true;
+        return ORG_OK;
+    }
     int commit_lumecm() override {
         GTRY(copy(ecm, ecmn, NN));
         return copy(lum, lumn, NN);
     }
     int rho_to_shape(double rho0, double tol) override {
+        rho_synced = false;
+        boxes_only = false;
         GLAUNCH((g_rho_to_shape<R>), dim3(grid_for(PP)), dim3(256), 0, (const R *)phi, rho, PP, rho0, tol);
         GHIP(hipStreamSynchronize(s));
         return ORG_OK;

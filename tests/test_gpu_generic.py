@@ -168,3 +168,41 @@ def test_f64_fields_roundtrip_and_precision():
     with pytest.raises(host.OrganoidError):
         host.Batch([g])
     g.close()
+
+
+@pytest.mark.parametrize("name,precision", [("two_cells_256", "f64"), ("two_cells_512_zone200", "f32")])
+def test_steps_and_per_function_calls_interleave(name, precision):
+    """org_step's shortcuts on the full-plane path (|grad phi| once per step
+    and only on the windows, the copy-back of rho on the spectral boxes
+    while rho and rhon agree elsewhere, the lumen and ECM loops sharing
+    their readbacks) leave the state the per-function sequence leaves:
+    whole steps and per-function steps in any mix (with a rewrite of rho
+    through org_set_field and org_prepare at the same point in both runs)
+    give the same fields bit for bit."""
+    wl = make_workload(name)
+    if precision == "f32":
+        round32(wl)
+    print(f"case {name} {precision}: 5 whole steps against a mix", flush=True)
+    a = host.Organoid(wl.params, wl.s0_cell, precision=precision)
+    b = host.Organoid(wl.params, wl.s0_cell, precision=precision)
+    assert a.generic and b.generic
+    a.load_workload(wl)
+    b.load_workload(wl)
+    a.step(2)
+    a.set_field(host.F_RHO, a.get_field(host.F_RHO))
+    a.prepare()
+    a.step(3)
+    b.step(1)
+    b.plain_step()
+    b.set_field(host.F_RHO, b.get_field(host.F_RHO))
+    b.prepare()
+    b.step(1)
+    b.plain_step()
+    b.step(1)
+    for f in (host.F_PHI, host.F_RHO, host.F_LUM, host.F_ECM, host.F_HFUNC, host.F_PSI, host.F_LUMN, host.F_ECMN):
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    assert all((x == y).all() for x, y in zip(a.centres(), b.centres()))
+    ia, ib = a.iterations(), b.iterations()
+    assert (ia[0] == ib[0]).all() and ia[1:] == ib[1:]
+    a.close()
+    b.close()

@@ -423,9 +423,13 @@ __global__ __launch_bounds__(1024) void g_sum_s_parts(GP<double> P, const double
 }
 __global__ void g_sum_s_fin(GP<double> P, const double *__restrict__ part, int nb, const double *__restrict__ s0,
                             double *__restrict__ s, double *__restrict__ f0) {
-    for (int n = threadIdx.x; n < P.nc; n += blockDim.x) {
+    __shared__ double sp[ORG_SUMS_PARTS];      /* workgroup n: cell n's parts, loaded at once */
+    const int n = blockIdx.x;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) sp[b] = part[(size_t)n * nb + b];
+    __syncthreads();
+    if (threadIdx.x == 0) {
         double tot = 0.0;
-        for (int b = 0; b < nb; b++) tot += part[(size_t)n * nb + b];
+        for (int b = 0; b < nb; b++) tot += sp[b];
         s[n] = tot;
         f0[n] = (double)(P.alpha * ((double)s0[n] - (double)tot));
     }
@@ -677,10 +681,15 @@ __global__ __launch_bounds__(1024) void g_sum_parts(const R *__restrict__ a, con
     const double t = gblock_sum(acc, red);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
+/* the parts added in order (nparts <= 1024): all loaded into LDS at once,
+ * then one thread's sum there */
 __global__ void g_sum_final(const double *__restrict__ part, int nparts, double *__restrict__ out) {
+    __shared__ double sp[1024];
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) sp[k] = part[k];
+    __syncthreads();
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s = 0.0;
-        for (int k = 0; k < nparts; k++) s += part[k];
+        for (int k = 0; k < nparts; k++) s += sp[k];
         *out = s;
     }
 }
@@ -2288,7 +2297,7 @@ struct GenCtx final : Generic {
              * the float build's f0 rounds the same way */
             const size_t per = ((size_t)N * N + ORG_SUMS_PARTS - 1) / ORG_SUMS_PARTS;
             GLAUNCH(g_sum_s_parts, dim3(ORG_SUMS_PARTS, nc), dim3(1024), 0, P, (const double *)hf, per, sparts);
-            GLAUNCH(g_sum_s_fin, dim3(1), dim3(64), 0, P, (const double *)sparts, ORG_SUMS_PARTS, (const double *)s0,
+            GLAUNCH(g_sum_s_fin, dim3(nc), dim3(64), 0, P, (const double *)sparts, ORG_SUMS_PARTS, (const double *)s0,
                     s_, f0);
         } else {
             GLAUNCH((g_sum_s<R>), dim3(nc), dim3(1024), 0, P, (const R *)hf, (const double *)s0, s_, f0);

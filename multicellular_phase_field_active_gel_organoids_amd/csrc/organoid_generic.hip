@@ -2417,8 +2417,9 @@ struct GenCtx final : Generic {
                     (const R *)ny, side);
             GTRY(plane_sum(u, un, 1, &ctl->rdev[f]));                                         /* :848-855 */
             GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, ctl->box[f], N);
-            GLAUNCH((g_nz_box<R>), dim3(std::max(1, N - 2)), dim3(256), 0, N, (const R *)nx, (const R *)ny,
-                    ctl->box[f]);
+            /* at most 240 workgroups: each ends with four atomics on the same four words */
+            GLAUNCH((g_nz_box<R>), dim3(std::max(1, std::min(N - 2, 240))), dim3(256), 0, N, (const R *)nx,
+                    (const R *)ny, ctl->box[f]);
         }
         return ORG_OK;
     }

@@ -69,6 +69,7 @@ struct org_ctx {
     float2 *spec = nullptr;
     /* per cell / scalars */
     int *xc = nullptr, *yc = nullptr, *box = nullptr, *counts = nullptr, *err = nullptr;
+    int *herr = nullptr;      /* pinned host word the error bitmask is read into (a pageable one costs ~25 us) */
     int *uni = nullptr, *jbox = nullptr;   /* union of windows; per-workgroup boxes of nonzero normals */
     int *xn = nullptr, *yn = nullptr, *bn = nullptr;  /* next batched step's centres and support boxes */
     int *xp = nullptr, *yp = nullptr;      /* copy of a step's centres for k_mid */
@@ -223,9 +224,9 @@ void note_relax_failure(org_ctx *c) {
 /* Read the device error bitmask, clear it for the next call, and map the
  * lowest set code other than ORG_ERELAX (a warning, see above). */
 int harvest_device_error(org_ctx *c) {
-    int bits = 0;
-    HIPCHK(hipMemcpyAsync(&bits, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->herr, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    int bits = *c->herr;
     if (!bits) return ORG_OK;
     HIPCHK(hipMemsetAsync(c->err, 0, sizeof(int), c->stream));
     if (bits & (1 << ORG_ERELAX)) {
@@ -618,6 +619,7 @@ int org_create(const org_params *p, const double *s0_cell, org_ctx **out) {
     A(&c->live, 8); A(&c->live_part, 8 * (size_t)c->nlp_cap); A(&c->counts, c->nc); A(&c->err, 1);
     A(&c->s, c->nc); A(&c->s0, c->nc); A(&c->sum_ecm, 1); A(&c->part_h, (size_t)c->nparts_h); A(&c->part_l, 2 * 2048);
     A(&c->f0, c->nc); A(&c->counters, 4); A(&c->lst, 2);
+    if (!rc && hipHostMalloc((void **)&c->herr, sizeof(int)) != hipSuccess) rc = note_error(ORG_EHIP);
     if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = note_error(ORG_EHIP);
     if (!rc) {
         /* timeout of the waits inside k_step: 2 s of the device's wall clock */
@@ -666,6 +668,7 @@ int org_destroy(org_ctx *c) {
     for (int k = 0; k < ORG_K_COUNT; k++)
         for (auto &pr : c->ev[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (void *q : c->allocs) (void)hipFree(q);
+    if (c->herr) (void)hipHostFree(c->herr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ORG_OK;

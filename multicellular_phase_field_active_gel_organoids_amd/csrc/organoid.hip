@@ -1040,8 +1040,11 @@ int org_step(org_ctx *c, int it0, int nsteps, int use_graph) {
     if (c->gen) {                          /* host-driven: the reshaping loops' stop tests read back */
         TRY(need_prepared(c));
         if (nsteps < 0) return note_error(ORG_EINVAL);
-        for (int s = 0; s < nsteps; s++) TRY(gen_step(c));
-        return ORG_OK;
+        int rc = ORG_OK;
+        c->gen->step_chain(true);          /* centroid sums carried from step to step */
+        for (int s = 0; s < nsteps && !rc; s++) rc = gen_step(c);
+        c->gen->step_chain(false);
+        return rc;
     }
     const bool ready = c->next_ready;
     TRY(need_prepared(c));

@@ -55,6 +55,13 @@ struct Generic {
      * from the step's start, off before its copy-back.  The per-function
      * entry points leave it off (each call forms its own, as the reference). */
     virtual void step_scope(bool on) = 0;
+    /* Around the steps of one org_step call: from one step's centroid to the
+     * next, phi changes only where the cell reshaping writes it (each cell's
+     * window), so the centroid's integer sums are carried over (the windows'
+     * pixels taken out before that write and put back after it) instead of
+     * summed again over the whole planes.  Off, every find_centres sums the
+     * whole planes. */
+    virtual void step_chain(bool on) = 0;
 };
 
 /* A context on `stream` (borrowed: the caller owns and destroys it). */

@@ -277,6 +277,34 @@ __global__ __launch_bounds__(256) void g_centroid_part(GP<R> P, const R *__restr
     }
 }
 
+/* g_centroid_part's sums restricted to each cell's window (rows and
+ * columns [c - zs, c + zs) around the current centre), added to acc (sign
+ * > 0) or taken out of it (sign < 0, two's complement: the 64-bit sums wrap
+ * and stay exact).  Block (r, n) does row r of cell n's window. */
+template <typename R>
+__global__ __launch_bounds__(256) void g_centroid_win(GP<R> P, const R *__restrict__ phi, const int *__restrict__ xc,
+                                                      const int *__restrict__ yc, unsigned long long *__restrict__ acc,
+                                                      int sign) {
+    __shared__ long long red[3 * 4];
+    const int N = P.N, n = blockIdx.y, zs = P.zs;
+    const int i = xc[n] - zs + blockIdx.x;
+    const R *ph = phi + (size_t)n * N * N + (size_t)i * N;
+    long long xs = 0, ys = 0, cnt = 0;
+    for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x)
+        if (ph[j] > 0.5) { xs += i; ys += j; cnt += 1; }
+    for (int o = 32; o > 0; o >>= 1) {
+        xs += __shfl_down(xs, o, 64); ys += __shfl_down(ys, o, 64); cnt += __shfl_down(cnt, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w] = xs; red[4 + w] = ys; red[8 + w] = cnt; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        long long v = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) v += red[4 * threadIdx.x + k];
+        if (v) atomicAdd(&acc[3 * n + threadIdx.x], (unsigned long long)(sign > 0 ? v : -v));
+    }
+}
+
 /* the centres from g_centroid_part's sums (integer division as the
  * reference's), then the window check (window plus one pixel inside rows /
  * columns 1..N-2) */
@@ -1979,6 +2007,9 @@ struct GenCtx final : Generic {
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
     bool in_step = false, ngp_fresh = false;   /* step_scope: ngp formed in this step from the current phi */
+    /* step_chain: cen_acc holds the centroid sums of the current phi (set by
+     * find_centres inside a chain, kept by reshape_cells' window deltas) */
+    bool chain = false, cen_live = false;
     /* rho equals rhon on every pixel (both planes' writers keep this up to
      * date), and this step's spectral update started from that state: its
      * copy-back then needs only the boxes it wrote */
@@ -2266,9 +2297,12 @@ struct GenCtx final : Generic {
     int find_centres() override {
         boxes_only = false;          /* the boxes move with the centres */
         GLAUNCH(g_zero_u32, dim3(1), dim3(64), 0, (unsigned *)err, 1);
-        GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
-        GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P,
-                (const R *)phi, cen_acc);
+        if (!(chain && cen_live)) {
+            GLAUNCH(g_zero_u32, dim3(1), dim3(256), 0, (unsigned *)cen_acc, 6 * nc);
+            GLAUNCH((g_centroid_part<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P,
+                    (const R *)phi, cen_acc);
+        }
+        cen_live = chain;
         GLAUNCH((g_centroid_fin<R>), dim3(1), dim3(64), 0, P, (const unsigned long long *)cen_acc, xc, yc, err);
         int e = 0;
         GTRY(read_ctl());
@@ -2276,6 +2310,11 @@ struct GenCtx final : Generic {
         if (e == ORG_EEMPTY_CELL) return e;
         win_err = e;
         return e;
+    }
+    int centroid_win(int sign) {
+        GLAUNCH((g_centroid_win<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, (const int *)xc, (const int *)yc,
+                cen_acc, sign);
+        return ORG_OK;
     }
     int hfunc_cells() override {
         GTRY(window_check());
@@ -2325,6 +2364,10 @@ struct GenCtx final : Generic {
     void step_scope(bool on) override {
         in_step = on;
         ngp_fresh = false;
+    }
+    void step_chain(bool on) override {
+        chain = on;
+        cen_live = false;
     }
     /* run_with_active_gels.cpp:130-154: turnover maps, prod_norm_delta and
      * the whole per-cell spectral loop of update_all_cortices
@@ -2541,10 +2584,14 @@ struct GenCtx final : Generic {
         const double thr = (double)(rtol * 2.0 * M_PI * Rr * wi * rdt);                       /* :475 */
         const double rdev0 = (double)(2.0 * rtol * 2.0 * M_PI * Rr * wi * rdt);               /* :473 */
         GTRY(zero_loop_ctl(0));
+        /* the loops write phi on the windows alone: carried centroid sums
+         * lose the windows' old pixels here and get their new ones after */
+        if (cen_live) GTRY(centroid_win(-1));
         if (kcl) {
             GLAUNCH((g_relax_cells_lds<ORG_GEN_SPAN_CELLS, R>), dim3(nc * kcl), dim3(1024), lds_l, P, phi, phin,
                     (const R *)jx, (const R *)jy, (const int *)xc, (const int *)yc, thr, rdev0, mb_tw, tag_next,
                     counts, mb_cmax, relax, abrt, wait_limit, kcl, rmax_l);
+            if (cen_live) GTRY(centroid_win(1));
             GTRY(read_ctl());
             const int f[3] = {hctl->loop[0].relax, hctl->loop[0].abrt, hctl->loop[0].cmax};
             if (f[1]) {
@@ -2566,6 +2613,7 @@ struct GenCtx final : Generic {
             GLAUNCH((g_relax_cells<R>), dim3(nc), dim3(1024), 0, P, phi, phin, (const R *)jx, (const R *)jy,
                     (const int *)xc, (const int *)yc, thr, rdev0, counts, relax);
         }
+        if (cen_live) GTRY(centroid_win(1));
         GTRY(read_ctl());
         const int f[2] = {hctl->loop[0].relax, hctl->loop[0].abrt};
         if (f[1]) {

@@ -326,10 +326,19 @@ __global__ void g_centroid_fin(GP<R> P, const unsigned long long *__restrict__ a
  * block (r, n) does row r of cell n's window. */
 template <typename R>
 __global__ void g_hfunc_cells(GP<R> P, const R *__restrict__ phi, R *__restrict__ hf, const int *__restrict__ xc,
-                              const int *__restrict__ yc) {
+                              const int *__restrict__ yc, int *__restrict__ hbox) {
     const int N = P.N, n = blockIdx.y, zs = P.zs;
     const size_t NN = (size_t)N * N;
     const int i = xc[n] - zs + blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {             /* the box grows by the window written */
+        int *b = hbox + 4 * n;
+        const int r0 = xc[n] - zs, c0 = yc[n] - zs, w = 2 * zs;
+        const bool empty = b[0] >= b[1] || b[2] >= b[3];
+        b[0] = empty ? r0 : min(b[0], r0);
+        b[1] = empty ? r0 + w : max(b[1], r0 + w);
+        b[2] = empty ? c0 : min(b[2], c0);
+        b[3] = empty ? c0 + w : max(b[3], c0 + w);
+    }
     for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x) {
         const R v = phi[GPL(n, i, j)];
         hf[GPL(n, i, j)] = v * v * (3.0 - 2.0 * v);
@@ -386,15 +395,60 @@ __global__ void g_norm_grad(GP<R> P, const R *__restrict__ phi, R *__restrict__ 
         }
 }
 
-/* compute_psi_kernels, useful_functions.cpp:305-322 (cells summed in order) */
-template <typename R>
-__global__ void g_psi(GP<R> P, const R *__restrict__ hf, R *__restrict__ psi) {
-    const size_t NN = (size_t)P.N * P.N;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < NN; k += (size_t)gridDim.x * blockDim.x) {
-        R acc = 0.0;
-        for (int n = 0; n < P.nc; n++) acc += hf[(size_t)n * NN + k];
-        psi[k] = acc;
+/* Each cell's hfunc plane is zero outside hbox[n] = {r0, r1, c0, c1}
+ * (rows [r0, r1) x columns [c0, c1); empty when r0 >= r1): g_hbox_scan
+ * sets the box from the plane after an upload, g_hfunc_cells widens it by
+ * each window it writes.  Adding zero leaves a sum's bits unchanged, so
+ * sums over the boxes alone equal the whole-plane sums bit for bit. */
+__global__ void g_hbox_reset(int *__restrict__ hbox, int nc, int N) {
+    for (int n = threadIdx.x; n < nc; n += blockDim.x) {
+        hbox[4 * n] = N; hbox[4 * n + 1] = 0; hbox[4 * n + 2] = N; hbox[4 * n + 3] = 0;
     }
+}
+template <typename R>
+__global__ __launch_bounds__(256) void g_hbox_scan(GP<R> P, const R *__restrict__ hf, int *__restrict__ hbox) {
+    __shared__ int bb[4];
+    const int N = P.N, n = blockIdx.y;
+    const R *h = hf + (size_t)n * N * N;
+    int i0 = N, i1 = -1, j0 = N, j1 = -1;
+    for (int i = blockIdx.x; i < N; i += gridDim.x)
+        for (int j = threadIdx.x; j < N; j += blockDim.x)
+            if (h[(size_t)i * N + j] != (R)0.0) { i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j); }
+    if (threadIdx.x == 0) { bb[0] = N; bb[1] = -1; bb[2] = N; bb[3] = -1; }
+    for (int o = 32; o > 0; o >>= 1) {
+        i0 = min(i0, __shfl_xor(i0, o, 64)); i1 = max(i1, __shfl_xor(i1, o, 64));
+        j0 = min(j0, __shfl_xor(j0, o, 64)); j1 = max(j1, __shfl_xor(j1, o, 64));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && i1 >= 0) {
+        atomicMin(&bb[0], i0); atomicMax(&bb[1], i1); atomicMin(&bb[2], j0); atomicMax(&bb[3], j1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bb[1] >= 0) {
+        int *b = hbox + 4 * n;
+        atomicMin(&b[0], bb[0]); atomicMax(&b[1], bb[1] + 1); atomicMin(&b[2], bb[2]); atomicMax(&b[3], bb[3] + 1);
+    }
+}
+
+/* compute_psi_kernels, useful_functions.cpp:305-322: psi = the cells'
+ * hfunc added in cell order, each cell's plane read inside its box only
+ * (the terms left out are zeros: the same bits as the whole-plane sum).
+ * Workgroup x on rows x, x + gridDim.x, ... */
+template <typename R>
+__global__ __launch_bounds__(256) void g_psi_box(GP<R> P, const R *__restrict__ hf, const int4 *__restrict__ hbox,
+                                                 R *__restrict__ psi) {
+    const int N = P.N, nc = P.nc;
+    const size_t NN = (size_t)N * N;
+    for (int i = blockIdx.x; i < N; i += gridDim.x)
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const size_t q = (size_t)i * N + j;
+            R acc = 0.0;
+            for (int n = 0; n < nc; n++) {
+                const int4 b = hbox[n];
+                if (i >= b.x && i < b.y && j >= b.z && j < b.w) acc += hf[(size_t)n * NN + q];
+            }
+            psi[q] = acc;
+        }
 }
 
 /* sum_s_kernels, useful_functions.cpp:168-190, one block per cell: the area
@@ -428,36 +482,46 @@ __global__ __launch_bounds__(1024) void g_sum_s(GP<R> P, const R *__restrict__ h
     }
 }
 
-#define ORG_SUMS_PARTS 64
-
-/* sum_s_kernels for FP64 state on many workgroups: workgroup (b, n) adds
- * cell n's pixels [b per, (b + 1) per) (each thread its strided share,
- * then the block sum) into part[n][b]; g_sum_s_fin adds the B parts in
- * order.  A fixed order, so the same value at every call; it differs from
- * g_sum_s's order only in the last bits of the area (which the FP64 state
- * carries on in double). */
-__global__ __launch_bounds__(1024) void g_sum_s_parts(GP<double> P, const double *__restrict__ hf, size_t per,
-                                                      double *__restrict__ part) {
-    __shared__ double red[16];
-    const size_t NN = (size_t)P.N * P.N;
-    const int n = blockIdx.y, b = blockIdx.x;
+/* sum_s_kernels for FP64 state in an order that does not depend on the
+ * hfunc boxes: each row's sum (lane l of a wave adds the row's columns j =
+ * l mod 64 in increasing j, then the wave's fixed tree), then the rows'
+ * sums in increasing row.  Every pixel outside the box is zero, so the
+ * rows and columns outside it are left out and the bits stay the same.
+ * g_area_rows: workgroup (b, n), wave w, on box rows r0 + 4 b + w,
+ * stepping 4 gridDim.x; rows[n][i] for the box's rows only. */
+__global__ __launch_bounds__(256) void g_area_rows(GP<double> P, const double *__restrict__ hf,
+                                                   const int4 *__restrict__ hbox, double *__restrict__ rows) {
+    const int N = P.N, n = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int4 b = hbox[n];
     const double dx = P.dx;
-    const double *h = hf + (size_t)n * NN;
-    const size_t k0 = (size_t)b * per, k1 = min(NN, k0 + per);
-    double acc = 0.0;
-    for (size_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += (double)(h[k] * dx * dx);
-    const double tot = gblock_sum(acc, red);
-    if (threadIdx.x == 0) part[(size_t)n * gridDim.x + b] = tot;
+    const double *h = hf + (size_t)n * N * N;
+    const int c_first = b.z + ((lane - b.z) % 64 + 64) % 64;      /* the lane's first column >= c0 */
+    for (int i = b.x + 4 * blockIdx.x + w; i < b.y; i += 4 * gridDim.x) {
+        const double *hr = h + (size_t)i * N;
+        double acc = 0.0;
+        for (int j = c_first; j < b.w; j += 64) acc += (double)(hr[j] * dx * dx);
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if (lane == 0) rows[(size_t)n * N + i] = acc;
+    }
 }
-__global__ void g_sum_s_fin(GP<double> P, const double *__restrict__ part, int nb, const double *__restrict__ s0,
-                            double *__restrict__ s, double *__restrict__ f0) {
-    __shared__ double sp[ORG_SUMS_PARTS];      /* workgroup n: cell n's parts, loaded at once */
-    const int n = blockIdx.x;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) sp[b] = part[(size_t)n * nb + b];
-    __syncthreads();
+/* workgroup n: cell n's row sums in increasing row (staged in LDS), then
+ * f0 = alpha (s0 - s) */
+__global__ __launch_bounds__(256) void g_area_fin(GP<double> P, const double *__restrict__ rows,
+                                                  const int4 *__restrict__ hbox, const double *__restrict__ s0,
+                                                  double *__restrict__ s, double *__restrict__ f0) {
+    __shared__ double sp[1024];
+    const int N = P.N, n = blockIdx.x;
+    const int4 b = hbox[n];
+    double tot = 0.0;
+    for (int c = b.x; c < b.y; c += 1024) {
+        const int m = min(1024, b.y - c);
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += blockDim.x) sp[k] = rows[(size_t)n * N + c + k];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < m; k++) tot += sp[k];
+    }
     if (threadIdx.x == 0) {
-        double tot = 0.0;
-        for (int b = 0; b < nb; b++) tot += sp[b];
         s[n] = tot;
         f0[n] = (double)(P.alpha * ((double)s0[n] - (double)tot));
     }
@@ -2002,7 +2066,9 @@ struct GenCtx final : Generic {
     unsigned tag_lum[2] = {1, 1};                        /* each one's next tag base */
     int *mb_cmax = nullptr;
     unsigned long long *cen_acc = nullptr;   /* the centroid sums, [nc][3] */
-    double *sparts = nullptr;                /* FP64 areas' parts, [nc][ORG_SUMS_PARTS] */
+    int *hbox = nullptr;                     /* [nc][4]: where each cell's hfunc may be nonzero */
+    bool hbox_stale = true;                  /* hfunc uploaded since the boxes were last scanned */
+    double *srows = nullptr;                 /* FP64 areas' row sums, [nc][N] */
     unsigned tag_next = 1;                   /* the cell loop's next tag base */
     long long wait_limit = 0;
     int count_lum = 0, count_ecm = 0;
@@ -2097,7 +2163,8 @@ struct GenCtx final : Generic {
         GHIP(hipHostMalloc((void **)&hctl, sizeof(GCtl)));
         err = &ctl->err; relax = &ctl->loop[0].relax; abrt = &ctl->loop[0].abrt; mb_cmax = &ctl->loop[0].cmax;
         GTRY(A_(&cen_acc, (size_t)3 * nc));
-        if (std::is_same<R, double>::value) GTRY(A_(&sparts, (size_t)ORG_SUMS_PARTS * nc));
+        GTRY(A_(&hbox, (size_t)4 * nc));
+        if (std::is_same<R, double>::value) GTRY(A_(&srows, (size_t)N * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
         GTRY(A_(&mb_ctr, (size_t)nc + 2));
@@ -2224,6 +2291,7 @@ struct GenCtx final : Generic {
         R *d = field(f, &n);
         if (!d || !host) return ORG_EINVAL;
         if (d == rho) rho_synced = false;
+        if (d == hf) hbox_stale = true;
         return upload(d, host, n, f64);
     }
     int get_field(int f, void *host, int f64) override {
@@ -2318,26 +2386,36 @@ struct GenCtx final : Generic {
     }
     int hfunc_cells() override {
         GTRY(window_check());
-        GLAUNCH((g_hfunc_cells<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, hf, (const int *)xc, (const int *)yc);
+        GLAUNCH((g_hfunc_cells<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, hf, (const int *)xc, (const int *)yc,
+                hbox);
         return ORG_OK;
     }
     int hfunc_lum_ecm() override {
         GLAUNCH((g_hfunc_lumecm<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)lum, (const R *)ecm, hl, he);
         return ORG_OK;
     }
+    /* the hfunc boxes from the planes, after an upload of hfunc */
+    int hbox_fresh() {
+        if (!hbox_stale) return ORG_OK;
+        GLAUNCH(g_hbox_reset, dim3(1), dim3(256), 0, hbox, nc, N);
+        GLAUNCH((g_hbox_scan<R>), dim3(std::min(N, std::max(1, 4096 / nc)), nc), dim3(256), 0, P, (const R *)hf, hbox);
+        hbox_stale = false;
+        return ORG_OK;
+    }
     int psi() override {
-        GLAUNCH((g_psi<R>), dim3(grid_for(NN)), dim3(256), 0, P, (const R *)hf, psi_);
+        GTRY(hbox_fresh());
+        GLAUNCH((g_psi_box<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)hf, (const int4 *)hbox, psi_);
         return ORG_OK;
     }
     int sum_s() override {
         if constexpr (std::is_same<R, double>::value) {
-            /* FP64 state: 64 workgroups per cell (one per cell is bound by a
-             * CU's loads in flight); FP32 keeps g_sum_s's order, whose sum
-             * the float build's f0 rounds the same way */
-            const size_t per = ((size_t)N * N + ORG_SUMS_PARTS - 1) / ORG_SUMS_PARTS;
-            GLAUNCH(g_sum_s_parts, dim3(ORG_SUMS_PARTS, nc), dim3(1024), 0, P, (const double *)hf, per, sparts);
-            GLAUNCH(g_sum_s_fin, dim3(nc), dim3(64), 0, P, (const double *)sparts, ORG_SUMS_PARTS, (const double *)s0,
-                    s_, f0);
+            /* FP64 state: row sums over each cell's hfunc box, on 16
+             * workgroups per cell; FP32 keeps g_sum_s's order, whose sum the
+             * float build's f0 rounds the same way */
+            GTRY(hbox_fresh());
+            GLAUNCH(g_area_rows, dim3(16, nc), dim3(256), 0, P, (const double *)hf, (const int4 *)hbox, srows);
+            GLAUNCH(g_area_fin, dim3(nc), dim3(256), 0, P, (const double *)srows, (const int4 *)hbox,
+                    (const double *)s0, s_, f0);
         } else {
             GLAUNCH((g_sum_s<R>), dim3(nc), dim3(1024), 0, P, (const R *)hf, (const double *)s0, s_, f0);
         }

@@ -15,6 +15,9 @@ gives M = 398, a Bluestein length).
   keep its literal types and rounding order) and within the float build's
   own separation from FP64, and the per-function step at the FP32
   tolerances.
+* psi and the areas, which read each cell's hfunc only inside the box
+  where it can be nonzero, against the whole-plane sums after an upload
+  and after a window write.
 """
 import numpy as np
 import pytest
@@ -206,3 +209,43 @@ def test_steps_and_per_function_calls_interleave(name, precision):
     assert (ia[0] == ib[0]).all() and ia[1:] == ib[1:]
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_psi_and_areas_over_uploaded_and_window_hfunc(precision):
+    """psi and the areas read each cell's hfunc only where it can be
+    nonzero (the bounding box of an uploaded plane's nonzero pixels, grown by
+    every window compute_hfunc writes).  An upload with nonzero pixels
+    scattered over the whole plane, ring included, then a window write on top:
+    psi must equal the cells' planes added in cell order bit for bit, and the
+    areas the whole-plane sums."""
+    wl, g = _trajectory("two_cells_512_zone200", precision, 0)
+    nc, N = wl.params.ncell, wl.params.N
+    dt = np.float64 if precision == "f64" else np.float32
+    rng = np.random.default_rng(77)
+    h = np.zeros((nc, N, N), dt)
+    for n in range(nc):
+        idx = rng.integers(0, N, size=(40, 2))
+        idx[:4] = [[0, 0], [N - 1, N - 1], [0, N - 1], [N - 1, 3]]
+        h[n, idx[:, 0], idx[:, 1]] = rng.uniform(-0.5, 1.0, 40).astype(dt)
+    g.set_field(host.F_HFUNC, h)
+    g.prepare()
+    dx = dt(wl.params.dx)
+
+    def check(tag):
+        hf = g.get_field(host.F_HFUNC)
+        want = np.zeros((N, N), dt)
+        for n in range(nc):
+            want = want + hf[n]
+        g.compute_psi_kernels()
+        assert np.array_equal(g.get_field(host.F_PSI), want), tag
+        s = g.sum_s_kernels()
+        ref = [float(np.sum((hf[n] * dx * dx).astype(np.float64))) for n in range(nc)]
+        assert np.allclose(s, ref, rtol=1e-12, atol=0), (tag, s, ref)
+        print(f"case {precision} {tag}: psi equal, areas {s}", flush=True)
+
+    check("uploaded")
+    g.find_centres_cells()
+    g.compute_hfunc_3d_tables_kernels_loop_n()
+    check("uploaded + window")
+    g.close()

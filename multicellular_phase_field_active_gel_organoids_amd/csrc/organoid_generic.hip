@@ -280,18 +280,19 @@ __global__ __launch_bounds__(256) void g_centroid_part(GP<R> P, const R *__restr
 /* g_centroid_part's sums restricted to each cell's window (rows and
  * columns [c - zs, c + zs) around the current centre), added to acc (sign
  * > 0) or taken out of it (sign < 0, two's complement: the 64-bit sums wrap
- * and stay exact).  Block (r, n) does row r of cell n's window. */
+ * and stay exact).  Block (b, n) does rows b, b + gridDim.x, ... of cell
+ * n's window. */
 template <typename R>
 __global__ __launch_bounds__(256) void g_centroid_win(GP<R> P, const R *__restrict__ phi, const int *__restrict__ xc,
                                                       const int *__restrict__ yc, unsigned long long *__restrict__ acc,
                                                       int sign) {
     __shared__ long long red[3 * 4];
     const int N = P.N, n = blockIdx.y, zs = P.zs;
-    const int i = xc[n] - zs + blockIdx.x;
-    const R *ph = phi + (size_t)n * N * N + (size_t)i * N;
+    const R *ph = phi + (size_t)n * N * N;
     long long xs = 0, ys = 0, cnt = 0;
-    for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x)
-        if (ph[j] > 0.5) { xs += i; ys += j; cnt += 1; }
+    for (int i = xc[n] - zs + blockIdx.x; i < xc[n] + zs; i += gridDim.x)
+        for (int j = yc[n] - zs + threadIdx.x; j < yc[n] + zs; j += blockDim.x)
+            if (ph[(size_t)i * N + j] > 0.5) { xs += i; ys += j; cnt += 1; }
     for (int o = 32; o > 0; o >>= 1) {
         xs += __shfl_down(xs, o, 64); ys += __shfl_down(ys, o, 64); cnt += __shfl_down(cnt, o, 64);
     }
@@ -433,22 +434,44 @@ __global__ __launch_bounds__(256) void g_hbox_scan(GP<R> P, const R *__restrict_
 /* compute_psi_kernels, useful_functions.cpp:305-322: psi = the cells'
  * hfunc added in cell order, each cell's plane read inside its box only
  * (the terms left out are zeros: the same bits as the whole-plane sum).
- * Workgroup x on rows x, x + gridDim.x, ... */
+ * Workgroup (x, y) on columns [256 x, 256 x + 256) of rows y, y +
+ * gridDim.y, ...: for each row it first lists in LDS, in cell order, the
+ * cells whose box holds the row (nc <= 1024). */
 template <typename R>
 __global__ __launch_bounds__(256) void g_psi_box(GP<R> P, const R *__restrict__ hf, const int4 *__restrict__ hbox,
                                                  R *__restrict__ psi) {
-    const int N = P.N, nc = P.nc;
+    __shared__ int4 lst[1024];
+    __shared__ int wcnt[4];
+    const int N = P.N, nc = P.nc, j = blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t NN = (size_t)N * N;
-    for (int i = blockIdx.x; i < N; i += gridDim.x)
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    for (int i = blockIdx.y; i < N; i += gridDim.y) {
+        int total = 0;
+        __syncthreads();                       /* the previous row's list has been read */
+        for (int c0 = 0; c0 < nc; c0 += 256) {
+            const int n = c0 + threadIdx.x;
+            int4 b = make_int4(0, 0, 0, 0);
+            bool in = false;
+            if (n < nc) { b = hbox[n]; in = i >= b.x && i < b.y; }
+            const unsigned long long m = __ballot(in);
+            if (lane == 0) wcnt[w] = __popcll(m);
+            __syncthreads();
+            int off = total;
+            for (int k = 0; k < w; k++) off += wcnt[k];
+            if (in) lst[off + __popcll(m & ((1ull << lane) - 1ull))] = make_int4(n, b.z, b.w, 0);
+            total += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            __syncthreads();
+        }
+        if (j < N) {
             const size_t q = (size_t)i * N + j;
             R acc = 0.0;
-            for (int n = 0; n < nc; n++) {
-                const int4 b = hbox[n];
-                if (i >= b.x && i < b.y && j >= b.z && j < b.w) acc += hf[(size_t)n * NN + q];
+            for (int k = 0; k < total; k++) {
+                const int4 e = lst[k];
+                if (j >= e.y && j < e.z) acc += hf[(size_t)e.x * NN + q];
             }
             psi[q] = acc;
         }
+    }
 }
 
 /* sum_s_kernels, useful_functions.cpp:168-190, one block per cell: the area
@@ -2380,8 +2403,8 @@ struct GenCtx final : Generic {
         return e;
     }
     int centroid_win(int sign) {
-        GLAUNCH((g_centroid_win<R>), dim3(W, nc), dim3(256), 0, P, (const R *)phi, (const int *)xc, (const int *)yc,
-                cen_acc, sign);
+        GLAUNCH((g_centroid_win<R>), dim3(std::min(W, 16), nc), dim3(256), 0, P, (const R *)phi, (const int *)xc,
+                (const int *)yc, cen_acc, sign);
         return ORG_OK;
     }
     int hfunc_cells() override {
@@ -2404,7 +2427,8 @@ struct GenCtx final : Generic {
     }
     int psi() override {
         GTRY(hbox_fresh());
-        GLAUNCH((g_psi_box<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)hf, (const int4 *)hbox, psi_);
+        GLAUNCH((g_psi_box<R>), dim3((N + 255) / 256, row_blocks(N)), dim3(256), 0, P, (const R *)hf,
+                (const int4 *)hbox, psi_);
         return ORG_OK;
     }
     int sum_s() override {

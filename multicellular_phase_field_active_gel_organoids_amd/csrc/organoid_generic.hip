@@ -2403,8 +2403,9 @@ struct GenCtx final : Generic {
         return e;
     }
     int centroid_win(int sign) {
-        GLAUNCH((g_centroid_win<R>), dim3(std::min(W, 16), nc), dim3(256), 0, P, (const R *)phi, (const int *)xc,
-                (const int *)yc, cen_acc, sign);
+        /* about 2048 workgroups in all, at least 16 rows' worth per cell */
+        GLAUNCH((g_centroid_win<R>), dim3(std::min(W, std::max(16, 2048 / nc)), nc), dim3(256), 0, P, (const R *)phi,
+                (const int *)xc, (const int *)yc, cen_acc, sign);
         return ORG_OK;
     }
     int hfunc_cells() override {

@@ -1788,8 +1788,9 @@ __global__ void g_ring_dirichlet(int N, R *__restrict__ t, R side) {
 
 /* The box of nonzero normals (interior), grown by one and clipped to rows /
  * columns 1..N-2; box = {N, 0, N, 0} on entry (empty). */
+#define ORG_NZ_PARTS 1024
 template <typename R>
-__global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ jy, int *__restrict__ box) {
+__global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ jy, int4 *__restrict__ part) {
     int i0 = N, i1 = -1, j0 = N, j1 = -1;
     for (int i = 1 + blockIdx.x; i < N - 1; i += gridDim.x)
         for (int j = 1 + threadIdx.x; j < N - 1; j += blockDim.x) {
@@ -1798,7 +1799,7 @@ __global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ 
                 i0 = min(i0, i); i1 = max(i1, i); j0 = min(j0, j); j1 = max(j1, j);
             }
         }
-    /* the workgroup's bounds (min / max in any order), then one atomic each */
+    /* the workgroup's bounds (min / max in any order) into its part */
     __shared__ int bb[4];
     if (threadIdx.x == 0) { bb[0] = N; bb[1] = -1; bb[2] = N; bb[3] = -1; }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1810,11 +1811,35 @@ __global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ 
         atomicMin(&bb[0], i0); atomicMax(&bb[1], i1); atomicMin(&bb[2], j0); atomicMax(&bb[3], j1);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && bb[1] >= 0) {
-        atomicMin(&box[0], max(1, bb[0] - 1));
-        atomicMax(&box[1], min(N - 1, bb[1] + 2));
-        atomicMin(&box[2], max(1, bb[2] - 1));
-        atomicMax(&box[3], min(N - 1, bb[3] + 2));
+    if (threadIdx.x == 0) part[blockIdx.x] = make_int4(bb[0], bb[1], bb[2], bb[3]);
+}
+/* g_nz_box's nb parts into box (grown by one, inside rows / columns
+ * 1..N-2; {N, 0, N, 0} when no normal is nonzero).  One workgroup of 256. */
+__global__ __launch_bounds__(256) void g_nz_box_fin(int N, const int4 *__restrict__ part, int nb,
+                                                    int *__restrict__ box) {
+    __shared__ int bb[4];
+    int i0 = N, i1 = -1, j0 = N, j1 = -1;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const int4 v = part[k];
+        i0 = min(i0, v.x); i1 = max(i1, v.y); j0 = min(j0, v.z); j1 = max(j1, v.w);
+    }
+    if (threadIdx.x == 0) { bb[0] = N; bb[1] = -1; bb[2] = N; bb[3] = -1; }
+    for (int o = 32; o > 0; o >>= 1) {
+        i0 = min(i0, __shfl_xor(i0, o, 64)); i1 = max(i1, __shfl_xor(i1, o, 64));
+        j0 = min(j0, __shfl_xor(j0, o, 64)); j1 = max(j1, __shfl_xor(j1, o, 64));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && i1 >= 0) {
+        atomicMin(&bb[0], i0); atomicMax(&bb[1], i1); atomicMin(&bb[2], j0); atomicMax(&bb[3], j1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (bb[1] >= 0) {
+            box[0] = max(1, bb[0] - 1); box[1] = min(N - 1, bb[1] + 2);
+            box[2] = max(1, bb[2] - 1); box[3] = min(N - 1, bb[3] + 2);
+        } else {
+            box[0] = N; box[1] = 0; box[2] = N; box[3] = 0;
+        }
     }
 }
 
@@ -2088,6 +2113,7 @@ struct GenCtx final : Generic {
     unsigned long long *mb_lw[2] = {nullptr, nullptr};   /* the lumen loop's, the ECM loop's */
     unsigned tag_lum[2] = {1, 1};                        /* each one's next tag base */
     int *mb_cmax = nullptr;
+    int4 *nz_part = nullptr;                 /* g_nz_box's per-workgroup bounds, [2][ORG_NZ_PARTS] */
     unsigned long long *cen_acc = nullptr;   /* the centroid sums, [nc][3] */
     int *hbox = nullptr;                     /* [nc][4]: where each cell's hfunc may be nonzero */
     bool hbox_stale = true;                  /* hfunc uploaded since the boxes were last scanned */
@@ -2187,6 +2213,7 @@ struct GenCtx final : Generic {
         err = &ctl->err; relax = &ctl->loop[0].relax; abrt = &ctl->loop[0].abrt; mb_cmax = &ctl->loop[0].cmax;
         GTRY(A_(&cen_acc, (size_t)3 * nc));
         GTRY(A_(&hbox, (size_t)4 * nc));
+        GTRY(A_(&nz_part, (size_t)2 * ORG_NZ_PARTS));
         if (std::is_same<R, double>::value) GTRY(A_(&srows, (size_t)N * nc));
         if (kc > 1) GTRY(A_(&swin, (size_t)nc * 2 * (W + 2) * (W + 2)));
         GTRY(A_(&mb_part, (size_t)2 * std::max(nc * kc, kl)));
@@ -2562,10 +2589,11 @@ struct GenCtx final : Generic {
             GLAUNCH((g_lum_sweep<R>), dim3(row_blocks(N)), dim3(256), 0, P, (const R *)un, u, (const R *)nx,
                     (const R *)ny, side);
             GTRY(plane_sum(u, un, 1, &ctl->rdev[f]));                                         /* :848-855 */
-            GLAUNCH(g_box_empty, dim3(1), dim3(64), 0, ctl->box[f], N);
-            /* at most 240 workgroups: each ends with four atomics on the same four words */
-            GLAUNCH((g_nz_box<R>), dim3(std::max(1, std::min(N - 2, 240))), dim3(256), 0, N, (const R *)nx,
-                    (const R *)ny, ctl->box[f]);
+            /* per-workgroup bounds, then one workgroup combines them */
+            const int nb = std::max(1, std::min(N - 2, ORG_NZ_PARTS));
+            int4 *part = nz_part + (size_t)f * ORG_NZ_PARTS;
+            GLAUNCH((g_nz_box<R>), dim3(nb), dim3(256), 0, N, (const R *)nx, (const R *)ny, part);
+            GLAUNCH(g_nz_box_fin, dim3(1), dim3(256), 0, N, (const int4 *)part, nb, ctl->box[f]);
         }
         return ORG_OK;
     }

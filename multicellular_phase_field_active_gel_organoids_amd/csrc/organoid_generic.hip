@@ -1786,9 +1786,11 @@ __global__ void g_ring_dirichlet(int N, R *__restrict__ t, R side) {
     }
 }
 
-/* The box of nonzero normals (interior), grown by one and clipped to rows /
- * columns 1..N-2; box = {N, 0, N, 0} on entry (empty). */
 #define ORG_NZ_PARTS 1024
+
+/* The box of nonzero normals (interior): workgroup b's bounds {rows min,
+ * max, columns min, max} over its rows 1 + b, 1 + b + gridDim.x, ... into
+ * part[b] ({N, -1, N, -1} when it saw none); g_nz_box_fin combines them. */
 template <typename R>
 __global__ void g_nz_box(int N, const R *__restrict__ jx, const R *__restrict__ jy, int4 *__restrict__ part) {
     int i0 = N, i1 = -1, j0 = N, j1 = -1;
@@ -1843,13 +1845,10 @@ __global__ __launch_bounds__(256) void g_nz_box_fin(int N, const int4 *__restric
     }
 }
 
-/* a launch's arrival counters and abort flag start at zero, the normals'
- * box empty (fill kernels, not memsets, so the sequence can be captured) */
+/* a launch's arrival counters and abort flag start at zero (a fill
+ * kernel, not a memset, so the sequence can be captured) */
 __global__ void g_zero_u32(unsigned *p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0u;
-}
-__global__ void g_box_empty(int *box, int N) {
-    if (threadIdx.x == 0) { box[0] = N; box[1] = 0; box[2] = N; box[3] = 0; }
 }
 
 template <typename R>
